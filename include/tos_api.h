@@ -1,0 +1,199 @@
+/*
+ * tos_api.h -- C ABI of the MI355X (gfx950) topology-optimisation hot path.
+ *
+ * One context (tos_ctx) owns every device buffer of one voxel design domain: the
+ * multigrid hierarchy, the load cases, densities, sensitivities, spline coefficients
+ * and the MMA state.  All calls on one context run on that context's HIP stream and
+ * are not thread-safe; different contexts may be used from different threads.
+ * Every call returns 0 on success or a negative TOS_E* code; tos_last_error() gives
+ * the message of the last failure on that context.  Host arrays are always
+ * lexicographic with x fastest:
+ *   element e = ex + nx*(ey + ny*ez)                      (nx*ny*nz entries)
+ *   node    n = ix + (nx+1)*(iy + (ny+1)*iz)              ((nx+1)(ny+1)(nz+1) entries)
+ *   vector  v[c*n_nodes + n] for component c in {x,y,z}   (structure of arrays)
+ * FP64 at the boundary for displacements, forces and compliance; FP32 for densities,
+ * sensitivities and spline coefficients (the reference's storage types).
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   grid::HierarchyGrid / grid::Grid     grid/Grid.h:214-1037
+ *   optimization free functions          optimization/optimization.h:41-126
+ *   MMA::mma_t                           optimization/mma_t.h:159-266
+ * Each entry point below names the reference function it stands in for.
+ */
+#ifndef TOS_API_H
+#define TOS_API_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TOS_OK 0
+#define TOS_E_ARG -1        /* bad argument (null pointer, size out of range)          */
+#define TOS_E_HIP -2        /* HIP runtime error (allocation, launch)                  */
+#define TOS_E_NOCONV -3     /* solver did not reach the tolerance within maxit         */
+#define TOS_E_NAN -4        /* NaN/Inf detected in an input or a reduction             */
+#define TOS_E_SINGULAR -5   /* coarsest-level Cholesky found a non-positive pivot      */
+#define TOS_E_STATE -6      /* call order violated (e.g. solve before update_operator) */
+
+typedef struct tos_ctx tos_ctx;
+
+/* Design domain.  Replaces HierarchyGrid::genFromMesh(bits, reso) (grid/Grid.cpp:977-1236)
+ * plus the support/load flagging of Grid::build (grid/Grid.cpp:2939-3161). */
+typedef struct tos_grid_desc {
+  int nx, ny, nz;           /* elements per axis (any positive sizes)                  */
+  double h;                 /* element edge length                                     */
+  const uint8_t* solid;     /* nx*ny*nz, nonzero = active element                      */
+  const uint8_t* support;   /* nodes, nonzero = node fixed in x, y and z (may be NULL) */
+  const uint8_t* shell;     /* nx*ny*nz, nonzero = density pinned to 1 (may be NULL)   */
+  int n_levels;             /* 0 = coarsen while a level has > coarsen_threshold active
+                               elements (grid/Grid.cpp:747); >0 = exactly this many    */
+  int coarsen_threshold;    /* 0 = 400                                                 */
+} tos_grid_desc;
+
+typedef struct tos_info {
+  int n_levels;
+  int level_dims[16][3];        /* element dims per level                              */
+  int64_t n_active_elements[16];
+  int64_t n_free_nodes[16];
+  int64_t n_elements, n_nodes;  /* fine level, dense counts                            */
+  int coarse_dofs;              /* size of the dense coarsest system                   */
+  int n_cases;                  /* load cases                                          */
+  int64_t n_coeffs;             /* spline coefficients (0 until tos_spline_init)       */
+} tos_info;
+
+typedef struct tos_solve_opts {
+  double rtol;              /* stop when ||f - K u||_2 / ||f||_2 <= rtol (per case)     */
+  int maxit;                /* outer (flexible PCG) iterations                          */
+  int precision;            /* 0: FP64 V-cycle; 1: FP32 V-cycle inside FP64 PCG        */
+  int pre, post;            /* Gauss-Seidel sweeps per level (reference: 1, 1)          */
+  int warm_start;           /* 1: start from the current displacement                   */
+  int use_pcg;              /* 1: flexible PCG (default); 0: plain V-cycle iteration as
+                               grid/Grid.cpp:2399-2444 with utils/test_utils.cpp:659-663 */
+} tos_solve_opts;
+
+typedef struct tos_solve_stats {
+  int iterations[64];       /* per case                                                */
+  double rel_res[64];       /* final true relative residual (recomputed in FP64)       */
+  double seconds;           /* wall time of the call                                   */
+} tos_solve_stats;
+
+/* ---- context ------------------------------------------------------------------ */
+int tos_ctx_create(const tos_grid_desc* desc, tos_ctx** out);
+int tos_ctx_destroy(tos_ctx* ctx);
+const char* tos_last_error(const tos_ctx* ctx);
+int tos_get_info(tos_ctx* ctx, tos_info* out);
+/* Node flags of the fine level (bit0 active, bit1 support, bit2 free, bits 3-5 GS
+ * colour x%2+2(y%2)+4(z%2)), compare Grid::Bitmask grid/Grid.h:376-416. */
+int tos_get_node_flags(tos_ctx* ctx, int level, uint8_t* out);
+
+/* ---- material / template matrix -------------------------------------------------
+ * Replaces setParameters (optimization/optimization.cpp:35-65) and
+ * uploadTemplateMatrix/initTemplateMatrix (optimization/optimization.cu:804-816,
+ * element/templateMatrix.cpp:31-99). */
+int tos_set_material(tos_ctx* ctx, double youngs, double poisson, double penal, double rho_min);
+int tos_get_template_matrix(tos_ctx* ctx, double* ke576);
+
+/* ---- loads ----------------------------------------------------------------------
+ * Replaces setForceSupport/getPreloadForce (optimization/projection.cu:344-368).
+ * f: n_cases x 3 x n_nodes.  Entries on non-free nodes are ignored (set to 0). */
+int tos_set_loads(tos_ctx* ctx, int n_cases, const double* f);
+int tos_get_loads(tos_ctx* ctx, int k, double* f);
+
+/* ---- densities -------------------------------------------------------------------
+ * Replaces initDensities (optimization.h:111), Grid::init_rho (Grid.h:610), getRho. */
+int tos_set_density(tos_ctx* ctx, const float* rho);
+int tos_get_density(tos_ctx* ctx, float* rho);
+
+/* ---- operator --------------------------------------------------------------------
+ * Replaces update_stencil (grid/Grid.cpp:4167-4178): element weights rho^p, Galerkin
+ * coarse stencils on every level, and the factorisation of the coarsest system. */
+int tos_update_operator(tos_ctx* ctx);
+/* Galerkin stencil of level >= 1: 27*9*n_nodes(level), entry (k*9+r*3+c)*nn + node,
+ * k = (dx+1)+3(dy+1)+9(dz+1) -- the reference's rxStencil layout (grid/Grid.cu:847). */
+int tos_get_stencil(tos_ctx* ctx, int level, double* out);
+/* y = K(rho) u on the fine level (applyK_OTFA_kernel, grid/Grid.cu:4278-4356). */
+int tos_apply_operator(tos_ctx* ctx, const double* u, double* y);
+/* z = B r, one symmetric V(pre,post)-cycle (HierarchyGrid::v_cycle, grid/Grid.cpp:2399-2444). */
+int tos_vcycle(tos_ctx* ctx, const double* r, double* z, int precision, int pre, int post);
+
+/* ---- solve -----------------------------------------------------------------------
+ * Replaces the V-cycle loop of the drivers (utils/test_utils.cpp:659-663). */
+int tos_solve(tos_ctx* ctx, const tos_solve_opts* opts, tos_solve_stats* stats);
+int tos_get_displacement(tos_ctx* ctx, int k, double* u);
+int tos_set_displacement(tos_ctx* ctx, int k, const double* u);
+/* c_k = f_k . u_k for every case (Grid::compliance, grid/Grid.cpp:4035-4038). */
+int tos_compliance(tos_ctx* ctx, double* c);
+/* sum(rho)/N_active over active elements (Grid::volumeRatio, grid/Grid.cu:5462-5470). */
+int tos_volume(tos_ctx* ctx, double* v);
+
+/* ---- sensitivities ---------------------------------------------------------------
+ * Sensitivity of case k: s_e = -p rho_e^(p-1) u_e^T Ke u_e (computeSensitivity_kernel,
+ * optimization/optimization.cu:96-159), into the context's sensitivity buffer. */
+int tos_sensitivity(tos_ctx* ctx, int k);
+/* which: 0 = compliance sensitivity, 1 = volume sensitivity. */
+int tos_get_sensitivity(tos_ctx* ctx, int which, float* out);
+int tos_set_sensitivity(tos_ctx* ctx, int which, const float* in);
+/* volume sensitivity := 1/N_active on active elements (computeSensitivity2, :263-267). */
+int tos_init_vol_sensitivity(tos_ctx* ctx);
+/* Radius-R filter (Grid::filterSensitivity / filterVolSensitivity, grid/Grid.cu:4131-4276). */
+int tos_filter(tos_ctx* ctx, int which, double radius);
+/* Element compliance rho^p u_e^T Ke u_e of case k (elementCompliance, grid/Grid.cu:5389-5430). */
+int tos_element_compliance(tos_ctx* ctx, int k, double* out);
+
+/* ---- density update --------------------------------------------------------------
+ * OC bisection (updateDensities, optimization/optimization.cu:383-460).  Returns the
+ * threshold g and the number of bisection steps. */
+int tos_oc_update(tos_ctx* ctx, double vgoal, double move, double damp,
+                  double* g_out, int* steps_out);
+/* Heaviside projection: keeps the current densities as the projection input and
+ * replaces them by the projected field (projectDensities, optimization.cu:339-353). */
+int tos_project_density(tos_ctx* ctx, double beta);
+/* Multiplies sensitivity `which` by d(projected)/d(input) (computeProjectionSensitivity,
+ * optimization.cu:161-183). */
+int tos_projection_sensitivity(tos_ctx* ctx, int which, double beta);
+
+/* ---- spline density --------------------------------------------------------------
+ * mode 0: clamped knots of order `order` with `param` interior partitions per axis on
+ *         the grid box enlarged by 2*box/(partition+1) per side (reference layout,
+ *         HierarchyGrid::set_spline_partition grid/Grid.cu:3134-3203,
+ *         Grid::set_spline_knot_series grid/Grid.cpp:3253-3377).
+ * mode 1: uniform lattice of order `order`, coefficient (i,j,k) at node
+ *         (param*i, param*j, param*k), boundary coefficients replicated. */
+int tos_spline_init(tos_ctx* ctx, int mode, int order, int param);
+int tos_spline_set_coeffs(tos_ctx* ctx, const float* c);
+int tos_spline_get_coeffs(tos_ctx* ctx, float* c);
+/* rho = clamp(B c, rho_min, 1), shell -> 1 (Grid::coeff2density, grid/Grid.cu:3361-3456). */
+int tos_spline_eval(tos_ctx* ctx);
+/* coefficient gradient of sensitivity `which` (Grid::ddensity2dcoeff_update /
+ * dvol2dcoeff, grid/Grid.cu:3643-3859), deterministic gather. */
+int tos_spline_adjoint(tos_ctx* ctx, int which);
+int tos_spline_get_coeff_sens(tos_ctx* ctx, int which, float* out);
+
+/* ---- MMA --------------------------------------------------------------------------
+ * Method of moving asymptotes over n design variables with m constraints
+ * (MMA::mma_t, optimization/mma_t.h:159-266; update mma_t.cpp:18-51).
+ * The context-bound variant optimises the spline coefficients (n = n_coeffs) when
+ * `on_coeffs` is 1, else the element densities. */
+typedef struct tos_mma tos_mma;
+int tos_mma_create(int n, int m, const double* xmin, const double* xmax, double move,
+                   tos_mma** out);
+int tos_mma_destroy(tos_mma* mma);
+int tos_mma_set_x(tos_mma* mma, const double* x);
+int tos_mma_get_x(tos_mma* mma, double* x);
+/* df: n; dg: m*n (constraint-major); g: m.  converged != 0 after 10 consecutive
+ * iterations with ||dx|| < 1e-3 (mma_t.cpp:40-48). */
+int tos_mma_update(tos_mma* mma, const double* df, const double* dg, const double* g,
+                   int* converged);
+/* One MMA step of the spline driver (utils/test_utils.cpp:1135-1188): df = sens_scale *
+ * coefficient sensitivity, dg = vol_scale * coefficient volume sensitivity,
+ * g = vol_scale*(volume - vfrac); the new x is written to the coefficients. */
+int tos_ctx_mma_step(tos_ctx* ctx, tos_mma* mma, double sens_scale, double vol_scale,
+                     double vfrac, int* converged);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TOS_API_H */

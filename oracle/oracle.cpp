@@ -1,0 +1,1628 @@
+// oracle.cpp -- TEST INFRASTRUCTURE ONLY (see oracle_core.hpp).
+//
+// FP64 CPU restatement of the reference's per-iteration hot path, exported with the
+// same C ABI as the HIP library (include/tos_api.h) under the prefix `orc_`, so that
+// tests drive both backends with identical calls.  Plain loops; OpenMP over
+// independent outputs only; every reduction sums fixed-size chunks in a fixed order so
+// results do not depend on the thread count.
+#include "oracle_core.hpp"
+#include "../include/tos_api.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <stdexcept>
+
+#define ORC_API extern "C" __attribute__((visibility("default")))
+
+namespace orc {
+
+// ------------------------------------------------------------------------------------
+// Deterministic reductions
+// ------------------------------------------------------------------------------------
+static constexpr int64_t kChunk = 8192;
+
+template <class F>
+static double chunked_sum(int64_t n, F term) {
+  int64_t nch = (n + kChunk - 1) / kChunk;
+  std::vector<double> part(std::max<int64_t>(nch, 1), 0.0);
+#pragma omp parallel for schedule(static)
+  for (int64_t c = 0; c < nch; ++c) {
+    double s = 0;
+    int64_t e = std::min(n, (c + 1) * kChunk);
+    for (int64_t i = c * kChunk; i < e; ++i) s += term(i);
+    part[c] = s;
+  }
+  double s = 0;
+  for (int64_t c = 0; c < nch; ++c) s += part[c];
+  return s;
+}
+
+static double dot3(const Level& L, const double* a, const double* b) {
+  return chunked_sum(3 * L.nn, [&](int64_t i) { return a[i] * b[i]; });
+}
+
+// ------------------------------------------------------------------------------------
+// Element template: element/templateMatrix.cpp:15-99.
+// Ke = sum over the 2x2x2 Gauss points (1 +- 1/sqrt3)/2 of B^T D B (h/2)^3, local node
+// i <-> (i&1, (i>>1)&1, i>>2), DOF 3i+axis, engineering shear rows yz, xz, xy.
+// ------------------------------------------------------------------------------------
+static void element_matrix(double h, double E, double nu, double Ke[24][24]) {
+  double D[6][6] = {};
+  double f = E / ((1 + nu) * (1 - 2 * nu));
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) D[i][j] = f * (i == j ? 1 - nu : nu);
+  for (int i = 3; i < 6; ++i) D[i][i] = f * (1 - 2 * nu) / 2;
+  double g = 1.0 / std::sqrt(3.0);
+  for (int i = 0; i < 24; ++i)
+    for (int j = 0; j < 24; ++j) Ke[i][j] = 0;
+  for (int gp = 0; gp < 8; ++gp) {
+    double xi[3] = {(1 + (2 * (gp & 1) - 1) * g) / 2, (1 + (2 * ((gp >> 1) & 1) - 1) * g) / 2,
+                    (1 + (2 * (gp >> 2) - 1) * g) / 2};
+    double dN[8][3];
+    for (int a = 0; a < 8; ++a) {
+      int id[3] = {a & 1, (a >> 1) & 1, a >> 2};
+      double r[3];
+      for (int k = 0; k < 3; ++k) r[k] = id[k] ? xi[k] : 1 - xi[k];
+      for (int k = 0; k < 3; ++k) {
+        double s = id[k] ? 1.0 / h : -1.0 / h;
+        dN[a][k] = s * r[(k + 1) % 3] * r[(k + 2) % 3];
+      }
+    }
+    double B[6][24] = {};
+    for (int a = 0; a < 8; ++a) {
+      for (int k = 0; k < 3; ++k) B[k][3 * a + k] = dN[a][k];
+      B[3][3 * a + 1] = dN[a][2]; B[3][3 * a + 2] = dN[a][1];  // yz
+      B[4][3 * a + 0] = dN[a][2]; B[4][3 * a + 2] = dN[a][0];  // xz
+      B[5][3 * a + 0] = dN[a][1]; B[5][3 * a + 1] = dN[a][0];  // xy
+    }
+    double DB[6][24];
+    for (int i = 0; i < 6; ++i)
+      for (int j = 0; j < 24; ++j) {
+        double s = 0;
+        for (int k = 0; k < 6; ++k) s += D[i][k] * B[k][j];
+        DB[i][j] = s;
+      }
+    for (int i = 0; i < 24; ++i)
+      for (int j = 0; j < 24; ++j) {
+        double s = 0;
+        for (int k = 0; k < 6; ++k) s += B[k][i] * DB[k][j];
+        Ke[i][j] += s;
+      }
+  }
+  double sc = (h / 2) * (h / 2) * (h / 2);
+  for (int i = 0; i < 24; ++i)
+    for (int j = 0; j < 24; ++j) Ke[i][j] *= sc;
+}
+
+// Trilinear prolongation weight of fine local node i (of the fine element at child
+// position c inside a coarse element) from coarse local node m: per axis t = c + i in
+// {0,1,2}, weight (2-t)/2 for the low coarse node and t/2 for the high one
+// (prolongate_correction_kernel, grid/Grid.cu:2594-2632).
+static double pweight(int c, int i, int m) {
+  double w = 1;
+  for (int a = 0; a < 3; ++a) {
+    int t = ((c >> a) & 1) + ((i >> a) & 1);
+    int ma = (m >> a) & 1;
+    w *= ma ? t * 0.5 : (2 - t) * 0.5;
+  }
+  return w;
+}
+
+static void galerkin_child_matrices(const double Ke[24][24], double Mc[8][24][24]) {
+  for (int c = 0; c < 8; ++c) {
+    double P[8][8];
+    for (int i = 0; i < 8; ++i)
+      for (int m = 0; m < 8; ++m) P[i][m] = pweight(c, i, m);
+    for (int m = 0; m < 8; ++m)
+      for (int n = 0; n < 8; ++n)
+        for (int r = 0; r < 3; ++r)
+          for (int s = 0; s < 3; ++s) {
+            double acc = 0;
+            for (int i = 0; i < 8; ++i)
+              for (int j = 0; j < 8; ++j) acc += P[i][m] * Ke[3 * i + r][3 * j + s] * P[j][n];
+            Mc[c][3 * m + r][3 * n + s] = acc;
+          }
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// Hierarchy: coarse element active iff any child active (setSolidElementFromFineGrid,
+// grid/Grid.cu:4453-4473); node active iff an incident element is active
+// (cubeGridSetSolidVertices, grid/Grid.cu:4394-4422); coarsen while the level holds
+// more than `threshold` active elements (grid/Grid.cpp:747).
+// Free nodes: fine = active and not supported; coarse = some free fine node lies in
+// the support of its trilinear basis function (the eliminated Galerkin row is then
+// nonzero).
+// ------------------------------------------------------------------------------------
+static void level_dims(Level& L, int ex, int ey, int ez) {
+  L.ex = ex; L.ey = ey; L.ez = ez;
+  L.nx = ex + 1; L.ny = ey + 1; L.nz = ez + 1;
+  L.ne = (int64_t)ex * ey * ez;
+  L.nn = (int64_t)L.nx * L.ny * L.nz;
+}
+
+static void build_hierarchy(Ctx& c, int n_levels, int threshold) {
+  c.L.clear();
+  Level L0;
+  level_dims(L0, c.nx, c.ny, c.nz);
+  L0.eact.assign(c.solid.begin(), c.solid.end());
+  for (auto& v : L0.eact) v = v ? 1 : 0;
+  c.L.push_back(std::move(L0));
+  auto count_active = [](const Level& L) {
+    int64_t n = 0;
+    for (auto v : L.eact) n += v;
+    return n;
+  };
+  while (true) {
+    const Level& F = c.L.back();
+    bool more = n_levels > 0 ? (int)c.L.size() < n_levels : count_active(F) > threshold;
+    if (!more || (F.ex == 1 && F.ey == 1 && F.ez == 1)) break;
+    Level C;
+    level_dims(C, (F.ex + 1) / 2, (F.ey + 1) / 2, (F.ez + 1) / 2);
+    C.eact.assign(C.ne, 0);
+    for (int z = 0; z < F.ez; ++z)
+      for (int y = 0; y < F.ey; ++y)
+        for (int x = 0; x < F.ex; ++x)
+          if (F.eact[F.eid(x, y, z)]) C.eact[C.eid(x / 2, y / 2, z / 2)] = 1;
+    c.L.push_back(std::move(C));
+  }
+  // free masks
+  for (size_t l = 0; l < c.L.size(); ++l) {
+    Level& L = c.L[l];
+    L.fre.assign(L.nn, 0);
+    if (l == 0) {
+      for (int z = 0; z < L.nz; ++z)
+        for (int y = 0; y < L.ny; ++y)
+          for (int x = 0; x < L.nx; ++x) {
+            bool act = false;
+            for (int a = 0; a < 8 && !act; ++a) {
+              int ex = x - 1 + (a & 1), ey = y - 1 + ((a >> 1) & 1), ez = z - 1 + (a >> 2);
+              if (ex < 0 || ey < 0 || ez < 0 || ex >= L.ex || ey >= L.ey || ez >= L.ez) continue;
+              act = L.eact[L.eid(ex, ey, ez)];
+            }
+            int64_t n = L.nid(x, y, z);
+            L.fre[n] = act && !(c.support.size() && c.support[n]);
+          }
+    } else {
+      const Level& F = c.L[l - 1];
+      for (int z = 0; z < L.nz; ++z)
+        for (int y = 0; y < L.ny; ++y)
+          for (int x = 0; x < L.nx; ++x) {
+            bool fr = false;
+            for (int dz = -1; dz <= 1 && !fr; ++dz)
+              for (int dy = -1; dy <= 1 && !fr; ++dy)
+                for (int dx = -1; dx <= 1 && !fr; ++dx) {
+                  int fx = 2 * x + dx, fy = 2 * y + dy, fz = 2 * z + dz;
+                  if (fx < 0 || fy < 0 || fz < 0 || fx >= F.nx || fy >= F.ny || fz >= F.nz) continue;
+                  fr = F.fre[F.nid(fx, fy, fz)];
+                }
+            L.fre[L.nid(x, y, z)] = fr;
+          }
+    }
+    L.u.assign(3 * L.nn, 0);
+    L.f.assign(3 * L.nn, 0);
+    L.r.assign(3 * L.nn, 0);
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// Fine-level matrix-free operator (update_residual_OTFA_WS_kernel grid/Grid.cu:2277-2333,
+// applyK_OTFA_kernel :4278-4356): y_i = sum_{e incident} rho_e^p sum_{j in e}
+// Ke[loc_i][loc_j] u_j; supported / inactive nodes carry no unknown (row and column
+// eliminated), so their u is treated as 0 and their output is 0.
+// ------------------------------------------------------------------------------------
+static void apply_fine(const Ctx& c, const double* u, double* y) {
+  const Level& L = c.L[0];
+  const int64_t nn = L.nn;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int z = 0; z < L.nz; ++z)
+    for (int yy = 0; yy < L.ny; ++yy)
+      for (int x = 0; x < L.nx; ++x) {
+        int64_t i = L.nid(x, yy, z);
+        double acc[3] = {0, 0, 0};
+        if (L.fre[i]) {
+          for (int a = 0; a < 8; ++a) {
+            int ex = x - 1 + (a & 1), ey = yy - 1 + ((a >> 1) & 1), ez = z - 1 + (a >> 2);
+            if (ex < 0 || ey < 0 || ez < 0 || ex >= L.ex || ey >= L.ey || ez >= L.ez) continue;
+            double w = c.w[L.eid(ex, ey, ez)];
+            if (w == 0) continue;
+            int li = 7 - a;
+            double t[3] = {0, 0, 0};
+            for (int j = 0; j < 8; ++j) {
+              int64_t nj = L.nid(ex + (j & 1), ey + ((j >> 1) & 1), ez + (j >> 2));
+              if (!L.fre[nj]) continue;
+              for (int r = 0; r < 3; ++r)
+                for (int s = 0; s < 3; ++s) t[r] += c.Ke[3 * li + r][3 * j + s] * u[s * nn + nj];
+            }
+            for (int r = 0; r < 3; ++r) acc[r] += w * t[r];
+          }
+        }
+        for (int r = 0; r < 3; ++r) y[r * nn + i] = acc[r];
+      }
+}
+
+static void apply_stencil(const Level& L, const double* u, double* y) {
+  const int64_t nn = L.nn;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int z = 0; z < L.nz; ++z)
+    for (int yy = 0; yy < L.ny; ++yy)
+      for (int x = 0; x < L.nx; ++x) {
+        int64_t i = L.nid(x, yy, z);
+        double acc[3] = {0, 0, 0};
+        if (L.fre[i]) {
+          for (int k = 0; k < 27; ++k) {
+            int dx = k % 3 - 1, dy = (k / 3) % 3 - 1, dz = k / 9 - 1;
+            int jx = x + dx, jy = yy + dy, jz = z + dz;
+            if (jx < 0 || jy < 0 || jz < 0 || jx >= L.nx || jy >= L.ny || jz >= L.nz) continue;
+            int64_t j = L.nid(jx, jy, jz);
+            if (!L.fre[j]) continue;
+            for (int r = 0; r < 3; ++r)
+              for (int s = 0; s < 3; ++s)
+                acc[r] += L.st[(k * 9 + r * 3 + s) * nn + i] * u[s * nn + j];
+          }
+        }
+        for (int r = 0; r < 3; ++r) y[r * nn + i] = acc[r];
+      }
+}
+
+static void apply_level(const Ctx& c, int l, const double* u, double* y) {
+  if (l == 0) apply_fine(c, u, y);
+  else apply_stencil(c.L[l], u, y);
+}
+
+// ------------------------------------------------------------------------------------
+// 8-colour block Gauss-Seidel (gs_relax_OTFA_WS_kernel grid/Grid.cu:1700-1842,
+// gs_relax_kernel :1455-1566): colour x%2 + 2(y%2) + 4(z%2) (compute_gscolor
+// :1267-1342); per node the 3x3 diagonal block is relaxed component by component
+// (x, y, z) with the updated components.  The backward sweep (colours 7..0, components
+// z, y, x) is the adjoint of the forward one, which makes the V-cycle symmetric.
+// ------------------------------------------------------------------------------------
+static void node_block(const Ctx& c, int l, int x, int y, int z, const double* u,
+                       double s[3], double D[9]) {
+  const Level& L = c.L[l];
+  const int64_t nn = L.nn;
+  for (int r = 0; r < 3; ++r) s[r] = 0;
+  for (int r = 0; r < 9; ++r) D[r] = 0;
+  if (l == 0) {
+    for (int a = 0; a < 8; ++a) {
+      int ex = x - 1 + (a & 1), ey = y - 1 + ((a >> 1) & 1), ez = z - 1 + (a >> 2);
+      if (ex < 0 || ey < 0 || ez < 0 || ex >= L.ex || ey >= L.ey || ez >= L.ez) continue;
+      double w = c.w[L.eid(ex, ey, ez)];
+      if (w == 0) continue;
+      int li = 7 - a;
+      for (int j = 0; j < 8; ++j) {
+        if (j == li) {
+          for (int r = 0; r < 3; ++r)
+            for (int q = 0; q < 3; ++q) D[r * 3 + q] += w * c.Ke[3 * li + r][3 * li + q];
+          continue;
+        }
+        int64_t nj = L.nid(ex + (j & 1), ey + ((j >> 1) & 1), ez + (j >> 2));
+        if (!L.fre[nj]) continue;
+        for (int r = 0; r < 3; ++r)
+          for (int q = 0; q < 3; ++q) s[r] += w * c.Ke[3 * li + r][3 * j + q] * u[q * nn + nj];
+      }
+    }
+  } else {
+    int64_t i = L.nid(x, y, z);
+    for (int k = 0; k < 27; ++k) {
+      if (k == 13) {
+        for (int e = 0; e < 9; ++e) D[e] = L.st[(13 * 9 + e) * nn + i];
+        continue;
+      }
+      int jx = x + k % 3 - 1, jy = y + (k / 3) % 3 - 1, jz = z + k / 9 - 1;
+      if (jx < 0 || jy < 0 || jz < 0 || jx >= L.nx || jy >= L.ny || jz >= L.nz) continue;
+      int64_t j = L.nid(jx, jy, jz);
+      if (!L.fre[j]) continue;
+      for (int r = 0; r < 3; ++r)
+        for (int q = 0; q < 3; ++q) s[r] += L.st[(k * 9 + r * 3 + q) * nn + i] * u[q * nn + j];
+    }
+  }
+}
+
+static void gs_sweep(const Ctx& c, int l, double* u, const double* f, bool backward) {
+  const Level& L = c.L[l];
+  const int64_t nn = L.nn;
+  for (int ci = 0; ci < 8; ++ci) {
+    int col = backward ? 7 - ci : ci;
+    int px = col & 1, py = (col >> 1) & 1, pz = col >> 2;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int z = pz; z < L.nz; z += 2)
+      for (int y = py; y < L.ny; y += 2)
+        for (int x = px; x < L.nx; x += 2) {
+          int64_t i = L.nid(x, y, z);
+          if (!L.fre[i]) continue;
+          double s[3], D[9];
+          node_block(c, l, x, y, z, u, s, D);
+          double v[3] = {u[i], u[nn + i], u[2 * nn + i]};
+          double b[3] = {f[i] - s[0], f[nn + i] - s[1], f[2 * nn + i] - s[2]};
+          if (!backward) {
+            v[0] = (b[0] - D[1] * v[1] - D[2] * v[2]) / D[0];
+            v[1] = (b[1] - D[3] * v[0] - D[5] * v[2]) / D[4];
+            v[2] = (b[2] - D[6] * v[0] - D[7] * v[1]) / D[8];
+          } else {
+            v[2] = (b[2] - D[6] * v[0] - D[7] * v[1]) / D[8];
+            v[1] = (b[1] - D[3] * v[0] - D[5] * v[2]) / D[4];
+            v[0] = (b[0] - D[1] * v[1] - D[2] * v[2]) / D[0];
+          }
+          u[i] = v[0]; u[nn + i] = v[1]; u[2 * nn + i] = v[2];
+        }
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// Transfers (restrict_residual_kernel grid/Grid.cu:2456-2507, prolongate_correction_kernel
+// :2594-2632): trilinear P with weights 1, 1/2, 1/4, 1/8; restriction is P^T.  Both are
+// masked to free nodes (P with the rows of eliminated fine nodes removed).
+// ------------------------------------------------------------------------------------
+static void restrict_to(const Level& F, Level& C) {
+  const int64_t nf = F.nn, nc = C.nn;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int z = 0; z < C.nz; ++z)
+    for (int y = 0; y < C.ny; ++y)
+      for (int x = 0; x < C.nx; ++x) {
+        int64_t I = C.nid(x, y, z);
+        double acc[3] = {0, 0, 0};
+        if (C.fre[I]) {
+          for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+              for (int dx = -1; dx <= 1; ++dx) {
+                int fx = 2 * x + dx, fy = 2 * y + dy, fz = 2 * z + dz;
+                if (fx < 0 || fy < 0 || fz < 0 || fx >= F.nx || fy >= F.ny || fz >= F.nz) continue;
+                int64_t a = F.nid(fx, fy, fz);
+                if (!F.fre[a]) continue;
+                double w = (dx ? 0.5 : 1.0) * (dy ? 0.5 : 1.0) * (dz ? 0.5 : 1.0);
+                for (int r = 0; r < 3; ++r) acc[r] += w * F.r[r * nf + a];
+              }
+        }
+        for (int r = 0; r < 3; ++r) C.f[r * nc + I] = acc[r];
+      }
+}
+
+static void prolong_add(const Level& C, Level& F) {
+  const int64_t nf = F.nn, nc = C.nn;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int z = 0; z < F.nz; ++z)
+    for (int y = 0; y < F.ny; ++y)
+      for (int x = 0; x < F.nx; ++x) {
+        int64_t a = F.nid(x, y, z);
+        if (!F.fre[a]) continue;
+        int cx[2], cy[2], cz[2];
+        double wx[2], wy[2], wz[2];
+        int nxp = (x & 1) ? 2 : 1, nyp = (y & 1) ? 2 : 1, nzp = (z & 1) ? 2 : 1;
+        if (x & 1) { cx[0] = x / 2; cx[1] = x / 2 + 1; wx[0] = wx[1] = 0.5; } else { cx[0] = x / 2; wx[0] = 1; }
+        if (y & 1) { cy[0] = y / 2; cy[1] = y / 2 + 1; wy[0] = wy[1] = 0.5; } else { cy[0] = y / 2; wy[0] = 1; }
+        if (z & 1) { cz[0] = z / 2; cz[1] = z / 2 + 1; wz[0] = wz[1] = 0.5; } else { cz[0] = z / 2; wz[0] = 1; }
+        double acc[3] = {0, 0, 0};
+        for (int k = 0; k < nzp; ++k)
+          for (int j = 0; j < nyp; ++j)
+            for (int i = 0; i < nxp; ++i) {
+              int64_t I = C.nid(cx[i], cy[j], cz[k]);
+              double w = wx[i] * wy[j] * wz[k];
+              for (int r = 0; r < 3; ++r) acc[r] += w * C.u[r * nc + I];
+            }
+        for (int r = 0; r < 3; ++r) F.u[r * nf + a] += acc[r];
+      }
+}
+
+// ------------------------------------------------------------------------------------
+// Galerkin coarse operators K_c = P^T K P with eliminated DOFs.  The reference assembles
+// them per coarse node directly from fine rho (restrict_stencil_nondyadic_OTFA_WS_kernel
+// grid/Grid.cu:1112-1225, restrict_stencil_dyadic_kernel :840-896).
+// Level 1: per coarse element sum over its active children of rho^p P_c^T Ke P_c, with
+// the rows/columns of eliminated fine nodes removed.  Deeper levels: triple product of
+// the stored 27-point stencil with trilinear P.
+// ------------------------------------------------------------------------------------
+static void galerkin_first(Ctx& c) {
+  const Level& F = c.L[0];
+  Level& C = c.L[1];
+  const int64_t nc = C.nn;
+  C.st.assign(243 * nc, 0.0);
+#pragma omp parallel for collapse(2) schedule(dynamic, 1)
+  for (int Z = 0; Z < C.nz; ++Z)
+    for (int Y = 0; Y < C.ny; ++Y)
+      for (int X = 0; X < C.nx; ++X) {
+        int64_t I = C.nid(X, Y, Z);
+        if (!C.fre[I]) continue;
+        double acc[27][9] = {};
+        for (int a = 0; a < 8; ++a) {  // coarse element E = I - 1 + bits(a)
+          int Ex = X - 1 + (a & 1), Ey = Y - 1 + ((a >> 1) & 1), Ez = Z - 1 + (a >> 2);
+          if (Ex < 0 || Ey < 0 || Ez < 0 || Ex >= C.ex || Ey >= C.ey || Ez >= C.ez) continue;
+          int mI = 7 - a;
+          for (int ch = 0; ch < 8; ++ch) {
+            int fx = 2 * Ex + (ch & 1), fy = 2 * Ey + ((ch >> 1) & 1), fz = 2 * Ez + (ch >> 2);
+            if (fx >= F.ex || fy >= F.ey || fz >= F.ez) continue;
+            double w = c.w[F.eid(fx, fy, fz)];
+            if (w == 0) continue;
+            bool fr[8];
+            bool all = true;
+            for (int i = 0; i < 8; ++i) {
+              fr[i] = F.fre[F.nid(fx + (i & 1), fy + ((i >> 1) & 1), fz + (i >> 2))];
+              all = all && fr[i];
+            }
+            for (int n = 0; n < 8; ++n) {
+              int dx = (a & 1) + (n & 1) - 1, dy = ((a >> 1) & 1) + ((n >> 1) & 1) - 1,
+                  dz = (a >> 2) + (n >> 2) - 1;
+              int k = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
+              for (int r = 0; r < 3; ++r)
+                for (int s = 0; s < 3; ++s) {
+                  double v;
+                  if (all) {
+                    v = c.Mc[ch][3 * mI + r][3 * n + s];
+                  } else {
+                    v = 0;
+                    for (int i = 0; i < 8; ++i) {
+                      if (!fr[i]) continue;
+                      double pi = pweight(ch, i, mI);
+                      if (pi == 0) continue;
+                      for (int j = 0; j < 8; ++j) {
+                        if (!fr[j]) continue;
+                        v += pi * c.Ke[3 * i + r][3 * j + s] * pweight(ch, j, n);
+                      }
+                    }
+                  }
+                  acc[k][r * 3 + s] += w * v;
+                }
+            }
+          }
+        }
+        for (int k = 0; k < 27; ++k) {
+          int jx = X + k % 3 - 1, jy = Y + (k / 3) % 3 - 1, jz = Z + k / 9 - 1;
+          bool ok = jx >= 0 && jy >= 0 && jz >= 0 && jx < C.nx && jy < C.ny && jz < C.nz &&
+                    C.fre[C.nid(jx, jy, jz)];
+          for (int e = 0; e < 9; ++e) C.st[(k * 9 + e) * nc + I] = ok ? acc[k][e] : 0.0;
+        }
+      }
+}
+
+static void galerkin_next(const Level& F, Level& C) {
+  const int64_t nf = F.nn, nc = C.nn;
+  C.st.assign(243 * nc, 0.0);
+#pragma omp parallel for collapse(2) schedule(dynamic, 1)
+  for (int Z = 0; Z < C.nz; ++Z)
+    for (int Y = 0; Y < C.ny; ++Y)
+      for (int X = 0; X < C.nx; ++X) {
+        int64_t I = C.nid(X, Y, Z);
+        if (!C.fre[I]) continue;
+        double acc[27][9] = {};
+        for (int az = -1; az <= 1; ++az)
+          for (int ay = -1; ay <= 1; ++ay)
+            for (int axx = -1; axx <= 1; ++axx) {
+              int fx = 2 * X + axx, fy = 2 * Y + ay, fz = 2 * Z + az;
+              if (fx < 0 || fy < 0 || fz < 0 || fx >= F.nx || fy >= F.ny || fz >= F.nz) continue;
+              int64_t A = F.nid(fx, fy, fz);
+              if (!F.fre[A]) continue;
+              double wa = (axx ? 0.5 : 1.0) * (ay ? 0.5 : 1.0) * (az ? 0.5 : 1.0);
+              for (int k = 0; k < 27; ++k) {
+                int bx = fx + k % 3 - 1, by = fy + (k / 3) % 3 - 1, bz = fz + k / 9 - 1;
+                if (bx < 0 || by < 0 || bz < 0 || bx >= F.nx || by >= F.ny || bz >= F.nz) continue;
+                if (!F.fre[F.nid(bx, by, bz)]) continue;
+                const double* blk[9];
+                double bv[9];
+                for (int e = 0; e < 9; ++e) { bv[e] = F.st[(k * 9 + e) * nf + A]; (void)blk; }
+                // parents of fine node b
+                int px[2], py[2], pz[2];
+                double wx[2], wy[2], wz[2];
+                int npx = (bx & 1) ? 2 : 1, npy = (by & 1) ? 2 : 1, npz = (bz & 1) ? 2 : 1;
+                if (bx & 1) { px[0] = bx / 2; px[1] = bx / 2 + 1; wx[0] = wx[1] = 0.5; } else { px[0] = bx / 2; wx[0] = 1; }
+                if (by & 1) { py[0] = by / 2; py[1] = by / 2 + 1; wy[0] = wy[1] = 0.5; } else { py[0] = by / 2; wy[0] = 1; }
+                if (bz & 1) { pz[0] = bz / 2; pz[1] = bz / 2 + 1; wz[0] = wz[1] = 0.5; } else { pz[0] = bz / 2; wz[0] = 1; }
+                for (int q = 0; q < npz; ++q)
+                  for (int p = 0; p < npy; ++p)
+                    for (int o = 0; o < npx; ++o) {
+                      int dx = px[o] - X, dy = py[p] - Y, dz = pz[q] - Z;
+                      int ko = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
+                      double wgt = wa * wx[o] * wy[p] * wz[q];
+                      for (int e = 0; e < 9; ++e) acc[ko][e] += wgt * bv[e];
+                    }
+              }
+            }
+        for (int k = 0; k < 27; ++k) {
+          int jx = X + k % 3 - 1, jy = Y + (k / 3) % 3 - 1, jz = Z + k / 9 - 1;
+          bool ok = jx >= 0 && jy >= 0 && jz >= 0 && jx < C.nx && jy < C.ny && jz < C.nz &&
+                    C.fre[C.nid(jx, jy, jz)];
+          for (int e = 0; e < 9; ++e) C.st[(k * 9 + e) * nc + I] = ok ? acc[k][e] : 0.0;
+        }
+      }
+}
+
+// ------------------------------------------------------------------------------------
+// Coarsest level: dense SPD system over the free nodes, Cholesky-factored once per
+// operator update.  The reference uses Eigen BDCSVD on the host (buildCoarsestSystem
+// grid/Grid.cpp:3614-3669, solve_fem_host :4040-4102); for an SPD matrix both give the
+// same solution.
+// ------------------------------------------------------------------------------------
+static int coarse_factor(Ctx& c) {
+  const Level& L = c.L.back();
+  if (c.L.size() == 1) return TOS_OK;  // single level: never used
+  c.cmap.assign(L.nn, -1);
+  c.cnode.clear();
+  for (int64_t i = 0; i < L.nn; ++i)
+    if (L.fre[i]) { c.cmap[i] = (int64_t)c.cnode.size(); c.cnode.push_back(i); }
+  int n = 3 * (int)c.cnode.size();
+  c.cn = n;
+  c.chol.assign((size_t)n * n, 0.0);
+  for (size_t p = 0; p < c.cnode.size(); ++p) {
+    int64_t i = c.cnode[p];
+    int x = i % L.nx, y = (i / L.nx) % L.ny, z = i / ((int64_t)L.nx * L.ny);
+    for (int k = 0; k < 27; ++k) {
+      int jx = x + k % 3 - 1, jy = y + (k / 3) % 3 - 1, jz = z + k / 9 - 1;
+      if (jx < 0 || jy < 0 || jz < 0 || jx >= L.nx || jy >= L.ny || jz >= L.nz) continue;
+      int64_t q = c.cmap[L.nid(jx, jy, jz)];
+      if (q < 0) continue;
+      for (int r = 0; r < 3; ++r)
+        for (int s = 0; s < 3; ++s)
+          c.chol[(size_t)(3 * p + r) * n + 3 * q + s] = L.st[(k * 9 + r * 3 + s) * L.nn + i];
+    }
+  }
+  // symmetrise (the stencil is symmetric up to roundoff)
+  double* A = c.chol.data();
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < i; ++j) {
+      double v = 0.5 * (A[(size_t)i * n + j] + A[(size_t)j * n + i]);
+      A[(size_t)i * n + j] = v; A[(size_t)j * n + i] = v;
+    }
+  for (int j = 0; j < n; ++j) {
+    double d = A[(size_t)j * n + j];
+    for (int k = 0; k < j; ++k) d -= A[(size_t)j * n + k] * A[(size_t)j * n + k];
+    if (!(d > 0)) { c.err = "coarsest Cholesky: non-positive pivot"; return TOS_E_SINGULAR; }
+    d = std::sqrt(d);
+    A[(size_t)j * n + j] = d;
+#pragma omp parallel for schedule(static)
+    for (int i = j + 1; i < n; ++i) {
+      double s = A[(size_t)i * n + j];
+      for (int k = 0; k < j; ++k) s -= A[(size_t)i * n + k] * A[(size_t)j * n + k];
+      A[(size_t)i * n + j] = s / d;
+    }
+  }
+  return TOS_OK;
+}
+
+static void coarse_solve(Ctx& c) {
+  Level& L = c.L.back();
+  int n = c.cn;
+  std::vector<double> b(n);
+  for (size_t p = 0; p < c.cnode.size(); ++p)
+    for (int r = 0; r < 3; ++r) b[3 * p + r] = L.f[r * L.nn + c.cnode[p]];
+  const double* A = c.chol.data();
+  for (int i = 0; i < n; ++i) {
+    double s = b[i];
+    for (int k = 0; k < i; ++k) s -= A[(size_t)i * n + k] * b[k];
+    b[i] = s / A[(size_t)i * n + i];
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    double s = b[i];
+    for (int k = i + 1; k < n; ++k) s -= A[(size_t)k * n + i] * b[k];
+    b[i] = s / A[(size_t)i * n + i];
+  }
+  std::fill(L.u.begin(), L.u.end(), 0.0);
+  for (size_t p = 0; p < c.cnode.size(); ++p)
+    for (int r = 0; r < 3; ++r) L.u[r * L.nn + c.cnode[p]] = b[3 * p + r];
+}
+
+// ------------------------------------------------------------------------------------
+// V-cycle (HierarchyGrid::v_cycle grid/Grid.cpp:2399-2444): on each level relax from
+// zero, restrict the residual, recurse, prolongate, relax with the adjoint sweep.
+// Reads level.f, writes level.u.
+// ------------------------------------------------------------------------------------
+static void vcycle_level(Ctx& c, size_t l, int pre, int post) {
+  Level& L = c.L[l];
+  if (l + 1 == c.L.size()) {
+    if (c.L.size() == 1) {  // single level: smooth only
+      std::fill(L.u.begin(), L.u.end(), 0.0);
+      for (int s = 0; s < std::max(1, pre); ++s) gs_sweep(c, l, L.u.data(), L.f.data(), false);
+      for (int s = 0; s < std::max(1, post); ++s) gs_sweep(c, l, L.u.data(), L.f.data(), true);
+      return;
+    }
+    coarse_solve(c);
+    return;
+  }
+  std::fill(L.u.begin(), L.u.end(), 0.0);
+  for (int s = 0; s < pre; ++s) gs_sweep(c, l, L.u.data(), L.f.data(), false);
+  apply_level(c, (int)l, L.u.data(), L.r.data());
+  for (int64_t i = 0; i < 3 * L.nn; ++i) L.r[i] = L.f[i] - L.r[i];
+  restrict_to(L, c.L[l + 1]);
+  vcycle_level(c, l + 1, pre, post);
+  prolong_add(c.L[l + 1], L);
+  for (int s = 0; s < post; ++s) gs_sweep(c, l, L.u.data(), L.f.data(), true);
+}
+
+static void precondition(Ctx& c, const double* r, double* z, int pre, int post) {
+  Level& L = c.L[0];
+  std::copy(r, r + 3 * L.nn, L.f.begin());
+  vcycle_level(c, 0, pre, post);
+  std::copy(L.u.begin(), L.u.end(), z);
+}
+
+static void mask_free(const Level& L, double* v) {
+  for (int r = 0; r < 3; ++r)
+    for (int64_t i = 0; i < L.nn; ++i)
+      if (!L.fre[i]) v[r * L.nn + i] = 0;
+}
+
+// ------------------------------------------------------------------------------------
+// Solve K u = f per load case.  use_pcg = 0 repeats u += B(f - K u) like the drivers'
+// V-cycle loop (utils/test_utils.cpp:659-663); use_pcg = 1 runs flexible CG (Polak-
+// Ribiere beta) preconditioned by one V-cycle.  Warm start from the stored u.
+// ------------------------------------------------------------------------------------
+static int solve_case(Ctx& c, int k, const tos_solve_opts& o, int* its, double* relres) {
+  Level& L = c.L[0];
+  const int64_t n3 = 3 * L.nn;
+  double* u = c.U.data() + k * n3;
+  const double* f = c.F.data() + k * n3;
+  if (!o.warm_start) std::fill(u, u + n3, 0.0);
+  mask_free(L, u);
+  double nf = std::sqrt(dot3(L, f, f));
+  *its = 0;
+  if (nf == 0) { std::fill(u, u + n3, 0.0); *relres = 0; return TOS_OK; }
+  std::vector<double> r(n3), z(n3), p(n3), q(n3);
+  apply_fine(c, u, q.data());
+  for (int64_t i = 0; i < n3; ++i) r[i] = f[i] - q[i];
+  double rn = std::sqrt(dot3(L, r.data(), r.data()));
+  if (!std::isfinite(rn)) { c.err = "NaN in residual"; return TOS_E_NAN; }
+  int it = 0;
+  if (!o.use_pcg) {
+    while (rn / nf > o.rtol && it < o.maxit) {
+      precondition(c, r.data(), z.data(), o.pre, o.post);
+      for (int64_t i = 0; i < n3; ++i) u[i] += z[i];
+      apply_fine(c, u, q.data());
+      for (int64_t i = 0; i < n3; ++i) r[i] = f[i] - q[i];
+      rn = std::sqrt(dot3(L, r.data(), r.data()));
+      ++it;
+    }
+  } else if (rn / nf > o.rtol) {
+    precondition(c, r.data(), z.data(), o.pre, o.post);
+    p = z;
+    double rz = dot3(L, r.data(), z.data());
+    while (it < o.maxit) {
+      apply_fine(c, p.data(), q.data());
+      double pq = dot3(L, p.data(), q.data());
+      double alpha = rz / pq;
+      for (int64_t i = 0; i < n3; ++i) { u[i] += alpha * p[i]; r[i] -= alpha * q[i]; }
+      ++it;
+      rn = std::sqrt(dot3(L, r.data(), r.data()));
+      if (!std::isfinite(rn)) { c.err = "NaN in PCG"; return TOS_E_NAN; }
+      if (rn / nf <= o.rtol) break;
+      precondition(c, r.data(), z.data(), o.pre, o.post);
+      double zr = dot3(L, z.data(), r.data());
+      double zq = dot3(L, z.data(), q.data());
+      double beta = -alpha * zq / rz;
+      rz = zr;
+      for (int64_t i = 0; i < n3; ++i) p[i] = z[i] + beta * p[i];
+    }
+  }
+  apply_fine(c, u, q.data());
+  for (int64_t i = 0; i < n3; ++i) r[i] = f[i] - q[i];
+  *relres = std::sqrt(dot3(L, r.data(), r.data())) / nf;
+  *its = it;
+  return *relres <= o.rtol ? TOS_OK : TOS_E_NOCONV;
+}
+
+// ------------------------------------------------------------------------------------
+// Sensitivity (computeSensitivity_kernel optimization/optimization.cu:96-159):
+// s_e = -p rho^(p-1) u_e^T Ke u_e, one deterministic value per element.
+// ------------------------------------------------------------------------------------
+static double element_energy(const Ctx& c, const double* u, int ex, int ey, int ez) {
+  const Level& L = c.L[0];
+  double ue[24];
+  for (int j = 0; j < 8; ++j) {
+    int64_t n = L.nid(ex + (j & 1), ey + ((j >> 1) & 1), ez + (j >> 2));
+    for (int s = 0; s < 3; ++s) ue[3 * j + s] = L.fre[n] ? u[s * L.nn + n] : 0.0;
+  }
+  double e = 0;
+  for (int i = 0; i < 24; ++i) {
+    double t = 0;
+    for (int j = 0; j < 24; ++j) t += c.Ke[i][j] * ue[j];
+    e += ue[i] * t;
+  }
+  return e;
+}
+
+// Filter weight 1 - 6r^2 + 8r^3 - 3r^4 with r = d/R over the window floor(R+1/2)
+// (filterSensitivity_kernel grid/Grid.cu:4131-4216, weight lambda :4226-4229).
+static void filter_field(const Ctx& c, std::vector<float>& v, double R) {
+  const Level& L = c.L[0];
+  std::vector<float> src(v);
+  int Ri = (int)(R + 0.5);
+  double R2 = R * R;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int z = 0; z < L.ez; ++z)
+    for (int y = 0; y < L.ey; ++y)
+      for (int x = 0; x < L.ex; ++x) {
+        int64_t e = L.eid(x, y, z);
+        if (!L.eact[e]) { v[e] = 0; continue; }
+        double gs = 0, ws = 0;
+        for (int dz = -Ri; dz <= Ri; ++dz)
+          for (int dy = -Ri; dy <= Ri; ++dy)
+            for (int dx = -Ri; dx <= Ri; ++dx) {
+              int qx = x + dx, qy = y + dy, qz = z + dz;
+              if (qx < 0 || qy < 0 || qz < 0 || qx >= L.ex || qy >= L.ey || qz >= L.ez) continue;
+              double r2 = dx * dx + dy * dy + dz * dz;
+              if (r2 > R2) continue;
+              int64_t q = L.eid(qx, qy, qz);
+              if (!L.eact[q]) continue;
+              double r = std::sqrt(r2 / R2);
+              double w = 1 - 6 * r * r + 8 * r * r * r - 3 * r * r * r * r;
+              gs += w * src[q];
+              ws += w;
+            }
+        v[e] = (float)(gs / ws);
+      }
+}
+
+// ------------------------------------------------------------------------------------
+// OC bisection (updateDensities / trySensMultiplier_kernel optimization/optimization.cu
+// :355-460): g in [0, max|s|], rho' = clamp(clamp(rho (max(0,-s)/g)^eta, rho-m, rho+m),
+// rho_min, 1), shell -> 1; stop at |V - Vgoal| <= 1e-4 or after 31 trials.
+// ------------------------------------------------------------------------------------
+static double oc_trial(const Ctx& c, double g, double move, double damp, std::vector<float>* out) {
+  const Level& L = c.L[0];
+  double vs = chunked_sum(L.ne, [&](int64_t e) -> double {
+    if (!L.eact[e]) return 0.0;
+    double s = c.sens[e];
+    double gg = s > 0 ? 0.0 : -s;
+    double r0 = c.rho[e];
+    double rn = r0 * std::pow(gg / g, damp);
+    rn = std::min(std::max(rn, r0 - move), r0 + move);
+    rn = std::min(std::max(rn, c.rho_min), 1.0);
+    if (c.shell.size() && c.shell[e]) rn = 1;
+    float rf = (float)rn;
+    if (out) (*out)[e] = rf;
+    return (double)rf;
+  });
+  return vs / (double)c.n_active;
+}
+
+// ------------------------------------------------------------------------------------
+// Spline tables.
+// ------------------------------------------------------------------------------------
+// Cox-de Boor (The NURBS Book, algorithm A2.2) as in SplineBasisX grid/Grid.cu:384-410.
+static void basis_funs(int span, double x, int p, const std::vector<double>& U, double* N) {
+  double left[16], right[16];
+  N[0] = 1.0;
+  for (int j = 1; j <= p; ++j) {
+    left[j] = x - U[span + 1 - j];
+    right[j] = U[span + j] - x;
+    double saved = 0.0;
+    for (int r = 0; r < j; ++r) {
+      double temp = N[r] / (right[r + 1] + left[j - r]);
+      N[r] = saved + right[r + 1] * temp;
+      saved = left[j - r] * temp;
+    }
+    N[j] = saved;
+  }
+}
+
+// Cardinal B-spline of order o on [0, o] (uniform knots).
+static double cardinal(int o, double t) {
+  if (t < 0 || t >= o) return 0.0;
+  if (o == 1) return 1.0;
+  return (t * cardinal(o - 1, t) + (o - t) * cardinal(o - 1, t - 1)) / (o - 1);
+}
+
+static void spline_axis(SplineAxis& A, int mode, int order, int param, int ne, double h, int* ncoef) {
+  A.order = order;
+  A.idx.assign((size_t)ne * order, 0);
+  A.wt.assign((size_t)ne * order, 0.0);
+  A.ok.assign(ne, 1);
+  if (mode == 0) {
+    // clamped knots on the box enlarged by 2*box/(P+1) per side
+    // (Grid::set_spline_knot_series grid/Grid.cpp:3253-3377)
+    int P = param;
+    double bmin = 0, bmax = ne * h, delta = bmax - bmin;
+    double lo = bmin - 2.0 * delta / (P + 1), hi = bmax + 2.0 * delta / (P + 1);
+    double step = (hi - lo) / (P + 1);
+    std::vector<double> U;
+    for (int i = 0; i < order; ++i) U.push_back(lo);
+    double t = lo;
+    for (int i = 0; i < P; ++i) { t += step; U.push_back(t); }
+    for (int i = 0; i < order; ++i) U.push_back(hi);
+    int nb = order + P;
+    *ncoef = nb;
+    for (int e = 0; e < ne; ++e) {
+      double x = (e + 0.5) * h;
+      int i = (int)((x - lo) / step) + order;  // coeff2density calc_node, grid/Grid.cu:3393-3402
+      if (i < order || i > nb) { A.ok[e] = 0; continue; }
+      double N[16];
+      basis_funs(i - 1, x, order - 1, U, N);
+      for (int q = 0; q < order; ++q) {
+        A.idx[(size_t)e * order + q] = i - order + q;
+        A.wt[(size_t)e * order + q] = N[q];
+      }
+    }
+  } else {
+    // uniform lattice, coefficient i at node param*i, boundary coefficients replicated
+    int s = param;
+    int nc = (ne + s - 1) / s + 1;
+    *ncoef = nc;
+    for (int e = 0; e < ne; ++e) {
+      double t = (e + 0.5) / s;                     // lattice coordinate of the centre
+      int i0 = (int)std::floor(t - order / 2.0) + 1;
+      for (int q = 0; q < order; ++q) {
+        int i = i0 + q;
+        double w = cardinal(order, t - i + order / 2.0);
+        A.idx[(size_t)e * order + q] = std::min(std::max(i, 0), nc - 1);
+        A.wt[(size_t)e * order + q] = w;
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// MMA (optimization/mma_t.cu, mma_t.cpp), FP64.
+// ------------------------------------------------------------------------------------
+static void mma_clamp_asymptotes(Mma& M) {  // mma_t.cu:747-761
+  for (int j = 0; j < M.n; ++j) {
+    double st = M.xmax[j] - M.xmin[j];
+    M.L[j] = std::min(std::max(M.L[j], M.x[j] - 10 * st), M.x[j] - 0.01 * st);
+    M.U[j] = std::min(std::max(M.U[j], M.x[j] + 0.01 * st), M.x[j] + 10 * st);
+    M.alpha[j] = std::max(M.xmin[j], std::max(M.L[j] + 0.1 * (M.x[j] - M.L[j]), M.x[j] - M.move * st));
+    M.beta[j] = std::min(M.xmax[j], std::min(M.U[j] - 0.1 * (M.U[j] - M.x[j]), M.x[j] + M.move * st));
+  }
+}
+
+static void mma_init_vars(Mma& M) {  // mma_t.cu:797-807 and the constructor mma_t.h:202-214
+  M.y.assign(M.m, 1); M.lam.assign(M.m, 1); M.s.assign(M.m, 1); M.mu.assign(M.m, 1);
+  M.a.assign(M.m, 0); M.c.assign(M.m, 10); M.d.assign(M.m, 1);
+  M.z = 1; M.zeta = 1; M.a0 = 1; M.eps = 1;
+  M.xi.assign(M.n, 1); M.eta.assign(M.n, 1);
+  for (int j = 0; j < M.n; ++j) {
+    M.x[j] = 0.5 * (M.alpha[j] + M.beta[j]);
+    M.xi[j] = std::max(1.0 / (M.x[j] - M.alpha[j]), 1.0);
+    M.eta[j] = std::max(1.0 / (M.beta[j] - M.x[j]), 1.0);
+  }
+  for (int i = 0; i < M.m; ++i) M.mu[i] = std::max(M.c[i] / 2, 1.0);
+}
+
+struct MmaW {  // w = (x, y, z, lambda, xi, eta, mu, zeta, s)
+  std::vector<double> x, y, lam, xi, eta, mu, s;
+  double z, zeta;
+};
+
+static void mma_pqlam(Mma& M, const std::vector<double>& lam) {  // update_pqlambda mma_t.cu:681-694
+  for (int j = 0; j < M.n; ++j) {
+    double pl = M.p[0][j], ql = M.q[0][j];
+    for (int i = 0; i < M.m; ++i) { pl += lam[i] * M.p[i + 1][j]; ql += lam[i] * M.q[i + 1][j]; }
+    M.plam[j] = pl; M.qlam[j] = ql;
+  }
+}
+
+static std::pair<double, double> mma_kkt(Mma& M, const MmaW& w, double damp) {  // mma_t.cu:574-678
+  double sum = 0, mx = -10;
+  mma_pqlam(M, w.lam);
+  auto acc = [&](double v) { sum += v * v; mx = std::max(mx, std::fabs(v)); };
+  double sx = 0, mxx = 0;
+  for (int j = 0; j < M.n; ++j) {
+    double ux = M.U[j] - w.x[j], xl = w.x[j] - M.L[j];
+    double v = M.plam[j] / (ux * ux) - M.qlam[j] / (xl * xl) - w.xi[j] + w.eta[j];
+    sx += v * v; mxx = std::max(mxx, std::fabs(v));
+  }
+  sum += sx; mx = std::max(mx, mxx);
+  for (int i = 0; i < M.m; ++i) acc(M.c[i] + M.d[i] * w.y[i] - w.lam[i] - w.mu[i]);
+  double la = 0;
+  for (int i = 0; i < M.m; ++i) la += w.lam[i] * M.a[i];
+  acc(M.a0 - w.zeta - la);
+  for (int i = 0; i < M.m; ++i) {
+    double g = 0;
+    for (int j = 0; j < M.n; ++j)
+      g += M.p[i + 1][j] / (M.U[j] - w.x[j]) + M.q[i + 1][j] / (w.x[j] - M.L[j]);
+    double v = (g - M.a[i] * w.z - w.y[i] + w.s[i] - M.b[i]) / damp;
+    acc(v);
+  }
+  double s1 = 0, m1 = 0, s2 = 0, m2 = 0;
+  for (int j = 0; j < M.n; ++j) {
+    double v1 = w.xi[j] * (w.x[j] - M.alpha[j]) - M.eps;
+    double v2 = w.eta[j] * (M.beta[j] - w.x[j]) - M.eps;
+    s1 += v1 * v1; m1 = std::max(m1, std::fabs(v1));
+    s2 += v2 * v2; m2 = std::max(m2, std::fabs(v2));
+  }
+  sum += s1; mx = std::max(mx, m1);
+  sum += s2; mx = std::max(mx, m2);
+  for (int i = 0; i < M.m; ++i) acc(w.mu[i] * w.y[i] - M.eps);
+  acc(w.zeta * w.z - M.eps);
+  for (int i = 0; i < M.m; ++i) acc(w.lam[i] * w.s[i] - M.eps);
+  return {std::sqrt(sum), mx};
+}
+
+static void solve_small(int n, std::vector<double>& A, std::vector<double>& b) {
+  // Gaussian elimination with partial pivoting (the reference uses Eigen
+  // colPivHouseholderQr, optimization/mma_t.cpp:83-118; same solution when regular).
+  for (int k = 0; k < n; ++k) {
+    int piv = k;
+    for (int i = k + 1; i < n; ++i)
+      if (std::fabs(A[i * n + k]) > std::fabs(A[piv * n + k])) piv = i;
+    if (piv != k) {
+      for (int j = 0; j < n; ++j) std::swap(A[k * n + j], A[piv * n + j]);
+      std::swap(b[k], b[piv]);
+    }
+    for (int i = k + 1; i < n; ++i) {
+      double f = A[i * n + k] / A[k * n + k];
+      for (int j = k; j < n; ++j) A[i * n + j] -= f * A[k * n + j];
+      b[i] -= f * b[k];
+    }
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    double s = b[i];
+    for (int j = i + 1; j < n; ++j) s -= A[i * n + j] * b[j];
+    b[i] = s / A[i * n + i];
+  }
+}
+
+static bool mma_update(Mma& M, const double* df, const double* dg, const double* g) {
+  const int n = M.n, m = M.m;
+  std::vector<double> curx = M.x;
+  // mma_subproblem_t::init, mma_t.cu:15-118
+  M.p.assign(m + 1, std::vector<double>(n));
+  M.q.assign(m + 1, std::vector<double>(n));
+  for (int i = 0; i <= m; ++i) {
+    const double* d = i == 0 ? df : dg + (size_t)(i - 1) * n;
+    for (int j = 0; j < n; ++j) {
+      double ux = M.U[j] - M.x[j], xl = M.x[j] - M.L[j];
+      double dp = std::max(d[j], 0.0), dm = std::max(-d[j], 0.0);
+      double r = 1e-5 / (M.xmax[j] - M.xmin[j]);
+      M.p[i][j] = ux * ux * (1.001 * dp + 0.001 * dm + r);
+      M.q[i][j] = xl * xl * (0.001 * dp + 1.001 * dm + r);
+    }
+  }
+  M.b.assign(m, 0);
+  for (int i = 0; i < m; ++i) {
+    double gp = 0;
+    for (int j = 0; j < n; ++j) gp += M.p[i + 1][j] / (M.U[j] - M.x[j]) + M.q[i + 1][j] / (M.x[j] - M.L[j]);
+    M.b[i] = gp - g[i];
+  }
+  // reset, mma_t.cu:702-713
+  for (int j = 0; j < n; ++j) {
+    M.x[j] = 0.5 * (M.alpha[j] + M.beta[j]);
+    M.xi[j] = std::max(1.0 / (M.x[j] - M.alpha[j]), 1.0);
+    M.eta[j] = std::max(1.0 / (M.beta[j] - M.x[j]), 1.0);
+  }
+  M.eps = 1; M.zeta = 1; M.z = 1;
+  for (int i = 0; i < m; ++i) { M.y[i] = 1; M.lam[i] = 1; M.s[i] = 1; M.mu[i] = std::max(M.c[i] / 2, 1.0); }
+  M.plam.assign(n, 0); M.qlam.assign(n, 0);
+  mma_pqlam(M, M.lam);
+
+  // mma_subproblem_t::solve, mma_t.cu:330-499
+  double lasterr = 0;
+  int nsub = 0;
+  bool overflow = false;
+  std::vector<double> Dx(n), dx(n), dxi(n), deta(n), G((size_t)m * n), gp(m), dlx(n);
+  for (int lev = 0; lev < 6 && !overflow; ++lev) {  // eps = 1 .. 1e-5
+    double err = 1e30, damp = 1;
+    int stays = 0;
+    while (err > 0.9 * M.eps) {
+      for (int j = 0; j < n; ++j) {
+        double ux = M.U[j] - M.x[j], xl = M.x[j] - M.L[j];
+        double psi = 2 * M.plam[j] / (ux * ux * ux) + 2 * M.qlam[j] / (xl * xl * xl);
+        Dx[j] = psi + M.xi[j] / (M.x[j] - M.alpha[j]) + M.eta[j] / (M.beta[j] - M.x[j]);
+        for (int i = 0; i < m; ++i)
+          G[(size_t)i * n + j] = M.p[i + 1][j] / (ux * ux) - M.q[i + 1][j] / (xl * xl);
+        double phi = M.plam[j] / (ux * ux) - M.qlam[j] / (xl * xl);
+        dlx[j] = -(M.eps / (M.x[j] - M.alpha[j]) - M.eps / (M.beta[j] - M.x[j]) - phi);
+      }
+      for (int i = 0; i < m; ++i) {
+        double s = 0;
+        for (int j = 0; j < n; ++j) s += M.p[i + 1][j] / (M.U[j] - M.x[j]) + M.q[i + 1][j] / (M.x[j] - M.L[j]);
+        gp[i] = s;
+      }
+      std::vector<double> Dy(m), dly(m), dll(m);
+      for (int i = 0; i < m; ++i) {
+        Dy[i] = M.d[i] + M.mu[i] / M.y[i];
+        dly[i] = -(M.lam[i] + M.eps / M.y[i] - M.c[i] - M.d[i] * M.y[i]);
+        dll[i] = -(M.z * M.a[i] + M.y[i] + M.b[i] - M.eps / M.lam[i] - gp[i]);
+      }
+      double la = 0;
+      for (int i = 0; i < m; ++i) la += M.lam[i] * M.a[i];
+      double dlz = -(la + M.eps / M.z - M.a0);
+      // reduced (m+1) system, mma_t.cu:228-328
+      std::vector<double> A((m + 1) * (m + 1), 0.0), rhs(m + 1);
+      for (int i = 0; i < m; ++i) {
+        A[i * (m + 1) + i] += M.s[i] / M.lam[i] + 1.0 / Dy[i];
+        for (int k = 0; k < m; ++k) {
+          double s = 0;
+          for (int j = 0; j < n; ++j) s += G[(size_t)i * n + j] * (G[(size_t)k * n + j] / Dx[j]);
+          A[i * (m + 1) + k] += s;
+        }
+        double s = 0;
+        for (int j = 0; j < n; ++j) s += G[(size_t)i * n + j] * (dlx[j] / Dx[j]);
+        rhs[i] = dll[i] + dly[i] / Dy[i] - s;
+        A[i * (m + 1) + m] = M.a[i];
+        A[m * (m + 1) + i] = M.a[i];
+      }
+      A[m * (m + 1) + m] = -M.zeta / M.z;
+      rhs[m] = dlz;
+      solve_small(m + 1, A, rhs);
+      std::vector<double> dlam(rhs.begin(), rhs.begin() + m);
+      double dz = rhs[m];
+      for (int j = 0; j < n; ++j) {
+        double v = -dlx[j] / Dx[j];
+        for (int i = 0; i < m; ++i) v -= dlam[i] * G[(size_t)i * n + j] / Dx[j];
+        dx[j] = v;
+      }
+      std::vector<double> dy(m), dmu(m), ds(m);
+      for (int i = 0; i < m; ++i) dy[i] = dlam[i] / Dy[i] - dly[i] / Dy[i];
+      for (int j = 0; j < n; ++j) {
+        double xa = M.x[j] - M.alpha[j], bx = M.beta[j] - M.x[j];
+        dxi[j] = M.eps / xa - dx[j] / xa * M.xi[j] - M.xi[j];
+        deta[j] = M.eps / bx + M.eta[j] / bx * dx[j] - M.eta[j];
+      }
+      for (int i = 0; i < m; ++i) {
+        dmu[i] = M.eps / M.y[i] - dy[i] * M.mu[i] / M.y[i] - M.mu[i];
+        ds[i] = M.eps / M.lam[i] - M.s[i] / M.lam[i] * dlam[i] - M.s[i];
+      }
+      double dzeta = M.eps / M.z - M.zeta - (M.zeta / M.z) * dz;
+      // step bound (fraction 0.99 to the boundary)
+      const double cf = 1.0 / 0.99;
+      double t1 = -1e300, t2 = -1e300, t3 = -1e300;
+      for (int j = 0; j < n; ++j) {
+        t1 = std::max(t1, -cf * (dx[j] / (M.x[j] - M.alpha[j])));
+        t2 = std::max(t2, cf * (dx[j] / (M.beta[j] - M.x[j])));
+      }
+      auto art = [&](double dv, double v) { t3 = std::max(t3, -cf * dv / v); };
+      for (int i = 0; i < m; ++i) art(dy[i], M.y[i]);
+      art(dz, M.z);
+      for (int i = 0; i < m; ++i) art(dlam[i], M.lam[i]);
+      for (int j = 0; j < n; ++j) art(dxi[j], M.xi[j]);
+      for (int j = 0; j < n; ++j) art(deta[j], M.eta[j]);
+      for (int i = 0; i < m; ++i) art(dmu[i], M.mu[i]);
+      art(dzeta, M.zeta);
+      for (int i = 0; i < m; ++i) art(ds[i], M.s[i]);
+      double step = 1.0 / std::max(std::max(t1, t2), std::max(t3, 1.0));
+      // backtracking line search on the KKT 2-norm, mma_t.cu:501-571
+      MmaW cur{M.x, M.y, M.lam, M.xi, M.eta, M.mu, M.s, M.z, M.zeta};
+      double old_err = mma_kkt(M, cur, 1.0).first;
+      double new_err = 1e30, new_inf = 1e30;
+      while (step > 1e-12 && new_err > old_err) {
+        MmaW t;
+        t.x.resize(n); t.xi.resize(n); t.eta.resize(n);
+        t.y.resize(m); t.lam.resize(m); t.mu.resize(m); t.s.resize(m);
+        for (int j = 0; j < n; ++j) {
+          t.x[j] = M.x[j] + dx[j] * step;
+          t.xi[j] = M.xi[j] + dxi[j] * step;
+          t.eta[j] = M.eta[j] + deta[j] * step;
+        }
+        for (int i = 0; i < m; ++i) {
+          t.y[i] = M.y[i] + dy[i] * step;
+          t.lam[i] = M.lam[i] + dlam[i] * step;
+          t.mu[i] = M.mu[i] + dmu[i] * step;
+          t.s[i] = M.s[i] + ds[i] * step;
+        }
+        t.z = M.z + dz * step;
+        t.zeta = M.zeta + dzeta * step;
+        auto e2 = mma_kkt(M, t, damp);
+        new_err = e2.first; new_inf = e2.second;
+        if (new_err < old_err) {
+          M.x = t.x; M.y = t.y; M.z = t.z; M.lam = t.lam; M.xi = t.xi; M.eta = t.eta;
+          M.mu = t.mu; M.zeta = t.zeta; M.s = t.s;
+          break;
+        }
+        step /= 2;
+      }
+      err = new_inf;
+      if ((err - lasterr) / err < 1e-5) ++stays; else stays = 0;
+      lasterr = err;
+      if (stays > 6) { damp *= 10; stays = 0; }
+      if (nsub++ > 80) { overflow = true; break; }
+    }
+    M.eps *= 0.1;
+  }
+  // adjust_asym, mma_t.cu:164-226
+  std::vector<double> cdx(n);
+  for (int j = 0; j < n; ++j) cdx[j] = M.x[j] - curx[j];
+  for (int j = 0; j < n; ++j) {
+    double gam = 1;
+    double d1 = cdx[j], d2 = M.lastdx[j];
+    if (d1 * d2 > 0) gam = 1.2;
+    if (d1 * d2 < 0) gam = 0.7;
+    double st = M.xmax[j] - M.xmin[j];
+    double xv = M.x[j];
+    double al = xv - gam * (xv - d1 - M.L[j]);
+    double au = xv + gam * (M.U[j] - xv + d1);
+    al = std::min(std::max(al, xv - 10 * st), xv - 0.01 * st);
+    au = std::min(std::max(au, xv + 0.01 * st), xv + 10 * st);
+    M.L[j] = al; M.U[j] = au;
+    double na = M.xmin[j];
+    na = std::max(na, al + 0.1 * (xv - al));
+    na = std::max(na, xv - M.move * st);
+    double nb = M.xmax[j];
+    nb = std::min(nb, au - 0.1 * (au - xv));
+    nb = std::min(nb, xv + M.move * st);
+    M.alpha[j] = na; M.beta[j] = nb;
+  }
+  M.lastdx = cdx;
+  double nrm = 0;
+  for (double v : cdx) nrm += v * v;
+  if (std::sqrt(nrm) < 1e-3) M.stop_counter++;
+  return M.stop_counter > 10;
+}
+
+}  // namespace orc
+
+using namespace orc;
+
+// ======================================================================================
+// C ABI (same signatures as include/tos_api.h, prefix orc_)
+// ======================================================================================
+struct tos_ctx : orc::Ctx {};
+struct tos_mma : orc::Mma {};
+
+#define GUARD(ctx)                        \
+  if (!(ctx)) return TOS_E_ARG;           \
+  try {
+#define END_GUARD(ctx)                    \
+  } catch (const std::exception& ex) {    \
+    (ctx)->err = ex.what();               \
+    return TOS_E_ARG;                     \
+  }
+
+ORC_API int orc_ctx_create(const tos_grid_desc* d, tos_ctx** out) {
+  if (!d || !out || d->nx <= 0 || d->ny <= 0 || d->nz <= 0 || !d->solid || !(d->h > 0)) return TOS_E_ARG;
+  tos_ctx* c = new tos_ctx();
+  c->nx = d->nx; c->ny = d->ny; c->nz = d->nz; c->h = d->h;
+  int64_t ne = (int64_t)d->nx * d->ny * d->nz;
+  int64_t nn = (int64_t)(d->nx + 1) * (d->ny + 1) * (d->nz + 1);
+  c->solid.assign(d->solid, d->solid + ne);
+  if (d->support) c->support.assign(d->support, d->support + nn);
+  if (d->shell) c->shell.assign(d->shell, d->shell + ne);
+  build_hierarchy(*c, d->n_levels, d->coarsen_threshold > 0 ? d->coarsen_threshold : 400);
+  c->n_active = 0;
+  for (auto v : c->L[0].eact) c->n_active += v;
+  c->rho.assign(ne, 0.0f);
+  c->rho_pre.assign(ne, 0.0f);
+  c->sens.assign(ne, 0.0f);
+  c->vsens.assign(ne, 0.0f);
+  c->w.assign(ne, 0.0);
+  element_matrix(c->h, c->E, c->nu, c->Ke);
+  galerkin_child_matrices(c->Ke, c->Mc);
+  c->K = 1;
+  c->F.assign(3 * nn, 0.0);
+  c->U.assign(3 * nn, 0.0);
+  *out = c;
+  return TOS_OK;
+}
+
+ORC_API int orc_ctx_destroy(tos_ctx* c) { delete c; return TOS_OK; }
+ORC_API const char* orc_last_error(const tos_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+ORC_API int orc_get_info(tos_ctx* c, tos_info* o) {
+  if (!c || !o) return TOS_E_ARG;
+  std::memset(o, 0, sizeof(*o));
+  o->n_levels = (int)c->L.size();
+  for (size_t l = 0; l < c->L.size() && l < 16; ++l) {
+    o->level_dims[l][0] = c->L[l].ex; o->level_dims[l][1] = c->L[l].ey; o->level_dims[l][2] = c->L[l].ez;
+    int64_t na = 0, nf = 0;
+    for (auto v : c->L[l].eact) na += v;
+    for (auto v : c->L[l].fre) nf += v;
+    o->n_active_elements[l] = na; o->n_free_nodes[l] = nf;
+  }
+  o->n_elements = c->L[0].ne; o->n_nodes = c->L[0].nn;
+  int64_t nfc = 0;
+  for (auto v : c->L.back().fre) nfc += v;
+  o->coarse_dofs = c->L.size() > 1 ? (int)(3 * nfc) : 0;
+  o->n_cases = c->K;
+  o->n_coeffs = (int64_t)c->ncx * c->ncy * c->ncz;
+  return TOS_OK;
+}
+
+ORC_API int orc_get_node_flags(tos_ctx* c, int level, uint8_t* out) {
+  if (!c || !out || level < 0 || level >= (int)c->L.size()) return TOS_E_ARG;
+  const Level& L = c->L[level];
+  for (int z = 0; z < L.nz; ++z)
+    for (int y = 0; y < L.ny; ++y)
+      for (int x = 0; x < L.nx; ++x) {
+        int64_t n = L.nid(x, y, z);
+        bool act = false;
+        for (int a = 0; a < 8 && !act; ++a) {
+          int ex = x - 1 + (a & 1), ey = y - 1 + ((a >> 1) & 1), ez = z - 1 + (a >> 2);
+          if (ex < 0 || ey < 0 || ez < 0 || ex >= L.ex || ey >= L.ey || ez >= L.ez) continue;
+          act = L.eact[L.eid(ex, ey, ez)];
+        }
+        bool sup = level == 0 && c->support.size() && c->support[n];
+        int col = (x & 1) + 2 * (y & 1) + 4 * (z & 1);
+        out[n] = (uint8_t)((act ? 1 : 0) | (sup ? 2 : 0) | (L.fre[n] ? 4 : 0) | (col << 3));
+      }
+  return TOS_OK;
+}
+
+ORC_API int orc_set_material(tos_ctx* c, double E, double nu, double p, double rmin) {
+  if (!c || !(E > 0) || !(nu > -1 && nu < 0.5) || !(p >= 1) || !(rmin > 0)) return TOS_E_ARG;
+  c->E = E; c->nu = nu; c->penal = p; c->rho_min = rmin;
+  element_matrix(c->h, E, nu, c->Ke);
+  galerkin_child_matrices(c->Ke, c->Mc);
+  c->op_ready = false;
+  return TOS_OK;
+}
+
+ORC_API int orc_get_template_matrix(tos_ctx* c, double* ke) {
+  if (!c || !ke) return TOS_E_ARG;
+  for (int i = 0; i < 24; ++i)
+    for (int j = 0; j < 24; ++j) ke[i * 24 + j] = c->Ke[i][j];
+  return TOS_OK;
+}
+
+ORC_API int orc_set_loads(tos_ctx* c, int K, const double* f) {
+  if (!c || K <= 0 || K > 64 || !f) return TOS_E_ARG;
+  const Level& L = c->L[0];
+  int64_t n3 = 3 * L.nn;
+  c->K = K;
+  c->F.assign(f, f + K * n3);
+  for (int k = 0; k < K; ++k) mask_free(L, c->F.data() + k * n3);
+  c->U.assign(K * n3, 0.0);
+  return TOS_OK;
+}
+
+ORC_API int orc_get_loads(tos_ctx* c, int k, double* f) {
+  if (!c || !f || k < 0 || k >= c->K) return TOS_E_ARG;
+  int64_t n3 = 3 * c->L[0].nn;
+  std::copy(c->F.begin() + k * n3, c->F.begin() + (k + 1) * n3, f);
+  return TOS_OK;
+}
+
+ORC_API int orc_set_density(tos_ctx* c, const float* r) {
+  if (!c || !r) return TOS_E_ARG;
+  for (int64_t e = 0; e < c->L[0].ne; ++e) {
+    if (!std::isfinite(r[e])) { c->err = "non-finite density"; return TOS_E_NAN; }
+    c->rho[e] = c->L[0].eact[e] ? r[e] : 0.0f;
+  }
+  c->op_ready = false;
+  return TOS_OK;
+}
+
+ORC_API int orc_get_density(tos_ctx* c, float* r) {
+  if (!c || !r) return TOS_E_ARG;
+  std::copy(c->rho.begin(), c->rho.end(), r);
+  return TOS_OK;
+}
+
+ORC_API int orc_update_operator(tos_ctx* c) {
+  if (!c) return TOS_E_ARG;
+  const Level& L0 = c->L[0];
+  for (int64_t e = 0; e < L0.ne; ++e)
+    c->w[e] = L0.eact[e] ? std::pow((double)c->rho[e], c->penal) : 0.0;
+  if (c->L.size() > 1) galerkin_first(*c);
+  for (size_t l = 2; l < c->L.size(); ++l) galerkin_next(c->L[l - 1], c->L[l]);
+  int rc = coarse_factor(*c);
+  if (rc) return rc;
+  c->op_ready = true;
+  return TOS_OK;
+}
+
+ORC_API int orc_get_stencil(tos_ctx* c, int level, double* out) {
+  if (!c || !out || level < 1 || level >= (int)c->L.size() || !c->op_ready) return TOS_E_ARG;
+  std::copy(c->L[level].st.begin(), c->L[level].st.end(), out);
+  return TOS_OK;
+}
+
+ORC_API int orc_apply_operator(tos_ctx* c, const double* u, double* y) {
+  if (!c || !u || !y) return TOS_E_ARG;
+  if (!c->op_ready) { c->err = "update_operator first"; return TOS_E_STATE; }
+  std::vector<double> uu(u, u + 3 * c->L[0].nn);
+  mask_free(c->L[0], uu.data());
+  apply_fine(*c, uu.data(), y);
+  return TOS_OK;
+}
+
+ORC_API int orc_vcycle(tos_ctx* c, const double* r, double* z, int precision, int pre, int post) {
+  (void)precision;
+  if (!c || !r || !z) return TOS_E_ARG;
+  if (!c->op_ready) { c->err = "update_operator first"; return TOS_E_STATE; }
+  std::vector<double> rr(r, r + 3 * c->L[0].nn);
+  mask_free(c->L[0], rr.data());
+  precondition(*c, rr.data(), z, pre, post);
+  return TOS_OK;
+}
+
+ORC_API int orc_solve(tos_ctx* c, const tos_solve_opts* o, tos_solve_stats* st) {
+  if (!c || !o) return TOS_E_ARG;
+  if (!c->op_ready) { c->err = "update_operator first"; return TOS_E_STATE; }
+  auto t0 = std::chrono::steady_clock::now();
+  int rc_all = TOS_OK;
+  for (int k = 0; k < c->K; ++k) {
+    int its; double rr;
+    int rc = solve_case(*c, k, *o, &its, &rr);
+    if (st && k < 64) { st->iterations[k] = its; st->rel_res[k] = rr; }
+    if (rc == TOS_E_NAN) return rc;
+    if (rc) rc_all = rc;
+  }
+  if (st) st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return rc_all;
+}
+
+ORC_API int orc_get_displacement(tos_ctx* c, int k, double* u) {
+  if (!c || !u || k < 0 || k >= c->K) return TOS_E_ARG;
+  int64_t n3 = 3 * c->L[0].nn;
+  std::copy(c->U.begin() + k * n3, c->U.begin() + (k + 1) * n3, u);
+  return TOS_OK;
+}
+
+ORC_API int orc_set_displacement(tos_ctx* c, int k, const double* u) {
+  if (!c || !u || k < 0 || k >= c->K) return TOS_E_ARG;
+  int64_t n3 = 3 * c->L[0].nn;
+  std::copy(u, u + n3, c->U.begin() + k * n3);
+  mask_free(c->L[0], c->U.data() + k * n3);
+  return TOS_OK;
+}
+
+ORC_API int orc_compliance(tos_ctx* c, double* out) {
+  if (!c || !out) return TOS_E_ARG;
+  const Level& L = c->L[0];
+  int64_t n3 = 3 * L.nn;
+  for (int k = 0; k < c->K; ++k) out[k] = dot3(L, c->F.data() + k * n3, c->U.data() + k * n3);
+  return TOS_OK;
+}
+
+ORC_API int orc_volume(tos_ctx* c, double* v) {
+  if (!c || !v) return TOS_E_ARG;
+  const Level& L = c->L[0];
+  double s = chunked_sum(L.ne, [&](int64_t e) { return L.eact[e] ? (double)c->rho[e] : 0.0; });
+  *v = s / (double)c->n_active;
+  return TOS_OK;
+}
+
+ORC_API int orc_sensitivity(tos_ctx* c, int k) {
+  if (!c || k < 0 || k >= c->K) return TOS_E_ARG;
+  const Level& L = c->L[0];
+  const double* u = c->U.data() + k * 3 * L.nn;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int z = 0; z < L.ez; ++z)
+    for (int y = 0; y < L.ey; ++y)
+      for (int x = 0; x < L.ex; ++x) {
+        int64_t e = L.eid(x, y, z);
+        if (!L.eact[e]) { c->sens[e] = 0; continue; }
+        double q = element_energy(*c, u, x, y, z);
+        c->sens[e] = (float)(-c->penal * std::pow((double)c->rho[e], c->penal - 1) * q);
+      }
+  return TOS_OK;
+}
+
+ORC_API int orc_element_compliance(tos_ctx* c, int k, double* out) {
+  if (!c || !out || k < 0 || k >= c->K) return TOS_E_ARG;
+  const Level& L = c->L[0];
+  const double* u = c->U.data() + k * 3 * L.nn;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int z = 0; z < L.ez; ++z)
+    for (int y = 0; y < L.ey; ++y)
+      for (int x = 0; x < L.ex; ++x) {
+        int64_t e = L.eid(x, y, z);
+        out[e] = L.eact[e] ? c->w[e] * element_energy(*c, u, x, y, z) : 0.0;
+      }
+  return TOS_OK;
+}
+
+static std::vector<float>* sens_buf(tos_ctx* c, int which) {
+  return which == 0 ? &c->sens : which == 1 ? &c->vsens : nullptr;
+}
+
+ORC_API int orc_get_sensitivity(tos_ctx* c, int which, float* out) {
+  auto* b = c ? sens_buf(c, which) : nullptr;
+  if (!b || !out) return TOS_E_ARG;
+  std::copy(b->begin(), b->end(), out);
+  return TOS_OK;
+}
+
+ORC_API int orc_set_sensitivity(tos_ctx* c, int which, const float* in) {
+  auto* b = c ? sens_buf(c, which) : nullptr;
+  if (!b || !in) return TOS_E_ARG;
+  for (int64_t e = 0; e < c->L[0].ne; ++e) (*b)[e] = c->L[0].eact[e] ? in[e] : 0.0f;
+  return TOS_OK;
+}
+
+ORC_API int orc_init_vol_sensitivity(tos_ctx* c) {
+  if (!c) return TOS_E_ARG;
+  float v = (float)(1.0 / (double)c->n_active);
+  for (int64_t e = 0; e < c->L[0].ne; ++e) c->vsens[e] = c->L[0].eact[e] ? v : 0.0f;
+  return TOS_OK;
+}
+
+ORC_API int orc_filter(tos_ctx* c, int which, double R) {
+  auto* b = c ? sens_buf(c, which) : nullptr;
+  if (!b || !(R > 0)) return TOS_E_ARG;
+  filter_field(*c, *b, R);
+  return TOS_OK;
+}
+
+ORC_API int orc_oc_update(tos_ctx* c, double vgoal, double move, double damp, double* g_out, int* steps) {
+  if (!c) return TOS_E_ARG;
+  const Level& L = c->L[0];
+  double gmax = 0;
+  for (int64_t e = 0; e < L.ne; ++e)
+    if (L.eact[e]) gmax = std::max(gmax, (double)std::fabs(c->sens[e]));
+  double lo = 0, hi = gmax, g = 0, V = 2;
+  int itn = 0, trials = 0;
+  do {
+    g = (lo + hi) / 2;
+    V = oc_trial(*c, g, move, damp, nullptr);
+    ++trials;
+    if (V > vgoal) lo = g;
+    else if (V < vgoal) hi = g;
+  } while (std::fabs(V - vgoal) > 1e-4 && itn++ < 30);
+  oc_trial(*c, g, move, damp, &c->rho);
+  for (int64_t e = 0; e < L.ne; ++e)
+    if (!L.eact[e]) c->rho[e] = 0;
+  if (g_out) *g_out = g;
+  if (steps) *steps = trials;
+  c->op_ready = false;
+  return TOS_OK;
+}
+
+// Heaviside projection (computeProjection_kernel optimization/optimization.cu:313-337).
+ORC_API int orc_project_density(tos_ctx* c, double beta) {
+  if (!c) return TOS_E_ARG;
+  const double eta = 0.5;
+  double den = std::tanh(beta * eta) + std::tanh(beta * (1 - eta));
+  c->rho_pre = c->rho;
+  for (int64_t e = 0; e < c->L[0].ne; ++e) {
+    if (!c->L[0].eact[e]) continue;
+    double r = (std::tanh(beta * eta) + std::tanh(beta * (c->rho_pre[e] - eta))) / den;
+    r = std::min(std::max(r, c->rho_min), 1.0);
+    if (c->shell.size() && c->shell[e]) r = 1;
+    c->rho[e] = (float)r;
+  }
+  c->op_ready = false;
+  return TOS_OK;
+}
+
+// computeProjectionSensitivity_kernel optimization/optimization.cu:161-183.
+ORC_API int orc_projection_sensitivity(tos_ctx* c, int which, double beta) {
+  auto* b = c ? sens_buf(c, which) : nullptr;
+  if (!b) return TOS_E_ARG;
+  const double eta = 0.5;
+  double den = std::tanh(beta * eta) + std::tanh(beta * (1 - eta));
+  for (int64_t e = 0; e < c->L[0].ne; ++e) {
+    if (!c->L[0].eact[e]) continue;
+    double t = std::tanh(beta * (c->rho_pre[e] - eta));
+    (*b)[e] = (float)((*b)[e] * (beta * (1 - t * t) / den));
+  }
+  return TOS_OK;
+}
+
+ORC_API int orc_spline_init(tos_ctx* c, int mode, int order, int param) {
+  if (!c || (mode != 0 && mode != 1) || order < 2 || order > 8 || param < 1) return TOS_E_ARG;
+  c->sp_mode = mode; c->sp_order = order; c->sp_param = param;
+  int dims[3] = {c->nx, c->ny, c->nz};
+  int nc[3];
+  for (int a = 0; a < 3; ++a) spline_axis(c->ax[a], mode, order, param, dims[a], c->h, &nc[a]);
+  c->ncx = nc[0]; c->ncy = nc[1]; c->ncz = nc[2];
+  size_t n = (size_t)nc[0] * nc[1] * nc[2];
+  c->coef.assign(n, 0.0f); c->csens.assign(n, 0.0f); c->cvsens.assign(n, 0.0f);
+  return TOS_OK;
+}
+
+ORC_API int orc_spline_set_coeffs(tos_ctx* c, const float* v) {
+  if (!c || !v || c->sp_mode < 0) return TOS_E_ARG;
+  std::copy(v, v + c->coef.size(), c->coef.begin());
+  return TOS_OK;
+}
+
+ORC_API int orc_spline_get_coeffs(tos_ctx* c, float* v) {
+  if (!c || !v || c->sp_mode < 0) return TOS_E_ARG;
+  std::copy(c->coef.begin(), c->coef.end(), v);
+  return TOS_OK;
+}
+
+// coeff2density (grid/Grid.cu:3325-3430): tensor-product sum at the element centre,
+// -0.2 outside the knot range, clamp to [rho_min, 1], shell -> 1.
+ORC_API int orc_spline_eval(tos_ctx* c) {
+  if (!c || c->sp_mode < 0) return TOS_E_ARG;
+  const Level& L = c->L[0];
+  const int o = c->sp_order;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int z = 0; z < L.ez; ++z)
+    for (int y = 0; y < L.ey; ++y)
+      for (int x = 0; x < L.ex; ++x) {
+        int64_t e = L.eid(x, y, z);
+        if (!L.eact[e]) { c->rho[e] = 0; continue; }
+        double v;
+        if (!c->ax[0].ok[x] || !c->ax[1].ok[y] || !c->ax[2].ok[z]) {
+          v = -0.2;
+        } else {
+          v = 0;
+          for (int k = 0; k < o; ++k)
+            for (int j = 0; j < o; ++j)
+              for (int i = 0; i < o; ++i) {
+                int64_t ci = c->ax[0].idx[x * o + i] +
+                             (int64_t)c->ncx * (c->ax[1].idx[y * o + j] + (int64_t)c->ncy * c->ax[2].idx[z * o + k]);
+                v += (double)c->coef[ci] * c->ax[0].wt[x * o + i] * c->ax[1].wt[y * o + j] * c->ax[2].wt[z * o + k];
+              }
+        }
+        v = std::min(std::max(v, c->rho_min), 1.0);
+        if (c->shell.size() && c->shell[e]) v = 1;
+        c->rho[e] = (float)v;
+      }
+  c->op_ready = false;
+  return TOS_OK;
+}
+
+// ddensity2dcoeff_update / dvol2dcoeff (grid/Grid.cu:3643-3859): dJ/dc = sum over active,
+// non-shell elements inside the knot range of g_e N_i N_j N_k (sequential scatter here).
+ORC_API int orc_spline_adjoint(tos_ctx* c, int which) {
+  auto* b = c ? sens_buf(c, which) : nullptr;
+  if (!b || c->sp_mode < 0) return TOS_E_ARG;
+  const Level& L = c->L[0];
+  const int o = c->sp_order;
+  std::vector<double> acc(c->coef.size(), 0.0);
+  for (int z = 0; z < L.ez; ++z)
+    for (int y = 0; y < L.ey; ++y)
+      for (int x = 0; x < L.ex; ++x) {
+        int64_t e = L.eid(x, y, z);
+        if (!L.eact[e] || (c->shell.size() && c->shell[e])) continue;
+        if (!c->ax[0].ok[x] || !c->ax[1].ok[y] || !c->ax[2].ok[z]) continue;
+        double g = (*b)[e];
+        for (int k = 0; k < o; ++k)
+          for (int j = 0; j < o; ++j)
+            for (int i = 0; i < o; ++i) {
+              int64_t ci = c->ax[0].idx[x * o + i] +
+                           (int64_t)c->ncx * (c->ax[1].idx[y * o + j] + (int64_t)c->ncy * c->ax[2].idx[z * o + k]);
+              acc[ci] += g * c->ax[0].wt[x * o + i] * c->ax[1].wt[y * o + j] * c->ax[2].wt[z * o + k];
+            }
+      }
+  auto& out = which == 0 ? c->csens : c->cvsens;
+  for (size_t i = 0; i < acc.size(); ++i) out[i] = (float)acc[i];
+  return TOS_OK;
+}
+
+ORC_API int orc_spline_get_coeff_sens(tos_ctx* c, int which, float* out) {
+  if (!c || !out || c->sp_mode < 0 || (which != 0 && which != 1)) return TOS_E_ARG;
+  auto& v = which == 0 ? c->csens : c->cvsens;
+  std::copy(v.begin(), v.end(), out);
+  return TOS_OK;
+}
+
+ORC_API int orc_mma_create(int n, int m, const double* xmin, const double* xmax, double move, tos_mma** out) {
+  if (n <= 0 || m <= 0 || m > 8 || !xmin || !xmax || !out || !(move > 0)) return TOS_E_ARG;
+  tos_mma* M = new tos_mma();
+  M->n = n; M->m = m; M->move = move;
+  M->xmin.assign(xmin, xmin + n); M->xmax.assign(xmax, xmax + n);
+  // set_bound + init_subproblem_variable (mma_t.cu:797-855)
+  M->alpha = M->xmin; M->beta = M->xmax;
+  M->L = M->alpha; M->U = M->beta;
+  M->x.resize(n);
+  for (int j = 0; j < n; ++j) M->x[j] = 0.5 * (M->alpha[j] + M->beta[j]);
+  mma_clamp_asymptotes(*M);
+  mma_init_vars(*M);
+  M->lastdx.assign(n, 0.0);
+  *out = M;
+  return TOS_OK;
+}
+
+ORC_API int orc_mma_destroy(tos_mma* M) { delete M; return TOS_OK; }
+
+ORC_API int orc_mma_set_x(tos_mma* M, const double* x) {
+  if (!M || !x) return TOS_E_ARG;
+  for (int j = 0; j < M->n; ++j) {
+    double st = M->xmax[j] - M->xmin[j];
+    M->x[j] = x[j];
+    M->L[j] = x[j] - 0.5 * st;
+    M->U[j] = x[j] + 0.5 * st;
+  }
+  mma_clamp_asymptotes(*M);
+  return TOS_OK;
+}
+
+ORC_API int orc_mma_get_x(tos_mma* M, double* x) {
+  if (!M || !x) return TOS_E_ARG;
+  std::copy(M->x.begin(), M->x.end(), x);
+  return TOS_OK;
+}
+
+ORC_API int orc_mma_update(tos_mma* M, const double* df, const double* dg, const double* g, int* conv) {
+  if (!M || !df || !dg || !g) return TOS_E_ARG;
+  bool cv = mma_update(*M, df, dg, g);
+  if (conv) *conv = cv ? 1 : 0;
+  return TOS_OK;
+}
+
+ORC_API int orc_ctx_mma_step(tos_ctx* c, tos_mma* M, double sscale, double vscale, double vfrac, int* conv) {
+  if (!c || !M || c->sp_mode < 0 || M->n != (int)c->coef.size() || M->m != 1) return TOS_E_ARG;
+  double vol;
+  orc_volume(c, &vol);
+  int n = M->n;
+  std::vector<double> df(n), dg(n);
+  for (int j = 0; j < n; ++j) { df[j] = sscale * (double)c->csens[j]; dg[j] = vscale * (double)c->cvsens[j]; }
+  double g = vscale * (vol - vfrac);
+  bool cv = mma_update(*M, df.data(), dg.data(), &g);
+  for (int j = 0; j < n; ++j) c->coef[j] = (float)M->x[j];
+  if (conv) *conv = cv ? 1 : 0;
+  return TOS_OK;
+}

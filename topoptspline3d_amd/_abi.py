@@ -1,0 +1,105 @@
+"""ctypes description of the C ABI declared in include/tos_api.h.
+
+`bind(lib, prefix)` attaches argument/return types to every entry point of a loaded
+shared library and returns a namespace of callables without the prefix.  The product
+binds the HIP library with prefix ``tos_``.
+"""
+import ctypes as C
+import types
+
+u8p = C.POINTER(C.c_uint8)
+f32p = C.POINTER(C.c_float)
+f64p = C.POINTER(C.c_double)
+i32p = C.POINTER(C.c_int)
+
+
+class GridDesc(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("h", C.c_double),
+                ("solid", u8p), ("support", u8p), ("shell", u8p),
+                ("n_levels", C.c_int), ("coarsen_threshold", C.c_int)]
+
+
+class Info(C.Structure):
+    _fields_ = [("n_levels", C.c_int), ("level_dims", (C.c_int * 3) * 16),
+                ("n_active_elements", C.c_int64 * 16), ("n_free_nodes", C.c_int64 * 16),
+                ("n_elements", C.c_int64), ("n_nodes", C.c_int64),
+                ("coarse_dofs", C.c_int), ("n_cases", C.c_int), ("n_coeffs", C.c_int64)]
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("rtol", C.c_double), ("maxit", C.c_int), ("precision", C.c_int),
+                ("pre", C.c_int), ("post", C.c_int), ("warm_start", C.c_int),
+                ("use_pcg", C.c_int)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("iterations", C.c_int * 64), ("rel_res", C.c_double * 64),
+                ("seconds", C.c_double)]
+
+
+vp = C.c_void_p
+vpp = C.POINTER(C.c_void_p)
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "ctx_create": (C.c_int, [C.POINTER(GridDesc), vpp]),
+    "ctx_destroy": (C.c_int, [vp]),
+    "last_error": (C.c_char_p, [vp]),
+    "get_info": (C.c_int, [vp, C.POINTER(Info)]),
+    "get_node_flags": (C.c_int, [vp, C.c_int, u8p]),
+    "set_material": (C.c_int, [vp, C.c_double, C.c_double, C.c_double, C.c_double]),
+    "get_template_matrix": (C.c_int, [vp, f64p]),
+    "set_loads": (C.c_int, [vp, C.c_int, f64p]),
+    "get_loads": (C.c_int, [vp, C.c_int, f64p]),
+    "set_density": (C.c_int, [vp, f32p]),
+    "get_density": (C.c_int, [vp, f32p]),
+    "update_operator": (C.c_int, [vp]),
+    "get_stencil": (C.c_int, [vp, C.c_int, f64p]),
+    "apply_operator": (C.c_int, [vp, f64p, f64p]),
+    "vcycle": (C.c_int, [vp, f64p, f64p, C.c_int, C.c_int, C.c_int]),
+    "solve": (C.c_int, [vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
+    "get_displacement": (C.c_int, [vp, C.c_int, f64p]),
+    "set_displacement": (C.c_int, [vp, C.c_int, f64p]),
+    "compliance": (C.c_int, [vp, f64p]),
+    "volume": (C.c_int, [vp, f64p]),
+    "sensitivity": (C.c_int, [vp, C.c_int]),
+    "get_sensitivity": (C.c_int, [vp, C.c_int, f32p]),
+    "set_sensitivity": (C.c_int, [vp, C.c_int, f32p]),
+    "init_vol_sensitivity": (C.c_int, [vp]),
+    "filter": (C.c_int, [vp, C.c_int, C.c_double]),
+    "element_compliance": (C.c_int, [vp, C.c_int, f64p]),
+    "oc_update": (C.c_int, [vp, C.c_double, C.c_double, C.c_double, f64p, i32p]),
+    "project_density": (C.c_int, [vp, C.c_double]),
+    "projection_sensitivity": (C.c_int, [vp, C.c_int, C.c_double]),
+    "spline_init": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
+    "spline_set_coeffs": (C.c_int, [vp, f32p]),
+    "spline_get_coeffs": (C.c_int, [vp, f32p]),
+    "spline_eval": (C.c_int, [vp]),
+    "spline_adjoint": (C.c_int, [vp, C.c_int]),
+    "spline_get_coeff_sens": (C.c_int, [vp, C.c_int, f32p]),
+    "mma_create": (C.c_int, [C.c_int, C.c_int, f64p, f64p, C.c_double, vpp]),
+    "mma_destroy": (C.c_int, [vp]),
+    "mma_set_x": (C.c_int, [vp, f64p]),
+    "mma_get_x": (C.c_int, [vp, f64p]),
+    "mma_update": (C.c_int, [vp, f64p, f64p, f64p, i32p]),
+    "ctx_mma_step": (C.c_int, [vp, vp, C.c_double, C.c_double, C.c_double, i32p]),
+}
+
+ERRORS = {0: "ok", -1: "bad argument", -2: "HIP error", -3: "not converged",
+          -4: "NaN/Inf", -5: "singular coarsest system", -6: "call order"}
+
+
+def bind(lib, prefix):
+    ns = types.SimpleNamespace()
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, prefix + name)
+        fn.restype = res
+        fn.argtypes = args
+        setattr(ns, name, fn)
+    ns._lib = lib
+    ns._prefix = prefix
+    return ns
+
+
+def exported_names(prefix):
+    return [prefix + n for n in SIGNATURES]
