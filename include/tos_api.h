@@ -193,6 +193,20 @@ int tos_mma_update(tos_mma* mma, const double* df, const double* dg, const doubl
 int tos_ctx_mma_step(tos_ctx* ctx, tos_mma* mma, double sens_scale, double vol_scale,
                      double vfrac, int* converged);
 
+/* ---- measurement ---------------------------------------------------------------------
+ * Kernel timing with HIP events recorded on the context's stream around every launch of
+ * the solver kernels.  enable = 1 clears and starts recording; enable = 0 stops, waits for
+ * the stream and fills `out` (may be NULL).  No counterpart in the reference (its
+ * tictoc timers, utils/tictoc.h:35-83, time host phases only). */
+#define TOS_PROF_CLASSES 16
+typedef struct tos_kernel_stats {
+  int n_classes;
+  char names[TOS_PROF_CLASSES][32];
+  double total_ms[TOS_PROF_CLASSES];
+  int64_t launches[TOS_PROF_CLASSES];
+} tos_kernel_stats;
+int tos_profile(tos_ctx* ctx, int enable, tos_kernel_stats* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -37,6 +37,11 @@ class SolveStats(C.Structure):
                 ("seconds", C.c_double)]
 
 
+class KernelStats(C.Structure):
+    _fields_ = [("n_classes", C.c_int), ("names", (C.c_char * 32) * 16),
+                ("total_ms", C.c_double * 16), ("launches", C.c_int64 * 16)]
+
+
 vp = C.c_void_p
 vpp = C.POINTER(C.c_void_p)
 
@@ -83,6 +88,7 @@ SIGNATURES = {
     "mma_get_x": (C.c_int, [vp, f64p]),
     "mma_update": (C.c_int, [vp, f64p, f64p, f64p, i32p]),
     "ctx_mma_step": (C.c_int, [vp, vp, C.c_double, C.c_double, C.c_double, i32p]),
+    "profile": (C.c_int, [vp, C.c_int, C.POINTER(KernelStats)]),
 }
 
 ERRORS = {0: "ok", -1: "bad argument", -2: "HIP error", -3: "not converged",

@@ -1,0 +1,438 @@
+// kernels_opt.hip -- element-wise and reduction kernels of the optimisation step:
+// sensitivities, filter, OC bisection, Heaviside projection, spline evaluation and its
+// adjoint, deterministic FP64 reductions and the PCG vector updates.
+#include "tos_internal.hpp"
+
+namespace tos {
+
+__device__ __forceinline__ int64_t tid64() { return (int64_t)blockIdx.x * blockDim.x + threadIdx.x; }
+
+// ---------------------------------------------------------------------------------------
+// Deterministic reductions: every block writes one partial (fixed block -> element
+// mapping), one block sums the partials in a fixed order.
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
+  return v;
+}
+
+// returns the block total in thread 0 (blockDim = 256)
+__device__ __forceinline__ double block_sum256(double v) {
+  __shared__ double sh[4];
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0;
+  if (threadIdx.x == 0) s = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+  __syncthreads();
+  return s;
+}
+__device__ __forceinline__ double block_max256(double v) {
+  __shared__ double sh[4];
+  v = wave_max(v);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0;
+  if (threadIdx.x == 0) s = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+  __syncthreads();
+  return s;
+}
+
+// partial[b] = sum over the grid-stride slice of block b of a[i]*b[i] (up to 3 pairs)
+__global__ __launch_bounds__(256) void k_dot_partial(const double* __restrict__ a0, const double* __restrict__ b0,
+                                                     const double* __restrict__ a1, const double* __restrict__ b1,
+                                                     int64_t n, double* __restrict__ partial) {
+  double s0 = 0, s1 = 0;
+  for (int64_t i = tid64(); i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    s0 += a0[i] * b0[i];
+    if (a1) s1 += a1[i] * b1[i];
+  }
+  s0 = block_sum256(s0);
+  s1 = block_sum256(s1);
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x] = s0;
+    partial[gridDim.x + blockIdx.x] = s1;
+  }
+}
+
+// Sum `np` partials (two interleaved sets when two != 0) into scal[slot0], scal[slot1],
+// then apply a small scalar program `op`.
+enum { OP_NONE = 0, OP_ALPHA, OP_BETA, OP_RN2 };
+__global__ __launch_bounds__(256) void k_finish(const double* __restrict__ partial, int np, int two, int slot0,
+                                                int slot1, int op, double* __restrict__ scal) {
+  double s0 = 0, s1 = 0;
+  for (int i = threadIdx.x; i < np; i += 256) {
+    s0 += partial[i];
+    if (two) s1 += partial[np + i];
+  }
+  s0 = block_sum256(s0);
+  s1 = block_sum256(s1);
+  if (threadIdx.x == 0) {
+    scal[slot0] = s0;
+    if (two) scal[slot1] = s1;
+    if (op == OP_ALPHA) scal[S_ALPHA] = scal[S_RZ] / scal[S_PQ];
+    if (op == OP_BETA) {  // flexible (Polak-Ribiere) beta = -alpha <z,q> / rz_old
+      scal[S_BETA] = -scal[S_ALPHA] * scal[S_ZQ] / scal[S_RZ];
+      scal[S_RZ] = scal[S_ZR];
+    }
+  }
+}
+
+// u += alpha p ; r -= alpha q ; partial ||r||^2
+__global__ __launch_bounds__(256) void k_pcg_ur(double* __restrict__ u, double* __restrict__ r,
+                                                const double* __restrict__ p, const double* __restrict__ q,
+                                                const double* __restrict__ scal, int64_t n, double* __restrict__ partial) {
+  const double al = scal[S_ALPHA];
+  double s = 0;
+  for (int64_t i = tid64(); i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    u[i] += al * p[i];
+    const double rv = r[i] - al * q[i];
+    r[i] = rv;
+    s += rv * rv;
+  }
+  s = block_sum256(s);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// p = z + beta p
+__global__ void k_pcg_p(double* __restrict__ p, const double* __restrict__ z, const double* __restrict__ scal,
+                        int64_t n) {
+  const double be = scal[S_BETA];
+  const int64_t i = tid64();
+  if (i < n) p[i] = z[i] + be * p[i];
+}
+
+// y = a*x + y (generic)
+__global__ void k_axpy(double a, const double* __restrict__ x, double* __restrict__ y, int64_t n) {
+  const int64_t i = tid64();
+  if (i < n) y[i] += a * x[i];
+}
+
+__global__ void k_sub(const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ c,
+                      int64_t n) {
+  const int64_t i = tid64();
+  if (i < n) c[i] = a[i] - b[i];
+}
+
+// ---------------------------------------------------------------------------------------
+// Element weights w = rho^p (FP64 and FP32 copies), 0 for inactive / ghost elements.
+// ---------------------------------------------------------------------------------------
+__global__ void k_weights(LevelView L, const uint8_t* __restrict__ eact, const float* __restrict__ rho,
+                          double penal, double* __restrict__ w64, float* __restrict__ w32) {
+  const int64_t i = tid64();
+  if (i >= L.NE) return;
+  const double v = eact[i] ? pow((double)rho[i], penal) : 0.0;
+  w64[i] = v;
+  w32[i] = (float)v;
+}
+
+// ---------------------------------------------------------------------------------------
+// Sensitivity s_e = -p rho^(p-1) u_e^T Ke u_e (computeSensitivity_kernel
+// optimization/optimization.cu:96-159), one thread per element, no atomics; MODE 1
+// writes the element compliance rho^p u_e^T Ke u_e (elementCompliance_kernel
+// grid/Grid.cu:5389-5430) as FP64 into out64.
+// ---------------------------------------------------------------------------------------
+template <int MODE>
+__global__ __launch_bounds__(256) void k_sensitivity(LevelView L, const uint8_t* __restrict__ eact,
+                                                     const float* __restrict__ rho, const double* __restrict__ Ke,
+                                                     const double* __restrict__ u, double penal,
+                                                     float* __restrict__ sens, double* __restrict__ out64) {
+  const int64_t t = tid64();
+  if (t >= (int64_t)L.ex * L.ey * L.ez) return;
+  const int x = t % L.ex, y = (t / L.ex) % L.ey, z = t / ((int64_t)L.ex * L.ey);
+  const int64_t e = L.pelem(x, y, z);
+  if (!eact[e]) {
+    if (MODE == 0) sens[e] = 0.f;
+    else out64[t] = 0.0;
+    return;
+  }
+  const int64_t n0 = L.pnode(x, y, z);
+  double ue[24];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int64_t q = n0 + (j & 1) + (int64_t)L.NX * (((j >> 1) & 1) + (int64_t)L.NY * (j >> 2));
+#pragma unroll
+    for (int c = 0; c < 3; ++c) ue[3 * j + c] = u[c * L.NP + q];
+  }
+  double en = 0;
+#pragma unroll
+  for (int i = 0; i < 24; ++i) {
+    double s = 0;
+#pragma unroll
+    for (int j = 0; j < 24; ++j) s += Ke[i * 24 + j] * ue[j];
+    en += ue[i] * s;
+  }
+  const double r = rho[e];
+  if (MODE == 0) sens[e] = (float)(-penal * pow(r, penal - 1.0) * en);
+  else out64[t] = pow(r, penal) * en;
+}
+
+// ---------------------------------------------------------------------------------------
+// Filter (filterSensitivity_kernel grid/Grid.cu:4131-4216): weighted mean over the active
+// elements of a ball; offsets and weights come from a host-built table.
+// ---------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_filter(LevelView L, const uint8_t* __restrict__ eact,
+                                                const float* __restrict__ src, float* __restrict__ dst,
+                                                const int* __restrict__ off, const double* __restrict__ wt, int nt) {
+  const int64_t t = tid64();
+  if (t >= (int64_t)L.ex * L.ey * L.ez) return;
+  const int x = t % L.ex, y = (t / L.ex) % L.ey, z = t / ((int64_t)L.ex * L.ey);
+  const int64_t e = L.pelem(x, y, z);
+  if (!eact[e]) { dst[e] = 0.f; return; }
+  double gs = 0, ws = 0;
+  for (int i = 0; i < nt; ++i) {
+    const int dx = off[3 * i], dy = off[3 * i + 1], dz = off[3 * i + 2];
+    const int qx = x + dx, qy = y + dy, qz = z + dz;
+    if (qx < 0 || qy < 0 || qz < 0 || qx >= L.ex || qy >= L.ey || qz >= L.ez) continue;
+    const int64_t q = e + dx + (int64_t)L.EX * (dy + (int64_t)L.EY * dz);
+    if (!eact[q]) continue;
+    gs += wt[i] * (double)src[q];
+    ws += wt[i];
+  }
+  dst[e] = (float)(gs / ws);
+}
+
+// ---------------------------------------------------------------------------------------
+// OC bisection on the device (updateDensities / trySensMultiplier_kernel
+// optimization/optimization.cu:355-460).  Each trial: partial sums of the trial density;
+// the finisher updates lo/hi, the next midpoint and the done flag, so the host only
+// waits once at the end.
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ float oc_rho(float r0, float s, double g, double move, double damp, double rmin,
+                                        bool shell) {
+  const double gg = s > 0 ? 0.0 : -(double)s;
+  double rn = (double)r0 * pow(gg / g, damp);
+  rn = fmin(fmax(rn, (double)r0 - move), (double)r0 + move);
+  rn = fmin(fmax(rn, rmin), 1.0);
+  if (shell) rn = 1.0;
+  return (float)rn;
+}
+
+__global__ __launch_bounds__(256) void k_absmax_partial(LevelView L, const uint8_t* __restrict__ eact,
+                                                        const float* __restrict__ s, double* __restrict__ partial) {
+  double m = 0;
+  for (int64_t i = tid64(); i < L.NE; i += (int64_t)gridDim.x * blockDim.x)
+    if (eact[i]) m = fmax(m, fabs((double)s[i]));
+  m = block_max256(m);
+  if (threadIdx.x == 0) partial[blockIdx.x] = m;
+}
+
+__global__ __launch_bounds__(256) void k_oc_init(const double* __restrict__ partial, int np, double* __restrict__ scal) {
+  double m = 0;
+  for (int i = threadIdx.x; i < np; i += 256) m = fmax(m, partial[i]);
+  m = block_max256(m);
+  if (threadIdx.x == 0) {
+    scal[S_OC_GMAX] = m;
+    scal[S_OC_LO] = 0.0;
+    scal[S_OC_HI] = m;
+    scal[S_OC_G] = 0.5 * m;
+    scal[S_OC_DONE] = 0.0;
+    scal[S_OC_STEPS] = 0.0;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_oc_trial(LevelView L, const uint8_t* __restrict__ eact,
+                                                  const uint8_t* __restrict__ shell, const float* __restrict__ rho,
+                                                  const float* __restrict__ s, double move, double damp, double rmin,
+                                                  const double* __restrict__ scal, double* __restrict__ partial) {
+  if (scal[S_OC_DONE] != 0.0) return;
+  const double g = scal[S_OC_G];
+  double v = 0;
+  for (int64_t i = tid64(); i < L.NE; i += (int64_t)gridDim.x * blockDim.x)
+    if (eact[i]) v += (double)oc_rho(rho[i], s[i], g, move, damp, rmin, shell[i] != 0);
+  v = block_sum256(v);
+  if (threadIdx.x == 0) partial[blockIdx.x] = v;
+}
+
+__global__ __launch_bounds__(256) void k_oc_decide(const double* __restrict__ partial, int np, double nact,
+                                                   double vgoal, double* __restrict__ scal) {
+  if (scal[S_OC_DONE] != 0.0) return;
+  double v = 0;
+  for (int i = threadIdx.x; i < np; i += 256) v += partial[i];
+  v = block_sum256(v);
+  if (threadIdx.x == 0) {
+    const double V = v / nact;
+    const double g = scal[S_OC_G];
+    scal[S_OC_V] = V;
+    scal[S_OC_STEPS] += 1.0;
+    if (V > vgoal) scal[S_OC_LO] = g;
+    else if (V < vgoal) scal[S_OC_HI] = g;
+    if (fabs(V - vgoal) <= 1e-4 || scal[S_OC_STEPS] >= 31.0) scal[S_OC_DONE] = 1.0;
+    else scal[S_OC_G] = 0.5 * (scal[S_OC_LO] + scal[S_OC_HI]);
+  }
+}
+
+__global__ void k_oc_apply(LevelView L, const uint8_t* __restrict__ eact, const uint8_t* __restrict__ shell,
+                           float* __restrict__ rho, const float* __restrict__ s, double move, double damp,
+                           double rmin, const double* __restrict__ scal) {
+  const int64_t i = tid64();
+  if (i >= L.NE || !eact[i]) return;
+  rho[i] = oc_rho(rho[i], s[i], scal[S_OC_G], move, damp, rmin, shell[i] != 0);
+}
+
+__global__ __launch_bounds__(256) void k_sum_active(LevelView L, const uint8_t* __restrict__ eact,
+                                                    const float* __restrict__ v, double* __restrict__ partial) {
+  double s = 0;
+  for (int64_t i = tid64(); i < L.NE; i += (int64_t)gridDim.x * blockDim.x)
+    if (eact[i]) s += (double)v[i];
+  s = block_sum256(s);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// ---------------------------------------------------------------------------------------
+// Heaviside projection (computeProjection_kernel / computeProjectionSensitivity_kernel
+// optimization/optimization.cu:161-183, 313-337)
+// ---------------------------------------------------------------------------------------
+__global__ void k_project(LevelView L, const uint8_t* __restrict__ eact, const uint8_t* __restrict__ shell,
+                          float* __restrict__ rho, float* __restrict__ rho_pre, double beta, double rmin) {
+  const int64_t i = tid64();
+  if (i >= L.NE || !eact[i]) return;
+  const double eta = 0.5, den = tanh(beta * eta) + tanh(beta * (1 - eta));
+  const float r0 = rho[i];
+  rho_pre[i] = r0;
+  double r = (tanh(beta * eta) + tanh(beta * ((double)r0 - eta))) / den;
+  r = fmin(fmax(r, rmin), 1.0);
+  if (shell[i]) r = 1.0;
+  rho[i] = (float)r;
+}
+
+__global__ void k_project_sens(LevelView L, const uint8_t* __restrict__ eact, const float* __restrict__ rho_pre,
+                               float* __restrict__ s, double beta) {
+  const int64_t i = tid64();
+  if (i >= L.NE || !eact[i]) return;
+  const double eta = 0.5, den = tanh(beta * eta) + tanh(beta * (1 - eta));
+  const double t = tanh(beta * ((double)rho_pre[i] - eta));
+  s[i] = (float)((double)s[i] * (beta * (1 - t * t) / den));
+}
+
+__global__ void k_mask_inactive(LevelView L, const uint8_t* __restrict__ eact, float* __restrict__ v) {
+  const int64_t i = tid64();
+  if (i < L.NE && !eact[i]) v[i] = 0.f;
+}
+
+__global__ void k_fill_active(LevelView L, const uint8_t* __restrict__ eact, float* __restrict__ v, float val) {
+  const int64_t i = tid64();
+  if (i < L.NE) v[i] = eact[i] ? val : 0.f;
+}
+
+// ---------------------------------------------------------------------------------------
+// Spline (coeff2density grid/Grid.cu:3325-3456; adjoint :3643-3859).  Per-axis tables
+// give, for each element index along an axis, `order` (coefficient index, basis value)
+// pairs; the 3-D value is their tensor product.
+// ---------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_spline_eval(LevelView L, const uint8_t* __restrict__ eact,
+                                                     const uint8_t* __restrict__ shell, const float* __restrict__ coef,
+                                                     int ncx, int ncy, int order,
+                                                     const int* __restrict__ ix, const float* __restrict__ wx,
+                                                     const uint8_t* __restrict__ okx,
+                                                     const int* __restrict__ iy, const float* __restrict__ wy,
+                                                     const uint8_t* __restrict__ oky,
+                                                     const int* __restrict__ iz, const float* __restrict__ wz,
+                                                     const uint8_t* __restrict__ okz,
+                                                     double rmin, float* __restrict__ rho) {
+  const int64_t t = tid64();
+  if (t >= (int64_t)L.ex * L.ey * L.ez) return;
+  const int x = t % L.ex, y = (t / L.ex) % L.ey, z = t / ((int64_t)L.ex * L.ey);
+  const int64_t e = L.pelem(x, y, z);
+  if (!eact[e]) { rho[e] = 0.f; return; }
+  float v;
+  if (!okx[x] || !oky[y] || !okz[z]) {
+    v = -0.2f;
+  } else {
+    v = 0.f;
+    for (int k = 0; k < order; ++k) {
+      const float wk = wz[z * order + k];
+      const int64_t bk = (int64_t)iz[z * order + k] * ncy;
+      for (int j = 0; j < order; ++j) {
+        const float wjk = wy[y * order + j] * wk;
+        const int64_t bj = (bk + iy[y * order + j]) * ncx;
+        float acc = 0.f;
+        for (int i = 0; i < order; ++i) acc += coef[bj + ix[x * order + i]] * wx[x * order + i];
+        v += acc * wjk;
+      }
+    }
+  }
+  double r = fmin(fmax((double)v, rmin), 1.0);
+  if (shell[e]) r = 1.0;
+  rho[e] = (float)r;
+}
+
+// g' = g on active, non-shell elements inside the knot range, else 0 (unpadded, FP64)
+__global__ void k_spline_mask(LevelView L, const uint8_t* __restrict__ eact, const uint8_t* __restrict__ shell,
+                              const float* __restrict__ g, const uint8_t* __restrict__ okx,
+                              const uint8_t* __restrict__ oky, const uint8_t* __restrict__ okz,
+                              double* __restrict__ out) {
+  const int64_t t = tid64();
+  if (t >= (int64_t)L.ex * L.ey * L.ez) return;
+  const int x = t % L.ex, y = (t / L.ex) % L.ey, z = t / ((int64_t)L.ex * L.ey);
+  const int64_t e = L.pelem(x, y, z);
+  const bool ok = eact[e] && !shell[e] && okx[x] && oky[y] && okz[z];
+  out[t] = ok ? (double)g[e] : 0.0;
+}
+
+// One separable adjoint stage along axis a of a 3-D array with dims (d0, d1, d2) (x
+// fastest): out[.. c ..] = sum over the CSR list of coefficient c of w * in[.. e ..],
+// where the contracted axis has length nin in `in` and nout in `out`.
+__global__ void k_spline_stage(const double* __restrict__ in, double* __restrict__ out, int axis,
+                               int d0, int d1, int d2,  // dims of `out`
+                               int nin, const int* __restrict__ cptr, const int* __restrict__ celm,
+                               const float* __restrict__ cw) {
+  const int64_t t = tid64();
+  if (t >= (int64_t)d0 * d1 * d2) return;
+  const int i0 = t % d0, i1 = (t / d0) % d1, i2 = t / ((int64_t)d0 * d1);
+  int c, s0, s1, s2;  // contracted index and strides of `in`
+  int64_t base, stride;
+  if (axis == 0) { c = i0; base = (int64_t)nin * (i1 + (int64_t)d1 * i2); stride = 1; }
+  else if (axis == 1) { c = i1; base = i0 + (int64_t)d0 * nin * i2; stride = d0; }
+  else { c = i2; base = i0 + (int64_t)d0 * i1; stride = (int64_t)d0 * d1; }
+  (void)s0; (void)s1; (void)s2;
+  double acc = 0;
+  for (int q = cptr[c]; q < cptr[c + 1]; ++q) acc += (double)cw[q] * in[base + stride * celm[q]];
+  out[t] = acc;
+}
+
+__global__ void k_d2f(const double* __restrict__ a, float* __restrict__ b, int64_t n) {
+  const int64_t i = tid64();
+  if (i < n) b[i] = (float)a[i];
+}
+
+// element array: unpadded host layout <-> padded device layout
+__global__ void k_pad_elems(LevelView L, const float* __restrict__ src, float* __restrict__ dst) {
+  const int64_t t = tid64();
+  if (t >= (int64_t)L.ex * L.ey * L.ez) return;
+  const int x = t % L.ex, y = (t / L.ex) % L.ey, z = t / ((int64_t)L.ex * L.ey);
+  dst[L.pelem(x, y, z)] = src[t];
+}
+__global__ void k_unpad_elems(LevelView L, const float* __restrict__ src, float* __restrict__ dst) {
+  const int64_t t = tid64();
+  if (t >= (int64_t)L.ex * L.ey * L.ez) return;
+  const int x = t % L.ex, y = (t / L.ex) % L.ey, z = t / ((int64_t)L.ex * L.ey);
+  dst[t] = src[L.pelem(x, y, z)];
+}
+// node vectors (3 SoA components): unpadded <-> padded, optional free mask on the way in
+__global__ void k_pad_vec(LevelView L, const double* __restrict__ src, double* __restrict__ dst,
+                          const uint8_t* __restrict__ fre) {
+  const int64_t t = tid64();
+  const int64_t n = (int64_t)L.nx * L.ny * L.nz;
+  if (t >= n) return;
+  const int x = t % L.nx, y = (t / L.nx) % L.ny, z = t / ((int64_t)L.nx * L.ny);
+  const int64_t p = L.pnode(x, y, z);
+  const bool ok = fre == nullptr || fre[p];
+  for (int c = 0; c < 3; ++c) dst[c * L.NP + p] = ok ? src[c * n + t] : 0.0;
+}
+__global__ void k_unpad_vec(LevelView L, const double* __restrict__ src, double* __restrict__ dst) {
+  const int64_t t = tid64();
+  const int64_t n = (int64_t)L.nx * L.ny * L.nz;
+  if (t >= n) return;
+  const int x = t % L.nx, y = (t / L.nx) % L.ny, z = t / ((int64_t)L.nx * L.ny);
+  const int64_t p = L.pnode(x, y, z);
+  for (int c = 0; c < 3; ++c) dst[c * n + t] = src[c * L.NP + p];
+}
+
+}  // namespace tos

@@ -1,0 +1,1525 @@
+// tos_api.hip -- host implementation of include/tos_api.h for gfx950.
+//
+// One context = one HIP stream; every call enqueues kernels on that stream and only
+// synchronises where a result must reach the host (convergence checks, returned
+// scalars, copies out).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+
+#include "tos_internal.hpp"
+
+#include "kernels_mg.hip"
+#include "kernels_opt.hip"
+#include "mma.hip"
+
+#define TOS_EXPORT extern "C" __attribute__((visibility("default")))
+
+using namespace tos;
+
+struct tos_ctx : tos::Ctx {};
+
+struct tos_mma {
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  MmaDev d{};
+  std::vector<void*> allocs;
+  double* h_out = nullptr;  // pinned [3]
+  double* stage = nullptr;  // device staging for df/dg/g
+};
+
+namespace {
+
+struct HipError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct ArgError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+#define HIPCHK(x)                                                                         \
+  do {                                                                                    \
+    hipError_t e_ = (x);                                                                  \
+    if (e_ != hipSuccess) throw HipError(std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define LAUNCH_CHECK() HIPCHK(hipGetLastError())
+
+void* raw_alloc(std::vector<void*>& list, hipStream_t s, size_t bytes) {
+  void* p = nullptr;
+  if (bytes == 0) bytes = 8;
+  HIPCHK(hipMalloc(&p, bytes));
+  HIPCHK(hipMemsetAsync(p, 0, bytes, s));
+  list.push_back(p);
+  return p;
+}
+
+template <class T>
+T* dalloc(Ctx& c, size_t n) {
+  return (T*)raw_alloc(c.allocs, c.stream, n * sizeof(T));
+}
+
+inline unsigned nblk(int64_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+// ---------------------------------------------------------------------------------------
+// Element stiffness of the 8-node trilinear hex (element/templateMatrix.cpp:31-99):
+// 2x2x2 Gauss points at (1 +- 1/sqrt3)/2, isotropic elasticity, weight (h/2)^3.
+// Written in Lame form: K_ab(r,s) = lam g_a,r g_b,s + mu g_a,s g_b,r + mu d_rs g_a.g_b.
+// ---------------------------------------------------------------------------------------
+void hex_stiffness(double h, double E, double nu, double* K) {
+  const double lam = E * nu / ((1 + nu) * (1 - 2 * nu)), mu = E / (2 * (1 + nu));
+  std::fill(K, K + 576, 0.0);
+  const double gq[2] = {0.5 - 0.5 / std::sqrt(3.0), 0.5 + 0.5 / std::sqrt(3.0)};
+  for (int g = 0; g < 8; ++g) {
+    const double pt[3] = {gq[g & 1], gq[(g >> 1) & 1], gq[g >> 2]};
+    double grad[8][3];
+    for (int a = 0; a < 8; ++a) {
+      const int b[3] = {a & 1, (a >> 1) & 1, a >> 2};
+      double N1[3], dN1[3];
+      for (int k = 0; k < 3; ++k) {
+        N1[k] = b[k] ? pt[k] : 1 - pt[k];
+        dN1[k] = (b[k] ? 1.0 : -1.0) / h;
+      }
+      grad[a][0] = dN1[0] * N1[1] * N1[2];
+      grad[a][1] = N1[0] * dN1[1] * N1[2];
+      grad[a][2] = N1[0] * N1[1] * dN1[2];
+    }
+    const double wq = (h * h * h) / 8.0;
+    for (int a = 0; a < 8; ++a)
+      for (int b = 0; b < 8; ++b) {
+        const double dd = grad[a][0] * grad[b][0] + grad[a][1] * grad[b][1] + grad[a][2] * grad[b][2];
+        for (int r = 0; r < 3; ++r)
+          for (int s = 0; s < 3; ++s)
+            K[(3 * a + r) * 24 + 3 * b + s] +=
+                wq * (lam * grad[a][r] * grad[b][s] + mu * grad[a][s] * grad[b][r] + (r == s ? mu * dd : 0.0));
+      }
+  }
+}
+
+double pweight_h(int c, int i, int m) {
+  double w = 1;
+  for (int a = 0; a < 3; ++a) {
+    const int t = ((c >> a) & 1) + ((i >> a) & 1);
+    w *= ((m >> a) & 1) ? t * 0.5 : (2 - t) * 0.5;
+  }
+  return w;
+}
+
+// Mc[c] = P_c^T Ke P_c for the 8 child positions c of a fine element in its coarse element
+void child_products(const double* Ke, double* Mc) {
+  for (int c = 0; c < 8; ++c) {
+    double P[8][8];
+    for (int i = 0; i < 8; ++i)
+      for (int m = 0; m < 8; ++m) P[i][m] = pweight_h(c, i, m);
+    static thread_local double T[24][24];
+    for (int i = 0; i < 24; ++i)
+      for (int n = 0; n < 8; ++n)
+        for (int s = 0; s < 3; ++s) {
+          double acc = 0;
+          for (int j = 0; j < 8; ++j) acc += Ke[i * 24 + 3 * j + s] * P[j][n];
+          T[i][3 * n + s] = acc;
+        }
+    for (int m = 0; m < 8; ++m)
+      for (int r = 0; r < 3; ++r)
+        for (int col = 0; col < 24; ++col) {
+          double acc = 0;
+          for (int i = 0; i < 8; ++i) acc += P[i][m] * T[3 * i + r][col];
+          Mc[c * 576 + (3 * m + r) * 24 + col] = acc;
+        }
+  }
+}
+
+LevelView make_view(int ex, int ey, int ez) {
+  LevelView v{};
+  v.ex = ex; v.ey = ey; v.ez = ez;
+  v.nx = ex + 1; v.ny = ey + 1; v.nz = ez + 1;
+  v.NX = v.nx + 3; v.NY = v.ny + 3;
+  v.NP = ((int64_t)v.NX * v.NY * (v.nz + 3) + 63) / 64 * 64;
+  v.EX = ex + 2; v.EY = ey + 2;
+  v.NE = ((int64_t)v.EX * v.EY * (ez + 2) + 63) / 64 * 64;
+  return v;
+}
+
+// Hierarchy on the host: coarse element active iff a child is active
+// (setSolidElementFromFineGrid_kernel grid/Grid.cu:4453-4473), coarsen while more than
+// `threshold` active elements (grid/Grid.cpp:747).  Free nodes: fine = touches an active
+// element and is not supported; coarse = some free fine node lies inside the support of
+// its trilinear basis function.
+void build_levels(Ctx& c, int n_levels, int threshold) {
+  struct HL {
+    int ex, ey, ez;
+    std::vector<uint8_t> act;
+  };
+  std::vector<HL> hl;
+  hl.push_back({c.nx, c.ny, c.nz, c.h_solid});
+  for (auto& a : hl[0].act) a = a ? 1 : 0;
+  auto nact = [](const HL& l) {
+    int64_t s = 0;
+    for (auto a : l.act) s += a;
+    return s;
+  };
+  for (;;) {
+    const HL& f = hl.back();
+    const bool more = n_levels > 0 ? (int)hl.size() < n_levels : nact(f) > threshold;
+    if (!more || (f.ex == 1 && f.ey == 1 && f.ez == 1) || hl.size() >= 16) break;
+    HL C{(f.ex + 1) / 2, (f.ey + 1) / 2, (f.ez + 1) / 2, {}};
+    C.act.assign((size_t)C.ex * C.ey * C.ez, 0);
+    for (int z = 0; z < f.ez; ++z)
+      for (int y = 0; y < f.ey; ++y)
+        for (int x = 0; x < f.ex; ++x)
+          if (f.act[x + (size_t)f.ex * (y + (size_t)f.ey * z)])
+            C.act[(x >> 1) + (size_t)C.ex * ((y >> 1) + (size_t)C.ey * (z >> 1))] = 1;
+    hl.push_back(std::move(C));
+  }
+  c.L.resize(hl.size());
+  for (size_t l = 0; l < hl.size(); ++l) {
+    Level& L = c.L[l];
+    L.v = make_view(hl[l].ex, hl[l].ey, hl[l].ez);
+    L.h_eact = std::move(hl[l].act);
+    int64_t na = 0;
+    for (auto a : L.h_eact) na += a;
+    L.n_active = na;
+    const LevelView& v = L.v;
+    L.h_free.assign((size_t)v.nx * v.ny * v.nz, 0);
+    if (l == 0) {
+      for (int z = 0; z < v.ez; ++z)
+        for (int y = 0; y < v.ey; ++y)
+          for (int x = 0; x < v.ex; ++x) {
+            if (!L.h_eact[x + (size_t)v.ex * (y + (size_t)v.ey * z)]) continue;
+            for (int j = 0; j < 8; ++j)
+              L.h_free[(x + (j & 1)) + (size_t)v.nx * ((y + ((j >> 1) & 1)) + (size_t)v.ny * (z + (j >> 2)))] = 1;
+          }
+      if (!c.h_support.empty())
+        for (size_t i = 0; i < L.h_free.size(); ++i)
+          if (c.h_support[i]) L.h_free[i] = 0;
+    } else {
+      const Level& F = c.L[l - 1];
+      const LevelView& fv = F.v;
+      for (int z = 0; z < fv.nz; ++z)
+        for (int y = 0; y < fv.ny; ++y)
+          for (int x = 0; x < fv.nx; ++x) {
+            if (!F.h_free[x + (size_t)fv.nx * (y + (size_t)fv.ny * z)]) continue;
+            for (int Z = z >> 1; Z <= (z + 1) >> 1; ++Z)
+              for (int Y = y >> 1; Y <= (y + 1) >> 1; ++Y)
+                for (int X = x >> 1; X <= (x + 1) >> 1; ++X)
+                  if (X < v.nx && Y < v.ny && Z < v.nz) L.h_free[X + (size_t)v.nx * (Y + (size_t)v.ny * Z)] = 1;
+          }
+    }
+    int64_t nf = 0;
+    for (auto a : L.h_free) nf += a;
+    L.n_free = nf;
+  }
+}
+
+std::vector<uint8_t> pad_nodes(const LevelView& v, const std::vector<uint8_t>& src) {
+  std::vector<uint8_t> out(v.NP, 0);
+  for (int z = 0; z < v.nz; ++z)
+    for (int y = 0; y < v.ny; ++y)
+      for (int x = 0; x < v.nx; ++x) out[v.pnode(x, y, z)] = src[x + (size_t)v.nx * (y + (size_t)v.ny * z)];
+  return out;
+}
+
+std::vector<uint8_t> pad_elems(const LevelView& v, const uint8_t* src) {
+  std::vector<uint8_t> out(v.NE, 0);
+  if (!src) return out;
+  for (int z = 0; z < v.ez; ++z)
+    for (int y = 0; y < v.ey; ++y)
+      for (int x = 0; x < v.ex; ++x) out[v.pelem(x, y, z)] = src[x + (size_t)v.ex * (y + (size_t)v.ey * z)] ? 1 : 0;
+  return out;
+}
+
+void upload(hipStream_t s, void* dst, const void* src, size_t bytes) {
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+  HIPCHK(hipStreamSynchronize(s));  // src is pageable host memory owned by the caller
+}
+void download(hipStream_t s, void* dst, const void* src, size_t bytes) {
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+}
+
+void* stage(Ctx& c, size_t bytes) {
+  if (c.stage_bytes < bytes) {
+    if (c.stage) HIPCHK(hipFree(c.stage));
+    c.stage = nullptr;
+    HIPCHK(hipMalloc(&c.stage, bytes));
+    c.stage_bytes = bytes;
+  }
+  return c.stage;
+}
+
+// ---------------------------------------------------------------------------------------
+// Reductions (results stay on the device in c.scal)
+// ---------------------------------------------------------------------------------------
+void dot_to(Ctx& c, const double* a, const double* b, const double* a2, const double* b2, int64_t n, int s0, int s1,
+            int op) {
+  hipLaunchKernelGGL(k_dot_partial, dim3(kRedBlocks), dim3(256), 0, c.stream, a, b, a2, b2, n, c.partial);
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, c.stream, (const double*)c.partial, kRedBlocks, a2 ? 1 : 0, s0,
+                     s1, op, c.scal);
+  LAUNCH_CHECK();
+}
+
+const double* read_scal(Ctx& c) {
+  download(c.stream, c.h_scal, c.scal, sizeof(double) * S_COUNT);
+  return c.h_scal;
+}
+
+// ---------------------------------------------------------------------------------------
+// Kernel timing: events recorded around launches while c.prof_on
+// ---------------------------------------------------------------------------------------
+enum {
+  P_FINE_APPLY64 = 0, P_FINE_RESID64, P_FINE_RESID32, P_FINE_GS64, P_FINE_GS32, P_ST_GS64, P_ST_GS32,
+  P_ST_RESID, P_TRANSFER, P_COARSE, P_GALERKIN, P_COARSE_FACTOR, P_VECTOR, P_SENS_FILTER, P_SPLINE, P_OPT,
+  P_COUNT
+};
+const char* kProfNames[P_COUNT] = {"fine_apply_f64", "fine_resid_f64", "fine_resid_f32", "fine_gs_f64",
+                                   "fine_gs_f32",    "st_gs_f64",      "st_gs_f32",      "st_resid",
+                                   "transfer",       "coarse_solve",   "galerkin",       "coarse_factor",
+                                   "vector",         "sens_filter",    "spline",         "oc_mma"};
+
+hipEvent_t next_event(Ctx& c) {
+  if (c.ev_used == c.ev_pool.size()) {
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    c.ev_pool.push_back(e);
+  }
+  return c.ev_pool[c.ev_used++];
+}
+
+struct Prof {
+  Ctx& c;
+  int cls;
+  hipEvent_t a{};
+  Prof(Ctx& c_, int cls_) : c(c_), cls(cls_) {
+    if (c.prof_on) {
+      a = next_event(c);
+      HIPCHK(hipEventRecord(a, c.stream));
+    }
+  }
+  ~Prof() {
+    if (c.prof_on) {
+      hipEvent_t b = next_event(c);
+      (void)hipEventRecord(b, c.stream);
+      c.prof.push_back({cls, a, b});
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------------------
+// Multigrid (V-cycle, HierarchyGrid::v_cycle grid/Grid.cpp:2399-2444)
+// ---------------------------------------------------------------------------------------
+template <class T>
+struct Prec;
+template <>
+struct Prec<double> {
+  static double* u(Level& L) { return L.u64; }
+  static double* f(Level& L) { return L.f64; }
+  static double* r(Level& L) { return L.r64; }
+  static const double* st(Level& L) { return L.st64; }
+  static const double* w(Ctx& c) { return c.w64; }
+  static const double* Ke(Ctx& c) { return c.Ke64; }
+};
+template <>
+struct Prec<float> {
+  static float* u(Level& L) { return L.u32; }
+  static float* f(Level& L) { return L.f32; }
+  static float* r(Level& L) { return L.r32; }
+  static const float* st(Level& L) { return L.st32; }
+  static const float* w(Ctx& c) { return c.w32; }
+  static const float* Ke(Ctx& c) { return c.Ke32; }
+};
+
+template <class T>
+void gs_sweep(Ctx& c, int l, T* u, const T* f, bool backward) {
+  Level& L = c.L[l];
+  const LevelView& v = L.v;
+  for (int ci = 0; ci < 8; ++ci) {
+    const int col = backward ? 7 - ci : ci;
+    const int px = col & 1, py = (col >> 1) & 1, pz = col >> 2;
+    const int64_t cnt = (int64_t)((v.nx - px + 1) >> 1) * ((v.ny - py + 1) >> 1) * ((v.nz - pz + 1) >> 1);
+    if (cnt == 0) continue;
+    Prof pr(c, l == 0 ? (sizeof(T) == 8 ? P_FINE_GS64 : P_FINE_GS32) : (sizeof(T) == 8 ? P_ST_GS64 : P_ST_GS32));
+    if (l == 0) {
+      if (backward)
+        hipLaunchKernelGGL((k_fine_gs<T, true>), dim3(nblk(cnt)), dim3(256), 0, c.stream, v, (const uint8_t*)L.fre,
+                           Prec<T>::w(c), Prec<T>::Ke(c), u, f, col);
+      else
+        hipLaunchKernelGGL((k_fine_gs<T, false>), dim3(nblk(cnt)), dim3(256), 0, c.stream, v, (const uint8_t*)L.fre,
+                           Prec<T>::w(c), Prec<T>::Ke(c), u, f, col);
+    } else {
+      if (backward)
+        hipLaunchKernelGGL((k_st_gs<T, true>), dim3(nblk(cnt)), dim3(256), 0, c.stream, v, (const uint8_t*)L.fre,
+                           Prec<T>::st(L), u, f, col);
+      else
+        hipLaunchKernelGGL((k_st_gs<T, false>), dim3(nblk(cnt)), dim3(256), 0, c.stream, v, (const uint8_t*)L.fre,
+                           Prec<T>::st(L), u, f, col);
+    }
+  }
+  LAUNCH_CHECK();
+}
+
+template <class T, int MODE>
+void apply_level(Ctx& c, int l, const T* u, const T* f, T* y) {
+  Level& L = c.L[l];
+  const int64_t n = (int64_t)L.v.nx * L.v.ny * L.v.nz;
+  Prof pr(c, l > 0 ? P_ST_RESID
+                   : (MODE == 0 ? P_FINE_APPLY64 : (sizeof(T) == 8 ? P_FINE_RESID64 : P_FINE_RESID32)));
+  if (l == 0)
+    hipLaunchKernelGGL((k_fine_apply<T, MODE>), dim3(nblk(n)), dim3(256), 0, c.stream, L.v, (const uint8_t*)L.fre,
+                       Prec<T>::w(c), Prec<T>::Ke(c), u, f, y);
+  else
+    hipLaunchKernelGGL((k_st_apply<T, MODE>), dim3(nblk(n)), dim3(256), 0, c.stream, L.v, (const uint8_t*)L.fre,
+                       Prec<T>::st(L), u, f, y);
+  LAUNCH_CHECK();
+}
+
+template <class T>
+void vcycle_level(Ctx& c, int l, const T* f, T* u, int pre, int post) {
+  Level& L = c.L[l];
+  const int last = (int)c.L.size() - 1;
+  const int64_t n3 = 3 * L.v.NP;
+  if (l == last && last > 0) {
+    if (c.cn == 0) return;  // no unknowns on the coarsest level (u stays 0)
+    Prof pr(c, P_COARSE);
+    hipLaunchKernelGGL((k_coarse_solve<T>), dim3(nblk((int64_t)c.cn * 64)), dim3(256), 0, c.stream, L.v,
+                       (const int64_t*)c.cidx, c.cn, (const double*)c.cinv, f, u);
+    LAUNCH_CHECK();
+    return;
+  }
+  hipLaunchKernelGGL((k_zero<T>), dim3(nblk(n3)), dim3(256), 0, c.stream, u, n3);
+  if (l == last) {  // a single level: smoothing only
+    for (int s = 0; s < std::max(1, pre); ++s) gs_sweep<T>(c, l, u, f, false);
+    for (int s = 0; s < std::max(1, post); ++s) gs_sweep<T>(c, l, u, f, true);
+    return;
+  }
+  for (int s = 0; s < pre; ++s) gs_sweep<T>(c, l, u, f, false);
+  T* r = Prec<T>::r(L);
+  apply_level<T, 1>(c, l, u, f, r);
+  Level& C = c.L[l + 1];
+  const int64_t nc = (int64_t)C.v.nx * C.v.ny * C.v.nz;
+  {
+    Prof pr(c, P_TRANSFER);
+    hipLaunchKernelGGL((k_restrict<T>), dim3(nblk(nc)), dim3(256), 0, c.stream, L.v, C.v, (const uint8_t*)C.fre,
+                       (const T*)r, Prec<T>::f(C));
+  }
+  vcycle_level<T>(c, l + 1, Prec<T>::f(C), Prec<T>::u(C), pre, post);
+  const int64_t nf = (int64_t)L.v.nx * L.v.ny * L.v.nz;
+  {
+    Prof pr(c, P_TRANSFER);
+    hipLaunchKernelGGL((k_prolong_add<T>), dim3(nblk(nf)), dim3(256), 0, c.stream, L.v, C.v, (const uint8_t*)L.fre,
+                       (const T*)Prec<T>::u(C), u);
+  }
+  LAUNCH_CHECK();
+  for (int s = 0; s < post; ++s) gs_sweep<T>(c, l, u, f, true);
+}
+
+// z = B r with FP64 input/output; `precision` selects the arithmetic of the cycle
+void precondition(Ctx& c, const double* r, double* z, int precision, int pre, int post) {
+  Level& L0 = c.L[0];
+  const int64_t n3 = 3 * L0.v.NP;
+  if (precision == 0) {
+    vcycle_level<double>(c, 0, r, z, pre, post);
+  } else {
+    hipLaunchKernelGGL((k_copy_cast<double, float>), dim3(nblk(n3)), dim3(256), 0, c.stream, r, L0.f32, n3);
+    vcycle_level<float>(c, 0, L0.f32, L0.u32, pre, post);
+    hipLaunchKernelGGL((k_copy_cast<float, double>), dim3(nblk(n3)), dim3(256), 0, c.stream, (const float*)L0.u32, z,
+                       n3);
+    LAUNCH_CHECK();
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Coarsest system (buildCoarsestSystem grid/Grid.cpp:3614-3669 assembles it on the host
+// and solves with Eigen BDCSVD each V-cycle): dense assembly on the device, blocked
+// symmetric Gauss-Jordan sweep leaving -A^{-1}, applied as a matvec in the V-cycle.
+// ---------------------------------------------------------------------------------------
+void coarse_alloc(Ctx& c) {
+  Level& L = c.L.back();
+  std::vector<int64_t> nodes;
+  std::vector<int> cmap(L.v.NP, -1);
+  for (int z = 0; z < L.v.nz; ++z)
+    for (int y = 0; y < L.v.ny; ++y)
+      for (int x = 0; x < L.v.nx; ++x)
+        if (L.h_free[x + (size_t)L.v.nx * (y + (size_t)L.v.ny * z)]) {
+          cmap[L.v.pnode(x, y, z)] = (int)nodes.size();
+          nodes.push_back(L.v.pnode(x, y, z));
+        }
+  c.cn = 3 * (int)nodes.size();
+  c.cidx = dalloc<int64_t>(c, nodes.size());
+  upload(c.stream, c.cidx, nodes.data(), nodes.size() * sizeof(int64_t));
+  c.cmap = dalloc<int>(c, cmap.size());
+  upload(c.stream, c.cmap, cmap.data(), cmap.size() * sizeof(int));
+  c.cinv = dalloc<double>(c, (size_t)c.cn * c.cn);
+  c.cwork = dalloc<double>(c, (size_t)c.cn * kGJB * 2 + kGJB * kGJB + c.cn);
+  c.cflag = dalloc<int>(c, 1);
+}
+
+int coarse_factor(Ctx& c) {
+  if (c.cn == 0) return TOS_OK;
+  Prof pr(c, P_COARSE_FACTOR);
+  Level& L = c.L.back();
+  const int n = c.cn, nnode = n / 3;
+  hipLaunchKernelGGL(k_zero<double>, dim3(nblk((int64_t)n * n)), dim3(256), 0, c.stream, c.cinv, (int64_t)n * n);
+  hipLaunchKernelGGL(k_coarse_assemble, dim3(nblk((int64_t)nnode * 27)), dim3(256), 0, c.stream, L.v,
+                     (const int64_t*)c.cidx, nnode, (const int*)c.cmap, (const double*)L.st64, c.cinv, n);
+  double* Dinv = c.cwork;
+  double* Cp = Dinv + kGJB * kGJB;
+  double* R = Cp + (size_t)n * kGJB;
+  double* diag0 = R + (size_t)n * kGJB;
+  hipLaunchKernelGGL(k_symmetrize, dim3(nblk((int64_t)n * n)), dim3(256), 0, c.stream, c.cinv, n, diag0);
+  HIPCHK(hipMemsetAsync(c.cflag, 0, sizeof(int), c.stream));
+  for (int K0 = 0; K0 < n; K0 += kGJB) {
+    const int b = std::min(kGJB, n - K0);
+    hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(256), 0, c.stream, c.cinv, n, K0, b, Dinv, R,
+                       (const double*)diag0, c.cflag);
+    hipLaunchKernelGGL(k_gj_panel, dim3(nblk((int64_t)n * b)), dim3(256), 0, c.stream, (const double*)c.cinv, n, K0,
+                       b, (const double*)Dinv, Cp);
+    hipLaunchKernelGGL(k_gj_update, dim3(nblk((int64_t)n * n)), dim3(256), 0, c.stream, c.cinv, n, K0, b,
+                       (const double*)Dinv, (const double*)Cp, (const double*)R);
+  }
+  LAUNCH_CHECK();
+  int flag = 0;
+  download(c.stream, &flag, c.cflag, sizeof(int));
+  if (flag) {
+    c.err = "coarsest system: non-positive pivot (no supports or a floating part?)";
+    return TOS_E_SINGULAR;
+  }
+  return TOS_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// Solve (flexible PCG or plain V-cycle iteration) for load case k
+// ---------------------------------------------------------------------------------------
+int solve_case(Ctx& c, int k, const tos_solve_opts& o, int* its, double* relres) {
+  Level& L0 = c.L[0];
+  const int64_t n3 = 3 * L0.v.NP;
+  double* u = c.U + (size_t)k * n3;
+  const double* f = c.F + (size_t)k * n3;
+  if (!o.warm_start) hipLaunchKernelGGL(k_zero<double>, dim3(nblk(n3)), dim3(256), 0, c.stream, u, n3);
+  apply_level<double, 1>(c, 0, u, f, c.r);  // r = f - K u
+  dot_to(c, f, f, c.r, c.r, n3, S_NF2, S_RN2, OP_NONE);
+  const double* hs = read_scal(c);
+  const double nf = std::sqrt(hs[S_NF2]);
+  double rn = std::sqrt(hs[S_RN2]);
+  *its = 0;
+  if (nf == 0) {
+    hipLaunchKernelGGL(k_zero<double>, dim3(nblk(n3)), dim3(256), 0, c.stream, u, n3);
+    *relres = 0;
+    return TOS_OK;
+  }
+  if (!std::isfinite(rn)) {
+    c.err = "NaN/Inf in the initial residual";
+    return TOS_E_NAN;
+  }
+  int it = 0;
+  if (!o.use_pcg) {
+    while (rn / nf > o.rtol && it < o.maxit) {
+      precondition(c, c.r, c.z, o.precision, o.pre, o.post);
+      hipLaunchKernelGGL(k_axpy, dim3(nblk(n3)), dim3(256), 0, c.stream, 1.0, (const double*)c.z, u, n3);
+      apply_level<double, 1>(c, 0, u, f, c.r);
+      dot_to(c, c.r, c.r, nullptr, nullptr, n3, S_RN2, 0, OP_NONE);
+      rn = std::sqrt(read_scal(c)[S_RN2]);
+      ++it;
+      if (!std::isfinite(rn)) { c.err = "NaN/Inf in the V-cycle iteration"; return TOS_E_NAN; }
+    }
+  } else if (rn / nf > o.rtol) {
+    precondition(c, c.r, c.z, o.precision, o.pre, o.post);
+    HIPCHK(hipMemcpyAsync(c.p, c.z, n3 * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    dot_to(c, c.r, c.z, nullptr, nullptr, n3, S_RZ, 0, OP_NONE);
+    while (it < o.maxit) {
+      apply_level<double, 0>(c, 0, c.p, nullptr, c.q);
+      dot_to(c, c.p, c.q, nullptr, nullptr, n3, S_PQ, 0, OP_ALPHA);
+      hipLaunchKernelGGL(k_pcg_ur, dim3(kRedBlocks), dim3(256), 0, c.stream, u, c.r, (const double*)c.p,
+                         (const double*)c.q, (const double*)c.scal, n3, c.partial);
+      hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, c.stream, (const double*)c.partial, kRedBlocks, 0, S_RN2, 0,
+                         OP_NONE, c.scal);
+      LAUNCH_CHECK();
+      ++it;
+      rn = std::sqrt(read_scal(c)[S_RN2]);
+      if (!std::isfinite(rn)) { c.err = "NaN/Inf in PCG"; return TOS_E_NAN; }
+      if (rn / nf <= o.rtol) break;
+      precondition(c, c.r, c.z, o.precision, o.pre, o.post);
+      dot_to(c, c.z, c.r, c.z, c.q, n3, S_ZR, S_ZQ, OP_BETA);
+      hipLaunchKernelGGL(k_pcg_p, dim3(nblk(n3)), dim3(256), 0, c.stream, c.p, (const double*)c.z,
+                         (const double*)c.scal, n3);
+      LAUNCH_CHECK();
+    }
+  }
+  // true FP64 residual
+  apply_level<double, 1>(c, 0, u, f, c.r);
+  dot_to(c, c.r, c.r, nullptr, nullptr, n3, S_RN2, 0, OP_NONE);
+  *relres = std::sqrt(read_scal(c)[S_RN2]) / nf;
+  *its = it;
+  return *relres <= o.rtol ? TOS_OK : TOS_E_NOCONV;
+}
+
+float* sens_buf(Ctx& c, int which) { return which == 0 ? c.sens : which == 1 ? c.vsens : nullptr; }
+
+double volume_of(Ctx& c) {
+  const LevelView& v = c.L[0].v;
+  hipLaunchKernelGGL(k_sum_active, dim3(kRedBlocks), dim3(256), 0, c.stream, v, (const uint8_t*)c.eact,
+                     (const float*)c.rho, c.partial);
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, c.stream, (const double*)c.partial, kRedBlocks, 0, S_VOL, 0,
+                     OP_NONE, c.scal);
+  LAUNCH_CHECK();
+  return read_scal(c)[S_VOL] / (double)c.n_active;
+}
+
+// ---------------------------------------------------------------------------------------
+// Spline tables (built on the host, once per tos_spline_init)
+// ---------------------------------------------------------------------------------------
+// Cox-de Boor recursion (The NURBS Book A2.2; SplineBasisX grid/Grid.cu:384-410): the
+// `p+1` nonzero basis values at x on knot span `span`.
+void cox_de_boor(int span, double x, int p, const std::vector<double>& U, double* N) {
+  std::vector<double> left(p + 1), right(p + 1);
+  N[0] = 1.0;
+  for (int j = 1; j <= p; ++j) {
+    left[j] = x - U[span + 1 - j];
+    right[j] = U[span + j] - x;
+    double saved = 0.0;
+    for (int r = 0; r < j; ++r) {
+      const double tmp = N[r] / (right[r + 1] + left[j - r]);
+      N[r] = saved + right[r + 1] * tmp;
+      saved = left[j - r] * tmp;
+    }
+    N[j] = saved;
+  }
+}
+
+// uniform B-spline of order o, support [0, o), by the recursion on degree
+double uniform_bspline(int o, double t) {
+  if (t < 0 || t >= o) return 0.0;
+  std::vector<double> b(o, 0.0);
+  const int i0 = (int)std::floor(t);
+  b[i0] = 1.0;  // order 1: indicator of [i0, i0+1)
+  for (int k = 2; k <= o; ++k)
+    for (int i = 0; i + k <= o; ++i)
+      b[i] = ((t - i) * b[i] + (i + k - t) * b[i + 1]) / (k - 1);
+  return b[0];
+}
+
+struct AxisTable {
+  std::vector<int> idx;
+  std::vector<float> wt;
+  std::vector<uint8_t> ok;
+  int nc = 0;
+};
+
+AxisTable spline_table(int mode, int order, int param, int ne, double h) {
+  AxisTable A;
+  A.idx.assign((size_t)ne * order, 0);
+  A.wt.assign((size_t)ne * order, 0.f);
+  A.ok.assign(ne, 1);
+  if (mode == 0) {
+    // clamped knots on the box enlarged by 2*box/(P+1) per side (Grid::set_spline_knot_series
+    // grid/Grid.cpp:3253-3377), span index as in coeff2density (grid/Grid.cu:3393-3402)
+    const int P = param;
+    const double box = ne * h;
+    const double lo = -2.0 * box / (P + 1), hi = box + 2.0 * box / (P + 1);
+    const double step = (hi - lo) / (P + 1);
+    std::vector<double> U(order, lo);
+    double t = lo;
+    for (int i = 0; i < P; ++i) { t += step; U.push_back(t); }
+    for (int i = 0; i < order; ++i) U.push_back(hi);
+    A.nc = order + P;
+    std::vector<double> N(order);
+    for (int e = 0; e < ne; ++e) {
+      const double x = (e + 0.5) * h;
+      const int i = (int)((x - lo) / step) + order;
+      if (i < order || i > A.nc) { A.ok[e] = 0; continue; }
+      cox_de_boor(i - 1, x, order - 1, U, N.data());
+      for (int q = 0; q < order; ++q) {
+        A.idx[(size_t)e * order + q] = i - order + q;
+        A.wt[(size_t)e * order + q] = (float)N[q];
+      }
+    }
+  } else {
+    // uniform lattice: coefficient i at node param*i, replicated at the boundary
+    const int s = param;
+    A.nc = (ne + s - 1) / s + 1;
+    for (int e = 0; e < ne; ++e) {
+      const double t = (e + 0.5) / s;
+      const int i0 = (int)std::floor(t - order / 2.0) + 1;
+      for (int q = 0; q < order; ++q) {
+        const int i = i0 + q;
+        A.idx[(size_t)e * order + q] = std::min(std::max(i, 0), A.nc - 1);
+        A.wt[(size_t)e * order + q] = (float)uniform_bspline(order, t - i + order / 2.0);
+      }
+    }
+  }
+  return A;
+}
+
+// ---------------------------------------------------------------------------------------
+// MMA handle
+// ---------------------------------------------------------------------------------------
+template <class T>
+T* malloc_mma(tos_mma* M, size_t n) {
+  return (T*)raw_alloc(M->allocs, M->stream, n * sizeof(T));
+}
+
+void mma_run(tos_mma* M, const double* df, const double* dg, const double* g, int* converged) {
+  M->d.df = df;
+  M->d.dg = dg;
+  M->d.g = g;
+  hipLaunchKernelGGL(k_mma_update, dim3(1), dim3(MT), 0, M->stream, M->d);
+  LAUNCH_CHECK();
+  download(M->stream, M->h_out, M->d.out, 3 * sizeof(double));
+  if (converged) *converged = M->h_out[0] != 0.0;
+}
+
+template <class F>
+int guard(Ctx* c, F&& f) {
+  if (!c) return TOS_E_ARG;
+  try {
+    HIPCHK(hipSetDevice(c->dev));
+    return f();
+  } catch (const HipError& e) {
+    c->err = e.what();
+    return TOS_E_HIP;
+  } catch (const ArgError& e) {
+    c->err = e.what();
+    return TOS_E_ARG;
+  } catch (const std::exception& e) {
+    c->err = e.what();
+    return TOS_E_ARG;
+  }
+}
+
+#define REQUIRE(cond, msg) \
+  if (!(cond)) throw ArgError(msg)
+
+}  // namespace
+
+// =========================================================================================
+// C ABI
+// =========================================================================================
+TOS_EXPORT int tos_ctx_create(const tos_grid_desc* d, tos_ctx** out) {
+  if (!d || !out || d->nx <= 0 || d->ny <= 0 || d->nz <= 0 || !d->solid || !(d->h > 0)) return TOS_E_ARG;
+  if ((int64_t)(d->nx + 4) * (d->ny + 4) * (d->nz + 4) > ((int64_t)1 << 31)) return TOS_E_ARG;
+  *out = nullptr;
+  tos_ctx* c = new tos_ctx();
+  try {
+    HIPCHK(hipGetDevice(&c->dev));
+    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    c->nx = d->nx; c->ny = d->ny; c->nz = d->nz; c->h = d->h;
+    const size_t ne = (size_t)d->nx * d->ny * d->nz;
+    const size_t nn = (size_t)(d->nx + 1) * (d->ny + 1) * (d->nz + 1);
+    c->h_solid.assign(d->solid, d->solid + ne);
+    if (d->support) c->h_support.assign(d->support, d->support + nn);
+    if (d->shell) c->h_shell.assign(d->shell, d->shell + ne);
+    build_levels(*c, d->n_levels, d->coarsen_threshold > 0 ? d->coarsen_threshold : 400);
+    c->n_active = c->L[0].n_active;
+    for (size_t l = 0; l < c->L.size(); ++l) {
+      Level& L = c->L[l];
+      const int64_t NP = L.v.NP;
+      L.fre = dalloc<uint8_t>(*c, NP);
+      auto pf = pad_nodes(L.v, L.h_free);
+      upload(c->stream, L.fre, pf.data(), NP);
+      if (l > 0) {
+        L.st64 = dalloc<double>(*c, 243 * NP);
+        L.st32 = dalloc<float>(*c, 243 * NP);
+        L.u64 = dalloc<double>(*c, 3 * NP);
+        L.f64 = dalloc<double>(*c, 3 * NP);
+        L.u32 = dalloc<float>(*c, 3 * NP);
+        L.f32 = dalloc<float>(*c, 3 * NP);
+      }
+      if (l + 1 < c->L.size()) {
+        L.r64 = dalloc<double>(*c, 3 * NP);
+        L.r32 = dalloc<float>(*c, 3 * NP);
+      }
+      if (l == 0) {
+        L.u32 = dalloc<float>(*c, 3 * NP);
+        L.f32 = dalloc<float>(*c, 3 * NP);
+      }
+    }
+    const LevelView& v0 = c->L[0].v;
+    c->eact = dalloc<uint8_t>(*c, v0.NE);
+    auto pe = pad_elems(v0, c->h_solid.data());
+    upload(c->stream, c->eact, pe.data(), v0.NE);
+    c->shell = dalloc<uint8_t>(*c, v0.NE);
+    auto ps = pad_elems(v0, c->h_shell.empty() ? nullptr : c->h_shell.data());
+    for (size_t i = 0; i < ps.size(); ++i) ps[i] = ps[i] && pe[i];
+    upload(c->stream, c->shell, ps.data(), v0.NE);
+    c->rho = dalloc<float>(*c, v0.NE);
+    c->rho_pre = dalloc<float>(*c, v0.NE);
+    c->sens = dalloc<float>(*c, v0.NE);
+    c->vsens = dalloc<float>(*c, v0.NE);
+    c->tmpf = dalloc<float>(*c, v0.NE);
+    c->w64 = dalloc<double>(*c, v0.NE);
+    c->w32 = dalloc<float>(*c, v0.NE);
+    c->Ke64 = dalloc<double>(*c, 576);
+    c->Ke32 = dalloc<float>(*c, 576);
+    c->Mc64 = dalloc<double>(*c, 8 * 576);
+    c->r = dalloc<double>(*c, 3 * v0.NP);
+    c->z = dalloc<double>(*c, 3 * v0.NP);
+    c->p = dalloc<double>(*c, 3 * v0.NP);
+    c->q = dalloc<double>(*c, 3 * v0.NP);
+    c->partial = dalloc<double>(*c, 2 * kRedBlocks);
+    c->scal = dalloc<double>(*c, S_COUNT);
+    HIPCHK(hipHostMalloc((void**)&c->h_scal, S_COUNT * sizeof(double)));
+    c->K = 1;
+    c->F = dalloc<double>(*c, 3 * v0.NP);
+    c->U = dalloc<double>(*c, 3 * v0.NP);
+    if (c->L.size() > 1) coarse_alloc(*c);
+    HIPCHK(hipStreamSynchronize(c->stream));
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "tos_ctx_create: %s\n", e.what());
+    for (void* p : c->allocs) (void)hipFree(p);
+    delete c;
+    return TOS_E_HIP;
+  }
+  *out = c;
+  return tos_set_material(c, c->E, c->nu, c->penal, c->rho_min);
+}
+
+TOS_EXPORT int tos_ctx_destroy(tos_ctx* c) {
+  if (!c) return TOS_E_ARG;
+  (void)hipSetDevice(c->dev);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (void* p : c->allocs) (void)hipFree(p);
+  for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+  if (c->stage) (void)hipFree(c->stage);
+  if (c->h_scal) (void)hipHostFree(c->h_scal);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return TOS_OK;
+}
+
+TOS_EXPORT const char* tos_last_error(const tos_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+TOS_EXPORT int tos_get_info(tos_ctx* c, tos_info* o) {
+  if (!c || !o) return TOS_E_ARG;
+  std::memset(o, 0, sizeof(*o));
+  o->n_levels = (int)c->L.size();
+  for (size_t l = 0; l < c->L.size() && l < 16; ++l) {
+    o->level_dims[l][0] = c->L[l].v.ex;
+    o->level_dims[l][1] = c->L[l].v.ey;
+    o->level_dims[l][2] = c->L[l].v.ez;
+    o->n_active_elements[l] = c->L[l].n_active;
+    o->n_free_nodes[l] = c->L[l].n_free;
+  }
+  o->n_elements = (int64_t)c->nx * c->ny * c->nz;
+  o->n_nodes = (int64_t)(c->nx + 1) * (c->ny + 1) * (c->nz + 1);
+  o->coarse_dofs = c->cn;
+  o->n_cases = c->K;
+  o->n_coeffs = (int64_t)c->nc[0] * c->nc[1] * c->nc[2];
+  return TOS_OK;
+}
+
+TOS_EXPORT int tos_get_node_flags(tos_ctx* c, int level, uint8_t* out) {
+  return guard(c, [&]() {
+    REQUIRE(out && level >= 0 && level < (int)c->L.size(), "bad level");
+    const Level& L = c->L[level];
+    const LevelView& v = L.v;
+    for (int z = 0; z < v.nz; ++z)
+      for (int y = 0; y < v.ny; ++y)
+        for (int x = 0; x < v.nx; ++x) {
+          bool act = false;
+          for (int a = 0; a < 8 && !act; ++a) {
+            const int ex = x - 1 + (a & 1), ey = y - 1 + ((a >> 1) & 1), ez = z - 1 + (a >> 2);
+            if (ex < 0 || ey < 0 || ez < 0 || ex >= v.ex || ey >= v.ey || ez >= v.ez) continue;
+            act = L.h_eact[ex + (size_t)v.ex * (ey + (size_t)v.ey * ez)];
+          }
+          const size_t n = x + (size_t)v.nx * (y + (size_t)v.ny * z);
+          const bool sup = level == 0 && !c->h_support.empty() && c->h_support[n];
+          const int col = (x & 1) + 2 * (y & 1) + 4 * (z & 1);
+          out[n] = (uint8_t)((act ? 1 : 0) | (sup ? 2 : 0) | (L.h_free[n] ? 4 : 0) | (col << 3));
+        }
+    return TOS_OK;
+  });
+}
+
+TOS_EXPORT int tos_set_material(tos_ctx* c, double E, double nu, double p, double rmin) {
+  return guard(c, [&]() {
+    REQUIRE(E > 0 && nu > -1 && nu < 0.5 && p >= 1 && rmin > 0, "bad material");
+    c->E = E; c->nu = nu; c->penal = p; c->rho_min = rmin;
+    hex_stiffness(c->h, E, nu, c->h_Ke);
+    child_products(c->h_Ke, c->h_Mc);
+    float k32[576];
+    for (int i = 0; i < 576; ++i) k32[i] = (float)c->h_Ke[i];
+    upload(c->stream, c->Ke64, c->h_Ke, sizeof(c->h_Ke));
+    upload(c->stream, c->Ke32, k32, sizeof(k32));
+    upload(c->stream, c->Mc64, c->h_Mc, sizeof(c->h_Mc));
+    c->op_ready = false;
+    return TOS_OK;
+  });
+}
+
+TOS_EXPORT int tos_get_template_matrix(tos_ctx* c, double* ke) {
+  if (!c || !ke) return TOS_E_ARG;
+  std::memcpy(ke, c->h_Ke, sizeof(c->h_Ke));
+  return TOS_OK;
+}
+
+TOS_EXPORT int tos_set_loads(tos_ctx* c, int K, const double* f) {
+  return guard(c, [&]() {
+    REQUIRE(f && K > 0 && K <= 64, "bad loads");
+    Level& L0 = c->L[0];
+    const int64_t n = (int64_t)L0.v.nx * L0.v.ny * L0.v.nz, n3p = 3 * L0.v.NP;
+    if (K != c->K) {
+      // grow F, U (old buffers stay in the allocation list until destroy)
+      c->F = dalloc<double>(*c, (size_t)K * n3p);
+      c->U = dalloc<double>(*c, (size_t)K * n3p);
+      c->K = K;
+    }
+    double* st = (double*)stage(*c, 3 * n * sizeof(double));
+    for (int k = 0; k < K; ++k) {
+      for (int64_t i = 0; i < 3 * n; ++i)
+        if (!std::isfinite(f[(size_t)k * 3 * n + i])) throw ArgError("non-finite load");
+      upload(c->stream, st, f + (size_t)k * 3 * n, 3 * n * sizeof(double));
+      hipLaunchKernelGGL(k_pad_vec, dim3(nblk(n)), dim3(256), 0, c->stream, L0.v, (const double*)st,
+                         c->F + (size_t)k * n3p, (const uint8_t*)L0.fre);
+      hipLaunchKernelGGL(k_zero<double>, dim3(nblk(n3p)), dim3(256), 0, c->stream, c->U + (size_t)k * n3p, n3p);
+    }
+    LAUNCH_CHECK();
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return TOS_OK;
+  });
+}
+
+static int get_vec(tos_ctx* c, const double* dsrc, double* out) {
+  Level& L0 = c->L[0];
+  const int64_t n = (int64_t)L0.v.nx * L0.v.ny * L0.v.nz;
+  double* st = (double*)stage(*c, 3 * n * sizeof(double));
+  hipLaunchKernelGGL(k_unpad_vec, dim3(nblk(n)), dim3(256), 0, c->stream, L0.v, dsrc, st);
+  LAUNCH_CHECK();
+  download(c->stream, out, st, 3 * n * sizeof(double));
+  return TOS_OK;
+}
+
+TOS_EXPORT int tos_get_loads(tos_ctx* c, int k, double* f) {
+  return guard(c, [&]() {
+    REQUIRE(f && k >= 0 && k < c->K, "bad case");
+    return get_vec(c, c->F + (size_t)k * 3 * c->L[0].v.NP, f);
+  });
+}
+
+TOS_EXPORT int tos_set_density(tos_ctx* c, const float* rho) {
+  return guard(c, [&]() {
+    REQUIRE(rho, "null density");
+    const LevelView& v = c->L[0].v;
+    const int64_t n = (int64_t)v.ex * v.ey * v.ez;
+    for (int64_t i = 0; i < n; ++i)
+      if (!std::isfinite(rho[i])) {
+        c->err = "non-finite density";
+        return TOS_E_NAN;
+      }
+    float* st = (float*)stage(*c, n * sizeof(float));
+    upload(c->stream, st, rho, n * sizeof(float));
+    hipLaunchKernelGGL(k_pad_elems, dim3(nblk(n)), dim3(256), 0, c->stream, v, (const float*)st, c->rho);
+    hipLaunchKernelGGL(k_mask_inactive, dim3(nblk(v.NE)), dim3(256), 0, c->stream, v, (const uint8_t*)c->eact, c->rho);
+    LAUNCH_CHECK();
+    c->op_ready = false;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return TOS_OK;
+  });
+}
+
+TOS_EXPORT int tos_get_density(tos_ctx* c, float* rho) {
+  return guard(c, [&]() {
+    REQUIRE(rho, "null density");
+    const LevelView& v = c->L[0].v;
+    const int64_t n = (int64_t)v.ex * v.ey * v.ez;
+    float* st = (float*)stage(*c, n * sizeof(float));
+    hipLaunchKernelGGL(k_unpad_elems, dim3(nblk(n)), dim3(256), 0, c->stream, v, (const float*)c->rho, st);
+    LAUNCH_CHECK();
+    download(c->stream, rho, st, n * sizeof(float));
+    return TOS_OK;
+  });
+}
+
+// update_stencil (grid/Grid.cpp:4167-4178): weights, Galerkin levels, coarsest factor
+TOS_EXPORT int tos_update_operator(tos_ctx* c) {
+  return guard(c, [&]() {
+    const LevelView& v0 = c->L[0].v;
+    Prof pr(*c, P_GALERKIN);
+    hipLaunchKernelGGL(k_weights, dim3(nblk(v0.NE)), dim3(256), 0, c->stream, v0, (const uint8_t*)c->eact,
+                       (const float*)c->rho, c->penal, c->w64, c->w32);
+    for (size_t l = 1; l < c->L.size(); ++l) {
+      Level& F = c->L[l - 1];
+      Level& C = c->L[l];
+      const int64_t nc = (int64_t)C.v.nx * C.v.ny * C.v.nz;
+      dim3 grid(nblk(nc), 27);
+      if (l == 1)
+        hipLaunchKernelGGL(k_galerkin_first, grid, dim3(256), 0, c->stream, F.v, C.v, (const uint8_t*)F.fre,
+                           (const uint8_t*)C.fre, (const double*)c->w64, (const double*)c->Ke64,
+                           (const double*)c->Mc64, C.st64);
+      else
+        hipLaunchKernelGGL(k_galerkin_next, grid, dim3(256), 0, c->stream, F.v, C.v, (const uint8_t*)C.fre,
+                           (const double*)F.st64, C.st64);
+      const int64_t ns = 243 * C.v.NP;
+      hipLaunchKernelGGL(k_cast_d2f, dim3(nblk(ns)), dim3(256), 0, c->stream, (const double*)C.st64, C.st32, ns);
+      LAUNCH_CHECK();
+    }
+    const int rc = coarse_factor(*c);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->op_ready = true;
+    return TOS_OK;
+  });
+}
+
+TOS_EXPORT int tos_get_stencil(tos_ctx* c, int level, double* out) {
+  return guard(c, [&]() {
+    REQUIRE(out && level >= 1 && level < (int)c->L.size(), "bad level");
+    if (!c->op_ready) { c->err = "update_operator first"; return TOS_E_STATE; }
+    const LevelView& v = c->L[level].v;
+    std::vector<double> h((size_t)243 * v.NP);
+    download(c->stream, h.data(), c->L[level].st64, h.size() * sizeof(double));
+    const size_t nn = (size_t)v.nx * v.ny * v.nz;
+    for (int e = 0; e < 243; ++e)
+      for (int z = 0; z < v.nz; ++z)
+        for (int y = 0; y < v.ny; ++y)
+          for (int x = 0; x < v.nx; ++x)
+            out[e * nn + x + (size_t)v.nx * (y + (size_t)v.ny * z)] = h[(size_t)e * v.NP + v.pnode(x, y, z)];
+    return TOS_OK;
+  });
+}
+
+TOS_EXPORT int tos_apply_operator(tos_ctx* c, const double* u, double* y) {
+  return guard(c, [&]() {
+    REQUIRE(u && y, "null vector");
+    if (!c->op_ready) { c->err = "update_operator first"; return TOS_E_STATE; }
+    Level& L0 = c->L[0];
+    const int64_t n = (int64_t)L0.v.nx * L0.v.ny * L0.v.nz;
+    double* st = (double*)stage(*c, 3 * n * sizeof(double));
+    upload(c->stream, st, u, 3 * n * sizeof(double));
+    hipLaunchKernelGGL(k_pad_vec, dim3(nblk(n)), dim3(256), 0, c->stream, L0.v, (const double*)st, c->p,
+                       (const uint8_t*)L0.fre);
+    apply_level<double, 0>(*c, 0, c->p, nullptr, c->q);
+    return get_vec(c, c->q, y);
+  });
+}
+
+TOS_EXPORT int tos_vcycle(tos_ctx* c, const double* r, double* z, int precision, int pre, int post) {
+  return guard(c, [&]() {
+    REQUIRE(r && z && pre >= 0 && post >= 0, "bad arguments");
+    if (!c->op_ready) { c->err = "update_operator first"; return TOS_E_STATE; }
+    Level& L0 = c->L[0];
+    const int64_t n = (int64_t)L0.v.nx * L0.v.ny * L0.v.nz;
+    double* st = (double*)stage(*c, 3 * n * sizeof(double));
+    upload(c->stream, st, r, 3 * n * sizeof(double));
+    hipLaunchKernelGGL(k_pad_vec, dim3(nblk(n)), dim3(256), 0, c->stream, L0.v, (const double*)st, c->r,
+                       (const uint8_t*)L0.fre);
+    precondition(*c, c->r, c->z, precision, pre, post);
+    return get_vec(c, c->z, z);
+  });
+}
+
+TOS_EXPORT int tos_solve(tos_ctx* c, const tos_solve_opts* o, tos_solve_stats* st) {
+  return guard(c, [&]() {
+    REQUIRE(o && o->rtol > 0 && o->maxit >= 0 && o->pre >= 0 && o->post >= 0, "bad options");
+    if (!c->op_ready) { c->err = "update_operator first"; return TOS_E_STATE; }
+    const auto t0 = std::chrono::steady_clock::now();
+    int rc_all = TOS_OK;
+    for (int k = 0; k < c->K; ++k) {
+      int its = 0;
+      double rr = 0;
+      const int rc = solve_case(*c, k, *o, &its, &rr);
+      if (st && k < 64) { st->iterations[k] = its; st->rel_res[k] = rr; }
+      if (rc == TOS_E_NAN) return rc;
+      if (rc) rc_all = rc;
+    }
+    if (rc_all == TOS_E_NOCONV) c->err = "solver did not reach rtol";
+    if (st) st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return rc_all;
+  });
+}
+
+TOS_EXPORT int tos_get_displacement(tos_ctx* c, int k, double* u) {
+  return guard(c, [&]() {
+    REQUIRE(u && k >= 0 && k < c->K, "bad case");
+    return get_vec(c, c->U + (size_t)k * 3 * c->L[0].v.NP, u);
+  });
+}
+
+TOS_EXPORT int tos_set_displacement(tos_ctx* c, int k, const double* u) {
+  return guard(c, [&]() {
+    REQUIRE(u && k >= 0 && k < c->K, "bad case");
+    Level& L0 = c->L[0];
+    const int64_t n = (int64_t)L0.v.nx * L0.v.ny * L0.v.nz;
+    double* st = (double*)stage(*c, 3 * n * sizeof(double));
+    upload(c->stream, st, u, 3 * n * sizeof(double));
+    hipLaunchKernelGGL(k_pad_vec, dim3(nblk(n)), dim3(256), 0, c->stream, L0.v, (const double*)st,
+                       c->U + (size_t)k * 3 * L0.v.NP, (const uint8_t*)L0.fre);
+    LAUNCH_CHECK();
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return TOS_OK;
+  });
+}
+
+TOS_EXPORT int tos_compliance(tos_ctx* c, double* out) {
+  return guard(c, [&]() {
+    REQUIRE(out, "null output");
+    const int64_t n3 = 3 * c->L[0].v.NP;
+    for (int k = 0; k < c->K; ++k) {
+      dot_to(*c, c->F + (size_t)k * n3, c->U + (size_t)k * n3, nullptr, nullptr, n3, S_TMP0, 0, OP_NONE);
+      out[k] = read_scal(*c)[S_TMP0];
+    }
+    return TOS_OK;
+  });
+}
+
+TOS_EXPORT int tos_volume(tos_ctx* c, double* v) {
+  return guard(c, [&]() {
+    REQUIRE(v, "null output");
+    *v = volume_of(*c);
+    return TOS_OK;
+  });
+}
+
+TOS_EXPORT int tos_sensitivity(tos_ctx* c, int k) {
+  return guard(c, [&]() {
+    REQUIRE(k >= 0 && k < c->K, "bad case");
+    const LevelView& v = c->L[0].v;
+    const int64_t n = (int64_t)v.ex * v.ey * v.ez;
+    hipLaunchKernelGGL((k_sensitivity<0>), dim3(nblk(n)), dim3(256), 0, c->stream, v, (const uint8_t*)c->eact,
+                       (const float*)c->rho, (const double*)c->Ke64, (const double*)(c->U + (size_t)k * 3 * v.NP),
+                       c->penal, c->sens, (double*)nullptr);
+    LAUNCH_CHECK();
+    return TOS_OK;
+  });
+}
+
+TOS_EXPORT int tos_element_compliance(tos_ctx* c, int k, double* out) {
+  return guard(c, [&]() {
+    REQUIRE(out && k >= 0 && k < c->K, "bad case");
+    const LevelView& v = c->L[0].v;
+    const int64_t n = (int64_t)v.ex * v.ey * v.ez;
+    double* st = (double*)stage(*c, n * sizeof(double));
+    hipLaunchKernelGGL((k_sensitivity<1>), dim3(nblk(n)), dim3(256), 0, c->stream, v, (const uint8_t*)c->eact,
+                       (const float*)c->rho, (const double*)c->Ke64, (const double*)(c->U + (size_t)k * 3 * v.NP),
+                       c->penal, (float*)nullptr, st);
+    LAUNCH_CHECK();
+    download(c->stream, out, st, n * sizeof(double));
+    return TOS_OK;
+  });
+}
+
+TOS_EXPORT int tos_get_sensitivity(tos_ctx* c, int which, float* out) {
+  return guard(c, [&]() {
+    float* b = sens_buf(*c, which);
+    REQUIRE(b && out, "bad sensitivity request");
+    const LevelView& v = c->L[0].v;
+    const int64_t n = (int64_t)v.ex * v.ey * v.ez;
+    float* st = (float*)stage(*c, n * sizeof(float));
+    hipLaunchKernelGGL(k_unpad_elems, dim3(nblk(n)), dim3(256), 0, c->stream, v, (const float*)b, st);
+    LAUNCH_CHECK();
+    download(c->stream, out, st, n * sizeof(float));
+    return TOS_OK;
+  });
+}
+
+TOS_EXPORT int tos_set_sensitivity(tos_ctx* c, int which, const float* in) {
+  return guard(c, [&]() {
+    float* b = sens_buf(*c, which);
+    REQUIRE(b && in, "bad sensitivity request");
+    const LevelView& v = c->L[0].v;
+    const int64_t n = (int64_t)v.ex * v.ey * v.ez;
+    float* st = (float*)stage(*c, n * sizeof(float));
+    upload(c->stream, st, in, n * sizeof(float));
+    hipLaunchKernelGGL(k_pad_elems, dim3(nblk(n)), dim3(256), 0, c->stream, v, (const float*)st, b);
+    hipLaunchKernelGGL(k_mask_inactive, dim3(nblk(v.NE)), dim3(256), 0, c->stream, v, (const uint8_t*)c->eact, b);
+    LAUNCH_CHECK();
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return TOS_OK;
+  });
+}
+
+TOS_EXPORT int tos_init_vol_sensitivity(tos_ctx* c) {
+  return guard(c, [&]() {
+    const LevelView& v = c->L[0].v;
+    hipLaunchKernelGGL(k_fill_active, dim3(nblk(v.NE)), dim3(256), 0, c->stream, v, (const uint8_t*)c->eact, c->vsens,
+                       (float)(1.0 / (double)c->n_active));
+    LAUNCH_CHECK();
+    return TOS_OK;
+  });
+}
+
+TOS_EXPORT int tos_filter(tos_ctx* c, int which, double R) {
+  return guard(c, [&]() {
+    float* b = sens_buf(*c, which);
+    REQUIRE(b && R > 0 && R < 64, "bad filter request");
+    // ball table: offsets within floor(R + 1/2) with d^2 <= R^2, weight 1-6r^2+8r^3-3r^4
+    const int Ri = (int)(R + 0.5);
+    std::vector<int> off;
+    std::vector<double> wt;
+    for (int dz = -Ri; dz <= Ri; ++dz)
+      for (int dy = -Ri; dy <= Ri; ++dy)
+        for (int dx = -Ri; dx <= Ri; ++dx) {
+          const double r2 = dx * dx + dy * dy + dz * dz;
+          if (r2 > R * R) continue;
+          const double r = std::sqrt(r2 / (R * R));
+          off.push_back(dx); off.push_back(dy); off.push_back(dz);
+          wt.push_back(1 - 6 * r * r + 8 * r * r * r - 3 * r * r * r * r);
+        }
+    const int nt = (int)wt.size();
+    char* st = (char*)stage(*c, nt * (3 * sizeof(int) + sizeof(double)) + 64);
+    int* doff = (int*)st;
+    double* dwt = (double*)(st + ((nt * 3 * sizeof(int) + 15) / 16) * 16);
+    upload(c->stream, doff, off.data(), off.size() * sizeof(int));
+    upload(c->stream, dwt, wt.data(), wt.size() * sizeof(double));
+    const LevelView& v = c->L[0].v;
+    HIPCHK(hipMemcpyAsync(c->tmpf, b, v.NE * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    const int64_t n = (int64_t)v.ex * v.ey * v.ez;
+    hipLaunchKernelGGL(k_filter, dim3(nblk(n)), dim3(256), 0, c->stream, v, (const uint8_t*)c->eact,
+                       (const float*)c->tmpf, b, (const int*)doff, (const double*)dwt, nt);
+    LAUNCH_CHECK();
+    HIPCHK(hipStreamSynchronize(c->stream));  // the table lives in the staging buffer
+    return TOS_OK;
+  });
+}
+
+TOS_EXPORT int tos_oc_update(tos_ctx* c, double vgoal, double move, double damp, double* g_out, int* steps) {
+  return guard(c, [&]() {
+    REQUIRE(move > 0 && damp > 0, "bad OC parameters");
+    const LevelView& v = c->L[0].v;
+    hipLaunchKernelGGL(k_absmax_partial, dim3(kRedBlocks), dim3(256), 0, c->stream, v, (const uint8_t*)c->eact,
+                       (const float*)c->sens, c->partial);
+    hipLaunchKernelGGL(k_oc_init, dim3(1), dim3(256), 0, c->stream, (const double*)c->partial, kRedBlocks, c->scal);
+    for (int it = 0; it < 31; ++it) {
+      hipLaunchKernelGGL(k_oc_trial, dim3(kRedBlocks), dim3(256), 0, c->stream, v, (const uint8_t*)c->eact,
+                         (const uint8_t*)c->shell, (const float*)c->rho, (const float*)c->sens, move, damp,
+                         c->rho_min, (const double*)c->scal, c->partial);
+      hipLaunchKernelGGL(k_oc_decide, dim3(1), dim3(256), 0, c->stream, (const double*)c->partial, kRedBlocks,
+                         (double)c->n_active, vgoal, c->scal);
+    }
+    hipLaunchKernelGGL(k_oc_apply, dim3(nblk(v.NE)), dim3(256), 0, c->stream, v, (const uint8_t*)c->eact,
+                       (const uint8_t*)c->shell, c->rho, (const float*)c->sens, move, damp, c->rho_min,
+                       (const double*)c->scal);
+    LAUNCH_CHECK();
+    const double* hs = read_scal(*c);
+    if (g_out) *g_out = hs[S_OC_G];
+    if (steps) *steps = (int)hs[S_OC_STEPS];
+    c->op_ready = false;
+    return TOS_OK;
+  });
+}
+
+TOS_EXPORT int tos_project_density(tos_ctx* c, double beta) {
+  return guard(c, [&]() {
+    REQUIRE(beta > 0, "bad beta");
+    const LevelView& v = c->L[0].v;
+    hipLaunchKernelGGL(k_project, dim3(nblk(v.NE)), dim3(256), 0, c->stream, v, (const uint8_t*)c->eact,
+                       (const uint8_t*)c->shell, c->rho, c->rho_pre, beta, c->rho_min);
+    LAUNCH_CHECK();
+    c->op_ready = false;
+    return TOS_OK;
+  });
+}
+
+TOS_EXPORT int tos_projection_sensitivity(tos_ctx* c, int which, double beta) {
+  return guard(c, [&]() {
+    float* b = sens_buf(*c, which);
+    REQUIRE(b && beta > 0, "bad projection request");
+    const LevelView& v = c->L[0].v;
+    hipLaunchKernelGGL(k_project_sens, dim3(nblk(v.NE)), dim3(256), 0, c->stream, v, (const uint8_t*)c->eact,
+                       (const float*)c->rho_pre, b, beta);
+    LAUNCH_CHECK();
+    return TOS_OK;
+  });
+}
+
+// ---- spline ------------------------------------------------------------------------------
+TOS_EXPORT int tos_spline_init(tos_ctx* c, int mode, int order, int param) {
+  return guard(c, [&]() {
+    REQUIRE((mode == 0 || mode == 1) && order >= 2 && order <= 8 && param >= 1, "bad spline parameters");
+    const int dims[3] = {c->nx, c->ny, c->nz};
+    AxisTable T[3];
+    for (int a = 0; a < 3; ++a) T[a] = spline_table(mode, order, param, dims[a], c->h);
+    c->sp_mode = mode; c->sp_order = order; c->sp_param = param;
+    for (int a = 0; a < 3; ++a) {
+      c->nc[a] = T[a].nc;
+      c->sp_idx[a] = dalloc<int>(*c, T[a].idx.size());
+      c->sp_wt[a] = dalloc<float>(*c, T[a].wt.size());
+      c->sp_ok[a] = dalloc<uint8_t>(*c, T[a].ok.size());
+      upload(c->stream, c->sp_idx[a], T[a].idx.data(), T[a].idx.size() * sizeof(int));
+      upload(c->stream, c->sp_wt[a], T[a].wt.data(), T[a].wt.size() * sizeof(float));
+      upload(c->stream, c->sp_ok[a], T[a].ok.data(), T[a].ok.size());
+      // transpose (coefficient -> list of (element, weight)) for the deterministic gather
+      std::vector<int> cnt(T[a].nc + 1, 0), ptr, elm;
+      std::vector<float> cw;
+      for (int e = 0; e < dims[a]; ++e)
+        if (T[a].ok[e])
+          for (int q = 0; q < order; ++q) cnt[T[a].idx[(size_t)e * order + q] + 1]++;
+      ptr.assign(T[a].nc + 1, 0);
+      for (int i = 0; i < T[a].nc; ++i) ptr[i + 1] = ptr[i] + cnt[i + 1];
+      elm.assign(ptr.back(), 0);
+      cw.assign(ptr.back(), 0.f);
+      std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+      for (int e = 0; e < dims[a]; ++e)
+        if (T[a].ok[e])
+          for (int q = 0; q < order; ++q) {
+            const int i = T[a].idx[(size_t)e * order + q];
+            elm[fill[i]] = e;
+            cw[fill[i]] = T[a].wt[(size_t)e * order + q];
+            fill[i]++;
+          }
+      c->sp_cptr[a] = dalloc<int>(*c, ptr.size());
+      c->sp_celm[a] = dalloc<int>(*c, elm.size());
+      c->sp_cw[a] = dalloc<float>(*c, cw.size());
+      upload(c->stream, c->sp_cptr[a], ptr.data(), ptr.size() * sizeof(int));
+      if (!elm.empty()) upload(c->stream, c->sp_celm[a], elm.data(), elm.size() * sizeof(int));
+      if (!cw.empty()) upload(c->stream, c->sp_cw[a], cw.data(), cw.size() * sizeof(float));
+    }
+    const size_t n = (size_t)c->nc[0] * c->nc[1] * c->nc[2];
+    c->coef = dalloc<float>(*c, n);
+    c->csens = dalloc<float>(*c, n);
+    c->cvsens = dalloc<float>(*c, n);
+    c->sp_g = dalloc<double>(*c, std::max((size_t)c->nx * c->ny * c->nz, n));  // also holds the last stage
+    c->sp_t1 = dalloc<double>(*c, (size_t)c->nc[0] * c->ny * c->nz);
+    c->sp_t2 = dalloc<double>(*c, std::max((size_t)c->nc[0] * c->nc[1] * c->nz, n));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return TOS_OK;
+  });
+}
+
+TOS_EXPORT int tos_spline_set_coeffs(tos_ctx* c, const float* v) {
+  return guard(c, [&]() {
+    REQUIRE(v && c->sp_mode >= 0, "spline not initialised");
+    upload(c->stream, c->coef, v, (size_t)c->nc[0] * c->nc[1] * c->nc[2] * sizeof(float));
+    return TOS_OK;
+  });
+}
+
+TOS_EXPORT int tos_spline_get_coeffs(tos_ctx* c, float* v) {
+  return guard(c, [&]() {
+    REQUIRE(v && c->sp_mode >= 0, "spline not initialised");
+    download(c->stream, v, c->coef, (size_t)c->nc[0] * c->nc[1] * c->nc[2] * sizeof(float));
+    return TOS_OK;
+  });
+}
+
+TOS_EXPORT int tos_spline_eval(tos_ctx* c) {
+  return guard(c, [&]() {
+    REQUIRE(c->sp_mode >= 0, "spline not initialised");
+    const LevelView& v = c->L[0].v;
+    const int64_t n = (int64_t)v.ex * v.ey * v.ez;
+    hipLaunchKernelGGL(k_spline_eval, dim3(nblk(n)), dim3(256), 0, c->stream, v, (const uint8_t*)c->eact,
+                       (const uint8_t*)c->shell, (const float*)c->coef, c->nc[0], c->nc[1], c->sp_order,
+                       (const int*)c->sp_idx[0], (const float*)c->sp_wt[0], (const uint8_t*)c->sp_ok[0],
+                       (const int*)c->sp_idx[1], (const float*)c->sp_wt[1], (const uint8_t*)c->sp_ok[1],
+                       (const int*)c->sp_idx[2], (const float*)c->sp_wt[2], (const uint8_t*)c->sp_ok[2], c->rho_min,
+                       c->rho);
+    LAUNCH_CHECK();
+    c->op_ready = false;
+    return TOS_OK;
+  });
+}
+
+TOS_EXPORT int tos_spline_adjoint(tos_ctx* c, int which) {
+  return guard(c, [&]() {
+    float* b = sens_buf(*c, which);
+    REQUIRE(b && c->sp_mode >= 0, "spline not initialised");
+    const LevelView& v = c->L[0].v;
+    const int64_t n = (int64_t)v.ex * v.ey * v.ez;
+    hipLaunchKernelGGL(k_spline_mask, dim3(nblk(n)), dim3(256), 0, c->stream, v, (const uint8_t*)c->eact,
+                       (const uint8_t*)c->shell, (const float*)b, (const uint8_t*)c->sp_ok[0],
+                       (const uint8_t*)c->sp_ok[1], (const uint8_t*)c->sp_ok[2], c->sp_g);
+    const int n0 = c->nc[0], n1 = c->nc[1], n2 = c->nc[2];
+    int64_t m1 = (int64_t)n0 * v.ey * v.ez, m2 = (int64_t)n0 * n1 * v.ez, m3 = (int64_t)n0 * n1 * n2;
+    hipLaunchKernelGGL(k_spline_stage, dim3(nblk(m1)), dim3(256), 0, c->stream, (const double*)c->sp_g, c->sp_t1, 0,
+                       n0, v.ey, v.ez, v.ex, (const int*)c->sp_cptr[0], (const int*)c->sp_celm[0],
+                       (const float*)c->sp_cw[0]);
+    hipLaunchKernelGGL(k_spline_stage, dim3(nblk(m2)), dim3(256), 0, c->stream, (const double*)c->sp_t1, c->sp_t2, 1,
+                       n0, n1, v.ez, v.ey, (const int*)c->sp_cptr[1], (const int*)c->sp_celm[1],
+                       (const float*)c->sp_cw[1]);
+    hipLaunchKernelGGL(k_spline_stage, dim3(nblk(m3)), dim3(256), 0, c->stream, (const double*)c->sp_t2, c->sp_g, 2,
+                       n0, n1, n2, v.ez, (const int*)c->sp_cptr[2], (const int*)c->sp_celm[2],
+                       (const float*)c->sp_cw[2]);
+    hipLaunchKernelGGL(k_d2f, dim3(nblk(m3)), dim3(256), 0, c->stream, (const double*)c->sp_g,
+                       which == 0 ? c->csens : c->cvsens, m3);
+    LAUNCH_CHECK();
+    return TOS_OK;
+  });
+}
+
+TOS_EXPORT int tos_spline_get_coeff_sens(tos_ctx* c, int which, float* out) {
+  return guard(c, [&]() {
+    REQUIRE(out && c->sp_mode >= 0 && (which == 0 || which == 1), "bad request");
+    download(c->stream, out, which == 0 ? c->csens : c->cvsens,
+             (size_t)c->nc[0] * c->nc[1] * c->nc[2] * sizeof(float));
+    return TOS_OK;
+  });
+}
+
+// ---- MMA -----------------------------------------------------------------------------------
+TOS_EXPORT int tos_mma_create(int n, int m, const double* xmin, const double* xmax, double move, tos_mma** out) {
+  if (n <= 0 || m <= 0 || m > MMAX || !xmin || !xmax || !out || !(move > 0)) return TOS_E_ARG;
+  tos_mma* M = new tos_mma();
+  try {
+    HIPCHK(hipGetDevice(&M->dev));
+    HIPCHK(hipStreamCreateWithFlags(&M->stream, hipStreamNonBlocking));
+    MmaDev& d = M->d;
+    d.n = n; d.m = m; d.move = move;
+    double** vecs[] = {&d.x, &d.xmin, &d.xmax, &d.L, &d.U, &d.alpha, &d.beta, &d.lastdx, &d.xi, &d.eta,
+                       &d.plam, &d.qlam, &d.curx, &d.Dx, &d.dlx, &d.dx, &d.dxi, &d.deta, &d.tx, &d.txi, &d.teta};
+    for (double** v : vecs) *v = malloc_mma<double>(M, n);
+    d.p = malloc_mma<double>(M, (size_t)(m + 1) * n);
+    d.q = malloc_mma<double>(M, (size_t)(m + 1) * n);
+    d.G = malloc_mma<double>(M, (size_t)m * n);
+    d.sv = malloc_mma<double>(M, SV_COUNT);
+    d.out = malloc_mma<double>(M, 3);
+    M->stage = malloc_mma<double>(M, (size_t)(m + 1) * n + m);
+    HIPCHK(hipHostMalloc((void**)&M->h_out, 3 * sizeof(double)));
+    // set_bound + clamp_asymptotes + init_subproblem_variable (mma_t.cu:747-855)
+    std::vector<double> x(n), L(n), U(n), al(n), be(n), xi(n), eta(n);
+    for (int j = 0; j < n; ++j) {
+      const double st = xmax[j] - xmin[j];
+      if (!(st > 0)) throw ArgError("xmax must exceed xmin");
+      x[j] = 0.5 * (xmin[j] + xmax[j]);
+      L[j] = std::min(std::max(xmin[j], x[j] - 10 * st), x[j] - 0.01 * st);
+      U[j] = std::min(std::max(xmax[j], x[j] + 0.01 * st), x[j] + 10 * st);
+      al[j] = std::max(xmin[j], std::max(L[j] + 0.1 * (x[j] - L[j]), x[j] - move * st));
+      be[j] = std::min(xmax[j], std::min(U[j] - 0.1 * (U[j] - x[j]), x[j] + move * st));
+      x[j] = 0.5 * (al[j] + be[j]);
+    }
+    upload(M->stream, d.x, x.data(), n * sizeof(double));
+    upload(M->stream, d.xmin, xmin, n * sizeof(double));
+    upload(M->stream, d.xmax, xmax, n * sizeof(double));
+    upload(M->stream, d.L, L.data(), n * sizeof(double));
+    upload(M->stream, d.U, U.data(), n * sizeof(double));
+    upload(M->stream, d.alpha, al.data(), n * sizeof(double));
+    upload(M->stream, d.beta, be.data(), n * sizeof(double));
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "tos_mma_create: %s\n", e.what());
+    for (void* p : M->allocs) (void)hipFree(p);
+    delete M;
+    return TOS_E_HIP;
+  }
+  *out = M;
+  return TOS_OK;
+}
+
+TOS_EXPORT int tos_mma_destroy(tos_mma* M) {
+  if (!M) return TOS_E_ARG;
+  (void)hipSetDevice(M->dev);
+  (void)hipStreamSynchronize(M->stream);
+  for (void* p : M->allocs) (void)hipFree(p);
+  if (M->h_out) (void)hipHostFree(M->h_out);
+  (void)hipStreamDestroy(M->stream);
+  delete M;
+  return TOS_OK;
+}
+
+TOS_EXPORT int tos_mma_set_x(tos_mma* M, const double* x) {
+  if (!M || !x) return TOS_E_ARG;
+  try {
+    HIPCHK(hipSetDevice(M->dev));
+    const int n = M->d.n;
+    std::vector<double> lo(n), hi(n), L(n), U(n), al(n), be(n);
+    download(M->stream, lo.data(), M->d.xmin, n * sizeof(double));
+    download(M->stream, hi.data(), M->d.xmax, n * sizeof(double));
+    for (int j = 0; j < n; ++j) {
+      const double st = hi[j] - lo[j];
+      L[j] = std::min(std::max(x[j] - 0.5 * st, x[j] - 10 * st), x[j] - 0.01 * st);
+      U[j] = std::min(std::max(x[j] + 0.5 * st, x[j] + 0.01 * st), x[j] + 10 * st);
+      al[j] = std::max(lo[j], std::max(L[j] + 0.1 * (x[j] - L[j]), x[j] - M->d.move * st));
+      be[j] = std::min(hi[j], std::min(U[j] - 0.1 * (U[j] - x[j]), x[j] + M->d.move * st));
+    }
+    upload(M->stream, M->d.x, x, n * sizeof(double));
+    upload(M->stream, M->d.L, L.data(), n * sizeof(double));
+    upload(M->stream, M->d.U, U.data(), n * sizeof(double));
+    upload(M->stream, M->d.alpha, al.data(), n * sizeof(double));
+    upload(M->stream, M->d.beta, be.data(), n * sizeof(double));
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "tos_mma_set_x: %s\n", e.what());
+    return TOS_E_HIP;
+  }
+  return TOS_OK;
+}
+
+TOS_EXPORT int tos_mma_get_x(tos_mma* M, double* x) {
+  if (!M || !x) return TOS_E_ARG;
+  try {
+    HIPCHK(hipSetDevice(M->dev));
+    download(M->stream, x, M->d.x, M->d.n * sizeof(double));
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "tos_mma_get_x: %s\n", e.what());
+    return TOS_E_HIP;
+  }
+  return TOS_OK;
+}
+
+TOS_EXPORT int tos_mma_update(tos_mma* M, const double* df, const double* dg, const double* g, int* converged) {
+  if (!M || !df || !dg || !g) return TOS_E_ARG;
+  try {
+    HIPCHK(hipSetDevice(M->dev));
+    const int n = M->d.n, m = M->d.m;
+    double* s = M->stage;
+    upload(M->stream, s, df, n * sizeof(double));
+    upload(M->stream, s + n, dg, (size_t)m * n * sizeof(double));
+    upload(M->stream, s + (size_t)(m + 1) * n, g, m * sizeof(double));
+    mma_run(M, s, s + n, s + (size_t)(m + 1) * n, converged);
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "tos_mma_update: %s\n", e.what());
+    return TOS_E_HIP;
+  }
+  return TOS_OK;
+}
+
+__global__ void k_mma_inputs(int n, const float* __restrict__ cs, const float* __restrict__ cv, double ss, double vs,
+                             double* __restrict__ df, double* __restrict__ dg, double* __restrict__ x,
+                             float* __restrict__ coef, int mode) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (mode == 0) {
+    df[i] = ss * (double)cs[i];
+    dg[i] = vs * (double)cv[i];
+  } else {
+    coef[i] = (float)x[i];
+  }
+}
+
+TOS_EXPORT int tos_ctx_mma_step(tos_ctx* c, tos_mma* M, double sscale, double vscale, double vfrac, int* converged) {
+  return guard(c, [&]() {
+    REQUIRE(M && c->sp_mode >= 0 && M->d.m == 1 &&
+                (int64_t)M->d.n == (int64_t)c->nc[0] * c->nc[1] * c->nc[2],
+            "MMA size does not match the spline coefficients");
+    const double vol = volume_of(*c);
+    const int n = M->d.n;
+    double* s = M->stage;
+    const double g = vscale * (vol - vfrac);
+    hipLaunchKernelGGL(k_mma_inputs, dim3(nblk(n)), dim3(256), 0, c->stream, n, (const float*)c->csens,
+                       (const float*)c->cvsens, sscale, vscale, s, s + n, (double*)nullptr, (float*)nullptr, 0);
+    HIPCHK(hipMemcpyAsync(s + 2 * n, &g, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    mma_run(M, s, s + n, s + 2 * n, converged);
+    hipLaunchKernelGGL(k_mma_inputs, dim3(nblk(n)), dim3(256), 0, c->stream, n, (const float*)nullptr,
+                       (const float*)nullptr, 0.0, 0.0, (double*)nullptr, (double*)nullptr, M->d.x, c->coef, 1);
+    LAUNCH_CHECK();
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return TOS_OK;
+  });
+}
+
+TOS_EXPORT int tos_profile(tos_ctx* c, int enable, tos_kernel_stats* out) {
+  return guard(c, [&]() {
+    if (enable) {
+      HIPCHK(hipStreamSynchronize(c->stream));
+      c->prof.clear();
+      c->ev_used = 0;
+      c->prof_on = true;
+      return TOS_OK;
+    }
+    c->prof_on = false;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (out) {
+      std::memset(out, 0, sizeof(*out));
+      out->n_classes = P_COUNT;
+      for (int i = 0; i < P_COUNT; ++i) std::snprintf(out->names[i], 32, "%s", kProfNames[i]);
+      for (const auto& r : c->prof) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, r.a, r.b));
+        out->total_ms[r.cls] += ms;
+        out->launches[r.cls] += 1;
+      }
+    }
+    c->prof.clear();
+    c->ev_used = 0;
+    return TOS_OK;
+  });
+}

@@ -1,0 +1,136 @@
+// tos_internal.hpp -- device layout and context of the gfx950 hot path.
+//
+// Layout in HBM (per multigrid level):
+//   nodes    padded box (nx+3) x (ny+3) x (nz+3), node (x,y,z) stored at (x+1,y+1,z+1),
+//            x fastest; one ghost layer below and two above, so every 27-point
+//            neighbourhood and every fine node 2I+-1 of a coarse node I addresses memory
+//            inside the array.  The component stride NP is a multiple of 64.
+//   vectors  3 components, structure of arrays, comp c at v[c*NP + p].  Ghost and
+//            eliminated (supported / inactive / unreachable) nodes hold 0 in every
+//            vector, so no kernel reads a mask for a neighbour.
+//   elements padded box (ex+2) x (ey+2) x (ez+2), element (ex,ey,ez) at (ex+1,...);
+//            ghost elements have weight 0.
+//   stencil  Galerkin 27-point 3x3 blocks st[(k*9 + r*3 + c)*NP + p], k = (dx+1) +
+//            3(dy+1) + 9(dz+1) (the reference's rxStencil order, grid/Grid.cu:847).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/tos_api.h"
+
+namespace tos {
+
+struct LevelView {  // passed by value to kernels
+  int nx, ny, nz;     // node dims
+  int NX, NY;         // padded node row / plane dims
+  int64_t NP;         // padded node count (vector component stride)
+  int ex, ey, ez;     // element dims
+  int EX, EY;         // padded element row / plane dims
+  int64_t NE;         // padded element count
+  __host__ __device__ int64_t pnode(int x, int y, int z) const {
+    return (int64_t)(x + 1) + (int64_t)NX * ((y + 1) + (int64_t)NY * (z + 1));
+  }
+  __host__ __device__ int64_t pelem(int x, int y, int z) const {
+    return (int64_t)(x + 1) + (int64_t)EX * ((y + 1) + (int64_t)EY * (z + 1));
+  }
+};
+
+struct Level {
+  LevelView v;
+  int64_t n_active = 0, n_free = 0;
+  std::vector<uint8_t> h_eact;   // host copy (unpadded) of element activity
+  std::vector<uint8_t> h_free;   // host copy (unpadded) of node freedom
+  uint8_t* fre = nullptr;        // device, padded nodes
+  double* st64 = nullptr;        // device stencil (levels >= 1)
+  float* st32 = nullptr;
+  double *u64 = nullptr, *f64 = nullptr, *r64 = nullptr;
+  float *u32 = nullptr, *f32 = nullptr, *r32 = nullptr;
+};
+
+struct Ctx {
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int nx, ny, nz;
+  double h;
+  double E = 1e6, nu = 0.4, penal = 3.0, rho_min = 1e-3;
+  std::vector<Level> L;
+  std::vector<uint8_t> h_solid, h_support, h_shell;
+  int64_t n_active = 0;
+  // fine-level element data (padded element layout)
+  uint8_t* eact = nullptr;
+  uint8_t* shell = nullptr;
+  float* rho = nullptr;
+  float* rho_pre = nullptr;
+  float* sens = nullptr;
+  float* vsens = nullptr;
+  float* tmpf = nullptr;        // element-sized scratch
+  double* w64 = nullptr;        // rho^p
+  float* w32 = nullptr;
+  // template matrices
+  double* Ke64 = nullptr;       // 576
+  float* Ke32 = nullptr;
+  double* Mc64 = nullptr;       // 8 x 576 Galerkin child products
+  double h_Ke[576];
+  double h_Mc[8 * 576];
+  // loads and solutions: K cases x 3 x NP (FP64)
+  int K = 0;
+  double* F = nullptr;
+  double* U = nullptr;
+  // PCG work vectors (FP64, 3*NP)
+  double *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
+  // reductions
+  double* partial = nullptr;    // kRedBlocks * 8
+  double* scal = nullptr;       // device scalars
+  double* h_scal = nullptr;     // pinned host mirror
+  // coarsest level
+  int cn = 0;                   // dense size (3 * free nodes)
+  int64_t* cidx = nullptr;      // padded node index of each coarse unknown node
+  double* cinv = nullptr;       // -A^{-1} after the sweep (cn x cn)
+  double* cwork = nullptr;      // panel buffers
+  int* cflag = nullptr;         // singular pivot flag
+  int* cmap = nullptr;          // padded coarsest node -> compact node index or -1
+  void* stage = nullptr;        // host<->device conversion buffer
+  size_t stage_bytes = 0;
+  bool op_ready = false;
+  // spline
+  int sp_mode = -1, sp_order = 0, sp_param = 0;
+  int nc[3] = {0, 0, 0};
+  int* sp_idx[3] = {nullptr, nullptr, nullptr};    // n_elem_axis * order
+  float* sp_wt[3] = {nullptr, nullptr, nullptr};
+  uint8_t* sp_ok[3] = {nullptr, nullptr, nullptr};
+  int* sp_cptr[3] = {nullptr, nullptr, nullptr};   // CSR transpose: per coefficient
+  int* sp_celm[3] = {nullptr, nullptr, nullptr};
+  float* sp_cw[3] = {nullptr, nullptr, nullptr};
+  float* coef = nullptr;
+  float* csens = nullptr;
+  float* cvsens = nullptr;
+  double* sp_t1 = nullptr;  // separable adjoint stages
+  double* sp_t2 = nullptr;
+  double* sp_g = nullptr;   // masked element gradient (unpadded, double)
+  std::vector<void*> allocs;
+  // kernel timing (HIP events on `stream`), see tos_profile
+  bool prof_on = false;
+  struct ProfRec {
+    int cls;
+    hipEvent_t a, b;
+  };
+  std::vector<ProfRec> prof;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+};
+
+constexpr int kRedBlocks = 1024;  // fixed partial count -> deterministic reductions
+
+}  // namespace tos
+
+// scalar slots in Ctx::scal
+enum {
+  S_RZ = 0, S_PQ, S_ALPHA, S_BETA, S_RN2, S_ZR, S_ZQ, S_NF2,
+  S_OC_LO, S_OC_HI, S_OC_G, S_OC_V, S_OC_DONE, S_OC_STEPS, S_OC_GMAX, S_VOL,
+  S_TMP0, S_TMP1, S_TMP2, S_TMP3,
+  S_COUNT = 32
+};
