@@ -1,0 +1,274 @@
+"""Benchmark: ms per topology-optimisation iteration on one MI355X.
+
+Default workload (the production-size case the headline metric is quoted on): the
+irregular 511x320x320 ellipsoid-union domain (~36% solid, z = 0 supported, unit load on
+the top surface along the occupancy normal, 3-voxel shell), cubic B-spline density on a
+lattice every 4 voxels, MMA on the coefficients, 7-level geometric MG.  One step = one
+optimisation iteration: spline evaluation, operator update (rho^p, Galerkin levels,
+coarsest factor), flexible PCG to 1e-6 relative residual (FP32 V-cycle, FP64 outer
+residual), compliance, sensitivity, filter, spline adjoint of the compliance and volume
+sensitivities, MMA step.  All inputs are resident on the GPU before the timed region.
+
+Prints one JSON line.  Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+[--workload c2|c1|c0] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "ms/TO-iter at 511-reso to 1e-6 residual, learned vs plain MG; HBM GB/s"
+HBM_PEAK = 8000.0  # GB/s, MI355X
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def active_nodes(solid):
+    nz, ny, nx = solid.shape
+    act = np.zeros((nz + 1, ny + 1, nx + 1), bool)
+    s = solid.astype(bool)
+    for dz in (0, 1):
+        for dy in (0, 1):
+            for dx in (0, 1):
+                act[dz:dz + nz, dy:dy + ny, dx:dx + nx] |= s
+    return int(act.sum())
+
+
+def make_workload(name):
+    from topoptspline3d_amd.domains import cantilever, ellipsoids
+    if name == "c2":
+        solid, support, f, shell, meta = ellipsoids(511, 320, 320, seed=2024)
+        return dict(name="c2", desc="ellipsoid union 511x320x320, spline spacing 4 (cubic), MMA",
+                    solid=solid, support=support, f=f, shell=shell, meta=meta, spline=(1, 4, 4),
+                    optimizer="mma", move=0.06, rtol=1e-6)
+    if name == "c1":
+        solid, support, f = cantilever(128, 64, 64)
+        return dict(name="c1", desc="cantilever 128x64x64, spline spacing 4 (cubic), MMA",
+                    solid=solid, support=support, f=f, shell=None, meta={}, spline=(1, 4, 4),
+                    optimizer="mma", move=0.06, rtol=1e-6)
+    if name == "c0":
+        solid, support, f = cantilever(24, 12, 12)
+        return dict(name="c0", desc="cantilever 24x12x12, element densities, OC", solid=solid,
+                    support=support, f=f, shell=None, meta={}, spline=None, optimizer="oc",
+                    move=0.03, rtol=1e-8)
+    raise SystemExit(f"unknown workload {name}")
+
+
+class Loop:
+    """One optimisation iteration over a context (api = HIP or oracle binding)."""
+
+    def __init__(self, api, wl, precision):
+        from topoptspline3d_amd.context import MMA, Context
+        self.api, self.wl, self.precision = api, wl, precision
+        ctx = Context(api, wl["solid"], wl["support"], shell=wl["shell"])
+        ctx.set_material(1e6, 0.4, 3.0, 1e-3)
+        ctx.set_loads(wl["f"])
+        self.ctx = ctx
+        if wl["spline"]:
+            mode, order, spacing = wl["spline"]
+            n = ctx.spline_init(mode, order, spacing)
+            rng = np.random.default_rng(11)
+            c0 = (0.4 + rng.uniform(-0.05, 0.05, n)).astype(np.float32)
+            ctx.set_coeffs(c0)
+            self.mma = MMA(api, n, 1, 1e-3, 1.0, move=wl["move"])
+            self.mma.set_x(c0.astype(np.float64))
+        else:
+            ctx.set_density(np.where(wl["solid"].ravel() > 0, 0.4, 0).astype(np.float32))
+        self.its, self.res, self.c = [], [], []
+        self.sens_scale = None
+
+    def step(self):
+        ctx, wl = self.ctx, self.wl
+        if wl["spline"]:
+            ctx.spline_eval()
+        ctx.update_operator()
+        st = ctx.solve(rtol=wl["rtol"], maxit=300, precision=self.precision)
+        self.its.append(st["iterations"][0])
+        self.res.append(st["rel_res"][0])
+        self.c.append(ctx.compliance()[0])
+        ctx.sensitivity()
+        ctx.filter(2.0, 0)
+        if wl["spline"]:
+            ctx.init_vol_sensitivity()
+            ctx.filter(2.0, 1)
+            ctx.spline_adjoint(0)
+            ctx.spline_adjoint(1)
+            if self.sens_scale is None:  # objective normalised by its first value
+                self.sens_scale = 1.0 / self.c[-1]
+            self.mma.step_on(ctx, self.sens_scale, 1e3, 0.4)
+        else:
+            ctx.oc_update(0.4, wl["move"], 0.5)
+
+
+def sync():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+
+
+def kernel_stats(api, ctx, enable):
+    import ctypes
+    from topoptspline3d_amd import _abi
+    ks = _abi.KernelStats()
+    rc = api.profile(ctx.h, 1 if enable else 0, ctypes.byref(ks))
+    if rc:
+        raise RuntimeError("profile failed")
+    if enable:
+        return None
+    return {ks.names[i].value.decode(): (ks.total_ms[i], ks.launches[i]) for i in range(ks.n_classes)}
+
+
+def roofline(stats, info, n_active_nodes, steps):
+    """Roofline of the dominant kernel class.  Algorithmic bytes per launch (each array
+    read or written once at storage precision, index/flag re-reads excluded):
+      fine GS colour pass (FP32): one sweep = 4 N_e + (9*4 + 4) N_v, one launch = 1/8
+      fine residual (FP32/FP64):  4 N_e + (9 s + 4) N_v
+      fine operator (FP64):       4 N_e + (6*8 + 4) N_v"""
+    Ne, Nv = info["n_active_elements"][0], n_active_nodes
+    per_launch = {
+        "fine_gs_f32": (4 * Ne + (9 * 4 + 4) * Nv) / 8.0,
+        "fine_gs_f64": (4 * Ne + (9 * 8 + 4) * Nv) / 8.0,
+        "fine_resid_f32": 4 * Ne + (9 * 4 + 4) * Nv,
+        "fine_resid_f64": 4 * Ne + (9 * 8 + 4) * Nv,
+        "fine_apply_f64": 4 * Ne + (6 * 8 + 4) * Nv,
+    }
+    flops = {"fine_gs_f32": 1170 * Nv / 8.0, "fine_gs_f64": 1170 * Nv / 8.0, "fine_resid_f32": 1152 * Nv,
+             "fine_resid_f64": 1152 * Nv, "fine_apply_f64": 1152 * Nv}
+    dom = max(stats, key=lambda k: stats[k][0])
+    tot = {k: v[0] for k, v in stats.items() if v[1] > 0}
+    out = {"bound": "hbm", "peak": HBM_PEAK, "unit": "GB/s", "traffic": None, "kernel": dom,
+           "kernel_ms_per_step": {k: round(v / steps, 3) for k, v in tot.items()}}
+    if dom in per_launch and stats[dom][1] > 0:
+        avg_ms = stats[dom][0] / stats[dom][1]
+        ach = per_launch[dom] / (avg_ms * 1e-3) / 1e9
+        out.update({"achieved": round(ach, 1), "frac": round(ach / HBM_PEAK, 4),
+                    "avg_launch_us": round(avg_ms * 1e3, 2), "launches": stats[dom][1],
+                    "bytes_per_launch": per_launch[dom],
+                    "achieved_tflops": round(flops[dom] / (avg_ms * 1e-3) / 1e12, 2)})
+    else:
+        out.update({"achieved": None, "frac": None})
+    return out
+
+
+def cpu_baseline(wl, gpu_its):
+    """FP64 oracle on the host cores: a bounded sample of the same workload -- one
+    operator update, one flexible-PCG iteration (operator apply + V-cycle) and one
+    sensitivity + filter + adjoint pass -- scaled to one optimisation iteration with the
+    GPU's mean PCG iteration count."""
+    import ctypes
+    from topoptspline3d_amd import _abi
+    from topoptspline3d_amd.context import Context
+    lib = os.path.join(ROOT, "oracle", "liboracle.so")
+    if not os.path.exists(lib):
+        import subprocess
+        subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
+    api = _abi.bind(ctypes.CDLL(lib), "orc_")
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    ctx = Context(api, wl["solid"], wl["support"], shell=wl["shell"])
+    ctx.set_material(1e6, 0.4, 3.0, 1e-3)
+    ctx.set_loads(wl["f"])
+    if wl["spline"]:
+        mode, order, spacing = wl["spline"]
+        n = ctx.spline_init(mode, order, spacing)
+        ctx.set_coeffs(np.full(n, 0.4, np.float32))
+        t = time.perf_counter(); ctx.spline_eval(); t_eval = time.perf_counter() - t
+    else:
+        ctx.set_density(np.where(wl["solid"].ravel() > 0, 0.4, 0).astype(np.float32))
+        t_eval = 0.0
+    t = time.perf_counter(); ctx.update_operator(); t_op = time.perf_counter() - t
+    r = ctx.get_loads(0)
+    t = time.perf_counter(); z = ctx.vcycle(r); t_vc = time.perf_counter() - t
+    t = time.perf_counter(); ctx.apply(z); t_ap = time.perf_counter() - t
+    t = time.perf_counter()
+    ctx.sensitivity(); ctx.filter(2.0, 0)
+    if wl["spline"]:
+        ctx.init_vol_sensitivity(); ctx.filter(2.0, 1); ctx.spline_adjoint(0); ctx.spline_adjoint(1)
+    t_post = time.perf_counter() - t
+    it = float(np.mean(gpu_its))
+    est = t_eval + t_op + it * (t_vc + t_ap) + t_post
+    ctx.close()
+    return {"value": round(est * 1e3, 1), "unit": "ms/TO-iter", "cores": cores, "kind": "port",
+            "sample": (f"FP64 oracle on the same domain: spline eval {t_eval:.2f}s + operator update {t_op:.2f}s"
+                       f" + {it:.1f} x (V-cycle {t_vc:.2f}s + operator {t_ap:.2f}s, the GPU's mean PCG"
+                       f" iterations) + sensitivity/filter/adjoint {t_post:.2f}s; MMA step not included")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--precision", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64-compare", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        sys.exit(2)
+    from topoptspline3d_amd.backend import hip_api
+    t0 = time.perf_counter()
+    wl = make_workload(args.workload)
+    log(f"[bench] workload {wl['name']} generated in {time.perf_counter() - t0:.1f}s {wl['meta'].get('fill', '')}")
+    api = hip_api()
+    loop = Loop(api, wl, args.precision)
+    info = loop.ctx.info()
+    nv = active_nodes(wl["solid"])
+    log(f"[bench] levels {info['n_levels']} dims {info['level_dims']} coarse dofs {info['coarse_dofs']}")
+    for i in range(args.warmup):
+        t = time.perf_counter(); loop.step()
+        log(f"[bench] warmup {i}: {1e3 * (time.perf_counter() - t):.1f} ms, pcg its {loop.its[-1]}")
+    sync()
+    kernel_stats(api, loop.ctx, True)
+    n0 = len(loop.its)
+    t = time.perf_counter()
+    for i in range(args.steps):
+        loop.step()
+        log(f"[bench] step {i}: pcg its {loop.its[-1]} rel_res {loop.res[-1]:.2e} c {loop.c[-1]:.6e}")
+    sync()
+    elapsed = time.perf_counter() - t
+    stats = kernel_stats(api, loop.ctx, False)
+    ms = elapsed * 1e3 / args.steps
+    its = loop.its[n0:]
+    line = {
+        "metric": METRIC, "value": round(ms, 3), "unit": "ms/TO-iter", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 V-cycle / f64 PCG" if args.precision else "f64",
+        "data": "synthetic",
+        "config": {"workload": wl["desc"], "grid": list(wl["solid"].shape[::-1]),
+                   "active_elements": info["n_active_elements"][0], "active_nodes": nv,
+                   "mg_levels": info["n_levels"], "rtol": wl["rtol"], "precision": args.precision,
+                   "pcg_iterations": its, "max_rel_res": max(loop.res[n0:])},
+        "roofline": roofline(stats, info, nv, args.steps),
+    }
+    if not args.no_fp64_compare:
+        # same operator, solve from zero to the same tolerance in plain FP64 and mixed
+        cmp = {}
+        for prec in (0, 1):
+            loop.ctx.set_displacement(np.zeros(3 * loop.ctx.nn))
+            st = loop.ctx.solve(rtol=wl["rtol"], maxit=400, precision=prec, warm_start=True)
+            cmp["fp64" if prec == 0 else "mixed"] = {"solve_ms": round(st["seconds"] * 1e3, 2),
+                                                     "iterations": st["iterations"][0],
+                                                     "rel_res": st["rel_res"][0]}
+        line["solve_from_zero"] = cmp
+    if not args.no_cpu_baseline:
+        t = time.perf_counter()
+        line["cpu_baseline"] = cpu_baseline(wl, its)
+        log(f"[bench] cpu baseline sample took {time.perf_counter() - t:.1f}s")
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

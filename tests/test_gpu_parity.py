@@ -196,3 +196,14 @@ def test_solver_errors_are_reported(hip):
     bad[3] = np.nan
     with pytest.raises(Exception):
         c2.set_density(bad)
+
+
+def test_spline_mma_trajectory_matches_oracle(hip):
+    """configs[1]-style run at 32x16x16: cubic spline every 4 voxels, MMA with move
+    0.06, 6 iterations: compliance per iteration within 1e-5 relative of the oracle."""
+    from topoptspline3d_amd.drivers import run_spline_mma
+    ro = run_spline_mma(oracle_api(), 32, 16, 16, iters=6, rtol=1e-8, precision=0)
+    rg = run_spline_mma(hip, 32, 16, 16, iters=6, rtol=1e-8, precision=1)
+    co, cg = np.array(ro["c"]), np.array(rg["c"])
+    assert np.all(np.abs(cg - co) <= 1e-5 * np.abs(co)), (co, cg)
+    assert rel(rg["ctx"].coeffs(), ro["ctx"].coeffs()) < 1e-5

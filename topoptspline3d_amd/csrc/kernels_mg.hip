@@ -1,39 +1,66 @@
 // kernels_mg.hip -- multigrid kernels for gfx950 (wave64).
 //
+// Launch geometry: every node kernel walks a chunk list.  A chunk is a run of at most 64
+// free nodes of one grid row (x-contiguous for the plain list, every second x for the
+// per-colour lists), so one wave covers one chunk with unit-stride (or stride-2) lanes
+// and void / eliminated parts of the box cost nothing.  int4 chunk = {padded index of the
+// first node, x0 | count << 16, y, z}.
+//
 // Fine level (matrix-free, "on-the-fly assembly"): each thread owns one node, keeps the
 // 27 neighbouring displacements in VGPRs and sums rho_e^p * Ke[loc_i][loc_j] u_j over the
 // 8 incident elements (update_residual_OTFA_WS_kernel grid/Grid.cu:2277-2333,
 // gs_relax_OTFA_WS_kernel :1700-1842).  Ke is read through a uniform pointer, so the
 // compiler serves it from the scalar cache into SGPR operands of the FMAs.
-// Coarse levels: stored 27x9 Galerkin stencil, one thread per node (gs_relax_kernel
-// :1455-1566, update_residual_kernel_1 :2127-2224).
+// Coarse levels: Galerkin 27x9 stencil stored colour-compact: slot = 64 * (index of the
+// chunk in the colour-ordered list) + lane, entry (k*9 + r*3 + c)*NS + slot, so a GS
+// colour pass reads its stencil rows fully coalesced (gs_relax_kernel :1455-1566,
+// update_residual_kernel_1 :2127-2224 use the same 27x9 layout in colour order).
 #include "tos_internal.hpp"
 
 namespace tos {
 
-__device__ __forceinline__ int64_t thread_id() {
-  return (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-}
+__device__ __forceinline__ int64_t thread_id() { return (int64_t)blockIdx.x * blockDim.x + threadIdx.x; }
 
-// neighbour offset of stencil slot k in the padded node array
-__device__ __forceinline__ int64_t noff(const LevelView& L, int k) {
-  return (int64_t)(k % 3 - 1) + (int64_t)L.NX * ((k / 3) % 3 - 1 + (int64_t)L.NY * (k / 9 - 1));
-}
-
-// ---------------------------------------------------------------------------------------
-// Fine level: y = K(w) u  (MODE 0)  or  y = f - K(w) u  (MODE 1), rows of non-free nodes 0.
-// ---------------------------------------------------------------------------------------
-template <class T>
-struct FineNode {
-  T un[27][3];
+struct NodeRef {
+  int64_t p;
+  int x, y, z, slot;
 };
 
+// node of this lane in chunk list `cl` (stride s); false for idle lanes
+__device__ __forceinline__ bool chunk_node(const int4* __restrict__ cl, int nch, int s, NodeRef& n) {
+  const int ch = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (ch >= nch) return false;
+  const int4 c = cl[ch];
+  if (lane >= (c.y >> 16)) return false;
+  n.x = (c.y & 0xffff) + s * lane;
+  n.y = c.z;
+  n.z = c.w;
+  n.p = (int64_t)c.x + (int64_t)s * lane;
+  n.slot = ch * 64 + lane;
+  return true;
+}
+
+// block (256 threads) sum written to partial[blockIdx.x + k*gridDim.x]; all threads call
+__device__ __forceinline__ void block_partial(double v, double* __restrict__ partial, int k) {
+  __shared__ double sh[4];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[(int64_t)k * gridDim.x + blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------------------
+// Fine level
+// ---------------------------------------------------------------------------------------
 template <class T>
 __device__ __forceinline__ void load_nbhd(const LevelView& L, const T* __restrict__ u, int64_t p,
                                           T (&un)[27][3]) {
 #pragma unroll
   for (int k = 0; k < 27; ++k) {
-    int64_t q = p + (int64_t)(k % 3 - 1) + (int64_t)L.NX * ((k / 3) % 3 - 1 + (int64_t)L.NY * (k / 9 - 1));
+    const int64_t q = p + (int64_t)(k % 3 - 1) + (int64_t)L.NX * ((k / 3) % 3 - 1 + (int64_t)L.NY * (k / 9 - 1));
 #pragma unroll
     for (int c = 0; c < 3; ++c) un[k][c] = u[c * L.NP + q];
   }
@@ -42,9 +69,8 @@ __device__ __forceinline__ void load_nbhd(const LevelView& L, const T* __restric
 // acc += sum_a w_a sum_j Ke[li][j] u_j ; with SELF = false the node itself (j = li) is
 // skipped and its diagonal block accumulated into D instead.
 template <class T, bool SELF>
-__device__ __forceinline__ void fine_row(const LevelView& L, const T* __restrict__ w,
-                                         const T* __restrict__ Ke, int x, int y, int z,
-                                         const T (&un)[27][3], T (&acc)[3], T (&D)[9]) {
+__device__ __forceinline__ void fine_row(const LevelView& L, const T* __restrict__ w, const T* __restrict__ Ke,
+                                         int x, int y, int z, const T (&un)[27][3], T (&acc)[3], T (&D)[9]) {
   const int64_t e0 = L.pelem(x - 1, y - 1, z - 1);
 #pragma unroll
   for (int a = 0; a < 8; ++a) {
@@ -74,68 +100,54 @@ __device__ __forceinline__ void fine_row(const LevelView& L, const T* __restrict
   }
 }
 
+// MODE 0: y = K u, optional partial sums of u.y;  MODE 1: y = f - K u, optional float
+// copy y32.  Only free nodes are touched (the others stay 0 in every vector).
 template <class T, int MODE>
-__global__ __launch_bounds__(256) void k_fine_apply(LevelView L, const uint8_t* __restrict__ fre,
+__global__ __launch_bounds__(256) void k_fine_apply(LevelView L, const int4* __restrict__ cl, int nch,
                                                     const T* __restrict__ w, const T* __restrict__ Ke,
                                                     const T* __restrict__ u, const T* __restrict__ f,
-                                                    T* __restrict__ y) {
-  const int64_t t = thread_id();
-  const int64_t n = (int64_t)L.nx * L.ny * L.nz;
-  if (t >= n) return;
-  const int x = t % L.nx, yy = (t / L.nx) % L.ny, z = t / ((int64_t)L.nx * L.ny);
-  const int64_t p = L.pnode(x, yy, z);
-  T acc[3] = {0, 0, 0};
-  if (fre[p]) {
+                                                    T* __restrict__ y, double* __restrict__ partial,
+                                                    float* __restrict__ y32) {
+  NodeRef n;
+  const bool ok = chunk_node(cl, nch, 1, n);
+  double d = 0;
+  if (ok) {
     T un[27][3];
-    load_nbhd(L, u, p, un);
-    T D[9];
-    fine_row<T, true>(L, w, Ke, x, yy, z, un, acc, D);
-  }
-  if (MODE == 0) {
-    y[p] = acc[0]; y[L.NP + p] = acc[1]; y[2 * L.NP + p] = acc[2];
-  } else {
-    const bool fr = fre[p];
-    y[p] = fr ? f[p] - acc[0] : T(0);
-    y[L.NP + p] = fr ? f[L.NP + p] - acc[1] : T(0);
-    y[2 * L.NP + p] = fr ? f[2 * L.NP + p] - acc[2] : T(0);
-  }
-}
-
-// y = K p fused with the partial sums of p.y (for PCG): partial[block]
-template <int BS>
-__device__ __forceinline__ double block_sum(double v, double* sh) {
+    load_nbhd(L, u, n.p, un);
+    T acc[3] = {0, 0, 0}, D[9];
+    fine_row<T, true>(L, w, Ke, n.x, n.y, n.z, un, acc, D);
+    if (MODE == 0) {
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  if (lane == 0) sh[wid] = v;
-  __syncthreads();
-  double s = 0;
-  if (threadIdx.x == 0) {
+      for (int c = 0; c < 3; ++c) {
+        y[c * L.NP + n.p] = acc[c];
+        d += (double)un[13][c] * (double)acc[c];
+      }
+    } else {
 #pragma unroll
-    for (int i = 0; i < BS / 64; ++i) s += sh[i];
+      for (int c = 0; c < 3; ++c) {
+        const T v = f[c * L.NP + n.p] - acc[c];
+        y[c * L.NP + n.p] = v;
+        if (y32) y32[c * L.NP + n.p] = (float)v;
+      }
+    }
   }
-  __syncthreads();
-  return s;
+  if (partial) block_partial(d, partial, 0);
 }
 
 // Gauss-Seidel colour pass on the fine level.  BACKWARD relaxes components z, y, x.
 template <class T, bool BACKWARD>
-__global__ __launch_bounds__(256) void k_fine_gs(LevelView L, const uint8_t* __restrict__ fre,
+__global__ __launch_bounds__(256) void k_fine_gs(LevelView L, const int4* __restrict__ cl, int nch,
                                                  const T* __restrict__ w, const T* __restrict__ Ke,
-                                                 T* __restrict__ u, const T* __restrict__ f, int col) {
-  const int px = col & 1, py = (col >> 1) & 1, pz = col >> 2;
-  const int cx = (L.nx - px + 1) >> 1, cy = (L.ny - py + 1) >> 1, cz = (L.nz - pz + 1) >> 1;
-  const int64_t t = thread_id();
-  if (t >= (int64_t)cx * cy * cz) return;
-  const int x = 2 * (t % cx) + px, yy = 2 * ((t / cx) % cy) + py, z = 2 * (t / ((int64_t)cx * cy)) + pz;
-  const int64_t p = L.pnode(x, yy, z);
-  if (!fre[p]) return;
+                                                 T* __restrict__ u, const T* __restrict__ f) {
+  NodeRef n;
+  if (!chunk_node(cl, nch, 2, n)) return;
   T un[27][3];
-  load_nbhd(L, u, p, un);
+  load_nbhd(L, u, n.p, un);
   T s[3] = {0, 0, 0};
   T D[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  fine_row<T, false>(L, w, Ke, x, yy, z, un, s, D);
-  T b0 = f[p] - s[0], b1 = f[L.NP + p] - s[1], b2 = f[2 * L.NP + p] - s[2];
+  fine_row<T, false>(L, w, Ke, n.x, n.y, n.z, un, s, D);
+  const int64_t p = n.p;
+  const T b0 = f[p] - s[0], b1 = f[L.NP + p] - s[1], b2 = f[2 * L.NP + p] - s[2];
   T v0 = un[13][0], v1 = un[13][1], v2 = un[13][2];
   if (!BACKWARD) {
     v0 = (b0 - D[1] * v1 - D[2] * v2) / D[0];
@@ -146,70 +158,62 @@ __global__ __launch_bounds__(256) void k_fine_gs(LevelView L, const uint8_t* __r
     v1 = (b1 - D[3] * v0 - D[5] * v2) / D[4];
     v0 = (b0 - D[1] * v1 - D[2] * v2) / D[0];
   }
-  u[p] = v0; u[L.NP + p] = v1; u[2 * L.NP + p] = v2;
+  u[p] = v0;
+  u[L.NP + p] = v1;
+  u[2 * L.NP + p] = v2;
 }
 
 // ---------------------------------------------------------------------------------------
-// Stored-stencil levels
+// Stored-stencil levels (colour-compact stencil, NS slots per entry)
 // ---------------------------------------------------------------------------------------
 template <class T, int MODE>
-__global__ __launch_bounds__(256) void k_st_apply(LevelView L, const uint8_t* __restrict__ fre,
-                                                  const T* __restrict__ st, const T* __restrict__ u,
+__global__ __launch_bounds__(256) void k_st_apply(LevelView L, const int4* __restrict__ cl, int nch,
+                                                  const T* __restrict__ st, int64_t NS, const T* __restrict__ u,
                                                   const T* __restrict__ f, T* __restrict__ y) {
-  const int64_t t = thread_id();
-  const int64_t n = (int64_t)L.nx * L.ny * L.nz;
-  if (t >= n) return;
-  const int x = t % L.nx, yy = (t / L.nx) % L.ny, z = t / ((int64_t)L.nx * L.ny);
-  const int64_t p = L.pnode(x, yy, z);
-  T acc[3] = {0, 0, 0};
-  const bool fr = fre[p];
-  if (fr) {
+  NodeRef n;
+  if (!chunk_node(cl, nch, 2, n)) return;
+  const int64_t p = n.p;
+  T a0 = 0, a1 = 0, a2 = 0;
 #pragma unroll
-    for (int k = 0; k < 27; ++k) {
-      const int64_t q = p + (int64_t)(k % 3 - 1) + (int64_t)L.NX * ((k / 3) % 3 - 1 + (int64_t)L.NY * (k / 9 - 1));
-      const T u0 = u[q], u1 = u[L.NP + q], u2 = u[2 * L.NP + q];
-      const T* s = st + (int64_t)k * 9 * L.NP + p;
-      acc[0] += s[0] * u0 + s[L.NP] * u1 + s[2 * L.NP] * u2;
-      acc[1] += s[3 * L.NP] * u0 + s[4 * L.NP] * u1 + s[5 * L.NP] * u2;
-      acc[2] += s[6 * L.NP] * u0 + s[7 * L.NP] * u1 + s[8 * L.NP] * u2;
-    }
+  for (int k = 0; k < 27; ++k) {
+    const int64_t q = p + (int64_t)(k % 3 - 1) + (int64_t)L.NX * ((k / 3) % 3 - 1 + (int64_t)L.NY * (k / 9 - 1));
+    const T u0 = u[q], u1 = u[L.NP + q], u2 = u[2 * L.NP + q];
+    const T* s = st + (int64_t)k * 9 * NS + n.slot;
+    a0 += s[0] * u0 + s[NS] * u1 + s[2 * NS] * u2;
+    a1 += s[3 * NS] * u0 + s[4 * NS] * u1 + s[5 * NS] * u2;
+    a2 += s[6 * NS] * u0 + s[7 * NS] * u1 + s[8 * NS] * u2;
   }
   if (MODE == 0) {
-    y[p] = acc[0]; y[L.NP + p] = acc[1]; y[2 * L.NP + p] = acc[2];
+    y[p] = a0; y[L.NP + p] = a1; y[2 * L.NP + p] = a2;
   } else {
-    y[p] = fr ? f[p] - acc[0] : T(0);
-    y[L.NP + p] = fr ? f[L.NP + p] - acc[1] : T(0);
-    y[2 * L.NP + p] = fr ? f[2 * L.NP + p] - acc[2] : T(0);
+    y[p] = f[p] - a0; y[L.NP + p] = f[L.NP + p] - a1; y[2 * L.NP + p] = f[2 * L.NP + p] - a2;
   }
 }
 
+// one colour: cl points at the colour's chunks, slot0 = 64 * (their offset in the list)
 template <class T, bool BACKWARD>
-__global__ __launch_bounds__(256) void k_st_gs(LevelView L, const uint8_t* __restrict__ fre,
-                                               const T* __restrict__ st, T* __restrict__ u,
-                                               const T* __restrict__ f, int col) {
-  const int px = col & 1, py = (col >> 1) & 1, pz = col >> 2;
-  const int cx = (L.nx - px + 1) >> 1, cy = (L.ny - py + 1) >> 1, cz = (L.nz - pz + 1) >> 1;
-  const int64_t t = thread_id();
-  if (t >= (int64_t)cx * cy * cz) return;
-  const int x = 2 * (t % cx) + px, yy = 2 * ((t / cx) % cy) + py, z = 2 * (t / ((int64_t)cx * cy)) + pz;
-  const int64_t p = L.pnode(x, yy, z);
-  if (!fre[p]) return;
+__global__ __launch_bounds__(256) void k_st_gs(LevelView L, const int4* __restrict__ cl, int nch, int64_t slot0,
+                                               const T* __restrict__ st, int64_t NS, T* __restrict__ u,
+                                               const T* __restrict__ f) {
+  NodeRef n;
+  if (!chunk_node(cl, nch, 2, n)) return;
+  const int64_t p = n.p, slot = slot0 + n.slot;
   T s0 = 0, s1 = 0, s2 = 0;
 #pragma unroll
   for (int k = 0; k < 27; ++k) {
     if (k == 13) continue;
     const int64_t q = p + (int64_t)(k % 3 - 1) + (int64_t)L.NX * ((k / 3) % 3 - 1 + (int64_t)L.NY * (k / 9 - 1));
     const T u0 = u[q], u1 = u[L.NP + q], u2 = u[2 * L.NP + q];
-    const T* s = st + (int64_t)k * 9 * L.NP + p;
-    s0 += s[0] * u0 + s[L.NP] * u1 + s[2 * L.NP] * u2;
-    s1 += s[3 * L.NP] * u0 + s[4 * L.NP] * u1 + s[5 * L.NP] * u2;
-    s2 += s[6 * L.NP] * u0 + s[7 * L.NP] * u1 + s[8 * L.NP] * u2;
+    const T* s = st + (int64_t)k * 9 * NS + slot;
+    s0 += s[0] * u0 + s[NS] * u1 + s[2 * NS] * u2;
+    s1 += s[3 * NS] * u0 + s[4 * NS] * u1 + s[5 * NS] * u2;
+    s2 += s[6 * NS] * u0 + s[7 * NS] * u1 + s[8 * NS] * u2;
   }
-  const T* d = st + (int64_t)13 * 9 * L.NP + p;
+  const T* d = st + (int64_t)13 * 9 * NS + slot;
   T D[9];
 #pragma unroll
-  for (int e = 0; e < 9; ++e) D[e] = d[e * L.NP];
-  T b0 = f[p] - s0, b1 = f[L.NP + p] - s1, b2 = f[2 * L.NP + p] - s2;
+  for (int e = 0; e < 9; ++e) D[e] = d[e * NS];
+  const T b0 = f[p] - s0, b1 = f[L.NP + p] - s1, b2 = f[2 * L.NP + p] - s2;
   T v0 = u[p], v1 = u[L.NP + p], v2 = u[2 * L.NP + p];
   if (!BACKWARD) {
     v0 = (b0 - D[1] * v1 - D[2] * v2) / D[0];
@@ -220,42 +224,39 @@ __global__ __launch_bounds__(256) void k_st_gs(LevelView L, const uint8_t* __res
     v1 = (b1 - D[3] * v0 - D[5] * v2) / D[4];
     v0 = (b0 - D[1] * v1 - D[2] * v2) / D[0];
   }
-  u[p] = v0; u[L.NP + p] = v1; u[2 * L.NP + p] = v2;
+  u[p] = v0;
+  u[L.NP + p] = v1;
+  u[2 * L.NP + p] = v2;
 }
 
 // ---------------------------------------------------------------------------------------
-// Transfers: trilinear P (weights 1, 1/2, 1/4, 1/8), restriction = P^T, both masked.
-// (restrict_residual_kernel grid/Grid.cu:2456-2507, prolongate_correction_kernel :2594-2632)
+// Transfers: trilinear P (weights 1, 1/2, 1/4, 1/8), restriction = P^T, both limited to
+// free nodes (restrict_residual_kernel grid/Grid.cu:2456-2507, prolongate_correction_kernel
+// :2594-2632)
 // ---------------------------------------------------------------------------------------
 template <class T>
-__global__ __launch_bounds__(256) void k_restrict(LevelView F, LevelView C, const uint8_t* __restrict__ cfre,
+__global__ __launch_bounds__(256) void k_restrict(LevelView F, LevelView C, const int4* __restrict__ clc, int nch,
                                                   const T* __restrict__ rf, T* __restrict__ fc) {
-  const int64_t t = thread_id();
-  if (t >= (int64_t)C.nx * C.ny * C.nz) return;
-  const int x = t % C.nx, y = (t / C.nx) % C.ny, z = t / ((int64_t)C.nx * C.ny);
-  const int64_t I = C.pnode(x, y, z);
+  NodeRef n;
+  if (!chunk_node(clc, nch, 1, n)) return;
   T a0 = 0, a1 = 0, a2 = 0;
-  if (cfre[I]) {
-    const int64_t c0 = F.pnode(2 * x, 2 * y, 2 * z);
+  const int64_t c0 = F.pnode(2 * n.x, 2 * n.y, 2 * n.z);
 #pragma unroll
-    for (int k = 0; k < 27; ++k) {
-      const int dx = k % 3 - 1, dy = (k / 3) % 3 - 1, dz = k / 9 - 1;
-      const T wgt = T((dx ? 0.5 : 1.0) * (dy ? 0.5 : 1.0) * (dz ? 0.5 : 1.0));
-      const int64_t a = c0 + dx + (int64_t)F.NX * (dy + (int64_t)F.NY * dz);
-      a0 += wgt * rf[a]; a1 += wgt * rf[F.NP + a]; a2 += wgt * rf[2 * F.NP + a];
-    }
+  for (int k = 0; k < 27; ++k) {
+    const int dx = k % 3 - 1, dy = (k / 3) % 3 - 1, dz = k / 9 - 1;
+    const T wgt = T((dx ? 0.5 : 1.0) * (dy ? 0.5 : 1.0) * (dz ? 0.5 : 1.0));
+    const int64_t a = c0 + dx + (int64_t)F.NX * (dy + (int64_t)F.NY * dz);
+    a0 += wgt * rf[a]; a1 += wgt * rf[F.NP + a]; a2 += wgt * rf[2 * F.NP + a];
   }
-  fc[I] = a0; fc[C.NP + I] = a1; fc[2 * C.NP + I] = a2;
+  fc[n.p] = a0; fc[C.NP + n.p] = a1; fc[2 * C.NP + n.p] = a2;
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void k_prolong_add(LevelView F, LevelView C, const uint8_t* __restrict__ ffre,
+__global__ __launch_bounds__(256) void k_prolong_add(LevelView F, LevelView C, const int4* __restrict__ clf, int nch,
                                                      const T* __restrict__ uc, T* __restrict__ uf) {
-  const int64_t t = thread_id();
-  if (t >= (int64_t)F.nx * F.ny * F.nz) return;
-  const int x = t % F.nx, y = (t / F.nx) % F.ny, z = t / ((int64_t)F.nx * F.ny);
-  const int64_t a = F.pnode(x, y, z);
-  if (!ffre[a]) return;
+  NodeRef n;
+  if (!chunk_node(clf, nch, 1, n)) return;
+  const int x = n.x, y = n.y, z = n.z;
   const int nxp = (x & 1) + 1, nyp = (y & 1) + 1, nzp = (z & 1) + 1;
   const T wx = (x & 1) ? T(0.5) : T(1), wy = (y & 1) ? T(0.5) : T(1), wz = (z & 1) ? T(0.5) : T(1);
   const int64_t I0 = C.pnode(x >> 1, y >> 1, z >> 1);
@@ -267,16 +268,32 @@ __global__ __launch_bounds__(256) void k_prolong_add(LevelView F, LevelView C, c
         const T wgt = wx * wy * wz;
         a0 += wgt * uc[I]; a1 += wgt * uc[C.NP + I]; a2 += wgt * uc[2 * C.NP + I];
       }
-  uf[a] += a0; uf[F.NP + a] += a1; uf[2 * F.NP + a] += a2;
+  uf[n.p] += a0; uf[F.NP + n.p] += a1; uf[2 * F.NP + n.p] += a2;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_zero_nodes(const int4* __restrict__ cl, int nch, int64_t NP, T* __restrict__ u) {
+  NodeRef n;
+  if (!chunk_node(cl, nch, 1, n)) return;
+  u[n.p] = T(0); u[NP + n.p] = T(0); u[2 * NP + n.p] = T(0);
+}
+
+template <class A, class B>
+__global__ __launch_bounds__(256) void k_cast_nodes(const int4* __restrict__ cl, int nch, int64_t NP,
+                                                    const A* __restrict__ a, B* __restrict__ b) {
+  NodeRef n;
+  if (!chunk_node(cl, nch, 1, n)) return;
+  b[n.p] = (B)a[n.p]; b[NP + n.p] = (B)a[NP + n.p]; b[2 * NP + n.p] = (B)a[2 * NP + n.p];
 }
 
 // ---------------------------------------------------------------------------------------
-// Galerkin coarse operators.  Thread = (coarse node I, neighbour slot k = blockIdx.y).
+// Galerkin coarse operators.  Thread = (coarse node, neighbour slot k = blockIdx.y), the
+// coarse nodes taken from the colour-ordered chunk list, output in the compact layout.
 // First coarse level: the coarse element matrices are sum over active children of
 // rho^p P_c^T Ke P_c (8 precomputed 24x24 products Mc), evaluated with the eliminated
-// rows/columns when a child touches a non-free fine node
+// rows/columns when a child touches a non-free fine node (allfree[e] = 0)
 // (restrict_stencil_nondyadic_OTFA_WS_kernel grid/Grid.cu:1112-1225 computes the same
-// triple product per coarse node directly).
+// triple product per coarse node directly from the fine densities).
 // ---------------------------------------------------------------------------------------
 __device__ __forceinline__ double pw(int c, int i, int m) {
   double w = 1;
@@ -288,50 +305,49 @@ __device__ __forceinline__ double pw(int c, int i, int m) {
   return w;
 }
 
-__global__ __launch_bounds__(256) void k_galerkin_first(LevelView F, LevelView C, const uint8_t* __restrict__ ffre,
+__global__ __launch_bounds__(256) void k_galerkin_first(LevelView F, LevelView C, const int4* __restrict__ clc, int nch,
+                                                        const uint8_t* __restrict__ ffre,
+                                                        const uint8_t* __restrict__ fallfree,
                                                         const uint8_t* __restrict__ cfre,
                                                         const double* __restrict__ w, const double* __restrict__ Ke,
-                                                        const double* __restrict__ Mc, double* __restrict__ st) {
+                                                        const double* __restrict__ Mc, double* __restrict__ st,
+                                                        int64_t NS) {
   __shared__ double sMc[8 * 576];
   for (int i = threadIdx.x; i < 8 * 576; i += blockDim.x) sMc[i] = Mc[i];
   __syncthreads();
+  NodeRef nr;
+  if (!chunk_node(clc, nch, 2, nr)) return;
   const int k = blockIdx.y;
-  const int64_t t = thread_id();
-  if (t >= (int64_t)C.nx * C.ny * C.nz) return;
-  const int X = t % C.nx, Y = (t / C.nx) % C.ny, Z = t / ((int64_t)C.nx * C.ny);
-  const int64_t I = C.pnode(X, Y, Z);
+  const int X = nr.x, Y = nr.y, Z = nr.z;
   const int dx = k % 3 - 1, dy = (k / 3) % 3 - 1, dz = k / 9 - 1;
   double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  const int64_t J = I + dx + (int64_t)C.NX * (dy + (int64_t)C.NY * dz);
-  if (cfre[I] && cfre[J]) {
-    // coarse elements E = I - 1 + bits(a) that also contain J
-    for (int a = 0; a < 8; ++a) {
+  const int64_t J = nr.p + dx + (int64_t)C.NX * (dy + (int64_t)C.NY * dz);
+  if (cfre[J]) {
+    for (int a = 0; a < 8; ++a) {  // coarse elements E = I - 1 + bits(a) that also hold J
       const int ax = a & 1, ay = (a >> 1) & 1, az = a >> 2;
-      const int nx_ = 1 + dx - ax, ny_ = 1 + dy - ay, nz_ = 1 + dz - az;  // J - E: J's local coords
-      if (nx_ < 0 || nx_ > 1 || ny_ < 0 || ny_ > 1 || nz_ < 0 || nz_ > 1) continue;
+      const int jx = 1 + dx - ax, jy = 1 + dy - ay, jz = 1 + dz - az;  // J - E
+      if (jx < 0 || jx > 1 || jy < 0 || jy > 1 || jz < 0 || jz > 1) continue;
       const int Ex = X - 1 + ax, Ey = Y - 1 + ay, Ez = Z - 1 + az;
       if (Ex < 0 || Ey < 0 || Ez < 0 || Ex >= C.ex || Ey >= C.ey || Ez >= C.ez) continue;
-      const int mI = 7 - a, n = nx_ + 2 * ny_ + 4 * nz_;
+      const int mI = 7 - a, n = jx + 2 * jy + 4 * jz;
       for (int ch = 0; ch < 8; ++ch) {
         const int fx = 2 * Ex + (ch & 1), fy = 2 * Ey + ((ch >> 1) & 1), fz = 2 * Ez + (ch >> 2);
         if (fx >= F.ex || fy >= F.ey || fz >= F.ez) continue;
-        const double we = w[F.pelem(fx, fy, fz)];
+        const int64_t fe = F.pelem(fx, fy, fz);
+        const double we = w[fe];
         if (we == 0) continue;
-        const int64_t fb = F.pnode(fx, fy, fz);
-        bool fr[8];
-        bool all = true;
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-          fr[i] = ffre[fb + (i & 1) + (int64_t)F.NX * (((i >> 1) & 1) + (int64_t)F.NY * (i >> 2))];
-          all = all && fr[i];
-        }
-        if (all) {
+        if (fallfree[fe]) {
           const double* m = sMc + ch * 576 + (3 * mI) * 24 + 3 * n;
 #pragma unroll
           for (int r = 0; r < 3; ++r)
 #pragma unroll
             for (int c = 0; c < 3; ++c) acc[r * 3 + c] += we * m[r * 24 + c];
         } else {
+          const int64_t fb = F.pnode(fx, fy, fz);
+          bool fr[8];
+#pragma unroll
+          for (int i = 0; i < 8; ++i)
+            fr[i] = ffre[fb + (i & 1) + (int64_t)F.NX * (((i >> 1) & 1) + (int64_t)F.NY * (i >> 2))];
           for (int i = 0; i < 8; ++i) {
             if (!fr[i]) continue;
             const double pi = pw(ch, i, mI);
@@ -352,26 +368,26 @@ __global__ __launch_bounds__(256) void k_galerkin_first(LevelView F, LevelView C
     }
   }
 #pragma unroll
-  for (int e = 0; e < 9; ++e) st[(int64_t)(k * 9 + e) * C.NP + I] = acc[e];
+  for (int e = 0; e < 9; ++e) st[(int64_t)(k * 9 + e) * NS + nr.slot] = acc[e];
 }
 
-// Deeper levels: K_c[I, I+d] = sum_{a near 2I} P[a,I] sum_{b} K_f[a,b] P[b,I+d], per axis
-// the admissible (fine offset of a, fine offset of b) pairs.
-__global__ __launch_bounds__(256) void k_galerkin_next(LevelView F, LevelView C, const uint8_t* __restrict__ cfre,
-                                                       const double* __restrict__ stf, double* __restrict__ stc) {
+// Deeper levels: K_c[I, I+d] = sum_{a near 2I} P[a,I] sum_{b} K_f[a,b] P[b,I+d]; per axis
+// the admissible (offset of a from 2I, stencil offset a->b) pairs, fine stencil read
+// through the fine level's slot map.
+__global__ __launch_bounds__(256) void k_galerkin_next(LevelView F, LevelView C, const int4* __restrict__ clc, int nch,
+                                                       const uint8_t* __restrict__ cfre,
+                                                       const int* __restrict__ fslot, const double* __restrict__ stf,
+                                                       int64_t NSF, double* __restrict__ stc, int64_t NSC) {
+  NodeRef nr;
+  if (!chunk_node(clc, nch, 2, nr)) return;
   const int k = blockIdx.y;
-  const int64_t t = thread_id();
-  if (t >= (int64_t)C.nx * C.ny * C.nz) return;
-  const int X = t % C.nx, Y = (t / C.nx) % C.ny, Z = t / ((int64_t)C.nx * C.ny);
-  const int64_t I = C.pnode(X, Y, Z);
   const int d[3] = {k % 3 - 1, (k / 3) % 3 - 1, k / 9 - 1};
   double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  const int64_t J = I + d[0] + (int64_t)C.NX * (d[1] + (int64_t)C.NY * d[2]);
-  if (cfre[I] && cfre[J]) {
-    // per-axis pair lists: a = 2I + da, b = 2J + db, |b - a| <= 1
+  const int64_t J = nr.p + d[0] + (int64_t)C.NX * (d[1] + (int64_t)C.NY * d[2]);
+  if (cfre[J]) {
     int pa[3][7], pk[3][7], np_[3];
     double pwt[3][7];
-    const int Ic[3] = {X, Y, Z};
+    const int Ic[3] = {nr.x, nr.y, nr.z};
     const int Fn[3] = {F.nx, F.ny, F.nz};
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax) {
@@ -389,20 +405,22 @@ __global__ __launch_bounds__(256) void k_galerkin_next(LevelView F, LevelView C,
         }
       np_[ax] = cnt;
     }
-    const int64_t A0 = F.pnode(2 * X, 2 * Y, 2 * Z);
+    const int64_t A0 = F.pnode(2 * nr.x, 2 * nr.y, 2 * nr.z);
     for (int iz = 0; iz < np_[2]; ++iz)
       for (int iy = 0; iy < np_[1]; ++iy)
         for (int ix = 0; ix < np_[0]; ++ix) {
           const int64_t A = A0 + pa[0][ix] + (int64_t)F.NX * (pa[1][iy] + (int64_t)F.NY * pa[2][iz]);
+          const int sa = fslot[A];
+          if (sa < 0) continue;  // eliminated fine node: zero row
           const int kin = (pk[0][ix] + 1) + 3 * (pk[1][iy] + 1) + 9 * (pk[2][iz] + 1);
           const double wgt = pwt[0][ix] * pwt[1][iy] * pwt[2][iz];
-          const double* s = stf + (int64_t)kin * 9 * F.NP + A;
+          const double* s = stf + (int64_t)kin * 9 * NSF + sa;
 #pragma unroll
-          for (int e = 0; e < 9; ++e) acc[e] += wgt * s[e * F.NP];
+          for (int e = 0; e < 9; ++e) acc[e] += wgt * s[e * NSF];
         }
   }
 #pragma unroll
-  for (int e = 0; e < 9; ++e) stc[(int64_t)(k * 9 + e) * C.NP + I] = acc[e];
+  for (int e = 0; e < 9; ++e) stc[(int64_t)(k * 9 + e) * NSC + nr.slot] = acc[e];
 }
 
 __global__ void k_cast_d2f(const double* __restrict__ a, float* __restrict__ b, int64_t n) {
@@ -415,8 +433,8 @@ __global__ void k_cast_d2f(const double* __restrict__ a, float* __restrict__ b, 
 // sweep (Gauss-Jordan) that leaves -A^{-1}; the V-cycle applies it as a dense matvec.
 // ---------------------------------------------------------------------------------------
 __global__ void k_coarse_assemble(LevelView L, const int64_t* __restrict__ cidx, int nnode,
-                                  const int* __restrict__ cmap_pad, const double* __restrict__ st,
-                                  double* __restrict__ A, int n) {
+                                  const int* __restrict__ cmap_pad, const int* __restrict__ slot,
+                                  const double* __restrict__ st, int64_t NS, double* __restrict__ A, int n) {
   const int64_t t = thread_id();
   if (t >= (int64_t)nnode * 27) return;
   const int pnode = t / 27, k = t % 27;
@@ -424,9 +442,9 @@ __global__ void k_coarse_assemble(LevelView L, const int64_t* __restrict__ cidx,
   const int64_t j = i + (k % 3 - 1) + (int64_t)L.NX * ((k / 3) % 3 - 1 + (int64_t)L.NY * (k / 9 - 1));
   const int q = cmap_pad[j];
   if (q < 0) return;
+  const int si = slot[i];
   for (int r = 0; r < 3; ++r)
-    for (int c = 0; c < 3; ++c)
-      A[(int64_t)(3 * pnode + r) * n + 3 * q + c] = st[(int64_t)(k * 9 + r * 3 + c) * L.NP + i];
+    for (int c = 0; c < 3; ++c) A[(int64_t)(3 * pnode + r) * n + 3 * q + c] = st[(int64_t)(k * 9 + r * 3 + c) * NS + si];
 }
 
 __global__ void k_symmetrize(double* A, int n, double* __restrict__ diag0) {
@@ -458,13 +476,13 @@ __global__ __launch_bounds__(256) void k_gj_pivot(double* __restrict__ A, int n,
       if (threadIdx.x == 0) *flag = 1;
       return;
     }
-    const double d = 1.0 / piv;
+    const double dd = 1.0 / piv;
     for (int t = threadIdx.x; t < b * b; t += blockDim.x) {
       const int i = t / b, j = t % b;
       double v = S[i][j];
-      if (i != k && j != k) v = v - S[i][k] * S[k][j] * d;
-      else if (i == k && j == k) v = -d;
-      else v = v * d;
+      if (i != k && j != k) v = v - S[i][k] * S[k][j] * dd;
+      else if (i == k && j == k) v = -dd;
+      else v = v * dd;
       T[i][j] = v;
     }
     __syncthreads();
@@ -526,8 +544,96 @@ __global__ __launch_bounds__(256) void k_coarse_solve(LevelView L, const int64_t
 }
 
 // ---------------------------------------------------------------------------------------
-// Vector kernels (FP64 PCG and casts)
+// PCG vector kernels over the fine level's chunk list (FP64, fused with their reductions)
 // ---------------------------------------------------------------------------------------
+// u += alpha p ; r -= alpha q ; partial ||r||^2 ; optional float copy of r
+__global__ __launch_bounds__(256) void k_pcg_ur(const int4* __restrict__ cl, int nch, int64_t NP,
+                                                double* __restrict__ u, double* __restrict__ r,
+                                                const double* __restrict__ p, const double* __restrict__ q,
+                                                const double* __restrict__ scal, float* __restrict__ r32,
+                                                double* __restrict__ partial) {
+  NodeRef n;
+  const bool ok = chunk_node(cl, nch, 1, n);
+  double s = 0;
+  if (ok) {
+    const double al = scal[S_ALPHA];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int64_t i = c * NP + n.p;
+      u[i] += al * p[i];
+      const double rv = r[i] - al * q[i];
+      r[i] = rv;
+      if (r32) r32[i] = (float)rv;
+      s += rv * rv;
+    }
+  }
+  block_partial(s, partial, 0);
+}
+
+// partials of z.r and z.q (z in FP32 or FP64)
+template <class TZ>
+__global__ __launch_bounds__(256) void k_pcg_zdots(const int4* __restrict__ cl, int nch, int64_t NP,
+                                                   const TZ* __restrict__ z, const double* __restrict__ r,
+                                                   const double* __restrict__ q, double* __restrict__ partial) {
+  NodeRef n;
+  const bool ok = chunk_node(cl, nch, 1, n);
+  double zr = 0, zq = 0;
+  if (ok) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int64_t i = c * NP + n.p;
+      const double zv = (double)z[i];
+      zr += zv * r[i];
+      if (q) zq += zv * q[i];
+    }
+  }
+  block_partial(zr, partial, 0);
+  block_partial(zq, partial, 1);
+}
+
+// p = z + beta p  (first != 0: p = z)
+template <class TZ>
+__global__ __launch_bounds__(256) void k_pcg_p(const int4* __restrict__ cl, int nch, int64_t NP,
+                                               double* __restrict__ p, const TZ* __restrict__ z,
+                                               const double* __restrict__ scal, int first) {
+  NodeRef n;
+  if (!chunk_node(cl, nch, 1, n)) return;
+  const double be = first ? 0.0 : scal[S_BETA];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const int64_t i = c * NP + n.p;
+    p[i] = first ? (double)z[i] : (double)z[i] + be * p[i];
+  }
+}
+
+// generic partials of a.b (and a2.b2 when a2 != null)
+__global__ __launch_bounds__(256) void k_dot_nodes(const int4* __restrict__ cl, int nch, int64_t NP,
+                                                   const double* __restrict__ a, const double* __restrict__ b,
+                                                   const double* __restrict__ a2, const double* __restrict__ b2,
+                                                   double* __restrict__ partial) {
+  NodeRef n;
+  const bool ok = chunk_node(cl, nch, 1, n);
+  double s0 = 0, s1 = 0;
+  if (ok) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int64_t i = c * NP + n.p;
+      s0 += a[i] * b[i];
+      if (a2) s1 += a2[i] * b2[i];
+    }
+  }
+  block_partial(s0, partial, 0);
+  block_partial(s1, partial, 1);
+}
+
+// u += z (plain V-cycle iteration)
+__global__ __launch_bounds__(256) void k_add_nodes(const int4* __restrict__ cl, int nch, int64_t NP,
+                                                   double* __restrict__ u, const double* __restrict__ z) {
+  NodeRef n;
+  if (!chunk_node(cl, nch, 1, n)) return;
+  u[n.p] += z[n.p]; u[NP + n.p] += z[NP + n.p]; u[2 * NP + n.p] += z[2 * NP + n.p];
+}
+
 template <class A, class B>
 __global__ void k_copy_cast(const A* __restrict__ a, B* __restrict__ b, int64_t n) {
   const int64_t t = thread_id();

@@ -44,23 +44,6 @@ __device__ __forceinline__ double block_max256(double v) {
   return s;
 }
 
-// partial[b] = sum over the grid-stride slice of block b of a[i]*b[i] (up to 3 pairs)
-__global__ __launch_bounds__(256) void k_dot_partial(const double* __restrict__ a0, const double* __restrict__ b0,
-                                                     const double* __restrict__ a1, const double* __restrict__ b1,
-                                                     int64_t n, double* __restrict__ partial) {
-  double s0 = 0, s1 = 0;
-  for (int64_t i = tid64(); i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    s0 += a0[i] * b0[i];
-    if (a1) s1 += a1[i] * b1[i];
-  }
-  s0 = block_sum256(s0);
-  s1 = block_sum256(s1);
-  if (threadIdx.x == 0) {
-    partial[blockIdx.x] = s0;
-    partial[gridDim.x + blockIdx.x] = s1;
-  }
-}
-
 // Sum `np` partials (two interleaved sets when two != 0) into scal[slot0], scal[slot1],
 // then apply a small scalar program `op`.
 enum { OP_NONE = 0, OP_ALPHA, OP_BETA, OP_RN2 };
@@ -82,42 +65,6 @@ __global__ __launch_bounds__(256) void k_finish(const double* __restrict__ parti
       scal[S_RZ] = scal[S_ZR];
     }
   }
-}
-
-// u += alpha p ; r -= alpha q ; partial ||r||^2
-__global__ __launch_bounds__(256) void k_pcg_ur(double* __restrict__ u, double* __restrict__ r,
-                                                const double* __restrict__ p, const double* __restrict__ q,
-                                                const double* __restrict__ scal, int64_t n, double* __restrict__ partial) {
-  const double al = scal[S_ALPHA];
-  double s = 0;
-  for (int64_t i = tid64(); i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    u[i] += al * p[i];
-    const double rv = r[i] - al * q[i];
-    r[i] = rv;
-    s += rv * rv;
-  }
-  s = block_sum256(s);
-  if (threadIdx.x == 0) partial[blockIdx.x] = s;
-}
-
-// p = z + beta p
-__global__ void k_pcg_p(double* __restrict__ p, const double* __restrict__ z, const double* __restrict__ scal,
-                        int64_t n) {
-  const double be = scal[S_BETA];
-  const int64_t i = tid64();
-  if (i < n) p[i] = z[i] + be * p[i];
-}
-
-// y = a*x + y (generic)
-__global__ void k_axpy(double a, const double* __restrict__ x, double* __restrict__ y, int64_t n) {
-  const int64_t i = tid64();
-  if (i < n) y[i] += a * x[i];
-}
-
-__global__ void k_sub(const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ c,
-                      int64_t n) {
-  const int64_t i = tid64();
-  if (i < n) c[i] = a[i] - b[i];
 }
 
 // ---------------------------------------------------------------------------------------

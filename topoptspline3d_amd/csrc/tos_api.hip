@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <stdexcept>
 
 #include "tos_internal.hpp"
@@ -25,10 +26,18 @@ struct tos_ctx : tos::Ctx {};
 struct tos_mma {
   int dev = 0;
   hipStream_t stream = nullptr;
-  MmaDev d{};
+  MmaVecs V{};
+  int nb = 1;                 // blocks of the n-length passes
+  double* partial = nullptr;  // MMA_CNT x nb
+  double* red = nullptr;      // folded columns (device)
+  double* h_red = nullptr;    // pinned host mirror
+  double* stage = nullptr;    // df / dg staging
   std::vector<void*> allocs;
-  double* h_out = nullptr;  // pinned [3]
-  double* stage = nullptr;  // device staging for df/dg/g
+  // scalar state of the subproblem (host)
+  double y[MMAX], lam[MMAX], mu[MMAX], s[MMAX], a[MMAX], c[MMAX], d[MMAX], b[MMAX];
+  double z = 1, zeta = 1, eps = 1, move = 0.5;
+  int stop_counter = 0;
+  int last_newton = 0;
 };
 
 namespace {
@@ -251,16 +260,65 @@ void* stage(Ctx& c, size_t bytes) {
 }
 
 // ---------------------------------------------------------------------------------------
-// Reductions (results stay on the device in c.scal)
+// Chunk lists (see kernels_mg.hip): runs of free nodes along x, at most 64 per chunk
 // ---------------------------------------------------------------------------------------
-void dot_to(Ctx& c, const double* a, const double* b, const double* a2, const double* b2, int64_t n, int s0, int s1,
-            int op) {
-  hipLaunchKernelGGL(k_dot_partial, dim3(kRedBlocks), dim3(256), 0, c.stream, a, b, a2, b2, n, c.partial);
-  hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, c.stream, (const double*)c.partial, kRedBlocks, a2 ? 1 : 0, s0,
-                     s1, op, c.scal);
-  LAUNCH_CHECK();
+void build_chunks(Ctx& c, Level& L, bool with_slots) {
+  const LevelView& v = L.v;
+  auto freeat = [&](int x, int y, int z) { return L.h_free[x + (size_t)v.nx * (y + (size_t)v.ny * z)] != 0; };
+  auto emit = [&](std::vector<int4>& out, int x0, int y, int z, int cnt) {
+    out.push_back(make_int4((int)v.pnode(x0, y, z), x0 | (cnt << 16), y, z));
+  };
+  std::vector<int4> lin, col;
+  for (int z = 0; z < v.nz; ++z)
+    for (int y = 0; y < v.ny; ++y) {
+      int x = 0;
+      while (x < v.nx) {
+        if (!freeat(x, y, z)) { ++x; continue; }
+        int x1 = x;
+        while (x1 < v.nx && freeat(x1, y, z) && x1 - x < 64) ++x1;
+        emit(lin, x, y, z, x1 - x);
+        x = x1;
+      }
+    }
+  for (int cidx = 0; cidx < 8; ++cidx) {
+    L.col_off[cidx] = (int)col.size();
+    const int px = cidx & 1, py = (cidx >> 1) & 1, pz = cidx >> 2;
+    for (int z = pz; z < v.nz; z += 2)
+      for (int y = py; y < v.ny; y += 2) {
+        int x = px;
+        while (x < v.nx) {
+          if (!freeat(x, y, z)) { x += 2; continue; }
+          int x1 = x, cnt = 0;
+          while (x1 < v.nx && freeat(x1, y, z) && cnt < 64) { x1 += 2; ++cnt; }
+          emit(col, x, y, z, cnt);
+          x = x1;
+        }
+      }
+  }
+  L.col_off[8] = (int)col.size();
+  L.n_lin = (int)lin.size();
+  L.n_col = (int)col.size();
+  L.cl_lin = dalloc<int4>(c, std::max<size_t>(lin.size(), 1));
+  L.cl_col = dalloc<int4>(c, std::max<size_t>(col.size(), 1));
+  if (!lin.empty()) upload(c.stream, L.cl_lin, lin.data(), lin.size() * sizeof(int4));
+  if (!col.empty()) upload(c.stream, L.cl_col, col.data(), col.size() * sizeof(int4));
+  L.NS = 64 * (int64_t)L.n_col;
+  if (with_slots) {
+    L.h_slot.assign(v.NP, -1);
+    for (size_t g = 0; g < col.size(); ++g) {
+      const int cnt = col[g].y >> 16;
+      for (int l = 0; l < cnt; ++l) L.h_slot[col[g].x + 2 * l] = (int)(g * 64 + l);
+    }
+    L.slot = dalloc<int>(c, v.NP);
+    upload(c.stream, L.slot, L.h_slot.data(), v.NP * sizeof(int));
+  }
 }
 
+inline unsigned cblk(int nch) { return (unsigned)std::max(1, (nch + 3) / 4); }
+
+// ---------------------------------------------------------------------------------------
+// Reductions (results stay on the device in c.scal)
+// ---------------------------------------------------------------------------------------
 const double* read_scal(Ctx& c) {
   download(c.stream, c.h_scal, c.scal, sizeof(double) * S_COUNT);
   return c.h_scal;
@@ -307,6 +365,22 @@ struct Prof {
   }
 };
 
+// finisher over the partials of a launch over the fine chunk list
+void finish0(Ctx& c, int two, int s0, int s1, int op) {
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, c.stream, (const double*)c.partial, c.nb0, two, s0, s1, op,
+                     c.scal);
+}
+
+// scal[s0] = a.b (and scal[s1] = a2.b2) over the free fine nodes
+void dot_to(Ctx& c, const double* a, const double* b, const double* a2, const double* b2, int s0, int s1, int op) {
+  Prof pr(c, P_VECTOR);
+  Level& L0 = c.L[0];
+  hipLaunchKernelGGL(k_dot_nodes, dim3(c.nb0), dim3(256), 0, c.stream, (const int4*)L0.cl_lin, L0.n_lin, L0.v.NP, a,
+                     b, a2, b2, c.partial);
+  finish0(c, a2 ? 1 : 0, s0, s1, op);
+  LAUNCH_CHECK();
+}
+
 // ---------------------------------------------------------------------------------------
 // Multigrid (V-cycle, HierarchyGrid::v_cycle grid/Grid.cpp:2399-2444)
 // ---------------------------------------------------------------------------------------
@@ -337,41 +411,44 @@ void gs_sweep(Ctx& c, int l, T* u, const T* f, bool backward) {
   const LevelView& v = L.v;
   for (int ci = 0; ci < 8; ++ci) {
     const int col = backward ? 7 - ci : ci;
-    const int px = col & 1, py = (col >> 1) & 1, pz = col >> 2;
-    const int64_t cnt = (int64_t)((v.nx - px + 1) >> 1) * ((v.ny - py + 1) >> 1) * ((v.nz - pz + 1) >> 1);
-    if (cnt == 0) continue;
+    const int off = L.col_off[col], nch = L.col_off[col + 1] - off;
+    if (nch == 0) continue;
+    const int4* cl = L.cl_col + off;
     Prof pr(c, l == 0 ? (sizeof(T) == 8 ? P_FINE_GS64 : P_FINE_GS32) : (sizeof(T) == 8 ? P_ST_GS64 : P_ST_GS32));
     if (l == 0) {
       if (backward)
-        hipLaunchKernelGGL((k_fine_gs<T, true>), dim3(nblk(cnt)), dim3(256), 0, c.stream, v, (const uint8_t*)L.fre,
-                           Prec<T>::w(c), Prec<T>::Ke(c), u, f, col);
+        hipLaunchKernelGGL((k_fine_gs<T, true>), dim3(cblk(nch)), dim3(256), 0, c.stream, v, cl, nch, Prec<T>::w(c),
+                           Prec<T>::Ke(c), u, f);
       else
-        hipLaunchKernelGGL((k_fine_gs<T, false>), dim3(nblk(cnt)), dim3(256), 0, c.stream, v, (const uint8_t*)L.fre,
-                           Prec<T>::w(c), Prec<T>::Ke(c), u, f, col);
+        hipLaunchKernelGGL((k_fine_gs<T, false>), dim3(cblk(nch)), dim3(256), 0, c.stream, v, cl, nch, Prec<T>::w(c),
+                           Prec<T>::Ke(c), u, f);
     } else {
       if (backward)
-        hipLaunchKernelGGL((k_st_gs<T, true>), dim3(nblk(cnt)), dim3(256), 0, c.stream, v, (const uint8_t*)L.fre,
-                           Prec<T>::st(L), u, f, col);
+        hipLaunchKernelGGL((k_st_gs<T, true>), dim3(cblk(nch)), dim3(256), 0, c.stream, v, cl, nch, 64 * (int64_t)off,
+                           Prec<T>::st(L), L.NS, u, f);
       else
-        hipLaunchKernelGGL((k_st_gs<T, false>), dim3(nblk(cnt)), dim3(256), 0, c.stream, v, (const uint8_t*)L.fre,
-                           Prec<T>::st(L), u, f, col);
+        hipLaunchKernelGGL((k_st_gs<T, false>), dim3(cblk(nch)), dim3(256), 0, c.stream, v, cl, nch,
+                           64 * (int64_t)off, Prec<T>::st(L), L.NS, u, f);
     }
   }
   LAUNCH_CHECK();
 }
 
+// level l: MODE 0 y = K u (fine level: partial sums of u.y when `partial`), MODE 1
+// y = f - K u (fine level: float copy y32 when given)
 template <class T, int MODE>
-void apply_level(Ctx& c, int l, const T* u, const T* f, T* y) {
+void apply_level(Ctx& c, int l, const T* u, const T* f, T* y, double* partial = nullptr, float* y32 = nullptr) {
   Level& L = c.L[l];
-  const int64_t n = (int64_t)L.v.nx * L.v.ny * L.v.nz;
   Prof pr(c, l > 0 ? P_ST_RESID
                    : (MODE == 0 ? P_FINE_APPLY64 : (sizeof(T) == 8 ? P_FINE_RESID64 : P_FINE_RESID32)));
-  if (l == 0)
-    hipLaunchKernelGGL((k_fine_apply<T, MODE>), dim3(nblk(n)), dim3(256), 0, c.stream, L.v, (const uint8_t*)L.fre,
-                       Prec<T>::w(c), Prec<T>::Ke(c), u, f, y);
-  else
-    hipLaunchKernelGGL((k_st_apply<T, MODE>), dim3(nblk(n)), dim3(256), 0, c.stream, L.v, (const uint8_t*)L.fre,
-                       Prec<T>::st(L), u, f, y);
+  if (l == 0) {
+    if (L.n_lin)
+      hipLaunchKernelGGL((k_fine_apply<T, MODE>), dim3(cblk(L.n_lin)), dim3(256), 0, c.stream, L.v,
+                         (const int4*)L.cl_lin, L.n_lin, Prec<T>::w(c), Prec<T>::Ke(c), u, f, y, partial, y32);
+  } else if (L.n_col) {
+    hipLaunchKernelGGL((k_st_apply<T, MODE>), dim3(cblk(L.n_col)), dim3(256), 0, c.stream, L.v,
+                       (const int4*)L.cl_col, L.n_col, Prec<T>::st(L), L.NS, u, f, y);
+  }
   LAUNCH_CHECK();
 }
 
@@ -379,7 +456,6 @@ template <class T>
 void vcycle_level(Ctx& c, int l, const T* f, T* u, int pre, int post) {
   Level& L = c.L[l];
   const int last = (int)c.L.size() - 1;
-  const int64_t n3 = 3 * L.v.NP;
   if (l == last && last > 0) {
     if (c.cn == 0) return;  // no unknowns on the coarsest level (u stays 0)
     Prof pr(c, P_COARSE);
@@ -388,7 +464,9 @@ void vcycle_level(Ctx& c, int l, const T* f, T* u, int pre, int post) {
     LAUNCH_CHECK();
     return;
   }
-  hipLaunchKernelGGL((k_zero<T>), dim3(nblk(n3)), dim3(256), 0, c.stream, u, n3);
+  if (L.n_lin == 0) return;
+  hipLaunchKernelGGL((k_zero_nodes<T>), dim3(cblk(L.n_lin)), dim3(256), 0, c.stream, (const int4*)L.cl_lin, L.n_lin,
+                     L.v.NP, u);
   if (l == last) {  // a single level: smoothing only
     for (int s = 0; s < std::max(1, pre); ++s) gs_sweep<T>(c, l, u, f, false);
     for (int s = 0; s < std::max(1, post); ++s) gs_sweep<T>(c, l, u, f, true);
@@ -398,36 +476,38 @@ void vcycle_level(Ctx& c, int l, const T* f, T* u, int pre, int post) {
   T* r = Prec<T>::r(L);
   apply_level<T, 1>(c, l, u, f, r);
   Level& C = c.L[l + 1];
-  const int64_t nc = (int64_t)C.v.nx * C.v.ny * C.v.nz;
-  {
+  if (C.n_lin) {
     Prof pr(c, P_TRANSFER);
-    hipLaunchKernelGGL((k_restrict<T>), dim3(nblk(nc)), dim3(256), 0, c.stream, L.v, C.v, (const uint8_t*)C.fre,
-                       (const T*)r, Prec<T>::f(C));
+    hipLaunchKernelGGL((k_restrict<T>), dim3(cblk(C.n_lin)), dim3(256), 0, c.stream, L.v, C.v,
+                       (const int4*)C.cl_lin, C.n_lin, (const T*)r, Prec<T>::f(C));
   }
   vcycle_level<T>(c, l + 1, Prec<T>::f(C), Prec<T>::u(C), pre, post);
-  const int64_t nf = (int64_t)L.v.nx * L.v.ny * L.v.nz;
   {
     Prof pr(c, P_TRANSFER);
-    hipLaunchKernelGGL((k_prolong_add<T>), dim3(nblk(nf)), dim3(256), 0, c.stream, L.v, C.v, (const uint8_t*)L.fre,
-                       (const T*)Prec<T>::u(C), u);
+    hipLaunchKernelGGL((k_prolong_add<T>), dim3(cblk(L.n_lin)), dim3(256), 0, c.stream, L.v, C.v,
+                       (const int4*)L.cl_lin, L.n_lin, (const T*)Prec<T>::u(C), u);
   }
   LAUNCH_CHECK();
   for (int s = 0; s < post; ++s) gs_sweep<T>(c, l, u, f, true);
 }
 
-// z = B r with FP64 input/output; `precision` selects the arithmetic of the cycle
-void precondition(Ctx& c, const double* r, double* z, int precision, int pre, int post) {
+// z = B r.  precision 0: FP64 cycle from c.r into c.z.  precision 1: FP32 cycle from
+// L0.f32 (a float copy of r, written here unless `have_r32`) into L0.u32; with
+// `want_z64` the result is also cast into c.z.
+void precondition(Ctx& c, int precision, int pre, int post, bool have_r32, bool want_z64) {
   Level& L0 = c.L[0];
-  const int64_t n3 = 3 * L0.v.NP;
   if (precision == 0) {
-    vcycle_level<double>(c, 0, r, z, pre, post);
-  } else {
-    hipLaunchKernelGGL((k_copy_cast<double, float>), dim3(nblk(n3)), dim3(256), 0, c.stream, r, L0.f32, n3);
-    vcycle_level<float>(c, 0, L0.f32, L0.u32, pre, post);
-    hipLaunchKernelGGL((k_copy_cast<float, double>), dim3(nblk(n3)), dim3(256), 0, c.stream, (const float*)L0.u32, z,
-                       n3);
-    LAUNCH_CHECK();
+    vcycle_level<double>(c, 0, c.r, c.z, pre, post);
+    return;
   }
+  if (!have_r32)
+    hipLaunchKernelGGL((k_cast_nodes<double, float>), dim3(cblk(L0.n_lin)), dim3(256), 0, c.stream,
+                       (const int4*)L0.cl_lin, L0.n_lin, L0.v.NP, (const double*)c.r, L0.f32);
+  vcycle_level<float>(c, 0, L0.f32, L0.u32, pre, post);
+  if (want_z64)
+    hipLaunchKernelGGL((k_cast_nodes<float, double>), dim3(cblk(L0.n_lin)), dim3(256), 0, c.stream,
+                       (const int4*)L0.cl_lin, L0.n_lin, L0.v.NP, (const float*)L0.u32, c.z);
+  LAUNCH_CHECK();
 }
 
 // ---------------------------------------------------------------------------------------
@@ -448,7 +528,7 @@ void coarse_alloc(Ctx& c) {
         }
   c.cn = 3 * (int)nodes.size();
   c.cidx = dalloc<int64_t>(c, nodes.size());
-  upload(c.stream, c.cidx, nodes.data(), nodes.size() * sizeof(int64_t));
+  if (!nodes.empty()) upload(c.stream, c.cidx, nodes.data(), nodes.size() * sizeof(int64_t));
   c.cmap = dalloc<int>(c, cmap.size());
   upload(c.stream, c.cmap, cmap.data(), cmap.size() * sizeof(int));
   c.cinv = dalloc<double>(c, (size_t)c.cn * c.cn);
@@ -463,7 +543,8 @@ int coarse_factor(Ctx& c) {
   const int n = c.cn, nnode = n / 3;
   hipLaunchKernelGGL(k_zero<double>, dim3(nblk((int64_t)n * n)), dim3(256), 0, c.stream, c.cinv, (int64_t)n * n);
   hipLaunchKernelGGL(k_coarse_assemble, dim3(nblk((int64_t)nnode * 27)), dim3(256), 0, c.stream, L.v,
-                     (const int64_t*)c.cidx, nnode, (const int*)c.cmap, (const double*)L.st64, c.cinv, n);
+                     (const int64_t*)c.cidx, nnode, (const int*)c.cmap, (const int*)L.slot, (const double*)L.st64,
+                     L.NS, c.cinv, n);
   double* Dinv = c.cwork;
   double* Cp = Dinv + kGJB * kGJB;
   double* R = Cp + (size_t)n * kGJB;
@@ -494,18 +575,23 @@ int coarse_factor(Ctx& c) {
 // ---------------------------------------------------------------------------------------
 int solve_case(Ctx& c, int k, const tos_solve_opts& o, int* its, double* relres) {
   Level& L0 = c.L[0];
-  const int64_t n3 = 3 * L0.v.NP;
+  const int64_t NP = L0.v.NP, n3 = 3 * NP;
   double* u = c.U + (size_t)k * n3;
   const double* f = c.F + (size_t)k * n3;
-  if (!o.warm_start) hipLaunchKernelGGL(k_zero<double>, dim3(nblk(n3)), dim3(256), 0, c.stream, u, n3);
-  apply_level<double, 1>(c, 0, u, f, c.r);  // r = f - K u
-  dot_to(c, f, f, c.r, c.r, n3, S_NF2, S_RN2, OP_NONE);
+  const bool f32 = o.precision != 0;
+  const unsigned nb = c.nb0;
+  const int4* cl = L0.cl_lin;
+  const int nch = L0.n_lin;
+  if (!o.warm_start && nch)
+    hipLaunchKernelGGL(k_zero_nodes<double>, dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, u);
+  apply_level<double, 1>(c, 0, u, f, c.r, nullptr, f32 ? L0.f32 : nullptr);  // r = f - K u
+  dot_to(c, f, f, c.r, c.r, S_NF2, S_RN2, OP_NONE);
   const double* hs = read_scal(c);
   const double nf = std::sqrt(hs[S_NF2]);
   double rn = std::sqrt(hs[S_RN2]);
   *its = 0;
   if (nf == 0) {
-    hipLaunchKernelGGL(k_zero<double>, dim3(nblk(n3)), dim3(256), 0, c.stream, u, n3);
+    if (nch) hipLaunchKernelGGL(k_zero_nodes<double>, dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, u);
     *relres = 0;
     return TOS_OK;
   }
@@ -516,40 +602,69 @@ int solve_case(Ctx& c, int k, const tos_solve_opts& o, int* its, double* relres)
   int it = 0;
   if (!o.use_pcg) {
     while (rn / nf > o.rtol && it < o.maxit) {
-      precondition(c, c.r, c.z, o.precision, o.pre, o.post);
-      hipLaunchKernelGGL(k_axpy, dim3(nblk(n3)), dim3(256), 0, c.stream, 1.0, (const double*)c.z, u, n3);
-      apply_level<double, 1>(c, 0, u, f, c.r);
-      dot_to(c, c.r, c.r, nullptr, nullptr, n3, S_RN2, 0, OP_NONE);
+      precondition(c, o.precision, o.pre, o.post, f32, true);
+      hipLaunchKernelGGL(k_add_nodes, dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, u, (const double*)c.z);
+      apply_level<double, 1>(c, 0, u, f, c.r, nullptr, f32 ? L0.f32 : nullptr);
+      dot_to(c, c.r, c.r, nullptr, nullptr, S_RN2, 0, OP_NONE);
       rn = std::sqrt(read_scal(c)[S_RN2]);
       ++it;
       if (!std::isfinite(rn)) { c.err = "NaN/Inf in the V-cycle iteration"; return TOS_E_NAN; }
     }
   } else if (rn / nf > o.rtol) {
-    precondition(c, c.r, c.z, o.precision, o.pre, o.post);
-    HIPCHK(hipMemcpyAsync(c.p, c.z, n3 * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
-    dot_to(c, c.r, c.z, nullptr, nullptr, n3, S_RZ, 0, OP_NONE);
+    // flexible CG, Polak-Ribiere beta = -alpha <z, q> / <z_old, r_old>
+    precondition(c, o.precision, o.pre, o.post, f32, false);
+    {
+      Prof pr(c, P_VECTOR);
+      if (f32) {
+        hipLaunchKernelGGL((k_pcg_p<float>), dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, c.p,
+                           (const float*)L0.u32, (const double*)c.scal, 1);
+        hipLaunchKernelGGL((k_pcg_zdots<float>), dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, (const float*)L0.u32,
+                           (const double*)c.r, (const double*)nullptr, c.partial);
+      } else {
+        hipLaunchKernelGGL((k_pcg_p<double>), dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, c.p,
+                           (const double*)c.z, (const double*)c.scal, 1);
+        hipLaunchKernelGGL((k_pcg_zdots<double>), dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, (const double*)c.z,
+                           (const double*)c.r, (const double*)nullptr, c.partial);
+      }
+      finish0(c, 0, S_RZ, 0, OP_NONE);
+    }
     while (it < o.maxit) {
-      apply_level<double, 0>(c, 0, c.p, nullptr, c.q);
-      dot_to(c, c.p, c.q, nullptr, nullptr, n3, S_PQ, 0, OP_ALPHA);
-      hipLaunchKernelGGL(k_pcg_ur, dim3(kRedBlocks), dim3(256), 0, c.stream, u, c.r, (const double*)c.p,
-                         (const double*)c.q, (const double*)c.scal, n3, c.partial);
-      hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, c.stream, (const double*)c.partial, kRedBlocks, 0, S_RN2, 0,
-                         OP_NONE, c.scal);
+      apply_level<double, 0>(c, 0, c.p, nullptr, c.q, c.partial);  // q = K p, partials of p.q
+      {
+        Prof pr(c, P_VECTOR);
+        finish0(c, 0, S_PQ, 0, OP_ALPHA);
+        hipLaunchKernelGGL(k_pcg_ur, dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, u, c.r, (const double*)c.p,
+                           (const double*)c.q, (const double*)c.scal, f32 ? L0.f32 : (float*)nullptr, c.partial);
+        finish0(c, 0, S_RN2, 0, OP_NONE);
+      }
       LAUNCH_CHECK();
       ++it;
       rn = std::sqrt(read_scal(c)[S_RN2]);
       if (!std::isfinite(rn)) { c.err = "NaN/Inf in PCG"; return TOS_E_NAN; }
       if (rn / nf <= o.rtol) break;
-      precondition(c, c.r, c.z, o.precision, o.pre, o.post);
-      dot_to(c, c.z, c.r, c.z, c.q, n3, S_ZR, S_ZQ, OP_BETA);
-      hipLaunchKernelGGL(k_pcg_p, dim3(nblk(n3)), dim3(256), 0, c.stream, c.p, (const double*)c.z,
-                         (const double*)c.scal, n3);
+      precondition(c, o.precision, o.pre, o.post, f32, false);
+      {
+        Prof pr(c, P_VECTOR);
+        if (f32) {
+          hipLaunchKernelGGL((k_pcg_zdots<float>), dim3(nb), dim3(256), 0, c.stream, cl, nch, NP,
+                             (const float*)L0.u32, (const double*)c.r, (const double*)c.q, c.partial);
+          finish0(c, 1, S_ZR, S_ZQ, OP_BETA);
+          hipLaunchKernelGGL((k_pcg_p<float>), dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, c.p,
+                             (const float*)L0.u32, (const double*)c.scal, 0);
+        } else {
+          hipLaunchKernelGGL((k_pcg_zdots<double>), dim3(nb), dim3(256), 0, c.stream, cl, nch, NP,
+                             (const double*)c.z, (const double*)c.r, (const double*)c.q, c.partial);
+          finish0(c, 1, S_ZR, S_ZQ, OP_BETA);
+          hipLaunchKernelGGL((k_pcg_p<double>), dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, c.p,
+                             (const double*)c.z, (const double*)c.scal, 0);
+        }
+      }
       LAUNCH_CHECK();
     }
   }
   // true FP64 residual
   apply_level<double, 1>(c, 0, u, f, c.r);
-  dot_to(c, c.r, c.r, nullptr, nullptr, n3, S_RN2, 0, OP_NONE);
+  dot_to(c, c.r, c.r, nullptr, nullptr, S_RN2, 0, OP_NONE);
   *relres = std::sqrt(read_scal(c)[S_RN2]) / nf;
   *its = it;
   return *relres <= o.rtol ? TOS_OK : TOS_E_NOCONV;
@@ -653,21 +768,178 @@ AxisTable spline_table(int mode, int order, int param, int ne, double h) {
 }
 
 // ---------------------------------------------------------------------------------------
-// MMA handle
+// MMA: host control of the primal-dual subproblem (mma_subproblem_t::solve mma_t.cu:330-499)
+// over the fused device passes of mma.hip.
 // ---------------------------------------------------------------------------------------
 template <class T>
 T* malloc_mma(tos_mma* M, size_t n) {
   return (T*)raw_alloc(M->allocs, M->stream, n * sizeof(T));
 }
 
-void mma_run(tos_mma* M, const double* df, const double* dg, const double* g, int* converged) {
-  M->d.df = df;
-  M->d.dg = dg;
-  M->d.g = g;
-  hipLaunchKernelGGL(k_mma_update, dim3(1), dim3(MT), 0, M->stream, M->d);
+// run a pass kernel, fold its `cnt` columns, read them to M->h_red
+template <class K, class... A>
+const double* mma_pass(tos_mma* M, hipStream_t st, K kernel, int cnt, unsigned maxmask, A... args) {
+  hipLaunchKernelGGL(kernel, dim3(M->nb), dim3(256), 0, st, args...);
+  hipLaunchKernelGGL(k_mma_fold, dim3(1), dim3(256), 0, st, (const double*)M->partial, M->nb, cnt, maxmask, M->red);
   LAUNCH_CHECK();
-  download(M->stream, M->h_out, M->d.out, 3 * sizeof(double));
-  if (converged) *converged = M->h_out[0] != 0.0;
+  download(st, M->h_red, M->red, cnt * sizeof(double));
+  return M->h_red;
+}
+
+// KKT residual (2-norm, inf-norm) from the folded n-length parts and the scalars
+// (kkt_err mma_t.cu:574-678, same order of accumulation)
+std::pair<double, double> mma_kkt_host(const tos_mma* M, const double* v, const double* y, const double* lam,
+                                       const double* mu, const double* s, double z, double zeta, double damp) {
+  const int m = M->V.m;
+  double sum = v[0], mx = std::max(-10.0, v[3 + MMAX]);
+  auto acc = [&](double r) { sum += r * r; mx = std::max(mx, std::fabs(r)); };
+  for (int i = 0; i < m; ++i) acc(M->c[i] + M->d[i] * y[i] - lam[i] - mu[i]);
+  double la = 0;
+  for (int i = 0; i < m; ++i) la += lam[i] * M->a[i];
+  acc(1.0 - zeta - la);
+  for (int i = 0; i < m; ++i) acc((v[3 + i] - M->a[i] * z - y[i] + s[i] - M->b[i]) / damp);
+  sum += v[1]; mx = std::max(mx, v[4 + MMAX]);
+  sum += v[2]; mx = std::max(mx, v[5 + MMAX]);
+  for (int i = 0; i < m; ++i) acc(mu[i] * y[i] - M->eps);
+  acc(zeta * z - M->eps);
+  for (int i = 0; i < m; ++i) acc(lam[i] * s[i] - M->eps);
+  return {std::sqrt(sum), mx};
+}
+
+bool mma_update_dev(tos_mma* M, hipStream_t st, const double* df, const double* dg, const double* g,
+                    std::function<void(int)> prof_mark = nullptr) {
+  (void)prof_mark;
+  MmaVecs& V = M->V;
+  const int m = V.m;
+  MmaSmall S{};
+  S.move = M->move;
+  constexpr unsigned KKT_MAX = (1u << (3 + MMAX)) | (1u << (4 + MMAX)) | (1u << (5 + MMAX));
+  // p, q, b and the start point
+  {
+    const double* v = mma_pass(M, st, k_mma_pq, m, 0u, V, df, dg, M->partial);
+    for (int i = 0; i < m; ++i) M->b[i] = v[i] - g[i];
+  }
+  for (int i = 0; i < m; ++i) {
+    M->y[i] = 1; M->lam[i] = 1; M->s[i] = 1; M->mu[i] = std::max(M->c[i] / 2, 1.0);
+  }
+  M->z = 1; M->zeta = 1; M->eps = 1;
+  for (int i = 0; i < m; ++i) S.lam[i] = M->lam[i];
+  hipLaunchKernelGGL(k_mma_plam, dim3(M->nb), dim3(256), 0, st, V, S);
+  LAUNCH_CHECK();
+  double lasterr = 0;
+  int nsub = 0;
+  bool overflow = false;
+  for (int lev = 0; lev < 6 && !overflow; ++lev) {
+    double err = 1e30, damp = 1;
+    int stays = 0;
+    while (err > 0.9 * M->eps) {
+      S.eps = M->eps;
+      constexpr int CA = MMAX + MMAX * MMAX + MMAX;
+      const double* v = mma_pass(M, st, k_mma_passA, CA, 0u, V, S, M->partial);
+      // reduced (m+1) system (solveLinearSystem mma_t.cu:228-328), Gaussian elimination
+      const int N1 = m + 1;
+      double A[(MMAX + 1) * (MMAX + 1)] = {}, rhs[MMAX + 1], Dy[MMAX], dly[MMAX], dll[MMAX];
+      for (int i = 0; i < m; ++i) {
+        Dy[i] = M->d[i] + M->mu[i] / M->y[i];
+        dly[i] = -(M->lam[i] + M->eps / M->y[i] - M->c[i] - M->d[i] * M->y[i]);
+        dll[i] = -(M->z * M->a[i] + M->y[i] + M->b[i] - M->eps / M->lam[i] - v[i]);
+      }
+      double la = 0;
+      for (int i = 0; i < m; ++i) la += M->lam[i] * M->a[i];
+      const double dlz = -(la + M->eps / M->z - 1.0);
+      for (int i = 0; i < m; ++i) {
+        A[i * N1 + i] += M->s[i] / M->lam[i] + 1.0 / Dy[i];
+        for (int k = 0; k < m; ++k) A[i * N1 + k] += v[MMAX + i * MMAX + k];
+        rhs[i] = dll[i] + dly[i] / Dy[i] - v[MMAX + MMAX * MMAX + i];
+        A[i * N1 + m] = M->a[i];
+        A[m * N1 + i] = M->a[i];
+      }
+      A[m * N1 + m] = -M->zeta / M->z;
+      rhs[m] = dlz;
+      for (int k = 0; k < N1; ++k) {
+        int piv = k;
+        for (int i = k + 1; i < N1; ++i)
+          if (std::fabs(A[i * N1 + k]) > std::fabs(A[piv * N1 + k])) piv = i;
+        if (piv != k) {
+          for (int j = 0; j < N1; ++j) std::swap(A[k * N1 + j], A[piv * N1 + j]);
+          std::swap(rhs[k], rhs[piv]);
+        }
+        for (int i = k + 1; i < N1; ++i) {
+          const double f = A[i * N1 + k] / A[k * N1 + k];
+          for (int j = k; j < N1; ++j) A[i * N1 + j] -= f * A[k * N1 + j];
+          rhs[i] -= f * rhs[k];
+        }
+      }
+      for (int i = N1 - 1; i >= 0; --i) {
+        double sacc = rhs[i];
+        for (int j = i + 1; j < N1; ++j) sacc -= A[i * N1 + j] * rhs[j];
+        rhs[i] = sacc / A[i * N1 + i];
+      }
+      double dlam[MMAX], dy[MMAX], dmu[MMAX], ds[MMAX];
+      for (int i = 0; i < m; ++i) {
+        dlam[i] = rhs[i];
+        dy[i] = dlam[i] / Dy[i] - dly[i] / Dy[i];
+        dmu[i] = M->eps / M->y[i] - dy[i] * M->mu[i] / M->y[i] - M->mu[i];
+        ds[i] = M->eps / M->lam[i] - M->s[i] / M->lam[i] * dlam[i] - M->s[i];
+        S.dlam[i] = dlam[i];
+      }
+      const double dz = rhs[m];
+      const double dzeta = M->eps / M->z - M->zeta - (M->zeta / M->z) * dz;
+      // step bound
+      const double* tv = mma_pass(M, st, k_mma_passB, 3, 7u, V, S, M->partial);
+      const double cf = 1.0 / 0.99;
+      double t3 = tv[2];
+      for (int i = 0; i < m; ++i) t3 = std::max(t3, -cf * dy[i] / M->y[i]);
+      t3 = std::max(t3, -cf * dz / M->z);
+      for (int i = 0; i < m; ++i) t3 = std::max(t3, -cf * dlam[i] / M->lam[i]);
+      for (int i = 0; i < m; ++i) t3 = std::max(t3, -cf * dmu[i] / M->mu[i]);
+      t3 = std::max(t3, -cf * dzeta / M->zeta);
+      for (int i = 0; i < m; ++i) t3 = std::max(t3, -cf * ds[i] / M->s[i]);
+      double step = 1.0 / std::max(std::max(tv[0], tv[1]), std::max(t3, 1.0));
+      // backtracking on the KKT 2-norm (search_step mma_t.cu:501-571)
+      for (int i = 0; i < m; ++i) S.lam[i] = M->lam[i];
+      const double* kv = mma_pass(M, st, k_mma_kkt<false>, 3 + MMAX + 3, KKT_MAX, V, S, M->partial);
+      const double old_err = mma_kkt_host(M, kv, M->y, M->lam, M->mu, M->s, M->z, M->zeta, 1.0).first;
+      double new_err = 1e30, new_inf = 1e30;
+      while (step > 1e-12 && new_err > old_err) {
+        double ty[MMAX], tl[MMAX], tm[MMAX], ts[MMAX];
+        for (int i = 0; i < m; ++i) {
+          ty[i] = M->y[i] + dy[i] * step;
+          tl[i] = M->lam[i] + dlam[i] * step;
+          tm[i] = M->mu[i] + dmu[i] * step;
+          ts[i] = M->s[i] + ds[i] * step;
+          S.lam[i] = tl[i];
+        }
+        const double tz = M->z + dz * step, tzeta = M->zeta + dzeta * step;
+        S.step = step;
+        const double* kt = mma_pass(M, st, k_mma_kkt<true>, 3 + MMAX + 3, KKT_MAX, V, S, M->partial);
+        const auto e2 = mma_kkt_host(M, kt, ty, tl, tm, ts, tz, tzeta, damp);
+        new_err = e2.first;
+        new_inf = e2.second;
+        if (new_err < old_err) {
+          std::swap(V.x, V.tx);
+          std::swap(V.xi, V.txi);
+          std::swap(V.eta, V.teta);
+          for (int i = 0; i < m; ++i) { M->y[i] = ty[i]; M->lam[i] = tl[i]; M->mu[i] = tm[i]; M->s[i] = ts[i]; }
+          M->z = tz;
+          M->zeta = tzeta;
+          break;
+        }
+        step /= 2;
+      }
+      err = new_inf;
+      if ((err - lasterr) / err < 1e-5) ++stays; else stays = 0;
+      lasterr = err;
+      if (stays > 6) { damp *= 10; stays = 0; }
+      if (nsub++ > 80) { overflow = true; break; }
+    }
+    M->eps *= 0.1;
+  }
+  M->last_newton = nsub;
+  S.move = M->move;
+  const double* av = mma_pass(M, st, k_mma_asym, 1, 0u, V, S, M->partial);
+  if (std::sqrt(av[0]) < 1e-3) M->stop_counter++;
+  return M->stop_counter > 10;
 }
 
 template <class F>
@@ -718,9 +990,10 @@ TOS_EXPORT int tos_ctx_create(const tos_grid_desc* d, tos_ctx** out) {
       L.fre = dalloc<uint8_t>(*c, NP);
       auto pf = pad_nodes(L.v, L.h_free);
       upload(c->stream, L.fre, pf.data(), NP);
+      build_chunks(*c, L, l > 0);
       if (l > 0) {
-        L.st64 = dalloc<double>(*c, 243 * NP);
-        L.st32 = dalloc<float>(*c, 243 * NP);
+        L.st64 = dalloc<double>(*c, 243 * std::max<int64_t>(L.NS, 1));
+        L.st32 = dalloc<float>(*c, 243 * std::max<int64_t>(L.NS, 1));
         L.u64 = dalloc<double>(*c, 3 * NP);
         L.f64 = dalloc<double>(*c, 3 * NP);
         L.u32 = dalloc<float>(*c, 3 * NP);
@@ -739,6 +1012,22 @@ TOS_EXPORT int tos_ctx_create(const tos_grid_desc* d, tos_ctx** out) {
     c->eact = dalloc<uint8_t>(*c, v0.NE);
     auto pe = pad_elems(v0, c->h_solid.data());
     upload(c->stream, c->eact, pe.data(), v0.NE);
+    {
+      // active elements whose 8 nodes are all free (Galerkin fast path)
+      std::vector<uint8_t> af(v0.NE, 0);
+      const Level& L0h = c->L[0];
+      for (int z = 0; z < v0.ez; ++z)
+        for (int y = 0; y < v0.ey; ++y)
+          for (int x = 0; x < v0.ex; ++x) {
+            if (!pe[v0.pelem(x, y, z)]) continue;
+            bool all = true;
+            for (int j = 0; j < 8 && all; ++j)
+              all = L0h.h_free[(x + (j & 1)) + (size_t)v0.nx * ((y + ((j >> 1) & 1)) + (size_t)v0.ny * (z + (j >> 2)))];
+            af[v0.pelem(x, y, z)] = all;
+          }
+      c->allfree = dalloc<uint8_t>(*c, v0.NE);
+      upload(c->stream, c->allfree, af.data(), v0.NE);
+    }
     c->shell = dalloc<uint8_t>(*c, v0.NE);
     auto ps = pad_elems(v0, c->h_shell.empty() ? nullptr : c->h_shell.data());
     for (size_t i = 0; i < ps.size(); ++i) ps[i] = ps[i] && pe[i];
@@ -757,7 +1046,8 @@ TOS_EXPORT int tos_ctx_create(const tos_grid_desc* d, tos_ctx** out) {
     c->z = dalloc<double>(*c, 3 * v0.NP);
     c->p = dalloc<double>(*c, 3 * v0.NP);
     c->q = dalloc<double>(*c, 3 * v0.NP);
-    c->partial = dalloc<double>(*c, 2 * kRedBlocks);
+    c->nb0 = (int)cblk(c->L[0].n_lin);
+    c->partial = dalloc<double>(*c, 2 * (size_t)std::max(kRedBlocks, c->nb0));
     c->scal = dalloc<double>(*c, S_COUNT);
     HIPCHK(hipHostMalloc((void**)&c->h_scal, S_COUNT * sizeof(double)));
     c->K = 1;
@@ -941,16 +1231,17 @@ TOS_EXPORT int tos_update_operator(tos_ctx* c) {
     for (size_t l = 1; l < c->L.size(); ++l) {
       Level& F = c->L[l - 1];
       Level& C = c->L[l];
-      const int64_t nc = (int64_t)C.v.nx * C.v.ny * C.v.nz;
-      dim3 grid(nblk(nc), 27);
+      if (C.n_col == 0) continue;
+      dim3 grid(cblk(C.n_col), 27);
       if (l == 1)
-        hipLaunchKernelGGL(k_galerkin_first, grid, dim3(256), 0, c->stream, F.v, C.v, (const uint8_t*)F.fre,
-                           (const uint8_t*)C.fre, (const double*)c->w64, (const double*)c->Ke64,
-                           (const double*)c->Mc64, C.st64);
+        hipLaunchKernelGGL(k_galerkin_first, grid, dim3(256), 0, c->stream, F.v, C.v, (const int4*)C.cl_col,
+                           C.n_col, (const uint8_t*)F.fre, (const uint8_t*)c->allfree, (const uint8_t*)C.fre,
+                           (const double*)c->w64, (const double*)c->Ke64, (const double*)c->Mc64, C.st64, C.NS);
       else
-        hipLaunchKernelGGL(k_galerkin_next, grid, dim3(256), 0, c->stream, F.v, C.v, (const uint8_t*)C.fre,
-                           (const double*)F.st64, C.st64);
-      const int64_t ns = 243 * C.v.NP;
+        hipLaunchKernelGGL(k_galerkin_next, grid, dim3(256), 0, c->stream, F.v, C.v, (const int4*)C.cl_col,
+                           C.n_col, (const uint8_t*)C.fre, (const int*)F.slot, (const double*)F.st64, F.NS, C.st64,
+                           C.NS);
+      const int64_t ns = 243 * C.NS;
       hipLaunchKernelGGL(k_cast_d2f, dim3(nblk(ns)), dim3(256), 0, c->stream, (const double*)C.st64, C.st32, ns);
       LAUNCH_CHECK();
     }
@@ -966,15 +1257,18 @@ TOS_EXPORT int tos_get_stencil(tos_ctx* c, int level, double* out) {
   return guard(c, [&]() {
     REQUIRE(out && level >= 1 && level < (int)c->L.size(), "bad level");
     if (!c->op_ready) { c->err = "update_operator first"; return TOS_E_STATE; }
-    const LevelView& v = c->L[level].v;
-    std::vector<double> h((size_t)243 * v.NP);
-    download(c->stream, h.data(), c->L[level].st64, h.size() * sizeof(double));
+    const Level& L = c->L[level];
+    const LevelView& v = L.v;
+    std::vector<double> h((size_t)243 * std::max<int64_t>(L.NS, 1));
+    download(c->stream, h.data(), L.st64, h.size() * sizeof(double));
     const size_t nn = (size_t)v.nx * v.ny * v.nz;
-    for (int e = 0; e < 243; ++e)
-      for (int z = 0; z < v.nz; ++z)
-        for (int y = 0; y < v.ny; ++y)
-          for (int x = 0; x < v.nx; ++x)
-            out[e * nn + x + (size_t)v.nx * (y + (size_t)v.ny * z)] = h[(size_t)e * v.NP + v.pnode(x, y, z)];
+    for (int z = 0; z < v.nz; ++z)
+      for (int y = 0; y < v.ny; ++y)
+        for (int x = 0; x < v.nx; ++x) {
+          const size_t node = x + (size_t)v.nx * (y + (size_t)v.ny * z);
+          const int sl = L.h_slot[v.pnode(x, y, z)];
+          for (int e = 0; e < 243; ++e) out[e * nn + node] = sl < 0 ? 0.0 : h[(size_t)e * L.NS + sl];
+        }
     return TOS_OK;
   });
 }
@@ -1004,7 +1298,7 @@ TOS_EXPORT int tos_vcycle(tos_ctx* c, const double* r, double* z, int precision,
     upload(c->stream, st, r, 3 * n * sizeof(double));
     hipLaunchKernelGGL(k_pad_vec, dim3(nblk(n)), dim3(256), 0, c->stream, L0.v, (const double*)st, c->r,
                        (const uint8_t*)L0.fre);
-    precondition(*c, c->r, c->z, precision, pre, post);
+    precondition(*c, precision, pre, post, false, true);
     return get_vec(c, c->z, z);
   });
 }
@@ -1056,7 +1350,7 @@ TOS_EXPORT int tos_compliance(tos_ctx* c, double* out) {
     REQUIRE(out, "null output");
     const int64_t n3 = 3 * c->L[0].v.NP;
     for (int k = 0; k < c->K; ++k) {
-      dot_to(*c, c->F + (size_t)k * n3, c->U + (size_t)k * n3, nullptr, nullptr, n3, S_TMP0, 0, OP_NONE);
+      dot_to(*c, c->F + (size_t)k * n3, c->U + (size_t)k * n3, nullptr, nullptr, S_TMP0, 0, OP_NONE);
       out[k] = read_scal(*c)[S_TMP0];
     }
     return TOS_OK;
@@ -1076,6 +1370,7 @@ TOS_EXPORT int tos_sensitivity(tos_ctx* c, int k) {
     REQUIRE(k >= 0 && k < c->K, "bad case");
     const LevelView& v = c->L[0].v;
     const int64_t n = (int64_t)v.ex * v.ey * v.ez;
+    Prof pr(*c, P_SENS_FILTER);
     hipLaunchKernelGGL((k_sensitivity<0>), dim3(nblk(n)), dim3(256), 0, c->stream, v, (const uint8_t*)c->eact,
                        (const float*)c->rho, (const double*)c->Ke64, (const double*)(c->U + (size_t)k * 3 * v.NP),
                        c->penal, c->sens, (double*)nullptr);
@@ -1165,6 +1460,7 @@ TOS_EXPORT int tos_filter(tos_ctx* c, int which, double R) {
     const LevelView& v = c->L[0].v;
     HIPCHK(hipMemcpyAsync(c->tmpf, b, v.NE * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
     const int64_t n = (int64_t)v.ex * v.ey * v.ez;
+    Prof pr(*c, P_SENS_FILTER);
     hipLaunchKernelGGL(k_filter, dim3(nblk(n)), dim3(256), 0, c->stream, v, (const uint8_t*)c->eact,
                        (const float*)c->tmpf, b, (const int*)doff, (const double*)dwt, nt);
     LAUNCH_CHECK();
@@ -1177,6 +1473,7 @@ TOS_EXPORT int tos_oc_update(tos_ctx* c, double vgoal, double move, double damp,
   return guard(c, [&]() {
     REQUIRE(move > 0 && damp > 0, "bad OC parameters");
     const LevelView& v = c->L[0].v;
+    Prof pr(*c, P_OPT);
     hipLaunchKernelGGL(k_absmax_partial, dim3(kRedBlocks), dim3(256), 0, c->stream, v, (const uint8_t*)c->eact,
                        (const float*)c->sens, c->partial);
     hipLaunchKernelGGL(k_oc_init, dim3(1), dim3(256), 0, c->stream, (const double*)c->partial, kRedBlocks, c->scal);
@@ -1298,6 +1595,7 @@ TOS_EXPORT int tos_spline_eval(tos_ctx* c) {
     REQUIRE(c->sp_mode >= 0, "spline not initialised");
     const LevelView& v = c->L[0].v;
     const int64_t n = (int64_t)v.ex * v.ey * v.ez;
+    Prof pr(*c, P_SPLINE);
     hipLaunchKernelGGL(k_spline_eval, dim3(nblk(n)), dim3(256), 0, c->stream, v, (const uint8_t*)c->eact,
                        (const uint8_t*)c->shell, (const float*)c->coef, c->nc[0], c->nc[1], c->sp_order,
                        (const int*)c->sp_idx[0], (const float*)c->sp_wt[0], (const uint8_t*)c->sp_ok[0],
@@ -1316,6 +1614,7 @@ TOS_EXPORT int tos_spline_adjoint(tos_ctx* c, int which) {
     REQUIRE(b && c->sp_mode >= 0, "spline not initialised");
     const LevelView& v = c->L[0].v;
     const int64_t n = (int64_t)v.ex * v.ey * v.ez;
+    Prof pr(*c, P_SPLINE);
     hipLaunchKernelGGL(k_spline_mask, dim3(nblk(n)), dim3(256), 0, c->stream, v, (const uint8_t*)c->eact,
                        (const uint8_t*)c->shell, (const float*)b, (const uint8_t*)c->sp_ok[0],
                        (const uint8_t*)c->sp_ok[1], (const uint8_t*)c->sp_ok[2], c->sp_g);
@@ -1353,20 +1652,23 @@ TOS_EXPORT int tos_mma_create(int n, int m, const double* xmin, const double* xm
   try {
     HIPCHK(hipGetDevice(&M->dev));
     HIPCHK(hipStreamCreateWithFlags(&M->stream, hipStreamNonBlocking));
-    MmaDev& d = M->d;
-    d.n = n; d.m = m; d.move = move;
-    double** vecs[] = {&d.x, &d.xmin, &d.xmax, &d.L, &d.U, &d.alpha, &d.beta, &d.lastdx, &d.xi, &d.eta,
-                       &d.plam, &d.qlam, &d.curx, &d.Dx, &d.dlx, &d.dx, &d.dxi, &d.deta, &d.tx, &d.txi, &d.teta};
+    MmaVecs& V = M->V;
+    V.n = n; V.m = m;
+    M->move = move;
+    for (int i = 0; i < MMAX; ++i) { M->a[i] = 0; M->c[i] = 10; M->d[i] = 1; }  // mma_t.h:206-207
+    double** vecs[] = {&V.x, &V.xmin, &V.xmax, &V.L, &V.U, &V.alpha, &V.beta, &V.lastdx, &V.xi, &V.eta,
+                       &V.plam, &V.qlam, &V.curx, &V.Dx, &V.dlx, &V.dx, &V.dxi, &V.deta, &V.tx, &V.txi, &V.teta};
     for (double** v : vecs) *v = malloc_mma<double>(M, n);
-    d.p = malloc_mma<double>(M, (size_t)(m + 1) * n);
-    d.q = malloc_mma<double>(M, (size_t)(m + 1) * n);
-    d.G = malloc_mma<double>(M, (size_t)m * n);
-    d.sv = malloc_mma<double>(M, SV_COUNT);
-    d.out = malloc_mma<double>(M, 3);
-    M->stage = malloc_mma<double>(M, (size_t)(m + 1) * n + m);
-    HIPCHK(hipHostMalloc((void**)&M->h_out, 3 * sizeof(double)));
+    V.p = malloc_mma<double>(M, (size_t)(m + 1) * n);
+    V.q = malloc_mma<double>(M, (size_t)(m + 1) * n);
+    V.G = malloc_mma<double>(M, (size_t)m * n);
+    M->nb = (int)std::min<int64_t>(512, std::max<int64_t>(1, (n + 255) / 256));
+    M->partial = malloc_mma<double>(M, (size_t)MMA_CNT * M->nb);
+    M->red = malloc_mma<double>(M, MMA_CNT);
+    M->stage = malloc_mma<double>(M, (size_t)(m + 1) * n);
+    HIPCHK(hipHostMalloc((void**)&M->h_red, MMA_CNT * sizeof(double)));
     // set_bound + clamp_asymptotes + init_subproblem_variable (mma_t.cu:747-855)
-    std::vector<double> x(n), L(n), U(n), al(n), be(n), xi(n), eta(n);
+    std::vector<double> x(n), L(n), U(n), al(n), be(n);
     for (int j = 0; j < n; ++j) {
       const double st = xmax[j] - xmin[j];
       if (!(st > 0)) throw ArgError("xmax must exceed xmin");
@@ -1377,18 +1679,18 @@ TOS_EXPORT int tos_mma_create(int n, int m, const double* xmin, const double* xm
       be[j] = std::min(xmax[j], std::min(U[j] - 0.1 * (U[j] - x[j]), x[j] + move * st));
       x[j] = 0.5 * (al[j] + be[j]);
     }
-    upload(M->stream, d.x, x.data(), n * sizeof(double));
-    upload(M->stream, d.xmin, xmin, n * sizeof(double));
-    upload(M->stream, d.xmax, xmax, n * sizeof(double));
-    upload(M->stream, d.L, L.data(), n * sizeof(double));
-    upload(M->stream, d.U, U.data(), n * sizeof(double));
-    upload(M->stream, d.alpha, al.data(), n * sizeof(double));
-    upload(M->stream, d.beta, be.data(), n * sizeof(double));
+    upload(M->stream, V.x, x.data(), n * sizeof(double));
+    upload(M->stream, V.xmin, xmin, n * sizeof(double));
+    upload(M->stream, V.xmax, xmax, n * sizeof(double));
+    upload(M->stream, V.L, L.data(), n * sizeof(double));
+    upload(M->stream, V.U, U.data(), n * sizeof(double));
+    upload(M->stream, V.alpha, al.data(), n * sizeof(double));
+    upload(M->stream, V.beta, be.data(), n * sizeof(double));
   } catch (const std::exception& e) {
     std::fprintf(stderr, "tos_mma_create: %s\n", e.what());
     for (void* p : M->allocs) (void)hipFree(p);
     delete M;
-    return TOS_E_HIP;
+    return e.what()[0] == 'x' ? TOS_E_ARG : TOS_E_HIP;
   }
   *out = M;
   return TOS_OK;
@@ -1399,7 +1701,7 @@ TOS_EXPORT int tos_mma_destroy(tos_mma* M) {
   (void)hipSetDevice(M->dev);
   (void)hipStreamSynchronize(M->stream);
   for (void* p : M->allocs) (void)hipFree(p);
-  if (M->h_out) (void)hipHostFree(M->h_out);
+  if (M->h_red) (void)hipHostFree(M->h_red);
   (void)hipStreamDestroy(M->stream);
   delete M;
   return TOS_OK;
@@ -1409,22 +1711,22 @@ TOS_EXPORT int tos_mma_set_x(tos_mma* M, const double* x) {
   if (!M || !x) return TOS_E_ARG;
   try {
     HIPCHK(hipSetDevice(M->dev));
-    const int n = M->d.n;
+    const int n = M->V.n;
     std::vector<double> lo(n), hi(n), L(n), U(n), al(n), be(n);
-    download(M->stream, lo.data(), M->d.xmin, n * sizeof(double));
-    download(M->stream, hi.data(), M->d.xmax, n * sizeof(double));
+    download(M->stream, lo.data(), M->V.xmin, n * sizeof(double));
+    download(M->stream, hi.data(), M->V.xmax, n * sizeof(double));
     for (int j = 0; j < n; ++j) {
       const double st = hi[j] - lo[j];
       L[j] = std::min(std::max(x[j] - 0.5 * st, x[j] - 10 * st), x[j] - 0.01 * st);
       U[j] = std::min(std::max(x[j] + 0.5 * st, x[j] + 0.01 * st), x[j] + 10 * st);
-      al[j] = std::max(lo[j], std::max(L[j] + 0.1 * (x[j] - L[j]), x[j] - M->d.move * st));
-      be[j] = std::min(hi[j], std::min(U[j] - 0.1 * (U[j] - x[j]), x[j] + M->d.move * st));
+      al[j] = std::max(lo[j], std::max(L[j] + 0.1 * (x[j] - L[j]), x[j] - M->move * st));
+      be[j] = std::min(hi[j], std::min(U[j] - 0.1 * (U[j] - x[j]), x[j] + M->move * st));
     }
-    upload(M->stream, M->d.x, x, n * sizeof(double));
-    upload(M->stream, M->d.L, L.data(), n * sizeof(double));
-    upload(M->stream, M->d.U, U.data(), n * sizeof(double));
-    upload(M->stream, M->d.alpha, al.data(), n * sizeof(double));
-    upload(M->stream, M->d.beta, be.data(), n * sizeof(double));
+    upload(M->stream, M->V.x, x, n * sizeof(double));
+    upload(M->stream, M->V.L, L.data(), n * sizeof(double));
+    upload(M->stream, M->V.U, U.data(), n * sizeof(double));
+    upload(M->stream, M->V.alpha, al.data(), n * sizeof(double));
+    upload(M->stream, M->V.beta, be.data(), n * sizeof(double));
   } catch (const std::exception& e) {
     std::fprintf(stderr, "tos_mma_set_x: %s\n", e.what());
     return TOS_E_HIP;
@@ -1436,7 +1738,7 @@ TOS_EXPORT int tos_mma_get_x(tos_mma* M, double* x) {
   if (!M || !x) return TOS_E_ARG;
   try {
     HIPCHK(hipSetDevice(M->dev));
-    download(M->stream, x, M->d.x, M->d.n * sizeof(double));
+    download(M->stream, x, M->V.x, M->V.n * sizeof(double));
   } catch (const std::exception& e) {
     std::fprintf(stderr, "tos_mma_get_x: %s\n", e.what());
     return TOS_E_HIP;
@@ -1448,12 +1750,12 @@ TOS_EXPORT int tos_mma_update(tos_mma* M, const double* df, const double* dg, co
   if (!M || !df || !dg || !g) return TOS_E_ARG;
   try {
     HIPCHK(hipSetDevice(M->dev));
-    const int n = M->d.n, m = M->d.m;
+    const int n = M->V.n, m = M->V.m;
     double* s = M->stage;
     upload(M->stream, s, df, n * sizeof(double));
     upload(M->stream, s + n, dg, (size_t)m * n * sizeof(double));
-    upload(M->stream, s + (size_t)(m + 1) * n, g, m * sizeof(double));
-    mma_run(M, s, s + n, s + (size_t)(m + 1) * n, converged);
+    const bool cv = mma_update_dev(M, M->stream, s, s + n, g);
+    if (converged) *converged = cv;
   } catch (const std::exception& e) {
     std::fprintf(stderr, "tos_mma_update: %s\n", e.what());
     return TOS_E_HIP;
@@ -1462,34 +1764,33 @@ TOS_EXPORT int tos_mma_update(tos_mma* M, const double* df, const double* dg, co
 }
 
 __global__ void k_mma_inputs(int n, const float* __restrict__ cs, const float* __restrict__ cv, double ss, double vs,
-                             double* __restrict__ df, double* __restrict__ dg, double* __restrict__ x,
-                             float* __restrict__ coef, int mode) {
+                             double* __restrict__ df, double* __restrict__ dg) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  if (mode == 0) {
-    df[i] = ss * (double)cs[i];
-    dg[i] = vs * (double)cv[i];
-  } else {
-    coef[i] = (float)x[i];
-  }
+  df[i] = ss * (double)cs[i];
+  dg[i] = vs * (double)cv[i];
 }
 
 TOS_EXPORT int tos_ctx_mma_step(tos_ctx* c, tos_mma* M, double sscale, double vscale, double vfrac, int* converged) {
   return guard(c, [&]() {
-    REQUIRE(M && c->sp_mode >= 0 && M->d.m == 1 &&
-                (int64_t)M->d.n == (int64_t)c->nc[0] * c->nc[1] * c->nc[2],
+    REQUIRE(M && c->sp_mode >= 0 && M->V.m == 1 &&
+                (int64_t)M->V.n == (int64_t)c->nc[0] * c->nc[1] * c->nc[2],
             "MMA size does not match the spline coefficients");
     const double vol = volume_of(*c);
-    const int n = M->d.n;
+    const int n = M->V.n;
     double* s = M->stage;
     const double g = vscale * (vol - vfrac);
     hipLaunchKernelGGL(k_mma_inputs, dim3(nblk(n)), dim3(256), 0, c->stream, n, (const float*)c->csens,
-                       (const float*)c->cvsens, sscale, vscale, s, s + n, (double*)nullptr, (float*)nullptr, 0);
-    HIPCHK(hipMemcpyAsync(s + 2 * n, &g, sizeof(double), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    mma_run(M, s, s + n, s + 2 * n, converged);
-    hipLaunchKernelGGL(k_mma_inputs, dim3(nblk(n)), dim3(256), 0, c->stream, n, (const float*)nullptr,
-                       (const float*)nullptr, 0.0, 0.0, (double*)nullptr, (double*)nullptr, M->d.x, c->coef, 1);
+                       (const float*)c->cvsens, sscale, vscale, s, s + n);
+    LAUNCH_CHECK();
+    bool cv;
+    {
+      Prof pr(*c, P_OPT);
+      cv = mma_update_dev(M, c->stream, s, s + n, &g);
+    }
+    if (converged) *converged = cv;
+    hipLaunchKernelGGL((k_copy_cast<double, float>), dim3(nblk(n)), dim3(256), 0, c->stream, (const double*)M->V.x,
+                       c->coef, (int64_t)n);
     LAUNCH_CHECK();
     HIPCHK(hipStreamSynchronize(c->stream));
     return TOS_OK;

@@ -48,6 +48,17 @@ struct Level {
   float* st32 = nullptr;
   double *u64 = nullptr, *f64 = nullptr, *r64 = nullptr;
   float *u32 = nullptr, *f32 = nullptr, *r32 = nullptr;
+  // chunk lists, int4 {padded index of first node, x0 | count << 16, y, z}: `lin` holds
+  // x-contiguous runs of free nodes, `col` the runs of each colour (x stride 2),
+  // concatenated in colour order with col_off[c] the first chunk of colour c
+  int4* cl_lin = nullptr;
+  int n_lin = 0;
+  int4* cl_col = nullptr;
+  int n_col = 0;
+  int col_off[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  int64_t NS = 0;                // stencil slots per entry = 64 * n_col
+  int* slot = nullptr;           // padded node -> stencil slot, -1 if not free
+  std::vector<int> h_slot;
 };
 
 struct Ctx {
@@ -62,6 +73,7 @@ struct Ctx {
   int64_t n_active = 0;
   // fine-level element data (padded element layout)
   uint8_t* eact = nullptr;
+  uint8_t* allfree = nullptr;   // active element whose 8 nodes are all free
   uint8_t* shell = nullptr;
   float* rho = nullptr;
   float* rho_pre = nullptr;
@@ -83,7 +95,8 @@ struct Ctx {
   // PCG work vectors (FP64, 3*NP)
   double *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
   // reductions
-  double* partial = nullptr;    // kRedBlocks * 8
+  double* partial = nullptr;    // 2 * max(kRedBlocks, blocks of the fine chunk list)
+  int nb0 = 0;                  // blocks of a launch over the fine chunk list
   double* scal = nullptr;       // device scalars
   double* h_scal = nullptr;     // pinned host mirror
   // coarsest level

@@ -40,31 +40,67 @@ def batched_patch_loads(nx, ny, nz, K=16, patch=8, seed=7):
     return F.reshape(K, -1), np.array(dirs)
 
 
+def _ellipsoid_union(nx, ny, nz, cs, ax):
+    solid = np.zeros((nz, ny, nx), bool)
+    for (cx, cy, cz), (a, b, c) in zip(cs, ax):
+        x0, x1 = max(int(cx - a) - 1, 0), min(int(cx + a) + 2, nx)
+        y0, y1 = max(int(cy - b) - 1, 0), min(int(cy + b) + 2, ny)
+        z0, z1 = max(int(cz - c) - 1, 0), min(int(cz + c) + 2, nz)
+        if x0 >= x1 or y0 >= y1 or z0 >= z1:
+            continue
+        X2 = ((np.arange(x0, x1, dtype=np.float32) + 0.5 - cx) / a) ** 2
+        Y2 = ((np.arange(y0, y1, dtype=np.float32) + 0.5 - cy) / b) ** 2
+        Z2 = ((np.arange(z0, z1, dtype=np.float32) + 0.5 - cz) / c) ** 2
+        xy = Y2[:, None] + X2[None, :]
+        for k, z2 in enumerate(Z2):
+            solid[z0 + k, y0:y1, x0:x1] |= xy <= (1.0 - z2)
+    return solid
+
+
+# The lattice ball {d : |d|^2 <= 9} is exactly the union of these cuboids (half-widths
+# hx, hy, hz): every point of the ball lies in one, and each cuboid's far corner has
+# |d|^2 = 9.
+_BALL3 = ((0, 0, 3), (3, 0, 0), (0, 3, 0), (2, 1, 2), (1, 2, 2), (2, 2, 1))
+
+
+def _within3_of_exterior(solid):
+    """Solid voxels whose centre is within Euclidean distance 3 of a non-solid voxel
+    centre inside the array (the same as distance_transform_edt(solid) <= 3)."""
+    from scipy import ndimage
+    ext = (~solid).astype(np.uint8)
+    near = np.zeros(solid.shape, bool)
+    for hx, hy, hz in _BALL3:
+        d = ext
+        for axis, hw in ((2, hx), (1, hy), (0, hz)):
+            if hw:
+                d = ndimage.maximum_filter1d(d, 2 * hw + 1, axis=axis, mode="constant", cval=0)
+        near |= d.astype(bool)
+    return solid & near
+
+
 def ellipsoids(nx=511, ny=320, nz=320, n=6, seed=2024, target=0.35, shell_width=3,
                axes=(60, 180), max_tries=64):
     """Union of n axis-aligned ellipsoids (semi-axes ~U[axes], centres uniform in the
-    central 60% of the box) whose fill is closest to `target`; keeps the component
-    connected to z = 0.  Returns solid, support, loads, shell, meta."""
+    central 60% of the box), the lowest one lowered to touch z = 0; keeps the part
+    connected to z = 0 and the first draw whose fill is within 2% of `target`.
+    Returns solid, support, loads, shell, meta."""
     from scipy import ndimage
 
+    assert shell_width == 3, "the cuboid decomposition covers the radius-3 ball"
     best = None
     rng = np.random.default_rng(seed)
-    z = np.arange(nz, dtype=np.float32)[:, None, None] + 0.5
-    y = np.arange(ny, dtype=np.float32)[None, :, None] + 0.5
-    x = np.arange(nx, dtype=np.float32)[None, None, :] + 0.5
     for t in range(max_tries):
         cs = np.stack([rng.uniform(0.2 * nx, 0.8 * nx, n), rng.uniform(0.2 * ny, 0.8 * ny, n),
                        rng.uniform(0.2 * nz, 0.8 * nz, n)], 1)
         ax = rng.uniform(axes[0], axes[1], (n, 3))
-        # the lowest ellipsoid is dropped onto z = 0 so that the solid touches the support
         lo = np.argmin(cs[:, 2] - ax[:, 2])
         cs[lo, 2] = ax[lo, 2] * 0.8
-        solid = np.zeros((nz, ny, nx), bool)
-        for (cx, cy, cz), (a, b, c) in zip(cs, ax):
-            solid |= ((x - cx) / a) ** 2 + ((y - cy) / b) ** 2 + ((z - cz) / c) ** 2 <= 1.0
+        solid = _ellipsoid_union(nx, ny, nz, cs, ax)
         lab, nl = ndimage.label(solid)
-        keep = np.unique(lab[0][lab[0] > 0])
-        conn = np.isin(lab, keep)
+        keep = np.zeros(nl + 1, bool)
+        keep[np.unique(lab[0])] = True
+        keep[0] = False
+        conn = keep[lab]
         fill = conn.mean()
         if best is None or abs(fill - target) < abs(best[0] - target):
             best = (fill, conn, cs, ax, t)
@@ -72,7 +108,6 @@ def ellipsoids(nx=511, ny=320, nz=320, n=6, seed=2024, target=0.35, shell_width=
             break
     fill, solid, cs, ax, tries = best
     solid_u8 = solid.astype(np.uint8)
-    # nodes of active elements
     act = np.zeros((nz + 1, ny + 1, nx + 1), bool)
     for dz in (0, 1):
         for dy in (0, 1):
@@ -80,25 +115,30 @@ def ellipsoids(nx=511, ny=320, nz=320, n=6, seed=2024, target=0.35, shell_width=
                 act[dz:dz + nz, dy:dy + ny, dx:dx + nx] |= solid
     support = np.zeros_like(act)
     support[0] = act[0]
-    # top-surface voxels: solid with empty (or outside) +z neighbour
+    # top-surface voxels: solid with an empty (or outside) +z neighbour
     top = solid.copy()
     top[:-1] &= ~solid[1:]
     occ = ndimage.uniform_filter(solid.astype(np.float32), size=3, mode="constant")
-    gz, gy, gx = np.gradient(occ)
-    nrm = -np.stack([gx, gy, gz], 0)
-    f = np.zeros((3, nz + 1, ny + 1, nx + 1))
     tz, ty, tx = np.nonzero(top)
+    f = np.zeros((3, nz + 1, ny + 1, nx + 1))
     if tz.size:
-        v = nrm[:, tz, ty, tx]
+        # outward normal -grad(occ) by central differences at the top voxels
+        def cd(axis, idx):
+            hi = list(idx); lo_ = list(idx)
+            hi[axis] = np.minimum(idx[axis] + 1, occ.shape[axis] - 1)
+            lo_[axis] = np.maximum(idx[axis] - 1, 0)
+            return (occ[tuple(hi)] - occ[tuple(lo_)]) * 0.5
+        idx = (tz, ty, tx)
+        v = -np.stack([cd(2, idx), cd(1, idx), cd(0, idx)], 0).astype(np.float64)
         ln = np.linalg.norm(v, axis=0)
-        v = np.where(ln > 1e-12, v / np.maximum(ln, 1e-12), np.array([[0], [0], [1.0]]))
+        v = np.where(ln > 1e-12, v / np.maximum(ln, 1e-12), np.array([[0.0], [0.0], [1.0]]))
         wgt = 1.0 / (4 * tz.size)
+        nn = (nz + 1) * (ny + 1) * (nx + 1)
         for (oy, ox) in ((0, 0), (0, 1), (1, 0), (1, 1)):
+            flat = (tx + ox) + (nx + 1) * ((ty + oy) + (ny + 1) * (tz + 1))
             for c in range(3):
-                np.add.at(f[c], (tz + 1, ty + oy, tx + ox), wgt * v[c])
-    # shell: elements within shell_width of the exterior
-    dist = ndimage.distance_transform_edt(solid)
-    shell = (solid & (dist <= shell_width)).astype(np.uint8)
+                f[c] += np.bincount(flat, weights=wgt * v[c], minlength=nn).reshape(f[c].shape)
+    shell = _within3_of_exterior(solid).astype(np.uint8)
     meta = {"fill": float(fill), "tries": int(tries + 1), "centres": cs.tolist(),
             "semi_axes": ax.tolist(), "n_top": int(tz.size), "n_solid": int(solid.sum())}
     return solid_u8, support.astype(np.uint8), f.reshape(-1), shell, meta
