@@ -171,6 +171,14 @@ int tos_spline_eval(tos_ctx* ctx);
  * dvol2dcoeff, grid/Grid.cu:3643-3859), deterministic gather. */
 int tos_spline_adjoint(tos_ctx* ctx, int which);
 int tos_spline_get_coeff_sens(tos_ctx* ctx, int which, float* out);
+/* OC bisection on the spline coefficients with the compliance coefficient sensitivity
+ * (updateCoeff / tryCSensMultiplier_kernel, optimization/optimization.cu:559-695): each
+ * trial c' = clamp(clamp(c (max(0,-s)/g)^damp, c - move, c + move), cmin, cmax) is
+ * evaluated to densities and the bisection stops at |V - vgoal| <= 1e-4 or 31 trials.
+ * The coefficients end at the last trial (the reference applies the multiplier once more
+ * on top of it). */
+int tos_oc_update_coeffs(tos_ctx* ctx, double vgoal, double move, double damp, double cmin,
+                         double cmax, double* g_out, int* steps_out);
 
 /* ---- MMA --------------------------------------------------------------------------
  * Method of moving asymptotes over n design variables with m constraints

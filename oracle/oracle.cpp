@@ -1572,6 +1572,40 @@ ORC_API int orc_spline_get_coeff_sens(tos_ctx* c, int which, float* out) {
   return TOS_OK;
 }
 
+// updateCoeff / tryCSensMultiplier_kernel (optimization/optimization.cu:559-695): OC
+// bisection on the coefficients, each trial evaluated to densities; the coefficients end
+// at the last trial.
+ORC_API int orc_oc_update_coeffs(tos_ctx* c, double vgoal, double move, double damp, double cmin, double cmax,
+                                  double* g_out, int* steps) {
+  if (!c || c->sp_mode < 0) return TOS_E_ARG;
+  const size_t n = c->coef.size();
+  double gmax = 0;
+  for (size_t j = 0; j < n; ++j) gmax = std::max(gmax, (double)std::fabs(c->csens[j]));
+  std::vector<float> old(c->coef);
+  double lo = 0, hi = gmax, g = 0, V = 2;
+  int itn = 0, trials = 0;
+  do {
+    g = (lo + hi) / 2;
+    for (size_t j = 0; j < n; ++j) {
+      double s = c->csens[j];
+      double gg = s > 0 ? 0.0 : -s;
+      double c0 = old[j];
+      double cn = c0 * std::pow(gg / g, damp);
+      cn = std::min(std::max(cn, c0 - move), c0 + move);
+      cn = std::min(std::max(cn, cmin), cmax);
+      c->coef[j] = (float)cn;
+    }
+    orc_spline_eval(c);
+    orc_volume(c, &V);
+    ++trials;
+    if (V > vgoal) lo = g;
+    else if (V < vgoal) hi = g;
+  } while (std::fabs(V - vgoal) > 1e-4 && itn++ < 30);
+  if (g_out) *g_out = g;
+  if (steps) *steps = trials;
+  return TOS_OK;
+}
+
 ORC_API int orc_mma_create(int n, int m, const double* xmin, const double* xmax, double move, tos_mma** out) {
   if (n <= 0 || m <= 0 || m > 8 || !xmin || !xmax || !out || !(move > 0)) return TOS_E_ARG;
   tos_mma* M = new tos_mma();

@@ -207,3 +207,27 @@ def test_spline_mma_trajectory_matches_oracle(hip):
     co, cg = np.array(ro["c"]), np.array(rg["c"])
     assert np.all(np.abs(cg - co) <= 1e-5 * np.abs(co)), (co, cg)
     assert rel(rg["ctx"].coeffs(), ro["ctx"].coeffs()) < 1e-5
+
+
+def test_oc_on_coefficients_matches_oracle(hip):
+    solid, support, f = cantilever(32, 16, 16)
+    o, g = pair(hip, solid, support, f)
+    n = o.spline_init(1, 4, 4)
+    g.spline_init(1, 4, 4)
+    c0 = np.full(n, 0.5, np.float32)
+    for c in (o, g):
+        c.set_coeffs(c0)
+        c.spline_eval()
+        c.update_operator()
+        c.solve(rtol=1e-10)
+        c.sensitivity()
+        c.filter(2.0)
+        c.spline_adjoint(0)
+    g.set_sensitivity(o.get_sensitivity())
+    g.spline_adjoint(0)
+    ro = o.oc_update_coeffs(0.45, move=0.1, damp=0.5)
+    rg = g.oc_update_coeffs(0.45, move=0.1, damp=0.5)
+    assert ro[1] == rg[1]
+    assert abs(ro[0] - rg[0]) <= 1e-9 * abs(ro[0])
+    assert rel(g.coeffs(), o.coeffs()) < 1e-6
+    assert abs(g.volume() - o.volume()) < 1e-7

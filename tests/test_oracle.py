@@ -374,3 +374,42 @@ def test_mma_toy_problem_reaches_kkt_point():
                    tol=1e-12).x
     assert np.allclose(x, ref, atol=2e-3)
     assert np.allclose(ref, [2.0175, 1.7800, 1.2378], atol=1e-3)
+
+
+def test_oc_on_spline_coefficients_hits_volume():
+    solid, support, f = cantilever(32, 16, 16)
+    ctx = Context(oracle_api(), solid, support)
+    ctx.set_material(E, NU, P, RMIN)
+    ctx.set_loads(f)
+    n = ctx.spline_init(1, 4, 4)
+    c0 = np.full(n, 0.5, np.float32)
+    ctx.set_coeffs(c0)
+    ctx.spline_eval()
+    ctx.update_operator()
+    ctx.solve(rtol=1e-8)
+    ctx.sensitivity()
+    ctx.filter(2.0)
+    ctx.spline_adjoint(0)
+    g, steps = ctx.oc_update_coeffs(0.45, move=0.1, damp=0.5)
+    c1 = ctx.coeffs()
+    assert np.all(np.abs(c1 - c0) <= 0.1 + 1e-6)
+    assert abs(ctx.volume() - 0.45) <= 1e-4 or steps == 31
+    # the densities are the spline of the final coefficients
+    rho = ctx.density()
+    ctx.set_coeffs(c1)
+    ctx.spline_eval()
+    assert np.array_equal(rho, ctx.density())
+
+
+def test_singular_and_nan_inputs_are_reported():
+    solid, support, f = cantilever(16, 8, 8)
+    c = Context(oracle_api(), solid, np.zeros_like(support))
+    c.set_material(E, NU, P, RMIN)
+    c.set_loads(f)
+    c.set_density(np.full(solid.size, 0.5, np.float32))
+    with pytest.raises(Exception):
+        c.update_operator()
+    bad = np.full(solid.size, 0.5, np.float32)
+    bad[0] = np.inf
+    with pytest.raises(Exception):
+        c.set_density(bad)

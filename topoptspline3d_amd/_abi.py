@@ -82,6 +82,7 @@ SIGNATURES = {
     "spline_eval": (C.c_int, [vp]),
     "spline_adjoint": (C.c_int, [vp, C.c_int]),
     "spline_get_coeff_sens": (C.c_int, [vp, C.c_int, f32p]),
+    "oc_update_coeffs": (C.c_int, [vp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, f64p, i32p]),
     "mma_create": (C.c_int, [C.c_int, C.c_int, f64p, f64p, C.c_double, vpp]),
     "mma_destroy": (C.c_int, [vp]),
     "mma_set_x": (C.c_int, [vp, f64p]),

@@ -230,6 +230,13 @@ class Context:
     def spline_adjoint(self, which=0):
         self._chk(self.api.spline_adjoint(self.h, which), "spline_adjoint")
 
+    def oc_update_coeffs(self, vgoal, move=0.03, damp=0.5, cmin=1e-3, cmax=1.0):
+        g = C.c_double()
+        n = C.c_int()
+        self._chk(self.api.oc_update_coeffs(self.h, vgoal, move, damp, cmin, cmax, C.byref(g), C.byref(n)),
+                  "oc_update_coeffs")
+        return g.value, n.value
+
     def coeff_sens(self, which=0):
         c = np.zeros(self.n_coeffs, np.float32)
         self._chk(self.api.spline_get_coeff_sens(self.h, which, _p(c, C.c_float)), "spline_get_coeff_sens")

@@ -344,6 +344,29 @@ __global__ void k_spline_stage(const double* __restrict__ in, double* __restrict
   out[t] = acc;
 }
 
+// OC on the coefficients (tryCSensMultiplier_kernel optimization/optimization.cu:559-587)
+__global__ __launch_bounds__(256) void k_coef_absmax_partial(int64_t n, const float* __restrict__ s,
+                                                             double* __restrict__ partial) {
+  double m = 0;
+  for (int64_t i = tid64(); i < n; i += (int64_t)gridDim.x * blockDim.x) m = fmax(m, fabs((double)s[i]));
+  m = block_max256(m);
+  if (threadIdx.x == 0) partial[blockIdx.x] = m;
+}
+
+__global__ void k_coef_trial(int64_t n, const float* __restrict__ old, const float* __restrict__ s, double move,
+                             double damp, double cmin, double cmax, const double* __restrict__ scal,
+                             float* __restrict__ coef) {
+  const int64_t i = tid64();
+  if (i >= n || scal[S_OC_DONE] != 0.0) return;
+  const double g = scal[S_OC_G];
+  const double gg = s[i] > 0 ? 0.0 : -(double)s[i];
+  const double c0 = old[i];
+  double cn = c0 * pow(gg / g, damp);
+  cn = fmin(fmax(cn, c0 - move), c0 + move);
+  cn = fmin(fmax(cn, cmin), cmax);
+  coef[i] = (float)cn;
+}
+
 __global__ void k_d2f(const double* __restrict__ a, float* __restrict__ b, int64_t n) {
   const int64_t i = tid64();
   if (i < n) b[i] = (float)a[i];

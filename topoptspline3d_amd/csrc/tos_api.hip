@@ -1564,6 +1564,7 @@ TOS_EXPORT int tos_spline_init(tos_ctx* c, int mode, int order, int param) {
     }
     const size_t n = (size_t)c->nc[0] * c->nc[1] * c->nc[2];
     c->coef = dalloc<float>(*c, n);
+    c->coef_old = dalloc<float>(*c, n);
     c->csens = dalloc<float>(*c, n);
     c->cvsens = dalloc<float>(*c, n);
     c->sp_g = dalloc<double>(*c, std::max((size_t)c->nx * c->ny * c->nz, n));  // also holds the last stage
@@ -1590,19 +1591,52 @@ TOS_EXPORT int tos_spline_get_coeffs(tos_ctx* c, float* v) {
   });
 }
 
+static void spline_eval_launch(tos_ctx* c) {
+  const LevelView& v = c->L[0].v;
+  const int64_t n = (int64_t)v.ex * v.ey * v.ez;
+  hipLaunchKernelGGL(k_spline_eval, dim3(nblk(n)), dim3(256), 0, c->stream, v, (const uint8_t*)c->eact,
+                     (const uint8_t*)c->shell, (const float*)c->coef, c->nc[0], c->nc[1], c->sp_order,
+                     (const int*)c->sp_idx[0], (const float*)c->sp_wt[0], (const uint8_t*)c->sp_ok[0],
+                     (const int*)c->sp_idx[1], (const float*)c->sp_wt[1], (const uint8_t*)c->sp_ok[1],
+                     (const int*)c->sp_idx[2], (const float*)c->sp_wt[2], (const uint8_t*)c->sp_ok[2], c->rho_min,
+                     c->rho);
+  LAUNCH_CHECK();
+}
+
 TOS_EXPORT int tos_spline_eval(tos_ctx* c) {
   return guard(c, [&]() {
     REQUIRE(c->sp_mode >= 0, "spline not initialised");
-    const LevelView& v = c->L[0].v;
-    const int64_t n = (int64_t)v.ex * v.ey * v.ez;
     Prof pr(*c, P_SPLINE);
-    hipLaunchKernelGGL(k_spline_eval, dim3(nblk(n)), dim3(256), 0, c->stream, v, (const uint8_t*)c->eact,
-                       (const uint8_t*)c->shell, (const float*)c->coef, c->nc[0], c->nc[1], c->sp_order,
-                       (const int*)c->sp_idx[0], (const float*)c->sp_wt[0], (const uint8_t*)c->sp_ok[0],
-                       (const int*)c->sp_idx[1], (const float*)c->sp_wt[1], (const uint8_t*)c->sp_ok[1],
-                       (const int*)c->sp_idx[2], (const float*)c->sp_wt[2], (const uint8_t*)c->sp_ok[2], c->rho_min,
-                       c->rho);
+    spline_eval_launch(c);
+    c->op_ready = false;
+    return TOS_OK;
+  });
+}
+
+TOS_EXPORT int tos_oc_update_coeffs(tos_ctx* c, double vgoal, double move, double damp, double cmin, double cmax,
+                                    double* g_out, int* steps) {
+  return guard(c, [&]() {
+    REQUIRE(c->sp_mode >= 0 && move > 0 && damp > 0 && cmax > cmin, "bad coefficient OC request");
+    Prof pr(*c, P_OPT);
+    const LevelView& v = c->L[0].v;
+    const int64_t n = (int64_t)c->nc[0] * c->nc[1] * c->nc[2];
+    HIPCHK(hipMemcpyAsync(c->coef_old, c->coef, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    hipLaunchKernelGGL(k_coef_absmax_partial, dim3(kRedBlocks), dim3(256), 0, c->stream, n, (const float*)c->csens,
+                       c->partial);
+    hipLaunchKernelGGL(k_oc_init, dim3(1), dim3(256), 0, c->stream, (const double*)c->partial, kRedBlocks, c->scal);
+    for (int it = 0; it < 31; ++it) {
+      hipLaunchKernelGGL(k_coef_trial, dim3(nblk(n)), dim3(256), 0, c->stream, n, (const float*)c->coef_old,
+                         (const float*)c->csens, move, damp, cmin, cmax, (const double*)c->scal, c->coef);
+      spline_eval_launch(c);
+      hipLaunchKernelGGL(k_sum_active, dim3(kRedBlocks), dim3(256), 0, c->stream, v, (const uint8_t*)c->eact,
+                         (const float*)c->rho, c->partial);
+      hipLaunchKernelGGL(k_oc_decide, dim3(1), dim3(256), 0, c->stream, (const double*)c->partial, kRedBlocks,
+                         (double)c->n_active, vgoal, c->scal);
+    }
     LAUNCH_CHECK();
+    const double* hs = read_scal(*c);
+    if (g_out) *g_out = hs[S_OC_G];
+    if (steps) *steps = (int)hs[S_OC_STEPS];
     c->op_ready = false;
     return TOS_OK;
   });

@@ -119,6 +119,7 @@ struct Ctx {
   int* sp_celm[3] = {nullptr, nullptr, nullptr};
   float* sp_cw[3] = {nullptr, nullptr, nullptr};
   float* coef = nullptr;
+  float* coef_old = nullptr;
   float* csens = nullptr;
   float* cvsens = nullptr;
   double* sp_t1 = nullptr;  // separable adjoint stages
