@@ -130,22 +130,26 @@ def kernel_stats(api, ctx, enable):
 
 
 def roofline(stats, info, n_active_nodes, steps):
-    """Roofline of the dominant kernel class.  Algorithmic bytes per launch (each array
-    read or written once at storage precision, index/flag re-reads excluded):
+    """Roofline of the dominant kernel class (by total time among the classes with a
+    single-level definition).  Algorithmic bytes per launch (each array read or written
+    once at storage precision, index/flag re-reads excluded):
       fine GS colour pass (FP32): one sweep = 4 N_e + (9*4 + 4) N_v, one launch = 1/8
       fine residual (FP32/FP64):  4 N_e + (9 s + 4) N_v
-      fine operator (FP64):       4 N_e + (6*8 + 4) N_v"""
+      fine operator (FP64):       4 N_e + (6*8 + 4) N_v
+      level-1 stencil GS colour pass (FP32): (243*4 + 9*4) N_c1 / 8"""
     Ne, Nv = info["n_active_elements"][0], n_active_nodes
+    Nc1 = info["n_free_nodes"][1] if info["n_levels"] > 1 else 0
     per_launch = {
         "fine_gs_f32": (4 * Ne + (9 * 4 + 4) * Nv) / 8.0,
         "fine_gs_f64": (4 * Ne + (9 * 8 + 4) * Nv) / 8.0,
         "fine_resid_f32": 4 * Ne + (9 * 4 + 4) * Nv,
         "fine_resid_f64": 4 * Ne + (9 * 8 + 4) * Nv,
         "fine_apply_f64": 4 * Ne + (6 * 8 + 4) * Nv,
+        "st_gs_f32_l1": (243 * 4 + 9 * 4) * Nc1 / 8.0,
     }
     flops = {"fine_gs_f32": 1170 * Nv / 8.0, "fine_gs_f64": 1170 * Nv / 8.0, "fine_resid_f32": 1152 * Nv,
-             "fine_resid_f64": 1152 * Nv, "fine_apply_f64": 1152 * Nv}
-    dom = max(stats, key=lambda k: stats[k][0])
+             "fine_resid_f64": 1152 * Nv, "fine_apply_f64": 1152 * Nv, "st_gs_f32_l1": 504 * Nc1 / 8.0}
+    dom = max((k for k in stats if k in per_launch), key=lambda k: stats[k][0])
     tot = {k: v[0] for k, v in stats.items() if v[1] > 0}
     out = {"bound": "hbm", "peak": HBM_PEAK, "unit": "GB/s", "traffic": None, "kernel": dom,
            "kernel_ms_per_step": {k: round(v / steps, 3) for k, v in tot.items()}}

@@ -206,7 +206,7 @@ int tos_ctx_mma_step(tos_ctx* ctx, tos_mma* mma, double sens_scale, double vol_s
  * the solver kernels.  enable = 1 clears and starts recording; enable = 0 stops, waits for
  * the stream and fills `out` (may be NULL).  No counterpart in the reference (its
  * tictoc timers, utils/tictoc.h:35-83, time host phases only). */
-#define TOS_PROF_CLASSES 16
+#define TOS_PROF_CLASSES 20
 typedef struct tos_kernel_stats {
   int n_classes;
   char names[TOS_PROF_CLASSES][32];

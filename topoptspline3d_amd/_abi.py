@@ -38,8 +38,8 @@ class SolveStats(C.Structure):
 
 
 class KernelStats(C.Structure):
-    _fields_ = [("n_classes", C.c_int), ("names", (C.c_char * 32) * 16),
-                ("total_ms", C.c_double * 16), ("launches", C.c_int64 * 16)]
+    _fields_ = [("n_classes", C.c_int), ("names", (C.c_char * 32) * 20),
+                ("total_ms", C.c_double * 20), ("launches", C.c_int64 * 20)]
 
 
 vp = C.c_void_p

@@ -330,12 +330,14 @@ const double* read_scal(Ctx& c) {
 enum {
   P_FINE_APPLY64 = 0, P_FINE_RESID64, P_FINE_RESID32, P_FINE_GS64, P_FINE_GS32, P_ST_GS64, P_ST_GS32,
   P_ST_RESID, P_TRANSFER, P_COARSE, P_GALERKIN, P_COARSE_FACTOR, P_VECTOR, P_SENS_FILTER, P_SPLINE, P_OPT,
-  P_COUNT
+  P_ST_GS32_L1, P_COUNT
 };
+static_assert(P_COUNT <= TOS_PROF_CLASSES, "timing classes");
 const char* kProfNames[P_COUNT] = {"fine_apply_f64", "fine_resid_f64", "fine_resid_f32", "fine_gs_f64",
                                    "fine_gs_f32",    "st_gs_f64",      "st_gs_f32",      "st_resid",
                                    "transfer",       "coarse_solve",   "galerkin",       "coarse_factor",
-                                   "vector",         "sens_filter",    "spline",         "oc_mma"};
+                                   "vector",         "sens_filter",    "spline",         "oc_mma",
+                                   "st_gs_f32_l1"};
 
 hipEvent_t next_event(Ctx& c) {
   if (c.ev_used == c.ev_pool.size()) {
@@ -414,7 +416,8 @@ void gs_sweep(Ctx& c, int l, T* u, const T* f, bool backward) {
     const int off = L.col_off[col], nch = L.col_off[col + 1] - off;
     if (nch == 0) continue;
     const int4* cl = L.cl_col + off;
-    Prof pr(c, l == 0 ? (sizeof(T) == 8 ? P_FINE_GS64 : P_FINE_GS32) : (sizeof(T) == 8 ? P_ST_GS64 : P_ST_GS32));
+    Prof pr(c, l == 0 ? (sizeof(T) == 8 ? P_FINE_GS64 : P_FINE_GS32)
+                      : (sizeof(T) == 8 ? P_ST_GS64 : (l == 1 ? P_ST_GS32_L1 : P_ST_GS32)));
     if (l == 0) {
       if (backward)
         hipLaunchKernelGGL((k_fine_gs<T, true>), dim3(cblk(nch)), dim3(256), 0, c.stream, v, cl, nch, Prec<T>::w(c),
