@@ -47,6 +47,26 @@ __device__ __forceinline__ double block_max256(double v) {
 // Sum `np` partials (two interleaved sets when two != 0) into scal[slot0], scal[slot1],
 // then apply a small scalar program `op`.
 enum { OP_NONE = 0, OP_ALPHA, OP_BETA, OP_RN2 };
+// first stage of a two-stage reduction of np (x2) block partials: block b sums a fixed
+// contiguous segment into out[b] (and out[kStage + b]); deterministic order.
+constexpr int kStage = 256;
+__global__ __launch_bounds__(256) void k_reduce_partials(const double* __restrict__ partial, int np, int two,
+                                                         double* __restrict__ out) {
+  const int seg = (np + kStage - 1) / kStage;
+  const int i0 = blockIdx.x * seg, i1 = min(np, i0 + seg);
+  double s0 = 0, s1 = 0;
+  for (int i = i0 + (int)threadIdx.x; i < i1; i += 256) {
+    s0 += partial[i];
+    if (two) s1 += partial[np + i];
+  }
+  s0 = block_sum256(s0);
+  s1 = block_sum256(s1);
+  if (threadIdx.x == 0) {
+    out[blockIdx.x] = s0;
+    out[kStage + blockIdx.x] = s1;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_finish(const double* __restrict__ partial, int np, int two, int slot0,
                                                 int slot1, int op, double* __restrict__ scal) {
   double s0 = 0, s1 = 0;

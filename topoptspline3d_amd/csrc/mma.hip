@@ -220,18 +220,26 @@ __global__ __launch_bounds__(256) void k_mma_asym(MmaVecs V, MmaSmall S, double*
   block_partials<1>(v, mx, partial);
 }
 
-// fold `cnt` columns of `nb` per-block partials in a fixed order
-__global__ __launch_bounds__(256) void k_mma_fold(const double* __restrict__ partial, int nb, int cnt,
-                                                  unsigned maxmask, double* __restrict__ out) {
-  const int c = threadIdx.x;
+// fold `cnt` columns of `nb` per-block partials in a fixed order: one 64-lane wave per
+// column (grid = cnt), lane-strided partial sums, then a fixed butterfly
+__global__ __launch_bounds__(64) void k_mma_fold(const double* __restrict__ partial, int nb, int cnt,
+                                                 unsigned maxmask, double* __restrict__ out) {
+  const int c = blockIdx.x;
   if (c >= cnt) return;
   const bool mx = (maxmask >> c) & 1u;
-  double s = partial[(int64_t)c * nb];
-  for (int b = 1; b < nb; ++b) {
-    const double v = partial[(int64_t)c * nb + b];
+  const int lane = threadIdx.x;
+  const double* col = partial + (int64_t)c * nb;
+  double s = lane < nb ? col[lane] : (mx ? -HUGE_VAL : 0.0);
+  for (int b = lane + 64; b < nb; b += 64) {
+    const double v = col[b];
     s = mx ? fmax(s, v) : s + v;
   }
-  out[c] = s;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const double v = __shfl_xor(s, o, 64);
+    s = mx ? fmax(s, v) : s + v;
+  }
+  if (lane == 0) out[c] = s;
 }
 
 }  // namespace tos

@@ -368,7 +368,16 @@ struct Prof {
 };
 
 // finisher over the partials of a launch over the fine chunk list
+// (two stages when there are many blocks: one long single-block loop is latency bound)
 void finish0(Ctx& c, int two, int s0, int s1, int op) {
+  if (c.nb0 > 4 * kStage) {
+    double* st = c.partial + 2 * (size_t)std::max(kRedBlocks, c.nb0);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(kStage), dim3(256), 0, c.stream, (const double*)c.partial, c.nb0,
+                       two, st);
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, c.stream, (const double*)st, kStage, two, s0, s1, op,
+                       c.scal);
+    return;
+  }
   hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, c.stream, (const double*)c.partial, c.nb0, two, s0, s1, op,
                      c.scal);
 }
@@ -783,7 +792,7 @@ T* malloc_mma(tos_mma* M, size_t n) {
 template <class K, class... A>
 const double* mma_pass(tos_mma* M, hipStream_t st, K kernel, int cnt, unsigned maxmask, A... args) {
   hipLaunchKernelGGL(kernel, dim3(M->nb), dim3(256), 0, st, args...);
-  hipLaunchKernelGGL(k_mma_fold, dim3(1), dim3(256), 0, st, (const double*)M->partial, M->nb, cnt, maxmask, M->red);
+  hipLaunchKernelGGL(k_mma_fold, dim3(cnt), dim3(64), 0, st, (const double*)M->partial, M->nb, cnt, maxmask, M->red);
   LAUNCH_CHECK();
   download(st, M->h_red, M->red, cnt * sizeof(double));
   return M->h_red;
@@ -1050,7 +1059,7 @@ TOS_EXPORT int tos_ctx_create(const tos_grid_desc* d, tos_ctx** out) {
     c->p = dalloc<double>(*c, 3 * v0.NP);
     c->q = dalloc<double>(*c, 3 * v0.NP);
     c->nb0 = (int)cblk(c->L[0].n_lin);
-    c->partial = dalloc<double>(*c, 2 * (size_t)std::max(kRedBlocks, c->nb0));
+    c->partial = dalloc<double>(*c, 2 * (size_t)std::max(kRedBlocks, c->nb0) + 2 * kStage);
     c->scal = dalloc<double>(*c, S_COUNT);
     HIPCHK(hipHostMalloc((void**)&c->h_scal, S_COUNT * sizeof(double)));
     c->K = 1;

@@ -95,7 +95,7 @@ struct Ctx {
   // PCG work vectors (FP64, 3*NP)
   double *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
   // reductions
-  double* partial = nullptr;    // 2 * max(kRedBlocks, blocks of the fine chunk list)
+  double* partial = nullptr;    // 2 * max(kRedBlocks, blocks of the fine chunk list) + 2 * 256 stage
   int nb0 = 0;                  // blocks of a launch over the fine chunk list
   double* scal = nullptr;       // device scalars
   double* h_scal = nullptr;     // pinned host mirror
