@@ -215,6 +215,51 @@ typedef struct tos_kernel_stats {
 } tos_kernel_stats;
 int tos_profile(tos_ctx* ctx, int enable, tos_kernel_stats* out);
 
+/* ---- learned coarse-level correction --------------------------------------------------
+ * No counterpart in the reference: a 3-level 3-D convolutional U-Net (16/32/64 channels,
+ * 3x3x3 kernels, BF16 weights on MFMA, FP32 accumulation) that maps the residual of a
+ * multigrid level (3 channels, scaled by the inverse square root of the Galerkin
+ * diagonal and by its RMS) plus the nodal log-density (1 channel) to an error correction
+ * (3 channels) on that level's node box.  Trained with these entry points on
+ * residual / error pairs logged from the solver's own V-cycles.
+ * Inputs of tos_unet_forward / tos_unet_loss_grad: feat[B][Z][Y][X][4] (x fastest),
+ * target[B][Z][Y][X][3], mask[B][Z][Y][X]; loss = 1/B sum_b |m (y - t)|^2 / |m t|^2.
+ * Parameters: per layer W[27][cin][cout] (tap = (dx+1) + 3(dy+1) + 9(dz+1)) then
+ * b[cout], layers in the order enc0a, enc0b, enc1a, enc1b, enc2a, enc2b, dec1a, dec1b,
+ * dec0a, dec0b, out. */
+typedef struct tos_unet tos_unet;
+typedef struct tos_dataset tos_dataset;
+int tos_unet_create(int c0, int c1, int c2, uint64_t seed, tos_unet** out);  /* 16, 32, 64 */
+int tos_unet_destroy(tos_unet* net);
+const char* tos_unet_last_error(const tos_unet* net);
+int tos_unet_num_params(const tos_unet* net, int64_t* n);
+int tos_unet_get_params(tos_unet* net, float* out);
+int tos_unet_set_params(tos_unet* net, const float* in);  /* also resets the Adam state */
+int tos_unet_forward(tos_unet* net, int B, int X, int Y, int Z, const float* feat, float* out);
+int tos_unet_loss_grad(tos_unet* net, int B, int X, int Y, int Z, const float* feat,
+                       const float* target, const uint8_t* mask, double* loss, float* grad);
+/* device-resident pair dataset on a fixed node box */
+int tos_dataset_create(int X, int Y, int Z, int capacity, tos_dataset** out);
+int tos_dataset_destroy(tos_dataset* ds);
+int tos_dataset_size(const tos_dataset* ds, int* n);
+int tos_dataset_get(tos_dataset* ds, int i, float* feat4, float* target, uint8_t* mask);
+/* Adam (beta 0.9 / 0.999) on random batches of samples [first, first + count) */
+int tos_unet_train(tos_unet* net, tos_dataset* ds, int first, int count, int steps, int batch,
+                   double lr, uint64_t seed, double* loss_first, double* loss_last);
+int tos_unet_eval(tos_unet* net, tos_dataset* ds, int first, int count, double* mean_loss);
+/* mode 1: inside FP32 V-cycles the network corrects level `level` after its coarse-grid
+ * correction; the correction is kept only when the scaled residual falls below theta
+ * times its value before (otherwise plain multigrid continues unchanged).  mode 0
+ * detaches.  The context keeps the pointer: the network must outlive the attachment. */
+int tos_ctx_learn_attach(tos_ctx* ctx, tos_unet* net, int level, int mode, double theta);
+/* log (features, target) pairs at `level` of every `stride`-th FP32 V-cycle into ds (grid =
+ * the level's node box); target = the level's error after `n_inner` nested V-cycles.
+ * ds = NULL stops logging. */
+int tos_ctx_learn_log(tos_ctx* ctx, tos_dataset* ds, int level, int n_inner, int stride);
+/* corrections tried / kept since the last attach, and the mean over the tries of
+ * |D^{-1/2}(r - A e)| / |D^{-1/2} r| (below 1: the network's proposal lowered the residual) */
+int tos_ctx_learn_stats(tos_ctx* ctx, int* tried, int* accepted, double* mean_ratio);
+
 #ifdef __cplusplus
 }
 #endif

@@ -92,13 +92,36 @@ SIGNATURES = {
     "profile": (C.c_int, [vp, C.c_int, C.POINTER(KernelStats)]),
 }
 
+# learned coarse-level correction (HIP library only; the network oracle is numpy)
+i64p = C.POINTER(C.c_int64)
+LEARN_SIGNATURES = {
+    "unet_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_uint64, vpp]),
+    "unet_destroy": (C.c_int, [vp]),
+    "unet_last_error": (C.c_char_p, [vp]),
+    "unet_num_params": (C.c_int, [vp, i64p]),
+    "unet_get_params": (C.c_int, [vp, f32p]),
+    "unet_set_params": (C.c_int, [vp, f32p]),
+    "unet_forward": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, f32p, f32p]),
+    "unet_loss_grad": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, f32p, f32p, u8p, f64p, f32p]),
+    "dataset_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, vpp]),
+    "dataset_destroy": (C.c_int, [vp]),
+    "dataset_size": (C.c_int, [vp, i32p]),
+    "dataset_get": (C.c_int, [vp, C.c_int, f32p, f32p, u8p]),
+    "unet_train": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_uint64, f64p, f64p]),
+    "unet_eval": (C.c_int, [vp, vp, C.c_int, C.c_int, f64p]),
+    "ctx_learn_attach": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_double]),
+    "ctx_learn_log": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_int]),
+    "ctx_learn_stats": (C.c_int, [vp, i32p, i32p, f64p]),
+}
+
 ERRORS = {0: "ok", -1: "bad argument", -2: "HIP error", -3: "not converged",
           -4: "NaN/Inf", -5: "singular coarsest system", -6: "call order"}
 
 
-def bind(lib, prefix):
+def bind(lib, prefix, learned=False):
     ns = types.SimpleNamespace()
-    for name, (res, args) in SIGNATURES.items():
+    table = dict(SIGNATURES, **LEARN_SIGNATURES) if learned else SIGNATURES
+    for name, (res, args) in table.items():
         fn = getattr(lib, prefix + name)
         fn.restype = res
         fn.argtypes = args
@@ -108,5 +131,6 @@ def bind(lib, prefix):
     return ns
 
 
-def exported_names(prefix):
-    return [prefix + n for n in SIGNATURES]
+def exported_names(prefix, learned=True):
+    names = list(SIGNATURES) + (list(LEARN_SIGNATURES) if learned else [])
+    return [prefix + n for n in names]

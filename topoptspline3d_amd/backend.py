@@ -17,7 +17,7 @@ def hip_api():
     if _api is None:
         if not os.path.exists(LIB):
             raise RuntimeError(f"{LIB} is missing: build it with `python -m topoptspline3d_amd.build`")
-        _api = _abi.bind(ctypes.CDLL(LIB), "tos_")
+        _api = _abi.bind(ctypes.CDLL(LIB), "tos_", learned=True)
     return _api
 
 

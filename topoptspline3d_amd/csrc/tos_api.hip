@@ -10,10 +10,13 @@
 #include <cstring>
 #include <functional>
 #include <stdexcept>
+#include <string>
+#include <type_traits>
 
 #include "tos_internal.hpp"
 
 #include "kernels_mg.hip"
+#include "kernels_nn.hip"
 #include "kernels_opt.hip"
 #include "mma.hip"
 
@@ -72,6 +75,12 @@ T* dalloc(Ctx& c, size_t n) {
 }
 
 inline unsigned nblk(int64_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+}  // namespace
+
+#include "unet_host.hpp"
+
+namespace {
 
 // ---------------------------------------------------------------------------------------
 // Element stiffness of the 8-node trilinear hex (element/templateMatrix.cpp:31-99):
@@ -330,14 +339,14 @@ const double* read_scal(Ctx& c) {
 enum {
   P_FINE_APPLY64 = 0, P_FINE_RESID64, P_FINE_RESID32, P_FINE_GS64, P_FINE_GS32, P_ST_GS64, P_ST_GS32,
   P_ST_RESID, P_TRANSFER, P_COARSE, P_GALERKIN, P_COARSE_FACTOR, P_VECTOR, P_SENS_FILTER, P_SPLINE, P_OPT,
-  P_ST_GS32_L1, P_COUNT
+  P_ST_GS32_L1, P_LEARNED, P_COUNT
 };
 static_assert(P_COUNT <= TOS_PROF_CLASSES, "timing classes");
 const char* kProfNames[P_COUNT] = {"fine_apply_f64", "fine_resid_f64", "fine_resid_f32", "fine_gs_f64",
                                    "fine_gs_f32",    "st_gs_f64",      "st_gs_f32",      "st_resid",
                                    "transfer",       "coarse_solve",   "galerkin",       "coarse_factor",
                                    "vector",         "sens_filter",    "spline",         "oc_mma",
-                                   "st_gs_f32_l1"};
+                                   "st_gs_f32_l1",   "learned"};
 
 hipEvent_t next_event(Ctx& c) {
   if (c.ev_used == c.ev_pool.size()) {
@@ -464,6 +473,9 @@ void apply_level(Ctx& c, int l, const T* u, const T* f, T* y, double* partial = 
   LAUNCH_CHECK();
 }
 
+void learned_step(Ctx& c, int l, const float* f, float* u, int pre, int post);
+bool learned_here(const Ctx& c, int l);
+
 template <class T>
 void vcycle_level(Ctx& c, int l, const T* f, T* u, int pre, int post) {
   Level& L = c.L[l];
@@ -500,6 +512,8 @@ void vcycle_level(Ctx& c, int l, const T* f, T* u, int pre, int post) {
                        (const int4*)L.cl_lin, L.n_lin, (const T*)Prec<T>::u(C), u);
   }
   LAUNCH_CHECK();
+  if constexpr (std::is_same<T, float>::value)
+    if (learned_here(c, l)) learned_step(c, l, f, u, pre, post);
   for (int s = 0; s < post; ++s) gs_sweep<T>(c, l, u, f, true);
 }
 
@@ -519,6 +533,149 @@ void precondition(Ctx& c, int precision, int pre, int post, bool have_r32, bool 
   if (want_z64)
     hipLaunchKernelGGL((k_cast_nodes<float, double>), dim3(cblk(L0.n_lin)), dim3(256), 0, c.stream,
                        (const int4*)L0.cl_lin, L0.n_lin, L0.v.NP, (const float*)L0.u32, c.z);
+  LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------------------
+// Learned coarse-level correction (no counterpart in the reference).  At level `level` of
+// an FP32 V-cycle, after the coarse-grid correction has been prolongated and before the
+// post-smoothing, the residual r = f - A u is formed and either
+//   logged: the pair (features of r, target e = A^{-1} r from `n_inner` nested V-cycles
+//           on this level) is appended to a device dataset, or
+//   used:   the network proposes e; it is kept (u += e) only when the scaled residual
+//           |D^{-1/2}(r - A e)| drops below theta |D^{-1/2} r|, so plain multigrid takes
+//           over whenever the network does not help.  The decision stays on the device.
+// ---------------------------------------------------------------------------------------
+}  // namespace
+
+namespace tos {
+struct LearnState {
+  tos_unet* net = nullptr;
+  int level = -1;
+  int mode = 0;
+  double theta = 1.0;
+  tos_dataset* ds = nullptr;
+  int n_inner = 8, stride = 1;
+  int64_t calls = 0;
+  bool busy = false;
+  float* lrho = nullptr;
+  float *e = nullptr, *t1 = nullptr, *t2 = nullptr, *t3 = nullptr;
+  int* counters = nullptr;
+  double* ratio_sum = nullptr;
+  NetActs* acts = nullptr;
+  std::vector<void*> mem;
+};
+}  // namespace tos
+
+namespace {
+
+bool learned_here(const Ctx& c, int l) {
+  const LearnState* S = c.learn;
+  return S && l == S->level && !S->busy && ((S->mode == 1 && S->net) || S->ds);
+}
+
+void learn_logrho(Ctx& c) {
+  LearnState& S = *c.learn;
+  const Level& L = c.L[S.level];
+  const int64_t V = (int64_t)L.v.nx * L.v.ny * L.v.nz;
+  hipLaunchKernelGGL(k_node_logrho, dim3(nblk(V)), dim3(256), 0, c.stream, c.L[0].v, (const uint8_t*)c.eact,
+                     (const float*)c.rho, c.nx, c.ny, c.nz, 1 << S.level, (float)c.rho_min, L.v.nx, L.v.ny, L.v.nz,
+                     S.lrho);
+  LAUNCH_CHECK();
+}
+
+// set up the per-level buffers once the level is known
+void learn_prepare(Ctx& c, int level) {
+  if (!c.learn) c.learn = new LearnState();
+  LearnState& S = *c.learn;
+  if (S.level == level) return;
+  for (void* p : S.mem) (void)hipFree(p);
+  S.mem.clear();
+  S.acts = nullptr;
+  S.level = level;
+  const Level& L = c.L[level];
+  const int64_t V = (int64_t)L.v.nx * L.v.ny * L.v.nz;
+  S.lrho = (float*)raw_alloc(S.mem, c.stream, V * sizeof(float));
+  S.e = (float*)raw_alloc(S.mem, c.stream, 3 * L.v.NP * sizeof(float));
+  S.t1 = (float*)raw_alloc(S.mem, c.stream, 3 * L.v.NP * sizeof(float));
+  S.t2 = (float*)raw_alloc(S.mem, c.stream, 3 * L.v.NP * sizeof(float));
+  S.t3 = (float*)raw_alloc(S.mem, c.stream, 3 * L.v.NP * sizeof(float));
+  S.counters = (int*)raw_alloc(S.mem, c.stream, 2 * sizeof(int));
+  S.ratio_sum = (double*)raw_alloc(S.mem, c.stream, sizeof(double));
+  learn_logrho(c);
+  HIPCHK(hipStreamSynchronize(c.stream));
+}
+
+void learn_release(Ctx& c) {
+  if (!c.learn) return;
+  for (void* p : c.learn->mem) (void)hipFree(p);
+  delete c.learn;
+  c.learn = nullptr;
+}
+
+// scaled squared norm sum_c r_c^2 / D_c over the free nodes of level l -> scal[slot]
+void scaled_norm(Ctx& c, int l, const float* r, int slot) {
+  Level& L = c.L[l];
+  const unsigned nb = cblk(L.n_lin);
+  hipLaunchKernelGGL(k_nn_norm, dim3(nb), dim3(256), 0, c.stream, L.v, (const int4*)L.cl_lin, L.n_lin, r,
+                     (const float*)L.st32, L.NS, (const int*)L.slot, c.partial);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, c.stream, (const double*)c.partial, (int)nb,
+                     c.scal + slot);
+  LAUNCH_CHECK();
+}
+
+void learned_step(Ctx& c, int l, const float* f, float* u, int pre, int post) {
+  LearnState& S = *c.learn;
+  Level& L = c.L[l];
+  const int64_t V = (int64_t)L.v.nx * L.v.ny * L.v.nz;
+  const double nfree = (double)std::max<int64_t>(L.n_free, 1);
+  float* r = L.r32;
+  apply_level<float, 1>(c, l, u, f, r);
+  scaled_norm(c, l, r, S_NN0);
+  tos_dataset* ds = S.ds;
+  if (ds && ds->n < ds->cap && (S.calls++ % S.stride) == 0) {
+    // target e = A^{-1} r by nested V-cycles on this level (u, f of the running cycle untouched)
+    const int64_t n3 = 3 * L.v.NP;
+    HIPCHK(hipMemcpyAsync(S.t1, r, n3 * sizeof(float), hipMemcpyDeviceToDevice, c.stream));
+    hipLaunchKernelGGL(k_zero<float>, dim3(nblk(n3)), dim3(256), 0, c.stream, S.e, n3);
+    S.busy = true;
+    for (int it = 0; it < S.n_inner; ++it) {
+      apply_level<float, 1>(c, l, S.e, S.t1, S.t2);
+      vcycle_level<float>(c, l, S.t2, S.t3, pre, post);
+      hipLaunchKernelGGL(k_add_nodes_f, dim3(cblk(L.n_lin)), dim3(256), 0, c.stream, (const int4*)L.cl_lin, L.n_lin,
+                         L.v.NP, S.e, (const float*)S.t3);
+    }
+    S.busy = false;
+    const int i = ds->n;
+    hipLaunchKernelGGL(k_nn_features, dim3(nblk(V)), dim3(256), 0, c.stream, L.v, (const float*)S.t1,
+                       (const float*)L.st32, L.NS, (const int*)L.slot, (const float*)S.lrho,
+                       (const double*)(c.scal + S_NN0), nfree, ds->feat + (int64_t)i * V * 8);
+    hipLaunchKernelGGL(k_nn_target, dim3(nblk(V)), dim3(256), 0, c.stream, L.v, (const float*)S.e,
+                       (const float*)L.st32, L.NS, (const int*)L.slot, (const double*)(c.scal + S_NN0), nfree,
+                       ds->tgt + (int64_t)i * V * 3, ds->mask + (int64_t)i * V);
+    hipLaunchKernelGGL(k_target_norm, dim3(256), dim3(256), 0, c.stream, (const float*)(ds->tgt + (int64_t)i * V * 3),
+                       (const uint8_t*)(ds->mask + (int64_t)i * V), V, c.partial);
+    hipLaunchKernelGGL(k_inv_den, dim3(1), dim3(256), 0, c.stream, (const double*)c.partial, 256, ds->inv_den + i);
+    LAUNCH_CHECK();
+    ds->n += 1;
+    return;
+  }
+  if (S.mode != 1 || !S.net) return;
+  {
+    Prof pr(c, P_LEARNED);
+    NetActs* A = S.acts;
+    hipLaunchKernelGGL(k_nn_features, dim3(nblk(V)), dim3(256), 0, c.stream, L.v, (const float*)r,
+                       (const float*)L.st32, L.NS, (const int*)L.slot, (const float*)S.lrho,
+                       (const double*)(c.scal + S_NN0), nfree, A->feat);
+    unet_forward(S.net, A, c.stream);
+    hipLaunchKernelGGL(k_nn_correction, dim3(nblk(V)), dim3(256), 0, c.stream, L.v, (const float*)A->y,
+                       (const float*)L.st32, L.NS, (const int*)L.slot, (const double*)(c.scal + S_NN0), nfree, S.e);
+    LAUNCH_CHECK();
+  }
+  apply_level<float, 1>(c, l, S.e, r, S.t2);  // residual after the correction
+  scaled_norm(c, l, S.t2, S_NN1);
+  hipLaunchKernelGGL(k_nn_accept, dim3(cblk(L.n_lin)), dim3(256), 0, c.stream, L.v, (const int4*)L.cl_lin, L.n_lin,
+                     (const double*)(c.scal + S_NN0), S.theta, (const float*)S.e, u, S.counters, S.ratio_sum);
   LAUNCH_CHECK();
 }
 
@@ -1081,6 +1238,7 @@ TOS_EXPORT int tos_ctx_destroy(tos_ctx* c) {
   if (!c) return TOS_E_ARG;
   (void)hipSetDevice(c->dev);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  learn_release(*c);
   for (void* p : c->allocs) (void)hipFree(p);
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
   if (c->stage) (void)hipFree(c->stage);
@@ -1259,6 +1417,7 @@ TOS_EXPORT int tos_update_operator(tos_ctx* c) {
     }
     const int rc = coarse_factor(*c);
     if (rc) return rc;
+    if (c->learn && c->learn->lrho) learn_logrho(*c);
     HIPCHK(hipStreamSynchronize(c->stream));
     c->op_ready = true;
     return TOS_OK;
@@ -1867,6 +2026,343 @@ TOS_EXPORT int tos_profile(tos_ctx* c, int enable, tos_kernel_stats* out) {
     }
     c->prof.clear();
     c->ev_used = 0;
+    return TOS_OK;
+  });
+}
+
+// =========================================================================================
+// Learned coarse-level correction: network, dataset, training, attachment to a context
+// =========================================================================================
+namespace {
+
+template <class F>
+int nguard(std::string* err, int dev, F&& f) {
+  try {
+    HIPCHK(hipSetDevice(dev));
+    return f();
+  } catch (const HipError& e) {
+    if (err) *err = e.what();
+    return TOS_E_HIP;
+  } catch (const std::exception& e) {
+    if (err) *err = e.what();
+    return TOS_E_ARG;
+  }
+}
+
+void unet_free(tos_unet* N) {
+  unet_free_acts(N);
+  for (void* p : N->allocs) (void)hipFree(p);
+  if (N->h_scal) (void)hipHostFree(N->h_scal);
+  if (N->stream) (void)hipStreamDestroy(N->stream);
+  delete N;
+}
+
+// copy sample `i` of ds into slot b of the batch tensors of A
+void load_sample(tos_unet* N, tos::NetActs* A, tos_dataset* ds, int b, int i) {
+  const int64_t V = ds->V;
+  HIPCHK(hipMemcpyAsync(A->feat + (int64_t)b * V * 8, ds->feat + (int64_t)i * V * 8, V * 8 * sizeof(uint16_t),
+                        hipMemcpyDeviceToDevice, N->stream));
+  HIPCHK(hipMemcpyAsync(A->tgt + (int64_t)b * V * 3, ds->tgt + (int64_t)i * V * 3, V * 3 * sizeof(float),
+                        hipMemcpyDeviceToDevice, N->stream));
+  HIPCHK(hipMemcpyAsync(A->mask + (int64_t)b * V, ds->mask + (int64_t)i * V, V, hipMemcpyDeviceToDevice, N->stream));
+  HIPCHK(hipMemcpyAsync(A->inv_den + b, ds->inv_den + i, sizeof(float), hipMemcpyDeviceToDevice, N->stream));
+}
+
+double read_loss(tos_unet* N) {
+  download(N->stream, N->h_scal, N->dscal, sizeof(double));
+  return N->h_scal[0];
+}
+
+}  // namespace
+
+TOS_EXPORT int tos_unet_create(int c0, int c1, int c2, uint64_t seed, tos_unet** out) {
+  if (!out || c0 != 16 || c1 != 32 || c2 != 64) return TOS_E_ARG;
+  *out = nullptr;
+  tos_unet* N = new tos_unet();
+  std::string err;
+  const int rc = nguard(&err, 0, [&]() {
+    HIPCHK(hipGetDevice(&N->dev));
+    HIPCHK(hipStreamCreateWithFlags(&N->stream, hipStreamNonBlocking));
+    N->ch[0] = c0; N->ch[1] = c1; N->ch[2] = c2;
+    unet_layout(N);
+    N->param = nalloc<float>(N->allocs, N->stream, N->nparam);
+    N->grad = nalloc<float>(N->allocs, N->stream, N->nparam);
+    N->am = nalloc<float>(N->allocs, N->stream, N->nparam);
+    N->av = nalloc<float>(N->allocs, N->stream, N->nparam);
+    size_t wp = 0;
+    for (size_t li = 0; li < N->layers.size(); ++li) {
+      tos::NetLayer& L = N->layers[li];
+      L.wf = nalloc<uint16_t>(N->allocs, N->stream, (size_t)L.nout * L.kpf);
+      if (li > 0) L.wb = nalloc<uint16_t>(N->allocs, N->stream, (size_t)L.cin * L.kpb);
+      wp = std::max(wp, (size_t)kWgradGroups * 27 * ((L.cinp + 15) / 16 * 16) * L.nout);
+    }
+    N->wpart = nalloc<float>(N->allocs, N->stream, wp);
+    N->wpart_n = wp;
+    N->bpart = nalloc<float>(N->allocs, N->stream, (size_t)kBiasGroups * 64);
+    N->dscal = nalloc<double>(N->allocs, N->stream, 8);
+    HIPCHK(hipHostMalloc((void**)&N->h_scal, 8 * sizeof(double)));
+    std::vector<float> h;
+    unet_init_params(N, seed, h);
+    upload(N->stream, N->param, h.data(), h.size() * sizeof(float));
+    return TOS_OK;
+  });
+  if (rc) {
+    std::fprintf(stderr, "tos_unet_create: %s\n", err.c_str());
+    unet_free(N);
+    return rc;
+  }
+  *out = N;
+  return TOS_OK;
+}
+
+TOS_EXPORT int tos_unet_destroy(tos_unet* N) {
+  if (!N) return TOS_E_ARG;
+  (void)hipSetDevice(N->dev);
+  if (N->stream) (void)hipStreamSynchronize(N->stream);
+  unet_free(N);
+  return TOS_OK;
+}
+
+TOS_EXPORT const char* tos_unet_last_error(const tos_unet* N) { return N ? N->err.c_str() : "null network"; }
+
+TOS_EXPORT int tos_unet_num_params(const tos_unet* N, int64_t* n) {
+  if (!N || !n) return TOS_E_ARG;
+  *n = N->nparam;
+  return TOS_OK;
+}
+
+TOS_EXPORT int tos_unet_get_params(tos_unet* N, float* out) {
+  if (!N || !out) return TOS_E_ARG;
+  return nguard(&N->err, N->dev, [&]() {
+    download(N->stream, out, N->param, N->nparam * sizeof(float));
+    return TOS_OK;
+  });
+}
+
+TOS_EXPORT int tos_unet_set_params(tos_unet* N, const float* in) {
+  if (!N || !in) return TOS_E_ARG;
+  return nguard(&N->err, N->dev, [&]() {
+    upload(N->stream, N->param, in, N->nparam * sizeof(float));
+    hipLaunchKernelGGL(k_zero<float>, dim3(nblk(N->nparam)), dim3(256), 0, N->stream, N->am, N->nparam);
+    hipLaunchKernelGGL(k_zero<float>, dim3(nblk(N->nparam)), dim3(256), 0, N->stream, N->av, N->nparam);
+    HIPCHK(hipStreamSynchronize(N->stream));
+    N->adam_t = 0;
+    N->dirty = true;
+    return TOS_OK;
+  });
+}
+
+TOS_EXPORT int tos_unet_forward(tos_unet* N, int B, int X, int Y, int Z, const float* feat, float* out) {
+  if (!N || !feat || !out || B <= 0 || X <= 0 || Y <= 0 || Z <= 0) return TOS_E_ARG;
+  return nguard(&N->err, N->dev, [&]() {
+    tos::NetActs* A = unet_acts(N, B, tos::NGrid{X, Y, Z}, false);
+    const int64_t n = (int64_t)B * X * Y * Z;
+    float* st = nalloc<float>(A->mem, N->stream, n * 4);
+    upload(N->stream, st, feat, n * 4 * sizeof(float));
+    hipLaunchKernelGGL(k_feat_pack, dim3(nblk(n)), dim3(256), 0, N->stream, (const float*)st, n, A->feat);
+    unet_forward(N, A, N->stream);
+    download(N->stream, out, A->y, n * 3 * sizeof(float));
+    return TOS_OK;
+  });
+}
+
+TOS_EXPORT int tos_unet_loss_grad(tos_unet* N, int B, int X, int Y, int Z, const float* feat, const float* target,
+                                  const uint8_t* mask, double* loss, float* grad) {
+  if (!N || !feat || !target || !mask || B <= 0 || X <= 0 || Y <= 0 || Z <= 0) return TOS_E_ARG;
+  return nguard(&N->err, N->dev, [&]() {
+    tos::NetActs* A = unet_acts(N, B, tos::NGrid{X, Y, Z}, true);
+    const int64_t n = (int64_t)B * X * Y * Z;
+    float* st = nalloc<float>(A->mem, N->stream, n * 4);
+    upload(N->stream, st, feat, n * 4 * sizeof(float));
+    hipLaunchKernelGGL(k_feat_pack, dim3(nblk(n)), dim3(256), 0, N->stream, (const float*)st, n, A->feat);
+    upload(N->stream, A->tgt, target, n * 3 * sizeof(float));
+    upload(N->stream, A->mask, mask, n);
+    batch_inv_den(N, A, N->stream);
+    unet_forward(N, A, N->stream);
+    unet_loss(N, A, N->stream);
+    unet_backward(N, A, N->stream);
+    if (loss) *loss = read_loss(N);
+    if (grad) download(N->stream, grad, N->grad, N->nparam * sizeof(float));
+    return TOS_OK;
+  });
+}
+
+TOS_EXPORT int tos_dataset_create(int X, int Y, int Z, int capacity, tos_dataset** out) {
+  if (!out || X <= 0 || Y <= 0 || Z <= 0 || capacity <= 0) return TOS_E_ARG;
+  *out = nullptr;
+  tos_dataset* D = new tos_dataset();
+  std::string err;
+  const int rc = nguard(&err, 0, [&]() {
+    HIPCHK(hipGetDevice(&D->dev));
+    HIPCHK(hipStreamCreateWithFlags(&D->stream, hipStreamNonBlocking));
+    D->g = tos::NGrid{X, Y, Z};
+    D->V = (int64_t)X * Y * Z;
+    D->cap = capacity;
+    D->feat = nalloc<uint16_t>(D->allocs, D->stream, (size_t)capacity * D->V * 8);
+    D->tgt = nalloc<float>(D->allocs, D->stream, (size_t)capacity * D->V * 3);
+    D->mask = nalloc<uint8_t>(D->allocs, D->stream, (size_t)capacity * D->V);
+    D->inv_den = nalloc<float>(D->allocs, D->stream, capacity);
+    HIPCHK(hipStreamSynchronize(D->stream));
+    return TOS_OK;
+  });
+  if (rc) {
+    std::fprintf(stderr, "tos_dataset_create: %s\n", err.c_str());
+    for (void* p : D->allocs) (void)hipFree(p);
+    if (D->stream) (void)hipStreamDestroy(D->stream);
+    delete D;
+    return rc;
+  }
+  *out = D;
+  return TOS_OK;
+}
+
+TOS_EXPORT int tos_dataset_destroy(tos_dataset* D) {
+  if (!D) return TOS_E_ARG;
+  (void)hipSetDevice(D->dev);
+  for (void* p : D->allocs) (void)hipFree(p);
+  if (D->stream) (void)hipStreamDestroy(D->stream);
+  delete D;
+  return TOS_OK;
+}
+
+TOS_EXPORT int tos_dataset_size(const tos_dataset* D, int* n) {
+  if (!D || !n) return TOS_E_ARG;
+  *n = D->n;
+  return TOS_OK;
+}
+
+// sample i: features (4 channels as float), target (3), mask; any output may be null
+TOS_EXPORT int tos_dataset_get(tos_dataset* D, int i, float* feat4, float* target, uint8_t* mask) {
+  if (!D || i < 0 || i >= D->n) return TOS_E_ARG;
+  return nguard(&D->err, D->dev, [&]() {
+    const int64_t V = D->V;
+    if (feat4) {
+      std::vector<uint16_t> h(V * 8);
+      download(D->stream, h.data(), D->feat + (int64_t)i * V * 8, V * 8 * sizeof(uint16_t));
+      for (int64_t v = 0; v < V; ++v)
+        for (int c = 0; c < 4; ++c) {
+          const uint32_t u = (uint32_t)h[v * 8 + c] << 16;
+          std::memcpy(&feat4[v * 4 + c], &u, 4);
+        }
+    }
+    if (target) download(D->stream, target, D->tgt + (int64_t)i * V * 3, V * 3 * sizeof(float));
+    if (mask) download(D->stream, mask, D->mask + (int64_t)i * V, V);
+    return TOS_OK;
+  });
+}
+
+// Adam on random batches of samples [first, first + count); loss_first = loss of the first
+// batch, loss_last = mean loss over the last min(50, steps) batches (before their update)
+TOS_EXPORT int tos_unet_train(tos_unet* N, tos_dataset* D, int first, int count, int steps, int batch, double lr,
+                              uint64_t seed, double* loss_first, double* loss_last) {
+  if (!N || !D || first < 0 || count <= 0 || first + count > D->n || steps < 0 || batch <= 0 || !(lr > 0))
+    return TOS_E_ARG;
+  return nguard(&N->err, N->dev, [&]() {
+    HIPCHK(hipStreamSynchronize(D->stream));
+    tos::NetActs* A = unet_acts(N, batch, D->g, true);
+    uint64_t s = seed * 0x9e3779b97f4a7c15ULL + 1;
+    auto next = [&]() {
+      uint64_t z = (s += 0x9e3779b97f4a7c15ULL);
+      z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+      z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+      return z ^ (z >> 31);
+    };
+    double lf = 0, ll = 0;
+    int nl = 0;
+    const int tail = std::min(50, steps);
+    for (int it = 0; it < steps; ++it) {
+      for (int b = 0; b < batch; ++b) load_sample(N, A, D, b, first + (int)(next() % (uint64_t)count));
+      unet_forward(N, A, N->stream);
+      unet_loss(N, A, N->stream);
+      if (it == 0) lf = read_loss(N);
+      if (it >= steps - tail) {
+        ll += read_loss(N);
+        ++nl;
+      }
+      unet_backward(N, A, N->stream);
+      unet_adam(N, lr, N->stream);
+    }
+    HIPCHK(hipStreamSynchronize(N->stream));
+    if (loss_first) *loss_first = lf;
+    if (loss_last) *loss_last = nl ? ll / nl : 0.0;
+    return TOS_OK;
+  });
+}
+
+// mean per-sample loss of the network over samples [first, first + count)
+TOS_EXPORT int tos_unet_eval(tos_unet* N, tos_dataset* D, int first, int count, double* mean_loss) {
+  if (!N || !D || !mean_loss || first < 0 || count <= 0 || first + count > D->n) return TOS_E_ARG;
+  return nguard(&N->err, N->dev, [&]() {
+    HIPCHK(hipStreamSynchronize(D->stream));
+    const int bs = std::min(count, 8);
+    double tot = 0;
+    for (int i0 = 0; i0 < count; i0 += bs) {
+      const int b = std::min(bs, count - i0);
+      tos::NetActs* A = unet_acts(N, b, D->g, true);
+      for (int j = 0; j < b; ++j) load_sample(N, A, D, j, first + i0 + j);
+      unet_forward(N, A, N->stream);
+      unet_loss(N, A, N->stream);
+      tot += read_loss(N) * b;
+    }
+    *mean_loss = tot / count;
+    return TOS_OK;
+  });
+}
+
+TOS_EXPORT int tos_ctx_learn_attach(tos_ctx* c, tos_unet* net, int level, int mode, double theta) {
+  return guard(c, [&]() {
+    REQUIRE(mode == 0 || (net && mode == 1), "bad mode");
+    if (mode == 0) {
+      if (c->learn) { c->learn->mode = 0; c->learn->net = nullptr; }
+      return TOS_OK;
+    }
+    REQUIRE(level >= 1 && level + 1 < (int)c->L.size(), "learned level must have a coarser level below it");
+    REQUIRE(theta > 0, "bad theta");
+    learn_prepare(*c, level);
+    LearnState& S = *c->learn;
+    const Level& L = c->L[level];
+    S.acts = unet_acts(net, 1, tos::NGrid{L.v.nx, L.v.ny, L.v.nz}, false);
+    S.net = net;
+    S.mode = 1;
+    S.theta = theta;
+    hipLaunchKernelGGL(k_zero<int>, dim3(1), dim3(64), 0, c->stream, S.counters, (int64_t)2);
+    hipLaunchKernelGGL(k_zero<double>, dim3(1), dim3(64), 0, c->stream, S.ratio_sum, (int64_t)1);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return TOS_OK;
+  });
+}
+
+TOS_EXPORT int tos_ctx_learn_log(tos_ctx* c, tos_dataset* ds, int level, int n_inner, int stride) {
+  return guard(c, [&]() {
+    if (!ds) {
+      if (c->learn) c->learn->ds = nullptr;
+      return TOS_OK;
+    }
+    REQUIRE(level >= 1 && level + 1 < (int)c->L.size(), "logged level must have a coarser level below it");
+    REQUIRE(n_inner >= 1 && stride >= 1, "bad arguments");
+    const Level& L = c->L[level];
+    REQUIRE(ds->g.X == L.v.nx && ds->g.Y == L.v.ny && ds->g.Z == L.v.nz, "dataset grid differs from the level");
+    REQUIRE(!c->learn || c->learn->level == level || c->learn->mode == 0, "network attached on another level");
+    learn_prepare(*c, level);
+    c->learn->ds = ds;
+    c->learn->n_inner = n_inner;
+    c->learn->stride = stride;
+    c->learn->calls = 0;
+    return TOS_OK;
+  });
+}
+
+TOS_EXPORT int tos_ctx_learn_stats(tos_ctx* c, int* tried, int* accepted, double* mean_ratio) {
+  return guard(c, [&]() {
+    int h[2] = {0, 0};
+    double rs = 0;
+    if (c->learn && c->learn->counters) {
+      download(c->stream, h, c->learn->counters, sizeof(h));
+      download(c->stream, &rs, c->learn->ratio_sum, sizeof(double));
+    }
+    if (tried) *tried = h[0];
+    if (accepted) *accepted = h[1];
+    if (mean_ratio) *mean_ratio = h[0] ? rs / h[0] : 0.0;
     return TOS_OK;
   });
 }

@@ -23,6 +23,8 @@
 
 namespace tos {
 
+struct LearnState;  // learned coarse-level correction (network, pair logging)
+
 struct LevelView {  // passed by value to kernels
   int nx, ny, nz;     // node dims
   int NX, NY;         // padded node row / plane dims
@@ -135,6 +137,7 @@ struct Ctx {
   std::vector<ProfRec> prof;
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
+  LearnState* learn = nullptr;
 };
 
 constexpr int kRedBlocks = 1024;  // fixed partial count -> deterministic reductions
@@ -146,5 +149,6 @@ enum {
   S_RZ = 0, S_PQ, S_ALPHA, S_BETA, S_RN2, S_ZR, S_ZQ, S_NF2,
   S_OC_LO, S_OC_HI, S_OC_G, S_OC_V, S_OC_DONE, S_OC_STEPS, S_OC_GMAX, S_VOL,
   S_TMP0, S_TMP1, S_TMP2, S_TMP3,
+  S_NN0, S_NN1,  // scaled residual norms before / after the learned correction
   S_COUNT = 32
 };
