@@ -180,6 +180,18 @@ int tos_spline_get_coeff_sens(tos_ctx* ctx, int which, float* out);
 int tos_oc_update_coeffs(tos_ctx* ctx, double vgoal, double move, double damp, double cmin,
                          double cmax, double* g_out, int* steps_out);
 
+/* All K load cases in one solve (no counterpart in the reference, which solves load
+ * cases one at a time): case-interleaved vectors so every stencil / density read serves
+ * all K cases, one flexible CG per case (FP64 outer vectors, per-case scalars and
+ * convergence flags on the device) around one FP32 V-cycle for all cases.  The body of
+ * an FCG iteration is captured once into a hipGraph and replayed; kernel block sizes
+ * are chosen on the first call by timing the candidates (64/128/256 threads) on this
+ * context's levels.  Requires precision = 1 and use_pcg = 1; same results contract as
+ * tos_solve per case. */
+int tos_solve_batched(tos_ctx* ctx, const tos_solve_opts* opts, tos_solve_stats* stats);
+/* text description of the block sizes chosen for the batched solve ("" before it ran) */
+int tos_batch_tuning(tos_ctx* ctx, char* buf, int len);
+
 /* ---- MMA --------------------------------------------------------------------------
  * Method of moving asymptotes over n design variables with m constraints
  * (MMA::mma_t, optimization/mma_t.h:159-266; update mma_t.cpp:18-51).

@@ -1671,3 +1671,14 @@ ORC_API int orc_profile(tos_ctx* c, int enable, tos_kernel_stats* out) {
   if (out) std::memset(out, 0, sizeof(*out));
   return TOS_OK;
 }
+
+// The batched solve is the same mathematics per case: the oracle solves the cases in turn.
+ORC_API int orc_solve_batched(tos_ctx* c, const tos_solve_opts* o, tos_solve_stats* st) {
+  return orc_solve(c, o, st);
+}
+
+ORC_API int orc_batch_tuning(tos_ctx* c, char* buf, int len) {
+  if (!c || !buf || len <= 0) return TOS_E_ARG;
+  buf[0] = 0;
+  return TOS_OK;
+}

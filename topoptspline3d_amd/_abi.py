@@ -90,6 +90,8 @@ SIGNATURES = {
     "mma_update": (C.c_int, [vp, f64p, f64p, f64p, i32p]),
     "ctx_mma_step": (C.c_int, [vp, vp, C.c_double, C.c_double, C.c_double, i32p]),
     "profile": (C.c_int, [vp, C.c_int, C.POINTER(KernelStats)]),
+    "solve_batched": (C.c_int, [vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
+    "batch_tuning": (C.c_int, [vp, C.c_char_p, C.c_int]),
 }
 
 # learned coarse-level correction (HIP library only; the network oracle is numpy)

@@ -154,6 +154,21 @@ class Context:
         return {"rc": rc, "iterations": [st.iterations[k] for k in range(self.K)],
                 "rel_res": [st.rel_res[k] for k in range(self.K)], "seconds": st.seconds}
 
+    def solve_batched(self, rtol=1e-6, maxit=200, pre=1, post=1, warm_start=False, check=True):
+        """all load cases in one interleaved FCG with an FP32 V-cycle (graph-captured)"""
+        o = _abi.SolveOpts(rtol, maxit, 1, pre, post, int(warm_start), 1)
+        st = _abi.SolveStats()
+        rc = self.api.solve_batched(self.h, C.byref(o), C.byref(st))
+        if check:
+            self._chk(rc, "solve_batched")
+        return {"rc": rc, "iterations": [st.iterations[k] for k in range(self.K)],
+                "rel_res": [st.rel_res[k] for k in range(self.K)], "seconds": st.seconds}
+
+    def batch_tuning(self):
+        buf = C.create_string_buffer(512)
+        self._chk(self.api.batch_tuning(self.h, buf, 512), "batch_tuning")
+        return buf.value.decode()
+
     def displacement(self, k=0):
         u = np.zeros(3 * self.nn)
         self._chk(self.api.get_displacement(self.h, k, _p(u, C.c_double)), "get_displacement")

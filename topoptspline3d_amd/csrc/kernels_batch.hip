@@ -1,0 +1,414 @@
+// kernels_batch.hip -- multigrid and FCG kernels for K load cases solved together.
+//
+// Layout: case-interleaved vectors, v[(c*NP + p)*K + k] (component c, padded node p,
+// case k), so the K values of one node and component are contiguous and every stencil,
+// density and template-matrix read is shared by the K lanes of a node (one load serves
+// K cases).  Thread mapping on a chunk list: 64*KP threads per chunk (KP = K rounded up
+// to a power of two), thread i of a chunk -> node lane i / KP, case i % KP (idle when
+// >= K).  Block size BS is a template parameter picked per kernel and level by the
+// autotuner.  Per-case block partials are written as partial[block * K + k].
+#include "tos_internal.hpp"
+
+namespace tos {
+
+template <int BS>
+__device__ __forceinline__ bool bnode(const int4* __restrict__ cl, int nch, int s, int K, int KP, NodeRef& n,
+                                      int& k) {
+  const int64_t g = (int64_t)blockIdx.x * BS + threadIdx.x;
+  const int64_t tpc = 64LL * KP;
+  const int64_t ch = g / tpc;
+  if (ch >= nch) return false;
+  const int i = (int)(g - ch * tpc);
+  const int lane = i / KP;
+  k = i - lane * KP;
+  if (k >= K) return false;
+  const int4 c = cl[ch];
+  if (lane >= (c.y >> 16)) return false;
+  n.x = (c.y & 0xffff) + s * lane;
+  n.y = c.z;
+  n.z = c.w;
+  n.p = (int64_t)c.x + (int64_t)s * lane;
+  n.slot = (int)ch * 64 + lane;
+  return true;
+}
+
+// per-case block sums: partial[blockIdx.x * K + k]; every thread calls (v = 0 when idle)
+template <int BS>
+__device__ __forceinline__ void bpartial(double v, int K, int KP, double* __restrict__ partial) {
+  __shared__ double sh[BS];
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  if (threadIdx.x < K && threadIdx.x < KP) {
+    double s = 0;
+    for (int j = threadIdx.x; j < BS; j += KP) s += sh[j];
+    partial[(int64_t)blockIdx.x * K + threadIdx.x] = s;
+  }
+  __syncthreads();
+}
+
+template <class T>
+__device__ __forceinline__ void load_nbhd_b(const LevelView& L, const T* __restrict__ u, int64_t p, int K, int k,
+                                            T (&un)[27][3]) {
+#pragma unroll
+  for (int q = 0; q < 27; ++q) {
+    const int64_t a = p + (int64_t)(q % 3 - 1) + (int64_t)L.NX * ((q / 3) % 3 - 1 + (int64_t)L.NY * (q / 9 - 1));
+#pragma unroll
+    for (int c = 0; c < 3; ++c) un[q][c] = u[(c * L.NP + a) * K + k];
+  }
+}
+
+// MODE 0: y = K u with per-case partials of u.y;  MODE 1: y = f - K u (+ float copy y32)
+template <class T, int MODE, int BS>
+__global__ __launch_bounds__(BS) void k_fine_apply_b(LevelView L, const int4* __restrict__ cl, int nch, int K, int KP,
+                                                     const T* __restrict__ w, const T* __restrict__ Ke,
+                                                     const T* __restrict__ u, const T* __restrict__ f,
+                                                     T* __restrict__ y, double* __restrict__ partial,
+                                                     float* __restrict__ y32) {
+  NodeRef n;
+  int k = 0;
+  const bool ok = bnode<BS>(cl, nch, 1, K, KP, n, k);
+  double d = 0;
+  if (ok) {
+    T un[27][3];
+    load_nbhd_b(L, u, n.p, K, k, un);
+    T acc[3] = {0, 0, 0}, D[9];
+    fine_row<T, true>(L, w, Ke, n.x, n.y, n.z, un, acc, D);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int64_t i = (c * L.NP + n.p) * K + k;
+      if (MODE == 0) {
+        y[i] = acc[c];
+        d += (double)un[13][c] * (double)acc[c];
+      } else {
+        const T v = f[i] - acc[c];
+        y[i] = v;
+        if (y32) y32[i] = (float)v;
+      }
+    }
+  }
+  if (partial) bpartial<BS>(d, K, KP, partial);
+}
+
+template <class T, bool BACKWARD, int BS>
+__global__ __launch_bounds__(BS) void k_fine_gs_b(LevelView L, const int4* __restrict__ cl, int nch, int K, int KP,
+                                                  const T* __restrict__ w, const T* __restrict__ Ke,
+                                                  T* __restrict__ u, const T* __restrict__ f) {
+  NodeRef n;
+  int k = 0;
+  if (!bnode<BS>(cl, nch, 2, K, KP, n, k)) return;
+  T un[27][3];
+  load_nbhd_b(L, u, n.p, K, k, un);
+  T s[3] = {0, 0, 0};
+  T D[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  fine_row<T, false>(L, w, Ke, n.x, n.y, n.z, un, s, D);
+  const int64_t i0 = n.p * K + k, cs = L.NP * K;
+  const T b0 = f[i0] - s[0], b1 = f[cs + i0] - s[1], b2 = f[2 * cs + i0] - s[2];
+  T v0 = un[13][0], v1 = un[13][1], v2 = un[13][2];
+  if (!BACKWARD) {
+    v0 = (b0 - D[1] * v1 - D[2] * v2) / D[0];
+    v1 = (b1 - D[3] * v0 - D[5] * v2) / D[4];
+    v2 = (b2 - D[6] * v0 - D[7] * v1) / D[8];
+  } else {
+    v2 = (b2 - D[6] * v0 - D[7] * v1) / D[8];
+    v1 = (b1 - D[3] * v0 - D[5] * v2) / D[4];
+    v0 = (b0 - D[1] * v1 - D[2] * v2) / D[0];
+  }
+  u[i0] = v0;
+  u[cs + i0] = v1;
+  u[2 * cs + i0] = v2;
+}
+
+template <class T, int MODE, int BS>
+__global__ __launch_bounds__(BS) void k_st_apply_b(LevelView L, const int4* __restrict__ cl, int nch, int K, int KP,
+                                                   const T* __restrict__ st, int64_t NS, const T* __restrict__ u,
+                                                   const T* __restrict__ f, T* __restrict__ y) {
+  NodeRef n;
+  int k = 0;
+  if (!bnode<BS>(cl, nch, 2, K, KP, n, k)) return;
+  const int64_t cs = L.NP * K;
+  T a0 = 0, a1 = 0, a2 = 0;
+#pragma unroll
+  for (int q = 0; q < 27; ++q) {
+    const int64_t a =
+        (n.p + (int64_t)(q % 3 - 1) + (int64_t)L.NX * ((q / 3) % 3 - 1 + (int64_t)L.NY * (q / 9 - 1))) * K + k;
+    const T u0 = u[a], u1 = u[cs + a], u2 = u[2 * cs + a];
+    const T* s = st + (int64_t)q * 9 * NS + n.slot;
+    a0 += s[0] * u0 + s[NS] * u1 + s[2 * NS] * u2;
+    a1 += s[3 * NS] * u0 + s[4 * NS] * u1 + s[5 * NS] * u2;
+    a2 += s[6 * NS] * u0 + s[7 * NS] * u1 + s[8 * NS] * u2;
+  }
+  const int64_t i0 = n.p * K + k;
+  if (MODE == 0) {
+    y[i0] = a0; y[cs + i0] = a1; y[2 * cs + i0] = a2;
+  } else {
+    y[i0] = f[i0] - a0; y[cs + i0] = f[cs + i0] - a1; y[2 * cs + i0] = f[2 * cs + i0] - a2;
+  }
+}
+
+template <class T, bool BACKWARD, int BS>
+__global__ __launch_bounds__(BS) void k_st_gs_b(LevelView L, const int4* __restrict__ cl, int nch, int64_t slot0,
+                                                int K, int KP, const T* __restrict__ st, int64_t NS,
+                                                T* __restrict__ u, const T* __restrict__ f) {
+  NodeRef n;
+  int k = 0;
+  if (!bnode<BS>(cl, nch, 2, K, KP, n, k)) return;
+  const int64_t slot = slot0 + n.slot, cs = L.NP * K;
+  T s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+  for (int q = 0; q < 27; ++q) {
+    if (q == 13) continue;
+    const int64_t a =
+        (n.p + (int64_t)(q % 3 - 1) + (int64_t)L.NX * ((q / 3) % 3 - 1 + (int64_t)L.NY * (q / 9 - 1))) * K + k;
+    const T u0 = u[a], u1 = u[cs + a], u2 = u[2 * cs + a];
+    const T* s = st + (int64_t)q * 9 * NS + slot;
+    s0 += s[0] * u0 + s[NS] * u1 + s[2 * NS] * u2;
+    s1 += s[3 * NS] * u0 + s[4 * NS] * u1 + s[5 * NS] * u2;
+    s2 += s[6 * NS] * u0 + s[7 * NS] * u1 + s[8 * NS] * u2;
+  }
+  const T* d = st + (int64_t)13 * 9 * NS + slot;
+  T D[9];
+#pragma unroll
+  for (int e = 0; e < 9; ++e) D[e] = d[e * NS];
+  const int64_t i0 = n.p * K + k;
+  const T b0 = f[i0] - s0, b1 = f[cs + i0] - s1, b2 = f[2 * cs + i0] - s2;
+  T v0 = u[i0], v1 = u[cs + i0], v2 = u[2 * cs + i0];
+  if (!BACKWARD) {
+    v0 = (b0 - D[1] * v1 - D[2] * v2) / D[0];
+    v1 = (b1 - D[3] * v0 - D[5] * v2) / D[4];
+    v2 = (b2 - D[6] * v0 - D[7] * v1) / D[8];
+  } else {
+    v2 = (b2 - D[6] * v0 - D[7] * v1) / D[8];
+    v1 = (b1 - D[3] * v0 - D[5] * v2) / D[4];
+    v0 = (b0 - D[1] * v1 - D[2] * v2) / D[0];
+  }
+  u[i0] = v0;
+  u[cs + i0] = v1;
+  u[2 * cs + i0] = v2;
+}
+
+template <class T, int BS>
+__global__ __launch_bounds__(BS) void k_restrict_b(LevelView F, LevelView C, const int4* __restrict__ clc, int nch,
+                                                   int K, int KP, const T* __restrict__ rf, T* __restrict__ fc) {
+  NodeRef n;
+  int k = 0;
+  if (!bnode<BS>(clc, nch, 1, K, KP, n, k)) return;
+  const int64_t fcs = F.NP * K, ccs = C.NP * K;
+  T a0 = 0, a1 = 0, a2 = 0;
+  const int64_t c0 = F.pnode(2 * n.x, 2 * n.y, 2 * n.z);
+#pragma unroll
+  for (int q = 0; q < 27; ++q) {
+    const int dx = q % 3 - 1, dy = (q / 3) % 3 - 1, dz = q / 9 - 1;
+    const T wgt = T((dx ? 0.5 : 1.0) * (dy ? 0.5 : 1.0) * (dz ? 0.5 : 1.0));
+    const int64_t a = (c0 + dx + (int64_t)F.NX * (dy + (int64_t)F.NY * dz)) * K + k;
+    a0 += wgt * rf[a]; a1 += wgt * rf[fcs + a]; a2 += wgt * rf[2 * fcs + a];
+  }
+  const int64_t i0 = n.p * K + k;
+  fc[i0] = a0; fc[ccs + i0] = a1; fc[2 * ccs + i0] = a2;
+}
+
+template <class T, int BS>
+__global__ __launch_bounds__(BS) void k_prolong_add_b(LevelView F, LevelView C, const int4* __restrict__ clf,
+                                                      int nch, int K, int KP, const T* __restrict__ uc,
+                                                      T* __restrict__ uf) {
+  NodeRef n;
+  int k = 0;
+  if (!bnode<BS>(clf, nch, 1, K, KP, n, k)) return;
+  const int x = n.x, y = n.y, z = n.z;
+  const int nxp = (x & 1) + 1, nyp = (y & 1) + 1, nzp = (z & 1) + 1;
+  const T wgt = ((x & 1) ? T(0.5) : T(1)) * ((y & 1) ? T(0.5) : T(1)) * ((z & 1) ? T(0.5) : T(1));
+  const int64_t I0 = C.pnode(x >> 1, y >> 1, z >> 1), ccs = C.NP * K, fcs = F.NP * K;
+  T a0 = 0, a1 = 0, a2 = 0;
+  for (int kz = 0; kz < nzp; ++kz)
+    for (int j = 0; j < nyp; ++j)
+      for (int i = 0; i < nxp; ++i) {
+        const int64_t I = (I0 + i + (int64_t)C.NX * (j + (int64_t)C.NY * kz)) * K + k;
+        a0 += wgt * uc[I]; a1 += wgt * uc[ccs + I]; a2 += wgt * uc[2 * ccs + I];
+      }
+  const int64_t i0 = n.p * K + k;
+  uf[i0] += a0; uf[fcs + i0] += a1; uf[2 * fcs + i0] += a2;
+}
+
+template <class T, int BS>
+__global__ __launch_bounds__(BS) void k_zero_nodes_b(const int4* __restrict__ cl, int nch, int64_t NP, int K, int KP,
+                                                     T* __restrict__ u) {
+  NodeRef n;
+  int k = 0;
+  if (!bnode<BS>(cl, nch, 1, K, KP, n, k)) return;
+  const int64_t i0 = n.p * K + k, cs = NP * K;
+  u[i0] = T(0); u[cs + i0] = T(0); u[2 * cs + i0] = T(0);
+}
+
+// coarsest level: u[:, k] = -cinv f[:, k] (cinv holds -A^{-1}), thread per (row, case)
+template <class T>
+__global__ __launch_bounds__(256) void k_coarse_solve_b(LevelView L, const int64_t* __restrict__ cidx, int n, int K,
+                                                        const double* __restrict__ cinv, const T* __restrict__ f,
+                                                        T* __restrict__ u) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)n * K) return;
+  const int row = (int)(t / K), k = (int)(t % K);
+  const int64_t cs = L.NP * K;
+  double s = 0;
+  for (int j = 0; j < n; ++j) {
+    const int64_t pj = cidx[j / 3];
+    s += cinv[(int64_t)row * n + j] * (double)f[(j % 3) * cs + pj * K + k];
+  }
+  const int64_t pr = cidx[row / 3];
+  u[(row % 3) * cs + pr * K + k] = (T)(-s);
+}
+
+// ---- batched FCG (FP64 outer vectors, per-case scalars sc[slot * K + k]) ----------------
+enum { B_RZ = 0, B_PQ, B_ALPHA, B_BETA, B_RN2, B_ZR, B_ZQ, B_NF2, B_ACTIVE, B_RTOL2, B_COUNT = 12 };
+
+template <int BS>
+__global__ __launch_bounds__(BS) void k_pcgb_ur(const int4* __restrict__ cl, int nch, int64_t NP, int K, int KP,
+                                                double* __restrict__ u, double* __restrict__ r,
+                                                const double* __restrict__ p, const double* __restrict__ q,
+                                                const double* __restrict__ sc, float* __restrict__ r32,
+                                                double* __restrict__ partial) {
+  NodeRef n;
+  int k = 0;
+  const bool ok = bnode<BS>(cl, nch, 1, K, KP, n, k);
+  double d = 0;
+  if (ok) {
+    const double a = sc[B_ALPHA * K + k];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int64_t i = (c * NP + n.p) * K + k;
+      u[i] += a * p[i];
+      const double v = r[i] - a * q[i];
+      r[i] = v;
+      r32[i] = (float)v;
+      d += v * v;
+    }
+  }
+  bpartial<BS>(d, K, KP, partial);
+}
+
+// partials of <z, r> (column 0) and <z, q> (column 1, when q != null): partial[2][nb][K]
+template <int BS>
+__global__ __launch_bounds__(BS) void k_pcgb_zdots(const int4* __restrict__ cl, int nch, int64_t NP, int K, int KP,
+                                                   const float* __restrict__ z, const double* __restrict__ r,
+                                                   const double* __restrict__ q, double* __restrict__ partial) {
+  NodeRef n;
+  int k = 0;
+  const bool ok = bnode<BS>(cl, nch, 1, K, KP, n, k);
+  double a = 0, b = 0;
+  if (ok) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int64_t i = (c * NP + n.p) * K + k;
+      const double zi = (double)z[i];
+      a += zi * r[i];
+      if (q) b += zi * q[i];
+    }
+  }
+  bpartial<BS>(a, K, KP, partial);
+  if (q) bpartial<BS>(b, K, KP, partial + (int64_t)gridDim.x * K);
+}
+
+// p = z + beta p (first: p = z)
+template <int BS>
+__global__ __launch_bounds__(BS) void k_pcgb_p(const int4* __restrict__ cl, int nch, int64_t NP, int K, int KP,
+                                               double* __restrict__ p, const float* __restrict__ z,
+                                               const double* __restrict__ sc, int first) {
+  NodeRef n;
+  int k = 0;
+  if (!bnode<BS>(cl, nch, 1, K, KP, n, k)) return;
+  const double b = first ? 0.0 : sc[B_BETA * K + k];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const int64_t i = (c * NP + n.p) * K + k;
+    p[i] = (double)z[i] + b * p[i];
+  }
+}
+
+// generic per-case partials of <a, b> (+ <a2, b2> into the second half)
+template <int BS>
+__global__ __launch_bounds__(BS) void k_dotb(const int4* __restrict__ cl, int nch, int64_t NP, int K, int KP,
+                                             const double* __restrict__ a, const double* __restrict__ b,
+                                             const double* __restrict__ a2, const double* __restrict__ b2,
+                                             double* __restrict__ partial) {
+  NodeRef n;
+  int k = 0;
+  const bool ok = bnode<BS>(cl, nch, 1, K, KP, n, k);
+  double s0 = 0, s1 = 0;
+  if (ok) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int64_t i = (c * NP + n.p) * K + k;
+      s0 += a[i] * b[i];
+      if (a2) s1 += a2[i] * b2[i];
+    }
+  }
+  bpartial<BS>(s0, K, KP, partial);
+  if (a2) bpartial<BS>(s1, K, KP, partial + (int64_t)gridDim.x * K);
+}
+
+// stage 1: group g (grid) folds blocks [g*seg, (g+1)*seg) of `ncol` columns of per-case
+// partials part[col][nb][K] into out[col][g][K]
+__global__ __launch_bounds__(256) void k_foldb1(const double* __restrict__ part, int nb, int K, int ncol,
+                                                double* __restrict__ out) {
+  const int G = gridDim.x, g = blockIdx.x;
+  const int seg = (nb + G - 1) / G;
+  const int b0 = g * seg, b1 = min(nb, b0 + seg);
+  for (int t = threadIdx.x; t < ncol * K; t += 256) {
+    const int col = t / K, k = t % K;
+    double s = 0;
+    for (int b = b0; b < b1; ++b) s += part[((int64_t)col * nb + b) * K + k];
+    out[((int64_t)col * G + g) * K + k] = s;
+  }
+}
+
+// stage 2: sc[slot_c * K + k] = sum_g out[c][g][K], then the FCG scalar update `op`
+enum { OPB_NONE = 0, OPB_ALPHA, OPB_BETA, OPB_RN2, OPB_INIT };
+__global__ __launch_bounds__(256) void k_foldb2(const double* __restrict__ st, int G, int K, int ncol, int slot0,
+                                                int slot1, int op, double* __restrict__ sc) {
+  const int k = threadIdx.x;
+  if (k >= K) return;
+  for (int col = 0; col < ncol; ++col) {
+    double s = 0;
+    for (int g = 0; g < G; ++g) s += st[((int64_t)col * G + g) * K + k];
+    sc[(col == 0 ? slot0 : slot1) * K + k] = s;
+  }
+  const bool act = sc[B_ACTIVE * K + k] != 0.0;
+  if (op == OPB_ALPHA) sc[B_ALPHA * K + k] = act && sc[B_PQ * K + k] != 0.0 ? sc[B_RZ * K + k] / sc[B_PQ * K + k] : 0.0;
+  if (op == OPB_RN2) sc[B_ACTIVE * K + k] = act && sc[B_RN2 * K + k] > sc[B_RTOL2 * K + k] ? 1.0 : 0.0;
+  if (op == OPB_BETA) {
+    sc[B_BETA * K + k] = sc[B_RZ * K + k] != 0.0 ? -sc[B_ALPHA * K + k] * sc[B_ZQ * K + k] / sc[B_RZ * K + k] : 0.0;
+    sc[B_RZ * K + k] = sc[B_ZR * K + k];
+  }
+}
+
+// interleave / de-interleave between case-major FP64 (K x 3 x NP) and the batched layout
+__global__ void k_interleave(const int4* __restrict__ cl, int nch, int64_t NP, int K, const double* __restrict__ a,
+                             double* __restrict__ b, int to_batched) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t per = 64LL * K;
+  const int64_t ch = t / per;
+  if (ch >= nch) return;
+  const int i = (int)(t - ch * per), lane = i / K, k = i % K;
+  const int4 c = cl[ch];
+  if (lane >= (c.y >> 16)) return;
+  const int64_t p = (int64_t)c.x + lane;
+#pragma unroll
+  for (int cc = 0; cc < 3; ++cc) {
+    const int64_t ib = (cc * NP + p) * K + k, ic = (int64_t)k * 3 * NP + cc * NP + p;
+    if (to_batched) b[ib] = a[ic];
+    else b[ic] = a[ib];
+  }
+}
+
+template <class A, class B, int BS>
+__global__ __launch_bounds__(BS) void k_cast_b(const int4* __restrict__ cl, int nch, int64_t NP, int K, int KP,
+                                               const A* __restrict__ a, B* __restrict__ b) {
+  NodeRef n;
+  int k = 0;
+  if (!bnode<BS>(cl, nch, 1, K, KP, n, k)) return;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const int64_t i = (c * NP + n.p) * K + k;
+    b[i] = (B)a[i];
+  }
+}
+
+}  // namespace tos

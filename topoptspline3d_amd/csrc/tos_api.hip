@@ -16,6 +16,7 @@
 #include "tos_internal.hpp"
 
 #include "kernels_mg.hip"
+#include "kernels_batch.hip"
 #include "kernels_nn.hip"
 #include "kernels_opt.hip"
 #include "mma.hip"
@@ -680,6 +681,372 @@ void learned_step(Ctx& c, int l, const float* f, float* u, int pre, int post) {
 }
 
 // ---------------------------------------------------------------------------------------
+// K load cases in one solve (no counterpart in the reference, which solves its load
+// cases one after another).  Case-interleaved vectors (one stencil / density read serves
+// all K cases), one FCG per case with per-case scalars and convergence flags on the
+// device, FP32 V-cycle with FP64 outer vectors.  The body of an FCG iteration (operator,
+// updates, V-cycle, dots) is captured once into a hipGraph and replayed; the host reads
+// K residual norms per iteration.  Block sizes (nodes x cases per block) of every kernel
+// class and level are picked by timing the candidates on the context's own data.
+// ---------------------------------------------------------------------------------------
+}  // namespace
+
+namespace tos {
+struct BatchState {
+  int K = 0, KP = 0;
+  std::vector<float*> u32, f32, r32;  // per level, 3 * NP * K
+  double *U = nullptr, *R = nullptr, *P = nullptr, *Q = nullptr, *F = nullptr;
+  double* part = nullptr;   // per-case block partials [2][nb][K]
+  double* fold = nullptr;   // [2][kFoldG][K]
+  double* sc = nullptr;     // [B_COUNT][K]
+  double* h_sc = nullptr;
+  // tuned block sizes
+  int bs_fine_gs = 256, bs_fine_res = 256, bs_transfer = 256;
+  std::vector<int> bs_st_gs, bs_st_res;
+  bool tuned = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<void*> mem;
+};
+}  // namespace tos
+
+namespace {
+
+constexpr int kFoldG = 128;  // first-stage groups of the per-case folds
+constexpr int kVecBS = 256;  // block size of the kernels that write per-case partials
+
+inline unsigned bgrid(int nch, int KP, int bs) { return (unsigned)(((int64_t)nch * 64 * KP + bs - 1) / bs); }
+
+void batch_release(Ctx& c) {
+  if (!c.batch) return;
+  BatchState& B = *c.batch;
+  if (B.exec) (void)hipGraphExecDestroy(B.exec);
+  if (B.graph) (void)hipGraphDestroy(B.graph);
+  for (void* p : B.mem) (void)hipFree(p);
+  if (B.h_sc) (void)hipHostFree(B.h_sc);
+  delete c.batch;
+  c.batch = nullptr;
+}
+
+BatchState& batch_prepare(Ctx& c) {
+  if (c.batch && c.batch->K == c.K) return *c.batch;
+  batch_release(c);
+  c.batch = new BatchState();
+  BatchState& B = *c.batch;
+  B.K = c.K;
+  B.KP = 1;
+  while (B.KP < B.K) B.KP <<= 1;
+  const int nl = (int)c.L.size();
+  B.u32.assign(nl, nullptr);
+  B.f32.assign(nl, nullptr);
+  B.r32.assign(nl, nullptr);
+  for (int l = 0; l < nl; ++l) {
+    const size_t n = 3 * (size_t)c.L[l].v.NP * B.K;
+    B.u32[l] = (float*)raw_alloc(B.mem, c.stream, n * sizeof(float));
+    B.f32[l] = (float*)raw_alloc(B.mem, c.stream, n * sizeof(float));
+    if (l + 1 < nl) B.r32[l] = (float*)raw_alloc(B.mem, c.stream, n * sizeof(float));
+  }
+  const size_t n0 = 3 * (size_t)c.L[0].v.NP * B.K;
+  B.U = (double*)raw_alloc(B.mem, c.stream, n0 * sizeof(double));
+  B.R = (double*)raw_alloc(B.mem, c.stream, n0 * sizeof(double));
+  B.P = (double*)raw_alloc(B.mem, c.stream, n0 * sizeof(double));
+  B.Q = (double*)raw_alloc(B.mem, c.stream, n0 * sizeof(double));
+  B.F = (double*)raw_alloc(B.mem, c.stream, n0 * sizeof(double));
+  const size_t nb = bgrid(c.L[0].n_lin, B.KP, kVecBS);
+  B.part = (double*)raw_alloc(B.mem, c.stream, 2 * std::max<size_t>(nb, 1) * B.K * sizeof(double));
+  B.fold = (double*)raw_alloc(B.mem, c.stream, 2 * (size_t)kFoldG * B.K * sizeof(double));
+  B.sc = (double*)raw_alloc(B.mem, c.stream, (size_t)B_COUNT * B.K * sizeof(double));
+  HIPCHK(hipHostMalloc((void**)&B.h_sc, (size_t)B_COUNT * B.K * sizeof(double)));
+  B.bs_st_gs.assign(nl, 256);
+  B.bs_st_res.assign(nl, 256);
+  HIPCHK(hipStreamSynchronize(c.stream));
+  return B;
+}
+
+#define TOS_BS_SWITCH(bs, GO) \
+  switch (bs) {               \
+    case 64: GO(64); break;   \
+    case 128: GO(128); break; \
+    default: GO(256); break;  \
+  }
+
+// one colour-ordered GS sweep of level l on the interleaved vectors
+void gs_b(Ctx& c, BatchState& B, int l, float* u, const float* f, bool backward) {
+  Level& L = c.L[l];
+  const LevelView& v = L.v;
+  const int bs = l == 0 ? B.bs_fine_gs : B.bs_st_gs[l];
+  for (int ci = 0; ci < 8; ++ci) {
+    const int col = backward ? 7 - ci : ci;
+    const int off = L.col_off[col], nch = L.col_off[col + 1] - off;
+    if (nch == 0) continue;
+    const int4* cl = L.cl_col + off;
+    const unsigned nb = bgrid(nch, B.KP, bs);
+    if (l == 0) {
+#define GO(BS)                                                                                                     \
+  if (backward)                                                                                                    \
+    hipLaunchKernelGGL((k_fine_gs_b<float, true, BS>), dim3(nb), dim3(BS), 0, c.stream, v, cl, nch, B.K, B.KP,     \
+                       (const float*)c.w32, (const float*)c.Ke32, u, f);                                           \
+  else                                                                                                             \
+    hipLaunchKernelGGL((k_fine_gs_b<float, false, BS>), dim3(nb), dim3(BS), 0, c.stream, v, cl, nch, B.K, B.KP,    \
+                       (const float*)c.w32, (const float*)c.Ke32, u, f);
+      TOS_BS_SWITCH(bs, GO)
+#undef GO
+    } else {
+#define GO(BS)                                                                                                     \
+  if (backward)                                                                                                    \
+    hipLaunchKernelGGL((k_st_gs_b<float, true, BS>), dim3(nb), dim3(BS), 0, c.stream, v, cl, nch,                  \
+                       64 * (int64_t)off, B.K, B.KP, (const float*)L.st32, L.NS, u, f);                            \
+  else                                                                                                             \
+    hipLaunchKernelGGL((k_st_gs_b<float, false, BS>), dim3(nb), dim3(BS), 0, c.stream, v, cl, nch,                 \
+                       64 * (int64_t)off, B.K, B.KP, (const float*)L.st32, L.NS, u, f);
+      TOS_BS_SWITCH(bs, GO)
+#undef GO
+    }
+  }
+  LAUNCH_CHECK();
+}
+
+// r = f - A u on level l (FP32, interleaved)
+void resid_b(Ctx& c, BatchState& B, int l, const float* u, const float* f, float* r) {
+  Level& L = c.L[l];
+  if (l == 0) {
+    if (!L.n_lin) return;
+    const int bs = B.bs_fine_res;
+    const unsigned nb = bgrid(L.n_lin, B.KP, bs);
+#define GO(BS)                                                                                                      \
+  hipLaunchKernelGGL((k_fine_apply_b<float, 1, BS>), dim3(nb), dim3(BS), 0, c.stream, L.v, (const int4*)L.cl_lin, \
+                     L.n_lin, B.K, B.KP, (const float*)c.w32, (const float*)c.Ke32, u, f, r, (double*)nullptr,      \
+                     (float*)nullptr);
+    TOS_BS_SWITCH(bs, GO)
+#undef GO
+  } else {
+    if (!L.n_col) return;
+    const int bs = B.bs_st_res[l];
+    const unsigned nb = bgrid(L.n_col, B.KP, bs);
+#define GO(BS)                                                                                                    \
+  hipLaunchKernelGGL((k_st_apply_b<float, 1, BS>), dim3(nb), dim3(BS), 0, c.stream, L.v, (const int4*)L.cl_col, \
+                     L.n_col, B.K, B.KP, (const float*)L.st32, L.NS, u, f, r);
+    TOS_BS_SWITCH(bs, GO)
+#undef GO
+  }
+  LAUNCH_CHECK();
+}
+
+void restrict_b(Ctx& c, BatchState& B, int l, const float* rf, float* fc) {
+  Level& F = c.L[l];
+  Level& C = c.L[l + 1];
+  if (!C.n_lin) return;
+  const int bs = B.bs_transfer;
+  const unsigned nb = bgrid(C.n_lin, B.KP, bs);
+#define GO(BS)                                                                                                     \
+  hipLaunchKernelGGL((k_restrict_b<float, BS>), dim3(nb), dim3(BS), 0, c.stream, F.v, C.v, (const int4*)C.cl_lin, \
+                     C.n_lin, B.K, B.KP, rf, fc);
+  TOS_BS_SWITCH(bs, GO)
+#undef GO
+}
+
+void prolong_b(Ctx& c, BatchState& B, int l, const float* uc, float* uf) {
+  Level& F = c.L[l];
+  Level& C = c.L[l + 1];
+  if (!F.n_lin) return;
+  const int bs = B.bs_transfer;
+  const unsigned nb = bgrid(F.n_lin, B.KP, bs);
+#define GO(BS)                                                                                                         \
+  hipLaunchKernelGGL((k_prolong_add_b<float, BS>), dim3(nb), dim3(BS), 0, c.stream, F.v, C.v, (const int4*)F.cl_lin, \
+                     F.n_lin, B.K, B.KP, uc, uf);
+  TOS_BS_SWITCH(bs, GO)
+#undef GO
+}
+
+void vcycle_b(Ctx& c, BatchState& B, int l, const float* f, float* u, int pre, int post) {
+  Level& L = c.L[l];
+  const int last = (int)c.L.size() - 1;
+  if (l == last && last > 0) {
+    if (c.cn == 0) return;
+    hipLaunchKernelGGL((k_coarse_solve_b<float>), dim3(nblk((int64_t)c.cn * B.K)), dim3(256), 0, c.stream, L.v,
+                       (const int64_t*)c.cidx, c.cn, B.K, (const double*)c.cinv, f, u);
+    LAUNCH_CHECK();
+    return;
+  }
+  if (L.n_lin == 0) return;
+  hipLaunchKernelGGL((k_zero_nodes_b<float, kVecBS>), dim3(bgrid(L.n_lin, B.KP, kVecBS)), dim3(kVecBS), 0, c.stream,
+                     (const int4*)L.cl_lin, L.n_lin, L.v.NP, B.K, B.KP, u);
+  if (l == last) {
+    for (int s = 0; s < std::max(1, pre); ++s) gs_b(c, B, l, u, f, false);
+    for (int s = 0; s < std::max(1, post); ++s) gs_b(c, B, l, u, f, true);
+    return;
+  }
+  for (int s = 0; s < pre; ++s) gs_b(c, B, l, u, f, false);
+  resid_b(c, B, l, u, f, B.r32[l]);
+  restrict_b(c, B, l, B.r32[l], B.f32[l + 1]);
+  vcycle_b(c, B, l + 1, B.f32[l + 1], B.u32[l + 1], pre, post);
+  prolong_b(c, B, l, B.u32[l + 1], u);
+  LAUNCH_CHECK();
+  for (int s = 0; s < post; ++s) gs_b(c, B, l, u, f, true);
+}
+
+// per-case fold of `ncol` partial columns of a launch with nb blocks into sc
+void foldb(Ctx& c, BatchState& B, unsigned nb, int ncol, int s0, int s1, int op) {
+  hipLaunchKernelGGL(k_foldb1, dim3(kFoldG), dim3(256), 0, c.stream, (const double*)B.part, (int)nb, B.K, ncol,
+                     B.fold);
+  hipLaunchKernelGGL(k_foldb2, dim3(1), dim3(256), 0, c.stream, (const double*)B.fold, kFoldG, B.K, ncol, s0, s1, op,
+                     B.sc);
+}
+
+// one FCG iteration (captured): q = A p, alpha, u/r update, |r|^2, V-cycle, beta, p
+void batch_iteration(Ctx& c, BatchState& B, int pre, int post) {
+  Level& L0 = c.L[0];
+  const int4* cl = L0.cl_lin;
+  const int nch = L0.n_lin;
+  const int64_t NP = L0.v.NP;
+  const unsigned nb = bgrid(nch, B.KP, kVecBS);
+  hipLaunchKernelGGL((k_fine_apply_b<double, 0, kVecBS>), dim3(nb), dim3(kVecBS), 0, c.stream, L0.v, cl, nch, B.K,
+                     B.KP, (const double*)c.w64, (const double*)c.Ke64, (const double*)B.P, (const double*)nullptr,
+                     B.Q, B.part, (float*)nullptr);
+  foldb(c, B, nb, 1, B_PQ, 0, OPB_ALPHA);
+  hipLaunchKernelGGL((k_pcgb_ur<kVecBS>), dim3(nb), dim3(kVecBS), 0, c.stream, cl, nch, NP, B.K, B.KP, B.U, B.R,
+                     (const double*)B.P, (const double*)B.Q, (const double*)B.sc, B.f32[0], B.part);
+  foldb(c, B, nb, 1, B_RN2, 0, OPB_RN2);
+  vcycle_b(c, B, 0, B.f32[0], B.u32[0], pre, post);
+  hipLaunchKernelGGL((k_pcgb_zdots<kVecBS>), dim3(nb), dim3(kVecBS), 0, c.stream, cl, nch, NP, B.K, B.KP,
+                     (const float*)B.u32[0], (const double*)B.R, (const double*)B.Q, B.part);
+  foldb(c, B, nb, 2, B_ZR, B_ZQ, OPB_BETA);
+  hipLaunchKernelGGL((k_pcgb_p<kVecBS>), dim3(nb), dim3(kVecBS), 0, c.stream, cl, nch, NP, B.K, B.KP, B.P,
+                     (const float*)B.u32[0], (const double*)B.sc, 0);
+  LAUNCH_CHECK();
+}
+
+// time `f` (reps launches after one warm-up) on the context stream, ms per launch
+template <class F>
+float time_ms(Ctx& c, F&& f, int reps = 3) {
+  hipEvent_t a, b;
+  HIPCHK(hipEventCreate(&a));
+  HIPCHK(hipEventCreate(&b));
+  f();
+  HIPCHK(hipEventRecord(a, c.stream));
+  for (int i = 0; i < reps; ++i) f();
+  HIPCHK(hipEventRecord(b, c.stream));
+  HIPCHK(hipEventSynchronize(b));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return ms / reps;
+}
+
+// pick the block size of every tuned kernel class by timing 64 / 128 / 256 on this
+// context's levels (the V-cycle vectors are scratch at this point)
+void batch_tune(Ctx& c, BatchState& B) {
+  const int cand[3] = {64, 128, 256};
+  auto pick = [&](auto&& run) {
+    int best = 256;
+    float tb = 1e30f;
+    for (int bs : cand) {
+      const float t = time_ms(c, [&]() { run(bs); });
+      if (t < tb) { tb = t; best = bs; }
+    }
+    return best;
+  };
+  const int nl = (int)c.L.size();
+  B.bs_fine_gs = pick([&](int bs) { B.bs_fine_gs = bs; gs_b(c, B, 0, B.u32[0], B.f32[0], false); });
+  if (nl > 1) B.bs_fine_res = pick([&](int bs) { B.bs_fine_res = bs; resid_b(c, B, 0, B.u32[0], B.f32[0], B.r32[0]); });
+  for (int l = 1; l + 1 < nl; ++l) {
+    B.bs_st_gs[l] = pick([&](int bs) { B.bs_st_gs[l] = bs; gs_b(c, B, l, B.u32[l], B.f32[l], false); });
+    B.bs_st_res[l] = pick([&](int bs) { B.bs_st_res[l] = bs; resid_b(c, B, l, B.u32[l], B.f32[l], B.r32[l]); });
+  }
+  if (nl > 1)
+    B.bs_transfer = pick([&](int bs) { B.bs_transfer = bs; prolong_b(c, B, 0, B.u32[1], B.r32[0]); });
+  B.tuned = true;
+}
+
+int solve_batched(Ctx& c, const tos_solve_opts& o, tos_solve_stats* st) {
+  BatchState& B = batch_prepare(c);
+  Level& L0 = c.L[0];
+  const int4* cl = L0.cl_lin;
+  const int nch = L0.n_lin, K = B.K;
+  const int64_t NP = L0.v.NP, n3k = 3 * NP * K;
+  if (nch == 0) {  // no free node: u = 0 for every case
+    for (int k = 0; k < K && st && k < 64; ++k) { st->iterations[k] = 0; st->rel_res[k] = 0; }
+    return TOS_OK;
+  }
+  if (!B.tuned) batch_tune(c, B);
+  const bool prof = c.prof_on;
+  const unsigned nb = bgrid(nch, B.KP, kVecBS);
+  const unsigned ni = (unsigned)(((int64_t)nch * 64 * K + 255) / 256);
+  // F, U -> interleaved; r = f - A u; |f|^2, |r|^2
+  hipLaunchKernelGGL(k_interleave, dim3(ni), dim3(256), 0, c.stream, cl, nch, NP, K, (const double*)c.F, B.F, 1);
+  if (o.warm_start)
+    hipLaunchKernelGGL(k_interleave, dim3(ni), dim3(256), 0, c.stream, cl, nch, NP, K, (const double*)c.U, B.U, 1);
+  else
+    hipLaunchKernelGGL(k_zero<double>, dim3(nblk(n3k)), dim3(256), 0, c.stream, B.U, n3k);
+  hipLaunchKernelGGL((k_fine_apply_b<double, 1, kVecBS>), dim3(nb), dim3(kVecBS), 0, c.stream, L0.v, cl, nch, K,
+                     B.KP, (const double*)c.w64, (const double*)c.Ke64, (const double*)B.U, (const double*)B.F, B.R,
+                     (double*)nullptr, B.f32[0]);
+  hipLaunchKernelGGL((k_dotb<kVecBS>), dim3(nb), dim3(kVecBS), 0, c.stream, cl, nch, NP, K, B.KP, (const double*)B.F,
+                     (const double*)B.F, (const double*)B.R, (const double*)B.R, B.part);
+  foldb(c, B, nb, 2, B_NF2, B_RN2, OPB_NONE);
+  download(c.stream, B.h_sc, B.sc, (size_t)B_COUNT * K * sizeof(double));
+  std::vector<double> nf(K), rt2(K, 0.0), act(K, 0.0);
+  int n_act = 0;
+  for (int k = 0; k < K; ++k) {
+    nf[k] = std::sqrt(B.h_sc[B_NF2 * K + k]);
+    rt2[k] = o.rtol * o.rtol * B.h_sc[B_NF2 * K + k];
+    act[k] = (nf[k] > 0 && B.h_sc[B_RN2 * K + k] > rt2[k]) ? 1.0 : 0.0;
+    n_act += act[k] != 0.0;
+    if (!std::isfinite(B.h_sc[B_RN2 * K + k])) { c.err = "NaN/Inf in the initial residual"; return TOS_E_NAN; }
+  }
+  upload(c.stream, B.sc + B_RTOL2 * K, rt2.data(), K * sizeof(double));
+  upload(c.stream, B.sc + B_ACTIVE * K, act.data(), K * sizeof(double));
+  std::vector<int> its(K, 0);
+  std::vector<double> rel(K, 0.0);
+  for (int k = 0; k < K; ++k) rel[k] = nf[k] > 0 ? std::sqrt(B.h_sc[B_RN2 * K + k]) / nf[k] : 0.0;
+  int it = 0;
+  if (n_act) {
+    vcycle_b(c, B, 0, B.f32[0], B.u32[0], o.pre, o.post);
+    hipLaunchKernelGGL((k_pcgb_p<kVecBS>), dim3(nb), dim3(kVecBS), 0, c.stream, cl, nch, NP, K, B.KP, B.P,
+                       (const float*)B.u32[0], (const double*)B.sc, 1);
+    hipLaunchKernelGGL((k_pcgb_zdots<kVecBS>), dim3(nb), dim3(kVecBS), 0, c.stream, cl, nch, NP, K, B.KP,
+                       (const float*)B.u32[0], (const double*)B.R, (const double*)nullptr, B.part);
+    foldb(c, B, nb, 1, B_RZ, 0, OPB_NONE);
+    LAUNCH_CHECK();
+    if (!B.exec) {
+      c.prof_on = false;
+      HIPCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+      batch_iteration(c, B, o.pre, o.post);
+      HIPCHK(hipStreamEndCapture(c.stream, &B.graph));
+      HIPCHK(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));
+      c.prof_on = prof;
+    }
+    while (it < o.maxit) {
+      HIPCHK(hipGraphLaunch(B.exec, c.stream));
+      ++it;
+      download(c.stream, B.h_sc, B.sc, (size_t)B_COUNT * K * sizeof(double));
+      int still = 0;
+      for (int k = 0; k < K; ++k) {
+        const double rn2 = B.h_sc[B_RN2 * K + k];
+        if (!std::isfinite(rn2)) { c.err = "NaN/Inf in batched FCG"; return TOS_E_NAN; }
+        if (act[k] != 0.0) {
+          its[k] = it;
+          rel[k] = nf[k] > 0 ? std::sqrt(rn2) / nf[k] : 0.0;
+        }
+        act[k] = B.h_sc[B_ACTIVE * K + k];
+        still += act[k] != 0.0;
+      }
+      if (!still) break;
+    }
+  }
+  hipLaunchKernelGGL(k_interleave, dim3(ni), dim3(256), 0, c.stream, cl, nch, NP, K, (const double*)B.U, c.U, 0);
+  LAUNCH_CHECK();
+  HIPCHK(hipStreamSynchronize(c.stream));
+  int rc = TOS_OK;
+  for (int k = 0; k < K; ++k) {
+    if (st && k < 64) { st->iterations[k] = its[k]; st->rel_res[k] = rel[k]; }
+    if (rel[k] > o.rtol) rc = TOS_E_NOCONV;
+  }
+  return rc;
+}
+
+// ---------------------------------------------------------------------------------------
 // Coarsest system (buildCoarsestSystem grid/Grid.cpp:3614-3669 assembles it on the host
 // and solves with Eigen BDCSVD each V-cycle): dense assembly on the device, blocked
 // symmetric Gauss-Jordan sweep leaving -A^{-1}, applied as a matvec in the V-cycle.
@@ -1239,6 +1606,7 @@ TOS_EXPORT int tos_ctx_destroy(tos_ctx* c) {
   (void)hipSetDevice(c->dev);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   learn_release(*c);
+  batch_release(*c);
   for (void* p : c->allocs) (void)hipFree(p);
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
   if (c->stage) (void)hipFree(c->stage);
@@ -1491,6 +1859,35 @@ TOS_EXPORT int tos_solve(tos_ctx* c, const tos_solve_opts* o, tos_solve_stats* s
     if (rc_all == TOS_E_NOCONV) c->err = "solver did not reach rtol";
     if (st) st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return rc_all;
+  });
+}
+
+TOS_EXPORT int tos_solve_batched(tos_ctx* c, const tos_solve_opts* o, tos_solve_stats* st) {
+  return guard(c, [&]() {
+    REQUIRE(o && o->rtol > 0 && o->maxit >= 0 && o->pre >= 0 && o->post >= 0, "bad options");
+    REQUIRE(o->precision == 1 && o->use_pcg, "the batched solve is FCG with an FP32 V-cycle");
+    if (!c->op_ready) { c->err = "update_operator first"; return TOS_E_STATE; }
+    const auto t0 = std::chrono::steady_clock::now();
+    const int rc = solve_batched(*c, *o, st);
+    if (rc == TOS_E_NOCONV) c->err = "solver did not reach rtol";
+    if (st) st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return rc;
+  });
+}
+
+TOS_EXPORT int tos_batch_tuning(tos_ctx* c, char* buf, int len) {
+  return guard(c, [&]() {
+    REQUIRE(buf && len > 0, "bad buffer");
+    std::string s;
+    if (c->batch && c->batch->tuned) {
+      const BatchState& B = *c->batch;
+      s = "K=" + std::to_string(B.K) + " fine_gs=" + std::to_string(B.bs_fine_gs) +
+          " fine_res=" + std::to_string(B.bs_fine_res) + " transfer=" + std::to_string(B.bs_transfer);
+      for (size_t l = 1; l + 1 < c->L.size(); ++l)
+        s += " L" + std::to_string(l) + ":gs=" + std::to_string(B.bs_st_gs[l]) + ",res=" + std::to_string(B.bs_st_res[l]);
+    }
+    std::snprintf(buf, (size_t)len, "%s", s.c_str());
+    return TOS_OK;
   });
 }
 

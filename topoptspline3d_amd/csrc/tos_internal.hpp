@@ -24,6 +24,7 @@
 namespace tos {
 
 struct LearnState;  // learned coarse-level correction (network, pair logging)
+struct BatchState;  // K load cases solved together (interleaved vectors, graph, tuning)
 
 struct LevelView {  // passed by value to kernels
   int nx, ny, nz;     // node dims
@@ -138,6 +139,7 @@ struct Ctx {
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
   LearnState* learn = nullptr;
+  BatchState* batch = nullptr;
 };
 
 constexpr int kRedBlocks = 1024;  // fixed partial count -> deterministic reductions
