@@ -9,8 +9,14 @@ coarsest factor), flexible PCG to 1e-6 relative residual (FP32 V-cycle, FP64 out
 residual), compliance, sensitivity, filter, spline adjoint of the compliance and volume
 sensitivities, MMA step.  All inputs are resident on the GPU before the timed region.
 
+Before the timed region (c2): the learned level-2 correction is trained on pairs logged
+from 200 optimisation iterations on 128^3 ellipsoid domains.  Timed: `steps` iterations of
+plain MG-PCG, then `steps` iterations with the learned correction attached (`value`), then
+plain FP64 iterations; plus solves from zero on the final operator in FP64, mixed and
+learned mode.
+
 Prints one JSON line.  Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
-[--workload c2|c1|c0] [--no-cpu-baseline]
+[--workload c2|c1|c0] [--no-cpu-baseline] [--no-learned] [--no-fp64-compare]
 """
 import argparse
 import json
@@ -208,6 +214,49 @@ def cpu_baseline(wl, gpu_its):
                        f" iterations) + sensitivity/filter/adjoint {t_post:.2f}s; MMA step not included")}
 
 
+def learned_phase(api, n_dom, iters, steps, level=2):
+    """Training of the learned level-2 correction: residual / error pairs logged from the
+    solver's own V-cycles over `iters` spline + MMA iterations on each of `n_dom` 128^3
+    ellipsoid-union domains (semi-axes scaled to 15-45 voxels), then Adam on 90 % of the
+    pairs; the rest is held out."""
+    from topoptspline3d_amd import learned as LN
+    from topoptspline3d_amd.domains import ellipsoids
+    t0 = time.perf_counter()
+    ds = None
+    its = []
+    for d in range(n_dom):
+        solid, support, f, shell, meta = ellipsoids(128, 128, 128, seed=100 + d, axes=(15, 45))
+        wl = dict(name="train", desc="", solid=solid, support=support, f=f, shell=shell, meta=meta,
+                  spline=(1, 4, 4), optimizer="mma", move=0.06, rtol=1e-6)
+        loop = Loop(api, wl, 1)
+        if ds is None:
+            ds = LN.PairDataset(api, LN.level_box(loop.ctx, level), n_dom * iters * 40)
+        LN.log_pairs(loop.ctx, ds, level=level, n_inner=8, stride=1)
+        for _ in range(iters):
+            loop.step()
+        its += loop.its
+        LN.log_pairs(loop.ctx, None)
+        loop.ctx.close()
+        log(f"[bench] learned: domain {d} fill {meta.get('fill', 0):.3f}, pairs so far {len(ds)}")
+    t_gen = time.perf_counter() - t0
+    n = len(ds)
+    ntr = max(1, int(0.9 * n))
+    net = LN.UNet(api, seed=7)
+    before = net.evaluate(ds, ntr, n - ntr) if n > ntr else None
+    t = time.perf_counter()
+    lf, ll = net.train(ds, 0, ntr, steps=steps, batch=8, lr=1e-3, seed=3)
+    t_train = time.perf_counter() - t
+    after = net.evaluate(ds, ntr, n - ntr) if n > ntr else None
+    info = {"pairs": n, "box": list(ds.box), "domains": n_dom, "to_iterations": n_dom * iters,
+            "mean_pcg_its": round(float(np.mean(its)), 2), "generate_s": round(t_gen, 1),
+            "train_steps": steps, "batch": 8, "train_s": round(t_train, 1),
+            "train_loss_first_last": [round(lf, 4), round(ll, 4)],
+            "heldout_loss_untrained_trained": [round(before, 4) if before is not None else None,
+                                               round(after, 4) if after is not None else None]}
+    log(f"[bench] learned: {info}")
+    return net, ds, info
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -217,11 +266,20 @@ def main():
     ap.add_argument("--precision", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-compare", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--learn-domains", type=int, default=4)
+    ap.add_argument("--learn-iters", type=int, default=50)
+    ap.add_argument("--train-steps", type=int, default=1500)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
     from topoptspline3d_amd.backend import hip_api
+    from topoptspline3d_amd import learned as LN
+    net, learn_info = None, None
+    if not args.no_learned and args.workload == "c2":
+        net, ds, learn_info = learned_phase(hip_api(), args.learn_domains, args.learn_iters, args.train_steps)
+        ds.close()
     t0 = time.perf_counter()
     wl = make_workload(args.workload)
     log(f"[bench] workload {wl['name']} generated in {time.perf_counter() - t0:.1f}s {wl['meta'].get('fill', '')}")
@@ -233,18 +291,31 @@ def main():
     for i in range(args.warmup):
         t = time.perf_counter(); loop.step()
         log(f"[bench] warmup {i}: {1e3 * (time.perf_counter() - t):.1f} ms, pcg its {loop.its[-1]}")
-    sync()
+
+    def timed(tag, steps):
+        sync()
+        n0 = len(loop.its)
+        t = time.perf_counter()
+        for i in range(steps):
+            loop.step()
+            log(f"[bench] {tag} step {i}: pcg its {loop.its[-1]} rel_res {loop.res[-1]:.2e} c {loop.c[-1]:.6e}")
+        sync()
+        return (time.perf_counter() - t) * 1e3 / steps, loop.its[n0:], max(loop.res[n0:])
+
+    # plain MG-PCG (FP32 V-cycle, FP64 refinement)
+    ms_plain, its_plain, res_plain = timed("plain", args.steps)
+    learned = None
+    if net is not None:
+        LN.attach(loop.ctx, net, level=2)
     kernel_stats(api, loop.ctx, True)
-    n0 = len(loop.its)
-    t = time.perf_counter()
-    for i in range(args.steps):
-        loop.step()
-        log(f"[bench] step {i}: pcg its {loop.its[-1]} rel_res {loop.res[-1]:.2e} c {loop.c[-1]:.6e}")
-    sync()
-    elapsed = time.perf_counter() - t
+    ms, its, res_max = timed("learned" if net is not None else "plain", args.steps)
     stats = kernel_stats(api, loop.ctx, False)
-    ms = elapsed * 1e3 / args.steps
-    its = loop.its[n0:]
+    if net is not None:
+        tried, kept, step = LN.stats(loop.ctx)
+        learned = dict(learn_info, corrections_tried=tried, corrections_kept=kept,
+                       mean_step_kept=round(step, 4), ms_per_iter=round(ms, 3),
+                       pcg_iterations=its)
+        LN.attach(loop.ctx, None)
     line = {
         "metric": METRIC, "value": round(ms, 3), "unit": "ms/TO-iter", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
@@ -254,18 +325,34 @@ def main():
         "config": {"workload": wl["desc"], "grid": list(wl["solid"].shape[::-1]),
                    "active_elements": info["n_active_elements"][0], "active_nodes": nv,
                    "mg_levels": info["n_levels"], "rtol": wl["rtol"], "precision": args.precision,
-                   "pcg_iterations": its, "max_rel_res": max(loop.res[n0:])},
+                   "solver": ("FCG, FP32 V-cycle + learned level-2 correction, FP64 refinement"
+                              if net is not None else "FCG, FP32 V-cycle, FP64 refinement"),
+                   "pcg_iterations": its, "max_rel_res": res_max},
         "roofline": roofline(stats, info, nv, args.steps),
+        "plain_mixed": {"ms_per_iter": round(ms_plain, 3), "pcg_iterations": its_plain,
+                        "max_rel_res": res_plain},
     }
+    if learned is not None:
+        line["learned"] = learned
     if not args.no_fp64_compare:
-        # same operator, solve from zero to the same tolerance in plain FP64 and mixed
+        # plain FP64 MG-PCG, whole iterations, continuing the same trajectory
+        loop.precision = 0
+        ms64, its64, res64 = timed("fp64", max(1, min(2, args.steps)))
+        loop.precision = args.precision
+        line["plain_fp64"] = {"ms_per_iter": round(ms64, 3), "pcg_iterations": its64, "max_rel_res": res64,
+                              "speedup_of_value": round(ms64 / ms, 3)}
+        # same operator, solve from zero to the same tolerance: FP64, mixed, learned
         cmp = {}
-        for prec in (0, 1):
+        modes = [("fp64", 0, False), ("mixed", 1, False)] + ([("learned", 1, True)] if net is not None else [])
+        for tag, prec, use_net in modes:
+            if use_net:
+                LN.attach(loop.ctx, net, level=2)
             loop.ctx.set_displacement(np.zeros(3 * loop.ctx.nn))
             st = loop.ctx.solve(rtol=wl["rtol"], maxit=400, precision=prec, warm_start=True)
-            cmp["fp64" if prec == 0 else "mixed"] = {"solve_ms": round(st["seconds"] * 1e3, 2),
-                                                     "iterations": st["iterations"][0],
-                                                     "rel_res": st["rel_res"][0]}
+            if use_net:
+                LN.attach(loop.ctx, None)
+            cmp[tag] = {"solve_ms": round(st["seconds"] * 1e3, 2), "iterations": st["iterations"][0],
+                        "rel_res": st["rel_res"][0]}
         line["solve_from_zero"] = cmp
     if not args.no_cpu_baseline:
         t = time.perf_counter()

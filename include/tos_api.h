@@ -259,18 +259,19 @@ int tos_dataset_get(tos_dataset* ds, int i, float* feat4, float* target, uint8_t
 int tos_unet_train(tos_unet* net, tos_dataset* ds, int first, int count, int steps, int batch,
                    double lr, uint64_t seed, double* loss_first, double* loss_last);
 int tos_unet_eval(tos_unet* net, tos_dataset* ds, int first, int count, double* mean_loss);
-/* mode 1: inside FP32 V-cycles the network corrects level `level` after its coarse-grid
- * correction; the correction is kept only when the scaled residual falls below theta
- * times its value before (otherwise plain multigrid continues unchanged).  mode 0
- * detaches.  The context keeps the pointer: the network must outlive the attachment. */
-int tos_ctx_learn_attach(tos_ctx* ctx, tos_unet* net, int level, int mode, double theta);
+/* mode 1: inside FP32 V-cycles the network proposes a correction e of level `level` after
+ * its coarse-grid correction.  It is applied with the energy-optimal step
+ * alpha = e.r / e.Ae and only when 0.2 <= alpha <= 5 (it lowers the A-norm error and is
+ * roughly of the right size); otherwise plain multigrid continues unchanged.  After 3 rejections in a row the network pauses until
+ * the next solve.  mode 0 detaches.  The context keeps the pointer: the network must
+ * outlive the attachment. */
+int tos_ctx_learn_attach(tos_ctx* ctx, tos_unet* net, int level, int mode);
 /* log (features, target) pairs at `level` of every `stride`-th FP32 V-cycle into ds (grid =
  * the level's node box); target = the level's error after `n_inner` nested V-cycles.
  * ds = NULL stops logging. */
 int tos_ctx_learn_log(tos_ctx* ctx, tos_dataset* ds, int level, int n_inner, int stride);
-/* corrections tried / kept since the last attach, and the mean over the tries of
- * |D^{-1/2}(r - A e)| / |D^{-1/2} r| (below 1: the network's proposal lowered the residual) */
-int tos_ctx_learn_stats(tos_ctx* ctx, int* tried, int* accepted, double* mean_ratio);
+/* corrections tried / kept since the last attach and the mean step alpha of those kept */
+int tos_ctx_learn_stats(tos_ctx* ctx, int* tried, int* accepted, double* mean_step);
 
 #ifdef __cplusplus
 }

@@ -111,7 +111,7 @@ LEARN_SIGNATURES = {
     "dataset_get": (C.c_int, [vp, C.c_int, f32p, f32p, u8p]),
     "unet_train": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_uint64, f64p, f64p]),
     "unet_eval": (C.c_int, [vp, vp, C.c_int, C.c_int, f64p]),
-    "ctx_learn_attach": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_double]),
+    "ctx_learn_attach": (C.c_int, [vp, vp, C.c_int, C.c_int]),
     "ctx_learn_log": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_int]),
     "ctx_learn_stats": (C.c_int, [vp, i32p, i32p, f64p]),
 }

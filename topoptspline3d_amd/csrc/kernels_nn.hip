@@ -567,26 +567,95 @@ __global__ void k_node_logrho(LevelView F, const uint8_t* __restrict__ eact, con
   out[v] = (float)(log10(m) / 3.0);
 }
 
-// accept the correction when it lowers the scaled residual: u += e if n2[1] < theta n2[0];
-// block 0 / thread 0 counts tries, acceptances and sums |r_new| / |r_old| (one writer)
-__global__ __launch_bounds__(256) void k_nn_accept(LevelView L, const int4* __restrict__ cl, int nch,
-                                                   const double* __restrict__ n2, double theta,
-                                                   const float* __restrict__ e, float* __restrict__ u,
-                                                   int* __restrict__ counters, double* __restrict__ ratio_sum) {
-  const bool acc = n2[1] < theta * n2[0];
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    counters[0] += 1;
-    if (acc) counters[1] += 1;
-    if (n2[0] > 0) ratio_sum[0] += sqrt(n2[1] / n2[0]);
+// block partials of e.r (column 0) and e.t (column 1) over the free nodes
+__global__ __launch_bounds__(256) void k_nn_dots(LevelView L, const int4* __restrict__ cl, int nch,
+                                                 const float* __restrict__ e, const float* __restrict__ r,
+                                                 const float* __restrict__ t, double* __restrict__ partial) {
+  __shared__ double sh[2][4];
+  double a = 0, b = 0;
+  const int ch = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (ch < nch) {
+    const int4 c = cl[ch];
+    if (lane < (c.y >> 16)) {
+      const int64_t p = (int64_t)c.x + lane;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const double ek = e[k * L.NP + p];
+        a += ek * r[k * L.NP + p];
+        b += ek * t[k * L.NP + p];
+      }
+    }
   }
-  if (!acc) return;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    a += __shfl_down(a, o, 64);
+    b += __shfl_down(b, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    sh[0][threadIdx.x >> 6] = a;
+    sh[1][threadIdx.x >> 6] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x] = (sh[0][0] + sh[0][1]) + (sh[0][2] + sh[0][3]);
+    partial[gridDim.x + blockIdx.x] = (sh[1][0] + sh[1][1]) + (sh[1][2] + sh[1][3]);
+  }
+}
+
+// out[0] = sum partial[0..np), out[1] = sum partial[np..2np)  (fixed order)
+__global__ __launch_bounds__(256) void k_sum2_partials(const double* __restrict__ partial, int np,
+                                                       double* __restrict__ out) {
+  __shared__ double sh[2][4];
+  double a = 0, b = 0;
+  for (int i = threadIdx.x; i < np; i += 256) {
+    a += partial[i];
+    b += partial[np + i];
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    a += __shfl_down(a, o, 64);
+    b += __shfl_down(b, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    sh[0][threadIdx.x >> 6] = a;
+    sh[1][threadIdx.x >> 6] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    out[0] = (sh[0][0] + sh[0][1]) + (sh[0][2] + sh[0][3]);
+    out[1] = (sh[1][0] + sh[1][1]) + (sh[1][2] + sh[1][3]);
+  }
+}
+
+// energy-optimal use of the proposal e for A x = r: with er = e.r and eAe = e.r - e.t
+// (t = r - A e) the step alpha = er / eAe minimises the A-norm error along e.  The
+// correction is kept (u += alpha e) only when alpha lies in [amin, amax]: alpha > 0 means
+// it lowers the energy error, and a step far from 1 means the proposal is mostly a wrong
+// direction or scale, so plain multigrid is left alone.  ab = {e.r, e.t}; block 0 /
+// thread 0 keeps the statistics st = {tries, kept, sum of alpha}.
+__global__ __launch_bounds__(256) void k_nn_accept(LevelView L, const int4* __restrict__ cl, int nch,
+                                                   const double* __restrict__ ab, double amin, double amax,
+                                                   const float* __restrict__ e, float* __restrict__ u,
+                                                   double* __restrict__ st) {
+  const double er = ab[0], eAe = ab[0] - ab[1];
+  double alpha = (er > 0 && eAe > 0) ? er / eAe : 0.0;
+  if (alpha < amin || alpha > amax) alpha = 0.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st[0] += 1;
+    if (alpha > 0) {
+      st[1] += 1;
+      st[2] += alpha;
+    }
+  }
+  if (!(alpha > 0)) return;
   const int ch = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (ch >= nch) return;
   const int4 c = cl[ch];
   if (lane >= (c.y >> 16)) return;
   const int64_t p = (int64_t)c.x + lane;
+  const float a = (float)alpha;
 #pragma unroll
-  for (int k = 0; k < 3; ++k) u[k * L.NP + p] += e[k * L.NP + p];
+  for (int k = 0; k < 3; ++k) u[k * L.NP + p] += a * e[k * L.NP + p];
 }
 
 // y += a on the free nodes (3 components)

@@ -554,15 +554,13 @@ struct LearnState {
   tos_unet* net = nullptr;
   int level = -1;
   int mode = 0;
-  double theta = 1.0;
   tos_dataset* ds = nullptr;
   int n_inner = 8, stride = 1;
   int64_t calls = 0;
   bool busy = false;
   float* lrho = nullptr;
   float *e = nullptr, *t1 = nullptr, *t2 = nullptr, *t3 = nullptr;
-  int* counters = nullptr;
-  double* ratio_sum = nullptr;
+  bool paused = false;  // set by the solver after repeated rejections, until the next solve
   NetActs* acts = nullptr;
   std::vector<void*> mem;
 };
@@ -570,9 +568,11 @@ struct LearnState {
 
 namespace {
 
+constexpr double kNNStepMin = 0.2, kNNStepMax = 5.0;  // accepted energy-optimal steps
+
 bool learned_here(const Ctx& c, int l) {
   const LearnState* S = c.learn;
-  return S && l == S->level && !S->busy && ((S->mode == 1 && S->net) || S->ds);
+  return S && l == S->level && !S->busy && ((S->mode == 1 && S->net && !S->paused) || S->ds);
 }
 
 void learn_logrho(Ctx& c) {
@@ -601,8 +601,6 @@ void learn_prepare(Ctx& c, int level) {
   S.t1 = (float*)raw_alloc(S.mem, c.stream, 3 * L.v.NP * sizeof(float));
   S.t2 = (float*)raw_alloc(S.mem, c.stream, 3 * L.v.NP * sizeof(float));
   S.t3 = (float*)raw_alloc(S.mem, c.stream, 3 * L.v.NP * sizeof(float));
-  S.counters = (int*)raw_alloc(S.mem, c.stream, 2 * sizeof(int));
-  S.ratio_sum = (double*)raw_alloc(S.mem, c.stream, sizeof(double));
   learn_logrho(c);
   HIPCHK(hipStreamSynchronize(c.stream));
 }
@@ -673,10 +671,15 @@ void learned_step(Ctx& c, int l, const float* f, float* u, int pre, int post) {
                        (const float*)L.st32, L.NS, (const int*)L.slot, (const double*)(c.scal + S_NN0), nfree, S.e);
     LAUNCH_CHECK();
   }
-  apply_level<float, 1>(c, l, S.e, r, S.t2);  // residual after the correction
-  scaled_norm(c, l, S.t2, S_NN1);
-  hipLaunchKernelGGL(k_nn_accept, dim3(cblk(L.n_lin)), dim3(256), 0, c.stream, L.v, (const int4*)L.cl_lin, L.n_lin,
-                     (const double*)(c.scal + S_NN0), S.theta, (const float*)S.e, u, S.counters, S.ratio_sum);
+  apply_level<float, 1>(c, l, S.e, r, S.t2);  // t2 = r - A e
+  const unsigned nb = cblk(L.n_lin);
+  hipLaunchKernelGGL(k_nn_dots, dim3(nb), dim3(256), 0, c.stream, L.v, (const int4*)L.cl_lin, L.n_lin,
+                     (const float*)S.e, (const float*)r, (const float*)S.t2, c.partial);
+  hipLaunchKernelGGL(k_sum2_partials, dim3(1), dim3(256), 0, c.stream, (const double*)c.partial, (int)nb,
+                     c.scal + S_NN1);
+  hipLaunchKernelGGL(k_nn_accept, dim3(nb), dim3(256), 0, c.stream, L.v, (const int4*)L.cl_lin, L.n_lin,
+                     (const double*)(c.scal + S_NN1), kNNStepMin, kNNStepMax, (const float*)S.e, u,
+                     c.scal + S_NN_TRIED);
   LAUNCH_CHECK();
 }
 
@@ -1136,6 +1139,18 @@ int solve_case(Ctx& c, int k, const tos_solve_opts& o, int* its, double* relres)
     return TOS_E_NAN;
   }
   int it = 0;
+  // the learned correction pauses for the rest of this solve after 3 rejections in a row
+  LearnState* LS = (c.learn && c.learn->mode == 1) ? c.learn : nullptr;
+  int nn_rej = 0;
+  double nn_tried = hs[S_NN_TRIED], nn_kept = hs[S_NN_KEPT];
+  if (LS) LS->paused = false;
+  auto nn_watch = [&](const double* h) {
+    if (!LS || LS->paused) return;
+    if (h[S_NN_TRIED] > nn_tried) nn_rej = h[S_NN_KEPT] > nn_kept ? 0 : nn_rej + 1;
+    nn_tried = h[S_NN_TRIED];
+    nn_kept = h[S_NN_KEPT];
+    if (nn_rej >= 3) LS->paused = true;
+  };
   if (!o.use_pcg) {
     while (rn / nf > o.rtol && it < o.maxit) {
       precondition(c, o.precision, o.pre, o.post, f32, true);
@@ -1175,7 +1190,9 @@ int solve_case(Ctx& c, int k, const tos_solve_opts& o, int* its, double* relres)
       }
       LAUNCH_CHECK();
       ++it;
-      rn = std::sqrt(read_scal(c)[S_RN2]);
+      const double* h = read_scal(c);
+      rn = std::sqrt(h[S_RN2]);
+      nn_watch(h);
       if (!std::isfinite(rn)) { c.err = "NaN/Inf in PCG"; return TOS_E_NAN; }
       if (rn / nf <= o.rtol) break;
       precondition(c, o.precision, o.pre, o.post, f32, false);
@@ -2706,7 +2723,7 @@ TOS_EXPORT int tos_unet_eval(tos_unet* N, tos_dataset* D, int first, int count, 
   });
 }
 
-TOS_EXPORT int tos_ctx_learn_attach(tos_ctx* c, tos_unet* net, int level, int mode, double theta) {
+TOS_EXPORT int tos_ctx_learn_attach(tos_ctx* c, tos_unet* net, int level, int mode) {
   return guard(c, [&]() {
     REQUIRE(mode == 0 || (net && mode == 1), "bad mode");
     if (mode == 0) {
@@ -2714,16 +2731,14 @@ TOS_EXPORT int tos_ctx_learn_attach(tos_ctx* c, tos_unet* net, int level, int mo
       return TOS_OK;
     }
     REQUIRE(level >= 1 && level + 1 < (int)c->L.size(), "learned level must have a coarser level below it");
-    REQUIRE(theta > 0, "bad theta");
     learn_prepare(*c, level);
     LearnState& S = *c->learn;
     const Level& L = c->L[level];
     S.acts = unet_acts(net, 1, tos::NGrid{L.v.nx, L.v.ny, L.v.nz}, false);
     S.net = net;
     S.mode = 1;
-    S.theta = theta;
-    hipLaunchKernelGGL(k_zero<int>, dim3(1), dim3(64), 0, c->stream, S.counters, (int64_t)2);
-    hipLaunchKernelGGL(k_zero<double>, dim3(1), dim3(64), 0, c->stream, S.ratio_sum, (int64_t)1);
+    S.paused = false;
+    hipLaunchKernelGGL(k_zero<double>, dim3(1), dim3(64), 0, c->stream, c->scal + S_NN_TRIED, (int64_t)3);
     HIPCHK(hipStreamSynchronize(c->stream));
     return TOS_OK;
   });
@@ -2749,17 +2764,13 @@ TOS_EXPORT int tos_ctx_learn_log(tos_ctx* c, tos_dataset* ds, int level, int n_i
   });
 }
 
-TOS_EXPORT int tos_ctx_learn_stats(tos_ctx* c, int* tried, int* accepted, double* mean_ratio) {
+TOS_EXPORT int tos_ctx_learn_stats(tos_ctx* c, int* tried, int* accepted, double* mean_step) {
   return guard(c, [&]() {
-    int h[2] = {0, 0};
-    double rs = 0;
-    if (c->learn && c->learn->counters) {
-      download(c->stream, h, c->learn->counters, sizeof(h));
-      download(c->stream, &rs, c->learn->ratio_sum, sizeof(double));
-    }
-    if (tried) *tried = h[0];
-    if (accepted) *accepted = h[1];
-    if (mean_ratio) *mean_ratio = h[0] ? rs / h[0] : 0.0;
+    const double* hs = read_scal(*c);
+    const bool on = c->learn != nullptr;
+    if (tried) *tried = on ? (int)hs[S_NN_TRIED] : 0;
+    if (accepted) *accepted = on ? (int)hs[S_NN_KEPT] : 0;
+    if (mean_step) *mean_step = on && hs[S_NN_KEPT] > 0 ? hs[S_NN_STEP] / hs[S_NN_KEPT] : 0.0;
     return TOS_OK;
   });
 }

@@ -151,6 +151,7 @@ enum {
   S_RZ = 0, S_PQ, S_ALPHA, S_BETA, S_RN2, S_ZR, S_ZQ, S_NF2,
   S_OC_LO, S_OC_HI, S_OC_G, S_OC_V, S_OC_DONE, S_OC_STEPS, S_OC_GMAX, S_VOL,
   S_TMP0, S_TMP1, S_TMP2, S_TMP3,
-  S_NN0, S_NN1,  // scaled residual norms before / after the learned correction
+  S_NN0, S_NN1, S_NN2,               // learned correction: |r~|^2, e.r, e.(r - A e)
+  S_NN_TRIED, S_NN_KEPT, S_NN_STEP,  // its statistics since the last attach
   S_COUNT = 32
 };

@@ -132,10 +132,9 @@ def level_box(ctx, level):
     return (ex + 1, ey + 1, ez + 1)
 
 
-def attach(ctx, net, level=2, theta=1.0):
+def attach(ctx, net, level=2):
     ctx._chk(ctx.api.ctx_learn_attach(ctx.h, net.h if net is not None else None, level,
-                                       1 if net is not None else 0, theta),
-             "ctx_learn_attach")
+                                       1 if net is not None else 0), "ctx_learn_attach")
 
 
 def log_pairs(ctx, ds, level=2, n_inner=8, stride=1):
@@ -143,7 +142,7 @@ def log_pairs(ctx, ds, level=2, n_inner=8, stride=1):
 
 
 def stats(ctx):
-    """(corrections tried, kept, mean residual ratio after / before)"""
+    """(corrections tried, kept, mean energy-optimal step of those kept)"""
     t, a, r = C.c_int(), C.c_int(), C.c_double()
     ctx._chk(ctx.api.ctx_learn_stats(ctx.h, C.byref(t), C.byref(a), C.byref(r)), "ctx_learn_stats")
     return t.value, a.value, r.value
