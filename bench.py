@@ -270,7 +270,19 @@ def main():
     ap.add_argument("--learn-domains", type=int, default=4)
     ap.add_argument("--learn-iters", type=int, default=50)
     ap.add_argument("--train-steps", type=int, default=1500)
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="only run a device-to-device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy_mib:
+        import torch
+        n = args.calibrate_copy_mib * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 5}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)

@@ -14,7 +14,7 @@ namespace tos {
 template <int BS>
 __device__ __forceinline__ bool bnode(const int4* __restrict__ cl, int nch, int s, int K, int KP, NodeRef& n,
                                       int& k) {
-  const int64_t g = (int64_t)blockIdx.x * BS + threadIdx.x;
+  const int64_t g = xcd_block() * BS + threadIdx.x;
   const int64_t tpc = 64LL * KP;
   const int64_t ch = g / tpc;
   if (ch >= nch) return false;
@@ -69,10 +69,11 @@ __global__ __launch_bounds__(BS) void k_fine_apply_b(LevelView L, const int4* __
   const bool ok = bnode<BS>(cl, nch, 1, K, KP, n, k);
   double d = 0;
   if (ok) {
-    T un[27][3];
+    T un[27][3], wa[8];
     load_nbhd_b(L, u, n.p, K, k, un);
+    load_weights(L, w, n.x, n.y, n.z, wa);
     T acc[3] = {0, 0, 0}, D[9];
-    fine_row<T, true>(L, w, Ke, n.x, n.y, n.z, un, acc, D);
+    fine_row<T, true>(wa, Ke, un, acc, D);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const int64_t i = (c * L.NP + n.p) * K + k;
@@ -96,11 +97,12 @@ __global__ __launch_bounds__(BS) void k_fine_gs_b(LevelView L, const int4* __res
   NodeRef n;
   int k = 0;
   if (!bnode<BS>(cl, nch, 2, K, KP, n, k)) return;
-  T un[27][3];
+  T un[27][3], wa[8];
   load_nbhd_b(L, u, n.p, K, k, un);
+  load_weights(L, w, n.x, n.y, n.z, wa);
   T s[3] = {0, 0, 0};
   T D[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  fine_row<T, false>(L, w, Ke, n.x, n.y, n.z, un, s, D);
+  fine_row<T, false>(wa, Ke, un, s, D);
   const int64_t i0 = n.p * K + k, cs = L.NP * K;
   const T b0 = f[i0] - s[0], b1 = f[cs + i0] - s[1], b2 = f[2 * cs + i0] - s[2];
   T v0 = un[13][0], v1 = un[13][1], v2 = un[13][2];
@@ -133,9 +135,9 @@ __global__ __launch_bounds__(BS) void k_st_apply_b(LevelView L, const int4* __re
         (n.p + (int64_t)(q % 3 - 1) + (int64_t)L.NX * ((q / 3) % 3 - 1 + (int64_t)L.NY * (q / 9 - 1))) * K + k;
     const T u0 = u[a], u1 = u[cs + a], u2 = u[2 * cs + a];
     const T* s = st + (int64_t)q * 9 * NS + n.slot;
-    a0 += s[0] * u0 + s[NS] * u1 + s[2 * NS] * u2;
-    a1 += s[3 * NS] * u0 + s[4 * NS] * u1 + s[5 * NS] * u2;
-    a2 += s[6 * NS] * u0 + s[7 * NS] * u1 + s[8 * NS] * u2;
+    a0 = fma(s[0], u0, a0); a0 = fma(s[NS], u1, a0); a0 = fma(s[2 * NS], u2, a0);
+    a1 = fma(s[3 * NS], u0, a1); a1 = fma(s[4 * NS], u1, a1); a1 = fma(s[5 * NS], u2, a1);
+    a2 = fma(s[6 * NS], u0, a2); a2 = fma(s[7 * NS], u1, a2); a2 = fma(s[8 * NS], u2, a2);
   }
   const int64_t i0 = n.p * K + k;
   if (MODE == 0) {
@@ -161,9 +163,9 @@ __global__ __launch_bounds__(BS) void k_st_gs_b(LevelView L, const int4* __restr
         (n.p + (int64_t)(q % 3 - 1) + (int64_t)L.NX * ((q / 3) % 3 - 1 + (int64_t)L.NY * (q / 9 - 1))) * K + k;
     const T u0 = u[a], u1 = u[cs + a], u2 = u[2 * cs + a];
     const T* s = st + (int64_t)q * 9 * NS + slot;
-    s0 += s[0] * u0 + s[NS] * u1 + s[2 * NS] * u2;
-    s1 += s[3 * NS] * u0 + s[4 * NS] * u1 + s[5 * NS] * u2;
-    s2 += s[6 * NS] * u0 + s[7 * NS] * u1 + s[8 * NS] * u2;
+    s0 = fma(s[0], u0, s0); s0 = fma(s[NS], u1, s0); s0 = fma(s[2 * NS], u2, s0);
+    s1 = fma(s[3 * NS], u0, s1); s1 = fma(s[4 * NS], u1, s1); s1 = fma(s[5 * NS], u2, s1);
+    s2 = fma(s[6 * NS], u0, s2); s2 = fma(s[7 * NS], u1, s2); s2 = fma(s[8 * NS], u2, s2);
   }
   const T* d = st + (int64_t)13 * 9 * NS + slot;
   T D[9];

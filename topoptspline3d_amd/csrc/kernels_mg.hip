@@ -10,7 +10,8 @@
 // 27 neighbouring displacements in VGPRs and sums rho_e^p * Ke[loc_i][loc_j] u_j over the
 // 8 incident elements (update_residual_OTFA_WS_kernel grid/Grid.cu:2277-2333,
 // gs_relax_OTFA_WS_kernel :1700-1842).  Ke is read through a uniform pointer, so the
-// compiler serves it from the scalar cache into SGPR operands of the FMAs.
+// compiler serves it from the scalar cache into SGPR operands of the FMAs; only the 14
+// distinct values of its first block row are used (reflection and axis symmetry).
 // Coarse levels: Galerkin 27x9 stencil stored colour-compact: slot = 64 * (index of the
 // chunk in the colour-ordered list) + lane, entry (k*9 + r*3 + c)*NS + slot, so a GS
 // colour pass reads its stencil rows fully coalesced (gs_relax_kernel :1455-1566,
@@ -28,7 +29,7 @@ struct NodeRef {
 
 // node of this lane in chunk list `cl` (stride s); false for idle lanes
 __device__ __forceinline__ bool chunk_node(const int4* __restrict__ cl, int nch, int s, NodeRef& n) {
-  const int ch = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int ch = (int)xcd_block() * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (ch >= nch) return false;
   const int4 c = cl[ch];
@@ -68,54 +69,87 @@ __device__ __forceinline__ void load_nbhd(const LevelView& L, const T* __restric
 
 // acc += sum_a w_a sum_j Ke[li][j] u_j ; with SELF = false the node itself (j = li) is
 // skipped and its diagonal block accumulated into D instead.
-template <class T, bool SELF>
-__device__ __forceinline__ void fine_row(const LevelView& L, const T* __restrict__ w, const T* __restrict__ Ke,
-                                         int x, int y, int z, const T (&un)[27][3], T (&acc)[3], T (&D)[9]) {
+// Only the first block row of Ke is needed: reflecting the cube along the axes set in li
+// maps local node li to 0 and flips those displacement components, so
+// Ke[li][j](r,s) = sg(li,r) sg(li,s) Ke[0][li^j](r,s) with sg = -1 on reflected axes, and
+// that row holds 14 distinct values Kc[ke_key(d, r, s)] (cube symmetry of the isotropic
+// element).  They stay in scalar registers (all 576 entries would spill).
+template <class T>
+__device__ __forceinline__ void load_weights(const LevelView& L, const T* __restrict__ w, int x, int y, int z,
+                                             T (&wa)[8]) {
   const int64_t e0 = L.pelem(x - 1, y - 1, z - 1);
+#pragma unroll
+  for (int a = 0; a < 8; ++a) wa[a] = w[e0 + (a & 1) + (int64_t)L.EX * (((a >> 1) & 1) + (int64_t)L.EY * (a >> 2))];
+}
+
+// every operand of a node's update is loaded before any arithmetic: one memory round trip
+// per wave instead of one per neighbour row (the scheduler would otherwise interleave
+// loads with FMAs to save registers and serialise the latencies)
+#define TOS_LOADS_DONE() __builtin_amdgcn_sched_barrier(0)
+
+template <class T, bool SELF>
+__device__ __forceinline__ void fine_row(const T (&wa)[8], const T* __restrict__ Kc, const T (&un)[27][3],
+                                         T (&acc)[3], T (&D)[9]) {
 #pragma unroll
   for (int a = 0; a < 8; ++a) {
     const int ax = a & 1, ay = (a >> 1) & 1, az = a >> 2;
-    const T we = w[e0 + ax + (int64_t)L.EX * (ay + (int64_t)L.EY * az)];
+    const T we = wa[a];
     const int li = 7 - a;
+    const T sx = (li & 1) ? T(-1) : T(1), sy = (li & 2) ? T(-1) : T(1), sz = (li & 4) ? T(-1) : T(1);
     T t0 = 0, t1 = 0, t2 = 0;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       const int nb = (ax + (j & 1)) + 3 * (ay + ((j >> 1) & 1)) + 9 * (az + (j >> 2));
       if (!SELF && j == li) continue;
-      const T* kr = Ke + (3 * li) * 24 + 3 * j;
-      t0 += kr[0] * un[nb][0] + kr[1] * un[nb][1] + kr[2] * un[nb][2];
-      t1 += kr[24] * un[nb][0] + kr[25] * un[nb][1] + kr[26] * un[nb][2];
-      t2 += kr[48] * un[nb][0] + kr[49] * un[nb][1] + kr[50] * un[nb][2];
+      const int d = li ^ j;
+      // neighbour components reflected into the frame of local node 0
+      const T v0 = sx * un[nb][0], v1 = sy * un[nb][1], v2 = sz * un[nb][2];
+      t0 = fma(Kc[ke_key(d, 0, 0)], v0, t0); t0 = fma(Kc[ke_key(d, 0, 1)], v1, t0);
+      t0 = fma(Kc[ke_key(d, 0, 2)], v2, t0);
+      t1 = fma(Kc[ke_key(d, 1, 0)], v0, t1); t1 = fma(Kc[ke_key(d, 1, 1)], v1, t1);
+      t1 = fma(Kc[ke_key(d, 1, 2)], v2, t1);
+      t2 = fma(Kc[ke_key(d, 2, 0)], v0, t2); t2 = fma(Kc[ke_key(d, 2, 1)], v1, t2);
+      t2 = fma(Kc[ke_key(d, 2, 2)], v2, t2);
     }
-    acc[0] += we * t0;
-    acc[1] += we * t1;
-    acc[2] += we * t2;
+    acc[0] = fma(we * sx, t0, acc[0]);
+    acc[1] = fma(we * sy, t1, acc[1]);
+    acc[2] = fma(we * sz, t2, acc[2]);
     if (!SELF) {
-      const T* kd = Ke + (3 * li) * 24 + 3 * li;
+      const T sg[3] = {sx, sy, sz};
 #pragma unroll
       for (int r = 0; r < 3; ++r)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) D[r * 3 + c] += we * kd[r * 24 + c];
+        for (int c = 0; c < 3; ++c) D[r * 3 + c] = fma(we * (sg[r] * sg[c]), Kc[ke_key(0, r, c)], D[r * 3 + c]);
     }
   }
 }
 
 // MODE 0: y = K u, optional partial sums of u.y;  MODE 1: y = f - K u, optional float
 // copy y32.  Only free nodes are touched (the others stay 0 in every vector).
+// FP32: all 81 neighbour values live at once, register budget sized for 4 waves per SIMD.
+// FP64 keeps the compiler's interleaved schedule (twice the registers would halve the
+// occupancy, which measured slower).
+#define TOS_FINE_BOUNDS(T) __launch_bounds__(256, sizeof(T) == 4 ? 4 : 1)
+
 template <class T, int MODE>
-__global__ __launch_bounds__(256) void k_fine_apply(LevelView L, const int4* __restrict__ cl, int nch,
-                                                    const T* __restrict__ w, const T* __restrict__ Ke,
-                                                    const T* __restrict__ u, const T* __restrict__ f,
-                                                    T* __restrict__ y, double* __restrict__ partial,
-                                                    float* __restrict__ y32) {
+__global__ TOS_FINE_BOUNDS(T) void k_fine_apply(LevelView L, const int4* __restrict__ cl, int nch,
+                                                const T* __restrict__ w, const T* __restrict__ Ke,
+                                                const T* __restrict__ u, const T* __restrict__ f,
+                                                T* __restrict__ y, double* __restrict__ partial,
+                                                float* __restrict__ y32) {
   NodeRef n;
   const bool ok = chunk_node(cl, nch, 1, n);
   double d = 0;
   if (ok) {
-    T un[27][3];
+    T un[27][3], wa[8], fv[3] = {0, 0, 0};
     load_nbhd(L, u, n.p, un);
+    load_weights(L, w, n.x, n.y, n.z, wa);
+    if (MODE == 1)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) fv[c] = f[c * L.NP + n.p];
+    if constexpr (sizeof(T) == 4) TOS_LOADS_DONE();
     T acc[3] = {0, 0, 0}, D[9];
-    fine_row<T, true>(L, w, Ke, n.x, n.y, n.z, un, acc, D);
+    fine_row<T, true>(wa, Ke, un, acc, D);
     if (MODE == 0) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
@@ -125,7 +159,7 @@ __global__ __launch_bounds__(256) void k_fine_apply(LevelView L, const int4* __r
     } else {
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        const T v = f[c * L.NP + n.p] - acc[c];
+        const T v = fv[c] - acc[c];
         y[c * L.NP + n.p] = v;
         if (y32) y32[c * L.NP + n.p] = (float)v;
       }
@@ -136,18 +170,21 @@ __global__ __launch_bounds__(256) void k_fine_apply(LevelView L, const int4* __r
 
 // Gauss-Seidel colour pass on the fine level.  BACKWARD relaxes components z, y, x.
 template <class T, bool BACKWARD>
-__global__ __launch_bounds__(256) void k_fine_gs(LevelView L, const int4* __restrict__ cl, int nch,
-                                                 const T* __restrict__ w, const T* __restrict__ Ke,
-                                                 T* __restrict__ u, const T* __restrict__ f) {
+__global__ TOS_FINE_BOUNDS(T) void k_fine_gs(LevelView L, const int4* __restrict__ cl, int nch,
+                                             const T* __restrict__ w, const T* __restrict__ Ke,
+                                             T* __restrict__ u, const T* __restrict__ f) {
   NodeRef n;
   if (!chunk_node(cl, nch, 2, n)) return;
-  T un[27][3];
+  T un[27][3], wa[8];
   load_nbhd(L, u, n.p, un);
+  load_weights(L, w, n.x, n.y, n.z, wa);
+  const int64_t p = n.p;
+  const T f0 = f[p], f1 = f[L.NP + p], f2 = f[2 * L.NP + p];
+  if constexpr (sizeof(T) == 4) TOS_LOADS_DONE();
   T s[3] = {0, 0, 0};
   T D[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  fine_row<T, false>(L, w, Ke, n.x, n.y, n.z, un, s, D);
-  const int64_t p = n.p;
-  const T b0 = f[p] - s[0], b1 = f[L.NP + p] - s[1], b2 = f[2 * L.NP + p] - s[2];
+  fine_row<T, false>(wa, Ke, un, s, D);
+  const T b0 = f0 - s[0], b1 = f1 - s[1], b2 = f2 - s[2];
   T v0 = un[13][0], v1 = un[13][1], v2 = un[13][2];
   if (!BACKWARD) {
     v0 = (b0 - D[1] * v1 - D[2] * v2) / D[0];
@@ -179,9 +216,9 @@ __global__ __launch_bounds__(256) void k_st_apply(LevelView L, const int4* __res
     const int64_t q = p + (int64_t)(k % 3 - 1) + (int64_t)L.NX * ((k / 3) % 3 - 1 + (int64_t)L.NY * (k / 9 - 1));
     const T u0 = u[q], u1 = u[L.NP + q], u2 = u[2 * L.NP + q];
     const T* s = st + (int64_t)k * 9 * NS + n.slot;
-    a0 += s[0] * u0 + s[NS] * u1 + s[2 * NS] * u2;
-    a1 += s[3 * NS] * u0 + s[4 * NS] * u1 + s[5 * NS] * u2;
-    a2 += s[6 * NS] * u0 + s[7 * NS] * u1 + s[8 * NS] * u2;
+    a0 = fma(s[0], u0, a0); a0 = fma(s[NS], u1, a0); a0 = fma(s[2 * NS], u2, a0);
+    a1 = fma(s[3 * NS], u0, a1); a1 = fma(s[4 * NS], u1, a1); a1 = fma(s[5 * NS], u2, a1);
+    a2 = fma(s[6 * NS], u0, a2); a2 = fma(s[7 * NS], u1, a2); a2 = fma(s[8 * NS], u2, a2);
   }
   if (MODE == 0) {
     y[p] = a0; y[L.NP + p] = a1; y[2 * L.NP + p] = a2;
@@ -205,9 +242,9 @@ __global__ __launch_bounds__(256) void k_st_gs(LevelView L, const int4* __restri
     const int64_t q = p + (int64_t)(k % 3 - 1) + (int64_t)L.NX * ((k / 3) % 3 - 1 + (int64_t)L.NY * (k / 9 - 1));
     const T u0 = u[q], u1 = u[L.NP + q], u2 = u[2 * L.NP + q];
     const T* s = st + (int64_t)k * 9 * NS + slot;
-    s0 += s[0] * u0 + s[NS] * u1 + s[2 * NS] * u2;
-    s1 += s[3 * NS] * u0 + s[4 * NS] * u1 + s[5 * NS] * u2;
-    s2 += s[6 * NS] * u0 + s[7 * NS] * u1 + s[8 * NS] * u2;
+    s0 = fma(s[0], u0, s0); s0 = fma(s[NS], u1, s0); s0 = fma(s[2 * NS], u2, s0);
+    s1 = fma(s[3 * NS], u0, s1); s1 = fma(s[4 * NS], u1, s1); s1 = fma(s[5 * NS], u2, s1);
+    s2 = fma(s[6 * NS], u0, s2); s2 = fma(s[7 * NS], u1, s2); s2 = fma(s[8 * NS], u2, s2);
   }
   const T* d = st + (int64_t)13 * 9 * NS + slot;
   T D[9];
@@ -421,6 +458,83 @@ __global__ __launch_bounds__(256) void k_galerkin_next(LevelView F, LevelView C,
   }
 #pragma unroll
   for (int e = 0; e < 9; ++e) stc[(int64_t)(k * 9 + e) * NSC + nr.slot] = acc[e];
+}
+
+// ---------------------------------------------------------------------------------------
+// Galerkin coarse operator by axes.  Trilinear P is the tensor product of 1-D node-aligned
+// interpolations, so P^T A P = (Pz Py Px)^T A (Px Py Pz) is formed in three passes, each
+// halving one axis of a 27-point 3x3-block stencil:
+//   A'[I][dX] = sum_{a,b in {-1,0,1}, kin = 2 dX + b - a in {-1,0,1}} w(a) w(b) A[2I + a][kin]
+// (w(0) = 1, w(+-1) = 1/2, offsets along the other axes unchanged).  The fine stencil is
+// zero on eliminated rows and towards eliminated columns, so the staged product equals
+// the masked one; the last pass drops coarse columns that are not free.  Every pass reads
+// each input stencil a few times instead of the ~13^3 / 27 times of the direct sum.
+// Input: the fine level's compact stencil (IN_CMP, slot map) or an intermediate box;
+// output: an intermediate box or the coarse level's compact stencil (OUT_CMP).
+// Thread = (output node, output neighbour k = blockIdx.y).
+// ---------------------------------------------------------------------------------------
+struct GBox {
+  int X, Y, Z;
+};
+
+template <int AX, bool IN_CMP, bool OUT_CMP>
+__global__ __launch_bounds__(256) void k_gal_axis(LevelView F, const int* __restrict__ fslot,
+                                                  const double* __restrict__ sin, int64_t NSI, GBox bi, GBox bo,
+                                                  LevelView C, const int4* __restrict__ clc, int nch,
+                                                  const uint8_t* __restrict__ cfre, double* __restrict__ sout,
+                                                  int64_t NSO) {
+  int o[3];
+  int64_t oidx;
+  const int k = blockIdx.y;
+  const int d[3] = {k % 3 - 1, (k / 3) % 3 - 1, k / 9 - 1};
+  bool keep = true;
+  if (OUT_CMP) {
+    NodeRef nr;
+    if (!chunk_node(clc, nch, 2, nr)) return;
+    o[0] = nr.x; o[1] = nr.y; o[2] = nr.z;
+    oidx = nr.slot;
+    keep = cfre[nr.p + d[0] + (int64_t)C.NX * (d[1] + (int64_t)C.NY * d[2])] != 0;
+  } else {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= (int64_t)bo.X * bo.Y * bo.Z) return;
+    o[0] = (int)(v % bo.X);
+    o[1] = (int)((v / bo.X) % bo.Y);
+    o[2] = (int)(v / ((int64_t)bo.X * bo.Y));
+    oidx = v;
+  }
+  double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  const int nin = AX == 0 ? bi.X : (AX == 1 ? bi.Y : bi.Z);
+  if (keep) {
+    for (int a = -1; a <= 1; ++a) {
+      int q[3] = {o[0], o[1], o[2]};
+      q[AX] = 2 * o[AX] + a;
+      if (q[AX] < 0 || q[AX] >= nin) continue;
+      int64_t src;
+      if (IN_CMP) {
+        const int sl = fslot[F.pnode(q[0], q[1], q[2])];
+        if (sl < 0) continue;  // eliminated fine node: zero row
+        src = sl;
+      } else {
+        src = q[0] + (int64_t)bi.X * (q[1] + (int64_t)bi.Y * q[2]);
+      }
+      const double wa = a ? 0.5 : 1.0;
+      for (int b = -1; b <= 1; ++b) {
+        const int kin = 2 * d[AX] + b - a;
+        if (kin < -1 || kin > 1) continue;
+        const int bq = 2 * (o[AX] + d[AX]) + b;
+        if (bq < 0 || bq >= nin) continue;
+        int e3[3] = {d[0], d[1], d[2]};
+        e3[AX] = kin;
+        const int ein = (e3[0] + 1) + 3 * (e3[1] + 1) + 9 * (e3[2] + 1);
+        const double wgt = wa * (b ? 0.5 : 1.0);
+        const double* sp = sin + (int64_t)ein * 9 * NSI + src;
+#pragma unroll
+        for (int e = 0; e < 9; ++e) acc[e] += wgt * sp[e * NSI];
+      }
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < 9; ++e) sout[(int64_t)(k * 9 + e) * NSO + oidx] = acc[e];
 }
 
 __global__ void k_cast_d2f(const double* __restrict__ a, float* __restrict__ b, int64_t n) {

@@ -422,7 +422,7 @@ __global__ __launch_bounds__(256) void k_nn_norm(LevelView L, const int4* __rest
                                                  const int* __restrict__ slot, double* __restrict__ partial) {
   __shared__ double sh[4];
   double s = 0;
-  const int ch = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int ch = (int)xcd_block() * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (ch < nch) {
     const int4 c = cl[ch];
     if (lane < (c.y >> 16)) {
@@ -573,7 +573,7 @@ __global__ __launch_bounds__(256) void k_nn_dots(LevelView L, const int4* __rest
                                                  const float* __restrict__ t, double* __restrict__ partial) {
   __shared__ double sh[2][4];
   double a = 0, b = 0;
-  const int ch = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int ch = (int)xcd_block() * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (ch < nch) {
     const int4 c = cl[ch];
     if (lane < (c.y >> 16)) {
@@ -648,7 +648,7 @@ __global__ __launch_bounds__(256) void k_nn_accept(LevelView L, const int4* __re
     }
   }
   if (!(alpha > 0)) return;
-  const int ch = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int ch = (int)xcd_block() * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (ch >= nch) return;
   const int4 c = cl[ch];
   if (lane >= (c.y >> 16)) return;
@@ -661,7 +661,7 @@ __global__ __launch_bounds__(256) void k_nn_accept(LevelView L, const int4* __re
 // y += a on the free nodes (3 components)
 __global__ __launch_bounds__(256) void k_add_nodes_f(const int4* __restrict__ cl, int nch, int64_t NP,
                                                      float* __restrict__ y, const float* __restrict__ a) {
-  const int ch = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int ch = (int)xcd_block() * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (ch >= nch) return;
   const int4 c = cl[ch];
   if (lane >= (c.y >> 16)) return;

@@ -414,7 +414,7 @@ struct Prec<double> {
   static double* r(Level& L) { return L.r64; }
   static const double* st(Level& L) { return L.st64; }
   static const double* w(Ctx& c) { return c.w64; }
-  static const double* Ke(Ctx& c) { return c.Ke64; }
+  static const double* Ke(Ctx& c) { return c.Kc64; }
 };
 template <>
 struct Prec<float> {
@@ -423,7 +423,7 @@ struct Prec<float> {
   static float* r(Level& L) { return L.r32; }
   static const float* st(Level& L) { return L.st32; }
   static const float* w(Ctx& c) { return c.w32; }
-  static const float* Ke(Ctx& c) { return c.Ke32; }
+  static const float* Ke(Ctx& c) { return c.Kc32; }
 };
 
 template <class T>
@@ -788,10 +788,10 @@ void gs_b(Ctx& c, BatchState& B, int l, float* u, const float* f, bool backward)
 #define GO(BS)                                                                                                     \
   if (backward)                                                                                                    \
     hipLaunchKernelGGL((k_fine_gs_b<float, true, BS>), dim3(nb), dim3(BS), 0, c.stream, v, cl, nch, B.K, B.KP,     \
-                       (const float*)c.w32, (const float*)c.Ke32, u, f);                                           \
+                       (const float*)c.w32, (const float*)c.Kc32, u, f);                                           \
   else                                                                                                             \
     hipLaunchKernelGGL((k_fine_gs_b<float, false, BS>), dim3(nb), dim3(BS), 0, c.stream, v, cl, nch, B.K, B.KP,    \
-                       (const float*)c.w32, (const float*)c.Ke32, u, f);
+                       (const float*)c.w32, (const float*)c.Kc32, u, f);
       TOS_BS_SWITCH(bs, GO)
 #undef GO
     } else {
@@ -818,7 +818,7 @@ void resid_b(Ctx& c, BatchState& B, int l, const float* u, const float* f, float
     const unsigned nb = bgrid(L.n_lin, B.KP, bs);
 #define GO(BS)                                                                                                      \
   hipLaunchKernelGGL((k_fine_apply_b<float, 1, BS>), dim3(nb), dim3(BS), 0, c.stream, L.v, (const int4*)L.cl_lin, \
-                     L.n_lin, B.K, B.KP, (const float*)c.w32, (const float*)c.Ke32, u, f, r, (double*)nullptr,      \
+                     L.n_lin, B.K, B.KP, (const float*)c.w32, (const float*)c.Kc32, u, f, r, (double*)nullptr,      \
                      (float*)nullptr);
     TOS_BS_SWITCH(bs, GO)
 #undef GO
@@ -904,7 +904,7 @@ void batch_iteration(Ctx& c, BatchState& B, int pre, int post) {
   const int64_t NP = L0.v.NP;
   const unsigned nb = bgrid(nch, B.KP, kVecBS);
   hipLaunchKernelGGL((k_fine_apply_b<double, 0, kVecBS>), dim3(nb), dim3(kVecBS), 0, c.stream, L0.v, cl, nch, B.K,
-                     B.KP, (const double*)c.w64, (const double*)c.Ke64, (const double*)B.P, (const double*)nullptr,
+                     B.KP, (const double*)c.w64, (const double*)c.Kc64, (const double*)B.P, (const double*)nullptr,
                      B.Q, B.part, (float*)nullptr);
   foldb(c, B, nb, 1, B_PQ, 0, OPB_ALPHA);
   hipLaunchKernelGGL((k_pcgb_ur<kVecBS>), dim3(nb), dim3(kVecBS), 0, c.stream, cl, nch, NP, B.K, B.KP, B.U, B.R,
@@ -983,7 +983,7 @@ int solve_batched(Ctx& c, const tos_solve_opts& o, tos_solve_stats* st) {
   else
     hipLaunchKernelGGL(k_zero<double>, dim3(nblk(n3k)), dim3(256), 0, c.stream, B.U, n3k);
   hipLaunchKernelGGL((k_fine_apply_b<double, 1, kVecBS>), dim3(nb), dim3(kVecBS), 0, c.stream, L0.v, cl, nch, K,
-                     B.KP, (const double*)c.w64, (const double*)c.Ke64, (const double*)B.U, (const double*)B.F, B.R,
+                     B.KP, (const double*)c.w64, (const double*)c.Kc64, (const double*)B.U, (const double*)B.F, B.R,
                      (double*)nullptr, B.f32[0]);
   hipLaunchKernelGGL((k_dotb<kVecBS>), dim3(nb), dim3(kVecBS), 0, c.stream, cl, nch, NP, K, B.KP, (const double*)B.F,
                      (const double*)B.F, (const double*)B.R, (const double*)B.R, B.part);
@@ -1594,6 +1594,8 @@ TOS_EXPORT int tos_ctx_create(const tos_grid_desc* d, tos_ctx** out) {
     c->w32 = dalloc<float>(*c, v0.NE);
     c->Ke64 = dalloc<double>(*c, 576);
     c->Ke32 = dalloc<float>(*c, 576);
+    c->Kc64 = dalloc<double>(*c, 16);
+    c->Kc32 = dalloc<float>(*c, 16);
     c->Mc64 = dalloc<double>(*c, 8 * 576);
     c->r = dalloc<double>(*c, 3 * v0.NP);
     c->z = dalloc<double>(*c, 3 * v0.NP);
@@ -1687,6 +1689,26 @@ TOS_EXPORT int tos_set_material(tos_ctx* c, double E, double nu, double p, doubl
     for (int i = 0; i < 576; ++i) k32[i] = (float)c->h_Ke[i];
     upload(c->stream, c->Ke64, c->h_Ke, sizeof(c->h_Ke));
     upload(c->stream, c->Ke32, k32, sizeof(k32));
+    // distinct values of the first block row; every entry of Ke must follow from them by
+    // the reflection / axis symmetry the matrix-free kernels rely on
+    double kc[16] = {0};
+    double kmax = 0;
+    for (int i = 0; i < 576; ++i) kmax = std::max(kmax, std::fabs(c->h_Ke[i]));
+    for (int d = 0; d < 8; ++d)
+      for (int r = 0; r < 3; ++r)
+        for (int s2 = 0; s2 < 3; ++s2) kc[ke_key(d, r, s2)] = c->h_Ke[r * 24 + 3 * d + s2];
+    for (int a = 0; a < 8; ++a)
+      for (int b = 0; b < 8; ++b)
+        for (int r = 0; r < 3; ++r)
+          for (int s2 = 0; s2 < 3; ++s2) {
+            const double sg = (((a >> r) & 1) ? -1.0 : 1.0) * (((a >> s2) & 1) ? -1.0 : 1.0);
+            if (std::fabs(sg * kc[ke_key(a ^ b, r, s2)] - c->h_Ke[(3 * a + r) * 24 + 3 * b + s2]) > 1e-12 * kmax)
+              throw std::runtime_error("element matrix lacks the cube symmetry of the hex element");
+          }
+    float kc32[16];
+    for (int i = 0; i < 16; ++i) kc32[i] = (float)kc[i];
+    upload(c->stream, c->Kc64, kc, sizeof(kc));
+    upload(c->stream, c->Kc32, kc32, sizeof(kc32));
     upload(c->stream, c->Mc64, c->h_Mc, sizeof(c->h_Mc));
     c->op_ready = false;
     return TOS_OK;
@@ -1777,6 +1799,28 @@ TOS_EXPORT int tos_get_density(tos_ctx* c, float* rho) {
 }
 
 // update_stencil (grid/Grid.cpp:4167-4178): weights, Galerkin levels, coarsest factor
+// coarse stencil of level C from level F (>= 1) in three axis passes through two boxes:
+// (Xc, Yf, Zf) then (Xc, Yc, Zf)
+void galerkin_by_axes(Ctx& c, Level& F, Level& C) {
+  const GBox bf{F.v.nx, F.v.ny, F.v.nz};
+  const GBox b1{C.v.nx, F.v.ny, F.v.nz}, b2{C.v.nx, C.v.ny, F.v.nz};
+  const int64_t n1 = (int64_t)b1.X * b1.Y * b1.Z, n2 = (int64_t)b2.X * b2.Y * b2.Z;
+  if (!C.gal_b1) {
+    C.gal_b1 = dalloc<double>(c, 243 * n1);
+    C.gal_b2 = dalloc<double>(c, 243 * n2);
+  }
+  hipLaunchKernelGGL((k_gal_axis<0, true, false>), dim3(nblk(n1), 27), dim3(256), 0, c.stream, F.v,
+                     (const int*)F.slot, (const double*)F.st64, F.NS, bf, b1, C.v, (const int4*)nullptr, 0,
+                     (const uint8_t*)nullptr, C.gal_b1, n1);
+  hipLaunchKernelGGL((k_gal_axis<1, false, false>), dim3(nblk(n2), 27), dim3(256), 0, c.stream, F.v,
+                     (const int*)nullptr, (const double*)C.gal_b1, n1, b1, b2, C.v, (const int4*)nullptr, 0,
+                     (const uint8_t*)nullptr, C.gal_b2, n2);
+  hipLaunchKernelGGL((k_gal_axis<2, false, true>), dim3(cblk(C.n_col), 27), dim3(256), 0, c.stream, F.v,
+                     (const int*)nullptr, (const double*)C.gal_b2, n2, b2, GBox{C.v.nx, C.v.ny, C.v.nz}, C.v,
+                     (const int4*)C.cl_col, C.n_col, (const uint8_t*)C.fre, C.st64, C.NS);
+  LAUNCH_CHECK();
+}
+
 TOS_EXPORT int tos_update_operator(tos_ctx* c) {
   return guard(c, [&]() {
     const LevelView& v0 = c->L[0].v;
@@ -1793,9 +1837,7 @@ TOS_EXPORT int tos_update_operator(tos_ctx* c) {
                            C.n_col, (const uint8_t*)F.fre, (const uint8_t*)c->allfree, (const uint8_t*)C.fre,
                            (const double*)c->w64, (const double*)c->Ke64, (const double*)c->Mc64, C.st64, C.NS);
       else
-        hipLaunchKernelGGL(k_galerkin_next, grid, dim3(256), 0, c->stream, F.v, C.v, (const int4*)C.cl_col,
-                           C.n_col, (const uint8_t*)C.fre, (const int*)F.slot, (const double*)F.st64, F.NS, C.st64,
-                           C.NS);
+        galerkin_by_axes(*c, F, C);
       const int64_t ns = 243 * C.NS;
       hipLaunchKernelGGL(k_cast_d2f, dim3(nblk(ns)), dim3(256), 0, c->stream, (const double*)C.st64, C.st32, ns);
       LAUNCH_CHECK();

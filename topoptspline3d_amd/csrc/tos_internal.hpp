@@ -26,6 +26,23 @@ namespace tos {
 struct LearnState;  // learned coarse-level correction (network, pair logging)
 struct BatchState;  // K load cases solved together (interleaved vectors, graph, tuning)
 
+// Index of entry (r, s) of block Ke[0][d] among the 14 distinct values of the first block
+// row of the isotropic hex stiffness: diagonal entries depend on bit r of d and on the
+// number of other set bits, off-diagonal entries on bits r, s and the third axis of d.
+__host__ __device__ constexpr int ke_key(int d, int r, int s) {
+  return r == s ? ((d >> r) & 1) * 3 + ((d >> ((r + 1) % 3)) & 1) + ((d >> ((r + 2) % 3)) & 1)
+                : 6 + ((d >> r) & 1) * 4 + ((d >> s) & 1) * 2 + ((d >> (3 - r - s)) & 1);
+}
+
+// Workgroups are dispatched to the 8 XCDs round-robin (block b -> XCD b % 8) and each XCD
+// has its own L2.  Renumbering blocks so that XCD x gets a contiguous 1/8 of the logical
+// block range keeps neighbouring grid rows (and their shared stencil data) in one L2.
+// A bijection on [0, G): the first 8*floor(G/8) blocks are transposed, the tail kept.
+__device__ __forceinline__ int64_t xcd_block() {
+  const int64_t b = blockIdx.x, G = gridDim.x, G8 = G & ~(int64_t)7;
+  return b < G8 ? (b & 7) * (G8 >> 3) + (b >> 3) : b;
+}
+
 struct LevelView {  // passed by value to kernels
   int nx, ny, nz;     // node dims
   int NX, NY;         // padded node row / plane dims
@@ -62,6 +79,8 @@ struct Level {
   int64_t NS = 0;                // stencil slots per entry = 64 * n_col
   int* slot = nullptr;           // padded node -> stencil slot, -1 if not free
   std::vector<int> h_slot;
+  double* gal_b1 = nullptr;      // Galerkin intermediates of this (coarse) level, box layout
+  double* gal_b2 = nullptr;
 };
 
 struct Ctx {
@@ -88,6 +107,8 @@ struct Ctx {
   // template matrices
   double* Ke64 = nullptr;       // 576
   float* Ke32 = nullptr;
+  double* Kc64 = nullptr;       // 16: the distinct entries of Ke's first block row (ke_key)
+  float* Kc32 = nullptr;
   double* Mc64 = nullptr;       // 8 x 576 Galerkin child products
   double h_Ke[576];
   double h_Mc[8 * 576];
