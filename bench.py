@@ -270,6 +270,10 @@ def main():
     ap.add_argument("--learn-domains", type=int, default=4)
     ap.add_argument("--learn-iters", type=int, default=50)
     ap.add_argument("--train-steps", type=int, default=1500)
+    ap.add_argument("--profile-steps", type=int, default=2,
+                    help="untimed iterations with per-kernel HIP-event timing (roofline)")
+    ap.add_argument("--sweep-mode", type=int, default=0,
+                    help="fine GS schedule: 0 colour passes, 1 colour-slab wavefront")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run a device-to-device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
@@ -297,6 +301,7 @@ def main():
     log(f"[bench] workload {wl['name']} generated in {time.perf_counter() - t0:.1f}s {wl['meta'].get('fill', '')}")
     api = hip_api()
     loop = Loop(api, wl, args.precision)
+    loop.ctx.set_sweep_mode(args.sweep_mode)
     info = loop.ctx.info()
     nv = active_nodes(wl["solid"])
     log(f"[bench] levels {info['n_levels']} dims {info['level_dims']} coarse dofs {info['coarse_dofs']}")
@@ -319,8 +324,14 @@ def main():
     learned = None
     if net is not None:
         LN.attach(loop.ctx, net, level=2)
-    kernel_stats(api, loop.ctx, True)
     ms, its, res_max = timed("learned" if net is not None else "plain", args.steps)
+    # per-kernel HIP-event timing in separate, untimed iterations (two events per launch
+    # would otherwise add host work to the timed region)
+    kernel_stats(api, loop.ctx, True)
+    for i in range(args.profile_steps):
+        loop.step()
+        log(f"[bench] profiled step {i}: pcg its {loop.its[-1]}")
+    sync()
     stats = kernel_stats(api, loop.ctx, False)
     if net is not None:
         tried, kept, step = LN.stats(loop.ctx)
@@ -340,7 +351,7 @@ def main():
                    "solver": ("FCG, FP32 V-cycle + learned level-2 correction, FP64 refinement"
                               if net is not None else "FCG, FP32 V-cycle, FP64 refinement"),
                    "pcg_iterations": its, "max_rel_res": res_max},
-        "roofline": roofline(stats, info, nv, args.steps),
+        "roofline": roofline(stats, info, nv, args.profile_steps),
         "plain_mixed": {"ms_per_iter": round(ms_plain, 3), "pcg_iterations": its_plain,
                         "max_rel_res": res_plain},
     }

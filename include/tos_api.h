@@ -213,6 +213,12 @@ int tos_mma_update(tos_mma* mma, const double* df, const double* dg, const doubl
 int tos_ctx_mma_step(tos_ctx* ctx, tos_mma* mma, double sens_scale, double vol_scale,
                      double vfrac, int* converged);
 
+/* Fine-level Gauss-Seidel schedule (no counterpart in the reference, which launches one
+ * pass per colour): 0 (default) = eight colour passes; 1 = colour-slab wavefront, colour c
+ * of z-slab s relaxed in step s + 2c (about 32 slabs).  Both perform exactly the same
+ * updates, so results are bit-identical across modes. */
+int tos_set_sweep_mode(tos_ctx* ctx, int mode);
+
 /* ---- measurement ---------------------------------------------------------------------
  * Kernel timing with HIP events recorded on the context's stream around every launch of
  * the solver kernels.  enable = 1 clears and starts recording; enable = 0 stops, waits for

@@ -91,6 +91,7 @@ SIGNATURES = {
     "ctx_mma_step": (C.c_int, [vp, vp, C.c_double, C.c_double, C.c_double, i32p]),
     "profile": (C.c_int, [vp, C.c_int, C.POINTER(KernelStats)]),
     "solve_batched": (C.c_int, [vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
+    "set_sweep_mode": (C.c_int, [vp, C.c_int]),
     "batch_tuning": (C.c_int, [vp, C.c_char_p, C.c_int]),
 }
 

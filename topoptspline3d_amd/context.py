@@ -164,6 +164,10 @@ class Context:
         return {"rc": rc, "iterations": [st.iterations[k] for k in range(self.K)],
                 "rel_res": [st.rel_res[k] for k in range(self.K)], "seconds": st.seconds}
 
+    def set_sweep_mode(self, mode):
+        """fine GS schedule: 0 colour passes, 1 colour-slab wavefront (same updates)"""
+        self._chk(self.api.set_sweep_mode(self.h, int(mode)), "set_sweep_mode")
+
     def batch_tuning(self):
         buf = C.create_string_buffer(512)
         self._chk(self.api.batch_tuning(self.h, buf, 512), "batch_tuning")

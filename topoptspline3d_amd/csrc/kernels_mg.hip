@@ -200,6 +200,57 @@ __global__ TOS_FINE_BOUNDS(T) void k_fine_gs(LevelView L, const int4* __restrict
   u[2 * L.NP + p] = v2;
 }
 
+// The same pass over several chunk ranges at once (one wavefront step): block b belongs
+// to the range g with tab.blk[g] <= b < tab.blk[g+1].
+__device__ __forceinline__ bool seg_node(const int4* __restrict__ cl, const SegTab& tab, NodeRef& n) {
+  const int b = (int)xcd_block();
+  int g = 0;
+#pragma unroll
+  for (int i = 1; i < 8; ++i)
+    if (i < tab.n && b >= tab.blk[i]) g = i;
+  const int loc = (b - tab.blk[g]) * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (loc >= tab.cnt[g]) return false;
+  const int4 c = cl[tab.beg[g] + loc];
+  if (lane >= (c.y >> 16)) return false;
+  n.x = (c.y & 0xffff) + 2 * lane;
+  n.y = c.z;
+  n.z = c.w;
+  n.p = (int64_t)c.x + 2 * lane;
+  n.slot = 0;
+  return true;
+}
+
+template <class T, bool BACKWARD>
+__global__ TOS_FINE_BOUNDS(T) void k_fine_gs_wf(LevelView L, const int4* __restrict__ cl, SegTab tab,
+                                                const T* __restrict__ w, const T* __restrict__ Ke,
+                                                T* __restrict__ u, const T* __restrict__ f) {
+  NodeRef n;
+  if (!seg_node(cl, tab, n)) return;
+  T un[27][3], wa[8];
+  load_nbhd(L, u, n.p, un);
+  load_weights(L, w, n.x, n.y, n.z, wa);
+  const int64_t p = n.p;
+  const T f0 = f[p], f1 = f[L.NP + p], f2 = f[2 * L.NP + p];
+  if constexpr (sizeof(T) == 4) TOS_LOADS_DONE();
+  T s[3] = {0, 0, 0};
+  T D[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  fine_row<T, false>(wa, Ke, un, s, D);
+  const T b0 = f0 - s[0], b1 = f1 - s[1], b2 = f2 - s[2];
+  T v0 = un[13][0], v1 = un[13][1], v2 = un[13][2];
+  if (!BACKWARD) {
+    v0 = (b0 - D[1] * v1 - D[2] * v2) / D[0];
+    v1 = (b1 - D[3] * v0 - D[5] * v2) / D[4];
+    v2 = (b2 - D[6] * v0 - D[7] * v1) / D[8];
+  } else {
+    v2 = (b2 - D[6] * v0 - D[7] * v1) / D[8];
+    v1 = (b1 - D[3] * v0 - D[5] * v2) / D[4];
+    v0 = (b0 - D[1] * v1 - D[2] * v2) / D[0];
+  }
+  u[p] = v0;
+  u[L.NP + p] = v1;
+  u[2 * L.NP + p] = v2;
+}
+
 // ---------------------------------------------------------------------------------------
 // Stored-stencil levels (colour-compact stencil, NS slots per entry)
 // ---------------------------------------------------------------------------------------

@@ -306,6 +306,20 @@ void build_chunks(Ctx& c, Level& L, bool with_slots) {
       }
   }
   L.col_off[8] = (int)col.size();
+  // colour-slab wavefront of the fine GS: ~32 z-slabs so that the slabs in flight during a
+  // few consecutive steps fit in the memory-side cache
+  if (!with_slots) {
+    L.wf_T = std::max(1, (v.nz + 31) / 32);
+    L.wf_S = (v.nz + L.wf_T - 1) / L.wf_T;
+    L.slab_off.assign(8 * (L.wf_S + 1), 0);
+    for (int cidx = 0; cidx < 8; ++cidx) {
+      int g = L.col_off[cidx];
+      for (int sl = 0; sl <= L.wf_S; ++sl) {
+        while (g < L.col_off[cidx + 1] && col[g].w < sl * L.wf_T) ++g;
+        L.slab_off[cidx * (L.wf_S + 1) + sl] = g;
+      }
+    }
+  }
   L.n_lin = (int)lin.size();
   L.n_col = (int)col.size();
   L.cl_lin = dalloc<int4>(c, std::max<size_t>(lin.size(), 1));
@@ -426,10 +440,54 @@ struct Prec<float> {
   static const float* Ke(Ctx& c) { return c.Kc32; }
 };
 
+// Fine-level sweep as a colour-slab wavefront: step t relaxes colour c (in sweep order
+// c' = 0..7) on z-slab s = t - 2 c'.  Neighbouring nodes differ by at most one slab, so a
+// node of colour c' is relaxed at least one step after every neighbour of an earlier
+// colour and before every neighbour of a later one: the same updates as eight full colour
+// passes (bit-identical).  Meant to reuse each slab's data across its 8 colours; on the
+// 511x320x320 workload the L2 misses per sweep stayed the same as with colour passes and
+// the 5x more launches made it slower, so it is opt-in (tos_set_sweep_mode).
+template <class T>
+void gs_sweep_wavefront(Ctx& c, T* u, const T* f, bool backward) {
+  Level& L = c.L[0];
+  const int S = L.wf_S;
+  for (int t = 0; t < S + 14; ++t) {
+    SegTab tab{};
+    int nb = 0;
+    for (int cc = 0; cc < 8; ++cc) {
+      const int sl = t - 2 * cc;
+      if (sl < 0 || sl >= S) continue;
+      const int col = backward ? 7 - cc : cc;
+      const int* so = L.slab_off.data() + col * (S + 1);
+      const int b = so[sl], e = so[sl + 1];
+      if (e <= b) continue;
+      tab.beg[tab.n] = b;
+      tab.cnt[tab.n] = e - b;
+      tab.blk[tab.n] = nb;
+      nb += (e - b + 3) / 4;
+      ++tab.n;
+    }
+    if (!tab.n) continue;
+    tab.blk[tab.n] = nb;
+    Prof pr(c, sizeof(T) == 8 ? P_FINE_GS64 : P_FINE_GS32);
+    if (backward)
+      hipLaunchKernelGGL((k_fine_gs_wf<T, true>), dim3(nb), dim3(256), 0, c.stream, L.v, (const int4*)L.cl_col, tab,
+                         Prec<T>::w(c), Prec<T>::Ke(c), u, f);
+    else
+      hipLaunchKernelGGL((k_fine_gs_wf<T, false>), dim3(nb), dim3(256), 0, c.stream, L.v, (const int4*)L.cl_col, tab,
+                         Prec<T>::w(c), Prec<T>::Ke(c), u, f);
+  }
+  LAUNCH_CHECK();
+}
+
 template <class T>
 void gs_sweep(Ctx& c, int l, T* u, const T* f, bool backward) {
   Level& L = c.L[l];
   const LevelView& v = L.v;
+  if (l == 0 && L.wf_S > 0 && c.sweep_mode == 1) {
+    gs_sweep_wavefront<T>(c, u, f, backward);
+    return;
+  }
   for (int ci = 0; ci < 8; ++ci) {
     const int col = backward ? 7 - ci : ci;
     const int off = L.col_off[col], nch = L.col_off[col + 1] - off;
@@ -1179,8 +1237,9 @@ int solve_case(Ctx& c, int k, const tos_solve_opts& o, int* its, double* relres)
       }
       finish0(c, 0, S_RZ, 0, OP_NONE);
     }
-    while (it < o.maxit) {
-      apply_level<double, 0>(c, 0, c.p, nullptr, c.q, c.partial);  // q = K p, partials of p.q
+    // q = K p, alpha, u += alpha p, r -= alpha q, |r|^2
+    auto tail = [&]() {
+      apply_level<double, 0>(c, 0, c.p, nullptr, c.q, c.partial);  // partials of p.q
       {
         Prof pr(c, P_VECTOR);
         finish0(c, 0, S_PQ, 0, OP_ALPHA);
@@ -1189,30 +1248,64 @@ int solve_case(Ctx& c, int k, const tos_solve_opts& o, int* its, double* relres)
         finish0(c, 0, S_RN2, 0, OP_NONE);
       }
       LAUNCH_CHECK();
+    };
+    // z = B r, beta, p = z + beta p
+    auto head = [&]() {
+      precondition(c, o.precision, o.pre, o.post, f32, false);
+      Prof pr(c, P_VECTOR);
+      if (f32) {
+        hipLaunchKernelGGL((k_pcg_zdots<float>), dim3(nb), dim3(256), 0, c.stream, cl, nch, NP,
+                           (const float*)L0.u32, (const double*)c.r, (const double*)c.q, c.partial);
+        finish0(c, 1, S_ZR, S_ZQ, OP_BETA);
+        hipLaunchKernelGGL((k_pcg_p<float>), dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, c.p,
+                           (const float*)L0.u32, (const double*)c.scal, 0);
+      } else {
+        hipLaunchKernelGGL((k_pcg_zdots<double>), dim3(nb), dim3(256), 0, c.stream, cl, nch, NP,
+                           (const double*)c.z, (const double*)c.r, (const double*)c.q, c.partial);
+        finish0(c, 1, S_ZR, S_ZQ, OP_BETA);
+        hipLaunchKernelGGL((k_pcg_p<double>), dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, c.p,
+                           (const double*)c.z, (const double*)c.scal, 0);
+      }
+      LAUNCH_CHECK();
+    };
+    // the iteration body [head, tail] is captured once into a graph and replayed (fixed
+    // pointers; scalars stay on the device); not while kernels are timed with events or
+    // while the learned correction takes host-side decisions
+    const bool graphed = !c.prof_on && !(c.learn && (c.learn->mode == 1 || c.learn->ds));
+    hipGraphExec_t exec = nullptr;
+    if (graphed) {
+      for (const Ctx::SolveGraph& g : c.graphs)
+        if (g.precision == o.precision && g.pre == o.pre && g.post == o.post && g.sweep == c.sweep_mode &&
+            g.u == u)
+          exec = g.exec;
+    }
+    bool first = true;
+    while (it < o.maxit) {
+      if (first) {
+        tail();
+        first = false;
+      } else if (graphed) {
+        if (!exec) {
+          Ctx::SolveGraph g{o.precision, o.pre, o.post, c.sweep_mode, u, nullptr, nullptr};
+          HIPCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+          head();
+          tail();
+          HIPCHK(hipStreamEndCapture(c.stream, &g.graph));
+          HIPCHK(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+          c.graphs.push_back(g);
+          exec = g.exec;
+        }
+        HIPCHK(hipGraphLaunch(exec, c.stream));
+      } else {
+        head();
+        tail();
+      }
       ++it;
       const double* h = read_scal(c);
       rn = std::sqrt(h[S_RN2]);
       nn_watch(h);
       if (!std::isfinite(rn)) { c.err = "NaN/Inf in PCG"; return TOS_E_NAN; }
       if (rn / nf <= o.rtol) break;
-      precondition(c, o.precision, o.pre, o.post, f32, false);
-      {
-        Prof pr(c, P_VECTOR);
-        if (f32) {
-          hipLaunchKernelGGL((k_pcg_zdots<float>), dim3(nb), dim3(256), 0, c.stream, cl, nch, NP,
-                             (const float*)L0.u32, (const double*)c.r, (const double*)c.q, c.partial);
-          finish0(c, 1, S_ZR, S_ZQ, OP_BETA);
-          hipLaunchKernelGGL((k_pcg_p<float>), dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, c.p,
-                             (const float*)L0.u32, (const double*)c.scal, 0);
-        } else {
-          hipLaunchKernelGGL((k_pcg_zdots<double>), dim3(nb), dim3(256), 0, c.stream, cl, nch, NP,
-                             (const double*)c.z, (const double*)c.r, (const double*)c.q, c.partial);
-          finish0(c, 1, S_ZR, S_ZQ, OP_BETA);
-          hipLaunchKernelGGL((k_pcg_p<double>), dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, c.p,
-                             (const double*)c.z, (const double*)c.scal, 0);
-        }
-      }
-      LAUNCH_CHECK();
     }
   }
   // true FP64 residual
@@ -1624,6 +1717,10 @@ TOS_EXPORT int tos_ctx_destroy(tos_ctx* c) {
   if (!c) return TOS_E_ARG;
   (void)hipSetDevice(c->dev);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (const Ctx::SolveGraph& g : c->graphs) {
+    (void)hipGraphExecDestroy(g.exec);
+    (void)hipGraphDestroy(g.graph);
+  }
   learn_release(*c);
   batch_release(*c);
   for (void* p : c->allocs) (void)hipFree(p);
@@ -2454,6 +2551,14 @@ TOS_EXPORT int tos_ctx_mma_step(tos_ctx* c, tos_mma* M, double sscale, double vs
                        c->coef, (int64_t)n);
     LAUNCH_CHECK();
     HIPCHK(hipStreamSynchronize(c->stream));
+    return TOS_OK;
+  });
+}
+
+TOS_EXPORT int tos_set_sweep_mode(tos_ctx* c, int mode) {
+  return guard(c, [&]() {
+    REQUIRE(mode == 0 || mode == 1, "bad sweep mode");
+    c->sweep_mode = mode;
     return TOS_OK;
   });
 }

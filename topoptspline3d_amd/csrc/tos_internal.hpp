@@ -43,6 +43,14 @@ __device__ __forceinline__ int64_t xcd_block() {
   return b < G8 ? (b & 7) * (G8 >> 3) + (b >> 3) : b;
 }
 
+// up to 8 chunk ranges of one launch (pieces of a colour-slab wavefront step)
+struct SegTab {
+  int n;
+  int beg[8];      // first chunk of the range in the colour-ordered list
+  int cnt[8];      // chunks in the range
+  int blk[9];      // first block of each range (4 chunks per block), blk[n] = total
+};
+
 struct LevelView {  // passed by value to kernels
   int nx, ny, nz;     // node dims
   int NX, NY;         // padded node row / plane dims
@@ -76,6 +84,10 @@ struct Level {
   int4* cl_col = nullptr;
   int n_col = 0;
   int col_off[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  // colour-slab wavefront of the GS sweep: z-slabs of wf_T node planes, slab_off[c][s]
+  // = first chunk (in the colour-ordered list) of colour c with z >= s * wf_T
+  int wf_T = 0, wf_S = 0;
+  std::vector<int> slab_off;    // 8 x (wf_S + 1)
   int64_t NS = 0;                // stencil slots per entry = 64 * n_col
   int* slot = nullptr;           // padded node -> stencil slot, -1 if not free
   std::vector<int> h_slot;
@@ -161,6 +173,14 @@ struct Ctx {
   size_t ev_used = 0;
   LearnState* learn = nullptr;
   BatchState* batch = nullptr;
+  int sweep_mode = 0;           // fine GS: 0 colour passes, 1 colour-slab wavefront
+  struct SolveGraph {           // captured FCG iteration body per solve configuration
+    int precision, pre, post, sweep;
+    const double* u;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+  };
+  std::vector<SolveGraph> graphs;
 };
 
 constexpr int kRedBlocks = 1024;  // fixed partial count -> deterministic reductions
