@@ -56,14 +56,30 @@ __device__ __forceinline__ void block_partial(double v, double* __restrict__ par
 // ---------------------------------------------------------------------------------------
 // Fine level
 // ---------------------------------------------------------------------------------------
-template <class T>
+// Neighbours read by a colour pass.  ZC = -1: all 27.  ZC = c in 0..7: the first sweep
+// from a zero initial guess at colour c, which reads only the neighbours of colours
+// relaxed before c in this sweep (the others are still 0); the neighbour at offset d has
+// colour c ^ (d_x&1 | (d_y&1)<<1 | (d_z&1)<<2), the node itself (still 0) is skipped too.
+__host__ __device__ constexpr unsigned nb_mask_of(int zc) {
+  unsigned m = 0;
+  for (int k = 0; k < 27; ++k) {
+    const int par = ((k % 3 - 1) & 1) | ((((k / 3) % 3 - 1) & 1) << 1) | (((k / 9 - 1) & 1) << 2);
+    if (zc < 0 || (zc ^ par) < zc) m |= 1u << k;
+  }
+  return m;
+}
+
+template <class T, int ZC = -1>
 __device__ __forceinline__ void load_nbhd(const LevelView& L, const T* __restrict__ u, int64_t p,
                                           T (&un)[27][3]) {
+  constexpr unsigned mask = nb_mask_of(ZC);
 #pragma unroll
   for (int k = 0; k < 27; ++k) {
     const int64_t q = p + (int64_t)(k % 3 - 1) + (int64_t)L.NX * ((k / 3) % 3 - 1 + (int64_t)L.NY * (k / 9 - 1));
 #pragma unroll
-    for (int c = 0; c < 3; ++c) un[k][c] = u[c * L.NP + q];
+    for (int c = 0; c < 3; ++c) {
+      un[k][c] = ((mask >> k) & 1u) ? u[c * L.NP + q] : T(0);
+    }
   }
 }
 
@@ -169,14 +185,15 @@ __global__ TOS_FINE_BOUNDS(T) void k_fine_apply(LevelView L, const int4* __restr
 }
 
 // Gauss-Seidel colour pass on the fine level.  BACKWARD relaxes components z, y, x.
-template <class T, bool BACKWARD>
+// ZC: see load_nbhd (-1 = every neighbour, c = first forward sweep from u = 0 at colour c)
+template <class T, bool BACKWARD, int ZC = -1>
 __global__ TOS_FINE_BOUNDS(T) void k_fine_gs(LevelView L, const int4* __restrict__ cl, int nch,
                                              const T* __restrict__ w, const T* __restrict__ Ke,
                                              T* __restrict__ u, const T* __restrict__ f) {
   NodeRef n;
   if (!chunk_node(cl, nch, 2, n)) return;
   T un[27][3], wa[8];
-  load_nbhd(L, u, n.p, un);
+  load_nbhd<T, ZC>(L, u, n.p, un);
   load_weights(L, w, n.x, n.y, n.z, wa);
   const int64_t p = n.p;
   const T f0 = f[p], f1 = f[L.NP + p], f2 = f[2 * L.NP + p];
@@ -279,17 +296,18 @@ __global__ __launch_bounds__(256) void k_st_apply(LevelView L, const int4* __res
 }
 
 // one colour: cl points at the colour's chunks, slot0 = 64 * (their offset in the list)
-template <class T, bool BACKWARD>
+template <class T, bool BACKWARD, int ZC = -1>
 __global__ __launch_bounds__(256) void k_st_gs(LevelView L, const int4* __restrict__ cl, int nch, int64_t slot0,
                                                const T* __restrict__ st, int64_t NS, T* __restrict__ u,
                                                const T* __restrict__ f) {
+  constexpr unsigned mask = nb_mask_of(ZC);
   NodeRef n;
   if (!chunk_node(cl, nch, 2, n)) return;
   const int64_t p = n.p, slot = slot0 + n.slot;
   T s0 = 0, s1 = 0, s2 = 0;
 #pragma unroll
   for (int k = 0; k < 27; ++k) {
-    if (k == 13) continue;
+    if (k == 13 || !((mask >> k) & 1u)) continue;  // skipped neighbours hold 0: no stencil read
     const int64_t q = p + (int64_t)(k % 3 - 1) + (int64_t)L.NX * ((k / 3) % 3 - 1 + (int64_t)L.NY * (k / 9 - 1));
     const T u0 = u[q], u1 = u[L.NP + q], u2 = u[2 * L.NP + q];
     const T* s = st + (int64_t)k * 9 * NS + slot;
@@ -302,7 +320,10 @@ __global__ __launch_bounds__(256) void k_st_gs(LevelView L, const int4* __restri
 #pragma unroll
   for (int e = 0; e < 9; ++e) D[e] = d[e * NS];
   const T b0 = f[p] - s0, b1 = f[L.NP + p] - s1, b2 = f[2 * L.NP + p] - s2;
-  T v0 = u[p], v1 = u[L.NP + p], v2 = u[2 * L.NP + p];
+  T v0 = 0, v1 = 0, v2 = 0;
+  if constexpr (ZC < 0) {
+    v0 = u[p]; v1 = u[L.NP + p]; v2 = u[2 * L.NP + p];
+  }
   if (!BACKWARD) {
     v0 = (b0 - D[1] * v1 - D[2] * v2) / D[0];
     v1 = (b1 - D[3] * v0 - D[5] * v2) / D[4];

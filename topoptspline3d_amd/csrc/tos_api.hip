@@ -480,11 +480,29 @@ void gs_sweep_wavefront(Ctx& c, T* u, const T* f, bool backward) {
   LAUNCH_CHECK();
 }
 
+#define TOS_COLOUR_SWITCH(col, GO) \
+  switch (col) {                   \
+    case 0: GO(0); break;          \
+    case 1: GO(1); break;          \
+    case 2: GO(2); break;          \
+    case 3: GO(3); break;          \
+    case 4: GO(4); break;          \
+    case 5: GO(5); break;          \
+    case 6: GO(6); break;          \
+    default: GO(7); break;         \
+  }
+
+// One colour-ordered sweep.  from_zero: u is 0 on entry (the first pre-smoothing sweep of a
+// V-cycle level), so colour c reads only the neighbours of colours < c; same updates as
+// a sweep over a zeroed u, without zeroing it first or reading the zeros.
 template <class T>
-void gs_sweep(Ctx& c, int l, T* u, const T* f, bool backward) {
+void gs_sweep(Ctx& c, int l, T* u, const T* f, bool backward, bool from_zero = false) {
   Level& L = c.L[l];
   const LevelView& v = L.v;
   if (l == 0 && L.wf_S > 0 && c.sweep_mode == 1) {
+    if (from_zero)
+      hipLaunchKernelGGL((k_zero_nodes<T>), dim3(cblk(L.n_lin)), dim3(256), 0, c.stream, (const int4*)L.cl_lin,
+                         L.n_lin, L.v.NP, u);
     gs_sweep_wavefront<T>(c, u, f, backward);
     return;
   }
@@ -495,20 +513,30 @@ void gs_sweep(Ctx& c, int l, T* u, const T* f, bool backward) {
     const int4* cl = L.cl_col + off;
     Prof pr(c, l == 0 ? (sizeof(T) == 8 ? P_FINE_GS64 : P_FINE_GS32)
                       : (sizeof(T) == 8 ? P_ST_GS64 : (l == 1 ? P_ST_GS32_L1 : P_ST_GS32)));
+    const dim3 g(cblk(nch)), b(256);
     if (l == 0) {
-      if (backward)
-        hipLaunchKernelGGL((k_fine_gs<T, true>), dim3(cblk(nch)), dim3(256), 0, c.stream, v, cl, nch, Prec<T>::w(c),
-                           Prec<T>::Ke(c), u, f);
-      else
-        hipLaunchKernelGGL((k_fine_gs<T, false>), dim3(cblk(nch)), dim3(256), 0, c.stream, v, cl, nch, Prec<T>::w(c),
-                           Prec<T>::Ke(c), u, f);
+      if (backward) {
+        hipLaunchKernelGGL((k_fine_gs<T, true>), g, b, 0, c.stream, v, cl, nch, Prec<T>::w(c), Prec<T>::Ke(c), u, f);
+      } else if (!from_zero) {
+        hipLaunchKernelGGL((k_fine_gs<T, false>), g, b, 0, c.stream, v, cl, nch, Prec<T>::w(c), Prec<T>::Ke(c), u,
+                           f);
+      } else {
+#define GO(ZC) hipLaunchKernelGGL((k_fine_gs<T, false, ZC>), g, b, 0, c.stream, v, cl, nch, Prec<T>::w(c), \
+                                  Prec<T>::Ke(c), u, f)
+        TOS_COLOUR_SWITCH(col, GO);
+#undef GO
+      }
     } else {
-      if (backward)
-        hipLaunchKernelGGL((k_st_gs<T, true>), dim3(cblk(nch)), dim3(256), 0, c.stream, v, cl, nch, 64 * (int64_t)off,
-                           Prec<T>::st(L), L.NS, u, f);
-      else
-        hipLaunchKernelGGL((k_st_gs<T, false>), dim3(cblk(nch)), dim3(256), 0, c.stream, v, cl, nch,
-                           64 * (int64_t)off, Prec<T>::st(L), L.NS, u, f);
+      const int64_t s0 = 64 * (int64_t)off;
+      if (backward) {
+        hipLaunchKernelGGL((k_st_gs<T, true>), g, b, 0, c.stream, v, cl, nch, s0, Prec<T>::st(L), L.NS, u, f);
+      } else if (!from_zero) {
+        hipLaunchKernelGGL((k_st_gs<T, false>), g, b, 0, c.stream, v, cl, nch, s0, Prec<T>::st(L), L.NS, u, f);
+      } else {
+#define GO(ZC) hipLaunchKernelGGL((k_st_gs<T, false, ZC>), g, b, 0, c.stream, v, cl, nch, s0, Prec<T>::st(L), L.NS, u, f)
+        TOS_COLOUR_SWITCH(col, GO);
+#undef GO
+      }
     }
   }
   LAUNCH_CHECK();
@@ -548,14 +576,16 @@ void vcycle_level(Ctx& c, int l, const T* f, T* u, int pre, int post) {
     return;
   }
   if (L.n_lin == 0) return;
-  hipLaunchKernelGGL((k_zero_nodes<T>), dim3(cblk(L.n_lin)), dim3(256), 0, c.stream, (const int4*)L.cl_lin, L.n_lin,
-                     L.v.NP, u);
+  // u = 0 on entry: the first forward sweep starts from zero without a zeroing pass
   if (l == last) {  // a single level: smoothing only
-    for (int s = 0; s < std::max(1, pre); ++s) gs_sweep<T>(c, l, u, f, false);
+    for (int s = 0; s < std::max(1, pre); ++s) gs_sweep<T>(c, l, u, f, false, s == 0);
     for (int s = 0; s < std::max(1, post); ++s) gs_sweep<T>(c, l, u, f, true);
     return;
   }
-  for (int s = 0; s < pre; ++s) gs_sweep<T>(c, l, u, f, false);
+  if (pre == 0)
+    hipLaunchKernelGGL((k_zero_nodes<T>), dim3(cblk(L.n_lin)), dim3(256), 0, c.stream, (const int4*)L.cl_lin,
+                       L.n_lin, L.v.NP, u);
+  for (int s = 0; s < pre; ++s) gs_sweep<T>(c, l, u, f, false, s == 0);
   T* r = Prec<T>::r(L);
   apply_level<T, 1>(c, l, u, f, r);
   Level& C = c.L[l + 1];
