@@ -68,11 +68,20 @@ __host__ __device__ constexpr unsigned nb_mask_of(int zc) {
   }
   return m;
 }
+// neighbours of colours after c (relaxed later in the sweep)
+__host__ __device__ constexpr unsigned nb_mask_after(int c) {
+  unsigned m = 0;
+  for (int k = 0; k < 27; ++k) {
+    const int par = ((k % 3 - 1) & 1) | ((((k / 3) % 3 - 1) & 1) << 1) | (((k / 9 - 1) & 1) << 2);
+    if ((c ^ par) > c) m |= 1u << k;
+  }
+  return m;
+}
 
-template <class T, int ZC = -1>
+template <class T, int ZC = -1, unsigned MASK = nb_mask_of(ZC)>
 __device__ __forceinline__ void load_nbhd(const LevelView& L, const T* __restrict__ u, int64_t p,
                                           T (&un)[27][3]) {
-  constexpr unsigned mask = nb_mask_of(ZC);
+  constexpr unsigned mask = MASK;
 #pragma unroll
   for (int k = 0; k < 27; ++k) {
     const int64_t q = p + (int64_t)(k % 3 - 1) + (int64_t)L.NX * ((k / 3) % 3 - 1 + (int64_t)L.NY * (k / 9 - 1));
@@ -103,7 +112,7 @@ __device__ __forceinline__ void load_weights(const LevelView& L, const T* __rest
 // loads with FMAs to save registers and serialise the latencies)
 #define TOS_LOADS_DONE() __builtin_amdgcn_sched_barrier(0)
 
-template <class T, bool SELF>
+template <class T, bool SELF, unsigned MASK = 0x7ffffffu>
 __device__ __forceinline__ void fine_row(const T (&wa)[8], const T* __restrict__ Kc, const T (&un)[27][3],
                                          T (&acc)[3], T (&D)[9]) {
 #pragma unroll
@@ -117,6 +126,7 @@ __device__ __forceinline__ void fine_row(const T (&wa)[8], const T* __restrict__
     for (int j = 0; j < 8; ++j) {
       const int nb = (ax + (j & 1)) + 3 * (ay + ((j >> 1) & 1)) + 9 * (az + (j >> 2));
       if (!SELF && j == li) continue;
+      if (!((MASK >> nb) & 1u)) continue;
       const int d = li ^ j;
       // neighbour components reflected into the frame of local node 0
       const T v0 = sx * un[nb][0], v1 = sy * un[nb][1], v2 = sz * un[nb][2];
@@ -215,6 +225,33 @@ __global__ TOS_FINE_BOUNDS(T) void k_fine_gs(LevelView L, const int4* __restrict
   u[p] = v0;
   u[L.NP + p] = v1;
   u[2 * L.NP + p] = v2;
+}
+
+// Residual after ONE forward sweep that started from u = 0, for the nodes of colour ZC.
+// When node i was relaxed, the neighbours of earlier colours already had their final
+// values and those of later colours were 0, and the point GS inside the 3x3 block left
+// the block equations with residual (-D01 v1 - D02 v2, -D12 v2, 0).  Hence
+//   r_i = (-D01 v1 - D02 v2, -D12 v2, 0) - sum_{j of a later colour} A_ij u_j :
+// only the later-colour half of the neighbourhood (and no f) is read.
+template <class T, int ZC>
+__global__ TOS_FINE_BOUNDS(T) void k_fine_resid_z(LevelView L, const int4* __restrict__ cl, int nch,
+                                                  const T* __restrict__ w, const T* __restrict__ Ke,
+                                                  const T* __restrict__ u, T* __restrict__ r) {
+  constexpr unsigned after = nb_mask_after(ZC);
+  NodeRef n;
+  if (!chunk_node(cl, nch, 2, n)) return;
+  T un[27][3], wa[8];
+  load_nbhd<T, ZC, after | (1u << 13)>(L, u, n.p, un);
+  load_weights(L, w, n.x, n.y, n.z, wa);
+  if constexpr (sizeof(T) == 4) TOS_LOADS_DONE();
+  T s[3] = {0, 0, 0};
+  T D[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  fine_row<T, false, after>(wa, Ke, un, s, D);
+  const T v1 = un[13][1], v2 = un[13][2];
+  const int64_t p = n.p;
+  r[p] = -D[1] * v1 - D[2] * v2 - s[0];
+  r[L.NP + p] = -D[5] * v2 - s[1];
+  r[2 * L.NP + p] = -s[2];
 }
 
 // The same pass over several chunk ranges at once (one wavefront step): block b belongs
@@ -336,6 +373,33 @@ __global__ __launch_bounds__(256) void k_st_gs(LevelView L, const int4* __restri
   u[p] = v0;
   u[L.NP + p] = v1;
   u[2 * L.NP + p] = v2;
+}
+
+// stored-stencil counterpart of k_fine_resid_z (nodes of colour ZC, later-colour blocks only)
+template <class T, int ZC>
+__global__ __launch_bounds__(256) void k_st_resid_z(LevelView L, const int4* __restrict__ cl, int nch, int64_t slot0,
+                                                    const T* __restrict__ st, int64_t NS, const T* __restrict__ u,
+                                                    T* __restrict__ r) {
+  constexpr unsigned after = nb_mask_after(ZC);
+  NodeRef n;
+  if (!chunk_node(cl, nch, 2, n)) return;
+  const int64_t p = n.p, slot = slot0 + n.slot;
+  T s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+  for (int k = 0; k < 27; ++k) {
+    if (!((after >> k) & 1u)) continue;
+    const int64_t q = p + (int64_t)(k % 3 - 1) + (int64_t)L.NX * ((k / 3) % 3 - 1 + (int64_t)L.NY * (k / 9 - 1));
+    const T u0 = u[q], u1 = u[L.NP + q], u2 = u[2 * L.NP + q];
+    const T* sp = st + (int64_t)k * 9 * NS + slot;
+    s0 = fma(sp[0], u0, s0); s0 = fma(sp[NS], u1, s0); s0 = fma(sp[2 * NS], u2, s0);
+    s1 = fma(sp[3 * NS], u0, s1); s1 = fma(sp[4 * NS], u1, s1); s1 = fma(sp[5 * NS], u2, s1);
+    s2 = fma(sp[6 * NS], u0, s2); s2 = fma(sp[7 * NS], u1, s2); s2 = fma(sp[8 * NS], u2, s2);
+  }
+  const T* d = st + (int64_t)13 * 9 * NS + slot;
+  const T v1 = u[L.NP + p], v2 = u[2 * L.NP + p];
+  r[p] = -d[NS] * v1 - d[2 * NS] * v2 - s0;
+  r[L.NP + p] = -d[5 * NS] * v2 - s1;
+  r[2 * L.NP + p] = -s2;
 }
 
 // ---------------------------------------------------------------------------------------

@@ -560,6 +560,33 @@ void apply_level(Ctx& c, int l, const T* u, const T* f, T* y, double* partial = 
   LAUNCH_CHECK();
 }
 
+// r = f - A u right after one forward sweep from u = 0 (colour by colour, later-colour
+// neighbours only; see k_fine_resid_z)
+template <class T>
+void resid_after_zero_sweep(Ctx& c, int l, const T* u, T* r) {
+  Level& L = c.L[l];
+  const LevelView& v = L.v;
+  Prof pr(c, l > 0 ? P_ST_RESID : (sizeof(T) == 8 ? P_FINE_RESID64 : P_FINE_RESID32));
+  for (int col = 0; col < 8; ++col) {
+    const int off = L.col_off[col], nch = L.col_off[col + 1] - off;
+    if (nch == 0) continue;
+    const int4* cl = L.cl_col + off;
+    const dim3 g(cblk(nch)), b(256);
+    if (l == 0) {
+#define GO(ZC) hipLaunchKernelGGL((k_fine_resid_z<T, ZC>), g, b, 0, c.stream, v, cl, nch, Prec<T>::w(c), \
+                                  Prec<T>::Ke(c), u, r)
+      TOS_COLOUR_SWITCH(col, GO);
+#undef GO
+    } else {
+      const int64_t s0 = 64 * (int64_t)off;
+#define GO(ZC) hipLaunchKernelGGL((k_st_resid_z<T, ZC>), g, b, 0, c.stream, v, cl, nch, s0, Prec<T>::st(L), L.NS, u, r)
+      TOS_COLOUR_SWITCH(col, GO);
+#undef GO
+    }
+  }
+  LAUNCH_CHECK();
+}
+
 void learned_step(Ctx& c, int l, const float* f, float* u, int pre, int post);
 bool learned_here(const Ctx& c, int l);
 
@@ -587,7 +614,10 @@ void vcycle_level(Ctx& c, int l, const T* f, T* u, int pre, int post) {
                        L.n_lin, L.v.NP, u);
   for (int s = 0; s < pre; ++s) gs_sweep<T>(c, l, u, f, false, s == 0);
   T* r = Prec<T>::r(L);
-  apply_level<T, 1>(c, l, u, f, r);
+  if (pre == 1)  // (the wavefront schedule performs the same updates from a zeroed u)
+    resid_after_zero_sweep<T>(c, l, u, r);
+  else
+    apply_level<T, 1>(c, l, u, f, r);
   Level& C = c.L[l + 1];
   if (C.n_lin) {
     Prof pr(c, P_TRANSFER);
