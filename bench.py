@@ -135,7 +135,7 @@ def kernel_stats(api, ctx, enable):
     return {ks.names[i].value.decode(): (ks.total_ms[i], ks.launches[i]) for i in range(ks.n_classes)}
 
 
-def roofline(stats, info, n_active_nodes, steps):
+def roofline(stats, info, n_active_nodes, steps, workload_tag=None):
     """Roofline of the dominant kernel class (by total time among the classes with a
     single-level definition).  Algorithmic bytes per launch (each array read or written
     once at storage precision, index/flag re-reads excluded):
@@ -159,6 +159,14 @@ def roofline(stats, info, n_active_nodes, steps):
     tot = {k: v[0] for k, v in stats.items() if v[1] > 0}
     out = {"bound": "hbm", "peak": HBM_PEAK, "unit": "GB/s", "traffic": None, "kernel": dom,
            "kernel_ms_per_step": {k: round(v / steps, 3) for k, v in tot.items()}}
+    # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE counters (scaled
+    # by a device-copy calibration) for this kernel class and workload, when recorded
+    tf = os.path.join(ROOT, "profiles", f"{dom}_{workload_tag}.json") if workload_tag else None
+    if tf and os.path.exists(tf):
+        t = json.load(open(tf)).get("traffic")
+        if t:
+            out["traffic"] = round(t["bytes_per_launch"])
+            out["traffic_source"] = os.path.relpath(tf, ROOT)
     if dom in per_launch and stats[dom][1] > 0:
         avg_ms = stats[dom][0] / stats[dom][1]
         ach = per_launch[dom] / (avg_ms * 1e-3) / 1e9
@@ -322,6 +330,8 @@ def main():
                     help="untimed iterations with per-kernel HIP-event timing (roofline)")
     ap.add_argument("--sweep-mode", type=int, default=0,
                     help="fine GS schedule: 0 colour passes, 1 colour-slab wavefront")
+    ap.add_argument("--no-graph-replay", action="store_true",
+                    help="launch the FCG iteration kernel by kernel (for launch-tracing tools)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run a device-to-device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
@@ -350,6 +360,7 @@ def main():
     api = hip_api()
     loop = Loop(api, wl, args.precision)
     loop.ctx.set_sweep_mode(args.sweep_mode)
+    loop.ctx.set_graph_replay(not args.no_graph_replay)
     info = loop.ctx.info()
     nv = active_nodes(wl["solid"])
     log(f"[bench] levels {info['n_levels']} dims {info['level_dims']} coarse dofs {info['coarse_dofs']}")
@@ -399,7 +410,8 @@ def main():
                    "solver": ("FCG, FP32 V-cycle + learned level-2 correction, FP64 refinement"
                               if net is not None else "FCG, FP32 V-cycle, FP64 refinement"),
                    "pcg_iterations": its, "max_rel_res": res_max},
-        "roofline": roofline(stats, info, nv, args.profile_steps),
+        "roofline": roofline(stats, info, nv, args.profile_steps,
+                             "ellipsoids_511x320x320" if wl["name"] == "c2" else None),
         "plain_mixed": {"ms_per_iter": round(ms_plain, 3), "pcg_iterations": its_plain,
                         "max_rel_res": res_plain},
     }

@@ -219,6 +219,11 @@ int tos_ctx_mma_step(tos_ctx* ctx, tos_mma* mma, double sens_scale, double vol_s
  * updates, so results are bit-identical across modes. */
 int tos_set_sweep_mode(tos_ctx* ctx, int mode);
 
+/* The body of an FCG iteration is captured once into a hipGraph and replayed (default
+ * enable = 1); enable = 0 launches it kernel by kernel, e.g. for tools that trace
+ * launches.  Kernel timing (tos_profile) always launches kernel by kernel. */
+int tos_set_graph_replay(tos_ctx* ctx, int enable);
+
 /* ---- measurement ---------------------------------------------------------------------
  * Kernel timing with HIP events recorded on the context's stream around every launch of
  * the solver kernels.  enable = 1 clears and starts recording; enable = 0 stops, waits for

@@ -1689,3 +1689,9 @@ ORC_API int orc_set_sweep_mode(tos_ctx* c, int mode) {
   if (!c || mode < 0 || mode > 1) return TOS_E_ARG;
   return TOS_OK;
 }
+
+// Graph replay is a GPU launch detail; the oracle accepts and ignores the switch.
+ORC_API int orc_set_graph_replay(tos_ctx* c, int enable) {
+  (void)enable;
+  return c ? TOS_OK : TOS_E_ARG;
+}

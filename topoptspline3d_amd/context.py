@@ -168,6 +168,10 @@ class Context:
         """fine GS schedule: 0 colour passes, 1 colour-slab wavefront (same updates)"""
         self._chk(self.api.set_sweep_mode(self.h, int(mode)), "set_sweep_mode")
 
+    def set_graph_replay(self, enable):
+        """replay the captured FCG iteration body (default) or launch kernel by kernel"""
+        self._chk(self.api.set_graph_replay(self.h, int(bool(enable))), "set_graph_replay")
+
     def batch_tuning(self):
         buf = C.create_string_buffer(512)
         self._chk(self.api.batch_tuning(self.h, buf, 512), "batch_tuning")

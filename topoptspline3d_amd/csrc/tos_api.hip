@@ -1130,7 +1130,7 @@ int solve_batched(Ctx& c, const tos_solve_opts& o, tos_solve_stats* st) {
                        (const float*)B.u32[0], (const double*)B.R, (const double*)nullptr, B.part);
     foldb(c, B, nb, 1, B_RZ, 0, OPB_NONE);
     LAUNCH_CHECK();
-    if (!B.exec) {
+    if (!B.exec && c.graph_replay) {
       c.prof_on = false;
       HIPCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
       batch_iteration(c, B, o.pre, o.post);
@@ -1139,7 +1139,10 @@ int solve_batched(Ctx& c, const tos_solve_opts& o, tos_solve_stats* st) {
       c.prof_on = prof;
     }
     while (it < o.maxit) {
-      HIPCHK(hipGraphLaunch(B.exec, c.stream));
+      if (c.graph_replay)
+        HIPCHK(hipGraphLaunch(B.exec, c.stream));
+      else
+        batch_iteration(c, B, o.pre, o.post);
       ++it;
       download(c.stream, B.h_sc, B.sc, (size_t)B_COUNT * K * sizeof(double));
       int still = 0;
@@ -1331,7 +1334,7 @@ int solve_case(Ctx& c, int k, const tos_solve_opts& o, int* its, double* relres)
     // the iteration body [head, tail] is captured once into a graph and replayed (fixed
     // pointers; scalars stay on the device); not while kernels are timed with events or
     // while the learned correction takes host-side decisions
-    const bool graphed = !c.prof_on && !(c.learn && (c.learn->mode == 1 || c.learn->ds));
+    const bool graphed = c.graph_replay && !c.prof_on && !(c.learn && (c.learn->mode == 1 || c.learn->ds));
     hipGraphExec_t exec = nullptr;
     if (graphed) {
       for (const Ctx::SolveGraph& g : c.graphs)
@@ -2611,6 +2614,13 @@ TOS_EXPORT int tos_ctx_mma_step(tos_ctx* c, tos_mma* M, double sscale, double vs
                        c->coef, (int64_t)n);
     LAUNCH_CHECK();
     HIPCHK(hipStreamSynchronize(c->stream));
+    return TOS_OK;
+  });
+}
+
+TOS_EXPORT int tos_set_graph_replay(tos_ctx* c, int enable) {
+  return guard(c, [&]() {
+    c->graph_replay = enable != 0;
     return TOS_OK;
   });
 }

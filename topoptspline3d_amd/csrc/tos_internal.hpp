@@ -174,6 +174,7 @@ struct Ctx {
   LearnState* learn = nullptr;
   BatchState* batch = nullptr;
   int sweep_mode = 0;           // fine GS: 0 colour passes, 1 colour-slab wavefront
+  bool graph_replay = true;     // replay the captured FCG iteration body
   struct SolveGraph {           // captured FCG iteration body per solve configuration
     int precision, pre, post, sweep;
     const double* u;

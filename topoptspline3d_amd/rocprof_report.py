@@ -1,0 +1,104 @@
+"""Summaries of rocprofv3 CSV output for the roofline report.
+
+* kernel time table from ``<name>_kernel_stats.csv`` (``rocprofv3 --kernel-trace --stats
+  --output-format csv``), with a regex aggregate (calls, mean duration);
+* HBM traffic per dispatch from two counter runs (``--pmc FETCH_SIZE`` and
+  ``--pmc WRITE_SIZE``, KiB per dispatch), each scaled by what the same counter reads for
+  a device copy of known size (``bench.py --calibrate-copy-mib N`` under the same
+  counter), because the counters need not count every byte once.
+
+Usage:
+  python -m topoptspline3d_amd.rocprof_report --stats DIR/c2_kernel_stats.csv \\
+      --kernel "k_fine_gs<float" --fetch F.csv --write W.csv --cal-fetch CF.csv \\
+      --cal-write CW.csv --copy-mib 1024 [--json out.json]
+"""
+import argparse
+import csv
+import json
+import re
+import sys
+
+
+def _short(name):
+    return re.sub(r"\(.*", "", name).replace("void ", "").replace("tos::", "")
+
+
+def kernel_table(stats_csv, top=30):
+    rows = list(csv.DictReader(open(stats_csv)))
+    tot = sum(float(r["TotalDurationNs"]) for r in rows)
+    out = ["| kernel | calls | total ms | avg us | % |", "|---|---:|---:|---:|---:|"]
+    for r in rows[:top]:
+        out.append(f"| `{_short(r['Name'])}` | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.2f} | "
+                   f"{float(r['AverageNs']) / 1e3:.2f} | {float(r['Percentage']):.2f} |")
+    out.append(f"\nall kernels: {len(rows)} names, {tot / 1e6:.1f} ms total")
+    return "\n".join(out)
+
+
+def kernel_mean(stats_csv, pattern):
+    """(calls, mean duration in us) over the kernels whose name starts with `pattern`"""
+    calls, ns = 0, 0.0
+    for r in csv.DictReader(open(stats_csv)):
+        if _short(r["Name"]).startswith(pattern):
+            calls += int(r["Calls"])
+            ns += float(r["TotalDurationNs"])
+    return calls, (ns / calls / 1e3 if calls else None)
+
+
+def counter_per_dispatch(counter_csv, pattern):
+    """(dispatches, mean counter value) over the dispatches of kernels matching `pattern`"""
+    n, v = 0, 0.0
+    for r in csv.DictReader(open(counter_csv)):
+        if _short(r["Kernel_Name"]).startswith(pattern):
+            n += 1
+            v += float(r["Counter_Value"])
+    return n, (v / n if n else None)
+
+
+def calibration(counter_csv, copy_bytes):
+    """bytes the copy moved / bytes the counter reported (counter in KiB)"""
+    n, v = counter_per_dispatch(counter_csv, "__amd_rocclr_copyBuffer")
+    if not n:
+        raise SystemExit(f"no copy dispatch in {counter_csv}")
+    return copy_bytes / (v * 1024.0)
+
+
+def traffic(fetch_csv, write_csv, cal_fetch_csv, cal_write_csv, copy_mib, pattern):
+    copy_bytes = copy_mib * (1 << 20)
+    kf = calibration(cal_fetch_csv, copy_bytes)
+    kw = calibration(cal_write_csv, copy_bytes)
+    nf, f = counter_per_dispatch(fetch_csv, pattern)
+    nw, w = counter_per_dispatch(write_csv, pattern)
+    return {"kernel": pattern, "dispatches": [nf, nw], "fetch_kib_raw": f, "write_kib_raw": w,
+            "fetch_scale": kf, "write_scale": kw,
+            "bytes_per_launch": f * 1024.0 * kf + w * 1024.0 * kw}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--stats", required=True)
+    ap.add_argument("--kernel", required=True)
+    ap.add_argument("--fetch")
+    ap.add_argument("--write")
+    ap.add_argument("--cal-fetch")
+    ap.add_argument("--cal-write")
+    ap.add_argument("--copy-mib", type=int, default=1024)
+    ap.add_argument("--top", type=int, default=30)
+    ap.add_argument("--json")
+    a = ap.parse_args(argv)
+    print(kernel_table(a.stats, a.top))
+    calls, us = kernel_mean(a.stats, a.kernel)
+    print(f"\n`{a.kernel}*`: {calls} launches, mean {us:.2f} us")
+    out = {"kernel": a.kernel, "launches": calls, "mean_us": us}
+    if a.fetch:
+        t = traffic(a.fetch, a.write, a.cal_fetch, a.cal_write, a.copy_mib, a.kernel)
+        out["traffic"] = t
+        print(f"\nHBM traffic per launch of `{a.kernel}*`: FETCH_SIZE {t['fetch_kib_raw']:.0f} KiB x "
+              f"{t['fetch_scale']:.3f} + WRITE_SIZE {t['write_kib_raw']:.0f} KiB x {t['write_scale']:.3f} = "
+              f"{t['bytes_per_launch'] / 1e6:.1f} MB ({t['dispatches'][0]} / {t['dispatches'][1]} dispatches)")
+    if a.json:
+        json.dump(out, open(a.json, "w"), indent=1)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
