@@ -484,10 +484,9 @@ __global__ __launch_bounds__(256) void k_galerkin_first(LevelView F, LevelView C
                                                         const uint8_t* __restrict__ cfre,
                                                         const double* __restrict__ w, const double* __restrict__ Ke,
                                                         const double* __restrict__ Mc, double* __restrict__ st,
-                                                        int64_t NS) {
-  __shared__ double sMc[8 * 576];
-  for (int i = threadIdx.x; i < 8 * 576; i += blockDim.x) sMc[i] = Mc[i];
-  __syncthreads();
+                                                        float* __restrict__ st32, int64_t NS) {
+  // Mc entries are read with wave-uniform indices (k is the block row, the element / child
+  // loops are uniform), so they come from the scalar cache as SGPR operands
   NodeRef nr;
   if (!chunk_node(clc, nch, 2, nr)) return;
   const int k = blockIdx.y;
@@ -510,7 +509,7 @@ __global__ __launch_bounds__(256) void k_galerkin_first(LevelView F, LevelView C
         const double we = w[fe];
         if (we == 0) continue;
         if (fallfree[fe]) {
-          const double* m = sMc + ch * 576 + (3 * mI) * 24 + 3 * n;
+          const double* m = Mc + ch * 576 + (3 * mI) * 24 + 3 * n;
 #pragma unroll
           for (int r = 0; r < 3; ++r)
 #pragma unroll
@@ -541,7 +540,10 @@ __global__ __launch_bounds__(256) void k_galerkin_first(LevelView F, LevelView C
     }
   }
 #pragma unroll
-  for (int e = 0; e < 9; ++e) st[(int64_t)(k * 9 + e) * NS + nr.slot] = acc[e];
+  for (int e = 0; e < 9; ++e) {
+    st[(int64_t)(k * 9 + e) * NS + nr.slot] = acc[e];
+    st32[(int64_t)(k * 9 + e) * NS + nr.slot] = (float)acc[e];
+  }
 }
 
 // Deeper levels: K_c[I, I+d] = sum_{a near 2I} P[a,I] sum_{b} K_f[a,b] P[b,I+d]; per axis
@@ -618,7 +620,7 @@ __global__ __launch_bounds__(256) void k_gal_axis(LevelView F, const int* __rest
                                                   const double* __restrict__ sin, int64_t NSI, GBox bi, GBox bo,
                                                   LevelView C, const int4* __restrict__ clc, int nch,
                                                   const uint8_t* __restrict__ cfre, double* __restrict__ sout,
-                                                  int64_t NSO) {
+                                                  float* __restrict__ sout32, int64_t NSO) {
   int o[3];
   int64_t oidx;
   const int k = blockIdx.y;
@@ -670,13 +672,12 @@ __global__ __launch_bounds__(256) void k_gal_axis(LevelView F, const int* __rest
     }
   }
 #pragma unroll
-  for (int e = 0; e < 9; ++e) sout[(int64_t)(k * 9 + e) * NSO + oidx] = acc[e];
+  for (int e = 0; e < 9; ++e) {
+    sout[(int64_t)(k * 9 + e) * NSO + oidx] = acc[e];
+    if (OUT_CMP) sout32[(int64_t)(k * 9 + e) * NSO + oidx] = (float)acc[e];
+  }
 }
 
-__global__ void k_cast_d2f(const double* __restrict__ a, float* __restrict__ b, int64_t n) {
-  const int64_t t = thread_id();
-  if (t < n) b[t] = (float)a[t];
-}
 
 // ---------------------------------------------------------------------------------------
 // Coarsest level: dense matrix over the free nodes, then an in-place blocked symmetric

@@ -1971,13 +1971,13 @@ void galerkin_by_axes(Ctx& c, Level& F, Level& C) {
   }
   hipLaunchKernelGGL((k_gal_axis<0, true, false>), dim3(nblk(n1), 27), dim3(256), 0, c.stream, F.v,
                      (const int*)F.slot, (const double*)F.st64, F.NS, bf, b1, C.v, (const int4*)nullptr, 0,
-                     (const uint8_t*)nullptr, C.gal_b1, n1);
+                     (const uint8_t*)nullptr, C.gal_b1, (float*)nullptr, n1);
   hipLaunchKernelGGL((k_gal_axis<1, false, false>), dim3(nblk(n2), 27), dim3(256), 0, c.stream, F.v,
                      (const int*)nullptr, (const double*)C.gal_b1, n1, b1, b2, C.v, (const int4*)nullptr, 0,
-                     (const uint8_t*)nullptr, C.gal_b2, n2);
+                     (const uint8_t*)nullptr, C.gal_b2, (float*)nullptr, n2);
   hipLaunchKernelGGL((k_gal_axis<2, false, true>), dim3(cblk(C.n_col), 27), dim3(256), 0, c.stream, F.v,
                      (const int*)nullptr, (const double*)C.gal_b2, n2, b2, GBox{C.v.nx, C.v.ny, C.v.nz}, C.v,
-                     (const int4*)C.cl_col, C.n_col, (const uint8_t*)C.fre, C.st64, C.NS);
+                     (const int4*)C.cl_col, C.n_col, (const uint8_t*)C.fre, C.st64, C.st32, C.NS);
   LAUNCH_CHECK();
 }
 
@@ -1995,11 +1995,10 @@ TOS_EXPORT int tos_update_operator(tos_ctx* c) {
       if (l == 1)
         hipLaunchKernelGGL(k_galerkin_first, grid, dim3(256), 0, c->stream, F.v, C.v, (const int4*)C.cl_col,
                            C.n_col, (const uint8_t*)F.fre, (const uint8_t*)c->allfree, (const uint8_t*)C.fre,
-                           (const double*)c->w64, (const double*)c->Ke64, (const double*)c->Mc64, C.st64, C.NS);
+                           (const double*)c->w64, (const double*)c->Ke64, (const double*)c->Mc64, C.st64, C.st32,
+                           C.NS);
       else
-        galerkin_by_axes(*c, F, C);
-      const int64_t ns = 243 * C.NS;
-      hipLaunchKernelGGL(k_cast_d2f, dim3(nblk(ns)), dim3(256), 0, c->stream, (const double*)C.st64, C.st32, ns);
+        galerkin_by_axes(*c, F, C);  // (both write the FP64 and FP32 stencils)
       LAUNCH_CHECK();
     }
     const int rc = coarse_factor(*c);
