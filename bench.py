@@ -5,15 +5,15 @@ irregular 511x320x320 ellipsoid-union domain (~36% solid, z = 0 supported, unit 
 the top surface along the occupancy normal, 3-voxel shell), cubic B-spline density on a
 lattice every 4 voxels, MMA on the coefficients, 7-level geometric MG.  One step = one
 optimisation iteration: spline evaluation, operator update (rho^p, Galerkin levels,
-coarsest factor), flexible PCG to 1e-6 relative residual (FP32 V-cycle, FP64 outer
-residual), compliance, sensitivity, filter, spline adjoint of the compliance and volume
+coarsest factor), flexible PCG to 1e-6 relative residual (FP32 V-cycle reading the
+coarse levels' stencils as FP16 -- precision 2 --, FP64 outer residual), compliance, sensitivity, filter, spline adjoint of the compliance and volume
 sensitivities, MMA step.  All inputs are resident on the GPU before the timed region.
 
 Before the timed region (c2): the learned level-2 correction is trained on pairs logged
 from 200 optimisation iterations on 128^3 ellipsoid domains.  Timed: `steps` iterations of
 plain MG-PCG, then `steps` iterations with the learned correction attached (`value`), then
-plain FP64 iterations; plus solves from zero on the final operator in FP64, mixed and
-learned mode.
+plain FP64 iterations; plus solves from zero on the final operator in FP64, mixed with
+FP32 and with FP16 coarse stencils, and learned mode.
 
 Prints one JSON line.  Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 [--workload c2|c1|c0] [--no-cpu-baseline] [--no-learned] [--no-fp64-compare]
@@ -135,14 +135,19 @@ def kernel_stats(api, ctx, enable):
     return {ks.names[i].value.decode(): (ks.total_ms[i], ks.launches[i]) for i in range(ks.n_classes)}
 
 
-def roofline(stats, info, n_active_nodes, steps, workload_tag=None):
+DTYPES = {0: "f64", 1: "f32 V-cycle / f64 PCG", 2: "f32 V-cycle (f16 coarse stencils) / f64 PCG"}
+SOLVERS = {0: "FP64 V-cycle", 1: "FP32 V-cycle", 2: "FP32 V-cycle on FP16 coarse stencils"}
+
+
+def roofline(stats, info, n_active_nodes, steps, workload_tag=None, precision=1):
     """Roofline of the dominant kernel class (by total time among the classes with a
     single-level definition).  Algorithmic bytes per launch (each array read or written
     once at storage precision, index/flag re-reads excluded):
       fine GS colour pass (FP32): one sweep = 4 N_e + (9*4 + 4) N_v, one launch = 1/8
       fine residual (FP32/FP64):  4 N_e + (9 s + 4) N_v
       fine operator (FP64):       4 N_e + (6*8 + 4) N_v
-      level-1 stencil GS colour pass (FP32): (243*4 + 9*4) N_c1 / 8"""
+      level-1 stencil GS colour pass (FP32 cycle): (243*4 + 9*4) N_c1 / 8, or with the
+        FP16 stencil copy (precision 2) (122*4 + 4 + 9*4) N_c1 / 8"""
     Ne, Nv = info["n_active_elements"][0], n_active_nodes
     Nc1 = info["n_free_nodes"][1] if info["n_levels"] > 1 else 0
     per_launch = {
@@ -151,7 +156,7 @@ def roofline(stats, info, n_active_nodes, steps, workload_tag=None):
         "fine_resid_f32": 4 * Ne + (9 * 4 + 4) * Nv,
         "fine_resid_f64": 4 * Ne + (9 * 8 + 4) * Nv,
         "fine_apply_f64": 4 * Ne + (6 * 8 + 4) * Nv,
-        "st_gs_f32_l1": (243 * 4 + 9 * 4) * Nc1 / 8.0,
+        "st_gs_f32_l1": ((122 * 4 + 4 if precision == 2 else 243 * 4) + 9 * 4) * Nc1 / 8.0,
     }
     flops = {"fine_gs_f32": 1170 * Nv / 8.0, "fine_gs_f64": 1170 * Nv / 8.0, "fine_resid_f32": 1152 * Nv,
              "fine_resid_f64": 1152 * Nv, "fine_apply_f64": 1152 * Nv, "st_gs_f32_l1": 504 * Nc1 / 8.0}
@@ -317,7 +322,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c2")
-    ap.add_argument("--precision", type=int, default=1)
+    ap.add_argument("--precision", type=int, default=2,
+                    help="0 FP64 V-cycle; 1 FP32 V-cycle; 2 FP32 V-cycle on FP16 coarse stencils")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-compare", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
@@ -329,7 +335,7 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=2,
                     help="untimed iterations with per-kernel HIP-event timing (roofline)")
     ap.add_argument("--sweep-mode", type=int, default=0,
-                    help="fine GS schedule: 0 colour passes, 1 colour-slab wavefront")
+                    help="fine GS schedule: 0 colour passes, 1 colour-slab wavefront, 2 row pairs")
     ap.add_argument("--no-graph-replay", action="store_true",
                     help="launch the FCG iteration kernel by kernel (for launch-tracing tools)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -378,7 +384,7 @@ def main():
         sync()
         return (time.perf_counter() - t) * 1e3 / steps, loop.its[n0:], max(loop.res[n0:])
 
-    # plain MG-PCG (FP32 V-cycle, FP64 refinement)
+    # plain MG-PCG at the chosen precision (no learned correction)
     ms_plain, its_plain, res_plain = timed("plain", args.steps)
     learned = None
     if net is not None:
@@ -402,16 +408,16 @@ def main():
         "metric": METRIC, "value": round(ms, 3), "unit": "ms/TO-iter", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 V-cycle / f64 PCG" if args.precision else "f64",
+        "dtype": DTYPES[args.precision],
         "data": "synthetic",
         "config": {"workload": wl["desc"], "grid": list(wl["solid"].shape[::-1]),
                    "active_elements": info["n_active_elements"][0], "active_nodes": nv,
                    "mg_levels": info["n_levels"], "rtol": wl["rtol"], "precision": args.precision,
-                   "solver": ("FCG, FP32 V-cycle + learned level-2 correction, FP64 refinement"
-                              if net is not None else "FCG, FP32 V-cycle, FP64 refinement"),
+                   "solver": ("FCG, " + SOLVERS[args.precision]
+                              + (" + learned level-2 correction" if net is not None else "") + ", FP64 refinement"),
                    "pcg_iterations": its, "max_rel_res": res_max},
         "roofline": roofline(stats, info, nv, args.profile_steps,
-                             "ellipsoids_511x320x320" if wl["name"] == "c2" else None),
+                             "ellipsoids_511x320x320" if wl["name"] == "c2" else None, args.precision),
         "plain_mixed": {"ms_per_iter": round(ms_plain, 3), "pcg_iterations": its_plain,
                         "max_rel_res": res_plain},
     }
@@ -426,7 +432,9 @@ def main():
                               "speedup_of_value": round(ms64 / ms, 3)}
         # same operator, solve from zero to the same tolerance: FP64, mixed, learned
         cmp = {}
-        modes = [("fp64", 0, False), ("mixed", 1, False)] + ([("learned", 1, True)] if net is not None else [])
+        modes = [("fp64", 0, False), ("mixed_fp32_stencils", 1, False), ("mixed_fp16_stencils", 2, False)]
+        if net is not None:
+            modes.append(("learned", args.precision, True))
         for tag, prec, use_net in modes:
             if use_net:
                 LN.attach(loop.ctx, net, level=2)

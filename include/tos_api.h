@@ -67,7 +67,10 @@ typedef struct tos_info {
 typedef struct tos_solve_opts {
   double rtol;              /* stop when ||f - K u||_2 / ||f||_2 <= rtol (per case)     */
   int maxit;                /* outer (flexible PCG) iterations                          */
-  int precision;            /* 0: FP64 V-cycle; 1: FP32 V-cycle inside FP64 PCG        */
+  int precision;            /* 0: FP64 V-cycle; 1: FP32 V-cycle inside FP64 PCG;
+                               2: as 1 with the stored stencils of the coarse levels read
+                               as FP16 (each node's 243 entries scaled by their largest
+                               magnitude); no counterpart in the reference            */
   int pre, post;            /* Gauss-Seidel sweeps per level (reference: 1, 1)          */
   int warm_start;           /* 1: start from the current displacement                   */
   int use_pcg;              /* 1: flexible PCG (default); 0: plain V-cycle iteration as
@@ -116,7 +119,8 @@ int tos_update_operator(tos_ctx* ctx);
 int tos_get_stencil(tos_ctx* ctx, int level, double* out);
 /* y = K(rho) u on the fine level (applyK_OTFA_kernel, grid/Grid.cu:4278-4356). */
 int tos_apply_operator(tos_ctx* ctx, const double* u, double* y);
-/* z = B r, one symmetric V(pre,post)-cycle (HierarchyGrid::v_cycle, grid/Grid.cpp:2399-2444). */
+/* z = B r, one symmetric V(pre,post)-cycle (HierarchyGrid::v_cycle, grid/Grid.cpp:2399-2444);
+ * precision as in tos_solve_opts. */
 int tos_vcycle(tos_ctx* ctx, const double* r, double* z, int precision, int pre, int post);
 
 /* ---- solve -----------------------------------------------------------------------
@@ -214,9 +218,11 @@ int tos_ctx_mma_step(tos_ctx* ctx, tos_mma* mma, double sens_scale, double vol_s
                      double vfrac, int* converged);
 
 /* Fine-level Gauss-Seidel schedule (no counterpart in the reference, which launches one
- * pass per colour): 0 (default) = eight colour passes; 1 = colour-slab wavefront, colour c
- * of z-slab s relaxed in step s + 2c (about 32 slabs).  Both perform exactly the same
- * updates, so results are bit-identical across modes. */
+ * pass per colour): 0 = eight colour passes; 1 = colour-slab wavefront, colour c of z-slab
+ * s relaxed in step s + 2c (about 32 slabs); 2 = four launches of two colours each, one
+ * workgroup per grid row relaxing the row's even-x colour then its odd-x colour.  All
+ * perform exactly the same updates, so results are bit-identical across modes; 0 is the
+ * default (fastest on the 511x320x320 ellipsoid workload). */
 int tos_set_sweep_mode(tos_ctx* ctx, int mode);
 
 /* The body of an FCG iteration is captured once into a hipGraph and replayed (default

@@ -1,0 +1,69 @@
+"""Precision 2: FP32 V-cycles that read the coarse levels' stored stencils as FP16 (each
+node's 243 entries divided by their largest magnitude).  On a two-phase design with a
+1e9 stiffness contrast the FCG must reach the same tolerance in about as many iterations
+as with FP32 stencils (within 10 %: 254 vs 268 on the harder design, whose floating stiff
+islands leave near-rigid modes to the Krylov space), give the same compliance, and use
+copies refreshed by every operator update."""
+import numpy as np
+import pytest
+
+from topoptspline3d_amd.context import Context
+from topoptspline3d_amd.domains import cantilever
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def hip():
+    from topoptspline3d_amd.backend import gpu_available, hip_api
+    if not gpu_available():
+        pytest.skip("no GPU")
+    return hip_api()
+
+
+def two_phase(shape, seed, sigma, thr):
+    from scipy.ndimage import gaussian_filter
+    rng = np.random.default_rng(seed)
+    g = gaussian_filter(rng.normal(size=shape), sigma)
+    rho = np.where(g > thr * g.std(), 1.0, 1e-3).astype(np.float32)
+    rho[:, :, :2] = 1.0
+    rho[:, :, -2:] = 1.0
+    return rho.ravel()
+
+
+def test_half_stencils_converge_like_fp32(hip):
+    nx, ny, nz = 64, 32, 32
+    solid, support, f = cantilever(nx, ny, nz)
+    c = Context(hip, solid, support)
+    c.set_material(1e6, 0.4, 3.0, 1e-3)
+    c.set_loads(f)
+    for seed, sigma, thr in ((0, 3.0, 0.0), (1, 2.0, 0.3)):
+        print(f"\n[half] seed={seed} sigma={sigma} thr={thr}", flush=True)
+        c.set_density(two_phase((nz, ny, nx), seed, sigma, thr))
+        c.update_operator()
+        out = {}
+        for prec in (1, 2, 1):
+            st = c.solve(rtol=1e-6, maxit=600, precision=prec, warm_start=False)
+            assert st["rel_res"][0] <= 1e-6, (prec, st)
+            out[prec] = (st["iterations"][0], c.compliance()[0])
+        (i1, c1), (i2, c2) = out[1], out[2]
+        print(f"[half] iterations fp32 {i1} fp16 {i2}", flush=True)
+        assert i2 <= i1 + max(2, i1 // 10), (i1, i2)
+        assert abs(c2 - c1) <= 1e-5 * abs(c1)
+
+
+def test_half_vcycle_close_to_fp32(hip):
+    nx, ny, nz = 32, 16, 24
+    solid, support, f = cantilever(nx, ny, nz)
+    c = Context(hip, solid, support)
+    c.set_material(1e6, 0.4, 3.0, 1e-3)
+    c.set_loads(f)
+    rng = np.random.default_rng(3)
+    r = rng.normal(size=3 * c.nn)
+    for seed in (4, 5):  # the second operator update must refresh the FP16 copies
+        c.set_density(rng.uniform(0.01, 1.0, solid.size).astype(np.float32))
+        c.update_operator()
+        z1, z2 = c.vcycle(r, precision=1), c.vcycle(r, precision=2)
+        err = np.abs(z2 - z1).max() / np.abs(z1).max()
+        print(f"\n[half] vcycle rel diff {err:.2e}", flush=True)
+        assert 0 < err < 1e-2

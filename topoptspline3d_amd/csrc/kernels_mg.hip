@@ -196,12 +196,10 @@ __global__ TOS_FINE_BOUNDS(T) void k_fine_apply(LevelView L, const int4* __restr
 
 // Gauss-Seidel colour pass on the fine level.  BACKWARD relaxes components z, y, x.
 // ZC: see load_nbhd (-1 = every neighbour, c = first forward sweep from u = 0 at colour c)
-template <class T, bool BACKWARD, int ZC = -1>
-__global__ TOS_FINE_BOUNDS(T) void k_fine_gs(LevelView L, const int4* __restrict__ cl, int nch,
-                                             const T* __restrict__ w, const T* __restrict__ Ke,
-                                             T* __restrict__ u, const T* __restrict__ f) {
-  NodeRef n;
-  if (!chunk_node(cl, nch, 2, n)) return;
+// relax node n (one colour pass body)
+template <class T, bool BACKWARD, int ZC>
+__device__ __forceinline__ void relax_fine(const LevelView& L, const T* __restrict__ w, const T* __restrict__ Ke,
+                                           T* __restrict__ u, const T* __restrict__ f, const NodeRef& n) {
   T un[27][3], wa[8];
   load_nbhd<T, ZC>(L, u, n.p, un);
   load_weights(L, w, n.x, n.y, n.z, wa);
@@ -225,6 +223,52 @@ __global__ TOS_FINE_BOUNDS(T) void k_fine_gs(LevelView L, const int4* __restrict
   u[p] = v0;
   u[L.NP + p] = v1;
   u[2 * L.NP + p] = v2;
+}
+
+// Two colours of one sweep in one launch.  The colours c and c^1 (same y, z parity,
+// opposite x parity) depend on each other only along x within a grid row, so one
+// workgroup takes a whole row: first its nodes of colour CA, then - after a workgroup
+// barrier - those of colour CB (CA relaxed first in the sweep order).  Same updates as
+// two colour passes; each neighbourhood is streamed from HBM once for both.
+// rows[r] = {first chunk of colour CA in the colour-ordered list, count, same for CB}.
+// ZA / ZB: -1, or the colour for the first sweep from zero (see load_nbhd).
+template <class T, bool BACKWARD, int ZA, int ZB>
+__global__ TOS_FINE_BOUNDS(T) void k_fine_gs_pair(LevelView L, const int4* __restrict__ cl,
+                                                  const int4* __restrict__ rows, int nrows,
+                                                  const T* __restrict__ w, const T* __restrict__ Ke,
+                                                  T* __restrict__ u, const T* __restrict__ f) {
+  const int r = (int)xcd_block();
+  if (r >= nrows) return;  // whole workgroup
+  const int4 R = rows[r];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll 1
+  for (int ph = 0; ph < 2; ++ph) {
+    if (ph) __syncthreads();
+    const int beg = ph ? R.z : R.x, cnt = ph ? R.w : R.y;
+    for (int i = wv; i < cnt; i += 4) {
+      const int4 c = cl[beg + i];
+      if (lane < (c.y >> 16)) {
+        const NodeRef n{(int64_t)c.x + 2 * lane, (c.y & 0xffff) + 2 * lane, c.z, c.w, 0};
+        if constexpr (ZA == ZB) {
+          relax_fine<T, BACKWARD, ZA>(L, w, Ke, u, f, n);
+        } else {
+          if (ph)
+            relax_fine<T, BACKWARD, ZB>(L, w, Ke, u, f, n);
+          else
+            relax_fine<T, BACKWARD, ZA>(L, w, Ke, u, f, n);
+        }
+      }
+    }
+  }
+}
+
+template <class T, bool BACKWARD, int ZC = -1>
+__global__ TOS_FINE_BOUNDS(T) void k_fine_gs(LevelView L, const int4* __restrict__ cl, int nch,
+                                             const T* __restrict__ w, const T* __restrict__ Ke,
+                                             T* __restrict__ u, const T* __restrict__ f) {
+  NodeRef n;
+  if (!chunk_node(cl, nch, 2, n)) return;
+  relax_fine<T, BACKWARD, ZC>(L, w, Ke, u, f, n);
 }
 
 // Residual after ONE forward sweep that started from u = 0, for the nodes of colour ZC.
@@ -280,50 +324,88 @@ __global__ TOS_FINE_BOUNDS(T) void k_fine_gs_wf(LevelView L, const int4* __restr
                                                 T* __restrict__ u, const T* __restrict__ f) {
   NodeRef n;
   if (!seg_node(cl, tab, n)) return;
-  T un[27][3], wa[8];
-  load_nbhd(L, u, n.p, un);
-  load_weights(L, w, n.x, n.y, n.z, wa);
-  const int64_t p = n.p;
-  const T f0 = f[p], f1 = f[L.NP + p], f2 = f[2 * L.NP + p];
-  if constexpr (sizeof(T) == 4) TOS_LOADS_DONE();
-  T s[3] = {0, 0, 0};
-  T D[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  fine_row<T, false>(wa, Ke, un, s, D);
-  const T b0 = f0 - s[0], b1 = f1 - s[1], b2 = f2 - s[2];
-  T v0 = un[13][0], v1 = un[13][1], v2 = un[13][2];
-  if (!BACKWARD) {
-    v0 = (b0 - D[1] * v1 - D[2] * v2) / D[0];
-    v1 = (b1 - D[3] * v0 - D[5] * v2) / D[4];
-    v2 = (b2 - D[6] * v0 - D[7] * v1) / D[8];
-  } else {
-    v2 = (b2 - D[6] * v0 - D[7] * v1) / D[8];
-    v1 = (b1 - D[3] * v0 - D[5] * v2) / D[4];
-    v0 = (b0 - D[1] * v1 - D[2] * v2) / D[0];
-  }
-  u[p] = v0;
-  u[L.NP + p] = v1;
-  u[2 * L.NP + p] = v2;
+  relax_fine<T, BACKWARD, -1>(L, w, Ke, u, f, n);
 }
 
 // ---------------------------------------------------------------------------------------
 // Stored-stencil levels (colour-compact stencil, NS slots per entry)
 // ---------------------------------------------------------------------------------------
-template <class T, int MODE>
-__global__ __launch_bounds__(256) void k_st_apply(LevelView L, const int4* __restrict__ cl, int nch,
-                                                  const T* __restrict__ st, int64_t NS, const T* __restrict__ u,
-                                                  const T* __restrict__ f, T* __restrict__ y) {
+// Readers of the compact stencil: entry e (3x3 row-major) of block k of the node in slot s.
+template <class T>
+struct StFull {  // T entries at (9k + e) NS + s
+  const T* __restrict__ p;
+  int64_t NS;
+  static constexpr bool kScaled = false;
+  __device__ __forceinline__ T get(int64_t s, int k, int e) const { return p[(int64_t)(k * 9 + e) * NS + s]; }
+  __device__ __forceinline__ T scale(int64_t) const { return T(1); }
+};
+
+__device__ __forceinline__ float half_bits(uint32_t h) {
+  return (float)__builtin_bit_cast(_Float16, (unsigned short)h);
+}
+__device__ __forceinline__ uint32_t to_half_bits(float x) {
+  return (uint32_t)__builtin_bit_cast(unsigned short, (_Float16)x);
+}
+
+// FP16 entries of the node's stencil divided by its largest magnitude m = msc[s] (every
+// stored value in [-1, 1]), two per 32-bit word: entries 2w, 2w+1 of block k in word
+// 4k + w (w < 4); entry 8 of block k in half k&1 of word 108 + k/2.  122 words per node
+// instead of 243 floats.
+constexpr int kHalfWords = 122;
+struct StHalf {
+  const uint32_t* __restrict__ p;
+  const float* __restrict__ msc;
+  int64_t NS;
+  static constexpr bool kScaled = true;
+  __device__ __forceinline__ float get(int64_t s, int k, int e) const {
+    const int w = e < 8 ? 4 * k + (e >> 1) : 108 + (k >> 1);
+    const int hi = e < 8 ? (e & 1) : (k & 1);
+    const uint32_t v = p[(int64_t)w * NS + s];
+    return half_bits(hi ? (v >> 16) : (v & 0xffffu));
+  }
+  __device__ __forceinline__ float scale(int64_t s) const { return msc[s]; }
+};
+
+// FP16 copy (StHalf layout) of a compact FP32 stencil; thread per slot of the colour list
+__global__ __launch_bounds__(256) void k_st_pack16(const int4* __restrict__ cl, int nch,
+                                                   const float* __restrict__ st, int64_t NS,
+                                                   uint32_t* __restrict__ h, float* __restrict__ msc) {
   NodeRef n;
   if (!chunk_node(cl, nch, 2, n)) return;
-  const int64_t p = n.p;
+  const int64_t s = n.slot;
+  float mx = 0;
+  for (int q = 0; q < 243; ++q) mx = fmaxf(mx, fabsf(st[(int64_t)q * NS + s]));
+  if (!(mx > 0)) mx = 1;
+  const float inv = 1.0f / mx;
+  msc[s] = mx;
+  auto hb = [&](int k, int e) { return to_half_bits(st[(int64_t)(k * 9 + e) * NS + s] * inv); };
+  for (int k = 0; k < 27; ++k)
+#pragma unroll
+    for (int w = 0; w < 4; ++w) h[(int64_t)(4 * k + w) * NS + s] = hb(k, 2 * w) | (hb(k, 2 * w + 1) << 16);
+  for (int t = 0; t < 14; ++t)
+    h[(int64_t)(108 + t) * NS + s] = hb(2 * t, 8) | (2 * t + 1 < 27 ? hb(2 * t + 1, 8) << 16 : 0u);
+}
+
+// y = K u (MODE 0) or f - K u (MODE 1) over the whole colour-ordered list
+template <class T, int MODE, class ST>
+__global__ __launch_bounds__(256) void k_st_apply(LevelView L, const int4* __restrict__ cl, int nch, ST st,
+                                                  const T* __restrict__ u, const T* __restrict__ f,
+                                                  T* __restrict__ y) {
+  NodeRef n;
+  if (!chunk_node(cl, nch, 2, n)) return;
+  const int64_t p = n.p, s = n.slot;
   T a0 = 0, a1 = 0, a2 = 0;
 #pragma unroll
   for (int k = 0; k < 27; ++k) {
     const int64_t q = p + (int64_t)(k % 3 - 1) + (int64_t)L.NX * ((k / 3) % 3 - 1 + (int64_t)L.NY * (k / 9 - 1));
     const T u0 = u[q], u1 = u[L.NP + q], u2 = u[2 * L.NP + q];
-    const T* s = st + (int64_t)k * 9 * NS + n.slot;
-    a0 = fma(s[0], u0, a0); a0 = fma(s[NS], u1, a0); a0 = fma(s[2 * NS], u2, a0);
-    a1 = fma(s[3 * NS], u0, a1); a1 = fma(s[4 * NS], u1, a1); a1 = fma(s[5 * NS], u2, a1);
-    a2 = fma(s[6 * NS], u0, a2); a2 = fma(s[7 * NS], u1, a2); a2 = fma(s[8 * NS], u2, a2);
+    a0 = fma(st.get(s, k, 0), u0, a0); a0 = fma(st.get(s, k, 1), u1, a0); a0 = fma(st.get(s, k, 2), u2, a0);
+    a1 = fma(st.get(s, k, 3), u0, a1); a1 = fma(st.get(s, k, 4), u1, a1); a1 = fma(st.get(s, k, 5), u2, a1);
+    a2 = fma(st.get(s, k, 6), u0, a2); a2 = fma(st.get(s, k, 7), u1, a2); a2 = fma(st.get(s, k, 8), u2, a2);
+  }
+  if constexpr (ST::kScaled) {
+    const T m = st.scale(s);
+    a0 *= m; a1 *= m; a2 *= m;
   }
   if (MODE == 0) {
     y[p] = a0; y[L.NP + p] = a1; y[2 * L.NP + p] = a2;
@@ -332,11 +414,12 @@ __global__ __launch_bounds__(256) void k_st_apply(LevelView L, const int4* __res
   }
 }
 
-// one colour: cl points at the colour's chunks, slot0 = 64 * (their offset in the list)
-template <class T, bool BACKWARD, int ZC = -1>
+// one colour: cl points at the colour's chunks, slot0 = 64 * (their offset in the list).
+// With a scaled stencil the block equations are solved in the scaled form
+// (K~ = K / m, f / m), which gives the same update.
+template <class T, bool BACKWARD, int ZC, class ST>
 __global__ __launch_bounds__(256) void k_st_gs(LevelView L, const int4* __restrict__ cl, int nch, int64_t slot0,
-                                               const T* __restrict__ st, int64_t NS, T* __restrict__ u,
-                                               const T* __restrict__ f) {
+                                               ST st, T* __restrict__ u, const T* __restrict__ f) {
   constexpr unsigned mask = nb_mask_of(ZC);
   NodeRef n;
   if (!chunk_node(cl, nch, 2, n)) return;
@@ -347,16 +430,19 @@ __global__ __launch_bounds__(256) void k_st_gs(LevelView L, const int4* __restri
     if (k == 13 || !((mask >> k) & 1u)) continue;  // skipped neighbours hold 0: no stencil read
     const int64_t q = p + (int64_t)(k % 3 - 1) + (int64_t)L.NX * ((k / 3) % 3 - 1 + (int64_t)L.NY * (k / 9 - 1));
     const T u0 = u[q], u1 = u[L.NP + q], u2 = u[2 * L.NP + q];
-    const T* s = st + (int64_t)k * 9 * NS + slot;
-    s0 = fma(s[0], u0, s0); s0 = fma(s[NS], u1, s0); s0 = fma(s[2 * NS], u2, s0);
-    s1 = fma(s[3 * NS], u0, s1); s1 = fma(s[4 * NS], u1, s1); s1 = fma(s[5 * NS], u2, s1);
-    s2 = fma(s[6 * NS], u0, s2); s2 = fma(s[7 * NS], u1, s2); s2 = fma(s[8 * NS], u2, s2);
+    s0 = fma(st.get(slot, k, 0), u0, s0); s0 = fma(st.get(slot, k, 1), u1, s0); s0 = fma(st.get(slot, k, 2), u2, s0);
+    s1 = fma(st.get(slot, k, 3), u0, s1); s1 = fma(st.get(slot, k, 4), u1, s1); s1 = fma(st.get(slot, k, 5), u2, s1);
+    s2 = fma(st.get(slot, k, 6), u0, s2); s2 = fma(st.get(slot, k, 7), u1, s2); s2 = fma(st.get(slot, k, 8), u2, s2);
   }
-  const T* d = st + (int64_t)13 * 9 * NS + slot;
   T D[9];
 #pragma unroll
-  for (int e = 0; e < 9; ++e) D[e] = d[e * NS];
-  const T b0 = f[p] - s0, b1 = f[L.NP + p] - s1, b2 = f[2 * L.NP + p] - s2;
+  for (int e = 0; e < 9; ++e) D[e] = st.get(slot, 13, e);
+  T f0 = f[p], f1 = f[L.NP + p], f2 = f[2 * L.NP + p];
+  if constexpr (ST::kScaled) {
+    const T inv = T(1) / st.scale(slot);
+    f0 *= inv; f1 *= inv; f2 *= inv;
+  }
+  const T b0 = f0 - s0, b1 = f1 - s1, b2 = f2 - s2;
   T v0 = 0, v1 = 0, v2 = 0;
   if constexpr (ZC < 0) {
     v0 = u[p]; v1 = u[L.NP + p]; v2 = u[2 * L.NP + p];
@@ -376,10 +462,9 @@ __global__ __launch_bounds__(256) void k_st_gs(LevelView L, const int4* __restri
 }
 
 // stored-stencil counterpart of k_fine_resid_z (nodes of colour ZC, later-colour blocks only)
-template <class T, int ZC>
+template <class T, int ZC, class ST>
 __global__ __launch_bounds__(256) void k_st_resid_z(LevelView L, const int4* __restrict__ cl, int nch, int64_t slot0,
-                                                    const T* __restrict__ st, int64_t NS, const T* __restrict__ u,
-                                                    T* __restrict__ r) {
+                                                    ST st, const T* __restrict__ u, T* __restrict__ r) {
   constexpr unsigned after = nb_mask_after(ZC);
   NodeRef n;
   if (!chunk_node(cl, nch, 2, n)) return;
@@ -390,16 +475,21 @@ __global__ __launch_bounds__(256) void k_st_resid_z(LevelView L, const int4* __r
     if (!((after >> k) & 1u)) continue;
     const int64_t q = p + (int64_t)(k % 3 - 1) + (int64_t)L.NX * ((k / 3) % 3 - 1 + (int64_t)L.NY * (k / 9 - 1));
     const T u0 = u[q], u1 = u[L.NP + q], u2 = u[2 * L.NP + q];
-    const T* sp = st + (int64_t)k * 9 * NS + slot;
-    s0 = fma(sp[0], u0, s0); s0 = fma(sp[NS], u1, s0); s0 = fma(sp[2 * NS], u2, s0);
-    s1 = fma(sp[3 * NS], u0, s1); s1 = fma(sp[4 * NS], u1, s1); s1 = fma(sp[5 * NS], u2, s1);
-    s2 = fma(sp[6 * NS], u0, s2); s2 = fma(sp[7 * NS], u1, s2); s2 = fma(sp[8 * NS], u2, s2);
+    s0 = fma(st.get(slot, k, 0), u0, s0); s0 = fma(st.get(slot, k, 1), u1, s0); s0 = fma(st.get(slot, k, 2), u2, s0);
+    s1 = fma(st.get(slot, k, 3), u0, s1); s1 = fma(st.get(slot, k, 4), u1, s1); s1 = fma(st.get(slot, k, 5), u2, s1);
+    s2 = fma(st.get(slot, k, 6), u0, s2); s2 = fma(st.get(slot, k, 7), u1, s2); s2 = fma(st.get(slot, k, 8), u2, s2);
   }
-  const T* d = st + (int64_t)13 * 9 * NS + slot;
   const T v1 = u[L.NP + p], v2 = u[2 * L.NP + p];
-  r[p] = -d[NS] * v1 - d[2 * NS] * v2 - s0;
-  r[L.NP + p] = -d[5 * NS] * v2 - s1;
-  r[2 * L.NP + p] = -s2;
+  T r0 = -st.get(slot, 13, 1) * v1 - st.get(slot, 13, 2) * v2 - s0;
+  T r1 = -st.get(slot, 13, 5) * v2 - s1;
+  T r2 = -s2;
+  if constexpr (ST::kScaled) {
+    const T m = st.scale(slot);
+    r0 *= m; r1 *= m; r2 *= m;
+  }
+  r[p] = r0;
+  r[L.NP + p] = r1;
+  r[2 * L.NP + p] = r2;
 }
 
 // ---------------------------------------------------------------------------------------

@@ -306,6 +306,30 @@ void build_chunks(Ctx& c, Level& L, bool with_slots) {
       }
   }
   L.col_off[8] = (int)col.size();
+  // row lists of the two-colour GS launches (fine level)
+  if (!with_slots) {
+    for (int cls = 0; cls < 4; ++cls) {
+      const int A = 2 * cls, B = A + 1;  // colours (even x, odd x) of class (y%2, z%2) = (cls&1, cls>>1)
+      int ia = L.col_off[A], ea = L.col_off[A + 1], ib = L.col_off[B], eb = L.col_off[B + 1];
+      std::vector<int4> fwd, bwd;
+      auto key = [&](int i) { return (int64_t)col[i].w * (1 << 20) + col[i].z; };
+      while (ia < ea || ib < eb) {
+        const int64_t k = std::min(ia < ea ? key(ia) : INT64_MAX, ib < eb ? key(ib) : INT64_MAX);
+        const int a0 = ia, b0 = ib;
+        while (ia < ea && key(ia) == k) ++ia;
+        while (ib < eb && key(ib) == k) ++ib;
+        fwd.push_back(make_int4(a0, ia - a0, b0, ib - b0));
+        bwd.push_back(make_int4(b0, ib - b0, a0, ia - a0));
+      }
+      L.n_pair[cls] = (int)fwd.size();
+      L.pair_fwd[cls] = dalloc<int4>(c, std::max<size_t>(fwd.size(), 1));
+      L.pair_bwd[cls] = dalloc<int4>(c, std::max<size_t>(bwd.size(), 1));
+      if (!fwd.empty()) {
+        upload(c.stream, L.pair_fwd[cls], fwd.data(), fwd.size() * sizeof(int4));
+        upload(c.stream, L.pair_bwd[cls], bwd.data(), bwd.size() * sizeof(int4));
+      }
+    }
+  }
   // colour-slab wavefront of the fine GS: ~32 z-slabs so that the slabs in flight during a
   // few consecutive steps fit in the memory-side cache
   if (!with_slots) {
@@ -440,6 +464,42 @@ struct Prec<float> {
   static const float* Ke(Ctx& c) { return c.Kc32; }
 };
 
+// calls fn with the reader of level L's stencil: the FP16 copy in an FP32 cycle of a
+// solve with precision 2, else the stencil at the cycle's precision
+template <class T, class Fn>
+void st_dispatch(Ctx& c, Level& L, Fn&& fn) {
+  if constexpr (std::is_same<T, float>::value) {
+    if (c.half_st) {
+      fn(StHalf{L.st16, L.st_m, L.NS});
+      return;
+    }
+  }
+  fn(StFull<T>{Prec<T>::st(L), L.NS});
+}
+
+// FP16 copies of the FP32 stencils of every stored-stencil level (allocated on first use)
+void pack_half_stencils(Ctx& c) {
+  for (size_t l = 1; l < c.L.size(); ++l) {
+    Level& L = c.L[l];
+    if (!L.n_col) continue;
+    if (!L.st16) {
+      L.st16 = dalloc<uint32_t>(c, (size_t)kHalfWords * L.NS);
+      L.st_m = dalloc<float>(c, L.NS);
+    }
+    Prof pr(c, P_GALERKIN);
+    hipLaunchKernelGGL(k_st_pack16, dim3(cblk(L.n_col)), dim3(256), 0, c.stream, (const int4*)L.cl_col, L.n_col,
+                       (const float*)L.st32, L.NS, L.st16, L.st_m);
+  }
+  LAUNCH_CHECK();
+  c.st16_valid = true;
+}
+
+// precision 2 = FP32 cycle on FP16 stored stencils: make sure the copies are current
+void select_stencils(Ctx& c, int precision) {
+  c.half_st = precision == 2;
+  if (c.half_st && !c.st16_valid) pack_half_stencils(c);
+}
+
 // Fine-level sweep as a colour-slab wavefront: step t relaxes colour c (in sweep order
 // c' = 0..7) on z-slab s = t - 2 c'.  Neighbouring nodes differ by at most one slab, so a
 // node of colour c' is relaxed at least one step after every neighbour of an earlier
@@ -492,6 +552,40 @@ void gs_sweep_wavefront(Ctx& c, T* u, const T* f, bool backward) {
     default: GO(7); break;         \
   }
 
+// Fine-level sweep in four two-colour launches (k_fine_gs_pair): classes (y%2, z%2) in
+// colour order, forward (0,0) (1,0) (0,1) (1,1), backward the reverse.
+template <class T>
+void gs_sweep_pairs(Ctx& c, T* u, const T* f, bool backward, bool from_zero) {
+  Level& L = c.L[0];
+  const LevelView& v = L.v;
+  for (int i = 0; i < 4; ++i) {
+    const int cls = backward ? 3 - i : i;
+    const int n = L.n_pair[cls];
+    if (!n) continue;
+    Prof pr(c, sizeof(T) == 8 ? P_FINE_GS64 : P_FINE_GS32);
+    const dim3 g(n), b(256);
+    const int4* cl = L.cl_col;
+    if (backward) {
+      hipLaunchKernelGGL((k_fine_gs_pair<T, true, -1, -1>), g, b, 0, c.stream, v, cl, (const int4*)L.pair_bwd[cls],
+                         n, Prec<T>::w(c), Prec<T>::Ke(c), u, f);
+    } else if (!from_zero) {
+      hipLaunchKernelGGL((k_fine_gs_pair<T, false, -1, -1>), g, b, 0, c.stream, v, cl, (const int4*)L.pair_fwd[cls],
+                         n, Prec<T>::w(c), Prec<T>::Ke(c), u, f);
+    } else {
+#define GO(A) hipLaunchKernelGGL((k_fine_gs_pair<T, false, A, A + 1>), g, b, 0, c.stream, v, cl, \
+                                 (const int4*)L.pair_fwd[cls], n, Prec<T>::w(c), Prec<T>::Ke(c), u, f)
+      switch (cls) {
+        case 0: GO(0); break;
+        case 1: GO(2); break;
+        case 2: GO(4); break;
+        default: GO(6); break;
+      }
+#undef GO
+    }
+  }
+  LAUNCH_CHECK();
+}
+
 // One colour-ordered sweep.  from_zero: u is 0 on entry (the first pre-smoothing sweep of a
 // V-cycle level), so colour c reads only the neighbours of colours < c; same updates as
 // a sweep over a zeroed u, without zeroing it first or reading the zeros.
@@ -499,6 +593,10 @@ template <class T>
 void gs_sweep(Ctx& c, int l, T* u, const T* f, bool backward, bool from_zero = false) {
   Level& L = c.L[l];
   const LevelView& v = L.v;
+  if (l == 0 && c.sweep_mode == 2) {
+    gs_sweep_pairs<T>(c, u, f, backward, from_zero);
+    return;
+  }
   if (l == 0 && L.wf_S > 0 && c.sweep_mode == 1) {
     if (from_zero)
       hipLaunchKernelGGL((k_zero_nodes<T>), dim3(cblk(L.n_lin)), dim3(256), 0, c.stream, (const int4*)L.cl_lin,
@@ -528,15 +626,18 @@ void gs_sweep(Ctx& c, int l, T* u, const T* f, bool backward, bool from_zero = f
       }
     } else {
       const int64_t s0 = 64 * (int64_t)off;
-      if (backward) {
-        hipLaunchKernelGGL((k_st_gs<T, true>), g, b, 0, c.stream, v, cl, nch, s0, Prec<T>::st(L), L.NS, u, f);
-      } else if (!from_zero) {
-        hipLaunchKernelGGL((k_st_gs<T, false>), g, b, 0, c.stream, v, cl, nch, s0, Prec<T>::st(L), L.NS, u, f);
-      } else {
-#define GO(ZC) hipLaunchKernelGGL((k_st_gs<T, false, ZC>), g, b, 0, c.stream, v, cl, nch, s0, Prec<T>::st(L), L.NS, u, f)
-        TOS_COLOUR_SWITCH(col, GO);
+      st_dispatch<T>(c, L, [&](auto st) {
+        using ST = decltype(st);
+        if (backward) {
+          hipLaunchKernelGGL((k_st_gs<T, true, -1, ST>), g, b, 0, c.stream, v, cl, nch, s0, st, u, f);
+        } else if (!from_zero) {
+          hipLaunchKernelGGL((k_st_gs<T, false, -1, ST>), g, b, 0, c.stream, v, cl, nch, s0, st, u, f);
+        } else {
+#define GO(ZC) hipLaunchKernelGGL((k_st_gs<T, false, ZC, ST>), g, b, 0, c.stream, v, cl, nch, s0, st, u, f)
+          TOS_COLOUR_SWITCH(col, GO);
 #undef GO
-      }
+        }
+      });
     }
   }
   LAUNCH_CHECK();
@@ -554,8 +655,10 @@ void apply_level(Ctx& c, int l, const T* u, const T* f, T* y, double* partial = 
       hipLaunchKernelGGL((k_fine_apply<T, MODE>), dim3(cblk(L.n_lin)), dim3(256), 0, c.stream, L.v,
                          (const int4*)L.cl_lin, L.n_lin, Prec<T>::w(c), Prec<T>::Ke(c), u, f, y, partial, y32);
   } else if (L.n_col) {
-    hipLaunchKernelGGL((k_st_apply<T, MODE>), dim3(cblk(L.n_col)), dim3(256), 0, c.stream, L.v,
-                       (const int4*)L.cl_col, L.n_col, Prec<T>::st(L), L.NS, u, f, y);
+    st_dispatch<T>(c, L, [&](auto st) {
+      hipLaunchKernelGGL((k_st_apply<T, MODE, decltype(st)>), dim3(cblk(L.n_col)), dim3(256), 0, c.stream, L.v,
+                         (const int4*)L.cl_col, L.n_col, st, u, f, y);
+    });
   }
   LAUNCH_CHECK();
 }
@@ -579,9 +682,11 @@ void resid_after_zero_sweep(Ctx& c, int l, const T* u, T* r) {
 #undef GO
     } else {
       const int64_t s0 = 64 * (int64_t)off;
-#define GO(ZC) hipLaunchKernelGGL((k_st_resid_z<T, ZC>), g, b, 0, c.stream, v, cl, nch, s0, Prec<T>::st(L), L.NS, u, r)
-      TOS_COLOUR_SWITCH(col, GO);
+      st_dispatch<T>(c, L, [&](auto st) {
+#define GO(ZC) hipLaunchKernelGGL((k_st_resid_z<T, ZC, decltype(st)>), g, b, 0, c.stream, v, cl, nch, s0, st, u, r)
+        TOS_COLOUR_SWITCH(col, GO);
 #undef GO
+      });
     }
   }
   LAUNCH_CHECK();
@@ -1984,6 +2089,7 @@ void galerkin_by_axes(Ctx& c, Level& F, Level& C) {
 TOS_EXPORT int tos_update_operator(tos_ctx* c) {
   return guard(c, [&]() {
     const LevelView& v0 = c->L[0].v;
+    c->st16_valid = false;
     Prof pr(*c, P_GALERKIN);
     hipLaunchKernelGGL(k_weights, dim3(nblk(v0.NE)), dim3(256), 0, c->stream, v0, (const uint8_t*)c->eact,
                        (const float*)c->rho, c->penal, c->w64, c->w32);
@@ -2047,8 +2153,9 @@ TOS_EXPORT int tos_apply_operator(tos_ctx* c, const double* u, double* y) {
 
 TOS_EXPORT int tos_vcycle(tos_ctx* c, const double* r, double* z, int precision, int pre, int post) {
   return guard(c, [&]() {
-    REQUIRE(r && z && pre >= 0 && post >= 0, "bad arguments");
+    REQUIRE(r && z && pre >= 0 && post >= 0 && precision >= 0 && precision <= 2, "bad arguments");
     if (!c->op_ready) { c->err = "update_operator first"; return TOS_E_STATE; }
+    select_stencils(*c, precision);
     Level& L0 = c->L[0];
     const int64_t n = (int64_t)L0.v.nx * L0.v.ny * L0.v.nz;
     double* st = (double*)stage(*c, 3 * n * sizeof(double));
@@ -2062,8 +2169,11 @@ TOS_EXPORT int tos_vcycle(tos_ctx* c, const double* r, double* z, int precision,
 
 TOS_EXPORT int tos_solve(tos_ctx* c, const tos_solve_opts* o, tos_solve_stats* st) {
   return guard(c, [&]() {
-    REQUIRE(o && o->rtol > 0 && o->maxit >= 0 && o->pre >= 0 && o->post >= 0, "bad options");
+    REQUIRE(o && o->rtol > 0 && o->maxit >= 0 && o->pre >= 0 && o->post >= 0 && o->precision >= 0 &&
+                o->precision <= 2,
+            "bad options");
     if (!c->op_ready) { c->err = "update_operator first"; return TOS_E_STATE; }
+    select_stencils(*c, o->precision);
     const auto t0 = std::chrono::steady_clock::now();
     int rc_all = TOS_OK;
     for (int k = 0; k < c->K; ++k) {
@@ -2626,7 +2736,7 @@ TOS_EXPORT int tos_set_graph_replay(tos_ctx* c, int enable) {
 
 TOS_EXPORT int tos_set_sweep_mode(tos_ctx* c, int mode) {
   return guard(c, [&]() {
-    REQUIRE(mode == 0 || mode == 1, "bad sweep mode");
+    REQUIRE(mode >= 0 && mode <= 2, "bad sweep mode");
     c->sweep_mode = mode;
     return TOS_OK;
   });

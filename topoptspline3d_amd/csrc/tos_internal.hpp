@@ -74,6 +74,8 @@ struct Level {
   uint8_t* fre = nullptr;        // device, padded nodes
   double* st64 = nullptr;        // device stencil (levels >= 1)
   float* st32 = nullptr;
+  uint32_t* st16 = nullptr;     // FP16 copy of st32 (122 words per slot) and its per-slot scale
+  float* st_m = nullptr;
   double *u64 = nullptr, *f64 = nullptr, *r64 = nullptr;
   float *u32 = nullptr, *f32 = nullptr, *r32 = nullptr;
   // chunk lists, int4 {padded index of first node, x0 | count << 16, y, z}: `lin` holds
@@ -88,6 +90,12 @@ struct Level {
   // = first chunk (in the colour-ordered list) of colour c with z >= s * wf_T
   int wf_T = 0, wf_S = 0;
   std::vector<int> slab_off;    // 8 x (wf_S + 1)
+  // grid rows of the (y%2, z%2) classes for the two-colour GS launches (fine level):
+  // fwd {first chunk of the even-x colour, count, first chunk of the odd-x colour, count},
+  // bwd the same with the two colours swapped
+  int4* pair_fwd[4] = {nullptr, nullptr, nullptr, nullptr};
+  int4* pair_bwd[4] = {nullptr, nullptr, nullptr, nullptr};
+  int n_pair[4] = {0, 0, 0, 0};
   int64_t NS = 0;                // stencil slots per entry = 64 * n_col
   int* slot = nullptr;           // padded node -> stencil slot, -1 if not free
   std::vector<int> h_slot;
@@ -173,7 +181,9 @@ struct Ctx {
   size_t ev_used = 0;
   LearnState* learn = nullptr;
   BatchState* batch = nullptr;
-  int sweep_mode = 0;           // fine GS: 0 colour passes, 1 colour-slab wavefront
+  bool half_st = false;         // the running FP32 cycle reads the FP16 stencils
+  bool st16_valid = false;      // FP16 stencils match the current operator
+  int sweep_mode = 0;           // fine GS: 0 colour passes, 1 colour-slab wavefront, 2 row pairs
   bool graph_replay = true;     // replay the captured FCG iteration body
   struct SolveGraph {           // captured FCG iteration body per solve configuration
     int precision, pre, post, sweep;
