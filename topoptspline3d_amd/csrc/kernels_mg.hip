@@ -112,8 +112,8 @@ __device__ __forceinline__ void load_weights(const LevelView& L, const T* __rest
 // loads with FMAs to save registers and serialise the latencies)
 #define TOS_LOADS_DONE() __builtin_amdgcn_sched_barrier(0)
 
-template <class T, bool SELF, unsigned MASK = 0x7ffffffu>
-__device__ __forceinline__ void fine_row(const T (&wa)[8], const T* __restrict__ Kc, const T (&un)[27][3],
+template <class T, bool SELF, unsigned MASK = 0x7ffffffu, class TU = T>
+__device__ __forceinline__ void fine_row(const T (&wa)[8], const T* __restrict__ Kc, const TU (&un)[27][3],
                                          T (&acc)[3], T (&D)[9]) {
 #pragma unroll
   for (int a = 0; a < 8; ++a) {
@@ -129,7 +129,7 @@ __device__ __forceinline__ void fine_row(const T (&wa)[8], const T* __restrict__
       if (!((MASK >> nb) & 1u)) continue;
       const int d = li ^ j;
       // neighbour components reflected into the frame of local node 0
-      const T v0 = sx * un[nb][0], v1 = sy * un[nb][1], v2 = sz * un[nb][2];
+      const T v0 = sx * T(un[nb][0]), v1 = sy * T(un[nb][1]), v2 = sz * T(un[nb][2]);
       t0 = fma(Kc[ke_key(d, 0, 0)], v0, t0); t0 = fma(Kc[ke_key(d, 0, 1)], v1, t0);
       t0 = fma(Kc[ke_key(d, 0, 2)], v2, t0);
       t1 = fma(Kc[ke_key(d, 1, 0)], v0, t1); t1 = fma(Kc[ke_key(d, 1, 1)], v1, t1);
@@ -152,22 +152,24 @@ __device__ __forceinline__ void fine_row(const T (&wa)[8], const T* __restrict__
 
 // MODE 0: y = K u, optional partial sums of u.y;  MODE 1: y = f - K u, optional float
 // copy y32.  Only free nodes are touched (the others stay 0 in every vector).
-// FP32: all 81 neighbour values live at once, register budget sized for 4 waves per SIMD.
-// FP64 keeps the compiler's interleaved schedule (twice the registers would halve the
-// occupancy, which measured slower).
+// TU: type of u (float u with FP64 arithmetic: the FCG search direction of the mixed
+// precision solves).  FP32 arithmetic: all 81 neighbour values live at once, register
+// budget sized for 4 waves per SIMD.  FP64 keeps the compiler's interleaved schedule
+// (twice the registers would halve the occupancy, which measured slower).
 #define TOS_FINE_BOUNDS(T) __launch_bounds__(256, sizeof(T) == 4 ? 4 : 1)
 
-template <class T, int MODE>
+template <class T, int MODE, class TU = T>
 __global__ TOS_FINE_BOUNDS(T) void k_fine_apply(LevelView L, const int4* __restrict__ cl, int nch,
-                                                const T* __restrict__ w, const T* __restrict__ Ke,
-                                                const T* __restrict__ u, const T* __restrict__ f,
-                                                T* __restrict__ y, double* __restrict__ partial,
-                                                float* __restrict__ y32) {
+                                                 const T* __restrict__ w, const T* __restrict__ Ke,
+                                                 const TU* __restrict__ u, const T* __restrict__ f,
+                                                 T* __restrict__ y, double* __restrict__ partial,
+                                                 float* __restrict__ y32) {
   NodeRef n;
   const bool ok = chunk_node(cl, nch, 1, n);
   double d = 0;
   if (ok) {
-    T un[27][3], wa[8], fv[3] = {0, 0, 0};
+    TU un[27][3];
+    T wa[8], fv[3] = {0, 0, 0};
     load_nbhd(L, u, n.p, un);
     load_weights(L, w, n.x, n.y, n.z, wa);
     if (MODE == 1)
@@ -175,7 +177,7 @@ __global__ TOS_FINE_BOUNDS(T) void k_fine_apply(LevelView L, const int4* __restr
       for (int c = 0; c < 3; ++c) fv[c] = f[c * L.NP + n.p];
     if constexpr (sizeof(T) == 4) TOS_LOADS_DONE();
     T acc[3] = {0, 0, 0}, D[9];
-    fine_row<T, true>(wa, Ke, un, acc, D);
+    fine_row<T, true, 0x7ffffffu, TU>(wa, Ke, un, acc, D);
     if (MODE == 0) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
@@ -888,9 +890,11 @@ __global__ __launch_bounds__(256) void k_coarse_solve(LevelView L, const int64_t
 // PCG vector kernels over the fine level's chunk list (FP64, fused with their reductions)
 // ---------------------------------------------------------------------------------------
 // u += alpha p ; r -= alpha q ; partial ||r||^2 ; optional float copy of r
+// (p in FP64, or FP32 in the mixed-precision solves)
+template <class TP>
 __global__ __launch_bounds__(256) void k_pcg_ur(const int4* __restrict__ cl, int nch, int64_t NP,
                                                 double* __restrict__ u, double* __restrict__ r,
-                                                const double* __restrict__ p, const double* __restrict__ q,
+                                                const TP* __restrict__ p, const double* __restrict__ q,
                                                 const double* __restrict__ scal, float* __restrict__ r32,
                                                 double* __restrict__ partial) {
   NodeRef n;
@@ -901,7 +905,7 @@ __global__ __launch_bounds__(256) void k_pcg_ur(const int4* __restrict__ cl, int
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const int64_t i = c * NP + n.p;
-      u[i] += al * p[i];
+      u[i] += al * (double)p[i];
       const double rv = r[i] - al * q[i];
       r[i] = rv;
       if (r32) r32[i] = (float)rv;
@@ -932,10 +936,10 @@ __global__ __launch_bounds__(256) void k_pcg_zdots(const int4* __restrict__ cl, 
   block_partial(zq, partial, 1);
 }
 
-// p = z + beta p  (first != 0: p = z)
-template <class TZ>
+// p = z + beta p  (first != 0: p = z), formed in FP64 and stored as TP
+template <class TZ, class TP>
 __global__ __launch_bounds__(256) void k_pcg_p(const int4* __restrict__ cl, int nch, int64_t NP,
-                                               double* __restrict__ p, const TZ* __restrict__ z,
+                                               TP* __restrict__ p, const TZ* __restrict__ z,
                                                const double* __restrict__ scal, int first) {
   NodeRef n;
   if (!chunk_node(cl, nch, 1, n)) return;
@@ -943,7 +947,7 @@ __global__ __launch_bounds__(256) void k_pcg_p(const int4* __restrict__ cl, int 
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     const int64_t i = c * NP + n.p;
-    p[i] = first ? (double)z[i] : (double)z[i] + be * p[i];
+    p[i] = (TP)(first ? (double)z[i] : (double)z[i] + be * (double)p[i]);
   }
 }
 

@@ -663,6 +663,18 @@ void apply_level(Ctx& c, int l, const T* u, const T* f, T* y, double* partial = 
   LAUNCH_CHECK();
 }
 
+// q = K p on the fine level for an FP32 search direction p (FP64 arithmetic), with the
+// partial sums of p.q
+void apply_fine_dir(Ctx& c, const float* p, double* q, double* partial) {
+  Level& L = c.L[0];
+  Prof pr(c, P_FINE_APPLY64);
+  if (L.n_lin)
+    hipLaunchKernelGGL((k_fine_apply<double, 0, float>), dim3(cblk(L.n_lin)), dim3(256), 0, c.stream, L.v,
+                       (const int4*)L.cl_lin, L.n_lin, Prec<double>::w(c), Prec<double>::Ke(c), p,
+                       (const double*)nullptr, q, partial, (float*)nullptr);
+  LAUNCH_CHECK();
+}
+
 // r = f - A u right after one forward sweep from u = 0 (colour by colour, later-colour
 // neighbours only; see k_fine_resid_z)
 template <class T>
@@ -1388,17 +1400,21 @@ int solve_case(Ctx& c, int k, const tos_solve_opts& o, int* its, double* relres)
       if (!std::isfinite(rn)) { c.err = "NaN/Inf in the V-cycle iteration"; return TOS_E_NAN; }
     }
   } else if (rn / nf > o.rtol) {
-    // flexible CG, Polak-Ribiere beta = -alpha <z, q> / <z_old, r_old>
+    // flexible CG, Polak-Ribiere beta = -alpha <z, q> / <z_old, r_old>.  With an FP32
+    // V-cycle the search direction p is stored in FP32 (c.p32): q = K p, u += alpha p and
+    // r -= alpha q stay FP64 operations on the stored p, so the recursive residual keeps
+    // tracking f - K u; only p's rounding changes the direction.
+    float* p32 = c.p32;
     precondition(c, o.precision, o.pre, o.post, f32, false);
     {
       Prof pr(c, P_VECTOR);
       if (f32) {
-        hipLaunchKernelGGL((k_pcg_p<float>), dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, c.p,
+        hipLaunchKernelGGL((k_pcg_p<float, float>), dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, p32,
                            (const float*)L0.u32, (const double*)c.scal, 1);
         hipLaunchKernelGGL((k_pcg_zdots<float>), dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, (const float*)L0.u32,
                            (const double*)c.r, (const double*)nullptr, c.partial);
       } else {
-        hipLaunchKernelGGL((k_pcg_p<double>), dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, c.p,
+        hipLaunchKernelGGL((k_pcg_p<double, double>), dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, c.p,
                            (const double*)c.z, (const double*)c.scal, 1);
         hipLaunchKernelGGL((k_pcg_zdots<double>), dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, (const double*)c.z,
                            (const double*)c.r, (const double*)nullptr, c.partial);
@@ -1407,12 +1423,20 @@ int solve_case(Ctx& c, int k, const tos_solve_opts& o, int* its, double* relres)
     }
     // q = K p, alpha, u += alpha p, r -= alpha q, |r|^2
     auto tail = [&]() {
-      apply_level<double, 0>(c, 0, c.p, nullptr, c.q, c.partial);  // partials of p.q
+      if (f32)
+        apply_fine_dir(c, p32, c.q, c.partial);
+      else
+        apply_level<double, 0>(c, 0, c.p, nullptr, c.q, c.partial);  // partials of p.q
       {
         Prof pr(c, P_VECTOR);
         finish0(c, 0, S_PQ, 0, OP_ALPHA);
-        hipLaunchKernelGGL(k_pcg_ur, dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, u, c.r, (const double*)c.p,
-                           (const double*)c.q, (const double*)c.scal, f32 ? L0.f32 : (float*)nullptr, c.partial);
+        if (f32)
+          hipLaunchKernelGGL((k_pcg_ur<float>), dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, u, c.r,
+                             (const float*)p32, (const double*)c.q, (const double*)c.scal, L0.f32, c.partial);
+        else
+          hipLaunchKernelGGL((k_pcg_ur<double>), dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, u, c.r,
+                             (const double*)c.p, (const double*)c.q, (const double*)c.scal, (float*)nullptr,
+                             c.partial);
         finish0(c, 0, S_RN2, 0, OP_NONE);
       }
       LAUNCH_CHECK();
@@ -1425,13 +1449,13 @@ int solve_case(Ctx& c, int k, const tos_solve_opts& o, int* its, double* relres)
         hipLaunchKernelGGL((k_pcg_zdots<float>), dim3(nb), dim3(256), 0, c.stream, cl, nch, NP,
                            (const float*)L0.u32, (const double*)c.r, (const double*)c.q, c.partial);
         finish0(c, 1, S_ZR, S_ZQ, OP_BETA);
-        hipLaunchKernelGGL((k_pcg_p<float>), dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, c.p,
+        hipLaunchKernelGGL((k_pcg_p<float, float>), dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, p32,
                            (const float*)L0.u32, (const double*)c.scal, 0);
       } else {
         hipLaunchKernelGGL((k_pcg_zdots<double>), dim3(nb), dim3(256), 0, c.stream, cl, nch, NP,
                            (const double*)c.z, (const double*)c.r, (const double*)c.q, c.partial);
         finish0(c, 1, S_ZR, S_ZQ, OP_BETA);
-        hipLaunchKernelGGL((k_pcg_p<double>), dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, c.p,
+        hipLaunchKernelGGL((k_pcg_p<double, double>), dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, c.p,
                            (const double*)c.z, (const double*)c.scal, 0);
       }
       LAUNCH_CHECK();
@@ -1861,6 +1885,7 @@ TOS_EXPORT int tos_ctx_create(const tos_grid_desc* d, tos_ctx** out) {
     c->r = dalloc<double>(*c, 3 * v0.NP);
     c->z = dalloc<double>(*c, 3 * v0.NP);
     c->p = dalloc<double>(*c, 3 * v0.NP);
+    c->p32 = dalloc<float>(*c, 3 * v0.NP);
     c->q = dalloc<double>(*c, 3 * v0.NP);
     c->nb0 = (int)cblk(c->L[0].n_lin);
     c->partial = dalloc<double>(*c, 2 * (size_t)std::max(kRedBlocks, c->nb0) + 2 * kStage);
