@@ -516,24 +516,34 @@ __global__ __launch_bounds__(256) void k_restrict(LevelView F, LevelView C, cons
   fc[n.p] = a0; fc[C.NP + n.p] = a1; fc[2 * C.NP + n.p] = a2;
 }
 
+// u_f += P u_c: a fine node takes the trilinear interpolation of the (up to 8) coarse
+// nodes around it.  All 8 corner nodes (X..X+1, Y..Y+1, Z..Z+1, within the padding) are
+// read with weight 0 where a fine coordinate is even, so the loads are issued together
+// instead of as a divergent loop.
 template <class T>
 __global__ __launch_bounds__(256) void k_prolong_add(LevelView F, LevelView C, const int4* __restrict__ clf, int nch,
                                                      const T* __restrict__ uc, T* __restrict__ uf) {
   NodeRef n;
   if (!chunk_node(clf, nch, 1, n)) return;
   const int x = n.x, y = n.y, z = n.z;
-  const int nxp = (x & 1) + 1, nyp = (y & 1) + 1, nzp = (z & 1) + 1;
-  const T wx = (x & 1) ? T(0.5) : T(1), wy = (y & 1) ? T(0.5) : T(1), wz = (z & 1) ? T(0.5) : T(1);
+  const T hx = (x & 1) ? T(0.5) : T(0), hy = (y & 1) ? T(0.5) : T(0), hz = (z & 1) ? T(0.5) : T(0);
+  const T wx[2] = {T(1) - hx, hx}, wy[2] = {T(1) - hy, hy}, wz[2] = {T(1) - hz, hz};
   const int64_t I0 = C.pnode(x >> 1, y >> 1, z >> 1);
   T a0 = 0, a1 = 0, a2 = 0;
-  for (int k = 0; k < nzp; ++k)
-    for (int j = 0; j < nyp; ++j)
-      for (int i = 0; i < nxp; ++i) {
-        const int64_t I = I0 + i + (int64_t)C.NX * (j + (int64_t)C.NY * k);
-        const T wgt = wx * wy * wz;
-        a0 += wgt * uc[I]; a1 += wgt * uc[C.NP + I]; a2 += wgt * uc[2 * C.NP + I];
-      }
-  uf[n.p] += a0; uf[F.NP + n.p] += a1; uf[2 * F.NP + n.p] += a2;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int i = k & 1, j = (k >> 1) & 1, l = k >> 2;
+    const T wgt = wx[i] * wy[j] * wz[l];
+    const int64_t I = I0 + i + (int64_t)C.NX * (j + (int64_t)C.NY * l);
+    a0 = fma(wgt, uc[I], a0);
+    a1 = fma(wgt, uc[C.NP + I], a1);
+    a2 = fma(wgt, uc[2 * C.NP + I], a2);
+  }
+  const int64_t p = n.p;
+  const T f0 = uf[p], f1 = uf[F.NP + p], f2 = uf[2 * F.NP + p];
+  uf[p] = f0 + a0;
+  uf[F.NP + p] = f1 + a1;
+  uf[2 * F.NP + p] = f2 + a2;
 }
 
 template <class T>
@@ -901,15 +911,23 @@ __global__ __launch_bounds__(256) void k_pcg_ur(const int4* __restrict__ cl, int
   const bool ok = chunk_node(cl, nch, 1, n);
   double s = 0;
   if (ok) {
+    // every load before the first store: the stores to u and r could alias the later
+    // components' loads as far as the compiler knows, which would serialise them
     const double al = scal[S_ALPHA];
+    double uv[3], pv[3], rv[3], qv[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const int64_t i = c * NP + n.p;
-      u[i] += al * (double)p[i];
-      const double rv = r[i] - al * q[i];
-      r[i] = rv;
-      if (r32) r32[i] = (float)rv;
-      s += rv * rv;
+      uv[c] = u[i]; pv[c] = (double)p[i]; rv[c] = r[i]; qv[c] = q[i];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int64_t i = c * NP + n.p;
+      const double rn = rv[c] - al * qv[c];
+      u[i] = uv[c] + al * pv[c];
+      r[i] = rn;
+      if (r32) r32[i] = (float)rn;
+      s += rn * rn;
     }
   }
   block_partial(s, partial, 0);
@@ -924,12 +942,18 @@ __global__ __launch_bounds__(256) void k_pcg_zdots(const int4* __restrict__ cl, 
   const bool ok = chunk_node(cl, nch, 1, n);
   double zr = 0, zq = 0;
   if (ok) {
+    double zv[3], rv[3], qv[3] = {0, 0, 0};
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const int64_t i = c * NP + n.p;
-      const double zv = (double)z[i];
-      zr += zv * r[i];
-      if (q) zq += zv * q[i];
+      zv[c] = (double)z[i];
+      rv[c] = r[i];
+      if (q) qv[c] = q[i];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      zr += zv[c] * rv[c];
+      zq += zv[c] * qv[c];
     }
   }
   block_partial(zr, partial, 0);
@@ -944,11 +968,15 @@ __global__ __launch_bounds__(256) void k_pcg_p(const int4* __restrict__ cl, int 
   NodeRef n;
   if (!chunk_node(cl, nch, 1, n)) return;
   const double be = first ? 0.0 : scal[S_BETA];
+  double zv[3], pv[3] = {0, 0, 0};
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     const int64_t i = c * NP + n.p;
-    p[i] = (TP)(first ? (double)z[i] : (double)z[i] + be * (double)p[i]);
+    zv[c] = (double)z[i];
+    if (!first) pv[c] = (double)p[i];
   }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) p[c * NP + n.p] = (TP)(zv[c] + be * pv[c]);
 }
 
 // generic partials of a.b (and a2.b2 when a2 != null)

@@ -704,6 +704,8 @@ void resid_after_zero_sweep(Ctx& c, int l, const T* u, T* r) {
   LAUNCH_CHECK();
 }
 
+constexpr int64_t kResidZMinNodes = 1 << 20;
+
 void learned_step(Ctx& c, int l, const float* f, float* u, int pre, int post);
 bool learned_here(const Ctx& c, int l);
 
@@ -731,7 +733,9 @@ void vcycle_level(Ctx& c, int l, const T* f, T* u, int pre, int post) {
                        L.n_lin, L.v.NP, u);
   for (int s = 0; s < pre; ++s) gs_sweep<T>(c, l, u, f, false, s == 0);
   T* r = Prec<T>::r(L);
-  if (pre == 1)  // (the wavefront schedule performs the same updates from a zeroed u)
+  // after a single sweep from zero the residual needs only the later-colour half of each
+  // neighbourhood, but costs 8 launches: worth it on the large levels only
+  if (pre == 1 && L.n_free >= kResidZMinNodes)
     resid_after_zero_sweep<T>(c, l, u, r);
   else
     apply_level<T, 1>(c, l, u, f, r);
