@@ -1686,7 +1686,7 @@ ORC_API int orc_batch_tuning(tos_ctx* c, char* buf, int len) {
 // The oracle always relaxes colour by colour (the GPU's wavefront schedule performs the
 // same updates).
 ORC_API int orc_set_sweep_mode(tos_ctx* c, int mode) {
-  if (!c || mode < 0 || mode > 2) return TOS_E_ARG;
+  if (!c || mode < 0 || mode > 3) return TOS_E_ARG;
   return TOS_OK;
 }
 

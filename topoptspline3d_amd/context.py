@@ -165,7 +165,8 @@ class Context:
                 "rel_res": [st.rel_res[k] for k in range(self.K)], "seconds": st.seconds}
 
     def set_sweep_mode(self, mode):
-        """fine GS schedule: 0 colour passes, 1 colour-slab wavefront, 2 row pairs (same updates)"""
+        """fine GS schedule: 0 colour passes, 1 colour-slab wavefront, 2 row pairs, 3 in-wave pairs
+        (FP32 cycles); all perform the same updates"""
         self._chk(self.api.set_sweep_mode(self.h, int(mode)), "set_sweep_mode")
 
     def set_graph_replay(self, enable):

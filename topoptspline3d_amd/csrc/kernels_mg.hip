@@ -198,6 +198,21 @@ __global__ TOS_FINE_BOUNDS(T) void k_fine_apply(LevelView L, const int4* __restr
 
 // Gauss-Seidel colour pass on the fine level.  BACKWARD relaxes components z, y, x.
 // ZC: see load_nbhd (-1 = every neighbour, c = first forward sweep from u = 0 at colour c)
+// point Gauss-Seidel on the node's 3x3 diagonal block: components x, y, z (BACKWARD:
+// z, y, x), each with the others' latest values; b = f - off-diagonal sum
+template <class T, bool BACKWARD>
+__device__ __forceinline__ void block_relax(T b0, T b1, T b2, const T (&D)[9], T& v0, T& v1, T& v2) {
+  if (!BACKWARD) {
+    v0 = (b0 - D[1] * v1 - D[2] * v2) / D[0];
+    v1 = (b1 - D[3] * v0 - D[5] * v2) / D[4];
+    v2 = (b2 - D[6] * v0 - D[7] * v1) / D[8];
+  } else {
+    v2 = (b2 - D[6] * v0 - D[7] * v1) / D[8];
+    v1 = (b1 - D[3] * v0 - D[5] * v2) / D[4];
+    v0 = (b0 - D[1] * v1 - D[2] * v2) / D[0];
+  }
+}
+
 // relax node n (one colour pass body)
 template <class T, bool BACKWARD, int ZC>
 __device__ __forceinline__ void relax_fine(const LevelView& L, const T* __restrict__ w, const T* __restrict__ Ke,
@@ -262,6 +277,140 @@ __global__ TOS_FINE_BOUNDS(T) void k_fine_gs_pair(LevelView L, const int4* __res
       }
     }
   }
+}
+
+// Two colours of one sweep in one pass, paired inside a wave.  Colours c1 (first in the
+// sweep order) and c2 = c1 ^ 1 share the y, z parities, so a c2 node depends on c1 only
+// through its two x-neighbours in the same grid row.  A wave takes a row segment: lane l
+// relaxes the c1 node x1 = xs + 2l and then the c2 node x1 + 1 (l < 63), whose c1
+// neighbours are lanes l and l + 1 -- their new values, and lane l+1's neighbour column,
+// arrive by lane shuffles instead of a second pass over memory.  Segments of a row start
+// 126 apart, so the c1 node of lane 63 is also lane 0 of the next segment (computed twice
+// from the same inputs, written once).  Those two computations read the old values of
+// the c2 nodes of lanes 0 and 62, so the new values of these two c2 nodes go to `ub` and
+// are copied into u by k_fine_pairw_copy after the pass.  Same updates as two colour
+// passes (bit-identical); each neighbourhood is streamed once for both colours.
+// seg = {padded index of (xs, y, z), (xs + 1) | (xe + 1) << 15 | cont << 30, y, z}: xe the
+// row's last free node, cont = 1 when lane 0 repeats the previous segment's lane 63.
+// ZA / ZB: -1, or the colours of a first forward sweep from zero (see load_nbhd).
+template <class T, bool BACKWARD, int ZA, int ZB>
+__global__ TOS_FINE_BOUNDS(T) void k_fine_gs_pairw(LevelView L, const int4* __restrict__ seg, int nseg,
+                                                   const uint8_t* __restrict__ fre, const T* __restrict__ w,
+                                                   const T* __restrict__ Kc, T* __restrict__ u,
+                                                   const T* __restrict__ f, T* __restrict__ ub) {
+  constexpr unsigned MA = nb_mask_of(ZA), MB = nb_mask_of(ZB);
+  const int si = (int)xcd_block() * 4 + (threadIdx.x >> 6);
+  if (si >= nseg) return;  // whole wave
+  const int lane = threadIdx.x & 63;
+  const int4 S = seg[si];
+  const int xs = (S.y & 0x7fff) - 1, xe = ((S.y >> 15) & 0x7fff) - 1;
+  const bool cont = (S.y >> 30) & 1;
+  const int x1 = xs + 2 * lane, y = S.z, z = S.w;
+  const int64_t p1 = (int64_t)S.x + 2 * lane;
+  const bool live = x1 <= xe + 1;
+  bool f1 = false, f2 = false;
+  if (live) {
+    f1 = fre[p1] != 0;  // (x1 = -1 is padding: never free)
+    f2 = lane < 63 && x1 + 1 <= xe && fre[p1 + 1] != 0;
+  }
+  // c1's neighbourhood (k = (dx+1) + 3 (dy+1) + 9 (dz+1)); column dx = -1 serves c1 only,
+  // columns 0 and +1 are also c2's columns -1 and 0, and column 0 is lane l-1's column +1
+  T un[27][3];
+#pragma unroll
+  for (int k = 0; k < 27; ++k) {
+    const int dx = k % 3 - 1;
+    const bool need = dx < 0 ? ((MA >> k) & 1u) != 0
+                              : (((MA >> k) & 1u) || ((MB >> (k - 1)) & 1u) ||
+                                 (dx == 0 && ((MB >> (k + 1)) & 1u)));
+    const bool ld = need && (dx < 0 ? f1 : live);
+    const int64_t q = p1 + dx + (int64_t)L.NX * ((k / 3) % 3 - 1 + (int64_t)L.NY * (k / 9 - 1));
+#pragma unroll
+    for (int c = 0; c < 3; ++c) un[k][c] = ld ? u[c * L.NP + q] : T(0);
+  }
+  T wa[8];  // c1's elements (x1 - 1 + ax, y - 1 + ay, z - 1 + az), a = ax + 2 ay + 4 az
+  {
+    const int64_t e0 = L.pelem(x1 - 1, y - 1, z - 1);
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      const bool ld = (a & 1) ? (f1 || f2) : (live && x1 >= 0);
+      wa[a] = ld ? w[e0 + (a & 1) + (int64_t)L.EX * (((a >> 1) & 1) + (int64_t)L.EY * (a >> 2))] : T(0);
+    }
+  }
+  T fa[3] = {0, 0, 0}, fb[3] = {0, 0, 0};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    if (f1) fa[c] = f[c * L.NP + p1];
+    if (f2) fb[c] = f[c * L.NP + p1 + 1];
+  }
+  if constexpr (sizeof(T) == 4) TOS_LOADS_DONE();
+  // c1 (from zero: its own start value is 0; un[13] may hold the old value, loaded for c2)
+  T v0 = ZA < 0 ? un[13][0] : T(0), v1 = ZA < 0 ? un[13][1] : T(0), v2 = ZA < 0 ? un[13][2] : T(0);
+  if (f1) {
+    T s[3] = {0, 0, 0};
+    T D[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    fine_row<T, false, MA>(wa, Kc, un, s, D);
+    block_relax<T, BACKWARD>(fa[0] - s[0], fa[1] - s[1], fa[2] - s[2], D, v0, v1, v2);
+  }
+  // c2's neighbourhood: own columns 0, +1 and lane l+1's column 0, with the new c1 values
+  T vn[27][3];
+#pragma unroll
+  for (int k = 0; k < 27; ++k) {
+    const int dx = k % 3 - 1;
+    if (dx < 0) {  // dx = -1: own column 0 (centre row: the new c1 value, set below)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) vn[k][c] = un[k + 1][c];
+    } else if (dx == 0) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) vn[k][c] = un[k + 1][c];
+    }
+  }
+  vn[12][0] = v0; vn[12][1] = v1; vn[12][2] = v2;
+#pragma unroll
+  for (int k = 2; k < 27; k += 3) {  // dx = +1: lane l+1's column 0
+    const int k0 = k - 1;             // its index in the neighbour lane's c1 frame
+    const bool need = ((MB >> k) & 1u) != 0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const T mine = k0 == 13 ? (c == 0 ? v0 : (c == 1 ? v1 : v2)) : un[k0][c];
+      vn[k][c] = need ? __shfl_down(mine, 1, 64) : T(0);
+    }
+  }
+  T wb[8];  // c2's elements: column x1 (own ax = 1) and x1 + 1 (lane l+1's ax = 0)
+#pragma unroll
+  for (int a = 0; a < 8; a += 2) {
+    wb[a] = wa[a + 1];
+    wb[a + 1] = __shfl_down(wa[a], 1, 64);
+  }
+  T t0 = ZB < 0 ? vn[13][0] : T(0), t1 = ZB < 0 ? vn[13][1] : T(0), t2 = ZB < 0 ? vn[13][2] : T(0);
+  if (f2) {
+    T s[3] = {0, 0, 0};
+    T D[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    fine_row<T, false, MB>(wb, Kc, vn, s, D);
+    block_relax<T, BACKWARD>(fb[0] - s[0], fb[1] - s[1], fb[2] - s[2], D, t0, t1, t2);
+  }
+  if (f1 && !(cont && lane == 0)) {
+    u[p1] = v0; u[L.NP + p1] = v1; u[2 * L.NP + p1] = v2;
+  }
+  if (f2) {
+    T* dst = (lane == 0 || lane == 62) ? ub : u;
+    dst[p1 + 1] = t0; dst[L.NP + p1 + 1] = t1; dst[2 * L.NP + p1 + 1] = t2;
+  }
+}
+
+// the deferred c2 nodes of k_fine_gs_pairw (lanes 0 and 62 of every segment): ub -> u
+template <class T>
+__global__ __launch_bounds__(256) void k_fine_pairw_copy(LevelView L, const int4* __restrict__ seg, int nseg,
+                                                         const uint8_t* __restrict__ fre, const T* __restrict__ ub,
+                                                         T* __restrict__ u) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 2 * (int64_t)nseg) return;
+  const int4 S = seg[t >> 1];
+  const int xs = (S.y & 0x7fff) - 1, xe = ((S.y >> 15) & 0x7fff) - 1;
+  const int dxn = (t & 1) ? 125 : 1;
+  if (xs + dxn > xe) return;
+  const int64_t p = (int64_t)S.x + dxn;
+  if (!fre[p]) return;
+  u[p] = ub[p]; u[L.NP + p] = ub[L.NP + p]; u[2 * L.NP + p] = ub[2 * L.NP + p];
 }
 
 template <class T, bool BACKWARD, int ZC = -1>

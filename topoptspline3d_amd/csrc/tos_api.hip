@@ -330,6 +330,37 @@ void build_chunks(Ctx& c, Level& L, bool with_slots) {
       }
     }
   }
+  // row segments of the in-wave two-colour GS (fine level): per row of class (y%2, z%2),
+  // the free nodes' x range [xmin, xe], first-colour parity P; segments start at xs (xmin
+  // or xmin - 1, of parity P) and every 126 nodes after it until the last c2 node is covered
+  if (!with_slots) {
+    if (v.nx + 1 >= 0x7fff) throw ArgError("grid too wide for the row segments (nx < 32766)");
+    for (int cls = 0; cls < 4; ++cls)
+      for (int P = 0; P < 2; ++P) {
+        const int py = cls & 1, pz = cls >> 1;
+        std::vector<int4> sg;
+        for (int z = pz; z < v.nz; z += 2)
+          for (int y = py; y < v.ny; y += 2) {
+            int xmin = -1, xmax = -1;
+            for (int x = 0; x < v.nx; ++x)
+              if (freeat(x, y, z)) {
+                if (xmin < 0) xmin = x;
+                xmax = x;
+              }
+            if (xmin < 0) continue;
+            const int xs = ((xmin & 1) == P) ? xmin : xmin - 1;
+            const int nseg = (xmax - xs + 1 + 125) / 126;
+            for (int q = 0; q < nseg; ++q) {
+              const int x0 = xs + 126 * q;
+              sg.push_back(make_int4((int)v.pnode(x0, y, z), (x0 + 1) | ((xmax + 1) << 15) | ((q > 0 ? 1 : 0) << 30),
+                                     y, z));
+            }
+          }
+        L.n_segw[cls][P] = (int)sg.size();
+        L.segw[cls][P] = dalloc<int4>(c, std::max<size_t>(sg.size(), 1));
+        if (!sg.empty()) upload(c.stream, L.segw[cls][P], sg.data(), sg.size() * sizeof(int4));
+      }
+  }
   // colour-slab wavefront of the fine GS: ~32 z-slabs so that the slabs in flight during a
   // few consecutive steps fit in the memory-side cache
   if (!with_slots) {
@@ -589,10 +620,55 @@ void gs_sweep_pairs(Ctx& c, T* u, const T* f, bool backward, bool from_zero) {
 // One colour-ordered sweep.  from_zero: u is 0 on entry (the first pre-smoothing sweep of a
 // V-cycle level), so colour c reads only the neighbours of colours < c; same updates as
 // a sweep over a zeroed u, without zeroing it first or reading the zeros.
+// Fine-level FP32 sweep in four in-wave two-colour passes (k_fine_gs_pairw), each followed
+// by the copy of its deferred nodes; classes in colour order, forward (0,0) (1,0) (0,1)
+// (1,1) with the even-x colour first, backward the reverse with the odd-x colour first.
+void gs_sweep_pairw(Ctx& c, float* u, const float* f, bool backward, bool from_zero) {
+  Level& L = c.L[0];
+  const LevelView& v = L.v;
+  if (!L.ub32) L.ub32 = dalloc<float>(c, 3 * (size_t)v.NP);
+  const int P = backward ? 1 : 0;
+  for (int i = 0; i < 4; ++i) {
+    const int cls = backward ? 3 - i : i;
+    const int n = L.n_segw[cls][P];
+    if (!n) continue;
+    Prof pr(c, P_FINE_GS32);
+    const dim3 g((n + 3) / 4), b(256);
+    const int4* sg = L.segw[cls][P];
+    const uint8_t* fr = L.fre;
+    if (backward) {
+      hipLaunchKernelGGL((k_fine_gs_pairw<float, true, -1, -1>), g, b, 0, c.stream, v, sg, n, fr, c.w32, c.Kc32, u, f,
+                         L.ub32);
+    } else if (!from_zero) {
+      hipLaunchKernelGGL((k_fine_gs_pairw<float, false, -1, -1>), g, b, 0, c.stream, v, sg, n, fr, c.w32, c.Kc32, u,
+                         f, L.ub32);
+    } else {
+#define GO(A) hipLaunchKernelGGL((k_fine_gs_pairw<float, false, A, A + 1>), g, b, 0, c.stream, v, sg, n, fr, c.w32, \
+                                 c.Kc32, u, f, L.ub32)
+      switch (cls) {
+        case 0: GO(0); break;
+        case 1: GO(2); break;
+        case 2: GO(4); break;
+        default: GO(6); break;
+      }
+#undef GO
+    }
+    hipLaunchKernelGGL((k_fine_pairw_copy<float>), dim3(nblk(2 * (int64_t)n)), dim3(256), 0, c.stream, v, sg, n, fr,
+                       (const float*)L.ub32, u);
+  }
+  LAUNCH_CHECK();
+}
+
 template <class T>
 void gs_sweep(Ctx& c, int l, T* u, const T* f, bool backward, bool from_zero = false) {
   Level& L = c.L[l];
   const LevelView& v = L.v;
+  if constexpr (std::is_same<T, float>::value) {
+    if (l == 0 && c.sweep_mode == 3) {
+      gs_sweep_pairw(c, u, f, backward, from_zero);
+      return;
+    }
+  }
   if (l == 0 && c.sweep_mode == 2) {
     gs_sweep_pairs<T>(c, u, f, backward, from_zero);
     return;
@@ -2765,7 +2841,7 @@ TOS_EXPORT int tos_set_graph_replay(tos_ctx* c, int enable) {
 
 TOS_EXPORT int tos_set_sweep_mode(tos_ctx* c, int mode) {
   return guard(c, [&]() {
-    REQUIRE(mode >= 0 && mode <= 2, "bad sweep mode");
+    REQUIRE(mode >= 0 && mode <= 3, "bad sweep mode");
     c->sweep_mode = mode;
     return TOS_OK;
   });

@@ -96,6 +96,11 @@ struct Level {
   int4* pair_fwd[4] = {nullptr, nullptr, nullptr, nullptr};
   int4* pair_bwd[4] = {nullptr, nullptr, nullptr, nullptr};
   int n_pair[4] = {0, 0, 0, 0};
+  // row segments of the in-wave two-colour GS (k_fine_gs_pairw), per class and first-colour
+  // x parity (0: forward, 1: backward), and the buffer of its deferred nodes
+  int4* segw[4][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
+  int n_segw[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+  float* ub32 = nullptr;
   int64_t NS = 0;                // stencil slots per entry = 64 * n_col
   int* slot = nullptr;           // padded node -> stencil slot, -1 if not free
   std::vector<int> h_slot;
@@ -184,7 +189,8 @@ struct Ctx {
   BatchState* batch = nullptr;
   bool half_st = false;         // the running FP32 cycle reads the FP16 stencils
   bool st16_valid = false;      // FP16 stencils match the current operator
-  int sweep_mode = 0;           // fine GS: 0 colour passes, 1 colour-slab wavefront, 2 row pairs
+  int sweep_mode = 0;           // fine GS: 0 colour passes, 1 colour-slab wavefront, 2 row pairs,
+                                // 3 in-wave pairs (FP32 cycles)
   bool graph_replay = true;     // replay the captured FCG iteration body
   struct SolveGraph {           // captured FCG iteration body per solve configuration
     int precision, pre, post, sweep;
