@@ -334,8 +334,9 @@ def main():
                     help="optimisation iterations of the 16-load-case batched workload (0: skip)")
     ap.add_argument("--profile-steps", type=int, default=2,
                     help="untimed iterations with per-kernel HIP-event timing (roofline)")
-    ap.add_argument("--sweep-mode", type=int, default=0,
-                    help="fine GS schedule: 0 colour passes, 1 colour-slab wavefront, 2 row pairs")
+    ap.add_argument("--sweep-mode", type=int, default=3,
+                    help="fine GS schedule: 0 colour passes, 1 colour-slab wavefront, 2 row pairs, "
+                         "3 in-wave pairs (FP32 cycles)")
     ap.add_argument("--no-graph-replay", action="store_true",
                     help="launch the FCG iteration kernel by kernel (for launch-tracing tools)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,

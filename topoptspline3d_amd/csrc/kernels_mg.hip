@@ -308,11 +308,13 @@ __global__ TOS_FINE_BOUNDS(T) void k_fine_gs_pairw(LevelView L, const int4* __re
   const int x1 = xs + 2 * lane, y = S.z, z = S.w;
   const int64_t p1 = (int64_t)S.x + 2 * lane;
   const bool live = x1 <= xe + 1;
-  bool f1 = false, f2 = false;
-  if (live) {
-    f1 = fre[p1] != 0;  // (x1 = -1 is padding: never free)
-    f2 = lane < 63 && x1 + 1 <= xe && fre[p1 + 1] != 0;
-  }
+  // loads are unconditional, from addresses clamped into the row (lanes past the row read
+  // its end, the x - 1 column of x1 = -1 reads x = -1): a value read at a clamped address
+  // is never used, and no load needs a branch of its own
+  const int xa = live ? x1 : xe + 1;
+  const int64_t pa = (int64_t)S.x + (xa - xs);
+  const bool f1 = live && fre[pa] != 0;  // (x1 = -1 is padding: never free)
+  const bool f2 = live && lane < 63 && x1 + 1 <= xe && fre[pa + 1] != 0;
   // c1's neighbourhood (k = (dx+1) + 3 (dy+1) + 9 (dz+1)); column dx = -1 serves c1 only,
   // columns 0 and +1 are also c2's columns -1 and 0, and column 0 is lane l-1's column +1
   T un[27][3];
@@ -322,25 +324,24 @@ __global__ TOS_FINE_BOUNDS(T) void k_fine_gs_pairw(LevelView L, const int4* __re
     const bool need = dx < 0 ? ((MA >> k) & 1u) != 0
                               : (((MA >> k) & 1u) || ((MB >> (k - 1)) & 1u) ||
                                  (dx == 0 && ((MB >> (k + 1)) & 1u)));
-    const bool ld = need && (dx < 0 ? f1 : live);
-    const int64_t q = p1 + dx + (int64_t)L.NX * ((k / 3) % 3 - 1 + (int64_t)L.NY * (k / 9 - 1));
+    const int64_t q = (dx < 0 ? (xa >= 0 ? pa - 1 : pa) : pa + dx) +
+                      (int64_t)L.NX * ((k / 3) % 3 - 1 + (int64_t)L.NY * (k / 9 - 1));
 #pragma unroll
-    for (int c = 0; c < 3; ++c) un[k][c] = ld ? u[c * L.NP + q] : T(0);
+    for (int c = 0; c < 3; ++c) un[k][c] = need ? u[c * L.NP + q] : T(0);
   }
   T wa[8];  // c1's elements (x1 - 1 + ax, y - 1 + ay, z - 1 + az), a = ax + 2 ay + 4 az
   {
-    const int64_t e0 = L.pelem(x1 - 1, y - 1, z - 1);
+    const int64_t e0 = L.pelem(xa >= 0 ? xa - 1 : -1, y - 1, z - 1);
+    const int64_t e1 = L.pelem(xa <= L.ex ? xa : L.ex, y - 1, z - 1);
 #pragma unroll
-    for (int a = 0; a < 8; ++a) {
-      const bool ld = (a & 1) ? (f1 || f2) : (live && x1 >= 0);
-      wa[a] = ld ? w[e0 + (a & 1) + (int64_t)L.EX * (((a >> 1) & 1) + (int64_t)L.EY * (a >> 2))] : T(0);
-    }
+    for (int a = 0; a < 8; ++a)
+      wa[a] = w[((a & 1) ? e1 : e0) + (int64_t)L.EX * (((a >> 1) & 1) + (int64_t)L.EY * (a >> 2))];
   }
-  T fa[3] = {0, 0, 0}, fb[3] = {0, 0, 0};
+  T fa[3], fb[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    if (f1) fa[c] = f[c * L.NP + p1];
-    if (f2) fb[c] = f[c * L.NP + p1 + 1];
+    fa[c] = f[c * L.NP + pa];
+    fb[c] = f[c * L.NP + pa + 1];
   }
   if constexpr (sizeof(T) == 4) TOS_LOADS_DONE();
   // c1 (from zero: its own start value is 0; un[13] may hold the old value, loaded for c2)

@@ -189,7 +189,7 @@ struct Ctx {
   BatchState* batch = nullptr;
   bool half_st = false;         // the running FP32 cycle reads the FP16 stencils
   bool st16_valid = false;      // FP16 stencils match the current operator
-  int sweep_mode = 0;           // fine GS: 0 colour passes, 1 colour-slab wavefront, 2 row pairs,
+  int sweep_mode = 3;           // fine GS: 0 colour passes, 1 colour-slab wavefront, 2 row pairs,
                                 // 3 in-wave pairs (FP32 cycles)
   bool graph_replay = true;     // replay the captured FCG iteration body
   struct SolveGraph {           // captured FCG iteration body per solve configuration
