@@ -83,3 +83,28 @@ def test_graph_replay_matches_eager(hip):
             out.append((c.displacement().copy(), st["iterations"][0]))
         assert out[0][1] == out[1][1] == out[2][1]
         assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][0], out[2][0])
+
+
+def test_large_level_schedules_identical(hip):
+    """Levels of at least 2^20 free nodes form the residual after the zero-start sweep from
+    the later-colour neighbours only, per colour (mode 0) or per colour pair (mode 3):
+    V-cycles must agree bit for bit."""
+    nx, ny, nz = 128, 96, 96
+    rng = np.random.default_rng(11)
+    solid, support, f = cantilever(nx, ny, nz)
+    solid = (rng.random(solid.shape) > 0.05).astype(np.uint8)
+    solid[:, :, 0] = 1
+    c = Context(hip, solid, support)
+    assert c.info()["n_free_nodes"][0] >= 1 << 20
+    c.set_material(1e6, 0.4, 3.0, 1e-3)
+    c.set_loads(f)
+    c.set_density(rng.uniform(0.05, 1.0, solid.size).astype(np.float32))
+    c.update_operator()
+    r = rng.normal(size=3 * c.nn)
+    out = {}
+    for mode in (0, 3):
+        print(f"\n[schedule] large level, mode {mode}", flush=True)
+        c.set_sweep_mode(mode)
+        out[mode] = (c.vcycle(r, precision=1), c.vcycle(r, precision=2))
+    assert np.array_equal(out[0][0], out[3][0])
+    assert np.array_equal(out[0][1], out[3][1])

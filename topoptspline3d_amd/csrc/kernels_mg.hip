@@ -398,6 +398,89 @@ __global__ TOS_FINE_BOUNDS(T) void k_fine_gs_pairw(LevelView L, const int4* __re
   }
 }
 
+// k_fine_resid_z for the two colours c1 = ZA, c2 = ZB = ZA + 1 of a class in one pass, on
+// the forward segments of k_fine_gs_pairw: lane l forms the residual of the c1 node x1 and
+// of the c2 node x1 + 1 from one load of x1's neighbourhood plus lane l+1's neighbour
+// column.  u is only read, so the repeated c1 node of lane 0 is simply written twice with
+// the same value.
+template <class T, int ZA, int ZB>
+__global__ TOS_FINE_BOUNDS(T) void k_fine_resid_pairw(LevelView L, const int4* __restrict__ seg, int nseg,
+                                                      const uint8_t* __restrict__ fre, const T* __restrict__ w,
+                                                      const T* __restrict__ Kc, const T* __restrict__ u,
+                                                      T* __restrict__ r) {
+  constexpr unsigned MA = nb_mask_after(ZA), MB = nb_mask_after(ZB);
+  constexpr unsigned LA = MA | (1u << 13), LB = MB | (1u << 13);  // (own values too)
+  const int si = (int)xcd_block() * 4 + (threadIdx.x >> 6);
+  if (si >= nseg) return;  // whole wave
+  const int lane = threadIdx.x & 63;
+  const int4 S = seg[si];
+  const int xs = (S.y & 0x7fff) - 1, xe = ((S.y >> 15) & 0x7fff) - 1;
+  const int x1 = xs + 2 * lane, y = S.z, z = S.w;
+  const bool live = x1 <= xe + 1;
+  const int xa = live ? x1 : xe + 1;  // clamped as in k_fine_gs_pairw
+  const int64_t pa = (int64_t)S.x + (xa - xs);
+  const bool f1 = live && fre[pa] != 0;
+  const bool f2 = live && lane < 63 && x1 + 1 <= xe && fre[pa + 1] != 0;
+  T un[27][3];
+#pragma unroll
+  for (int k = 0; k < 27; ++k) {
+    const int dx = k % 3 - 1;
+    const bool need = dx < 0 ? ((LA >> k) & 1u) != 0
+                              : (((LA >> k) & 1u) || ((LB >> (k - 1)) & 1u) ||
+                                 (dx == 0 && ((LB >> (k + 1)) & 1u)));
+    const int64_t q = (dx < 0 ? (xa >= 0 ? pa - 1 : pa) : pa + dx) +
+                      (int64_t)L.NX * ((k / 3) % 3 - 1 + (int64_t)L.NY * (k / 9 - 1));
+#pragma unroll
+    for (int c = 0; c < 3; ++c) un[k][c] = need ? u[c * L.NP + q] : T(0);
+  }
+  T wa[8];
+  {
+    const int64_t e0 = L.pelem(xa >= 0 ? xa - 1 : -1, y - 1, z - 1);
+    const int64_t e1 = L.pelem(xa <= L.ex ? xa : L.ex, y - 1, z - 1);
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+      wa[a] = w[((a & 1) ? e1 : e0) + (int64_t)L.EX * (((a >> 1) & 1) + (int64_t)L.EY * (a >> 2))];
+  }
+  if constexpr (sizeof(T) == 4) TOS_LOADS_DONE();
+  if (f1) {
+    T s[3] = {0, 0, 0};
+    T D[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    fine_row<T, false, MA>(wa, Kc, un, s, D);
+    const T v1 = un[13][1], v2 = un[13][2];
+    r[pa] = -D[1] * v1 - D[2] * v2 - s[0];
+    r[L.NP + pa] = -D[5] * v2 - s[1];
+    r[2 * L.NP + pa] = -s[2];
+  }
+  T vn[27][3];
+#pragma unroll
+  for (int k = 0; k < 27; ++k) {
+    const int dx = k % 3 - 1;
+    if (dx <= 0) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) vn[k][c] = un[k + 1][c];
+    } else {
+      const bool need = ((LB >> k) & 1u) != 0;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) vn[k][c] = need ? __shfl_down(un[k - 1][c], 1, 64) : T(0);
+    }
+  }
+  T wb[8];
+#pragma unroll
+  for (int a = 0; a < 8; a += 2) {
+    wb[a] = wa[a + 1];
+    wb[a + 1] = __shfl_down(wa[a], 1, 64);
+  }
+  if (f2) {
+    T s[3] = {0, 0, 0};
+    T D[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    fine_row<T, false, MB>(wb, Kc, vn, s, D);
+    const T v1 = vn[13][1], v2 = vn[13][2];
+    r[pa + 1] = -D[1] * v1 - D[2] * v2 - s[0];
+    r[L.NP + pa + 1] = -D[5] * v2 - s[1];
+    r[2 * L.NP + pa + 1] = -s[2];
+  }
+}
+
 // the deferred c2 nodes of k_fine_gs_pairw (lanes 0 and 62 of every segment): ub -> u
 template <class T>
 __global__ __launch_bounds__(256) void k_fine_pairw_copy(LevelView L, const int4* __restrict__ seg, int nseg,

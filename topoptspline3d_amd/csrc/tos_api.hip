@@ -758,6 +758,26 @@ void resid_after_zero_sweep(Ctx& c, int l, const T* u, T* r) {
   Level& L = c.L[l];
   const LevelView& v = L.v;
   Prof pr(c, l > 0 ? P_ST_RESID : (sizeof(T) == 8 ? P_FINE_RESID64 : P_FINE_RESID32));
+  if constexpr (std::is_same<T, float>::value) {
+    if (l == 0 && c.sweep_mode == 3) {  // two colours per pass on the forward segments
+      for (int cls = 0; cls < 4; ++cls) {
+        const int n = L.n_segw[cls][0];
+        if (!n) continue;
+        const dim3 g((n + 3) / 4), b(256);
+#define GO(A) hipLaunchKernelGGL((k_fine_resid_pairw<float, A, A + 1>), g, b, 0, c.stream, v, (const int4*)L.segw[cls][0], \
+                                 n, (const uint8_t*)L.fre, (const float*)c.w32, (const float*)c.Kc32, u, r)
+        switch (cls) {
+          case 0: GO(0); break;
+          case 1: GO(2); break;
+          case 2: GO(4); break;
+          default: GO(6); break;
+        }
+#undef GO
+      }
+      LAUNCH_CHECK();
+      return;
+    }
+  }
   for (int col = 0; col < 8; ++col) {
     const int off = L.col_off[col], nch = L.col_off[col + 1] - off;
     if (nch == 0) continue;
