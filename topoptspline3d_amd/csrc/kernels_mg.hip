@@ -943,7 +943,9 @@ __global__ __launch_bounds__(256) void k_galerkin_next(LevelView F, LevelView C,
 // the masked one; the last pass drops coarse columns that are not free.  Every pass reads
 // each input stencil a few times instead of the ~13^3 / 27 times of the direct sum.
 // Input: the fine level's compact stencil (IN_CMP, slot map) or an intermediate box;
-// output: an intermediate box or the coarse level's compact stencil (OUT_CMP).
+// output: an intermediate box (only the nodes `omask` marks as read by the next pass:
+// the box spans the whole grid, the domain often a third of it) or the coarse level's
+// compact stencil (OUT_CMP).
 // Thread = (output node, output neighbour k = blockIdx.y).
 // ---------------------------------------------------------------------------------------
 struct GBox {
@@ -955,7 +957,8 @@ __global__ __launch_bounds__(256) void k_gal_axis(LevelView F, const int* __rest
                                                   const double* __restrict__ sin, int64_t NSI, GBox bi, GBox bo,
                                                   LevelView C, const int4* __restrict__ clc, int nch,
                                                   const uint8_t* __restrict__ cfre, double* __restrict__ sout,
-                                                  float* __restrict__ sout32, int64_t NSO) {
+                                                  float* __restrict__ sout32, int64_t NSO,
+                                                  const uint8_t* __restrict__ omask) {
   int o[3];
   int64_t oidx;
   const int k = blockIdx.y;
@@ -969,7 +972,7 @@ __global__ __launch_bounds__(256) void k_gal_axis(LevelView F, const int* __rest
     keep = cfre[nr.p + d[0] + (int64_t)C.NX * (d[1] + (int64_t)C.NY * d[2])] != 0;
   } else {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= (int64_t)bo.X * bo.Y * bo.Z) return;
+    if (v >= (int64_t)bo.X * bo.Y * bo.Z || !omask[v]) return;  // (entries no later pass reads)
     o[0] = (int)(v % bo.X);
     o[1] = (int)((v / bo.X) % bo.Y);
     o[2] = (int)(v / ((int64_t)bo.X * bo.Y));

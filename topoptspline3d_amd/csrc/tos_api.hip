@@ -2198,16 +2198,39 @@ void galerkin_by_axes(Ctx& c, Level& F, Level& C) {
   if (!C.gal_b1) {
     C.gal_b1 = dalloc<double>(c, 243 * n1);
     C.gal_b2 = dalloc<double>(c, 243 * n2);
+    // b2 (X, Y, z) is read for coarse free nodes (X, Y, Z), |z - 2Z| <= 1; b1 (X, y, z) for
+    // read b2 nodes (X, Y, z), |y - 2Y| <= 1
+    std::vector<uint8_t> m2((size_t)n2, 0), m1((size_t)n1, 0);
+    const LevelView& cv = C.v;
+    for (int Z = 0; Z < cv.nz; ++Z)
+      for (int Y = 0; Y < cv.ny; ++Y)
+        for (int X = 0; X < cv.nx; ++X) {
+          if (!C.h_free[X + (size_t)cv.nx * (Y + (size_t)cv.ny * Z)]) continue;
+          for (int z = 2 * Z - 1; z <= 2 * Z + 1; ++z)
+            if (z >= 0 && z < b2.Z) m2[X + (size_t)b2.X * (Y + (size_t)b2.Y * z)] = 1;
+        }
+    for (int z = 0; z < b2.Z; ++z)
+      for (int Y = 0; Y < b2.Y; ++Y)
+        for (int X = 0; X < b2.X; ++X) {
+          if (!m2[X + (size_t)b2.X * (Y + (size_t)b2.Y * z)]) continue;
+          for (int y = 2 * Y - 1; y <= 2 * Y + 1; ++y)
+            if (y >= 0 && y < b1.Y) m1[X + (size_t)b1.X * (y + (size_t)b1.Y * z)] = 1;
+        }
+    C.gal_m1 = dalloc<uint8_t>(c, n1);
+    C.gal_m2 = dalloc<uint8_t>(c, n2);
+    upload(c.stream, C.gal_m1, m1.data(), n1);
+    upload(c.stream, C.gal_m2, m2.data(), n2);
   }
   hipLaunchKernelGGL((k_gal_axis<0, true, false>), dim3(nblk(n1), 27), dim3(256), 0, c.stream, F.v,
                      (const int*)F.slot, (const double*)F.st64, F.NS, bf, b1, C.v, (const int4*)nullptr, 0,
-                     (const uint8_t*)nullptr, C.gal_b1, (float*)nullptr, n1);
+                     (const uint8_t*)nullptr, C.gal_b1, (float*)nullptr, n1, (const uint8_t*)C.gal_m1);
   hipLaunchKernelGGL((k_gal_axis<1, false, false>), dim3(nblk(n2), 27), dim3(256), 0, c.stream, F.v,
                      (const int*)nullptr, (const double*)C.gal_b1, n1, b1, b2, C.v, (const int4*)nullptr, 0,
-                     (const uint8_t*)nullptr, C.gal_b2, (float*)nullptr, n2);
+                     (const uint8_t*)nullptr, C.gal_b2, (float*)nullptr, n2, (const uint8_t*)C.gal_m2);
   hipLaunchKernelGGL((k_gal_axis<2, false, true>), dim3(cblk(C.n_col), 27), dim3(256), 0, c.stream, F.v,
                      (const int*)nullptr, (const double*)C.gal_b2, n2, b2, GBox{C.v.nx, C.v.ny, C.v.nz}, C.v,
-                     (const int4*)C.cl_col, C.n_col, (const uint8_t*)C.fre, C.st64, C.st32, C.NS);
+                     (const int4*)C.cl_col, C.n_col, (const uint8_t*)C.fre, C.st64, C.st32, C.NS,
+                     (const uint8_t*)nullptr);
   LAUNCH_CHECK();
 }
 

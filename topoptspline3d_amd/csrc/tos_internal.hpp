@@ -106,6 +106,8 @@ struct Level {
   std::vector<int> h_slot;
   double* gal_b1 = nullptr;      // Galerkin intermediates of this (coarse) level, box layout
   double* gal_b2 = nullptr;
+  uint8_t* gal_m1 = nullptr;     // which intermediate nodes a later pass reads (the rest are skipped)
+  uint8_t* gal_m2 = nullptr;
 };
 
 struct Ctx {
