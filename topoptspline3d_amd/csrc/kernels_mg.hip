@@ -1053,15 +1053,17 @@ constexpr int kGJB = 32;
 // A pivot (Schur complement) below 1e-12 of the unswept diagonal marks the system as
 // singular -- a floating part or a structure without supports.
 __global__ __launch_bounds__(256) void k_gj_pivot(double* __restrict__ A, int n, int K0, int b,
-                                                  double* __restrict__ Dinv, double* __restrict__ R,
-                                                  const double* __restrict__ diag0, int* __restrict__ flag) {
+                                                  double* __restrict__ Dinv, const double* __restrict__ diag0,
+                                                  int* __restrict__ flag) {
   __shared__ double S[kGJB][kGJB + 1];
   __shared__ double T[kGJB][kGJB + 1];
+  __shared__ double dg[kGJB];
   for (int t = threadIdx.x; t < b * b; t += blockDim.x) S[t / b][t % b] = A[(int64_t)(K0 + t / b) * n + K0 + t % b];
+  if (threadIdx.x < b) dg[threadIdx.x] = diag0[K0 + threadIdx.x];
   __syncthreads();
   for (int k = 0; k < b; ++k) {
-    const double piv = S[k][k];  // same LDS word for every thread: the branch is uniform
-    if (!(piv > 1e-12 * diag0[K0 + k])) {
+    const double piv = S[k][k];  // same LDS words for every thread: the branch is uniform
+    if (!(piv > 1e-12 * dg[k])) {
       if (threadIdx.x == 0) *flag = 1;
       return;
     }
@@ -1079,19 +1081,18 @@ __global__ __launch_bounds__(256) void k_gj_pivot(double* __restrict__ A, int n,
     __syncthreads();
   }
   for (int t = threadIdx.x; t < b * b; t += blockDim.x) Dinv[t] = -S[t / b][t % b];  // +A_KK^{-1}
-  __syncthreads();
-  for (int64_t t = threadIdx.x; t < (int64_t)b * n; t += blockDim.x) R[t] = A[(int64_t)(K0 + t / n) * n + t % n];
 }
 
-// C = A[:, K] * Dinv  (n x b)
+// C = A[:, K] * Dinv  (n x b), and R = A[K, :] (b x n) before the update overwrites it
 __global__ void k_gj_panel(const double* __restrict__ A, int n, int K0, int b, const double* __restrict__ Dinv,
-                           double* __restrict__ Cp) {
+                           double* __restrict__ Cp, double* __restrict__ R) {
   const int64_t t = thread_id();
   if (t >= (int64_t)n * b) return;
   const int i = t / b, j = t % b;
   double s = 0;
   for (int q = 0; q < b; ++q) s += A[(int64_t)i * n + K0 + q] * Dinv[q * b + j];
   Cp[t] = s;
+  R[(int64_t)j * n + i] = A[(int64_t)(K0 + j) * n + i];
 }
 
 __global__ void k_gj_update(double* __restrict__ A, int n, int K0, int b, const double* __restrict__ Dinv,
