@@ -139,26 +139,28 @@ DTYPES = {0: "f64", 1: "f32 V-cycle / f64 PCG", 2: "f32 V-cycle (f16 coarse sten
 SOLVERS = {0: "FP64 V-cycle", 1: "FP32 V-cycle", 2: "FP32 V-cycle on FP16 coarse stencils"}
 
 
-def roofline(stats, info, n_active_nodes, steps, workload_tag=None, precision=1):
+def roofline(stats, info, n_active_nodes, steps, workload_tag=None, precision=1, sweep_mode=0):
     """Roofline of the dominant kernel class (by total time among the classes with a
     single-level definition).  Algorithmic bytes per launch (each array read or written
     once at storage precision, index/flag re-reads excluded):
-      fine GS colour pass (FP32): one sweep = 4 N_e + (9*4 + 4) N_v, one launch = 1/8
+      fine GS (FP32): one sweep = 4 N_e + (9*4 + 4) N_v; one launch = 1/8 (a colour) or,
+        with the in-wave paired schedule (sweep mode 3), 1/4 (two colours)
       fine residual (FP32/FP64):  4 N_e + (9 s + 4) N_v
       fine operator (FP64):       4 N_e + (6*8 + 4) N_v
       level-1 stencil GS colour pass (FP32 cycle): (243*4 + 9*4) N_c1 / 8, or with the
         FP16 stencil copy (precision 2) (122*4 + 4 + 9*4) N_c1 / 8"""
     Ne, Nv = info["n_active_elements"][0], n_active_nodes
     Nc1 = info["n_free_nodes"][1] if info["n_levels"] > 1 else 0
+    colours = 2 if sweep_mode == 3 else 1  # colours per fine FP32 GS launch
     per_launch = {
-        "fine_gs_f32": (4 * Ne + (9 * 4 + 4) * Nv) / 8.0,
+        "fine_gs_f32": (4 * Ne + (9 * 4 + 4) * Nv) / 8.0 * colours,
         "fine_gs_f64": (4 * Ne + (9 * 8 + 4) * Nv) / 8.0,
         "fine_resid_f32": 4 * Ne + (9 * 4 + 4) * Nv,
         "fine_resid_f64": 4 * Ne + (9 * 8 + 4) * Nv,
         "fine_apply_f64": 4 * Ne + (6 * 8 + 4) * Nv,
         "st_gs_f32_l1": ((122 * 4 + 4 if precision == 2 else 243 * 4) + 9 * 4) * Nc1 / 8.0,
     }
-    flops = {"fine_gs_f32": 1170 * Nv / 8.0, "fine_gs_f64": 1170 * Nv / 8.0, "fine_resid_f32": 1152 * Nv,
+    flops = {"fine_gs_f32": 1170 * Nv / 8.0 * colours, "fine_gs_f64": 1170 * Nv / 8.0, "fine_resid_f32": 1152 * Nv,
              "fine_resid_f64": 1152 * Nv, "fine_apply_f64": 1152 * Nv, "st_gs_f32_l1": 504 * Nc1 / 8.0}
     dom = max((k for k in stats if k in per_launch), key=lambda k: stats[k][0])
     tot = {k: v[0] for k, v in stats.items() if v[1] > 0}
@@ -168,8 +170,10 @@ def roofline(stats, info, n_active_nodes, steps, workload_tag=None, precision=1)
     # by a device-copy calibration) for this kernel class and workload, when recorded
     tf = os.path.join(ROOT, "profiles", f"{dom}_{workload_tag}.json") if workload_tag else None
     if tf and os.path.exists(tf):
-        t = json.load(open(tf)).get("traffic")
-        if t:
+        js = json.load(open(tf))
+        t = js.get("traffic")
+        # (a GS profile holds for the schedule it was taken with)
+        if t and (not dom.startswith("fine_gs") or int(js.get("sweep_mode", 0)) == (sweep_mode if dom == "fine_gs_f32" else 0)):
             out["traffic"] = round(t["bytes_per_launch"])
             out["traffic_source"] = os.path.relpath(tf, ROOT)
     if dom in per_launch and stats[dom][1] > 0:
@@ -418,7 +422,8 @@ def main():
                               + (" + learned level-2 correction" if net is not None else "") + ", FP64 refinement"),
                    "pcg_iterations": its, "max_rel_res": res_max},
         "roofline": roofline(stats, info, nv, args.profile_steps,
-                             "ellipsoids_511x320x320" if wl["name"] == "c2" else None, args.precision),
+                             "ellipsoids_511x320x320" if wl["name"] == "c2" else None, args.precision,
+                             args.sweep_mode),
         "plain_mixed": {"ms_per_iter": round(ms_plain, 3), "pcg_iterations": its_plain,
                         "max_rel_res": res_plain},
     }

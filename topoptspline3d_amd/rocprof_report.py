@@ -7,10 +7,14 @@
   a device copy of known size (``bench.py --calibrate-copy-mib N`` under the same
   counter), because the counters need not count every byte once.
 
+A kernel class can consist of several kernels launched together (repeat --kernel, e.g.
+the paired GS pass and the copy of its deferred nodes): their per-dispatch bytes and mean
+durations are added.
+
 Usage:
   python -m topoptspline3d_amd.rocprof_report --stats DIR/c2_kernel_stats.csv \\
-      --kernel "k_fine_gs<float" --fetch F.csv --write W.csv --cal-fetch CF.csv \\
-      --cal-write CW.csv --copy-mib 1024 [--json out.json]
+      --kernel "k_fine_gs<float" [--kernel ...] --fetch F.csv --write W.csv \\
+      --cal-fetch CF.csv --cal-write CW.csv --copy-mib 1024 [--json out.json] [--meta k=v]
 """
 import argparse
 import csv
@@ -76,7 +80,7 @@ def traffic(fetch_csv, write_csv, cal_fetch_csv, cal_write_csv, copy_mib, patter
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--stats", required=True)
-    ap.add_argument("--kernel", required=True)
+    ap.add_argument("--kernel", required=True, action="append")
     ap.add_argument("--fetch")
     ap.add_argument("--write")
     ap.add_argument("--cal-fetch")
@@ -84,17 +88,31 @@ def main(argv=None):
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--top", type=int, default=30)
     ap.add_argument("--json")
+    ap.add_argument("--meta", action="append", default=[], help="key=value recorded in the JSON")
     a = ap.parse_args(argv)
     print(kernel_table(a.stats, a.top))
-    calls, us = kernel_mean(a.stats, a.kernel)
-    print(f"\n`{a.kernel}*`: {calls} launches, mean {us:.2f} us")
-    out = {"kernel": a.kernel, "launches": calls, "mean_us": us}
+    names = " + ".join(f"`{k}*`" for k in a.kernel)
+    calls, us = 0, 0.0
+    for k in a.kernel:
+        ck, uk = kernel_mean(a.stats, k)
+        print(f"\n`{k}*`: {ck} launches, mean {uk:.2f} us")
+        calls = ck if not calls else min(calls, ck)
+        us += uk
+    out = {"kernel": a.kernel if len(a.kernel) > 1 else a.kernel[0], "launches": calls, "mean_us": us}
+    for kv in a.meta:
+        k, v = kv.split("=", 1)
+        out[k] = int(v) if v.lstrip("-").isdigit() else v
     if a.fetch:
-        t = traffic(a.fetch, a.write, a.cal_fetch, a.cal_write, a.copy_mib, a.kernel)
+        parts = [traffic(a.fetch, a.write, a.cal_fetch, a.cal_write, a.copy_mib, k) for k in a.kernel]
+        t = dict(parts[0]) if len(parts) == 1 else {"kernel": a.kernel, "parts": parts,
+                                                    "bytes_per_launch": sum(p["bytes_per_launch"] for p in parts)}
         out["traffic"] = t
-        print(f"\nHBM traffic per launch of `{a.kernel}*`: FETCH_SIZE {t['fetch_kib_raw']:.0f} KiB x "
-              f"{t['fetch_scale']:.3f} + WRITE_SIZE {t['write_kib_raw']:.0f} KiB x {t['write_scale']:.3f} = "
-              f"{t['bytes_per_launch'] / 1e6:.1f} MB ({t['dispatches'][0]} / {t['dispatches'][1]} dispatches)")
+        for p in parts:
+            print(f"\nHBM traffic per launch of `{p['kernel']}*`: FETCH_SIZE {p['fetch_kib_raw']:.0f} KiB x "
+                  f"{p['fetch_scale']:.3f} + WRITE_SIZE {p['write_kib_raw']:.0f} KiB x {p['write_scale']:.3f} = "
+                  f"{p['bytes_per_launch'] / 1e6:.1f} MB ({p['dispatches'][0]} / {p['dispatches'][1]} dispatches)")
+        if len(parts) > 1:
+            print(f"\nclass {names}: {t['bytes_per_launch'] / 1e6:.1f} MB per launch, mean {us:.2f} us")
     if a.json:
         json.dump(out, open(a.json, "w"), indent=1)
     return 0
