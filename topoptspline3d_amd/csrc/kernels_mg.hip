@@ -601,24 +601,33 @@ struct StHalf {
   __device__ __forceinline__ float scale(int64_t s) const { return msc[s]; }
 };
 
-// FP16 copy (StHalf layout) of a compact FP32 stencil; thread per slot of the colour list
+// FP16 copy (StHalf layout) of a compact FP32 stencil; thread per slot of the colour list.
+// The node's 243 entries are read once into registers (all loads in flight together).
 __global__ __launch_bounds__(256) void k_st_pack16(const int4* __restrict__ cl, int nch,
                                                    const float* __restrict__ st, int64_t NS,
                                                    uint32_t* __restrict__ h, float* __restrict__ msc) {
   NodeRef n;
   if (!chunk_node(cl, nch, 2, n)) return;
   const int64_t s = n.slot;
+  float v[243];
+#pragma unroll
+  for (int q = 0; q < 243; ++q) v[q] = st[(int64_t)q * NS + s];
   float mx = 0;
-  for (int q = 0; q < 243; ++q) mx = fmaxf(mx, fabsf(st[(int64_t)q * NS + s]));
+#pragma unroll
+  for (int q = 0; q < 243; ++q) mx = fmaxf(mx, fabsf(v[q]));
   if (!(mx > 0)) mx = 1;
   const float inv = 1.0f / mx;
   msc[s] = mx;
-  auto hb = [&](int k, int e) { return to_half_bits(st[(int64_t)(k * 9 + e) * NS + s] * inv); };
+#pragma unroll
   for (int k = 0; k < 27; ++k)
 #pragma unroll
-    for (int w = 0; w < 4; ++w) h[(int64_t)(4 * k + w) * NS + s] = hb(k, 2 * w) | (hb(k, 2 * w + 1) << 16);
+    for (int w = 0; w < 4; ++w)
+      h[(int64_t)(4 * k + w) * NS + s] =
+          to_half_bits(v[k * 9 + 2 * w] * inv) | (to_half_bits(v[k * 9 + 2 * w + 1] * inv) << 16);
+#pragma unroll
   for (int t = 0; t < 14; ++t)
-    h[(int64_t)(108 + t) * NS + s] = hb(2 * t, 8) | (2 * t + 1 < 27 ? hb(2 * t + 1, 8) << 16 : 0u);
+    h[(int64_t)(108 + t) * NS + s] =
+        to_half_bits(v[2 * t * 9 + 8] * inv) | (2 * t + 1 < 27 ? to_half_bits(v[(2 * t + 1) * 9 + 8] * inv) << 16 : 0u);
 }
 
 // y = K u (MODE 0) or f - K u (MODE 1) over the whole colour-ordered list
@@ -649,22 +658,18 @@ __global__ __launch_bounds__(256) void k_st_apply(LevelView L, const int4* __res
   }
 }
 
-// one colour: cl points at the colour's chunks, slot0 = 64 * (their offset in the list).
-// With a scaled stencil the block equations are solved in the scaled form
-// (K~ = K / m, f / m), which gives the same update.
-template <class T, bool BACKWARD, int ZC, class ST>
-__global__ __launch_bounds__(256) void k_st_gs(LevelView L, const int4* __restrict__ cl, int nch, int64_t slot0,
-                                               ST st, T* __restrict__ u, const T* __restrict__ f) {
-  constexpr unsigned mask = nb_mask_of(ZC);
-  NodeRef n;
-  if (!chunk_node(cl, nch, 2, n)) return;
-  const int64_t p = n.p, slot = slot0 + n.slot;
+// Relax one node from its loaded neighbourhood un (k = (dx+1) + 3 (dy+1) + 9 (dz+1)) and
+// its stencil row at `slot`; MASK: the neighbours read (the others hold 0 and are skipped);
+// v: the node's value (its start value on entry).  With a scaled stencil the block
+// equations are solved in the scaled form (K~ = K / m, f / m), which gives the same update.
+template <class T, bool BACKWARD, unsigned MASK, class ST>
+__device__ __forceinline__ void st_relax(const ST& st, int64_t slot, const T (&un)[27][3], T f0, T f1, T f2,
+                                         T& v0, T& v1, T& v2) {
   T s0 = 0, s1 = 0, s2 = 0;
 #pragma unroll
   for (int k = 0; k < 27; ++k) {
-    if (k == 13 || !((mask >> k) & 1u)) continue;  // skipped neighbours hold 0: no stencil read
-    const int64_t q = p + (int64_t)(k % 3 - 1) + (int64_t)L.NX * ((k / 3) % 3 - 1 + (int64_t)L.NY * (k / 9 - 1));
-    const T u0 = u[q], u1 = u[L.NP + q], u2 = u[2 * L.NP + q];
+    if (k == 13 || !((MASK >> k) & 1u)) continue;  // skipped neighbours hold 0: no stencil read
+    const T u0 = un[k][0], u1 = un[k][1], u2 = un[k][2];
     s0 = fma(st.get(slot, k, 0), u0, s0); s0 = fma(st.get(slot, k, 1), u1, s0); s0 = fma(st.get(slot, k, 2), u2, s0);
     s1 = fma(st.get(slot, k, 3), u0, s1); s1 = fma(st.get(slot, k, 4), u1, s1); s1 = fma(st.get(slot, k, 5), u2, s1);
     s2 = fma(st.get(slot, k, 6), u0, s2); s2 = fma(st.get(slot, k, 7), u1, s2); s2 = fma(st.get(slot, k, 8), u2, s2);
@@ -672,28 +677,109 @@ __global__ __launch_bounds__(256) void k_st_gs(LevelView L, const int4* __restri
   T D[9];
 #pragma unroll
   for (int e = 0; e < 9; ++e) D[e] = st.get(slot, 13, e);
-  T f0 = f[p], f1 = f[L.NP + p], f2 = f[2 * L.NP + p];
   if constexpr (ST::kScaled) {
     const T inv = T(1) / st.scale(slot);
     f0 *= inv; f1 *= inv; f2 *= inv;
   }
-  const T b0 = f0 - s0, b1 = f1 - s1, b2 = f2 - s2;
+  block_relax<T, BACKWARD>(f0 - s0, f1 - s1, f2 - s2, D, v0, v1, v2);
+}
+
+// one colour: cl points at the colour's chunks, slot0 = 64 * (their offset in the list)
+template <class T, bool BACKWARD, int ZC, class ST>
+__global__ __launch_bounds__(256) void k_st_gs(LevelView L, const int4* __restrict__ cl, int nch, int64_t slot0,
+                                               ST st, T* __restrict__ u, const T* __restrict__ f) {
+  constexpr unsigned mask = nb_mask_of(ZC);
+  NodeRef n;
+  if (!chunk_node(cl, nch, 2, n)) return;
+  const int64_t p = n.p, slot = slot0 + n.slot;
+  T un[27][3];
+#pragma unroll
+  for (int k = 0; k < 27; ++k) {
+    const bool ld = k != 13 && ((mask >> k) & 1u);
+    const int64_t q = p + (int64_t)(k % 3 - 1) + (int64_t)L.NX * ((k / 3) % 3 - 1 + (int64_t)L.NY * (k / 9 - 1));
+#pragma unroll
+    for (int c = 0; c < 3; ++c) un[k][c] = ld ? u[c * L.NP + q] : T(0);
+  }
   T v0 = 0, v1 = 0, v2 = 0;
   if constexpr (ZC < 0) {
     v0 = u[p]; v1 = u[L.NP + p]; v2 = u[2 * L.NP + p];
   }
-  if (!BACKWARD) {
-    v0 = (b0 - D[1] * v1 - D[2] * v2) / D[0];
-    v1 = (b1 - D[3] * v0 - D[5] * v2) / D[4];
-    v2 = (b2 - D[6] * v0 - D[7] * v1) / D[8];
-  } else {
-    v2 = (b2 - D[6] * v0 - D[7] * v1) / D[8];
-    v1 = (b1 - D[3] * v0 - D[5] * v2) / D[4];
-    v0 = (b0 - D[1] * v1 - D[2] * v2) / D[0];
-  }
+  st_relax<T, BACKWARD, mask>(st, slot, un, f[p], f[L.NP + p], f[2 * L.NP + p], v0, v1, v2);
   u[p] = v0;
   u[L.NP + p] = v1;
   u[2 * L.NP + p] = v2;
+}
+
+// Stored-stencil counterpart of k_fine_gs_pairw for levels whose rows fit one segment
+// (span of free nodes <= 126 along x): one wave per row, no segment boundaries, so no
+// deferred nodes.  Stencil rows are found through the level's slot map.
+template <class T, bool BACKWARD, int ZA, int ZB, class ST>
+__global__ __launch_bounds__(256) void k_st_gs_pairw(LevelView L, const int4* __restrict__ seg, int nseg,
+                                                     const uint8_t* __restrict__ fre,
+                                                     const int* __restrict__ slotmap, ST st,
+                                                     T* __restrict__ u, const T* __restrict__ f) {
+  constexpr unsigned MA = nb_mask_of(ZA), MB = nb_mask_of(ZB);
+  const int si = (int)xcd_block() * 4 + (threadIdx.x >> 6);
+  if (si >= nseg) return;  // whole wave
+  const int lane = threadIdx.x & 63;
+  const int4 S = seg[si];
+  const int xs = (S.y & 0x7fff) - 1, xe = ((S.y >> 15) & 0x7fff) - 1;
+  const int x1 = xs + 2 * lane;
+  const bool live = x1 <= xe + 1;
+  const int xa = live ? x1 : xe + 1;  // loads from addresses clamped into the row
+  const int64_t pa = (int64_t)S.x + (xa - xs);
+  const bool f1 = live && fre[pa] != 0;
+  const bool f2 = live && lane < 63 && x1 + 1 <= xe && fre[pa + 1] != 0;
+  T un[27][3];
+#pragma unroll
+  for (int k = 0; k < 27; ++k) {
+    const int dx = k % 3 - 1;
+    const bool need = dx < 0 ? ((MA >> k) & 1u) != 0
+                              : (((MA >> k) & 1u) || ((MB >> (k - 1)) & 1u) ||
+                                 (dx == 0 && ((MB >> (k + 1)) & 1u)));
+    const int64_t q = (dx < 0 ? (xa >= 0 ? pa - 1 : pa) : pa + dx) +
+                      (int64_t)L.NX * ((k / 3) % 3 - 1 + (int64_t)L.NY * (k / 9 - 1));
+#pragma unroll
+    for (int c = 0; c < 3; ++c) un[k][c] = need ? u[c * L.NP + q] : T(0);
+  }
+  const int sl1 = slotmap[pa], sl2 = slotmap[pa + 1];
+  T fa[3], fb[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    fa[c] = f[c * L.NP + pa];
+    fb[c] = f[c * L.NP + pa + 1];
+  }
+  // c1 (from zero: its own start value is 0)
+  T v0 = ZA < 0 ? un[13][0] : T(0), v1 = ZA < 0 ? un[13][1] : T(0), v2 = ZA < 0 ? un[13][2] : T(0);
+  if (f1) st_relax<T, BACKWARD, MA>(st, sl1, un, fa[0], fa[1], fa[2], v0, v1, v2);
+  // c2's neighbourhood: own columns 0, +1 and lane l+1's column 0, with the new c1 values
+  T vn[27][3];
+#pragma unroll
+  for (int k = 0; k < 27; ++k) {
+    if (k % 3 - 1 <= 0) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) vn[k][c] = un[k + 1][c];
+    }
+  }
+  vn[12][0] = v0; vn[12][1] = v1; vn[12][2] = v2;
+#pragma unroll
+  for (int k = 2; k < 27; k += 3) {
+    const int k0 = k - 1;
+    const bool need = ((MB >> k) & 1u) != 0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const T mine = k0 == 13 ? (c == 0 ? v0 : (c == 1 ? v1 : v2)) : un[k0][c];
+      vn[k][c] = need ? __shfl_down(mine, 1, 64) : T(0);
+    }
+  }
+  T t0 = ZB < 0 ? vn[13][0] : T(0), t1 = ZB < 0 ? vn[13][1] : T(0), t2 = ZB < 0 ? vn[13][2] : T(0);
+  if (f2) st_relax<T, BACKWARD, MB>(st, sl2, vn, fb[0], fb[1], fb[2], t0, t1, t2);
+  if (f1) {
+    u[pa] = v0; u[L.NP + pa] = v1; u[2 * L.NP + pa] = v2;
+  }
+  if (f2) {
+    u[pa + 1] = t0; u[L.NP + pa + 1] = t1; u[2 * L.NP + pa + 1] = t2;
+  }
 }
 
 // stored-stencil counterpart of k_fine_resid_z (nodes of colour ZC, later-colour blocks only)

@@ -330,11 +330,12 @@ void build_chunks(Ctx& c, Level& L, bool with_slots) {
       }
     }
   }
-  // row segments of the in-wave two-colour GS (fine level): per row of class (y%2, z%2),
-  // the free nodes' x range [xmin, xe], first-colour parity P; segments start at xs (xmin
-  // or xmin - 1, of parity P) and every 126 nodes after it until the last c2 node is covered
-  if (!with_slots) {
-    if (v.nx + 1 >= 0x7fff) throw ArgError("grid too wide for the row segments (nx < 32766)");
+  // row segments of the in-wave two-colour GS: per row of class (y%2, z%2), the free
+  // nodes' x range [xmin, xe], first-colour parity P; segments start at xs (xmin or
+  // xmin - 1, of parity P) and every 126 nodes after it until the last c2 node is covered.
+  // Stencil levels use them only when every row is a single segment.
+  if (v.nx + 1 < 0x7fff) {
+    bool single = true;
     for (int cls = 0; cls < 4; ++cls)
       for (int P = 0; P < 2; ++P) {
         const int py = cls & 1, pz = cls >> 1;
@@ -350,6 +351,7 @@ void build_chunks(Ctx& c, Level& L, bool with_slots) {
             if (xmin < 0) continue;
             const int xs = ((xmin & 1) == P) ? xmin : xmin - 1;
             const int nseg = (xmax - xs + 1 + 125) / 126;
+            if (nseg > 1) single = false;
             for (int q = 0; q < nseg; ++q) {
               const int x0 = xs + 126 * q;
               sg.push_back(make_int4((int)v.pnode(x0, y, z), (x0 + 1) | ((xmax + 1) << 15) | ((q > 0 ? 1 : 0) << 30),
@@ -360,6 +362,9 @@ void build_chunks(Ctx& c, Level& L, bool with_slots) {
         L.segw[cls][P] = dalloc<int4>(c, std::max<size_t>(sg.size(), 1));
         if (!sg.empty()) upload(c.stream, L.segw[cls][P], sg.data(), sg.size() * sizeof(int4));
       }
+    L.pair_ok = with_slots && single;
+  } else if (!with_slots) {
+    throw ArgError("grid too wide for the row segments (nx < 32766)");
   }
   // colour-slab wavefront of the fine GS: ~32 z-slabs so that the slabs in flight during a
   // few consecutive steps fit in the memory-side cache
@@ -659,6 +664,44 @@ void gs_sweep_pairw(Ctx& c, float* u, const float* f, bool backward, bool from_z
   LAUNCH_CHECK();
 }
 
+// Stored-stencil level sweep in four in-wave two-colour passes (k_st_gs_pairw), for
+// levels whose rows are single segments; same class order as gs_sweep_pairw.
+template <class T>
+void gs_sweep_st_pairw(Ctx& c, int l, T* u, const T* f, bool backward, bool from_zero) {
+  Level& L = c.L[l];
+  const LevelView& v = L.v;
+  const int P = backward ? 1 : 0;
+  for (int i = 0; i < 4; ++i) {
+    const int cls = backward ? 3 - i : i;
+    const int n = L.n_segw[cls][P];
+    if (!n) continue;
+    Prof pr(c, sizeof(T) == 8 ? P_ST_GS64 : (l == 1 ? P_ST_GS32_L1 : P_ST_GS32));
+    const dim3 g((n + 3) / 4), b(256);
+    const int4* sg = L.segw[cls][P];
+    st_dispatch<T>(c, L, [&](auto st) {
+      using ST = decltype(st);
+      if (backward) {
+        hipLaunchKernelGGL((k_st_gs_pairw<T, true, -1, -1, ST>), g, b, 0, c.stream, v, sg, n, (const uint8_t*)L.fre,
+                           (const int*)L.slot, st, u, f);
+      } else if (!from_zero) {
+        hipLaunchKernelGGL((k_st_gs_pairw<T, false, -1, -1, ST>), g, b, 0, c.stream, v, sg, n, (const uint8_t*)L.fre,
+                           (const int*)L.slot, st, u, f);
+      } else {
+#define GO(A) hipLaunchKernelGGL((k_st_gs_pairw<T, false, A, A + 1, ST>), g, b, 0, c.stream, v, sg, n, \
+                                 (const uint8_t*)L.fre, (const int*)L.slot, st, u, f)
+        switch (cls) {
+          case 0: GO(0); break;
+          case 1: GO(2); break;
+          case 2: GO(4); break;
+          default: GO(6); break;
+        }
+#undef GO
+      }
+    });
+  }
+  LAUNCH_CHECK();
+}
+
 template <class T>
 void gs_sweep(Ctx& c, int l, T* u, const T* f, bool backward, bool from_zero = false) {
   Level& L = c.L[l];
@@ -668,6 +711,10 @@ void gs_sweep(Ctx& c, int l, T* u, const T* f, bool backward, bool from_zero = f
       gs_sweep_pairw(c, u, f, backward, from_zero);
       return;
     }
+  }
+  if (l > 0 && c.sweep_mode == 3 && L.pair_ok) {
+    gs_sweep_st_pairw<T>(c, l, u, f, backward, from_zero);
+    return;
   }
   if (l == 0 && c.sweep_mode == 2) {
     gs_sweep_pairs<T>(c, u, f, backward, from_zero);
