@@ -865,6 +865,54 @@ __global__ __launch_bounds__(256) void k_prolong_add(LevelView F, LevelView C, c
   uf[2 * F.NP + p] = f2 + a2;
 }
 
+// u_f += P u_c over row segments (even-x first, all rows): lane l takes the fine nodes
+// x1 = xs + 2l (coarse column X = x1/2, weight 1 along x) and x1 + 1 (columns X and X+1,
+// 1/2 each), so one wave reads each coarse column once for two fine nodes.  Lane 0 of a
+// continuation segment repeats the previous segment's lane 63 and does not write.
+template <class T>
+__global__ __launch_bounds__(256) void k_prolong_add_pairw(LevelView F, LevelView C, const int4* __restrict__ seg,
+                                                           int nseg, const uint8_t* __restrict__ fre,
+                                                           const T* __restrict__ uc, T* __restrict__ uf) {
+  const int si = (int)xcd_block() * 4 + (threadIdx.x >> 6);
+  if (si >= nseg) return;
+  const int lane = threadIdx.x & 63;
+  const int4 S = seg[si];
+  const int xs = (S.y & 0x7fff) - 1, xe = ((S.y >> 15) & 0x7fff) - 1;
+  const bool cont = (S.y >> 30) & 1;
+  const int x1 = xs + 2 * lane, y = S.z, z = S.w;
+  const bool live = x1 <= xe + 1;
+  const int xa = live ? x1 : xe + 1;  // (even: the segments start on even x)
+  const int64_t pa = (int64_t)S.x + (xa - xs);
+  const bool f1 = live && !(cont && lane == 0) && fre[pa] != 0;
+  const bool f2 = live && lane < 63 && x1 + 1 <= xe && fre[pa + 1] != 0;
+  const T hy = (y & 1) ? T(0.5) : T(0), hz = (z & 1) ? T(0.5) : T(0);
+  const T wy[2] = {T(1) - hy, hy}, wz[2] = {T(1) - hz, hz};
+  const int64_t I0 = C.pnode(xa >> 1, y >> 1, z >> 1);
+  T a[2][3] = {{0, 0, 0}, {0, 0, 0}};  // columns X and X+1
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int i = k & 1, j = (k >> 1) & 1, l = k >> 2;
+    const T wgt = wy[j] * wz[l];
+    const int64_t I = I0 + i + (int64_t)C.NX * (j + (int64_t)C.NY * l);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) a[i][c] = fma(wgt, uc[c * C.NP + I], a[i][c]);
+  }
+  T g1[3], g2[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    g1[c] = uf[c * F.NP + pa];
+    g2[c] = uf[c * F.NP + pa + 1];
+  }
+  if (f1) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) uf[c * F.NP + pa] = g1[c] + a[0][c];
+  }
+  if (f2) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) uf[c * F.NP + pa + 1] = g2[c] + T(0.5) * (a[0][c] + a[1][c]);
+  }
+}
+
 template <class T>
 __global__ __launch_bounds__(256) void k_zero_nodes(const int4* __restrict__ cl, int nch, int64_t NP, T* __restrict__ u) {
   NodeRef n;

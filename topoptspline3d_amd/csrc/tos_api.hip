@@ -363,6 +363,26 @@ void build_chunks(Ctx& c, Level& L, bool with_slots) {
         if (!sg.empty()) upload(c.stream, L.segw[cls][P], sg.data(), sg.size() * sizeof(int4));
       }
     L.pair_ok = with_slots && single;
+    std::vector<int4> sp;  // all rows, even x first
+    for (int z = 0; z < v.nz; ++z)
+      for (int y = 0; y < v.ny; ++y) {
+        int xmin = -1, xmax = -1;
+        for (int x = 0; x < v.nx; ++x)
+          if (freeat(x, y, z)) {
+            if (xmin < 0) xmin = x;
+            xmax = x;
+          }
+        if (xmin < 0) continue;
+        const int xs = xmin & ~1;
+        const int nseg = (xmax - xs + 1 + 125) / 126;
+        for (int q = 0; q < nseg; ++q) {
+          const int x0 = xs + 126 * q;
+          sp.push_back(make_int4((int)v.pnode(x0, y, z), (x0 + 1) | ((xmax + 1) << 15) | ((q > 0 ? 1 : 0) << 30), y, z));
+        }
+      }
+    L.n_segp = (int)sp.size();
+    L.segp = dalloc<int4>(c, std::max<size_t>(sp.size(), 1));
+    if (!sp.empty()) upload(c.stream, L.segp, sp.data(), sp.size() * sizeof(int4));
   } else if (!with_slots) {
     throw ArgError("grid too wide for the row segments (nx < 32766)");
   }
@@ -891,8 +911,12 @@ void vcycle_level(Ctx& c, int l, const T* f, T* u, int pre, int post) {
   vcycle_level<T>(c, l + 1, Prec<T>::f(C), Prec<T>::u(C), pre, post);
   {
     Prof pr(c, P_TRANSFER);
-    hipLaunchKernelGGL((k_prolong_add<T>), dim3(cblk(L.n_lin)), dim3(256), 0, c.stream, L.v, C.v,
-                       (const int4*)L.cl_lin, L.n_lin, (const T*)Prec<T>::u(C), u);
+    if (L.n_segp)
+      hipLaunchKernelGGL((k_prolong_add_pairw<T>), dim3((L.n_segp + 3) / 4), dim3(256), 0, c.stream, L.v, C.v,
+                         (const int4*)L.segp, L.n_segp, (const uint8_t*)L.fre, (const T*)Prec<T>::u(C), u);
+    else
+      hipLaunchKernelGGL((k_prolong_add<T>), dim3(cblk(L.n_lin)), dim3(256), 0, c.stream, L.v, C.v,
+                         (const int4*)L.cl_lin, L.n_lin, (const T*)Prec<T>::u(C), u);
   }
   LAUNCH_CHECK();
   if constexpr (std::is_same<T, float>::value)

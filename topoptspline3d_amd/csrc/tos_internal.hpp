@@ -102,6 +102,8 @@ struct Level {
   int n_segw[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
   float* ub32 = nullptr;
   bool pair_ok = false;          // stencil level: every row is one segment (k_st_gs_pairw)
+  int4* segp = nullptr;          // even-x segments of all rows (paired prolongation into this level)
+  int n_segp = 0;
   int64_t NS = 0;                // stencil slots per entry = 64 * n_col
   int* slot = nullptr;           // padded node -> stencil slot, -1 if not free
   std::vector<int> h_slot;
