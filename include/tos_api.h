@@ -281,9 +281,11 @@ int tos_unet_eval(tos_unet* net, tos_dataset* ds, int first, int count, double* 
 /* mode 1: inside FP32 V-cycles the network proposes a correction e of level `level` after
  * its coarse-grid correction.  It is applied with the energy-optimal step
  * alpha = e.r / e.Ae and only when 0.2 <= alpha <= 5 (it lowers the A-norm error and is
- * roughly of the right size); otherwise plain multigrid continues unchanged.  After 3 rejections in a row the network pauses until
- * the next solve.  mode 0 detaches.  The context keeps the pointer: the network must
- * outlive the attachment. */
+ * roughly of the right size); otherwise plain multigrid continues unchanged.  After 3
+ * rejections in a row the network pauses until the next solve; after a solve in which it
+ * was tried and never kept it sits out the next 1, 3, 7, 15 solves (reset when a
+ * correction is kept).  mode 0 detaches.  The context keeps the pointer: the network
+ * must outlive the attachment. */
 int tos_ctx_learn_attach(tos_ctx* ctx, tos_unet* net, int level, int mode);
 /* log (features, target) pairs at `level` of every `stride`-th FP32 V-cycle into ds (grid =
  * the level's node box); target = the level's error after `n_inner` nested V-cycles.

@@ -1304,6 +1304,78 @@ __global__ __launch_bounds__(256) void k_pcg_ur(const int4* __restrict__ cl, int
   block_partial(s, partial, 0);
 }
 
+// r -= alpha q ; float copy of r ; partial ||r||^2 (the u update is deferred to k_pcg_up)
+__global__ __launch_bounds__(256) void k_pcg_r(const int4* __restrict__ cl, int nch, int64_t NP,
+                                               double* __restrict__ r, const double* __restrict__ q,
+                                               const double* __restrict__ scal, float* __restrict__ r32,
+                                               double* __restrict__ partial) {
+  NodeRef n;
+  const bool ok = chunk_node(cl, nch, 1, n);
+  double s = 0;
+  if (ok) {
+    const double al = scal[S_ALPHA];
+    double rv[3], qv[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      rv[c] = r[c * NP + n.p];
+      qv[c] = q[c * NP + n.p];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double rn = rv[c] - al * qv[c];
+      r[c * NP + n.p] = rn;
+      r32[c * NP + n.p] = (float)rn;
+      s += rn * rn;
+    }
+  }
+  block_partial(s, partial, 0);
+}
+
+// u += alpha p (the previous iteration's step, unless first) ; p = z + beta p (first: p = z)
+template <class TZ, class TP>
+__global__ __launch_bounds__(256) void k_pcg_up(const int4* __restrict__ cl, int nch, int64_t NP,
+                                                double* __restrict__ u, TP* __restrict__ p,
+                                                const TZ* __restrict__ z, const double* __restrict__ scal,
+                                                int first) {
+  NodeRef n;
+  if (!chunk_node(cl, nch, 1, n)) return;
+  const double al = first ? 0.0 : scal[S_ALPHA], be = first ? 0.0 : scal[S_BETA];
+  double zv[3], pv[3] = {0, 0, 0}, uv[3] = {0, 0, 0};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const int64_t i = c * NP + n.p;
+    zv[c] = (double)z[i];
+    if (!first) {
+      pv[c] = (double)p[i];
+      uv[c] = u[i];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const int64_t i = c * NP + n.p;
+    if (!first) u[i] = uv[c] + al * pv[c];
+    p[i] = (TP)(zv[c] + be * pv[c]);
+  }
+}
+
+// u += alpha p (the last deferred step of a solve)
+template <class TP>
+__global__ __launch_bounds__(256) void k_pcg_ustep(const int4* __restrict__ cl, int nch, int64_t NP,
+                                                   double* __restrict__ u, const TP* __restrict__ p,
+                                                   const double* __restrict__ scal) {
+  NodeRef n;
+  if (!chunk_node(cl, nch, 1, n)) return;
+  const double al = scal[S_ALPHA];
+  double uv[3], pv[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    uv[c] = u[c * NP + n.p];
+    pv[c] = (double)p[c * NP + n.p];
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) u[c * NP + n.p] = uv[c] + al * pv[c];
+}
+
 // partials of z.r and z.q (z in FP32 or FP64)
 template <class TZ>
 __global__ __launch_bounds__(256) void k_pcg_zdots(const int4* __restrict__ cl, int nch, int64_t NP,

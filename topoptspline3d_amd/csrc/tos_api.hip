@@ -967,6 +967,8 @@ struct LearnState {
   float* lrho = nullptr;
   float *e = nullptr, *t1 = nullptr, *t2 = nullptr, *t3 = nullptr;
   bool paused = false;  // set by the solver after repeated rejections, until the next solve
+  int dry = 0;           // consecutive solves whose corrections were all rejected
+  int skip = 0;          // solves still to run without the correction (back-off)
   NetActs* acts = nullptr;
   std::vector<void*> mem;
 };
@@ -1548,11 +1550,18 @@ int solve_case(Ctx& c, int k, const tos_solve_opts& o, int* its, double* relres)
     return TOS_E_NAN;
   }
   int it = 0;
-  // the learned correction pauses for the rest of this solve after 3 rejections in a row
+  // the learned correction pauses for the rest of this solve after 3 rejections in a row;
+  // after a solve in which it was tried and never kept, it sits out the next 1, 3, 7, 15
+  // solves (doubling while it keeps failing, reset when a correction is kept)
   LearnState* LS = (c.learn && c.learn->mode == 1) ? c.learn : nullptr;
   int nn_rej = 0;
   double nn_tried = hs[S_NN_TRIED], nn_kept = hs[S_NN_KEPT];
-  if (LS) LS->paused = false;
+  const double nn_tried0 = nn_tried, nn_kept0 = nn_kept;
+  if (LS) {
+    LS->paused = LS->skip > 0;
+    if (LS->skip > 0) --LS->skip;
+  }
+  const bool nn_off = !LS || LS->paused;  // no correction anywhere in this solve
   auto nn_watch = [&](const double* h) {
     if (!LS || LS->paused) return;
     if (h[S_NN_TRIED] > nn_tried) nn_rej = h[S_NN_KEPT] > nn_kept ? 0 : nn_rej + 1;
@@ -1580,7 +1589,7 @@ int solve_case(Ctx& c, int k, const tos_solve_opts& o, int* its, double* relres)
     {
       Prof pr(c, P_VECTOR);
       if (f32) {
-        hipLaunchKernelGGL((k_pcg_p<float, float>), dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, p32,
+        hipLaunchKernelGGL((k_pcg_up<float, float>), dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, u, p32,
                            (const float*)L0.u32, (const double*)c.scal, 1);
         hipLaunchKernelGGL((k_pcg_zdots<float>), dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, (const float*)L0.u32,
                            (const double*)c.r, (const double*)nullptr, c.partial);
@@ -1601,9 +1610,9 @@ int solve_case(Ctx& c, int k, const tos_solve_opts& o, int* its, double* relres)
       {
         Prof pr(c, P_VECTOR);
         finish0(c, 0, S_PQ, 0, OP_ALPHA);
-        if (f32)
-          hipLaunchKernelGGL((k_pcg_ur<float>), dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, u, c.r,
-                             (const float*)p32, (const double*)c.q, (const double*)c.scal, L0.f32, c.partial);
+        if (f32)  // (u += alpha p is deferred to the next direction update)
+          hipLaunchKernelGGL(k_pcg_r, dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, c.r, (const double*)c.q,
+                             (const double*)c.scal, L0.f32, c.partial);
         else
           hipLaunchKernelGGL((k_pcg_ur<double>), dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, u, c.r,
                              (const double*)c.p, (const double*)c.q, (const double*)c.scal, (float*)nullptr,
@@ -1620,7 +1629,7 @@ int solve_case(Ctx& c, int k, const tos_solve_opts& o, int* its, double* relres)
         hipLaunchKernelGGL((k_pcg_zdots<float>), dim3(nb), dim3(256), 0, c.stream, cl, nch, NP,
                            (const float*)L0.u32, (const double*)c.r, (const double*)c.q, c.partial);
         finish0(c, 1, S_ZR, S_ZQ, OP_BETA);
-        hipLaunchKernelGGL((k_pcg_p<float, float>), dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, p32,
+        hipLaunchKernelGGL((k_pcg_up<float, float>), dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, u, p32,
                            (const float*)L0.u32, (const double*)c.scal, 0);
       } else {
         hipLaunchKernelGGL((k_pcg_zdots<double>), dim3(nb), dim3(256), 0, c.stream, cl, nch, NP,
@@ -1634,7 +1643,8 @@ int solve_case(Ctx& c, int k, const tos_solve_opts& o, int* its, double* relres)
     // the iteration body [head, tail] is captured once into a graph and replayed (fixed
     // pointers; scalars stay on the device); not while kernels are timed with events or
     // while the learned correction takes host-side decisions
-    const bool graphed = c.graph_replay && !c.prof_on && !(c.learn && (c.learn->mode == 1 || c.learn->ds));
+    const bool graphed =
+        c.graph_replay && !c.prof_on && !(c.learn && (c.learn->ds || (c.learn->mode == 1 && !nn_off)));
     hipGraphExec_t exec = nullptr;
     if (graphed) {
       for (const Ctx::SolveGraph& g : c.graphs)
@@ -1669,6 +1679,17 @@ int solve_case(Ctx& c, int k, const tos_solve_opts& o, int* its, double* relres)
       nn_watch(h);
       if (!std::isfinite(rn)) { c.err = "NaN/Inf in PCG"; return TOS_E_NAN; }
       if (rn / nf <= o.rtol) break;
+    }
+    if (f32 && it > 0)  // the last iteration's deferred u += alpha p
+      hipLaunchKernelGGL((k_pcg_ustep<float>), dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, u, (const float*)p32,
+                         (const double*)c.scal);
+  }
+  if (LS && !nn_off && nn_tried > nn_tried0) {
+    if (nn_kept > nn_kept0) {
+      LS->dry = 0;
+    } else {
+      LS->dry = std::min(LS->dry + 1, 4);
+      LS->skip = (1 << LS->dry) - 1;
     }
   }
   // true FP64 residual
@@ -3283,6 +3304,8 @@ TOS_EXPORT int tos_ctx_learn_attach(tos_ctx* c, tos_unet* net, int level, int mo
     S.net = net;
     S.mode = 1;
     S.paused = false;
+    S.dry = 0;
+    S.skip = 0;
     hipLaunchKernelGGL(k_zero<double>, dim3(1), dim3(64), 0, c->stream, c->scal + S_NN_TRIED, (int64_t)3);
     HIPCHK(hipStreamSynchronize(c->stream));
     return TOS_OK;
