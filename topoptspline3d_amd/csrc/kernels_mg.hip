@@ -929,8 +929,8 @@ __global__ __launch_bounds__(256) void k_cast_nodes(const int4* __restrict__ cl,
 }
 
 // ---------------------------------------------------------------------------------------
-// Galerkin coarse operators.  Thread = (coarse node, neighbour slot k = blockIdx.y), the
-// coarse nodes taken from the colour-ordered chunk list, output in the compact layout.
+// Galerkin coarse operators, the coarse nodes taken from the colour-ordered chunk list,
+// output in the compact layout.
 // First coarse level: the coarse element matrices are sum over active children of
 // rho^p P_c^T Ke P_c (8 precomputed 24x24 products Mc), evaluated with the eliminated
 // rows/columns when a child touches a non-free fine node (allfree[e] = 0)
@@ -947,71 +947,116 @@ __device__ __forceinline__ double pw(int c, int i, int m) {
   return w;
 }
 
-__global__ __launch_bounds__(256) void k_galerkin_first(LevelView F, LevelView C, const int4* __restrict__ clc, int nch,
-                                                        const uint8_t* __restrict__ ffre,
-                                                        const uint8_t* __restrict__ fallfree,
-                                                        const uint8_t* __restrict__ cfre,
-                                                        const double* __restrict__ w, const double* __restrict__ Ke,
-                                                        const double* __restrict__ Mc, double* __restrict__ st,
-                                                        float* __restrict__ st32, int64_t NS) {
-  // Mc entries are read with wave-uniform indices (k is the block row, the element / child
-  // loops are uniform), so they come from the scalar cache as SGPR operands
-  NodeRef nr;
-  if (!chunk_node(clc, nch, 2, nr)) return;
-  const int k = blockIdx.y;
-  const int X = nr.x, Y = nr.y, Z = nr.z;
-  const int dx = k % 3 - 1, dy = (k / 3) % 3 - 1, dz = k / 9 - 1;
-  double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  const int64_t J = nr.p + dx + (int64_t)C.NX * (dy + (int64_t)C.NY * dz);
-  if (cfre[J]) {
-    for (int a = 0; a < 8; ++a) {  // coarse elements E = I - 1 + bits(a) that also hold J
-      const int ax = a & 1, ay = (a >> 1) & 1, az = a >> 2;
-      const int jx = 1 + dx - ax, jy = 1 + dy - ay, jz = 1 + dz - az;  // J - E
-      if (jx < 0 || jx > 1 || jy < 0 || jy > 1 || jz < 0 || jz > 1) continue;
-      const int Ex = X - 1 + ax, Ey = Y - 1 + ay, Ez = Z - 1 + az;
-      if (Ex < 0 || Ey < 0 || Ez < 0 || Ex >= C.ex || Ey >= C.ey || Ez >= C.ez) continue;
-      const int mI = 7 - a, n = jx + 2 * jy + 4 * jz;
-      for (int ch = 0; ch < 8; ++ch) {
-        const int fx = 2 * Ex + (ch & 1), fy = 2 * Ey + ((ch >> 1) & 1), fz = 2 * Ez + (ch >> 2);
-        if (fx >= F.ex || fy >= F.ey || fz >= F.ez) continue;
-        const int64_t fe = F.pelem(fx, fy, fz);
-        const double we = w[fe];
-        if (we == 0) continue;
-        if (fallfree[fe]) {
-          const double* m = Mc + ch * 576 + (3 * mI) * 24 + 3 * n;
+// the two blocks (jx = 0, 1) of one child whose element has an eliminated fine node:
+// sum over its free fine nodes i, j of we P[i, mI] P[j, n] Ke[i][j]
+struct GalPair {
+  double v[2][9];
+};
+__device__ __noinline__ GalPair gal_row_partial(const LevelView F, const uint8_t* __restrict__ ffre,
+                                                const double* __restrict__ Ke, int fx, int fy, int fz, int ch,
+                                                int mI, int jy, int jz, double we) {
+  GalPair t;
+  for (int jx = 0; jx < 2; ++jx)
+    for (int e = 0; e < 9; ++e) t.v[jx][e] = 0;
+  const int64_t fb = F.pnode(fx, fy, fz);
+  bool fr[8];
+  for (int i = 0; i < 8; ++i)
+    fr[i] = ffre[fb + (i & 1) + (int64_t)F.NX * (((i >> 1) & 1) + (int64_t)F.NY * (i >> 2))];
 #pragma unroll
-          for (int r = 0; r < 3; ++r)
+  for (int jx = 0; jx < 2; ++jx) {
+    const int n = jx + 2 * jy + 4 * jz;
+    for (int i = 0; i < 8; ++i) {
+      if (!fr[i]) continue;
+      const double pi = pw(ch, i, mI);
+      if (pi == 0) continue;
+      for (int j = 0; j < 8; ++j) {
+        if (!fr[j]) continue;
+        const double pj = pw(ch, j, n);
+        if (pj == 0) continue;
+        const double sc = we * pi * pj;
 #pragma unroll
-            for (int c = 0; c < 3; ++c) acc[r * 3 + c] += we * m[r * 24 + c];
-        } else {
-          const int64_t fb = F.pnode(fx, fy, fz);
-          bool fr[8];
+        for (int r = 0; r < 3; ++r)
 #pragma unroll
-          for (int i = 0; i < 8; ++i)
-            fr[i] = ffre[fb + (i & 1) + (int64_t)F.NX * (((i >> 1) & 1) + (int64_t)F.NY * (i >> 2))];
-          for (int i = 0; i < 8; ++i) {
-            if (!fr[i]) continue;
-            const double pi = pw(ch, i, mI);
-            if (pi == 0) continue;
-            for (int j = 0; j < 8; ++j) {
-              if (!fr[j]) continue;
-              const double pj = pw(ch, j, n);
-              if (pj == 0) continue;
-              const double s = we * pi * pj;
-#pragma unroll
-              for (int r = 0; r < 3; ++r)
-#pragma unroll
-                for (int c = 0; c < 3; ++c) acc[r * 3 + c] += s * Ke[(3 * i + r) * 24 + 3 * j + c];
-            }
-          }
-        }
+          for (int c = 0; c < 3; ++c) t.v[jx][r * 3 + c] += sc * Ke[(3 * i + r) * 24 + 3 * j + c];
       }
     }
   }
+  return t;
+}
+
+template <int AX>
+__device__ __forceinline__ void gal_elem_row(const LevelView& F, const LevelView& C, int X, int Y, int Z, int ay,
+                                             int az, int jy, int jz, const uint8_t* __restrict__ ffre,
+                                             const uint8_t* __restrict__ fallfree, const double* __restrict__ w,
+                                             const double* __restrict__ Ke, const double* __restrict__ Mc,
+                                             double (&acc)[3][9]) {
+  const int Ex = X - 1 + AX, Ey = Y - 1 + ay, Ez = Z - 1 + az;
+  if (Ex < 0 || Ey < 0 || Ez < 0 || Ex >= C.ex || Ey >= C.ey || Ez >= C.ez) return;
+  const int mI = 7 - (AX + 2 * ay + 4 * az);
+  for (int ch = 0; ch < 8; ++ch) {
+    const int fx = 2 * Ex + (ch & 1), fy = 2 * Ey + ((ch >> 1) & 1), fz = 2 * Ez + (ch >> 2);
+    if (fx >= F.ex || fy >= F.ey || fz >= F.ez) continue;
+    const int64_t fe = F.pelem(fx, fy, fz);
+    const double we = w[fe];
+    if (we == 0) continue;
+    if (fallfree[fe]) {
+      const double* m0 = Mc + ch * 576 + (3 * mI) * 24 + 3 * (2 * jy + 4 * jz);
 #pragma unroll
-  for (int e = 0; e < 9; ++e) {
-    st[(int64_t)(k * 9 + e) * NS + nr.slot] = acc[e];
-    st32[(int64_t)(k * 9 + e) * NS + nr.slot] = (float)acc[e];
+      for (int jx = 0; jx < 2; ++jx) {
+        const double* m = m0 + 3 * jx;
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) acc[jx + AX][r * 3 + c] += we * m[r * 24 + c];
+      }
+    } else {  // a fine node of the child is eliminated: rare, out of line
+      const GalPair t = gal_row_partial(F, ffre, Ke, fx, fy, fz, ch, mI, jy, jz, we);
+#pragma unroll
+      for (int jx = 0; jx < 2; ++jx)
+#pragma unroll
+        for (int e = 0; e < 9; ++e) acc[jx + AX][e] += t.v[jx][e];
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_galerkin_first_row(LevelView F, LevelView C, const int4* __restrict__ clc,
+                                                            int nch, const uint8_t* __restrict__ ffre,
+                                                            const uint8_t* __restrict__ fallfree,
+                                                            const uint8_t* __restrict__ cfre,
+                                                            const double* __restrict__ w,
+                                                            const double* __restrict__ Ke,
+                                                            const double* __restrict__ Mc, double* __restrict__ st,
+                                                            float* __restrict__ st32, int64_t NS) {
+  NodeRef nr;
+  if (!chunk_node(clc, nch, 2, nr)) return;
+  const int dy = (int)blockIdx.y % 3 - 1, dz = (int)blockIdx.y / 3 - 1;
+  const int X = nr.x, Y = nr.y, Z = nr.z;
+  double acc[3][9];
+#pragma unroll
+  for (int b = 0; b < 3; ++b)
+#pragma unroll
+    for (int e = 0; e < 9; ++e) acc[b][e] = 0;
+  for (int az = 0; az < 2; ++az) {  // elements in a order: ax inner, then ay, then az
+    const int jz = 1 + dz - az;
+    if (jz < 0 || jz > 1) continue;
+    for (int ay = 0; ay < 2; ++ay) {
+      const int jy = 1 + dy - ay;
+      if (jy < 0 || jy > 1) continue;
+      gal_elem_row<0>(F, C, X, Y, Z, ay, az, jy, jz, ffre, fallfree, w, Ke, Mc, acc);
+      gal_elem_row<1>(F, C, X, Y, Z, ay, az, jy, jz, ffre, fallfree, w, Ke, Mc, acc);
+    }
+  }
+#pragma unroll
+  for (int b = 0; b < 3; ++b) {
+    const int dx = b - 1;
+    const int k = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
+    const bool keep = cfre[nr.p + dx + (int64_t)C.NX * (dy + (int64_t)C.NY * dz)] != 0;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) {
+      const double v = keep ? acc[b][e] : 0.0;
+      st[(int64_t)(k * 9 + e) * NS + nr.slot] = v;
+      st32[(int64_t)(k * 9 + e) * NS + nr.slot] = (float)v;
+    }
   }
 }
 

@@ -2337,12 +2337,11 @@ TOS_EXPORT int tos_update_operator(tos_ctx* c) {
       Level& F = c->L[l - 1];
       Level& C = c->L[l];
       if (C.n_col == 0) continue;
-      dim3 grid(cblk(C.n_col), 27);
       if (l == 1)
-        hipLaunchKernelGGL(k_galerkin_first, grid, dim3(256), 0, c->stream, F.v, C.v, (const int4*)C.cl_col,
-                           C.n_col, (const uint8_t*)F.fre, (const uint8_t*)c->allfree, (const uint8_t*)C.fre,
-                           (const double*)c->w64, (const double*)c->Ke64, (const double*)c->Mc64, C.st64, C.st32,
-                           C.NS);
+        hipLaunchKernelGGL(k_galerkin_first_row, dim3(cblk(C.n_col), 9), dim3(256), 0, c->stream, F.v, C.v,
+                           (const int4*)C.cl_col, C.n_col, (const uint8_t*)F.fre, (const uint8_t*)c->allfree,
+                           (const uint8_t*)C.fre, (const double*)c->w64, (const double*)c->Ke64,
+                           (const double*)c->Mc64, C.st64, C.st32, C.NS);
       else
         galerkin_by_axes(*c, F, C);  // (both write the FP64 and FP32 stencils)
       LAUNCH_CHECK();
