@@ -2729,12 +2729,23 @@ TOS_EXPORT int tos_spline_get_coeffs(tos_ctx* c, float* v) {
 static void spline_eval_launch(tos_ctx* c) {
   const LevelView& v = c->L[0].v;
   const int64_t n = (int64_t)v.ex * v.ey * v.ez;
-  hipLaunchKernelGGL(k_spline_eval, dim3(nblk(n)), dim3(256), 0, c->stream, v, (const uint8_t*)c->eact,
-                     (const uint8_t*)c->shell, (const float*)c->coef, c->nc[0], c->nc[1], c->sp_order,
-                     (const int*)c->sp_idx[0], (const float*)c->sp_wt[0], (const uint8_t*)c->sp_ok[0],
-                     (const int*)c->sp_idx[1], (const float*)c->sp_wt[1], (const uint8_t*)c->sp_ok[1],
-                     (const int*)c->sp_idx[2], (const float*)c->sp_wt[2], (const uint8_t*)c->sp_ok[2], c->rho_min,
-                     c->rho);
+#define TOS_SPLINE_TABLES                                                                              \
+  (const int*)c->sp_idx[0], (const float*)c->sp_wt[0], (const uint8_t*)c->sp_ok[0], (const int*)c->sp_idx[1], \
+      (const float*)c->sp_wt[1], (const uint8_t*)c->sp_ok[1], (const int*)c->sp_idx[2],                        \
+      (const float*)c->sp_wt[2], (const uint8_t*)c->sp_ok[2]
+  if (c->sp_order == 4)
+    hipLaunchKernelGGL(k_spline_eval_o<4>, dim3(nblk(n)), dim3(256), 0, c->stream, v, (const uint8_t*)c->eact,
+                       (const uint8_t*)c->shell, (const float*)c->coef, c->nc[0], c->nc[1], TOS_SPLINE_TABLES,
+                       c->rho_min, c->rho);
+  else if (c->sp_order == 3)
+    hipLaunchKernelGGL(k_spline_eval_o<3>, dim3(nblk(n)), dim3(256), 0, c->stream, v, (const uint8_t*)c->eact,
+                       (const uint8_t*)c->shell, (const float*)c->coef, c->nc[0], c->nc[1], TOS_SPLINE_TABLES,
+                       c->rho_min, c->rho);
+  else
+    hipLaunchKernelGGL(k_spline_eval, dim3(nblk(n)), dim3(256), 0, c->stream, v, (const uint8_t*)c->eact,
+                       (const uint8_t*)c->shell, (const float*)c->coef, c->nc[0], c->nc[1], c->sp_order,
+                       TOS_SPLINE_TABLES, c->rho_min, c->rho);
+#undef TOS_SPLINE_TABLES
   LAUNCH_CHECK();
 }
 
