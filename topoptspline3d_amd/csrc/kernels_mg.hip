@@ -710,20 +710,22 @@ __global__ __launch_bounds__(256) void k_st_gs(LevelView L, const int4* __restri
   u[2 * L.NP + p] = v2;
 }
 
-// Stored-stencil counterpart of k_fine_gs_pairw for levels whose rows fit one segment
-// (span of free nodes <= 126 along x): one wave per row, no segment boundaries, so no
-// deferred nodes.  Stencil rows are found through the level's slot map.
+// Stored-stencil counterpart of k_fine_gs_pairw (same segments, the same repeated c1 node
+// at segment boundaries and the same two deferred c2 nodes per segment, written to `ub`
+// and copied by k_fine_pairw_copy; levels whose rows are single segments pass ub = null
+// and need no copy).  Stencil rows are found through the level's slot map.
 template <class T, bool BACKWARD, int ZA, int ZB, class ST>
 __global__ __launch_bounds__(256) void k_st_gs_pairw(LevelView L, const int4* __restrict__ seg, int nseg,
                                                      const uint8_t* __restrict__ fre,
                                                      const int* __restrict__ slotmap, ST st,
-                                                     T* __restrict__ u, const T* __restrict__ f) {
+                                                     T* __restrict__ u, const T* __restrict__ f, T* __restrict__ ub) {
   constexpr unsigned MA = nb_mask_of(ZA), MB = nb_mask_of(ZB);
   const int si = (int)xcd_block() * 4 + (threadIdx.x >> 6);
   if (si >= nseg) return;  // whole wave
   const int lane = threadIdx.x & 63;
   const int4 S = seg[si];
   const int xs = (S.y & 0x7fff) - 1, xe = ((S.y >> 15) & 0x7fff) - 1;
+  const bool cont = (S.y >> 30) & 1;
   const int x1 = xs + 2 * lane;
   const bool live = x1 <= xe + 1;
   const int xa = live ? x1 : xe + 1;  // loads from addresses clamped into the row
@@ -774,11 +776,12 @@ __global__ __launch_bounds__(256) void k_st_gs_pairw(LevelView L, const int4* __
   }
   T t0 = ZB < 0 ? vn[13][0] : T(0), t1 = ZB < 0 ? vn[13][1] : T(0), t2 = ZB < 0 ? vn[13][2] : T(0);
   if (f2) st_relax<T, BACKWARD, MB>(st, sl2, vn, fb[0], fb[1], fb[2], t0, t1, t2);
-  if (f1) {
+  if (f1 && !(cont && lane == 0)) {
     u[pa] = v0; u[L.NP + pa] = v1; u[2 * L.NP + pa] = v2;
   }
   if (f2) {
-    u[pa + 1] = t0; u[L.NP + pa + 1] = t1; u[2 * L.NP + pa + 1] = t2;
+    T* dst = (ub && (lane == 0 || lane == 62)) ? ub : u;
+    dst[pa + 1] = t0; dst[L.NP + pa + 1] = t1; dst[2 * L.NP + pa + 1] = t2;
   }
 }
 

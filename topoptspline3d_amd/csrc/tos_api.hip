@@ -362,7 +362,8 @@ void build_chunks(Ctx& c, Level& L, bool with_slots) {
         L.segw[cls][P] = dalloc<int4>(c, std::max<size_t>(sg.size(), 1));
         if (!sg.empty()) upload(c.stream, L.segw[cls][P], sg.data(), sg.size() * sizeof(int4));
       }
-    L.pair_ok = with_slots && single;
+    L.pair_ok = with_slots;
+    L.pair_single = single;
     std::vector<int4> sp;  // all rows, even x first
     for (int z = 0; z < v.nz; ++z)
       for (int y = 0; y < v.ny; ++y) {
@@ -684,13 +685,16 @@ void gs_sweep_pairw(Ctx& c, float* u, const float* f, bool backward, bool from_z
   LAUNCH_CHECK();
 }
 
-// Stored-stencil level sweep in four in-wave two-colour passes (k_st_gs_pairw), for
-// levels whose rows are single segments; same class order as gs_sweep_pairw.
+// Stored-stencil level sweep in four in-wave two-colour passes (k_st_gs_pairw), each
+// followed by the copy of its deferred nodes when rows span several segments; same class
+// order as gs_sweep_pairw.
 template <class T>
 void gs_sweep_st_pairw(Ctx& c, int l, T* u, const T* f, bool backward, bool from_zero) {
   Level& L = c.L[l];
   const LevelView& v = L.v;
   const int P = backward ? 1 : 0;
+  if (!L.pair_single && !L.ubuf) L.ubuf = dalloc<double>(c, 3 * (size_t)v.NP);
+  T* ub = L.pair_single ? nullptr : (T*)L.ubuf;
   for (int i = 0; i < 4; ++i) {
     const int cls = backward ? 3 - i : i;
     const int n = L.n_segw[cls][P];
@@ -702,13 +706,13 @@ void gs_sweep_st_pairw(Ctx& c, int l, T* u, const T* f, bool backward, bool from
       using ST = decltype(st);
       if (backward) {
         hipLaunchKernelGGL((k_st_gs_pairw<T, true, -1, -1, ST>), g, b, 0, c.stream, v, sg, n, (const uint8_t*)L.fre,
-                           (const int*)L.slot, st, u, f);
+                           (const int*)L.slot, st, u, f, ub);
       } else if (!from_zero) {
         hipLaunchKernelGGL((k_st_gs_pairw<T, false, -1, -1, ST>), g, b, 0, c.stream, v, sg, n, (const uint8_t*)L.fre,
-                           (const int*)L.slot, st, u, f);
+                           (const int*)L.slot, st, u, f, ub);
       } else {
 #define GO(A) hipLaunchKernelGGL((k_st_gs_pairw<T, false, A, A + 1, ST>), g, b, 0, c.stream, v, sg, n, \
-                                 (const uint8_t*)L.fre, (const int*)L.slot, st, u, f)
+                                 (const uint8_t*)L.fre, (const int*)L.slot, st, u, f, ub)
         switch (cls) {
           case 0: GO(0); break;
           case 1: GO(2); break;
@@ -718,6 +722,9 @@ void gs_sweep_st_pairw(Ctx& c, int l, T* u, const T* f, bool backward, bool from
 #undef GO
       }
     });
+    if (ub)
+      hipLaunchKernelGGL((k_fine_pairw_copy<T>), dim3(nblk(2 * (int64_t)n)), dim3(256), 0, c.stream, v, sg, n,
+                         (const uint8_t*)L.fre, (const T*)ub, u);
   }
   LAUNCH_CHECK();
 }

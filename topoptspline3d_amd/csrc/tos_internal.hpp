@@ -101,7 +101,9 @@ struct Level {
   int4* segw[4][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
   int n_segw[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
   float* ub32 = nullptr;
-  bool pair_ok = false;          // stencil level: every row is one segment (k_st_gs_pairw)
+  bool pair_ok = false;          // stencil level: row segments built (k_st_gs_pairw)
+  bool pair_single = false;      // ... and every row is one segment (no deferred nodes)
+  void* ubuf = nullptr;          // deferred nodes of the stencil-level paired GS (3 NP doubles)
   int4* segp = nullptr;          // even-x segments of all rows (paired prolongation into this level)
   int n_segp = 0;
   int64_t NS = 0;                // stencil slots per entry = 64 * n_col
