@@ -223,8 +223,9 @@ int tos_ctx_mma_step(tos_ctx* ctx, tos_mma* mma, double sens_scale, double vol_s
  * workgroup per grid row relaxing the row's even-x colour then its odd-x colour; 3 = four
  * launches of two colours each, paired inside a wave (each lane relaxes a node of the
  * first colour, then the next node along x, using its neighbours' new values by lane
- * shuffles), FP32 cycles only (FP64 cycles use mode 0); 3 is the default.  All perform
- * exactly the same updates, so results are bit-identical across modes. */
+ * shuffles), also on the stored-stencil levels, FP32 cycles only (FP64 cycles use mode 0);
+ * 3 is the default.  All perform exactly the same updates, so results are bit-identical
+ * across modes. */
 int tos_set_sweep_mode(tos_ctx* ctx, int mode);
 
 /* The body of an FCG iteration is captured once into a hipGraph and replayed (default

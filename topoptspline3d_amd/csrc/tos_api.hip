@@ -739,7 +739,8 @@ void gs_sweep(Ctx& c, int l, T* u, const T* f, bool backward, bool from_zero = f
       return;
     }
   }
-  if (l > 0 && c.sweep_mode == 3 && L.pair_ok) {
+  // (FP32 cycles only: with FP64 values the paired kernel's registers halve its occupancy)
+  if (std::is_same<T, float>::value && l > 0 && c.sweep_mode == 3 && L.pair_ok) {
     gs_sweep_st_pairw<T>(c, l, u, f, backward, from_zero);
     return;
   }
