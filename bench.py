@@ -95,6 +95,8 @@ class Loop:
         ctx, wl = self.ctx, self.wl
         if wl["spline"]:
             ctx.spline_eval()
+        # FP32 cycles: coarse operators built into FP32 storage; FP64 cycles: FP64
+        ctx.set_galerkin_precision(1 if self.precision else 0)
         ctx.update_operator()
         st = ctx.solve(rtol=wl["rtol"], maxit=300, precision=self.precision)
         self.its.append(st["iterations"][0])
@@ -289,6 +291,7 @@ def batched_phase(api, steps):
     ctx.set_material(1e6, 0.4, 3.0, 1e-3)
     ctx.set_loads(F)
     ctx.set_density(np.full(solid.size, 0.4, np.float32))
+    ctx.set_galerkin_precision(1)  # FP32 cycles only
 
     def it(i):
         ctx.update_operator()

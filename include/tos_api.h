@@ -228,6 +228,18 @@ int tos_ctx_mma_step(tos_ctx* ctx, tos_mma* mma, double sens_scale, double vol_s
  * across modes. */
 int tos_set_sweep_mode(tos_ctx* ctx, int mode);
 
+/* Storage precision of the Galerkin coarse operators built by tos_update_operator (no
+ * counterpart in the reference, whose rxStencil is FP64, grid/Grid.h:306):
+ * 0 (default) = built and stored in FP64, the FP32 copies of the FP32 cycles rounded from
+ * them; 1 = built with FP64 sums into FP32 storage (input stencils, intermediate boxes and
+ * result), for contexts that solve with precision 1 or 2: half the Galerkin traffic.  With
+ * 1 the FP64 stencils and their coarsest factor are built on first demand after each
+ * update (a precision-0 solve or V-cycle, tos_get_stencil), from the same FP64 pipeline as
+ * with 0, so FP64 results do not depend on the setting; the FP32 cycles keep the
+ * FP32-built stencils.  A change takes effect at the next tos_update_operator, which is
+ * then required. */
+int tos_set_galerkin_precision(tos_ctx* ctx, int prec);
+
 /* The body of an FCG iteration is captured once into a hipGraph and replayed (default
  * enable = 1); enable = 0 launches it kernel by kernel, e.g. for tools that trace
  * launches.  Kernel timing (tos_profile) always launches kernel by kernel. */

@@ -1690,6 +1690,13 @@ ORC_API int orc_set_sweep_mode(tos_ctx* c, int mode) {
   return TOS_OK;
 }
 
+// The oracle builds and keeps FP64 stencils only; it accepts and ignores the storage
+// precision of the library's FP32 cycles.
+ORC_API int orc_set_galerkin_precision(tos_ctx* c, int prec) {
+  if (!c || prec < 0 || prec > 1) return TOS_E_ARG;
+  return TOS_OK;
+}
+
 // Graph replay is a GPU launch detail; the oracle accepts and ignores the switch.
 ORC_API int orc_set_graph_replay(tos_ctx* c, int enable) {
   (void)enable;

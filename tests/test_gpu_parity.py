@@ -131,6 +131,20 @@ def test_sensitivity_filter_oc_projection(hip):
     assert rel(g.get_sensitivity(1), o.get_sensitivity(1)) < 1e-6
 
 
+@pytest.mark.parametrize("radius", [0.4, 1.0, 1.5, 2.0, 2.6, 3.2, 4.3])
+def test_filter_radii_across_tiles(hip, radius):
+    """The filter's LDS tiles (64x4x4 elements, halo 1..3) and its global form (larger
+    radii) against the oracle on a domain of several tiles per axis with voids."""
+    solid, support, f, rho = random_domain(70, 9, 10, 4, 0.3)
+    o, g = pair(hip, solid, support, f, rho)
+    s = np.random.default_rng(5).normal(size=solid.size).astype(np.float32)
+    for c in (o, g):
+        c.set_sensitivity(s)
+        c.filter(radius)
+    fo, fg = o.get_sensitivity(), g.get_sensitivity()
+    assert rel(fg, fo) < 1e-6
+
+
 @pytest.mark.parametrize("mode,order,param", [(0, 3, 10), (1, 4, 4)])
 def test_spline_eval_adjoint(hip, mode, order, param):
     solid = np.ones((16, 16, 32), np.uint8)

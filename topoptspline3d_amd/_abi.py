@@ -92,6 +92,7 @@ SIGNATURES = {
     "profile": (C.c_int, [vp, C.c_int, C.POINTER(KernelStats)]),
     "solve_batched": (C.c_int, [vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     "set_sweep_mode": (C.c_int, [vp, C.c_int]),
+    "set_galerkin_precision": (C.c_int, [vp, C.c_int]),
     "set_graph_replay": (C.c_int, [vp, C.c_int]),
     "batch_tuning": (C.c_int, [vp, C.c_char_p, C.c_int]),
 }

@@ -169,6 +169,11 @@ class Context:
         (FP32 cycles); all perform the same updates"""
         self._chk(self.api.set_sweep_mode(self.h, int(mode)), "set_sweep_mode")
 
+    def set_galerkin_precision(self, prec):
+        """coarse operators built into FP64 (0, default) or FP32 storage (1, for precision 1/2
+        solves; FP64 stencils then built on demand); takes effect at the next update_operator"""
+        self._chk(self.api.set_galerkin_precision(self.h, int(prec)), "set_galerkin_precision")
+
     def set_graph_replay(self, enable):
         """replay the captured FCG iteration body (default) or launch kernel by kernel"""
         self._chk(self.api.set_graph_replay(self.h, int(bool(enable))), "set_graph_replay")

@@ -1057,62 +1057,10 @@ __global__ __launch_bounds__(256) void k_galerkin_first_row(LevelView F, LevelVi
 #pragma unroll
     for (int e = 0; e < 9; ++e) {
       const double v = keep ? acc[b][e] : 0.0;
-      st[(int64_t)(k * 9 + e) * NS + nr.slot] = v;
-      st32[(int64_t)(k * 9 + e) * NS + nr.slot] = (float)v;
+      if (st) st[(int64_t)(k * 9 + e) * NS + nr.slot] = v;
+      if (st32) st32[(int64_t)(k * 9 + e) * NS + nr.slot] = (float)v;
     }
   }
-}
-
-// Deeper levels: K_c[I, I+d] = sum_{a near 2I} P[a,I] sum_{b} K_f[a,b] P[b,I+d]; per axis
-// the admissible (offset of a from 2I, stencil offset a->b) pairs, fine stencil read
-// through the fine level's slot map.
-__global__ __launch_bounds__(256) void k_galerkin_next(LevelView F, LevelView C, const int4* __restrict__ clc, int nch,
-                                                       const uint8_t* __restrict__ cfre,
-                                                       const int* __restrict__ fslot, const double* __restrict__ stf,
-                                                       int64_t NSF, double* __restrict__ stc, int64_t NSC) {
-  NodeRef nr;
-  if (!chunk_node(clc, nch, 2, nr)) return;
-  const int k = blockIdx.y;
-  const int d[3] = {k % 3 - 1, (k / 3) % 3 - 1, k / 9 - 1};
-  double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  const int64_t J = nr.p + d[0] + (int64_t)C.NX * (d[1] + (int64_t)C.NY * d[2]);
-  if (cfre[J]) {
-    int pa[3][7], pk[3][7], np_[3];
-    double pwt[3][7];
-    const int Ic[3] = {nr.x, nr.y, nr.z};
-    const int Fn[3] = {F.nx, F.ny, F.nz};
-#pragma unroll
-    for (int ax = 0; ax < 3; ++ax) {
-      int cnt = 0;
-      for (int da = -1; da <= 1; ++da)
-        for (int db = -1; db <= 1; ++db) {
-          const int kin = 2 * d[ax] + db - da;
-          if (kin < -1 || kin > 1) continue;
-          const int ai = 2 * Ic[ax] + da, bi = ai + kin;
-          if (ai < 0 || ai >= Fn[ax] || bi < 0 || bi >= Fn[ax]) continue;
-          pa[ax][cnt] = da;
-          pk[ax][cnt] = kin;
-          pwt[ax][cnt] = (da ? 0.5 : 1.0) * (db ? 0.5 : 1.0);
-          ++cnt;
-        }
-      np_[ax] = cnt;
-    }
-    const int64_t A0 = F.pnode(2 * nr.x, 2 * nr.y, 2 * nr.z);
-    for (int iz = 0; iz < np_[2]; ++iz)
-      for (int iy = 0; iy < np_[1]; ++iy)
-        for (int ix = 0; ix < np_[0]; ++ix) {
-          const int64_t A = A0 + pa[0][ix] + (int64_t)F.NX * (pa[1][iy] + (int64_t)F.NY * pa[2][iz]);
-          const int sa = fslot[A];
-          if (sa < 0) continue;  // eliminated fine node: zero row
-          const int kin = (pk[0][ix] + 1) + 3 * (pk[1][iy] + 1) + 9 * (pk[2][iz] + 1);
-          const double wgt = pwt[0][ix] * pwt[1][iy] * pwt[2][iz];
-          const double* s = stf + (int64_t)kin * 9 * NSF + sa;
-#pragma unroll
-          for (int e = 0; e < 9; ++e) acc[e] += wgt * s[e * NSF];
-        }
-  }
-#pragma unroll
-  for (int e = 0; e < 9; ++e) stc[(int64_t)(k * 9 + e) * NSC + nr.slot] = acc[e];
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1127,18 +1075,20 @@ __global__ __launch_bounds__(256) void k_galerkin_next(LevelView F, LevelView C,
 // Input: the fine level's compact stencil (IN_CMP, slot map) or an intermediate box;
 // output: an intermediate box (only the nodes `omask` marks as read by the next pass:
 // the box spans the whole grid, the domain often a third of it) or the coarse level's
-// compact stencil (OUT_CMP).
+// compact stencil (OUT_CMP; with an FP64 pipeline optionally also its FP32 copy).
+// TS = storage type of input, boxes and output (double, or float for the FP32-stored
+// operator of the mixed-precision cycles); sums are FP64 either way.
 // Thread = (output node, output neighbour k = blockIdx.y).
 // ---------------------------------------------------------------------------------------
 struct GBox {
   int X, Y, Z;
 };
 
-template <int AX, bool IN_CMP, bool OUT_CMP>
+template <int AX, bool IN_CMP, bool OUT_CMP, class TS>
 __global__ __launch_bounds__(256) void k_gal_axis(LevelView F, const int* __restrict__ fslot,
-                                                  const double* __restrict__ sin, int64_t NSI, GBox bi, GBox bo,
+                                                  const TS* __restrict__ sin, int64_t NSI, GBox bi, GBox bo,
                                                   LevelView C, const int4* __restrict__ clc, int nch,
-                                                  const uint8_t* __restrict__ cfre, double* __restrict__ sout,
+                                                  const uint8_t* __restrict__ cfre, TS* __restrict__ sout,
                                                   float* __restrict__ sout32, int64_t NSO,
                                                   const uint8_t* __restrict__ omask) {
   int o[3];
@@ -1185,16 +1135,16 @@ __global__ __launch_bounds__(256) void k_gal_axis(LevelView F, const int* __rest
         e3[AX] = kin;
         const int ein = (e3[0] + 1) + 3 * (e3[1] + 1) + 9 * (e3[2] + 1);
         const double wgt = wa * (b ? 0.5 : 1.0);
-        const double* sp = sin + (int64_t)ein * 9 * NSI + src;
+        const TS* sp = sin + (int64_t)ein * 9 * NSI + src;
 #pragma unroll
-        for (int e = 0; e < 9; ++e) acc[e] += wgt * sp[e * NSI];
+        for (int e = 0; e < 9; ++e) acc[e] += wgt * (double)sp[e * NSI];
       }
     }
   }
 #pragma unroll
   for (int e = 0; e < 9; ++e) {
-    sout[(int64_t)(k * 9 + e) * NSO + oidx] = acc[e];
-    if (OUT_CMP) sout32[(int64_t)(k * 9 + e) * NSO + oidx] = (float)acc[e];
+    sout[(int64_t)(k * 9 + e) * NSO + oidx] = (TS)acc[e];
+    if (OUT_CMP && sout32) sout32[(int64_t)(k * 9 + e) * NSO + oidx] = (float)acc[e];
   }
 }
 
@@ -1203,9 +1153,10 @@ __global__ __launch_bounds__(256) void k_gal_axis(LevelView F, const int* __rest
 // Coarsest level: dense matrix over the free nodes, then an in-place blocked symmetric
 // sweep (Gauss-Jordan) that leaves -A^{-1}; the V-cycle applies it as a dense matvec.
 // ---------------------------------------------------------------------------------------
+template <class TS>
 __global__ void k_coarse_assemble(LevelView L, const int64_t* __restrict__ cidx, int nnode,
                                   const int* __restrict__ cmap_pad, const int* __restrict__ slot,
-                                  const double* __restrict__ st, int64_t NS, double* __restrict__ A, int n) {
+                                  const TS* __restrict__ st, int64_t NS, double* __restrict__ A, int n) {
   const int64_t t = thread_id();
   if (t >= (int64_t)nnode * 27) return;
   const int pnode = t / 27, k = t % 27;
@@ -1215,7 +1166,7 @@ __global__ void k_coarse_assemble(LevelView L, const int64_t* __restrict__ cidx,
   if (q < 0) return;
   const int si = slot[i];
   for (int r = 0; r < 3; ++r)
-    for (int c = 0; c < 3; ++c) A[(int64_t)(3 * pnode + r) * n + 3 * q + c] = st[(int64_t)(k * 9 + r * 3 + c) * NS + si];
+    for (int c = 0; c < 3; ++c) A[(int64_t)(3 * pnode + r) * n + 3 * q + c] = (double)st[(int64_t)(k * 9 + r * 3 + c) * NS + si];
 }
 
 __global__ void k_symmetrize(double* A, int n, double* __restrict__ diag0) {

@@ -143,7 +143,52 @@ __global__ __launch_bounds__(256) void k_sensitivity(LevelView L, const uint8_t*
 // ---------------------------------------------------------------------------------------
 // Filter (filterSensitivity_kernel grid/Grid.cu:4131-4216): weighted mean over the active
 // elements of a ball; offsets and weights come from a host-built table.
+// k_filter_tile<H> (table offsets within H per axis): a workgroup owns a 64 x 4 x 4
+// element tile, stages it with an H-element halo in LDS as (value, 1) for active
+// elements and (0, 0) otherwise, and every thread forms four outputs from LDS.  Both
+// kernels walk the table in its order with FP64 sums, so they give the same result (an
+// inactive or outside tap adds exactly zero to both sums).  k_filter serves larger radii.
 // ---------------------------------------------------------------------------------------
+constexpr int kFiltTX = 64, kFiltTY = 4, kFiltTZ = 4;
+
+template <int H>
+__global__ __launch_bounds__(256) void k_filter_tile(LevelView L, const uint8_t* __restrict__ eact,
+                                                     const float* __restrict__ src, float* __restrict__ dst,
+                                                     const int* __restrict__ off, const double* __restrict__ wt,
+                                                     int nt) {
+  constexpr int LX = kFiltTX + 2 * H, LY = kFiltTY + 2 * H, LZ = kFiltTZ + 2 * H;
+  __shared__ float2 tile[LZ * LY * LX];
+  const int x0 = blockIdx.x * kFiltTX, y0 = blockIdx.y * kFiltTY, z0 = blockIdx.z * kFiltTZ;
+  for (int i = threadIdx.x; i < LZ * LY * LX; i += 256) {
+    const int gx = x0 + i % LX - H, gy = y0 + (i / LX) % LY - H, gz = z0 + i / (LX * LY) - H;
+    float2 s = make_float2(0.f, 0.f);
+    if (gx >= 0 && gy >= 0 && gz >= 0 && gx < L.ex && gy < L.ey && gz < L.ez) {
+      const int64_t q = L.pelem(gx, gy, gz);
+      if (eact[q]) s = make_float2(src[q], 1.f);
+    }
+    tile[i] = s;
+  }
+  __syncthreads();
+  const int tx = threadIdx.x % kFiltTX, ty = threadIdx.x / kFiltTX;
+  const int x = x0 + tx, y = y0 + ty;
+  if (x >= L.ex || y >= L.ey) return;
+  for (int tz = 0; tz < kFiltTZ && z0 + tz < L.ez; ++tz) {
+    const int64_t e = L.pelem(x, y, z0 + tz);
+    const int c = ((tz + H) * LY + ty + H) * LX + tx + H;
+    if (tile[c].y == 0.f) {
+      dst[e] = 0.f;
+      continue;
+    }
+    double gs = 0, ws = 0;
+    for (int i = 0; i < nt; ++i) {
+      const float2 s = tile[c + off[3 * i] + LX * (off[3 * i + 1] + LY * off[3 * i + 2])];
+      gs += wt[i] * (double)s.x;
+      ws += wt[i] * (double)s.y;
+    }
+    dst[e] = (float)(gs / ws);
+  }
+}
+
 __global__ __launch_bounds__(256) void k_filter(LevelView L, const uint8_t* __restrict__ eact,
                                                 const float* __restrict__ src, float* __restrict__ dst,
                                                 const int* __restrict__ off, const double* __restrict__ wt, int nt) {

@@ -551,10 +551,21 @@ void pack_half_stencils(Ctx& c) {
   c.st16_valid = true;
 }
 
-// precision 2 = FP32 cycle on FP16 stored stencils: make sure the copies are current
-void select_stencils(Ctx& c, int precision) {
+int ensure_fp64_operator(Ctx& c);
+
+// precision 2 = FP32 cycle on FP16 stored stencils: make sure the copies are current;
+// precision 0 = FP64 cycle: make sure the FP64 stencils are (Galerkin precision 1)
+int select_stencils(Ctx& c, int precision) {
   c.half_st = precision == 2;
   if (c.half_st && !c.st16_valid) pack_half_stencils(c);
+  return precision == 0 ? ensure_fp64_operator(c) : TOS_OK;
+}
+
+// coarsest-level factor for a cycle of type T (FP32 cycles use the one of the FP32-built
+// stencils when the operator was built in FP32)
+template <class T>
+const double* coarse_inverse(const Ctx& c) {
+  return (std::is_same<T, double>::value || c.gal_prec == 0) ? c.cinv : c.cinv32;
 }
 
 // Fine-level sweep as a colour-slab wavefront: step t relaxes colour c (in sweep order
@@ -888,7 +899,7 @@ void vcycle_level(Ctx& c, int l, const T* f, T* u, int pre, int post) {
     if (c.cn == 0) return;  // no unknowns on the coarsest level (u stays 0)
     Prof pr(c, P_COARSE);
     hipLaunchKernelGGL((k_coarse_solve<T>), dim3(nblk((int64_t)c.cn * 64)), dim3(256), 0, c.stream, L.v,
-                       (const int64_t*)c.cidx, c.cn, (const double*)c.cinv, f, u);
+                       (const int64_t*)c.cidx, c.cn, coarse_inverse<T>(c), f, u);
     LAUNCH_CHECK();
     return;
   }
@@ -1125,6 +1136,7 @@ struct BatchState {
   bool tuned = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  int graph_gal = -1;  // Galerkin precision the graph was captured with (its coarsest factor)
   std::vector<void*> mem;
 };
 }  // namespace tos
@@ -1283,7 +1295,7 @@ void vcycle_b(Ctx& c, BatchState& B, int l, const float* f, float* u, int pre, i
   if (l == last && last > 0) {
     if (c.cn == 0) return;
     hipLaunchKernelGGL((k_coarse_solve_b<float>), dim3(nblk((int64_t)c.cn * B.K)), dim3(256), 0, c.stream, L.v,
-                       (const int64_t*)c.cidx, c.cn, B.K, (const double*)c.cinv, f, u);
+                       (const int64_t*)c.cidx, c.cn, B.K, coarse_inverse<float>(c), f, u);
     LAUNCH_CHECK();
     return;
   }
@@ -1428,7 +1440,14 @@ int solve_batched(Ctx& c, const tos_solve_opts& o, tos_solve_stats* st) {
                        (const float*)B.u32[0], (const double*)B.R, (const double*)nullptr, B.part);
     foldb(c, B, nb, 1, B_RZ, 0, OPB_NONE);
     LAUNCH_CHECK();
+    if (B.exec && B.graph_gal != c.gal_prec) {
+      (void)hipGraphExecDestroy(B.exec);
+      (void)hipGraphDestroy(B.graph);
+      B.exec = nullptr;
+      B.graph = nullptr;
+    }
     if (!B.exec && c.graph_replay) {
+      B.graph_gal = c.gal_prec;
       c.prof_on = false;
       HIPCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
       batch_iteration(c, B, o.pre, o.post);
@@ -1494,28 +1513,37 @@ void coarse_alloc(Ctx& c) {
   c.cflag = dalloc<int>(c, 1);
 }
 
-int coarse_factor(Ctx& c) {
+// factor of the coarsest level's FP64 stencil into cinv, or (from32) of its FP32 stencil
+// into cinv32
+int coarse_factor(Ctx& c, bool from32) {
   if (c.cn == 0) return TOS_OK;
   Prof pr(c, P_COARSE_FACTOR);
   Level& L = c.L.back();
   const int n = c.cn, nnode = n / 3;
-  hipLaunchKernelGGL(k_zero<double>, dim3(nblk((int64_t)n * n)), dim3(256), 0, c.stream, c.cinv, (int64_t)n * n);
-  hipLaunchKernelGGL(k_coarse_assemble, dim3(nblk((int64_t)nnode * 27)), dim3(256), 0, c.stream, L.v,
-                     (const int64_t*)c.cidx, nnode, (const int*)c.cmap, (const int*)L.slot, (const double*)L.st64,
-                     L.NS, c.cinv, n);
+  if (from32 && !c.cinv32) c.cinv32 = dalloc<double>(c, (size_t)n * n);
+  double* A = from32 ? c.cinv32 : c.cinv;
+  hipLaunchKernelGGL(k_zero<double>, dim3(nblk((int64_t)n * n)), dim3(256), 0, c.stream, A, (int64_t)n * n);
+  if (from32)
+    hipLaunchKernelGGL(k_coarse_assemble<float>, dim3(nblk((int64_t)nnode * 27)), dim3(256), 0, c.stream, L.v,
+                       (const int64_t*)c.cidx, nnode, (const int*)c.cmap, (const int*)L.slot,
+                       (const float*)L.st32, L.NS, A, n);
+  else
+    hipLaunchKernelGGL(k_coarse_assemble<double>, dim3(nblk((int64_t)nnode * 27)), dim3(256), 0, c.stream, L.v,
+                       (const int64_t*)c.cidx, nnode, (const int*)c.cmap, (const int*)L.slot,
+                       (const double*)L.st64, L.NS, A, n);
   double* Dinv = c.cwork;
   double* Cp = Dinv + kGJB * kGJB;
   double* R = Cp + (size_t)n * kGJB;
   double* diag0 = R + (size_t)n * kGJB;
-  hipLaunchKernelGGL(k_symmetrize, dim3(nblk((int64_t)n * n)), dim3(256), 0, c.stream, c.cinv, n, diag0);
+  hipLaunchKernelGGL(k_symmetrize, dim3(nblk((int64_t)n * n)), dim3(256), 0, c.stream, A, n, diag0);
   HIPCHK(hipMemsetAsync(c.cflag, 0, sizeof(int), c.stream));
   for (int K0 = 0; K0 < n; K0 += kGJB) {
     const int b = std::min(kGJB, n - K0);
-    hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(256), 0, c.stream, c.cinv, n, K0, b, Dinv, (const double*)diag0,
+    hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(256), 0, c.stream, A, n, K0, b, Dinv, (const double*)diag0,
                        c.cflag);
-    hipLaunchKernelGGL(k_gj_panel, dim3(nblk((int64_t)n * b)), dim3(256), 0, c.stream, (const double*)c.cinv, n, K0,
+    hipLaunchKernelGGL(k_gj_panel, dim3(nblk((int64_t)n * b)), dim3(256), 0, c.stream, (const double*)A, n, K0,
                        b, (const double*)Dinv, Cp, R);
-    hipLaunchKernelGGL(k_gj_update, dim3(nblk((int64_t)n * n)), dim3(256), 0, c.stream, c.cinv, n, K0, b,
+    hipLaunchKernelGGL(k_gj_update, dim3(nblk((int64_t)n * n)), dim3(256), 0, c.stream, A, n, K0, b,
                        (const double*)Dinv, (const double*)Cp, (const double*)R);
   }
   LAUNCH_CHECK();
@@ -1657,7 +1685,7 @@ int solve_case(Ctx& c, int k, const tos_solve_opts& o, int* its, double* relres)
     if (graphed) {
       for (const Ctx::SolveGraph& g : c.graphs)
         if (g.precision == o.precision && g.pre == o.pre && g.post == o.post && g.sweep == c.sweep_mode &&
-            g.u == u)
+            g.gal == c.gal_prec && g.u == u)
           exec = g.exec;
     }
     bool first = true;
@@ -1667,7 +1695,7 @@ int solve_case(Ctx& c, int k, const tos_solve_opts& o, int* its, double* relres)
         first = false;
       } else if (graphed) {
         if (!exec) {
-          Ctx::SolveGraph g{o.precision, o.pre, o.post, c.sweep_mode, u, nullptr, nullptr};
+          Ctx::SolveGraph g{o.precision, o.pre, o.post, c.sweep_mode, c.gal_prec, u, nullptr, nullptr};
           HIPCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
           head();
           tail();
@@ -2290,8 +2318,9 @@ TOS_EXPORT int tos_get_density(tos_ctx* c, float* rho) {
 
 // update_stencil (grid/Grid.cpp:4167-4178): weights, Galerkin levels, coarsest factor
 // coarse stencil of level C from level F (>= 1) in three axis passes through two boxes:
-// (Xc, Yf, Zf) then (Xc, Yc, Zf)
-void galerkin_by_axes(Ctx& c, Level& F, Level& C) {
+// (Xc, Yf, Zf) then (Xc, Yc, Zf), stored as TS (FP64: also the FP32 copy when copy32)
+template <class TS>
+void galerkin_by_axes(Ctx& c, Level& F, Level& C, bool copy32) {
   const GBox bf{F.v.nx, F.v.ny, F.v.nz};
   const GBox b1{C.v.nx, F.v.ny, F.v.nz}, b2{C.v.nx, C.v.ny, F.v.nz};
   const int64_t n1 = (int64_t)b1.X * b1.Y * b1.Z, n2 = (int64_t)b2.X * b2.Y * b2.Z;
@@ -2321,41 +2350,79 @@ void galerkin_by_axes(Ctx& c, Level& F, Level& C) {
     upload(c.stream, C.gal_m1, m1.data(), n1);
     upload(c.stream, C.gal_m2, m2.data(), n2);
   }
-  hipLaunchKernelGGL((k_gal_axis<0, true, false>), dim3(nblk(n1), 27), dim3(256), 0, c.stream, F.v,
-                     (const int*)F.slot, (const double*)F.st64, F.NS, bf, b1, C.v, (const int4*)nullptr, 0,
-                     (const uint8_t*)nullptr, C.gal_b1, (float*)nullptr, n1, (const uint8_t*)C.gal_m1);
-  hipLaunchKernelGGL((k_gal_axis<1, false, false>), dim3(nblk(n2), 27), dim3(256), 0, c.stream, F.v,
-                     (const int*)nullptr, (const double*)C.gal_b1, n1, b1, b2, C.v, (const int4*)nullptr, 0,
-                     (const uint8_t*)nullptr, C.gal_b2, (float*)nullptr, n2, (const uint8_t*)C.gal_m2);
-  hipLaunchKernelGGL((k_gal_axis<2, false, true>), dim3(cblk(C.n_col), 27), dim3(256), 0, c.stream, F.v,
-                     (const int*)nullptr, (const double*)C.gal_b2, n2, b2, GBox{C.v.nx, C.v.ny, C.v.nz}, C.v,
-                     (const int4*)C.cl_col, C.n_col, (const uint8_t*)C.fre, C.st64, C.st32, C.NS,
-                     (const uint8_t*)nullptr);
+  constexpr bool d64 = std::is_same<TS, double>::value;
+  const TS* fin = d64 ? (const TS*)F.st64 : (const TS*)F.st32;
+  TS* outp = d64 ? (TS*)C.st64 : (TS*)C.st32;
+  TS* b1p = (TS*)C.gal_b1;  // (sized for FP64 entries)
+  TS* b2p = (TS*)C.gal_b2;
+  hipLaunchKernelGGL((k_gal_axis<0, true, false, TS>), dim3(nblk(n1), 27), dim3(256), 0, c.stream, F.v,
+                     (const int*)F.slot, fin, F.NS, bf, b1, C.v, (const int4*)nullptr, 0, (const uint8_t*)nullptr,
+                     b1p, (float*)nullptr, n1, (const uint8_t*)C.gal_m1);
+  hipLaunchKernelGGL((k_gal_axis<1, false, false, TS>), dim3(nblk(n2), 27), dim3(256), 0, c.stream, F.v,
+                     (const int*)nullptr, (const TS*)b1p, n1, b1, b2, C.v, (const int4*)nullptr, 0,
+                     (const uint8_t*)nullptr, b2p, (float*)nullptr, n2, (const uint8_t*)C.gal_m2);
+  hipLaunchKernelGGL((k_gal_axis<2, false, true, TS>), dim3(cblk(C.n_col), 27), dim3(256), 0, c.stream, F.v,
+                     (const int*)nullptr, (const TS*)b2p, n2, b2, GBox{C.v.nx, C.v.ny, C.v.nz}, C.v,
+                     (const int4*)C.cl_col, C.n_col, (const uint8_t*)C.fre, outp,
+                     (d64 && copy32) ? C.st32 : (float*)nullptr, C.NS, (const uint8_t*)nullptr);
   LAUNCH_CHECK();
 }
+
+// Galerkin stencils of every level >= 1 stored as TS; an FP64 build also writes the FP32
+// copies when copy32.  Level 1 comes from the element matrices (FP64 sums either way).
+template <class TS>
+void galerkin_levels(Ctx& c, bool copy32) {
+  constexpr bool d64 = std::is_same<TS, double>::value;
+  for (size_t l = 1; l < c.L.size(); ++l) {
+    Level& F = c.L[l - 1];
+    Level& C = c.L[l];
+    if (C.n_col == 0) continue;
+    if (l == 1)
+      hipLaunchKernelGGL(k_galerkin_first_row, dim3(cblk(C.n_col), 9), dim3(256), 0, c.stream, F.v, C.v,
+                         (const int4*)C.cl_col, C.n_col, (const uint8_t*)F.fre, (const uint8_t*)c.allfree,
+                         (const uint8_t*)C.fre, (const double*)c.w64, (const double*)c.Ke64,
+                         (const double*)c.Mc64, d64 ? C.st64 : (double*)nullptr,
+                         (!d64 || copy32) ? C.st32 : (float*)nullptr, C.NS);
+    else
+      galerkin_by_axes<TS>(c, F, C, copy32);
+    LAUNCH_CHECK();
+  }
+}
+
+namespace {
+// Galerkin precision 1: the FP64 stencils and their coarsest factor, built when first
+// needed after an operator update (the FP32 stencils and their factor stay as built)
+int ensure_fp64_operator(Ctx& c) {
+  if (c.st64_valid) return TOS_OK;
+  {
+    Prof pr(c, P_GALERKIN);
+    galerkin_levels<double>(c, false);
+  }
+  const int rc = coarse_factor(c, false);
+  if (rc) return rc;
+  c.st64_valid = true;
+  return TOS_OK;
+}
+}  // namespace
 
 TOS_EXPORT int tos_update_operator(tos_ctx* c) {
   return guard(c, [&]() {
     const LevelView& v0 = c->L[0].v;
     c->st16_valid = false;
-    Prof pr(*c, P_GALERKIN);
-    hipLaunchKernelGGL(k_weights, dim3(nblk(v0.NE)), dim3(256), 0, c->stream, v0, (const uint8_t*)c->eact,
-                       (const float*)c->rho, c->penal, c->w64, c->w32);
-    for (size_t l = 1; l < c->L.size(); ++l) {
-      Level& F = c->L[l - 1];
-      Level& C = c->L[l];
-      if (C.n_col == 0) continue;
-      if (l == 1)
-        hipLaunchKernelGGL(k_galerkin_first_row, dim3(cblk(C.n_col), 9), dim3(256), 0, c->stream, F.v, C.v,
-                           (const int4*)C.cl_col, C.n_col, (const uint8_t*)F.fre, (const uint8_t*)c->allfree,
-                           (const uint8_t*)C.fre, (const double*)c->w64, (const double*)c->Ke64,
-                           (const double*)c->Mc64, C.st64, C.st32, C.NS);
+    c->st64_valid = false;
+    c->op_ready = false;
+    {
+      Prof pr(*c, P_GALERKIN);
+      hipLaunchKernelGGL(k_weights, dim3(nblk(v0.NE)), dim3(256), 0, c->stream, v0, (const uint8_t*)c->eact,
+                         (const float*)c->rho, c->penal, c->w64, c->w32);
+      if (c->gal_prec == 0)
+        galerkin_levels<double>(*c, true);
       else
-        galerkin_by_axes(*c, F, C);  // (both write the FP64 and FP32 stencils)
-      LAUNCH_CHECK();
+        galerkin_levels<float>(*c, false);
     }
-    const int rc = coarse_factor(*c);
+    const int rc = coarse_factor(*c, c->gal_prec != 0);
     if (rc) return rc;
+    c->st64_valid = c->gal_prec == 0;
     if (c->learn && c->learn->lrho) learn_logrho(*c);
     HIPCHK(hipStreamSynchronize(c->stream));
     c->op_ready = true;
@@ -2367,6 +2434,8 @@ TOS_EXPORT int tos_get_stencil(tos_ctx* c, int level, double* out) {
   return guard(c, [&]() {
     REQUIRE(out && level >= 1 && level < (int)c->L.size(), "bad level");
     if (!c->op_ready) { c->err = "update_operator first"; return TOS_E_STATE; }
+    const int rc = ensure_fp64_operator(*c);
+    if (rc) return rc;
     const Level& L = c->L[level];
     const LevelView& v = L.v;
     std::vector<double> h((size_t)243 * std::max<int64_t>(L.NS, 1));
@@ -2402,7 +2471,8 @@ TOS_EXPORT int tos_vcycle(tos_ctx* c, const double* r, double* z, int precision,
   return guard(c, [&]() {
     REQUIRE(r && z && pre >= 0 && post >= 0 && precision >= 0 && precision <= 2, "bad arguments");
     if (!c->op_ready) { c->err = "update_operator first"; return TOS_E_STATE; }
-    select_stencils(*c, precision);
+    const int rs = select_stencils(*c, precision);
+    if (rs) return rs;
     Level& L0 = c->L[0];
     const int64_t n = (int64_t)L0.v.nx * L0.v.ny * L0.v.nz;
     double* st = (double*)stage(*c, 3 * n * sizeof(double));
@@ -2420,7 +2490,8 @@ TOS_EXPORT int tos_solve(tos_ctx* c, const tos_solve_opts* o, tos_solve_stats* s
                 o->precision <= 2,
             "bad options");
     if (!c->op_ready) { c->err = "update_operator first"; return TOS_E_STATE; }
-    select_stencils(*c, o->precision);
+    const int rs = select_stencils(*c, o->precision);
+    if (rs) return rs;
     const auto t0 = std::chrono::steady_clock::now();
     int rc_all = TOS_OK;
     for (int k = 0; k < c->K; ++k) {
@@ -2604,8 +2675,21 @@ TOS_EXPORT int tos_filter(tos_ctx* c, int which, double R) {
     HIPCHK(hipMemcpyAsync(c->tmpf, b, v.NE * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
     const int64_t n = (int64_t)v.ex * v.ey * v.ez;
     Prof pr(*c, P_SENS_FILTER);
-    hipLaunchKernelGGL(k_filter, dim3(nblk(n)), dim3(256), 0, c->stream, v, (const uint8_t*)c->eact,
-                       (const float*)c->tmpf, b, (const int*)doff, (const double*)dwt, nt);
+    const dim3 tg((v.ex + kFiltTX - 1) / kFiltTX, (v.ey + kFiltTY - 1) / kFiltTY, (v.ez + kFiltTZ - 1) / kFiltTZ);
+    const uint8_t* ea = (const uint8_t*)c->eact;
+    const float* sb = (const float*)c->tmpf;
+    if (Ri <= 1)
+      hipLaunchKernelGGL(k_filter_tile<1>, tg, dim3(256), 0, c->stream, v, ea, sb, b, (const int*)doff,
+                         (const double*)dwt, nt);
+    else if (Ri == 2)
+      hipLaunchKernelGGL(k_filter_tile<2>, tg, dim3(256), 0, c->stream, v, ea, sb, b, (const int*)doff,
+                         (const double*)dwt, nt);
+    else if (Ri == 3)
+      hipLaunchKernelGGL(k_filter_tile<3>, tg, dim3(256), 0, c->stream, v, ea, sb, b, (const int*)doff,
+                         (const double*)dwt, nt);
+    else
+      hipLaunchKernelGGL(k_filter, dim3(nblk(n)), dim3(256), 0, c->stream, v, ea, sb, b, (const int*)doff,
+                         (const double*)dwt, nt);
     LAUNCH_CHECK();
     HIPCHK(hipStreamSynchronize(c->stream));  // the table lives in the staging buffer
     return TOS_OK;
@@ -2988,6 +3072,17 @@ TOS_EXPORT int tos_ctx_mma_step(tos_ctx* c, tos_mma* M, double sscale, double vs
 TOS_EXPORT int tos_set_graph_replay(tos_ctx* c, int enable) {
   return guard(c, [&]() {
     c->graph_replay = enable != 0;
+    return TOS_OK;
+  });
+}
+
+TOS_EXPORT int tos_set_galerkin_precision(tos_ctx* c, int prec) {
+  return guard(c, [&]() {
+    REQUIRE(prec == 0 || prec == 1, "bad Galerkin precision");
+    if (prec != c->gal_prec) {
+      c->gal_prec = prec;
+      c->op_ready = false;
+    }
     return TOS_OK;
   });
 }

@@ -159,7 +159,8 @@ struct Ctx {
   // coarsest level
   int cn = 0;                   // dense size (3 * free nodes)
   int64_t* cidx = nullptr;      // padded node index of each coarse unknown node
-  double* cinv = nullptr;       // -A^{-1} after the sweep (cn x cn)
+  double* cinv = nullptr;       // -A^{-1} after the sweep (cn x cn), from the FP64 stencils
+  double* cinv32 = nullptr;     // the same from the FP32-built stencils (Galerkin precision 1)
   double* cwork = nullptr;      // panel buffers
   int* cflag = nullptr;         // singular pivot flag
   int* cmap = nullptr;          // padded coarsest node -> compact node index or -1
@@ -196,11 +197,15 @@ struct Ctx {
   BatchState* batch = nullptr;
   bool half_st = false;         // the running FP32 cycle reads the FP16 stencils
   bool st16_valid = false;      // FP16 stencils match the current operator
+  int gal_prec = 0;             // coarse operators: 0 built and stored in FP64 (FP32 copies
+                                // rounded from them); 1 built into FP32 storage, FP64 built
+                                // on first demand (FP64 cycles, tos_get_stencil)
+  bool st64_valid = false;      // FP64 stencils and cinv match the current operator
   int sweep_mode = 3;           // fine GS: 0 colour passes, 1 colour-slab wavefront, 2 row pairs,
                                 // 3 in-wave pairs (FP32 cycles)
   bool graph_replay = true;     // replay the captured FCG iteration body
   struct SolveGraph {           // captured FCG iteration body per solve configuration
-    int precision, pre, post, sweep;
+    int precision, pre, post, sweep, gal;
     const double* u;
     hipGraph_t graph;
     hipGraphExec_t exec;
