@@ -996,13 +996,23 @@ __device__ __forceinline__ void gal_elem_row(const LevelView& F, const LevelView
   const int Ex = X - 1 + AX, Ey = Y - 1 + ay, Ez = Z - 1 + az;
   if (Ex < 0 || Ey < 0 || Ez < 0 || Ex >= C.ex || Ey >= C.ey || Ez >= C.ez) return;
   const int mI = 7 - (AX + 2 * ay + 4 * az);
+  // the eight children's weights and flags are loaded together before any is used
+  double wc[8];
+  uint8_t af[8];
+#pragma unroll
   for (int ch = 0; ch < 8; ++ch) {
     const int fx = 2 * Ex + (ch & 1), fy = 2 * Ey + ((ch >> 1) & 1), fz = 2 * Ez + (ch >> 2);
-    if (fx >= F.ex || fy >= F.ey || fz >= F.ez) continue;
-    const int64_t fe = F.pelem(fx, fy, fz);
-    const double we = w[fe];
+    const bool in = fx < F.ex && fy < F.ey && fz < F.ez;
+    const int64_t fe = F.pelem(in ? fx : 0, in ? fy : 0, in ? fz : 0);
+    wc[ch] = w[fe];
+    af[ch] = fallfree[fe];
+    if (!in) wc[ch] = 0;
+  }
+#pragma unroll
+  for (int ch = 0; ch < 8; ++ch) {
+    const double we = wc[ch];
     if (we == 0) continue;
-    if (fallfree[fe]) {
+    if (af[ch]) {
       const double* m0 = Mc + ch * 576 + (3 * mI) * 24 + 3 * (2 * jy + 4 * jz);
 #pragma unroll
       for (int jx = 0; jx < 2; ++jx) {
@@ -1013,6 +1023,7 @@ __device__ __forceinline__ void gal_elem_row(const LevelView& F, const LevelView
           for (int c = 0; c < 3; ++c) acc[jx + AX][r * 3 + c] += we * m[r * 24 + c];
       }
     } else {  // a fine node of the child is eliminated: rare, out of line
+      const int fx = 2 * Ex + (ch & 1), fy = 2 * Ey + ((ch >> 1) & 1), fz = 2 * Ez + (ch >> 2);
       const GalPair t = gal_row_partial(F, ffre, Ke, fx, fy, fz, ch, mI, jy, jz, we);
 #pragma unroll
       for (int jx = 0; jx < 2; ++jx)
