@@ -1193,38 +1193,55 @@ __global__ void k_symmetrize(double* A, int n, double* __restrict__ diag0) {
 
 constexpr int kGJB = 32;
 
-// Invert the diagonal block K (size b) by a scalar sweep in LDS; copy row panel R.
+// Invert the diagonal block K (size b <= 32) by a scalar sweep, one wave: lane l keeps
+// the elements (i, j) = ((l >> 5) + 2 m, l & 31), m < 16, in registers and publishes them
+// in LDS after every step, where the next step reads its pivot row and column (one wave
+// issues its LDS operations in order, so no barrier is needed).
 // A pivot (Schur complement) below 1e-12 of the unswept diagonal marks the system as
 // singular -- a floating part or a structure without supports.
-__global__ __launch_bounds__(256) void k_gj_pivot(double* __restrict__ A, int n, int K0, int b,
-                                                  double* __restrict__ Dinv, const double* __restrict__ diag0,
-                                                  int* __restrict__ flag) {
+__global__ __launch_bounds__(64) void k_gj_pivot(double* __restrict__ A, int n, int K0, int b,
+                                                 double* __restrict__ Dinv, const double* __restrict__ diag0,
+                                                 int* __restrict__ flag) {
+  constexpr int M = kGJB / 2;
   __shared__ double S[kGJB][kGJB + 1];
-  __shared__ double T[kGJB][kGJB + 1];
-  __shared__ double dg[kGJB];
-  for (int t = threadIdx.x; t < b * b; t += blockDim.x) S[t / b][t % b] = A[(int64_t)(K0 + t / b) * n + K0 + t % b];
-  if (threadIdx.x < b) dg[threadIdx.x] = diag0[K0 + threadIdx.x];
-  __syncthreads();
+  const int l = threadIdx.x, j = l & 31, i0 = l >> 5;
+  double v[M];
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+    const int i = i0 + 2 * m;
+    v[m] = (i < b && j < b) ? A[(int64_t)(K0 + i) * n + K0 + j] : 0.0;
+    S[i][j] = v[m];
+  }
+  __builtin_amdgcn_wave_barrier();
   for (int k = 0; k < b; ++k) {
-    const double piv = S[k][k];  // same LDS words for every thread: the branch is uniform
-    if (!(piv > 1e-12 * dg[k])) {
-      if (threadIdx.x == 0) *flag = 1;
+    const double piv = S[k][k];  // same LDS word for every lane: the branch is uniform
+    if (!(piv > 1e-12 * diag0[K0 + k])) {
+      if (l == 0) *flag = 1;
       return;
     }
     const double dd = 1.0 / piv;
-    for (int t = threadIdx.x; t < b * b; t += blockDim.x) {
-      const int i = t / b, j = t % b;
-      double v = S[i][j];
-      if (i != k && j != k) v = v - S[i][k] * S[k][j] * dd;
-      else if (i == k && j == k) v = -dd;
-      else v = v * dd;
-      T[i][j] = v;
+    const double skj = S[k][j];
+    double sik[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) sik[m] = S[i0 + 2 * m][k];
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      const int i = i0 + 2 * m;
+      double w = v[m];
+      if (i != k && j != k) w = w - sik[m] * skj * dd;
+      else if (i == k && j == k) w = -dd;
+      else w = w * dd;
+      v[m] = w;
+      S[i][j] = w;
     }
-    __syncthreads();
-    for (int t = threadIdx.x; t < b * b; t += blockDim.x) S[t / b][t % b] = T[t / b][t % b];
-    __syncthreads();
+    __builtin_amdgcn_wave_barrier();
   }
-  for (int t = threadIdx.x; t < b * b; t += blockDim.x) Dinv[t] = -S[t / b][t % b];  // +A_KK^{-1}
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+    const int i = i0 + 2 * m;
+    if (i < b && j < b) Dinv[i * b + j] = -v[m];  // +A_KK^{-1}
+  }
 }
 
 // C = A[:, K] * Dinv  (n x b), and R = A[K, :] (b x n) before the update overwrites it

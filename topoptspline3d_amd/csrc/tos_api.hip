@@ -1539,7 +1539,7 @@ int coarse_factor(Ctx& c, bool from32) {
   HIPCHK(hipMemsetAsync(c.cflag, 0, sizeof(int), c.stream));
   for (int K0 = 0; K0 < n; K0 += kGJB) {
     const int b = std::min(kGJB, n - K0);
-    hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(256), 0, c.stream, A, n, K0, b, Dinv, (const double*)diag0,
+    hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(64), 0, c.stream, A, n, K0, b, Dinv, (const double*)diag0,
                        c.cflag);
     hipLaunchKernelGGL(k_gj_panel, dim3(nblk((int64_t)n * b)), dim3(256), 0, c.stream, (const double*)A, n, K0,
                        b, (const double*)Dinv, Cp, R);
