@@ -127,13 +127,14 @@ __global__ __launch_bounds__(256) void k_sensitivity(LevelView L, const uint8_t*
 #pragma unroll
     for (int c = 0; c < 3; ++c) ue[3 * j + c] = u[c * L.NP + q];
   }
+  // u^T Ke u over the upper triangle (Ke is symmetric): sum_i u_i (Ke_ii u_i + 2 sum_{j>i} Ke_ij u_j)
   double en = 0;
 #pragma unroll
   for (int i = 0; i < 24; ++i) {
     double s = 0;
 #pragma unroll
-    for (int j = 0; j < 24; ++j) s += Ke[i * 24 + j] * ue[j];
-    en += ue[i] * s;
+    for (int j = i + 1; j < 24; ++j) s += Ke[i * 24 + j] * ue[j];
+    en += ue[i] * (Ke[i * 25] * ue[i] + 2.0 * s);
   }
   const double r = rho[e];
   if (MODE == 0) sens[e] = (float)(-penal * pow(r, penal - 1.0) * en);
@@ -157,16 +158,22 @@ __global__ __launch_bounds__(256) void k_filter_tile(LevelView L, const uint8_t*
                                                      const int* __restrict__ off, const double* __restrict__ wt,
                                                      int nt) {
   constexpr int LX = kFiltTX + 2 * H, LY = kFiltTY + 2 * H, LZ = kFiltTZ + 2 * H;
+  constexpr int NT = (2 * H + 1) * (2 * H + 1) * (2 * H + 1);  // table size bound
   __shared__ float2 tile[LZ * LY * LX];
+  __shared__ int toff[NT];  // tap offsets within the tile
+  __shared__ double twt[NT];
   const int x0 = blockIdx.x * kFiltTX, y0 = blockIdx.y * kFiltTY, z0 = blockIdx.z * kFiltTZ;
+  for (int i = threadIdx.x; i < nt; i += 256) {
+    toff[i] = off[3 * i] + LX * (off[3 * i + 1] + LY * off[3 * i + 2]);
+    twt[i] = wt[i];
+  }
   for (int i = threadIdx.x; i < LZ * LY * LX; i += 256) {
     const int gx = x0 + i % LX - H, gy = y0 + (i / LX) % LY - H, gz = z0 + i / (LX * LY) - H;
-    float2 s = make_float2(0.f, 0.f);
-    if (gx >= 0 && gy >= 0 && gz >= 0 && gx < L.ex && gy < L.ey && gz < L.ez) {
-      const int64_t q = L.pelem(gx, gy, gz);
-      if (eact[q]) s = make_float2(src[q], 1.f);
-    }
-    tile[i] = s;
+    const bool in = gx >= 0 && gy >= 0 && gz >= 0 && gx < L.ex && gy < L.ey && gz < L.ez;
+    const int64_t q = L.pelem(in ? gx : 0, in ? gy : 0, in ? gz : 0);
+    const uint8_t a = eact[q];  // both loads issue before either is used
+    const float v = src[q];
+    tile[i] = (in && a) ? make_float2(v, 1.f) : make_float2(0.f, 0.f);
   }
   __syncthreads();
   const int tx = threadIdx.x % kFiltTX, ty = threadIdx.x / kFiltTX;
@@ -181,9 +188,9 @@ __global__ __launch_bounds__(256) void k_filter_tile(LevelView L, const uint8_t*
     }
     double gs = 0, ws = 0;
     for (int i = 0; i < nt; ++i) {
-      const float2 s = tile[c + off[3 * i] + LX * (off[3 * i + 1] + LY * off[3 * i + 2])];
-      gs += wt[i] * (double)s.x;
-      ws += wt[i] * (double)s.y;
+      const float2 s = tile[c + toff[i]];
+      gs += twt[i] * (double)s.x;
+      ws += twt[i] * (double)s.y;
     }
     dst[e] = (float)(gs / ws);
   }
@@ -432,17 +439,28 @@ __global__ __launch_bounds__(256) void k_spline_eval_o(LevelView L, const uint8_
   rho[e] = (float)r;
 }
 
-// g' = g on active, non-shell elements inside the knot range, else 0 (unpadded, FP64)
-__global__ void k_spline_mask(LevelView L, const uint8_t* __restrict__ eact, const uint8_t* __restrict__ shell,
-                              const float* __restrict__ g, const uint8_t* __restrict__ okx,
-                              const uint8_t* __restrict__ oky, const uint8_t* __restrict__ okz,
-                              double* __restrict__ out) {
+// First adjoint stage (x) read straight from the padded element field g: the masked
+// value (active, outside the shell, y and z rows inside the spline support; x is masked
+// by the CSR list, which holds only supported elements) of element (x, y, z) is
+// g[e] or 0, the same FP64 sum as a masked copy followed by k_spline_stage axis 0.
+__global__ void k_spline_stage0(LevelView L, const uint8_t* __restrict__ eact, const uint8_t* __restrict__ shell,
+                                const float* __restrict__ g, const uint8_t* __restrict__ oky,
+                                const uint8_t* __restrict__ okz, double* __restrict__ out, int n0,
+                                const int* __restrict__ cptr, const int* __restrict__ celm,
+                                const float* __restrict__ cw) {
   const int64_t t = tid64();
-  if (t >= (int64_t)L.ex * L.ey * L.ez) return;
-  const int x = t % L.ex, y = (t / L.ex) % L.ey, z = t / ((int64_t)L.ex * L.ey);
-  const int64_t e = L.pelem(x, y, z);
-  const bool ok = eact[e] && !shell[e] && okx[x] && oky[y] && okz[z];
-  out[t] = ok ? (double)g[e] : 0.0;
+  if (t >= (int64_t)n0 * L.ey * L.ez) return;
+  const int c = t % n0, y = (t / n0) % L.ey, z = t / ((int64_t)n0 * L.ey);
+  double acc = 0;
+  if (oky[y] && okz[z]) {
+    const int64_t row = L.pelem(0, y, z);
+    for (int q = cptr[c]; q < cptr[c + 1]; ++q) {
+      const int64_t e = row + celm[q];
+      const double v = (eact[e] && !shell[e]) ? (double)g[e] : 0.0;
+      acc += (double)cw[q] * v;
+    }
+  }
+  out[t] = acc;
 }
 
 // One separable adjoint stage along axis a of a 3-D array with dims (d0, d1, d2) (x

@@ -2794,7 +2794,7 @@ TOS_EXPORT int tos_spline_init(tos_ctx* c, int mode, int order, int param) {
     c->coef_old = dalloc<float>(*c, n);
     c->csens = dalloc<float>(*c, n);
     c->cvsens = dalloc<float>(*c, n);
-    c->sp_g = dalloc<double>(*c, std::max((size_t)c->nx * c->ny * c->nz, n));  // also holds the last stage
+    c->sp_g = dalloc<double>(*c, n);  // last adjoint stage
     c->sp_t1 = dalloc<double>(*c, (size_t)c->nc[0] * c->ny * c->nz);
     c->sp_t2 = dalloc<double>(*c, std::max((size_t)c->nc[0] * c->nc[1] * c->nz, n));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -2885,16 +2885,13 @@ TOS_EXPORT int tos_spline_adjoint(tos_ctx* c, int which) {
     float* b = sens_buf(*c, which);
     REQUIRE(b && c->sp_mode >= 0, "spline not initialised");
     const LevelView& v = c->L[0].v;
-    const int64_t n = (int64_t)v.ex * v.ey * v.ez;
     Prof pr(*c, P_SPLINE);
-    hipLaunchKernelGGL(k_spline_mask, dim3(nblk(n)), dim3(256), 0, c->stream, v, (const uint8_t*)c->eact,
-                       (const uint8_t*)c->shell, (const float*)b, (const uint8_t*)c->sp_ok[0],
-                       (const uint8_t*)c->sp_ok[1], (const uint8_t*)c->sp_ok[2], c->sp_g);
     const int n0 = c->nc[0], n1 = c->nc[1], n2 = c->nc[2];
     int64_t m1 = (int64_t)n0 * v.ey * v.ez, m2 = (int64_t)n0 * n1 * v.ez, m3 = (int64_t)n0 * n1 * n2;
-    hipLaunchKernelGGL(k_spline_stage, dim3(nblk(m1)), dim3(256), 0, c->stream, (const double*)c->sp_g, c->sp_t1, 0,
-                       n0, v.ey, v.ez, v.ex, (const int*)c->sp_cptr[0], (const int*)c->sp_celm[0],
-                       (const float*)c->sp_cw[0]);
+    hipLaunchKernelGGL(k_spline_stage0, dim3(nblk(m1)), dim3(256), 0, c->stream, v, (const uint8_t*)c->eact,
+                       (const uint8_t*)c->shell, (const float*)b, (const uint8_t*)c->sp_ok[1],
+                       (const uint8_t*)c->sp_ok[2], c->sp_t1, n0, (const int*)c->sp_cptr[0],
+                       (const int*)c->sp_celm[0], (const float*)c->sp_cw[0]);
     hipLaunchKernelGGL(k_spline_stage, dim3(nblk(m2)), dim3(256), 0, c->stream, (const double*)c->sp_t1, c->sp_t2, 1,
                        n0, n1, v.ez, v.ey, (const int*)c->sp_cptr[1], (const int*)c->sp_celm[1],
                        (const float*)c->sp_cw[1]);

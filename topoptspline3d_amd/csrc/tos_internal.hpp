@@ -182,7 +182,7 @@ struct Ctx {
   float* cvsens = nullptr;
   double* sp_t1 = nullptr;  // separable adjoint stages
   double* sp_t2 = nullptr;
-  double* sp_g = nullptr;   // masked element gradient (unpadded, double)
+  double* sp_g = nullptr;   // last adjoint stage (per coefficient, FP64)
   std::vector<void*> allocs;
   // kernel timing (HIP events on `stream`), see tos_profile
   bool prof_on = false;
