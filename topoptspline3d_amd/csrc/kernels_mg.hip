@@ -27,10 +27,9 @@ struct NodeRef {
   int x, y, z, slot;
 };
 
-// node of this lane in chunk list `cl` (stride s); false for idle lanes
-__device__ __forceinline__ bool chunk_node(const int4* __restrict__ cl, int nch, int s, NodeRef& n) {
-  const int ch = (int)xcd_block() * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
+// node of lane `lane` of chunk `ch` of list `cl` (stride s); false for idle lanes
+__device__ __forceinline__ bool chunk_node_at(const int4* __restrict__ cl, int nch, int ch, int lane, int s,
+                                              NodeRef& n) {
   if (ch >= nch) return false;
   const int4 c = cl[ch];
   if (lane >= (c.y >> 16)) return false;
@@ -40,6 +39,11 @@ __device__ __forceinline__ bool chunk_node(const int4* __restrict__ cl, int nch,
   n.p = (int64_t)c.x + (int64_t)s * lane;
   n.slot = ch * 64 + lane;
   return true;
+}
+
+// node of this lane in chunk list `cl` (stride s); false for idle lanes
+__device__ __forceinline__ bool chunk_node(const int4* __restrict__ cl, int nch, int s, NodeRef& n) {
+  return chunk_node_at(cl, nch, (int)xcd_block() * 4 + (threadIdx.x >> 6), threadIdx.x & 63, s, n);
 }
 
 // block (256 threads) sum written to partial[blockIdx.x + k*gridDim.x]; all threads call
@@ -566,12 +570,27 @@ __global__ TOS_FINE_BOUNDS(T) void k_fine_gs_wf(LevelView L, const int4* __restr
 // Stored-stencil levels (colour-compact stencil, NS slots per entry)
 // ---------------------------------------------------------------------------------------
 // Readers of the compact stencil: entry e (3x3 row-major) of block k of the node in slot s.
+// row[i] of a stored-stencil row through a buffer resource: the row base is uniform
+// (kernel arguments and unrolled constants), so the address needs no per-lane 64-bit
+// arithmetic (scalar base, 32-bit lane offset; slot counts stay far below 2^30)
+template <class T>
+__device__ __forceinline__ T ld_row(const T* row, uint32_t i) {
+  const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(row), (short)0, 0x7fffffff,
+                                                                      0x00020000);
+  if constexpr (sizeof(T) == 8)
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, i * 8u, 0, 0));
+  else
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, i * 4u, 0, 0));
+}
+
 template <class T>
 struct StFull {  // T entries at (9k + e) NS + s
   const T* __restrict__ p;
   int64_t NS;
   static constexpr bool kScaled = false;
-  __device__ __forceinline__ T get(int64_t s, int k, int e) const { return p[(int64_t)(k * 9 + e) * NS + s]; }
+  __device__ __forceinline__ T get(int64_t s, int k, int e) const {
+    return ld_row(p + (int64_t)(k * 9 + e) * NS, (uint32_t)s);
+  }
   __device__ __forceinline__ T scale(int64_t) const { return T(1); }
 };
 
@@ -595,7 +614,7 @@ struct StHalf {
   __device__ __forceinline__ float get(int64_t s, int k, int e) const {
     const int w = e < 8 ? 4 * k + (e >> 1) : 108 + (k >> 1);
     const int hi = e < 8 ? (e & 1) : (k & 1);
-    const uint32_t v = p[(int64_t)w * NS + s];
+    const uint32_t v = ld_row(p + (int64_t)w * NS, (uint32_t)s);
     return half_bits(hi ? (v >> 16) : (v & 0xffffu));
   }
   __device__ __forceinline__ float scale(int64_t s) const { return msc[s]; }
@@ -630,14 +649,11 @@ __global__ __launch_bounds__(256) void k_st_pack16(const int4* __restrict__ cl, 
         to_half_bits(v[2 * t * 9 + 8] * inv) | (2 * t + 1 < 27 ? to_half_bits(v[(2 * t + 1) * 9 + 8] * inv) << 16 : 0u);
 }
 
-// y = K u (MODE 0) or f - K u (MODE 1) over the whole colour-ordered list
+// y = K u (MODE 0) or f - K u (MODE 1) at node p with stencil slot s
 template <class T, int MODE, class ST>
-__global__ __launch_bounds__(256) void k_st_apply(LevelView L, const int4* __restrict__ cl, int nch, ST st,
-                                                  const T* __restrict__ u, const T* __restrict__ f,
-                                                  T* __restrict__ y) {
-  NodeRef n;
-  if (!chunk_node(cl, nch, 2, n)) return;
-  const int64_t p = n.p, s = n.slot;
+__device__ __forceinline__ void st_apply_node(const LevelView& L, int64_t p, int64_t s, const ST& st,
+                                              const T* __restrict__ u, const T* __restrict__ f,
+                                              T* __restrict__ y) {
   T a0 = 0, a1 = 0, a2 = 0;
 #pragma unroll
   for (int k = 0; k < 27; ++k) {
@@ -656,6 +672,16 @@ __global__ __launch_bounds__(256) void k_st_apply(LevelView L, const int4* __res
   } else {
     y[p] = f[p] - a0; y[L.NP + p] = f[L.NP + p] - a1; y[2 * L.NP + p] = f[2 * L.NP + p] - a2;
   }
+}
+
+// y = K u (MODE 0) or f - K u (MODE 1) over the whole colour-ordered list
+template <class T, int MODE, class ST>
+__global__ __launch_bounds__(256) void k_st_apply(LevelView L, const int4* __restrict__ cl, int nch, ST st,
+                                                  const T* __restrict__ u, const T* __restrict__ f,
+                                                  T* __restrict__ y) {
+  NodeRef n;
+  if (!chunk_node(cl, nch, 2, n)) return;
+  st_apply_node<T, MODE>(L, n.p, n.slot, st, u, f, y);
 }
 
 // Relax one node from its loaded neighbourhood un (k = (dx+1) + 3 (dy+1) + 9 (dz+1)) and
@@ -684,14 +710,11 @@ __device__ __forceinline__ void st_relax(const ST& st, int64_t slot, const T (&u
   block_relax<T, BACKWARD>(f0 - s0, f1 - s1, f2 - s2, D, v0, v1, v2);
 }
 
-// one colour: cl points at the colour's chunks, slot0 = 64 * (their offset in the list)
+// relax node p (stencil slot `slot`) of colour ZC (-1: any colour, all neighbours read)
 template <class T, bool BACKWARD, int ZC, class ST>
-__global__ __launch_bounds__(256) void k_st_gs(LevelView L, const int4* __restrict__ cl, int nch, int64_t slot0,
-                                               ST st, T* __restrict__ u, const T* __restrict__ f) {
+__device__ __forceinline__ void st_gs_node(const LevelView& L, int64_t p, int64_t slot, const ST& st,
+                                           T* __restrict__ u, const T* __restrict__ f) {
   constexpr unsigned mask = nb_mask_of(ZC);
-  NodeRef n;
-  if (!chunk_node(cl, nch, 2, n)) return;
-  const int64_t p = n.p, slot = slot0 + n.slot;
   T un[27][3];
 #pragma unroll
   for (int k = 0; k < 27; ++k) {
@@ -708,6 +731,15 @@ __global__ __launch_bounds__(256) void k_st_gs(LevelView L, const int4* __restri
   u[p] = v0;
   u[L.NP + p] = v1;
   u[2 * L.NP + p] = v2;
+}
+
+// one colour: cl points at the colour's chunks, slot0 = 64 * (their offset in the list)
+template <class T, bool BACKWARD, int ZC, class ST>
+__global__ __launch_bounds__(256) void k_st_gs(LevelView L, const int4* __restrict__ cl, int nch, int64_t slot0,
+                                               ST st, T* __restrict__ u, const T* __restrict__ f) {
+  NodeRef n;
+  if (!chunk_node(cl, nch, 2, n)) return;
+  st_gs_node<T, BACKWARD, ZC>(L, n.p, slot0 + n.slot, st, u, f);
 }
 
 // Stored-stencil counterpart of k_fine_gs_pairw (same segments, the same repeated c1 node
@@ -822,10 +854,8 @@ __global__ __launch_bounds__(256) void k_st_resid_z(LevelView L, const int4* __r
 // :2594-2632)
 // ---------------------------------------------------------------------------------------
 template <class T>
-__global__ __launch_bounds__(256) void k_restrict(LevelView F, LevelView C, const int4* __restrict__ clc, int nch,
-                                                  const T* __restrict__ rf, T* __restrict__ fc) {
-  NodeRef n;
-  if (!chunk_node(clc, nch, 1, n)) return;
+__device__ __forceinline__ void restrict_node(const LevelView& F, const LevelView& C, const NodeRef& n,
+                                              const T* __restrict__ rf, T* __restrict__ fc) {
   T a0 = 0, a1 = 0, a2 = 0;
   const int64_t c0 = F.pnode(2 * n.x, 2 * n.y, 2 * n.z);
 #pragma unroll
@@ -838,15 +868,21 @@ __global__ __launch_bounds__(256) void k_restrict(LevelView F, LevelView C, cons
   fc[n.p] = a0; fc[C.NP + n.p] = a1; fc[2 * C.NP + n.p] = a2;
 }
 
+template <class T>
+__global__ __launch_bounds__(256) void k_restrict(LevelView F, LevelView C, const int4* __restrict__ clc, int nch,
+                                                  const T* __restrict__ rf, T* __restrict__ fc) {
+  NodeRef n;
+  if (!chunk_node(clc, nch, 1, n)) return;
+  restrict_node(F, C, n, rf, fc);
+}
+
 // u_f += P u_c: a fine node takes the trilinear interpolation of the (up to 8) coarse
 // nodes around it.  All 8 corner nodes (X..X+1, Y..Y+1, Z..Z+1, within the padding) are
 // read with weight 0 where a fine coordinate is even, so the loads are issued together
 // instead of as a divergent loop.
 template <class T>
-__global__ __launch_bounds__(256) void k_prolong_add(LevelView F, LevelView C, const int4* __restrict__ clf, int nch,
-                                                     const T* __restrict__ uc, T* __restrict__ uf) {
-  NodeRef n;
-  if (!chunk_node(clf, nch, 1, n)) return;
+__device__ __forceinline__ void prolong_node(const LevelView& F, const LevelView& C, const NodeRef& n,
+                                             const T* __restrict__ uc, T* __restrict__ uf) {
   const int x = n.x, y = n.y, z = n.z;
   const T hx = (x & 1) ? T(0.5) : T(0), hy = (y & 1) ? T(0.5) : T(0), hz = (z & 1) ? T(0.5) : T(0);
   const T wx[2] = {T(1) - hx, hx}, wy[2] = {T(1) - hy, hy}, wz[2] = {T(1) - hz, hz};
@@ -868,18 +904,22 @@ __global__ __launch_bounds__(256) void k_prolong_add(LevelView F, LevelView C, c
   uf[2 * F.NP + p] = f2 + a2;
 }
 
+template <class T>
+__global__ __launch_bounds__(256) void k_prolong_add(LevelView F, LevelView C, const int4* __restrict__ clf, int nch,
+                                                     const T* __restrict__ uc, T* __restrict__ uf) {
+  NodeRef n;
+  if (!chunk_node(clf, nch, 1, n)) return;
+  prolong_node(F, C, n, uc, uf);
+}
+
 // u_f += P u_c over row segments (even-x first, all rows): lane l takes the fine nodes
 // x1 = xs + 2l (coarse column X = x1/2, weight 1 along x) and x1 + 1 (columns X and X+1,
 // 1/2 each), so one wave reads each coarse column once for two fine nodes.  Lane 0 of a
 // continuation segment repeats the previous segment's lane 63 and does not write.
 template <class T>
-__global__ __launch_bounds__(256) void k_prolong_add_pairw(LevelView F, LevelView C, const int4* __restrict__ seg,
-                                                           int nseg, const uint8_t* __restrict__ fre,
-                                                           const T* __restrict__ uc, T* __restrict__ uf) {
-  const int si = (int)xcd_block() * 4 + (threadIdx.x >> 6);
-  if (si >= nseg) return;
-  const int lane = threadIdx.x & 63;
-  const int4 S = seg[si];
+__device__ __forceinline__ void prolong_pairw_lane(const LevelView& F, const LevelView& C, const int4 S, int lane,
+                                                   const uint8_t* __restrict__ fre, const T* __restrict__ uc,
+                                                   T* __restrict__ uf) {
   const int xs = (S.y & 0x7fff) - 1, xe = ((S.y >> 15) & 0x7fff) - 1;
   const bool cont = (S.y >> 30) & 1;
   const int x1 = xs + 2 * lane, y = S.z, z = S.w;
@@ -914,6 +954,15 @@ __global__ __launch_bounds__(256) void k_prolong_add_pairw(LevelView F, LevelVie
 #pragma unroll
     for (int c = 0; c < 3; ++c) uf[c * F.NP + pa + 1] = g2[c] + T(0.5) * (a[0][c] + a[1][c]);
   }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_prolong_add_pairw(LevelView F, LevelView C, const int4* __restrict__ seg,
+                                                           int nseg, const uint8_t* __restrict__ fre,
+                                                           const T* __restrict__ uc, T* __restrict__ uf) {
+  const int si = (int)xcd_block() * 4 + (threadIdx.x >> 6);
+  if (si >= nseg) return;
+  prolong_pairw_lane(F, C, seg[si], threadIdx.x & 63, fre, uc, uf);
 }
 
 template <class T>
