@@ -572,10 +572,11 @@ __global__ TOS_FINE_BOUNDS(T) void k_fine_gs_wf(LevelView L, const int4* __restr
 // Readers of the compact stencil: entry e (3x3 row-major) of block k of the node in slot s.
 // row[i] of a stored-stencil row through a buffer resource: the row base is uniform
 // (kernel arguments and unrolled constants), so the address needs no per-lane 64-bit
-// arithmetic (scalar base, 32-bit lane offset; slot counts stay far below 2^30)
+// arithmetic (scalar base, 32-bit lane byte offset: a level >= 1 has at most 2^28 slots
+// under the 2^31 padded-node limit of a context)
 template <class T>
 __device__ __forceinline__ T ld_row(const T* row, uint32_t i) {
-  const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(row), (short)0, 0x7fffffff,
+  const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(row), (short)0, -1,
                                                                       0x00020000);
   if constexpr (sizeof(T) == 8)
     return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, i * 8u, 0, 0));
