@@ -46,22 +46,24 @@ __device__ __forceinline__ void bpartial(double v, int K, int KP, double* __rest
   __syncthreads();
 }
 
-template <class T>
+// neighbourhood of node p, case k; neighbours outside MASK read as 0 without a load
+template <class T, unsigned MASK = 0x7ffffffu>
 __device__ __forceinline__ void load_nbhd_b(const LevelView& L, const T* __restrict__ u, int64_t p, int K, int k,
                                             T (&un)[27][3]) {
 #pragma unroll
   for (int q = 0; q < 27; ++q) {
     const int64_t a = p + (int64_t)(q % 3 - 1) + (int64_t)L.NX * ((q / 3) % 3 - 1 + (int64_t)L.NY * (q / 9 - 1));
 #pragma unroll
-    for (int c = 0; c < 3; ++c) un[q][c] = u[(c * L.NP + a) * K + k];
+    for (int c = 0; c < 3; ++c) un[q][c] = ((MASK >> q) & 1u) ? u[(c * L.NP + a) * K + k] : T(0);
   }
 }
 
-// MODE 0: y = K u with per-case partials of u.y;  MODE 1: y = f - K u (+ float copy y32)
-template <class T, int MODE, int BS>
+// MODE 0: y = K u with per-case partials of u.y;  MODE 1: y = f - K u (+ float copy y32).
+// TU: type of u (float: the FCG search direction, with FP64 arithmetic)
+template <class T, int MODE, int BS, class TU = T>
 __global__ __launch_bounds__(BS) void k_fine_apply_b(LevelView L, const int4* __restrict__ cl, int nch, int K, int KP,
                                                      const T* __restrict__ w, const T* __restrict__ Ke,
-                                                     const T* __restrict__ u, const T* __restrict__ f,
+                                                     const TU* __restrict__ u, const T* __restrict__ f,
                                                      T* __restrict__ y, double* __restrict__ partial,
                                                      float* __restrict__ y32) {
   NodeRef n;
@@ -69,11 +71,12 @@ __global__ __launch_bounds__(BS) void k_fine_apply_b(LevelView L, const int4* __
   const bool ok = bnode<BS>(cl, nch, 1, K, KP, n, k);
   double d = 0;
   if (ok) {
-    T un[27][3], wa[8];
+    TU un[27][3];
+    T wa[8];
     load_nbhd_b(L, u, n.p, K, k, un);
     load_weights(L, w, n.x, n.y, n.z, wa);
     T acc[3] = {0, 0, 0}, D[9];
-    fine_row<T, true>(wa, Ke, un, acc, D);
+    fine_row<T, true, 0x7ffffffu, TU>(wa, Ke, un, acc, D);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const int64_t i = (c * L.NP + n.p) * K + k;
@@ -90,22 +93,25 @@ __global__ __launch_bounds__(BS) void k_fine_apply_b(LevelView L, const int4* __
   if (partial) bpartial<BS>(d, K, KP, partial);
 }
 
-template <class T, bool BACKWARD, int BS>
+// ZC >= 0: colour ZC of a first forward sweep from u = 0 (u is not cleared beforehand:
+// only earlier colours' neighbours are read, the node's own start value is 0)
+template <class T, bool BACKWARD, int BS, int ZC = -1>
 __global__ __launch_bounds__(BS) void k_fine_gs_b(LevelView L, const int4* __restrict__ cl, int nch, int K, int KP,
                                                   const T* __restrict__ w, const T* __restrict__ Ke,
                                                   T* __restrict__ u, const T* __restrict__ f) {
+  constexpr unsigned mask = nb_mask_of(ZC);
   NodeRef n;
   int k = 0;
   if (!bnode<BS>(cl, nch, 2, K, KP, n, k)) return;
   T un[27][3], wa[8];
-  load_nbhd_b(L, u, n.p, K, k, un);
+  load_nbhd_b<T, ZC < 0 ? 0x7ffffffu : mask>(L, u, n.p, K, k, un);
   load_weights(L, w, n.x, n.y, n.z, wa);
   T s[3] = {0, 0, 0};
   T D[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  fine_row<T, false>(wa, Ke, un, s, D);
+  fine_row<T, false, mask>(wa, Ke, un, s, D);
   const int64_t i0 = n.p * K + k, cs = L.NP * K;
   const T b0 = f[i0] - s[0], b1 = f[cs + i0] - s[1], b2 = f[2 * cs + i0] - s[2];
-  T v0 = un[13][0], v1 = un[13][1], v2 = un[13][2];
+  T v0 = ZC < 0 ? un[13][0] : T(0), v1 = ZC < 0 ? un[13][1] : T(0), v2 = ZC < 0 ? un[13][2] : T(0);
   if (!BACKWARD) {
     v0 = (b0 - D[1] * v1 - D[2] * v2) / D[0];
     v1 = (b1 - D[3] * v0 - D[5] * v2) / D[4];
@@ -118,6 +124,29 @@ __global__ __launch_bounds__(BS) void k_fine_gs_b(LevelView L, const int4* __res
   u[i0] = v0;
   u[cs + i0] = v1;
   u[2 * cs + i0] = v2;
+}
+
+// residual after a first forward sweep from zero, nodes of colour ZC: later-colour
+// neighbours and the node's own block only (as k_fine_resid_z, per case)
+template <class T, int ZC, int BS>
+__global__ __launch_bounds__(BS) void k_fine_resid_z_b(LevelView L, const int4* __restrict__ cl, int nch, int K,
+                                                       int KP, const T* __restrict__ w, const T* __restrict__ Ke,
+                                                       const T* __restrict__ u, T* __restrict__ r) {
+  constexpr unsigned after = nb_mask_after(ZC);
+  NodeRef n;
+  int k = 0;
+  if (!bnode<BS>(cl, nch, 2, K, KP, n, k)) return;
+  T un[27][3], wa[8];
+  load_nbhd_b<T, after | (1u << 13)>(L, u, n.p, K, k, un);
+  load_weights(L, w, n.x, n.y, n.z, wa);
+  T s[3] = {0, 0, 0};
+  T D[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  fine_row<T, false, after>(wa, Ke, un, s, D);
+  const T v1 = un[13][1], v2 = un[13][2];
+  const int64_t i0 = n.p * K + k, cs = L.NP * K;
+  r[i0] = -D[1] * v1 - D[2] * v2 - s[0];
+  r[cs + i0] = -D[5] * v2 - s[1];
+  r[2 * cs + i0] = -s[2];
 }
 
 template <class T, int MODE, int BS>
@@ -264,7 +293,7 @@ enum { B_RZ = 0, B_PQ, B_ALPHA, B_BETA, B_RN2, B_ZR, B_ZQ, B_NF2, B_ACTIVE, B_RT
 template <int BS>
 __global__ __launch_bounds__(BS) void k_pcgb_ur(const int4* __restrict__ cl, int nch, int64_t NP, int K, int KP,
                                                 double* __restrict__ u, double* __restrict__ r,
-                                                const double* __restrict__ p, const double* __restrict__ q,
+                                                const float* __restrict__ p, const double* __restrict__ q,
                                                 const double* __restrict__ sc, float* __restrict__ r32,
                                                 double* __restrict__ partial) {
   NodeRef n;
@@ -276,7 +305,7 @@ __global__ __launch_bounds__(BS) void k_pcgb_ur(const int4* __restrict__ cl, int
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const int64_t i = (c * NP + n.p) * K + k;
-      u[i] += a * p[i];
+      u[i] += a * (double)p[i];
       const double v = r[i] - a * q[i];
       r[i] = v;
       r32[i] = (float)v;
@@ -308,10 +337,10 @@ __global__ __launch_bounds__(BS) void k_pcgb_zdots(const int4* __restrict__ cl, 
   if (q) bpartial<BS>(b, K, KP, partial + (int64_t)gridDim.x * K);
 }
 
-// p = z + beta p (first: p = z)
+// p = z + beta p (first: p = z), formed in FP64 and stored in FP32
 template <int BS>
 __global__ __launch_bounds__(BS) void k_pcgb_p(const int4* __restrict__ cl, int nch, int64_t NP, int K, int KP,
-                                               double* __restrict__ p, const float* __restrict__ z,
+                                               float* __restrict__ p, const float* __restrict__ z,
                                                const double* __restrict__ sc, int first) {
   NodeRef n;
   int k = 0;
@@ -320,7 +349,7 @@ __global__ __launch_bounds__(BS) void k_pcgb_p(const int4* __restrict__ cl, int 
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     const int64_t i = (c * NP + n.p) * K + k;
-    p[i] = (double)z[i] + b * p[i];
+    p[i] = (float)((double)z[i] + b * (double)p[i]);
   }
 }
 
@@ -347,17 +376,28 @@ __global__ __launch_bounds__(BS) void k_dotb(const int4* __restrict__ cl, int nc
 }
 
 // stage 1: group g (grid) folds blocks [g*seg, (g+1)*seg) of `ncol` columns of per-case
-// partials part[col][nb][K] into out[col][g][K]
+// partials part[col][nb][K] into out[col][g][K].  Thread t = j*K + k (j < J = 256/K) sums
+// blocks b0 + j, b0 + j + J, ... of case k (rows of K cases are contiguous, so a group of
+// K threads reads one coalesced row); the J sums of a case are then added in order j.
 __global__ __launch_bounds__(256) void k_foldb1(const double* __restrict__ part, int nb, int K, int ncol,
                                                 double* __restrict__ out) {
+  __shared__ double sh[256];
   const int G = gridDim.x, g = blockIdx.x;
   const int seg = (nb + G - 1) / G;
   const int b0 = g * seg, b1 = min(nb, b0 + seg);
-  for (int t = threadIdx.x; t < ncol * K; t += 256) {
-    const int col = t / K, k = t % K;
+  const int J = 256 / K, t = threadIdx.x, j = t / K, k = t % K;
+  for (int col = 0; col < ncol; ++col) {
     double s = 0;
-    for (int b = b0; b < b1; ++b) s += part[((int64_t)col * nb + b) * K + k];
-    out[((int64_t)col * G + g) * K + k] = s;
+    if (j < J)
+      for (int b = b0 + j; b < b1; b += J) s += part[((int64_t)col * nb + b) * K + k];
+    sh[t] = s;
+    __syncthreads();
+    if (t < K) {
+      double a = 0;
+      for (int q = 0; q < J; ++q) a += sh[q * K + t];
+      out[((int64_t)col * G + g) * K + t] = a;
+    }
+    __syncthreads();
   }
 }
 

@@ -1125,7 +1125,8 @@ namespace tos {
 struct BatchState {
   int K = 0, KP = 0;
   std::vector<float*> u32, f32, r32;  // per level, 3 * NP * K
-  double *U = nullptr, *R = nullptr, *P = nullptr, *Q = nullptr, *F = nullptr;
+  double *U = nullptr, *R = nullptr, *Q = nullptr, *F = nullptr;
+  float* P = nullptr;       // FCG search direction (FP32 storage, FP64 arithmetic)
   double* part = nullptr;   // per-case block partials [2][nb][K]
   double* fold = nullptr;   // [2][kFoldG][K]
   double* sc = nullptr;     // [B_COUNT][K]
@@ -1143,7 +1144,7 @@ struct BatchState {
 
 namespace {
 
-constexpr int kFoldG = 128;  // first-stage groups of the per-case folds
+constexpr int kFoldG = 512;  // first-stage groups of the per-case folds
 constexpr int kVecBS = 256;  // block size of the kernels that write per-case partials
 
 inline unsigned bgrid(int nch, int KP, int bs) { return (unsigned)(((int64_t)nch * 64 * KP + bs - 1) / bs); }
@@ -1180,7 +1181,7 @@ BatchState& batch_prepare(Ctx& c) {
   const size_t n0 = 3 * (size_t)c.L[0].v.NP * B.K;
   B.U = (double*)raw_alloc(B.mem, c.stream, n0 * sizeof(double));
   B.R = (double*)raw_alloc(B.mem, c.stream, n0 * sizeof(double));
-  B.P = (double*)raw_alloc(B.mem, c.stream, n0 * sizeof(double));
+  B.P = (float*)raw_alloc(B.mem, c.stream, n0 * sizeof(float));
   B.Q = (double*)raw_alloc(B.mem, c.stream, n0 * sizeof(double));
   B.F = (double*)raw_alloc(B.mem, c.stream, n0 * sizeof(double));
   const size_t nb = bgrid(c.L[0].n_lin, B.KP, kVecBS);
@@ -1200,6 +1201,39 @@ BatchState& batch_prepare(Ctx& c) {
     case 128: GO(128); break; \
     default: GO(256); break;  \
   }
+
+// fine level, colour `col` of a first forward sweep from zero / of the residual after it
+template <int BS>
+void fine_zero_colour_b(Ctx& c, BatchState& B, const LevelView& v, const int4* cl, int nch, int col, float* u,
+                        const float* f, float* r) {
+  const unsigned nb = bgrid(nch, B.KP, BS);
+  if (!r) {
+#define GO(ZC) hipLaunchKernelGGL((k_fine_gs_b<float, false, BS, ZC>), dim3(nb), dim3(BS), 0, c.stream, v, cl, nch, \
+                                  B.K, B.KP, (const float*)c.w32, (const float*)c.Kc32, u, f)
+    TOS_COLOUR_SWITCH(col, GO);
+#undef GO
+  } else {
+#define GO(ZC) hipLaunchKernelGGL((k_fine_resid_z_b<float, ZC, BS>), dim3(nb), dim3(BS), 0, c.stream, v, cl, nch, \
+                                  B.K, B.KP, (const float*)c.w32, (const float*)c.Kc32, (const float*)u, r)
+    TOS_COLOUR_SWITCH(col, GO);
+#undef GO
+  }
+}
+
+// fine level: the first forward sweep from zero (r = null), or the residual after it into
+// r (later-colour neighbours only), colour by colour
+void fine_zero_sweep_b(Ctx& c, BatchState& B, float* u, const float* f, float* r) {
+  Level& L = c.L[0];
+  const int bs = r ? B.bs_fine_res : B.bs_fine_gs;
+  for (int col = 0; col < 8; ++col) {
+    const int off = L.col_off[col], nch = L.col_off[col + 1] - off;
+    if (nch == 0) continue;
+#define GO(BS) fine_zero_colour_b<BS>(c, B, L.v, L.cl_col + off, nch, col, u, f, r)
+    TOS_BS_SWITCH(bs, GO)
+#undef GO
+  }
+  LAUNCH_CHECK();
+}
 
 // one colour-ordered GS sweep of level l on the interleaved vectors
 void gs_b(Ctx& c, BatchState& B, int l, float* u, const float* f, bool backward) {
@@ -1300,6 +1334,20 @@ void vcycle_b(Ctx& c, BatchState& B, int l, const float* f, float* u, int pre, i
     return;
   }
   if (L.n_lin == 0) return;
+  if (l == 0 && l < last && pre >= 1) {  // fine level: first sweep from zero without clearing u
+    fine_zero_sweep_b(c, B, u, f, nullptr);
+    for (int s = 1; s < pre; ++s) gs_b(c, B, l, u, f, false);
+    if (pre == 1)
+      fine_zero_sweep_b(c, B, u, f, B.r32[0]);
+    else
+      resid_b(c, B, l, u, f, B.r32[l]);
+    restrict_b(c, B, l, B.r32[l], B.f32[l + 1]);
+    vcycle_b(c, B, l + 1, B.f32[l + 1], B.u32[l + 1], pre, post);
+    prolong_b(c, B, l, B.u32[l + 1], u);
+    LAUNCH_CHECK();
+    for (int s = 0; s < post; ++s) gs_b(c, B, l, u, f, true);
+    return;
+  }
   hipLaunchKernelGGL((k_zero_nodes_b<float, kVecBS>), dim3(bgrid(L.n_lin, B.KP, kVecBS)), dim3(kVecBS), 0, c.stream,
                      (const int4*)L.cl_lin, L.n_lin, L.v.NP, B.K, B.KP, u);
   if (l == last) {
@@ -1331,12 +1379,12 @@ void batch_iteration(Ctx& c, BatchState& B, int pre, int post) {
   const int nch = L0.n_lin;
   const int64_t NP = L0.v.NP;
   const unsigned nb = bgrid(nch, B.KP, kVecBS);
-  hipLaunchKernelGGL((k_fine_apply_b<double, 0, kVecBS>), dim3(nb), dim3(kVecBS), 0, c.stream, L0.v, cl, nch, B.K,
-                     B.KP, (const double*)c.w64, (const double*)c.Kc64, (const double*)B.P, (const double*)nullptr,
+  hipLaunchKernelGGL((k_fine_apply_b<double, 0, kVecBS, float>), dim3(nb), dim3(kVecBS), 0, c.stream, L0.v, cl, nch,
+                     B.K, B.KP, (const double*)c.w64, (const double*)c.Kc64, (const float*)B.P, (const double*)nullptr,
                      B.Q, B.part, (float*)nullptr);
   foldb(c, B, nb, 1, B_PQ, 0, OPB_ALPHA);
   hipLaunchKernelGGL((k_pcgb_ur<kVecBS>), dim3(nb), dim3(kVecBS), 0, c.stream, cl, nch, NP, B.K, B.KP, B.U, B.R,
-                     (const double*)B.P, (const double*)B.Q, (const double*)B.sc, B.f32[0], B.part);
+                     (const float*)B.P, (const double*)B.Q, (const double*)B.sc, B.f32[0], B.part);
   foldb(c, B, nb, 1, B_RN2, 0, OPB_RN2);
   vcycle_b(c, B, 0, B.f32[0], B.u32[0], pre, post);
   hipLaunchKernelGGL((k_pcgb_zdots<kVecBS>), dim3(nb), dim3(kVecBS), 0, c.stream, cl, nch, NP, B.K, B.KP,
