@@ -47,14 +47,19 @@ __device__ __forceinline__ void bpartial(double v, int K, int KP, double* __rest
 }
 
 // neighbourhood of node p, case k; neighbours outside MASK read as 0 without a load
+// (one per-lane base address; every neighbour and component offset is uniform, so each
+// load costs one address add instead of a 64-bit multiply chain)
 template <class T, unsigned MASK = 0x7ffffffu>
 __device__ __forceinline__ void load_nbhd_b(const LevelView& L, const T* __restrict__ u, int64_t p, int K, int k,
                                             T (&un)[27][3]) {
+  const T* __restrict__ b = u + p * K + k;
+  const int64_t cs = L.NP * K;
 #pragma unroll
   for (int q = 0; q < 27; ++q) {
-    const int64_t a = p + (int64_t)(q % 3 - 1) + (int64_t)L.NX * ((q / 3) % 3 - 1 + (int64_t)L.NY * (q / 9 - 1));
+    const int64_t o =
+        ((int64_t)(q % 3 - 1) + (int64_t)L.NX * ((q / 3) % 3 - 1 + (int64_t)L.NY * (q / 9 - 1))) * K;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) un[q][c] = ((MASK >> q) & 1u) ? u[(c * L.NP + a) * K + k] : T(0);
+    for (int c = 0; c < 3; ++c) un[q][c] = ((MASK >> q) & 1u) ? b[c * cs + o] : T(0);
   }
 }
 
@@ -157,16 +162,17 @@ __global__ __launch_bounds__(BS) void k_st_apply_b(LevelView L, const int4* __re
   int k = 0;
   if (!bnode<BS>(cl, nch, 2, K, KP, n, k)) return;
   const int64_t cs = L.NP * K;
+  const StFull<T> S{st, NS};
+  T un[27][3];
+  load_nbhd_b(L, u, n.p, K, k, un);
   T a0 = 0, a1 = 0, a2 = 0;
 #pragma unroll
   for (int q = 0; q < 27; ++q) {
-    const int64_t a =
-        (n.p + (int64_t)(q % 3 - 1) + (int64_t)L.NX * ((q / 3) % 3 - 1 + (int64_t)L.NY * (q / 9 - 1))) * K + k;
-    const T u0 = u[a], u1 = u[cs + a], u2 = u[2 * cs + a];
-    const T* s = st + (int64_t)q * 9 * NS + n.slot;
-    a0 = fma(s[0], u0, a0); a0 = fma(s[NS], u1, a0); a0 = fma(s[2 * NS], u2, a0);
-    a1 = fma(s[3 * NS], u0, a1); a1 = fma(s[4 * NS], u1, a1); a1 = fma(s[5 * NS], u2, a1);
-    a2 = fma(s[6 * NS], u0, a2); a2 = fma(s[7 * NS], u1, a2); a2 = fma(s[8 * NS], u2, a2);
+    const T u0 = un[q][0], u1 = un[q][1], u2 = un[q][2];
+    const int64_t sl = n.slot;
+    a0 = fma(S.get(sl, q, 0), u0, a0); a0 = fma(S.get(sl, q, 1), u1, a0); a0 = fma(S.get(sl, q, 2), u2, a0);
+    a1 = fma(S.get(sl, q, 3), u0, a1); a1 = fma(S.get(sl, q, 4), u1, a1); a1 = fma(S.get(sl, q, 5), u2, a1);
+    a2 = fma(S.get(sl, q, 6), u0, a2); a2 = fma(S.get(sl, q, 7), u1, a2); a2 = fma(S.get(sl, q, 8), u2, a2);
   }
   const int64_t i0 = n.p * K + k;
   if (MODE == 0) {
@@ -184,25 +190,24 @@ __global__ __launch_bounds__(BS) void k_st_gs_b(LevelView L, const int4* __restr
   int k = 0;
   if (!bnode<BS>(cl, nch, 2, K, KP, n, k)) return;
   const int64_t slot = slot0 + n.slot, cs = L.NP * K;
+  const StFull<T> S{st, NS};
+  T un[27][3];
+  load_nbhd_b(L, u, n.p, K, k, un);
   T s0 = 0, s1 = 0, s2 = 0;
 #pragma unroll
   for (int q = 0; q < 27; ++q) {
     if (q == 13) continue;
-    const int64_t a =
-        (n.p + (int64_t)(q % 3 - 1) + (int64_t)L.NX * ((q / 3) % 3 - 1 + (int64_t)L.NY * (q / 9 - 1))) * K + k;
-    const T u0 = u[a], u1 = u[cs + a], u2 = u[2 * cs + a];
-    const T* s = st + (int64_t)q * 9 * NS + slot;
-    s0 = fma(s[0], u0, s0); s0 = fma(s[NS], u1, s0); s0 = fma(s[2 * NS], u2, s0);
-    s1 = fma(s[3 * NS], u0, s1); s1 = fma(s[4 * NS], u1, s1); s1 = fma(s[5 * NS], u2, s1);
-    s2 = fma(s[6 * NS], u0, s2); s2 = fma(s[7 * NS], u1, s2); s2 = fma(s[8 * NS], u2, s2);
+    const T u0 = un[q][0], u1 = un[q][1], u2 = un[q][2];
+    s0 = fma(S.get(slot, q, 0), u0, s0); s0 = fma(S.get(slot, q, 1), u1, s0); s0 = fma(S.get(slot, q, 2), u2, s0);
+    s1 = fma(S.get(slot, q, 3), u0, s1); s1 = fma(S.get(slot, q, 4), u1, s1); s1 = fma(S.get(slot, q, 5), u2, s1);
+    s2 = fma(S.get(slot, q, 6), u0, s2); s2 = fma(S.get(slot, q, 7), u1, s2); s2 = fma(S.get(slot, q, 8), u2, s2);
   }
-  const T* d = st + (int64_t)13 * 9 * NS + slot;
   T D[9];
 #pragma unroll
-  for (int e = 0; e < 9; ++e) D[e] = d[e * NS];
+  for (int e = 0; e < 9; ++e) D[e] = S.get(slot, 13, e);
   const int64_t i0 = n.p * K + k;
   const T b0 = f[i0] - s0, b1 = f[cs + i0] - s1, b2 = f[2 * cs + i0] - s2;
-  T v0 = u[i0], v1 = u[cs + i0], v2 = u[2 * cs + i0];
+  T v0 = un[13][0], v1 = un[13][1], v2 = un[13][2];
   if (!BACKWARD) {
     v0 = (b0 - D[1] * v1 - D[2] * v2) / D[0];
     v1 = (b1 - D[3] * v0 - D[5] * v2) / D[4];
@@ -403,15 +408,26 @@ __global__ __launch_bounds__(256) void k_foldb1(const double* __restrict__ part,
 
 // stage 2: sc[slot_c * K + k] = sum_g out[c][g][K], then the FCG scalar update `op`
 enum { OPB_NONE = 0, OPB_ALPHA, OPB_BETA, OPB_RN2, OPB_INIT };
+// (thread t = j*K + k sums groups j, j + J, ... of case k; the J sums are added in order j)
 __global__ __launch_bounds__(256) void k_foldb2(const double* __restrict__ st, int G, int K, int ncol, int slot0,
                                                 int slot1, int op, double* __restrict__ sc) {
-  const int k = threadIdx.x;
-  if (k >= K) return;
+  __shared__ double sh[256];
+  const int J = 256 / K, t = threadIdx.x, j = t / K, kk = t % K;
   for (int col = 0; col < ncol; ++col) {
     double s = 0;
-    for (int g = 0; g < G; ++g) s += st[((int64_t)col * G + g) * K + k];
-    sc[(col == 0 ? slot0 : slot1) * K + k] = s;
+    if (j < J)
+      for (int g = j; g < G; g += J) s += st[((int64_t)col * G + g) * K + kk];
+    sh[t] = s;
+    __syncthreads();
+    if (t < K) {
+      double a = 0;
+      for (int q = 0; q < J; ++q) a += sh[q * K + t];
+      sc[(col == 0 ? slot0 : slot1) * K + t] = a;
+    }
+    __syncthreads();
   }
+  const int k = t;
+  if (k >= K) return;
   const bool act = sc[B_ACTIVE * K + k] != 0.0;
   if (op == OPB_ALPHA) sc[B_ALPHA * K + k] = act && sc[B_PQ * K + k] != 0.0 ? sc[B_RZ * K + k] / sc[B_PQ * K + k] : 0.0;
   if (op == OPB_RN2) sc[B_ACTIVE * K + k] = act && sc[B_RN2 * K + k] > sc[B_RTOL2 * K + k] ? 1.0 : 0.0;
