@@ -230,13 +230,13 @@ __global__ __launch_bounds__(BS) void k_restrict_b(LevelView F, LevelView C, con
   if (!bnode<BS>(clc, nch, 1, K, KP, n, k)) return;
   const int64_t fcs = F.NP * K, ccs = C.NP * K;
   T a0 = 0, a1 = 0, a2 = 0;
-  const int64_t c0 = F.pnode(2 * n.x, 2 * n.y, 2 * n.z);
+  const T* __restrict__ b = rf + F.pnode(2 * n.x, 2 * n.y, 2 * n.z) * K + k;
 #pragma unroll
   for (int q = 0; q < 27; ++q) {
     const int dx = q % 3 - 1, dy = (q / 3) % 3 - 1, dz = q / 9 - 1;
     const T wgt = T((dx ? 0.5 : 1.0) * (dy ? 0.5 : 1.0) * (dz ? 0.5 : 1.0));
-    const int64_t a = (c0 + dx + (int64_t)F.NX * (dy + (int64_t)F.NY * dz)) * K + k;
-    a0 += wgt * rf[a]; a1 += wgt * rf[fcs + a]; a2 += wgt * rf[2 * fcs + a];
+    const int64_t o = (dx + (int64_t)F.NX * (dy + (int64_t)F.NY * dz)) * K;  // (uniform)
+    a0 += wgt * b[o]; a1 += wgt * b[fcs + o]; a2 += wgt * b[2 * fcs + o];
   }
   const int64_t i0 = n.p * K + k;
   fc[i0] = a0; fc[ccs + i0] = a1; fc[2 * ccs + i0] = a2;
@@ -249,17 +249,21 @@ __global__ __launch_bounds__(BS) void k_prolong_add_b(LevelView F, LevelView C, 
   NodeRef n;
   int k = 0;
   if (!bnode<BS>(clf, nch, 1, K, KP, n, k)) return;
+  // all 8 corners (inside the padding) with weight 0 where a fine coordinate is even: the
+  // loads are issued together instead of in a divergent loop
   const int x = n.x, y = n.y, z = n.z;
-  const int nxp = (x & 1) + 1, nyp = (y & 1) + 1, nzp = (z & 1) + 1;
-  const T wgt = ((x & 1) ? T(0.5) : T(1)) * ((y & 1) ? T(0.5) : T(1)) * ((z & 1) ? T(0.5) : T(1));
-  const int64_t I0 = C.pnode(x >> 1, y >> 1, z >> 1), ccs = C.NP * K, fcs = F.NP * K;
+  const T hx = (x & 1) ? T(0.5) : T(0), hy = (y & 1) ? T(0.5) : T(0), hz = (z & 1) ? T(0.5) : T(0);
+  const T wx[2] = {T(1) - hx, hx}, wy[2] = {T(1) - hy, hy}, wz[2] = {T(1) - hz, hz};
+  const int64_t ccs = C.NP * K, fcs = F.NP * K;
+  const T* __restrict__ b = uc + C.pnode(x >> 1, y >> 1, z >> 1) * K + k;
   T a0 = 0, a1 = 0, a2 = 0;
-  for (int kz = 0; kz < nzp; ++kz)
-    for (int j = 0; j < nyp; ++j)
-      for (int i = 0; i < nxp; ++i) {
-        const int64_t I = (I0 + i + (int64_t)C.NX * (j + (int64_t)C.NY * kz)) * K + k;
-        a0 += wgt * uc[I]; a1 += wgt * uc[ccs + I]; a2 += wgt * uc[2 * ccs + I];
-      }
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int i = q & 1, j = (q >> 1) & 1, l = q >> 2;
+    const T wgt = wx[i] * wy[j] * wz[l];
+    const int64_t o = (i + (int64_t)C.NX * (j + (int64_t)C.NY * l)) * K;  // (uniform)
+    a0 = fma(wgt, b[o], a0); a1 = fma(wgt, b[ccs + o], a1); a2 = fma(wgt, b[2 * ccs + o], a2);
+  }
   const int64_t i0 = n.p * K + k;
   uf[i0] += a0; uf[fcs + i0] += a1; uf[2 * fcs + i0] += a2;
 }
