@@ -279,21 +279,26 @@ __global__ __launch_bounds__(BS) void k_zero_nodes_b(const int4* __restrict__ cl
 }
 
 // coarsest level: u[:, k] = -cinv f[:, k] (cinv holds -A^{-1}), thread per (row, case)
+// u_c = -M f_c for K cases: block = one row of M; thread t = j*K + k (j < J = 256/K) sums
+// the columns j, j + J, ... for case k, the J sums are added in order j
 template <class T>
 __global__ __launch_bounds__(256) void k_coarse_solve_b(LevelView L, const int64_t* __restrict__ cidx, int n, int K,
                                                         const double* __restrict__ cinv, const T* __restrict__ f,
                                                         T* __restrict__ u) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)n * K) return;
-  const int row = (int)(t / K), k = (int)(t % K);
+  __shared__ double sh[256];
+  const int row = blockIdx.x;
+  const int J = 256 / K, t = threadIdx.x, jl = t / K, k = t % K;
   const int64_t cs = L.NP * K;
   double s = 0;
-  for (int j = 0; j < n; ++j) {
-    const int64_t pj = cidx[j / 3];
-    s += cinv[(int64_t)row * n + j] * (double)f[(j % 3) * cs + pj * K + k];
+  if (jl < J)
+    for (int j = jl; j < n; j += J) s += cinv[(int64_t)row * n + j] * (double)f[(j % 3) * cs + cidx[j / 3] * K + k];
+  sh[t] = s;
+  __syncthreads();
+  if (t < K) {
+    double a = 0;
+    for (int q = 0; q < J; ++q) a += sh[q * K + t];
+    u[(row % 3) * cs + cidx[row / 3] * K + t] = (T)(-a);
   }
-  const int64_t pr = cidx[row / 3];
-  u[(row % 3) * cs + pr * K + k] = (T)(-s);
 }
 
 // ---- batched FCG (FP64 outer vectors, per-case scalars sc[slot * K + k]) ----------------

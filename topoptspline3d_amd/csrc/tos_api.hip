@@ -1328,7 +1328,7 @@ void vcycle_b(Ctx& c, BatchState& B, int l, const float* f, float* u, int pre, i
   const int last = (int)c.L.size() - 1;
   if (l == last && last > 0) {
     if (c.cn == 0) return;
-    hipLaunchKernelGGL((k_coarse_solve_b<float>), dim3(nblk((int64_t)c.cn * B.K)), dim3(256), 0, c.stream, L.v,
+    hipLaunchKernelGGL((k_coarse_solve_b<float>), dim3(c.cn), dim3(256), 0, c.stream, L.v,
                        (const int64_t*)c.cidx, c.cn, B.K, coarse_inverse<float>(c), f, u);
     LAUNCH_CHECK();
     return;
