@@ -9,7 +9,10 @@
 
 A kernel class can consist of several kernels launched together (repeat --kernel, e.g.
 the paired GS pass and the copy of its deferred nodes): their per-dispatch bytes and mean
-durations are added.
+durations are added.  --min-grid keeps only the counter dispatches with at least that
+many work-items (a kernel launched on several multigrid levels: the largest level's
+launches); with --trace (a `<name>_kernel_trace.csv`) the class's mean durations come from
+the traced dispatches under the same filter, else from the per-name stats.
 
 Usage:
   python -m topoptspline3d_amd.rocprof_report --stats DIR/c2_kernel_stats.csv \\
@@ -48,11 +51,30 @@ def kernel_mean(stats_csv, pattern):
     return calls, (ns / calls / 1e3 if calls else None)
 
 
-def counter_per_dispatch(counter_csv, pattern):
-    """(dispatches, mean counter value) over the dispatches of kernels matching `pattern`"""
+def _grid(r):
+    """work-items of a dispatch row (counter CSVs: Grid_Size; trace CSVs: Grid_Size_X/Y/Z)"""
+    if r.get("Grid_Size"):
+        return int(r["Grid_Size"])
+    return int(r.get("Grid_Size_X", 0) or 0) * int(r.get("Grid_Size_Y", 1) or 1) * int(r.get("Grid_Size_Z", 1) or 1)
+
+
+def trace_mean(trace_csv, pattern, min_grid=0):
+    """(dispatches, mean duration in us) of the traced dispatches of kernels matching
+    `pattern` with at least `min_grid` work-items"""
+    n, ns = 0, 0.0
+    for r in csv.DictReader(open(trace_csv)):
+        if _short(r["Kernel_Name"]).startswith(pattern) and _grid(r) >= min_grid:
+            n += 1
+            ns += float(r["End_Timestamp"]) - float(r["Start_Timestamp"])
+    return n, (ns / n / 1e3 if n else None)
+
+
+def counter_per_dispatch(counter_csv, pattern, min_grid=0):
+    """(dispatches, mean counter value) over the dispatches of kernels matching `pattern`
+    with at least `min_grid` work-items"""
     n, v = 0, 0.0
     for r in csv.DictReader(open(counter_csv)):
-        if _short(r["Kernel_Name"]).startswith(pattern):
+        if _short(r["Kernel_Name"]).startswith(pattern) and _grid(r) >= min_grid:
             n += 1
             v += float(r["Counter_Value"])
     return n, (v / n if n else None)
@@ -66,12 +88,12 @@ def calibration(counter_csv, copy_bytes):
     return copy_bytes / (v * 1024.0)
 
 
-def traffic(fetch_csv, write_csv, cal_fetch_csv, cal_write_csv, copy_mib, pattern):
+def traffic(fetch_csv, write_csv, cal_fetch_csv, cal_write_csv, copy_mib, pattern, min_grid=0):
     copy_bytes = copy_mib * (1 << 20)
     kf = calibration(cal_fetch_csv, copy_bytes)
     kw = calibration(cal_write_csv, copy_bytes)
-    nf, f = counter_per_dispatch(fetch_csv, pattern)
-    nw, w = counter_per_dispatch(write_csv, pattern)
+    nf, f = counter_per_dispatch(fetch_csv, pattern, min_grid)
+    nw, w = counter_per_dispatch(write_csv, pattern, min_grid)
     return {"kernel": pattern, "dispatches": [nf, nw], "fetch_kib_raw": f, "write_kib_raw": w,
             "fetch_scale": kf, "write_scale": kw,
             "bytes_per_launch": f * 1024.0 * kf + w * 1024.0 * kw}
@@ -86,6 +108,8 @@ def main(argv=None):
     ap.add_argument("--cal-fetch")
     ap.add_argument("--cal-write")
     ap.add_argument("--copy-mib", type=int, default=1024)
+    ap.add_argument("--min-grid", type=int, default=0, help="dispatches with at least this many work-items")
+    ap.add_argument("--trace", help="kernel trace CSV for the class's mean durations")
     ap.add_argument("--top", type=int, default=30)
     ap.add_argument("--json")
     ap.add_argument("--meta", action="append", default=[], help="key=value recorded in the JSON")
@@ -94,16 +118,18 @@ def main(argv=None):
     names = " + ".join(f"`{k}*`" for k in a.kernel)
     calls, us = 0, 0.0
     for k in a.kernel:
-        ck, uk = kernel_mean(a.stats, k)
+        ck, uk = trace_mean(a.trace, k, a.min_grid) if a.trace else kernel_mean(a.stats, k)
         print(f"\n`{k}*`: {ck} launches, mean {uk:.2f} us")
         calls = ck if not calls else min(calls, ck)
         us += uk
     out = {"kernel": a.kernel if len(a.kernel) > 1 else a.kernel[0], "launches": calls, "mean_us": us}
+    if a.min_grid:
+        out["min_grid"] = a.min_grid
     for kv in a.meta:
         k, v = kv.split("=", 1)
         out[k] = int(v) if v.lstrip("-").isdigit() else v
     if a.fetch:
-        parts = [traffic(a.fetch, a.write, a.cal_fetch, a.cal_write, a.copy_mib, k) for k in a.kernel]
+        parts = [traffic(a.fetch, a.write, a.cal_fetch, a.cal_write, a.copy_mib, k, a.min_grid) for k in a.kernel]
         t = dict(parts[0]) if len(parts) == 1 else {"kernel": a.kernel, "parts": parts,
                                                     "bytes_per_launch": sum(p["bytes_per_launch"] for p in parts)}
         out["traffic"] = t
