@@ -56,6 +56,26 @@ def test_batched_matches_oracle(hip, dims, K, void):
         assert np.abs(ug - uo).max() <= 1e-6 * np.abs(uo).max()
 
 
+@pytest.mark.parametrize("pre,post", [(2, 2), (0, 1)])
+def test_batched_smoothing_counts_match_oracle(hip, pre, post):
+    """V(pre, post) cycles other than V(1,1): the fine level's first sweep from zero
+    followed by ordinary sweeps and the full residual (pre = 2), or no pre-smoothing
+    (pre = 0: cleared start, full residual); run after a V(1,1) solve on the same context,
+    so the captured iteration body must follow the smoothing counts."""
+    dims, K = (24, 12, 12), 3
+    print(f"\n[batched V({pre},{post}) vs oracle] dims={dims} K={K}", flush=True)
+    g, rho = make(hip, *dims, K, seed=6)
+    o, _ = make(oracle_api(), *dims, K, seed=6)
+    g.solve_batched(rtol=1e-8, maxit=300)  # V(1,1) first: its captured iteration must not be replayed below
+    stg = g.solve_batched(rtol=1e-8, maxit=400, pre=pre, post=post)
+    o.solve(rtol=1e-10, maxit=400, precision=0)
+    cg, co = g.compliance(), o.compliance()
+    print("iterations", stg["iterations"])
+    for k in range(K):
+        assert stg["rel_res"][k] <= 1e-8
+        assert abs(cg[k] - co[k]) <= 1e-7 * abs(co[k])
+
+
 def test_batched_matches_sequential_and_replays(hip):
     nx, ny, nz, K = 64, 32, 32, 16
     print(f"\n[batched vs sequential] {nx}x{ny}x{nz} K={K}", flush=True)

@@ -1137,7 +1137,8 @@ struct BatchState {
   bool tuned = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  int graph_gal = -1;  // Galerkin precision the graph was captured with (its coarsest factor)
+  int graph_gal = -1;  // Galerkin precision and smoothing counts the graph was captured with
+  int graph_pre = -1, graph_post = -1;
   std::vector<void*> mem;
 };
 }  // namespace tos
@@ -1488,7 +1489,7 @@ int solve_batched(Ctx& c, const tos_solve_opts& o, tos_solve_stats* st) {
                        (const float*)B.u32[0], (const double*)B.R, (const double*)nullptr, B.part);
     foldb(c, B, nb, 1, B_RZ, 0, OPB_NONE);
     LAUNCH_CHECK();
-    if (B.exec && B.graph_gal != c.gal_prec) {
+    if (B.exec && (B.graph_gal != c.gal_prec || B.graph_pre != o.pre || B.graph_post != o.post)) {
       (void)hipGraphExecDestroy(B.exec);
       (void)hipGraphDestroy(B.graph);
       B.exec = nullptr;
@@ -1496,6 +1497,8 @@ int solve_batched(Ctx& c, const tos_solve_opts& o, tos_solve_stats* st) {
     }
     if (!B.exec && c.graph_replay) {
       B.graph_gal = c.gal_prec;
+      B.graph_pre = o.pre;
+      B.graph_post = o.post;
       c.prof_on = false;
       HIPCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
       batch_iteration(c, B, o.pre, o.post);
