@@ -465,17 +465,4 @@ __global__ void k_interleave(const int4* __restrict__ cl, int nch, int64_t NP, i
   }
 }
 
-template <class A, class B, int BS>
-__global__ __launch_bounds__(BS) void k_cast_b(const int4* __restrict__ cl, int nch, int64_t NP, int K, int KP,
-                                               const A* __restrict__ a, B* __restrict__ b) {
-  NodeRef n;
-  int k = 0;
-  if (!bnode<BS>(cl, nch, 1, K, KP, n, k)) return;
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    const int64_t i = (c * NP + n.p) * K + k;
-    b[i] = (B)a[i];
-  }
-}
-
 }  // namespace tos
