@@ -108,3 +108,26 @@ def test_large_level_schedules_identical(hip):
         out[mode] = (c.vcycle(r, precision=1), c.vcycle(r, precision=2))
     assert np.array_equal(out[0][0], out[3][0])
     assert np.array_equal(out[0][1], out[3][1])
+
+
+def test_large_stencil_level_residual_identical(hip):
+    """A stored-stencil level above 2^20 free nodes (level 1 of a 206^3 box) forms its
+    residual after the zero-start sweep from the later colours only, per colour (mode 0)
+    or per colour pair (mode 3): V-cycles must agree bit for bit."""
+    n = 206
+    solid, support, f = cantilever(n, n, n)
+    c = Context(hip, solid, support)
+    assert c.info()["n_free_nodes"][1] >= 1 << 20
+    rng = np.random.default_rng(12)
+    c.set_material(1e6, 0.4, 3.0, 1e-3)
+    c.set_loads(f)
+    c.set_density(rng.uniform(0.05, 1.0, solid.size).astype(np.float32))
+    c.update_operator()
+    r = rng.normal(size=3 * c.nn)
+    out = {}
+    for mode in (0, 3):
+        print(f"\n[schedule] large stencil level, mode {mode}", flush=True)
+        c.set_sweep_mode(mode)
+        out[mode] = (c.vcycle(r, precision=1), c.vcycle(r, precision=2))
+    assert np.array_equal(out[0][0], out[3][0])
+    assert np.array_equal(out[0][1], out[3][1])

@@ -818,6 +818,33 @@ __global__ __launch_bounds__(256) void k_st_gs_pairw(LevelView L, const int4* __
   }
 }
 
+// residual after a first forward sweep from zero of a node with stencil slot `slot` from
+// its loaded neighbourhood un: later-colour blocks (MASK = nb_mask_after of its colour)
+// and its own block (un[13] holds its value)
+template <class T, unsigned MASK, class ST>
+__device__ __forceinline__ void st_resid_after(const ST& st, int64_t slot, const T (&un)[27][3], T& r0, T& r1,
+                                               T& r2) {
+  T s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+  for (int k = 0; k < 27; ++k) {
+    if (!((MASK >> k) & 1u)) continue;
+    const T u0 = un[k][0], u1 = un[k][1], u2 = un[k][2];
+    s0 = fma(st.get(slot, k, 0), u0, s0); s0 = fma(st.get(slot, k, 1), u1, s0); s0 = fma(st.get(slot, k, 2), u2, s0);
+    s1 = fma(st.get(slot, k, 3), u0, s1); s1 = fma(st.get(slot, k, 4), u1, s1); s1 = fma(st.get(slot, k, 5), u2, s1);
+    s2 = fma(st.get(slot, k, 6), u0, s2); s2 = fma(st.get(slot, k, 7), u1, s2); s2 = fma(st.get(slot, k, 8), u2, s2);
+  }
+  // (explicit fmas: every kernel that calls this rounds the same way, whatever the
+  // compiler would contract around it)
+  const T v1 = un[13][1], v2 = un[13][2];
+  r0 = fma(-st.get(slot, 13, 1), v1, fma(-st.get(slot, 13, 2), v2, -s0));
+  r1 = fma(-st.get(slot, 13, 5), v2, -s1);
+  r2 = -s2;
+  if constexpr (ST::kScaled) {
+    const T m = st.scale(slot);
+    r0 *= m; r1 *= m; r2 *= m;
+  }
+}
+
 // stored-stencil counterpart of k_fine_resid_z (nodes of colour ZC, later-colour blocks only)
 template <class T, int ZC, class ST>
 __global__ __launch_bounds__(256) void k_st_resid_z(LevelView L, const int4* __restrict__ cl, int nch, int64_t slot0,
@@ -825,28 +852,73 @@ __global__ __launch_bounds__(256) void k_st_resid_z(LevelView L, const int4* __r
   constexpr unsigned after = nb_mask_after(ZC);
   NodeRef n;
   if (!chunk_node(cl, nch, 2, n)) return;
-  const int64_t p = n.p, slot = slot0 + n.slot;
-  T s0 = 0, s1 = 0, s2 = 0;
-#pragma unroll
-  for (int k = 0; k < 27; ++k) {
-    if (!((after >> k) & 1u)) continue;
-    const int64_t q = p + (int64_t)(k % 3 - 1) + (int64_t)L.NX * ((k / 3) % 3 - 1 + (int64_t)L.NY * (k / 9 - 1));
-    const T u0 = u[q], u1 = u[L.NP + q], u2 = u[2 * L.NP + q];
-    s0 = fma(st.get(slot, k, 0), u0, s0); s0 = fma(st.get(slot, k, 1), u1, s0); s0 = fma(st.get(slot, k, 2), u2, s0);
-    s1 = fma(st.get(slot, k, 3), u0, s1); s1 = fma(st.get(slot, k, 4), u1, s1); s1 = fma(st.get(slot, k, 5), u2, s1);
-    s2 = fma(st.get(slot, k, 6), u0, s2); s2 = fma(st.get(slot, k, 7), u1, s2); s2 = fma(st.get(slot, k, 8), u2, s2);
-  }
-  const T v1 = u[L.NP + p], v2 = u[2 * L.NP + p];
-  T r0 = -st.get(slot, 13, 1) * v1 - st.get(slot, 13, 2) * v2 - s0;
-  T r1 = -st.get(slot, 13, 5) * v2 - s1;
-  T r2 = -s2;
-  if constexpr (ST::kScaled) {
-    const T m = st.scale(slot);
-    r0 *= m; r1 *= m; r2 *= m;
-  }
+  const int64_t p = n.p;
+  T un[27][3];
+  load_nbhd<T, -1, after | (1u << 13)>(L, u, p, un);
+  T r0, r1, r2;
+  st_resid_after<T, after>(st, slot0 + n.slot, un, r0, r1, r2);
   r[p] = r0;
   r[L.NP + p] = r1;
   r[2 * L.NP + p] = r2;
+}
+
+// k_st_resid_z for the two colours ZA, ZB = ZA + 1 of a class in one pass, on the forward
+// segments of k_st_gs_pairw (lane l: the ZA node x1 and the ZB node x1 + 1 from one load
+// of x1's neighbourhood plus lane l+1's neighbour column; stencil rows by the slot map)
+template <class T, int ZA, int ZB, class ST>
+__global__ __launch_bounds__(256) void k_st_resid_pairw(LevelView L, const int4* __restrict__ seg, int nseg,
+                                                        const uint8_t* __restrict__ fre,
+                                                        const int* __restrict__ slotmap, ST st,
+                                                        const T* __restrict__ u, T* __restrict__ r) {
+  constexpr unsigned MA = nb_mask_after(ZA), MB = nb_mask_after(ZB);
+  constexpr unsigned LA = MA | (1u << 13), LB = MB | (1u << 13);  // (own values too)
+  const int si = (int)xcd_block() * 4 + (threadIdx.x >> 6);
+  if (si >= nseg) return;  // whole wave
+  const int lane = threadIdx.x & 63;
+  const int4 S = seg[si];
+  const int xs = (S.y & 0x7fff) - 1, xe = ((S.y >> 15) & 0x7fff) - 1;
+  const int x1 = xs + 2 * lane;
+  const bool live = x1 <= xe + 1;
+  const int xa = live ? x1 : xe + 1;  // clamped as in k_st_gs_pairw
+  const int64_t pa = (int64_t)S.x + (xa - xs);
+  const bool f1 = live && fre[pa] != 0;
+  const bool f2 = live && lane < 63 && x1 + 1 <= xe && fre[pa + 1] != 0;
+  T un[27][3];
+#pragma unroll
+  for (int k = 0; k < 27; ++k) {
+    const int dx = k % 3 - 1;
+    const bool need = dx < 0 ? ((LA >> k) & 1u) != 0
+                              : (((LA >> k) & 1u) || ((LB >> (k - 1)) & 1u) ||
+                                 (dx == 0 && ((LB >> (k + 1)) & 1u)));
+    const int64_t q = (dx < 0 ? (xa >= 0 ? pa - 1 : pa) : pa + dx) +
+                      (int64_t)L.NX * ((k / 3) % 3 - 1 + (int64_t)L.NY * (k / 9 - 1));
+#pragma unroll
+    for (int c = 0; c < 3; ++c) un[k][c] = need ? u[c * L.NP + q] : T(0);
+  }
+  const int sl1 = slotmap[pa], sl2 = slotmap[pa + 1];
+  if (f1) {
+    T r0, r1, r2;
+    st_resid_after<T, MA>(st, sl1, un, r0, r1, r2);
+    r[pa] = r0; r[L.NP + pa] = r1; r[2 * L.NP + pa] = r2;
+  }
+  T vn[27][3];
+#pragma unroll
+  for (int k = 0; k < 27; ++k) {
+    const int dx = k % 3 - 1;
+    if (dx <= 0) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) vn[k][c] = un[k + 1][c];
+    } else {
+      const bool need = ((LB >> k) & 1u) != 0;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) vn[k][c] = need ? __shfl_down(un[k - 1][c], 1, 64) : T(0);
+    }
+  }
+  if (f2) {
+    T r0, r1, r2;
+    st_resid_after<T, MB>(st, sl2, vn, r0, r1, r2);
+    r[pa + 1] = r0; r[L.NP + pa + 1] = r1; r[2 * L.NP + pa + 1] = r2;
+  }
 }
 
 // ---------------------------------------------------------------------------------------

@@ -863,6 +863,27 @@ void resid_after_zero_sweep(Ctx& c, int l, const T* u, T* r) {
       LAUNCH_CHECK();
       return;
     }
+    if (l > 0 && c.sweep_mode == 3 && L.pair_ok) {  // stored stencils: two colours per pass likewise
+      st_dispatch<T>(c, L, [&](auto st) {
+        using ST = decltype(st);
+        for (int cls = 0; cls < 4; ++cls) {
+          const int n = L.n_segw[cls][0];
+          if (!n) continue;
+          const dim3 g((n + 3) / 4), b(256);
+#define GO(A) hipLaunchKernelGGL((k_st_resid_pairw<float, A, A + 1, ST>), g, b, 0, c.stream, v, \
+                                 (const int4*)L.segw[cls][0], n, (const uint8_t*)L.fre, (const int*)L.slot, st, u, r)
+          switch (cls) {
+            case 0: GO(0); break;
+            case 1: GO(2); break;
+            case 2: GO(4); break;
+            default: GO(6); break;
+          }
+#undef GO
+        }
+      });
+      LAUNCH_CHECK();
+      return;
+    }
   }
   for (int col = 0; col < 8; ++col) {
     const int off = L.col_off[col], nch = L.col_off[col + 1] - off;
