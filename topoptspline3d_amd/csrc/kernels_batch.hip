@@ -115,17 +115,8 @@ __global__ __launch_bounds__(BS) void k_fine_gs_b(LevelView L, const int4* __res
   T D[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   fine_row<T, false, mask>(wa, Ke, un, s, D);
   const int64_t i0 = n.p * K + k, cs = L.NP * K;
-  const T b0 = f[i0] - s[0], b1 = f[cs + i0] - s[1], b2 = f[2 * cs + i0] - s[2];
   T v0 = ZC < 0 ? un[13][0] : T(0), v1 = ZC < 0 ? un[13][1] : T(0), v2 = ZC < 0 ? un[13][2] : T(0);
-  if (!BACKWARD) {
-    v0 = (b0 - D[1] * v1 - D[2] * v2) / D[0];
-    v1 = (b1 - D[3] * v0 - D[5] * v2) / D[4];
-    v2 = (b2 - D[6] * v0 - D[7] * v1) / D[8];
-  } else {
-    v2 = (b2 - D[6] * v0 - D[7] * v1) / D[8];
-    v1 = (b1 - D[3] * v0 - D[5] * v2) / D[4];
-    v0 = (b0 - D[1] * v1 - D[2] * v2) / D[0];
-  }
+  block_relax<T, BACKWARD>(f[i0] - s[0], f[cs + i0] - s[1], f[2 * cs + i0] - s[2], D, v0, v1, v2);
   u[i0] = v0;
   u[cs + i0] = v1;
   u[2 * cs + i0] = v2;
@@ -206,17 +197,8 @@ __global__ __launch_bounds__(BS) void k_st_gs_b(LevelView L, const int4* __restr
 #pragma unroll
   for (int e = 0; e < 9; ++e) D[e] = S.get(slot, 13, e);
   const int64_t i0 = n.p * K + k;
-  const T b0 = f[i0] - s0, b1 = f[cs + i0] - s1, b2 = f[2 * cs + i0] - s2;
   T v0 = un[13][0], v1 = un[13][1], v2 = un[13][2];
-  if (!BACKWARD) {
-    v0 = (b0 - D[1] * v1 - D[2] * v2) / D[0];
-    v1 = (b1 - D[3] * v0 - D[5] * v2) / D[4];
-    v2 = (b2 - D[6] * v0 - D[7] * v1) / D[8];
-  } else {
-    v2 = (b2 - D[6] * v0 - D[7] * v1) / D[8];
-    v1 = (b1 - D[3] * v0 - D[5] * v2) / D[4];
-    v0 = (b0 - D[1] * v1 - D[2] * v2) / D[0];
-  }
+  block_relax<T, BACKWARD>(f[i0] - s0, f[cs + i0] - s1, f[2 * cs + i0] - s2, D, v0, v1, v2);
   u[i0] = v0;
   u[cs + i0] = v1;
   u[2 * cs + i0] = v2;

@@ -205,15 +205,17 @@ __global__ TOS_FINE_BOUNDS(T) void k_fine_apply(LevelView L, const int4* __restr
 // point Gauss-Seidel on the node's 3x3 diagonal block: components x, y, z (BACKWARD:
 // z, y, x), each with the others' latest values; b = f - off-diagonal sum
 template <class T, bool BACKWARD>
+// (explicit fmas: every GS kernel rounds the same way, so the schedules stay bit-identical
+// whatever the compiler would contract in each of them)
 __device__ __forceinline__ void block_relax(T b0, T b1, T b2, const T (&D)[9], T& v0, T& v1, T& v2) {
   if (!BACKWARD) {
-    v0 = (b0 - D[1] * v1 - D[2] * v2) / D[0];
-    v1 = (b1 - D[3] * v0 - D[5] * v2) / D[4];
-    v2 = (b2 - D[6] * v0 - D[7] * v1) / D[8];
+    v0 = fma(-D[2], v2, fma(-D[1], v1, b0)) / D[0];
+    v1 = fma(-D[5], v2, fma(-D[3], v0, b1)) / D[4];
+    v2 = fma(-D[7], v1, fma(-D[6], v0, b2)) / D[8];
   } else {
-    v2 = (b2 - D[6] * v0 - D[7] * v1) / D[8];
-    v1 = (b1 - D[3] * v0 - D[5] * v2) / D[4];
-    v0 = (b0 - D[1] * v1 - D[2] * v2) / D[0];
+    v2 = fma(-D[7], v1, fma(-D[6], v0, b2)) / D[8];
+    v1 = fma(-D[5], v2, fma(-D[3], v0, b1)) / D[4];
+    v0 = fma(-D[2], v2, fma(-D[1], v1, b0)) / D[0];
   }
 }
 
@@ -230,17 +232,8 @@ __device__ __forceinline__ void relax_fine(const LevelView& L, const T* __restri
   T s[3] = {0, 0, 0};
   T D[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   fine_row<T, false>(wa, Ke, un, s, D);
-  const T b0 = f0 - s[0], b1 = f1 - s[1], b2 = f2 - s[2];
   T v0 = un[13][0], v1 = un[13][1], v2 = un[13][2];
-  if (!BACKWARD) {
-    v0 = (b0 - D[1] * v1 - D[2] * v2) / D[0];
-    v1 = (b1 - D[3] * v0 - D[5] * v2) / D[4];
-    v2 = (b2 - D[6] * v0 - D[7] * v1) / D[8];
-  } else {
-    v2 = (b2 - D[6] * v0 - D[7] * v1) / D[8];
-    v1 = (b1 - D[3] * v0 - D[5] * v2) / D[4];
-    v0 = (b0 - D[1] * v1 - D[2] * v2) / D[0];
-  }
+  block_relax<T, BACKWARD>(f0 - s[0], f1 - s[1], f2 - s[2], D, v0, v1, v2);
   u[p] = v0;
   u[L.NP + p] = v1;
   u[2 * L.NP + p] = v2;
@@ -291,7 +284,8 @@ __global__ TOS_FINE_BOUNDS(T) void k_fine_gs_pair(LevelView L, const int4* __res
 // arrive by lane shuffles instead of a second pass over memory.  Segments of a row start
 // 126 apart, so the c1 node of lane 63 is also lane 0 of the next segment (computed twice
 // from the same inputs, written once).  Those two computations read the old values of
-// the c2 nodes of lanes 0 and 62, so the new values of these two c2 nodes go to `ub` and
+// that c1 node itself (its start value) and of the c2 nodes of lanes 0 and 62, and the
+// two waves run in no fixed order, so the new values of these three nodes go to `ub` and
 // are copied into u by k_fine_pairw_copy after the pass.  Same updates as two colour
 // passes (bit-identical); each neighbourhood is streamed once for both colours.
 // seg = {padded index of (xs, y, z), (xs + 1) | (xe + 1) << 15 | cont << 30, y, z}: xe the
@@ -394,7 +388,8 @@ __global__ TOS_FINE_BOUNDS(T) void k_fine_gs_pairw(LevelView L, const int4* __re
     block_relax<T, BACKWARD>(fb[0] - s[0], fb[1] - s[1], fb[2] - s[2], D, t0, t1, t2);
   }
   if (f1 && !(cont && lane == 0)) {
-    u[p1] = v0; u[L.NP + p1] = v1; u[2 * L.NP + p1] = v2;
+    T* dst = lane == 63 ? ub : u;  // (the next segment's lane 0 reads its start value)
+    dst[p1] = v0; dst[L.NP + p1] = v1; dst[2 * L.NP + p1] = v2;
   }
   if (f2) {
     T* dst = (lane == 0 || lane == 62) ? ub : u;
@@ -485,16 +480,19 @@ __global__ TOS_FINE_BOUNDS(T) void k_fine_resid_pairw(LevelView L, const int4* _
   }
 }
 
-// the deferred c2 nodes of k_fine_gs_pairw (lanes 0 and 62 of every segment): ub -> u
+// the deferred nodes of k_fine_gs_pairw / k_st_gs_pairw (the c2 nodes of lanes 0 and 62
+// and the c1 node of lane 63 of every segment, x = xs + 1, xs + 125, xs + 126): ub -> u.
+// Each was written in the pass exactly when it is free and at most xe.
 template <class T>
 __global__ __launch_bounds__(256) void k_fine_pairw_copy(LevelView L, const int4* __restrict__ seg, int nseg,
                                                          const uint8_t* __restrict__ fre, const T* __restrict__ ub,
                                                          T* __restrict__ u) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= 2 * (int64_t)nseg) return;
-  const int4 S = seg[t >> 1];
+  if (t >= 3 * (int64_t)nseg) return;
+  const int4 S = seg[t / 3];
   const int xs = (S.y & 0x7fff) - 1, xe = ((S.y >> 15) & 0x7fff) - 1;
-  const int dxn = (t & 1) ? 125 : 1;
+  const int j = (int)(t % 3);
+  const int dxn = j == 0 ? 1 : (j == 1 ? 125 : 126);
   if (xs + dxn > xe) return;
   const int64_t p = (int64_t)S.x + dxn;
   if (!fre[p]) return;
@@ -810,7 +808,8 @@ __global__ __launch_bounds__(256) void k_st_gs_pairw(LevelView L, const int4* __
   T t0 = ZB < 0 ? vn[13][0] : T(0), t1 = ZB < 0 ? vn[13][1] : T(0), t2 = ZB < 0 ? vn[13][2] : T(0);
   if (f2) st_relax<T, BACKWARD, MB>(st, sl2, vn, fb[0], fb[1], fb[2], t0, t1, t2);
   if (f1 && !(cont && lane == 0)) {
-    u[pa] = v0; u[L.NP + pa] = v1; u[2 * L.NP + pa] = v2;
+    T* dst = (ub && lane == 63) ? ub : u;  // (the next segment's lane 0 reads its start value)
+    dst[pa] = v0; dst[L.NP + pa] = v1; dst[2 * L.NP + pa] = v2;
   }
   if (f2) {
     T* dst = (ub && (lane == 0 || lane == 62)) ? ub : u;

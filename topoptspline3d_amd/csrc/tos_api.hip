@@ -690,7 +690,7 @@ void gs_sweep_pairw(Ctx& c, float* u, const float* f, bool backward, bool from_z
       }
 #undef GO
     }
-    hipLaunchKernelGGL((k_fine_pairw_copy<float>), dim3(nblk(2 * (int64_t)n)), dim3(256), 0, c.stream, v, sg, n, fr,
+    hipLaunchKernelGGL((k_fine_pairw_copy<float>), dim3(nblk(3 * (int64_t)n)), dim3(256), 0, c.stream, v, sg, n, fr,
                        (const float*)L.ub32, u);
   }
   LAUNCH_CHECK();
@@ -734,7 +734,7 @@ void gs_sweep_st_pairw(Ctx& c, int l, T* u, const T* f, bool backward, bool from
       }
     });
     if (ub)
-      hipLaunchKernelGGL((k_fine_pairw_copy<T>), dim3(nblk(2 * (int64_t)n)), dim3(256), 0, c.stream, v, sg, n,
+      hipLaunchKernelGGL((k_fine_pairw_copy<T>), dim3(nblk(3 * (int64_t)n)), dim3(256), 0, c.stream, v, sg, n,
                          (const uint8_t*)L.fre, (const T*)ub, u);
   }
   LAUNCH_CHECK();
