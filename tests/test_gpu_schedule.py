@@ -113,7 +113,7 @@ def test_large_level_schedules_identical(hip):
 def test_large_stencil_level_residual_identical(hip):
     """A stored-stencil level above 2^20 free nodes (level 1 of a 206^3 box) forms its
     residual after the zero-start sweep from the later colours only, per colour (mode 0)
-    or per colour pair (mode 3): V-cycles must agree bit for bit."""
+    or per colour pair (mode 3): V-cycles must agree bit for bit, and repeat bit for bit."""
     n = 206
     solid, support, f = cantilever(n, n, n)
     c = Context(hip, solid, support)
@@ -131,3 +131,8 @@ def test_large_stencil_level_residual_identical(hip):
         out[mode] = (c.vcycle(r, precision=1), c.vcycle(r, precision=2))
     assert np.array_equal(out[0][0], out[3][0])
     assert np.array_equal(out[0][1], out[3][1])
+    # rows of two segments on the fine level: the paired passes must not depend on the
+    # order in which the waves of a row's segments run
+    for rep in range(3):
+        assert np.array_equal(c.vcycle(r, precision=1), out[3][0]), rep
+        assert np.array_equal(c.vcycle(r, precision=2), out[3][1]), rep
