@@ -133,6 +133,41 @@ int tos_compliance(tos_ctx* ctx, double* c);
 /* sum(rho)/N_active over active elements (Grid::volumeRatio, grid/Grid.cu:5462-5470). */
 int tos_volume(tos_ctx* ctx, double* v);
 
+/* ---- worst-case load (modified power method) --------------------------------------
+ * Replaces modifiedPM (optimization/optimization.cpp:213-351) with forceProject
+ * (optimization/projection.cpp:289-350, optimization/projection.cu:193-250) on a grid
+ * with supports.  The force lives on the load nodes only.  Start: a random force on the
+ * load nodes, projected and scaled to unit 2-norm, u = 0.  Each step runs one
+ * V(pre,post)-cycle on the displacement of load case 0 (u += B(f - K u)), takes the new
+ * force from u on the load nodes (its component along each node's normal when normals
+ * are given, all of it otherwise), scales it to unit 2-norm, and stops once
+ * ||f_new - f_old||_2 / ||f_old||_2 <= fch_tol and the cycle's relative residual
+ * ||f - K u|| / ||f|| <= res_tol, or after max_iter cycles.  On return load case 0
+ * holds the last force and displacement and stats->compliance = f . u (the worst-case
+ * compliance).  A cycle residual above 1e4 returns TOS_E_NOCONV (the reference exits). */
+typedef struct tos_worst_opts {
+  int n_load;               /* load nodes: free fine nodes, no repeats                  */
+  const int64_t* nodes;     /* their lexicographic node ids                             */
+  const double* normals;    /* 3*n_load (x, y, z per node; any nonzero length) or NULL:
+                               the force direction is free                              */
+  uint64_t seed;            /* start force: component c of load node i is the
+                               (3i+c+1)-th splitmix64 value of seed mapped to [-1, 1)
+                               (the reference draws it from a time-seeded cuRAND)       */
+  int precision;            /* V-cycle precision as in tos_solve_opts                   */
+  int pre, post;            /* reference: 1, 1                                          */
+  int max_iter;             /* reference: 500                                           */
+  double fch_tol, res_tol;  /* reference: 1e-4, 1e-2                                    */
+} tos_worst_opts;
+
+typedef struct tos_worst_stats {
+  int iterations;           /* V-cycles run                                             */
+  double compliance;        /* f . u on return                                          */
+  double fch, rel_res;      /* last force change and last cycle residual                */
+  double seconds;
+} tos_worst_stats;
+
+int tos_worst_compliance(tos_ctx* ctx, const tos_worst_opts* opts, tos_worst_stats* stats);
+
 /* ---- sensitivities ---------------------------------------------------------------
  * Sensitivity of case k: s_e = -p rho_e^(p-1) u_e^T Ke u_e (computeSensitivity_kernel,
  * optimization/optimization.cu:96-159), into the context's sensitivity buffer. */

@@ -1363,6 +1363,115 @@ ORC_API int orc_volume(tos_ctx* c, double* v) {
   return TOS_OK;
 }
 
+// ------------------------------------------------------------------------------------
+// Worst-case load by the modified power method: modifiedPM (optimization/
+// optimization.cpp:213-351) with forceProject (optimization/projection.cpp:289-350) on a
+// grid with supports.  Start force: random on the load nodes (Grid::randForce,
+// grid/Grid.cu:3018-3021, here splitmix64 instead of a time-seeded cuRAND), projected,
+// unit 2-norm (Grid::unitizeForce :3023-3029), u = 0.  Step: one V-cycle on u
+// (HierarchyGrid::v_cycle relaxes u against f, i.e. u += B(f - K u)), relative residual,
+// new force = u on the load nodes, projected, unit norm; force change
+// |f_new - f_old| / |f_old| (Grid::supportForceCh / supportForceNorm :3031-3061).
+// Projection with a normal n (unit): f <- (n . f) n, which is what removing the two
+// tangential components with an orthonormal tangent pair (f -= N^T N f) leaves.
+// ------------------------------------------------------------------------------------
+static double splitmix_pm1(uint64_t& s) {  // splitmix64 (Steele, Lea, Flood 2014) -> [-1, 1)
+  uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return (double)(z >> 11) * 0x1.0p-52 - 1.0;
+}
+
+ORC_API int orc_worst_compliance(tos_ctx* c, const tos_worst_opts* o, tos_worst_stats* st) {
+  if (!c || !o || o->n_load <= 0 || !o->nodes || o->max_iter < 0 || o->pre < 0 || o->post < 0 ||
+      o->precision < 0 || o->precision > 2 || o->fch_tol < 0 || o->res_tol < 0)
+    return TOS_E_ARG;
+  if (!c->op_ready) { c->err = "update_operator first"; return TOS_E_STATE; }
+  Level& L = c->L[0];
+  const int64_t nn = L.nn, n3 = 3 * nn;
+  const int n = o->n_load;
+  std::vector<uint8_t> seen(nn, 0);
+  std::vector<double> nrm(3 * (size_t)n, 0.0), fs(3 * (size_t)n), g(3 * (size_t)n);
+  for (int i = 0; i < n; ++i) {
+    const int64_t id = o->nodes[i];
+    if (id < 0 || id >= nn || !L.fre[id] || seen[id]) { c->err = "bad load node"; return TOS_E_ARG; }
+    seen[id] = 1;
+    if (o->normals) {
+      const double* a = o->normals + 3 * (size_t)i;
+      const double len = std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+      if (!(len > 0) || !std::isfinite(len)) { c->err = "bad normal"; return TOS_E_ARG; }
+      for (int k = 0; k < 3; ++k) nrm[3 * (size_t)i + k] = a[k] / len;
+    }
+  }
+  auto t0 = std::chrono::steady_clock::now();
+  double* u = c->U.data();  // load case 0
+  double* f = c->F.data();
+  // g <- projection of g, then f = g / |g| on the load nodes (0 elsewhere); returns |g|^2
+  auto set_force = [&]() {
+    double g2 = 0;
+    for (int i = 0; i < n; ++i) {
+      double* gi = &g[3 * (size_t)i];
+      if (o->normals) {
+        const double* ni = &nrm[3 * (size_t)i];
+        const double d = ni[0] * gi[0] + ni[1] * gi[1] + ni[2] * gi[2];
+        for (int k = 0; k < 3; ++k) gi[k] = d * ni[k];
+      }
+      g2 += gi[0] * gi[0] + gi[1] * gi[1] + gi[2] * gi[2];
+    }
+    if (!(g2 > 0) || !std::isfinite(g2)) return g2;
+    const double inv = 1.0 / std::sqrt(g2);
+    for (int i = 0; i < n; ++i)
+      for (int k = 0; k < 3; ++k) f[k * nn + o->nodes[i]] = g[3 * (size_t)i + k] * inv;
+    return g2;
+  };
+  uint64_t seed = o->seed;
+  for (double& x : g) x = splitmix_pm1(seed);
+  std::fill(f, f + n3, 0.0);
+  std::fill(u, u + n3, 0.0);
+  const double g0 = set_force();
+  if (!(g0 > 0) || !std::isfinite(g0)) { c->err = "the start force is zero after the projection"; return TOS_E_ARG; }
+  for (int i = 0; i < n; ++i)
+    for (int k = 0; k < 3; ++k) fs[3 * (size_t)i + k] = f[k * nn + o->nodes[i]];
+  std::vector<double> r(n3), z(n3), q(n3);
+  int it = 0;
+  double fch = 1, rel = 1;
+  while (it < o->max_iter && (fch > o->fch_tol || rel > o->res_tol)) {
+    apply_fine(*c, u, q.data());
+    for (int64_t i = 0; i < n3; ++i) r[i] = f[i] - q[i];
+    precondition(*c, r.data(), z.data(), o->pre, o->post);
+    for (int64_t i = 0; i < n3; ++i) u[i] += z[i];
+    apply_fine(*c, u, q.data());
+    for (int64_t i = 0; i < n3; ++i) r[i] = f[i] - q[i];
+    rel = std::sqrt(dot3(L, r.data(), r.data()) / dot3(L, f, f));
+    ++it;
+    if (!std::isfinite(rel)) { c->err = "NaN in the power iteration"; return TOS_E_NAN; }
+    if (rel > 1e4) { c->err = "power iteration diverged"; return TOS_E_NOCONV; }
+    for (int i = 0; i < n; ++i)
+      for (int k = 0; k < 3; ++k) g[3 * (size_t)i + k] = u[k * nn + o->nodes[i]];
+    const double g2 = set_force();
+    if (!(g2 > 0) || !std::isfinite(g2)) { c->err = "displacement vanished on the load nodes"; return TOS_E_NAN; }
+    double dd = 0, oo = 0;
+    for (int i = 0; i < n; ++i)
+      for (int k = 0; k < 3; ++k) {
+        const double fn = f[k * nn + o->nodes[i]], fo = fs[3 * (size_t)i + k];
+        dd += (fn - fo) * (fn - fo);
+        oo += fo * fo;
+        fs[3 * (size_t)i + k] = fn;
+      }
+    fch = std::sqrt(dd) / std::sqrt(oo);
+  }
+  const double comp = dot3(L, f, u);
+  if (st) {
+    st->iterations = it;
+    st->compliance = comp;
+    st->fch = fch;
+    st->rel_res = rel;
+    st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  return TOS_OK;
+}
+
 ORC_API int orc_sensitivity(tos_ctx* c, int k) {
   if (!c || k < 0 || k >= c->K) return TOS_E_ARG;
   const Level& L = c->L[0];

@@ -37,6 +37,17 @@ class SolveStats(C.Structure):
                 ("seconds", C.c_double)]
 
 
+class WorstOpts(C.Structure):
+    _fields_ = [("n_load", C.c_int), ("nodes", C.POINTER(C.c_int64)), ("normals", C.POINTER(C.c_double)),
+                ("seed", C.c_uint64), ("precision", C.c_int), ("pre", C.c_int), ("post", C.c_int),
+                ("max_iter", C.c_int), ("fch_tol", C.c_double), ("res_tol", C.c_double)]
+
+
+class WorstStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("compliance", C.c_double), ("fch", C.c_double),
+                ("rel_res", C.c_double), ("seconds", C.c_double)]
+
+
 class KernelStats(C.Structure):
     _fields_ = [("n_classes", C.c_int), ("names", (C.c_char * 32) * 20),
                 ("total_ms", C.c_double * 20), ("launches", C.c_int64 * 20)]
@@ -67,6 +78,7 @@ SIGNATURES = {
     "set_displacement": (C.c_int, [vp, C.c_int, f64p]),
     "compliance": (C.c_int, [vp, f64p]),
     "volume": (C.c_int, [vp, f64p]),
+    "worst_compliance": (C.c_int, [vp, C.POINTER(WorstOpts), C.POINTER(WorstStats)]),
     "sensitivity": (C.c_int, [vp, C.c_int]),
     "get_sensitivity": (C.c_int, [vp, C.c_int, f32p]),
     "set_sensitivity": (C.c_int, [vp, C.c_int, f32p]),

@@ -202,6 +202,26 @@ class Context:
         self._chk(self.api.volume(self.h, C.byref(v)), "volume")
         return v.value
 
+    def worst_compliance(self, nodes, normals=None, seed=0, precision=0, pre=1, post=1, max_iter=500,
+                         fch_tol=1e-4, res_tol=1e-2, check=True):
+        """worst-case load over the load nodes by the modified power method (modifiedPM,
+        optimization/optimization.cpp:213-351); load case 0 then holds the worst force and
+        its displacement.  normals: (n, 3) force directions, or None for a free direction."""
+        nodes = np.ascontiguousarray(nodes, dtype=np.int64).ravel()
+        o = _abi.WorstOpts()
+        o.n_load, o.nodes, o.seed = nodes.size, _p(nodes, C.c_int64), int(seed)
+        if normals is not None:
+            normals = np.ascontiguousarray(normals, dtype=np.float64).reshape(nodes.size, 3)
+            o.normals = _p(normals, C.c_double)
+        o.precision, o.pre, o.post, o.max_iter = int(precision), int(pre), int(post), int(max_iter)
+        o.fch_tol, o.res_tol = float(fch_tol), float(res_tol)
+        st = _abi.WorstStats()
+        rc = self.api.worst_compliance(self.h, C.byref(o), C.byref(st))
+        if check:
+            self._chk(rc, "worst_compliance")
+        return {"rc": rc, "iterations": st.iterations, "compliance": st.compliance, "fch": st.fch,
+                "rel_res": st.rel_res, "seconds": st.seconds}
+
     def sensitivity(self, k=0):
         self._chk(self.api.sensitivity(self.h, k), "sensitivity")
 

@@ -545,4 +545,82 @@ __global__ void k_unpad_vec(LevelView L, const double* __restrict__ src, double*
   for (int c = 0; c < 3; ++c) dst[c * n + t] = src[c * L.NP + p];
 }
 
+// ---------------------------------------------------------------------------------------
+// Worst-case load, modified power method (optimization/optimization.cpp:289-300 with
+// forceProject, optimization/projection.cpp:289-350).  Load node i sits at padded index
+// pidx[i]; its force direction is free (nrm == nullptr) or along the unit normal
+// nrm[3i..3i+2].  The candidate force is g_i = u_i, or (n_i . u_i) n_i; a node-major
+// source (src_nm, 3 values per node) replaces u at the start.  The reference removes the
+// two tangential components (f -= N^T N f with an orthonormal tangent pair), which
+// leaves the same normal component.  Grid-stride loops over a fixed grid: every block
+// writes one partial, so the sums do not depend on timing.
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ void worst_candidate(int i, const int64_t* __restrict__ pidx,
+                                                const double* __restrict__ nrm, int64_t NP,
+                                                const double* __restrict__ u, const double* __restrict__ src_nm,
+                                                double g[3]) {
+  if (src_nm) {
+    g[0] = src_nm[3 * i]; g[1] = src_nm[3 * i + 1]; g[2] = src_nm[3 * i + 2];
+  } else {
+    const int64_t p = pidx[i];
+    g[0] = u[p]; g[1] = u[NP + p]; g[2] = u[2 * NP + p];
+  }
+  if (nrm) {
+    const double nx = nrm[3 * i], ny = nrm[3 * i + 1], nz = nrm[3 * i + 2];
+    const double d = fma(nz, g[2], fma(ny, g[1], nx * g[0]));
+    g[0] = d * nx; g[1] = d * ny; g[2] = d * nz;
+  }
+}
+
+// partial[b] = sum of |g_i|^2 over the block's load nodes
+__global__ __launch_bounds__(256) void k_worst_norm(int n, const int64_t* __restrict__ pidx,
+                                                    const double* __restrict__ nrm, int64_t NP,
+                                                    const double* __restrict__ u, const double* __restrict__ src_nm,
+                                                    double* __restrict__ partial) {
+  double s = 0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    double g[3];
+    worst_candidate(i, pidx, nrm, NP, u, src_nm, g);
+    s = fma(g[2], g[2], fma(g[1], g[1], fma(g[0], g[0], s)));
+  }
+  block_partial(s, partial, 0);
+}
+
+// f_i = g_i / sqrt(scal[g2_slot]) on the load nodes; r (and its FP32 copy r32, when
+// given) += f_new - f_old there, so r stays f - K u; fs (node-major) = f_new.
+// partials: set 0 |f_new - f_old|^2, set 1 |f_old|^2.  `first`: f_old = 0, r untouched.
+__global__ __launch_bounds__(256) void k_worst_apply(int n, const int64_t* __restrict__ pidx,
+                                                     const double* __restrict__ nrm, int64_t NP,
+                                                     const double* __restrict__ u,
+                                                     const double* __restrict__ src_nm,
+                                                     const double* __restrict__ scal, int g2_slot, int first,
+                                                     double* __restrict__ f, double* __restrict__ r,
+                                                     float* __restrict__ r32, double* __restrict__ fs,
+                                                     double* __restrict__ partial) {
+  const double inv = 1.0 / sqrt(scal[g2_slot]);
+  double sd = 0, so = 0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    double g[3];
+    worst_candidate(i, pidx, nrm, NP, u, src_nm, g);
+    const int64_t p = pidx[i];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double fn = g[c] * inv;
+      const double fo = first ? 0.0 : fs[3 * i + c];
+      const double d = fn - fo;
+      sd = fma(d, d, sd);
+      so = fma(fo, fo, so);
+      f[c * NP + p] = fn;
+      fs[3 * i + c] = fn;
+      if (!first) {
+        const double rn = r[c * NP + p] + d;
+        r[c * NP + p] = rn;
+        if (r32) r32[c * NP + p] = (float)rn;
+      }
+    }
+  }
+  block_partial(sd, partial, 0);
+  block_partial(so, partial, 1);
+}
+
 }  // namespace tos

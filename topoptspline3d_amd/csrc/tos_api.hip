@@ -2651,6 +2651,127 @@ TOS_EXPORT int tos_volume(tos_ctx* c, double* v) {
   });
 }
 
+// splitmix64 (Steele, Lea and Flood, "Fast splittable pseudorandom number generators",
+// OOPSLA 2014): the start force of the worst-case load search, mapped to [-1, 1)
+static double splitmix_pm1(uint64_t& s) {
+  uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return (double)(z >> 11) * 0x1.0p-52 - 1.0;
+}
+
+// Modified power method for the worst-case load (modifiedPM, optimization/optimization.cpp:
+// 213-351).  Device-resident loop: per step one V-cycle, the residual r = f - K u with its
+// squared norm, and the force update on the load nodes (two small passes, which also move
+// r to the new force, so the next cycle needs no extra operator application); one
+// readback of the scalars per step for the stop test, as the reference's host loop.
+TOS_EXPORT int tos_worst_compliance(tos_ctx* c, const tos_worst_opts* o, tos_worst_stats* st) {
+  return guard(c, [&]() {
+    REQUIRE(o && o->n_load > 0 && o->nodes && o->max_iter >= 0 && o->pre >= 0 && o->post >= 0 &&
+                o->precision >= 0 && o->precision <= 2 && o->fch_tol >= 0 && o->res_tol >= 0,
+            "bad options");
+    if (!c->op_ready) { c->err = "update_operator first"; return TOS_E_STATE; }
+    Ctx& C = *c;
+    Level& L0 = C.L[0];
+    const LevelView& v = L0.v;
+    const int64_t nn = (int64_t)v.nx * v.ny * v.nz, NP = v.NP, n3p = 3 * NP;
+    const int n = o->n_load;
+    std::vector<int64_t> pidx(n);
+    std::vector<double> nrm(o->normals ? 3 * (size_t)n : 0), f0(3 * (size_t)n);
+    std::vector<uint8_t> seen(nn, 0);
+    for (int i = 0; i < n; ++i) {
+      const int64_t id = o->nodes[i];
+      REQUIRE(id >= 0 && id < nn, "load node out of range");
+      REQUIRE(L0.h_free[id], "load node is not a free node");
+      REQUIRE(!seen[id], "repeated load node");
+      seen[id] = 1;
+      pidx[i] = v.pnode((int)(id % v.nx), (int)((id / v.nx) % v.ny), (int)(id / ((int64_t)v.nx * v.ny)));
+      if (o->normals) {
+        const double* a = o->normals + 3 * (size_t)i;
+        const double len = std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+        REQUIRE(std::isfinite(len) && len > 0, "zero or non-finite normal");
+        for (int k = 0; k < 3; ++k) nrm[3 * (size_t)i + k] = a[k] / len;
+      }
+    }
+    uint64_t seed = o->seed;
+    for (double& x : f0) x = splitmix_pm1(seed);
+    if (C.wc_cap < n) {
+      C.wc_idx = dalloc<int64_t>(C, n);
+      C.wc_nrm = dalloc<double>(C, 3 * (size_t)n);
+      C.wc_fs = dalloc<double>(C, 3 * (size_t)n);
+      C.wc_src = dalloc<double>(C, 3 * (size_t)n);
+      C.wc_cap = n;
+    }
+    upload(C.stream, C.wc_idx, pidx.data(), sizeof(int64_t) * n);
+    upload(C.stream, C.wc_src, f0.data(), sizeof(double) * 3 * n);
+    if (o->normals) upload(C.stream, C.wc_nrm, nrm.data(), sizeof(double) * 3 * n);
+    const double* nrm_d = o->normals ? C.wc_nrm : nullptr;
+    const int rs = select_stencils(C, o->precision);
+    if (rs) return rs;
+    const auto t0 = std::chrono::steady_clock::now();
+    const bool f32 = o->precision != 0;
+    float* r32 = f32 ? L0.f32 : nullptr;
+    double* u = C.U;  // load case 0
+    double* f = C.F;
+    const unsigned nbw = (unsigned)std::min<int64_t>(kRedBlocks, (n + 255) / 256);
+    // scal[S_TMP1] = |g|^2, f = g / |g| on the load nodes, scal[S_TMP2] = |f_new - f_old|^2,
+    // scal[S_TMP3] = |f_old|^2
+    auto force_update = [&](const double* src_nm, int first) {
+      Prof pr(C, P_VECTOR);
+      hipLaunchKernelGGL(k_worst_norm, dim3(nbw), dim3(256), 0, C.stream, n, (const int64_t*)C.wc_idx, nrm_d, NP,
+                         (const double*)u, src_nm, C.partial);
+      hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, C.stream, (const double*)C.partial, (int)nbw, 0, S_TMP1,
+                         0, OP_NONE, C.scal);
+      hipLaunchKernelGGL(k_worst_apply, dim3(nbw), dim3(256), 0, C.stream, n, (const int64_t*)C.wc_idx, nrm_d, NP,
+                         (const double*)u, src_nm, (const double*)C.scal, (int)S_TMP1, first, f, C.r, r32, C.wc_fs,
+                         C.partial);
+      hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, C.stream, (const double*)C.partial, (int)nbw, 1, S_TMP2,
+                         S_TMP3, OP_NONE, C.scal);
+      LAUNCH_CHECK();
+    };
+    hipLaunchKernelGGL(k_zero<double>, dim3(nblk(n3p)), dim3(256), 0, C.stream, f, n3p);
+    hipLaunchKernelGGL(k_zero<double>, dim3(nblk(n3p)), dim3(256), 0, C.stream, u, n3p);
+    force_update(C.wc_src, 1);
+    const double* h = read_scal(C);
+    if (!(h[S_TMP1] > 0) || !std::isfinite(h[S_TMP1])) {
+      C.err = "the start force is zero after the projection";
+      return TOS_E_ARG;
+    }
+    apply_level<double, 1>(C, 0, u, f, C.r, nullptr, r32);  // r = f - K u
+    int it = 0;
+    double fch = 1, rel = 1;
+    while (it < o->max_iter && (fch > o->fch_tol || rel > o->res_tol)) {
+      precondition(C, o->precision, o->pre, o->post, f32, true);
+      hipLaunchKernelGGL(k_add_nodes, dim3(C.nb0), dim3(256), 0, C.stream, (const int4*)L0.cl_lin, L0.n_lin, NP, u,
+                         (const double*)C.z);
+      apply_level<double, 1>(C, 0, u, f, C.r, nullptr, r32);
+      dot_to(C, C.r, C.r, f, f, S_RN2, S_NF2, OP_NONE);
+      force_update(nullptr, 0);
+      h = read_scal(C);
+      ++it;
+      rel = std::sqrt(h[S_RN2] / h[S_NF2]);
+      if (!std::isfinite(rel)) { C.err = "NaN/Inf in the power iteration"; return TOS_E_NAN; }
+      if (rel > 1e4) { C.err = "power iteration diverged (cycle residual above 1e4)"; return TOS_E_NOCONV; }
+      if (!(h[S_TMP1] > 0) || !std::isfinite(h[S_TMP1])) {
+        C.err = "the displacement vanished on the load nodes";
+        return TOS_E_NAN;
+      }
+      fch = std::sqrt(h[S_TMP2]) / std::sqrt(h[S_TMP3]);
+    }
+    dot_to(C, f, u, nullptr, nullptr, S_TMP0, 0, OP_NONE);
+    const double comp = read_scal(C)[S_TMP0];
+    if (st) {
+      st->iterations = it;
+      st->compliance = comp;
+      st->fch = fch;
+      st->rel_res = rel;
+      st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return TOS_OK;
+  });
+}
+
 TOS_EXPORT int tos_sensitivity(tos_ctx* c, int k) {
   return guard(c, [&]() {
     REQUIRE(k >= 0 && k < c->K, "bad case");

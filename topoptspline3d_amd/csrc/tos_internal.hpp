@@ -167,6 +167,11 @@ struct Ctx {
   void* stage = nullptr;        // host<->device conversion buffer
   size_t stage_bytes = 0;
   bool op_ready = false;
+  // worst-case load: load nodes (padded index), unit normals and the last force
+  // (node-major, 3 per node), start force; grown on demand
+  int64_t wc_cap = 0;
+  int64_t* wc_idx = nullptr;
+  double *wc_nrm = nullptr, *wc_fs = nullptr, *wc_src = nullptr;
   // spline
   int sp_mode = -1, sp_order = 0, sp_param = 0;
   int nc[3] = {0, 0, 0};

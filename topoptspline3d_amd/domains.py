@@ -23,6 +23,24 @@ def cantilever(nx, ny, nz):
     return solid, support, f.reshape(-1)
 
 
+def face_load_region(nx, ny, nz, face="x+", inset=0):
+    """Load region for the worst-case load search: the nodes of one box face (\"x+\",
+    \"y-\", ... ) at least `inset` nodes away from that face's border, with the face's
+    outward unit normal each.  Returns (lexicographic node ids, (n, 3) normals)."""
+    axis, sign = "xyz".index(face[0]), 1 if face[1] == "+" else -1
+    dims = (nx + 1, ny + 1, nz + 1)
+    grids = [np.arange(d) for d in dims]
+    grids[axis] = np.array([dims[axis] - 1 if sign > 0 else 0])
+    for a in range(3):
+        if a != axis and inset:
+            grids[a] = grids[a][inset:dims[a] - inset]
+    z, y, x = np.meshgrid(grids[2], grids[1], grids[0], indexing="ij")
+    ids = (x + dims[0] * (y + dims[1] * z)).ravel().astype(np.int64)
+    nrm = np.zeros((ids.size, 3))
+    nrm[:, axis] = sign
+    return ids, nrm
+
+
 def batched_patch_loads(nx, ny, nz, K=16, patch=8, seed=7):
     """K unit forces with directions uniform on the sphere, each spread over a seeded
     random patch x patch node patch of the x = max face."""
