@@ -13,7 +13,8 @@ Before the timed region (c2): the learned level-2 correction is trained on pairs
 from 200 optimisation iterations on 128^3 ellipsoid domains.  Timed: `steps` iterations of
 plain MG-PCG, then `steps` iterations with the learned correction attached (`value`), then
 plain FP64 iterations; plus solves from zero on the final operator in FP64, mixed with
-FP32 and with FP16 coarse stencils, and learned mode.
+FP32 and with FP16 coarse stencils, and learned mode, and the worst-case load search
+(modified power method) over the workload's load region on the final operator.
 
 Prints one JSON line.  Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 [--workload c2|c1|c0] [--no-cpu-baseline] [--no-learned] [--no-fp64-compare]
@@ -323,6 +324,27 @@ def batched_phase(api, steps):
     return out
 
 
+def worst_phase(ctx, wl, precision):
+    """Worst-case load over the workload's load region by the modified power method
+    (modifiedPM, optimization/optimization.cpp:213-351): the loaded nodes, each force kept
+    along that node's load vector, seeded start force, the reference's stop rule (force
+    change 1e-4, cycle residual 1e-2, at most 500 V-cycles); on the final operator, with
+    the mixed-precision cycles and with FP64 cycles.  Load case 0 is restored afterwards."""
+    f = wl["f"].reshape(3, -1)
+    nodes = np.nonzero(np.abs(f).sum(0) > 0)[0]
+    normals = np.ascontiguousarray(f[:, nodes].T)
+    out = {"load_nodes": int(nodes.size)}
+    for tag, prec in (("mixed", precision), ("fp64", 0)):
+        st = ctx.worst_compliance(nodes, normals=normals, seed=1, precision=prec)
+        its = st["iterations"]
+        out[tag] = {"cycles": its, "seconds": round(st["seconds"], 3),
+                    "ms_per_cycle": round(st["seconds"] * 1e3 / max(its, 1), 3),
+                    "worst_compliance": st["compliance"], "fch": st["fch"], "rel_res": st["rel_res"]}
+        log(f"[bench] worst-case load ({tag}): {its} cycles, {st['seconds']:.2f} s, c {st['compliance']:.6e}")
+    ctx.set_loads(wl["f"])
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -334,6 +356,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-compare", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-worst", action="store_true", help="skip the worst-case load search")
     ap.add_argument("--learn-domains", type=int, default=4)
     ap.add_argument("--learn-iters", type=int, default=50)
     ap.add_argument("--train-steps", type=int, default=1500)
@@ -454,6 +477,8 @@ def main():
             cmp[tag] = {"solve_ms": round(st["seconds"] * 1e3, 2), "iterations": st["iterations"][0],
                         "rel_res": st["rel_res"][0]}
         line["solve_from_zero"] = cmp
+    if not args.no_worst:
+        line["worst_case_load"] = worst_phase(loop.ctx, wl, args.precision)
     if args.batched_steps > 0 and args.workload == "c2":
         loop.ctx.close()
         t = time.perf_counter()

@@ -64,6 +64,7 @@ struct Api {
   TOS_FN(int, mma_get_x, tos_mma*, double*)
   TOS_FN(int, mma_update, tos_mma*, const double*, const double*, const double*, int*)
   TOS_FN(int, ctx_mma_step, tos_ctx*, tos_mma*, double, double, double, int*)
+  TOS_FN(int, worst_compliance, tos_ctx*, const tos_worst_opts*, tos_worst_stats*)
 #undef TOS_FN
   void* handle = nullptr;
 
@@ -89,6 +90,7 @@ struct Api {
     TOS_SYM(projection_sensitivity) TOS_SYM(spline_init) TOS_SYM(spline_set_coeffs) TOS_SYM(spline_get_coeffs)
     TOS_SYM(spline_eval) TOS_SYM(spline_adjoint) TOS_SYM(oc_update_coeffs) TOS_SYM(mma_create)
     TOS_SYM(mma_destroy) TOS_SYM(mma_set_x) TOS_SYM(mma_get_x) TOS_SYM(mma_update) TOS_SYM(ctx_mma_step)
+    TOS_SYM(worst_compliance)
 #undef TOS_SYM
     return a;
   }
@@ -201,6 +203,22 @@ class HierarchyGrid {
     if (rc != TOS_OK && rc != TOS_E_NOCONV) tos::check(rc, c_, "v_cycle");
     return st.rel_res[0];
   }
+  // load region of the worst-case search (setLoadNodes, optimization/projection.cpp:129-268):
+  // lexicographic node ids and, unless the force direction is free, one normal per node
+  void setLoadNodes(const std::vector<int64_t>& nodes, const std::vector<double>& normals) {
+    load_nodes_ = nodes;
+    load_normals_ = normals;
+  }
+  // worst-case compliance by the modified power method (modifiedPM,
+  // optimization/optimization.cpp:213-351); load case 0 then holds the worst force
+  double worstCompliance(uint64_t seed = 0, int precision = 0) {
+    tos_worst_opts o{(int)load_nodes_.size(), load_nodes_.data(),
+                     load_normals_.empty() ? nullptr : load_normals_.data(), seed, precision, 1, 1, 500, 1e-4, 1e-2};
+    tos_worst_stats st;
+    tos::check(tos::Api::get().worst_compliance(c_, &o, &st), c_, "worst_compliance");
+    last_iterations_ = st.iterations;
+    return st.compliance;
+  }
   int n_grid() const { return (int)grids_.size(); }
   Grid& operator[](int i) { return grids_.at(i); }
   int last_iterations() const { return last_iterations_; }
@@ -210,6 +228,8 @@ class HierarchyGrid {
   tos_ctx* c_ = nullptr;
   std::vector<Grid> grids_;
   int last_iterations_ = 0;
+  std::vector<int64_t> load_nodes_;
+  std::vector<double> load_normals_;
 };
 
 }  // namespace grid
@@ -244,6 +264,7 @@ inline void initCoeffs(float v) {
   tos::check(tos::Api::get().spline_set_coeffs(c, cf.data()), c, "spline_set_coeffs");
 }
 inline void update_stencil() { getGrids().update_stencil(); }
+inline double modifiedPM(uint64_t seed = 0) { return getGrids().worstCompliance(seed); }
 inline void computeSensitivity() {
   auto c = getGrids().ctx();
   tos::check(tos::Api::get().sensitivity(c, 0), c, "sensitivity");

@@ -11,13 +11,23 @@ from oracle_api import LIB as ORACLE_LIB, oracle_api
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SRC = os.path.join(ROOT, "topoptspline3d_amd", "examples", "ordinary_topopt.cpp")
+SRC_WORST = os.path.join(ROOT, "topoptspline3d_amd", "examples", "worst_case_topopt.cpp")
+
+
+def _compile(tmp_path_factory, src, name):
+    exe = str(tmp_path_factory.mktemp("facade") / name)
+    subprocess.check_call(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), src, "-o", exe, "-ldl"])
+    return exe
 
 
 @pytest.fixture(scope="module")
 def driver(tmp_path_factory):
-    exe = str(tmp_path_factory.mktemp("facade") / "ordinary_topopt")
-    subprocess.check_call(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), SRC, "-o", exe, "-ldl"])
-    return exe
+    return _compile(tmp_path_factory, SRC, "ordinary_topopt")
+
+
+@pytest.fixture(scope="module")
+def worst_driver(tmp_path_factory):
+    return _compile(tmp_path_factory, SRC_WORST, "worst_case_topopt")
 
 
 def run(exe, lib, prefix, dims, iters):
@@ -42,3 +52,28 @@ def test_facade_drives_the_hip_library(driver):
     rows = run(driver, LIB, "tos_", (24, 12, 12), 4)
     ref = run(driver, ORACLE_LIB, "orc_", (24, 12, 12), 4)
     assert np.all(np.abs(rows[:, 1] - ref[:, 1]) <= 1e-6 * np.abs(ref[:, 1]))
+
+
+def test_facade_worst_case_loop_on_the_oracle(worst_driver):
+    """optimization()'s loop (modifiedPM + sensitivity + OC) through the facade equals the
+    Python driver over the same library (to the 13 printed digits)."""
+    oracle_api()
+    rows = run(worst_driver, ORACLE_LIB, "orc_", (16, 8, 8), 3)
+    from topoptspline3d_amd.drivers import run_worst_oc
+    # the facade's Parameter holds float fields, as the reference's (optimization.h:15-35)
+    ref = run_worst_oc(oracle_api(), 16, 8, 8, iters=3, nu=float(np.float32(0.4)),
+                       rho_min=float(np.float32(1e-3)))
+    assert np.allclose(rows[:, 1], ref["c"], rtol=1e-11, atol=0)  # printed with 13 digits
+    assert np.array_equal(rows[:, 3].astype(int), ref["its"])
+    assert np.all(rows[:, 3] < 500)
+
+
+@pytest.mark.gpu
+def test_facade_worst_case_loop_on_the_hip_library(worst_driver):
+    from topoptspline3d_amd.backend import LIB, gpu_available
+    if not gpu_available():
+        pytest.skip("no GPU")
+    rows = run(worst_driver, LIB, "tos_", (16, 8, 8), 3)
+    ref = run(worst_driver, ORACLE_LIB, "orc_", (16, 8, 8), 3)
+    assert np.all(np.abs(rows[:, 1] - ref[:, 1]) <= 1e-6 * np.abs(ref[:, 1]))
+    assert np.all(np.abs(rows[:, 3] - ref[:, 3]) <= 1)

@@ -6,14 +6,17 @@ run_oc       -- TestSuit::testOrdinaryTopopt (utils/test_utils.cpp:604-706): per
 run_spline_mma -- TestSuit::testOrdinarySplineTopoptMMA (utils/test_utils.cpp:972-1217):
                 coefficients -> density, solve, sensitivity, filter, spline adjoint of the
                 compliance and volume sensitivities, MMA step on the coefficients.
-Both return per-iteration records (compliance, volume, solver iterations, ms per phase).
+run_worst_oc -- optimization() (optimization/optimization.cpp:673-767), the reference's
+                default loop: per iteration update_stencil, worst-case load by the modified
+                power method over a load region, its sensitivity, filter, OC update.
+All return per-iteration records (compliance, volume, solver iterations, ms per phase).
 """
 import time
 
 import numpy as np
 
 from .context import MMA, Context
-from .domains import cantilever
+from .domains import cantilever, face_load_region
 
 DEFAULTS = dict(E=1e6, nu=0.4, penal=3.0, rho_min=1e-3, vfrac=0.4, radius=2.0, move=0.03, damp=0.5)
 
@@ -119,6 +122,38 @@ def run_spline_mma(api, nx, ny, nz, iters=100, rtol=1e-6, precision=1, spacing=4
         _timed(ph, "mma", mma.step_on, ctx, sens_scale, vol_scale, p["vfrac"])
         rec["c"].append(c); rec["v"].append(v)
         rec["its"].append(st["iterations"][0]); rec["rel_res"].append(st["rel_res"][0])
+        rec["ms"].append((time.perf_counter() - t0) * 1e3)
+    rec["phases_ms"] = {k: float(np.median(v)) for k, v in ph.t.items()}
+    rec["ctx"] = ctx
+    return rec
+
+
+def run_worst_oc(api, nx, ny, nz, iters=20, face="x+", inset=0, free_direction=False, precision=0,
+                 **kw):
+    """Box with x = 0 fixed; load region: the nodes of one face (`face_load_region`), force
+    along the face normal unless `free_direction`.  Iteration i (from 1) draws its start
+    force with seed i (the reference draws a new random force each time, Grid::randForce)."""
+    p = dict(DEFAULTS, **kw)
+    solid, support, f = cantilever(nx, ny, nz)
+    ids, nrm = face_load_region(nx, ny, nz, face, inset)
+    ctx = Context(api, solid, support)
+    ctx.set_material(p["E"], p["nu"], p["penal"], p["rho_min"])
+    ctx.set_loads(np.zeros_like(f))
+    ctx.set_density(np.where(solid.ravel() > 0, p["vfrac"], 0).astype(np.float32))
+    rec = {"c": [], "v": [], "its": [], "ms": []}
+    ph = Phases()
+    for it in range(1, iters + 1):
+        t0 = time.perf_counter()
+        _timed(ph, "operator", ctx.update_operator)
+        st = _timed(ph, "worst_case", ctx.worst_compliance, ids, None if free_direction else nrm, seed=it,
+                    precision=precision)
+        v = ctx.volume()
+        t1 = time.perf_counter()
+        ctx.sensitivity()
+        ctx.filter(p["radius"], 0)
+        ph.add("sens_filter", time.perf_counter() - t1)
+        _timed(ph, "oc", ctx.oc_update, p["vfrac"], p["move"], p["damp"])
+        rec["c"].append(st["compliance"]); rec["v"].append(v); rec["its"].append(st["iterations"])
         rec["ms"].append((time.perf_counter() - t0) * 1e3)
     rec["phases_ms"] = {k: float(np.median(v)) for k, v in ph.t.items()}
     rec["ctx"] = ctx
