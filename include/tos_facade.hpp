@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "tos_api.h"
+#include "tos_bio.hpp"
 
 namespace tos {
 
@@ -65,6 +66,7 @@ struct Api {
   TOS_FN(int, mma_update, tos_mma*, const double*, const double*, const double*, int*)
   TOS_FN(int, ctx_mma_step, tos_ctx*, tos_mma*, double, double, double, int*)
   TOS_FN(int, worst_compliance, tos_ctx*, const tos_worst_opts*, tos_worst_stats*)
+  TOS_FN(int, get_loads, tos_ctx*, int, double*)
 #undef TOS_FN
   void* handle = nullptr;
 
@@ -90,7 +92,7 @@ struct Api {
     TOS_SYM(projection_sensitivity) TOS_SYM(spline_init) TOS_SYM(spline_set_coeffs) TOS_SYM(spline_get_coeffs)
     TOS_SYM(spline_eval) TOS_SYM(spline_adjoint) TOS_SYM(oc_update_coeffs) TOS_SYM(mma_create)
     TOS_SYM(mma_destroy) TOS_SYM(mma_set_x) TOS_SYM(mma_get_x) TOS_SYM(mma_update) TOS_SYM(ctx_mma_step)
-    TOS_SYM(worst_compliance)
+    TOS_SYM(worst_compliance) TOS_SYM(get_loads)
 #undef TOS_SYM
     return a;
   }
@@ -219,6 +221,31 @@ class HierarchyGrid {
     last_iterations_ = st.iterations;
     return st.compliance;
   }
+  // records in the reference's bio format (grid/Grid.cpp:1238-1262, 2136-2170): x, y, z
+  // interleaved per load node (in the order given to setLoadNodes) or per node
+  // (lexicographic), coefficients x fastest
+  void writeSupportForce(const std::string& file) {
+    const std::vector<double> f = nodeVector(tos::Api::get().get_loads, "get_loads");
+    const size_t nn = f.size() / 3;
+    std::vector<double> fs[3];
+    for (int k = 0; k < 3; ++k)
+      for (int64_t n : load_nodes_) fs[k].push_back(f[k * nn + (size_t)n]);
+    if (!bio::write_vectors<double, 3>(file, fs)) throw tos::Error(TOS_E_ARG, "cannot write " + file);
+  }
+  void writeDisplacement(const std::string& file) {
+    const std::vector<double> u = nodeVector(tos::Api::get().get_displacement, "get_displacement");
+    const size_t nn = u.size() / 3;
+    std::vector<double> us[3];
+    for (int k = 0; k < 3; ++k) us[k].assign(u.begin() + k * nn, u.begin() + (k + 1) * nn);
+    if (!bio::write_vectors<double, 3>(file, us)) throw tos::Error(TOS_E_ARG, "cannot write " + file);
+  }
+  void writeCoeff(const std::string& file) {
+    tos_info i;
+    tos::check(tos::Api::get().get_info(c_, &i), c_, "get_info");
+    std::vector<float> cf((size_t)i.n_coeffs);
+    tos::check(tos::Api::get().spline_get_coeffs(c_, cf.data()), c_, "spline_get_coeffs");
+    if (!bio::write_vector(file, cf)) throw tos::Error(TOS_E_ARG, "cannot write " + file);
+  }
   int n_grid() const { return (int)grids_.size(); }
   Grid& operator[](int i) { return grids_.at(i); }
   int last_iterations() const { return last_iterations_; }
@@ -230,6 +257,14 @@ class HierarchyGrid {
   int last_iterations_ = 0;
   std::vector<int64_t> load_nodes_;
   std::vector<double> load_normals_;
+  template <class Get>
+  std::vector<double> nodeVector(Get get, const char* what) {
+    tos_info i;
+    tos::check(tos::Api::get().get_info(c_, &i), c_, "get_info");
+    std::vector<double> v(3 * (size_t)i.n_nodes);
+    tos::check(get(c_, 0, v.data()), c_, what);
+    return v;
+  }
 };
 
 }  // namespace grid

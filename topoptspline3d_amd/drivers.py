@@ -11,10 +11,12 @@ run_worst_oc -- optimization() (optimization/optimization.cpp:673-767), the refe
                 power method over a load region, its sensitivity, filter, OC update.
 All return per-iteration records (compliance, volume, solver iterations, ms per phase).
 """
+import os
 import time
 
 import numpy as np
 
+from . import bio
 from .context import MMA, Context
 from .domains import cantilever, face_load_region
 
@@ -129,10 +131,13 @@ def run_spline_mma(api, nx, ny, nz, iters=100, rtol=1e-6, precision=1, spacing=4
 
 
 def run_worst_oc(api, nx, ny, nz, iters=20, face="x+", inset=0, free_direction=False, precision=0,
-                 **kw):
+                 out_dir=None, **kw):
     """Box with x = 0 fixed; load region: the nodes of one face (`face_load_region`), force
     along the face normal unless `free_direction`.  Iteration i (from 1) draws its start
-    force with seed i (the reference draws a new random force each time, Grid::randForce)."""
+    force with seed i (the reference draws a new random force each time, Grid::randForce).
+    With `out_dir`, writes the reference's records there (optimization.cpp:728-765):
+    iter<i>_fs and crec_iter / vrec_iter / trec_iter every iteration, then cworst, vrec,
+    trec, flast, ulast."""
     p = dict(DEFAULTS, **kw)
     solid, support, f = cantilever(nx, ny, nz)
     ids, nrm = face_load_region(nx, ny, nz, face, inset)
@@ -155,6 +160,14 @@ def run_worst_oc(api, nx, ny, nz, iters=20, face="x+", inset=0, free_direction=F
         _timed(ph, "oc", ctx.oc_update, p["vfrac"], p["move"], p["damp"])
         rec["c"].append(st["compliance"]); rec["v"].append(v); rec["its"].append(st["iterations"])
         rec["ms"].append((time.perf_counter() - t0) * 1e3)
+        if out_dir:
+            bio.write_support_force(os.path.join(out_dir, f"iter{it}_fs"), ctx, ids)
+            bio.write_history(out_dir, rec)
+    if out_dir:
+        bio.write_history(out_dir, rec, final=True)
+        bio.write_support_force(os.path.join(out_dir, "flast"), ctx, ids)
+        bio.write_displacement(os.path.join(out_dir, "ulast"), ctx)
     rec["phases_ms"] = {k: float(np.median(v)) for k, v in ph.t.items()}
     rec["ctx"] = ctx
+    rec["load_nodes"] = ids
     return rec

@@ -3,11 +3,13 @@
 // load by the modified power method (modifiedPM), its sensitivity, OC update; written
 // against the reference-named facade.
 //
-//   worst_case_topopt <library.so> <symbol prefix> <nx> <ny> <nz> <iterations>
+//   worst_case_topopt <library.so> <symbol prefix> <nx> <ny> <nz> <iterations> [out_dir]
 //
 // Box with x = 0 fixed; load region: the nodes of the x = nx face, force along the face
 // normal (+x).  Iteration i draws its start force with seed i.  Prints one line per
-// iteration: "itn c_worst v power_steps".
+// iteration: "itn c_worst v power_steps".  With out_dir, writes the reference's records
+// there at the end (optimization.cpp:755-765): cworst, vrec (float64 per iteration),
+// flast (worst force on the load nodes), ulast (its displacement).
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -43,13 +45,23 @@ int main(int argc, char** argv) {
     getGrids().setLoadNodes(load, normals);
     initDensities(params().volume_ratio);
     const float Vgoal = params().volume_ratio;
+    std::vector<double> cRecord, volRecord;
     for (int itn = 1; itn <= iters; ++itn) {
       update_stencil();
       const double c = modifiedPM((uint64_t)itn);
       const double v = getGrids()[0].volumeRatio();
       std::printf("%d %.12e %.12f %d\n", itn, c, v, getGrids().last_iterations());
+      cRecord.push_back(c);
+      volRecord.push_back(v);
       computeSensitivity();
       updateDensities(Vgoal);
+    }
+    if (argc > 7) {
+      const std::string dir = argv[7];
+      bio::write_vector(dir + "/cworst", cRecord);
+      bio::write_vector(dir + "/vrec", volRecord);
+      getGrids().writeSupportForce(dir + "/flast");
+      getGrids().writeDisplacement(dir + "/ulast");
     }
   } catch (const std::exception& e) {
     std::fprintf(stderr, "error: %s\n", e.what());
