@@ -120,3 +120,25 @@ def test_large_worst_load_is_an_eigenpair(hip):
     Pu = np.zeros_like(u)
     Pu.reshape(3, -1)[0, ids] = u.reshape(3, -1)[0, ids]  # normal (1, 0, 0)
     assert np.linalg.norm(Pu - cu * f) <= 1e-2 * np.linalg.norm(Pu)
+
+
+def test_bench_config_domain_matches_oracle(hip):
+    """The reference's bench input cube24x12x12/config11 at resolution 48 (mesh voxelised,
+    regions and load field from the JSON): two iterations of the worst-case loop and of the
+    fixed-load OC loop, GPU against the oracle."""
+    import os
+    from oracle_api import ROOT
+    from topoptspline3d_amd import mesh_domain
+    from topoptspline3d_amd.drivers import run_oc, run_worst_oc
+    d = mesh_domain.load_config(os.path.join(ROOT, "tests", "golden", "bench_cube24x12x12", "config11.json"), 48)
+    print(f"\n[bench config] solid {d['solid'].shape} supports {int(d['support'].sum())} "
+          f"load nodes {d['load_nodes'].size}", flush=True)
+    rg = run_worst_oc(hip, 0, 0, 0, iters=2, domain=d)
+    ro = run_worst_oc(oracle_api(), 0, 0, 0, iters=2, domain=d)
+    print("worst gpu", rg["c"], rg["its"], "oracle", ro["c"], ro["its"], flush=True)
+    assert np.allclose(rg["c"], ro["c"], rtol=1e-6, atol=0)
+    dom = (d["solid"], d["support"], d["f"], d["shell"])
+    og = run_oc(hip, 0, 0, 0, iters=2, rtol=1e-8, precision=1, domain=dom)
+    oo = run_oc(oracle_api(), 0, 0, 0, iters=2, rtol=1e-8, precision=0, domain=dom)
+    print("oc gpu", og["c"], "oracle", oo["c"], flush=True)
+    assert np.allclose(og["c"], oo["c"], rtol=1e-6, atol=0)

@@ -131,17 +131,23 @@ def run_spline_mma(api, nx, ny, nz, iters=100, rtol=1e-6, precision=1, spacing=4
 
 
 def run_worst_oc(api, nx, ny, nz, iters=20, face="x+", inset=0, free_direction=False, precision=0,
-                 out_dir=None, **kw):
+                 out_dir=None, domain=None, **kw):
     """Box with x = 0 fixed; load region: the nodes of one face (`face_load_region`), force
     along the face normal unless `free_direction`.  Iteration i (from 1) draws its start
     force with seed i (the reference draws a new random force each time, Grid::randForce).
     With `out_dir`, writes the reference's records there (optimization.cpp:728-765):
     iter<i>_fs and crec_iter / vrec_iter / trec_iter every iteration, then cworst, vrec,
-    trec, flast, ulast."""
+    trec, flast, ulast.  `domain`: a dict of `mesh_domain.load_config` (solid, support,
+    shell, load_nodes, normals) in place of the box (nx, ny, nz are then ignored)."""
     p = dict(DEFAULTS, **kw)
-    solid, support, f = cantilever(nx, ny, nz)
-    ids, nrm = face_load_region(nx, ny, nz, face, inset)
-    ctx = Context(api, solid, support)
+    shell = None
+    if domain is None:
+        solid, support, f = cantilever(nx, ny, nz)
+        ids, nrm = face_load_region(nx, ny, nz, face, inset)
+    else:
+        solid, support, f, shell = domain["solid"], domain["support"], domain["f"], domain["shell"]
+        ids, nrm = domain["load_nodes"], domain["normals"]
+    ctx = Context(api, solid, support, shell=shell)
     ctx.set_material(p["E"], p["nu"], p["penal"], p["rho_min"])
     ctx.set_loads(np.zeros_like(f))
     ctx.set_density(np.where(solid.ravel() > 0, p["vfrac"], 0).astype(np.float32))
