@@ -155,7 +155,8 @@ __device__ __forceinline__ void fine_row(const T (&wa)[8], const T* __restrict__
 }
 
 // MODE 0: y = K u, optional partial sums of u.y;  MODE 1: y = f - K u, optional float
-// copy y32.  Only free nodes are touched (the others stay 0 in every vector).
+// copy y32, optional partial sums of y.y (set 0) and f.f (set 1).  Only free nodes are
+// touched (the others stay 0 in every vector).
 // TU: type of u (float u with FP64 arithmetic: the FCG search direction of the mixed
 // precision solves).  FP32 arithmetic: all 81 neighbour values live at once, register
 // budget sized for 4 waves per SIMD.  FP64 keeps the compiler's interleaved schedule
@@ -170,7 +171,7 @@ __global__ TOS_FINE_BOUNDS(T) void k_fine_apply(LevelView L, const int4* __restr
                                                  float* __restrict__ y32) {
   NodeRef n;
   const bool ok = chunk_node(cl, nch, 1, n);
-  double d = 0;
+  double d = 0, d2 = 0;
   if (ok) {
     TU un[27][3];
     T wa[8], fv[3] = {0, 0, 0};
@@ -194,10 +195,15 @@ __global__ TOS_FINE_BOUNDS(T) void k_fine_apply(LevelView L, const int4* __restr
         const T v = fv[c] - acc[c];
         y[c * L.NP + n.p] = v;
         if (y32) y32[c * L.NP + n.p] = (float)v;
+        d += (double)v * (double)v;
+        d2 += (double)fv[c] * (double)fv[c];
       }
     }
   }
-  if (partial) block_partial(d, partial, 0);
+  if (partial) {
+    block_partial(d, partial, 0);
+    if (MODE == 1) block_partial(d2, partial, 1);
+  }
 }
 
 // Gauss-Seidel colour pass on the fine level.  BACKWARD relaxes components z, y, x.
@@ -1590,12 +1596,14 @@ __global__ __launch_bounds__(256) void k_dot_nodes(const int4* __restrict__ cl, 
   block_partial(s1, partial, 1);
 }
 
-// u += z (plain V-cycle iteration)
+// u += z (plain V-cycle iteration); z FP64, or the FP32 cycle's output read directly
+template <class TZ>
 __global__ __launch_bounds__(256) void k_add_nodes(const int4* __restrict__ cl, int nch, int64_t NP,
-                                                   double* __restrict__ u, const double* __restrict__ z) {
+                                                   double* __restrict__ u, const TZ* __restrict__ z) {
   NodeRef n;
   if (!chunk_node(cl, nch, 1, n)) return;
-  u[n.p] += z[n.p]; u[NP + n.p] += z[NP + n.p]; u[2 * NP + n.p] += z[2 * NP + n.p];
+  const double z0 = (double)z[n.p], z1 = (double)z[NP + n.p], z2 = (double)z[2 * NP + n.p];
+  u[n.p] += z0; u[NP + n.p] += z1; u[2 * NP + n.p] += z2;
 }
 
 template <class A, class B>

@@ -1680,7 +1680,7 @@ int solve_case(Ctx& c, int k, const tos_solve_opts& o, int* its, double* relres)
   if (!o.use_pcg) {
     while (rn / nf > o.rtol && it < o.maxit) {
       precondition(c, o.precision, o.pre, o.post, f32, true);
-      hipLaunchKernelGGL(k_add_nodes, dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, u, (const double*)c.z);
+      hipLaunchKernelGGL(k_add_nodes<double>, dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, u, (const double*)c.z);
       apply_level<double, 1>(c, 0, u, f, c.r, nullptr, f32 ? L0.f32 : nullptr);
       dot_to(c, c.r, c.r, nullptr, nullptr, S_RN2, 0, OP_NONE);
       rn = std::sqrt(read_scal(c)[S_RN2]);
@@ -2742,11 +2742,21 @@ TOS_EXPORT int tos_worst_compliance(tos_ctx* c, const tos_worst_opts* o, tos_wor
     int it = 0;
     double fch = 1, rel = 1;
     while (it < o->max_iter && (fch > o->fch_tol || rel > o->res_tol)) {
-      precondition(C, o->precision, o->pre, o->post, f32, true);
-      hipLaunchKernelGGL(k_add_nodes, dim3(C.nb0), dim3(256), 0, C.stream, (const int4*)L0.cl_lin, L0.n_lin, NP, u,
-                         (const double*)C.z);
-      apply_level<double, 1>(C, 0, u, f, C.r, nullptr, r32);
-      dot_to(C, C.r, C.r, f, f, S_RN2, S_NF2, OP_NONE);
+      // u += B r (an FP32 cycle's output added directly), r = f - K u with |r|^2 and |f|^2
+      // (u stays FP64: an FP32 copy of u in the residual would add an error of the order
+      // of 1e-7 |K| |u|, above the stop tolerance on high-contrast designs)
+      precondition(C, o->precision, o->pre, o->post, f32, false);
+      if (f32)
+        hipLaunchKernelGGL(k_add_nodes<float>, dim3(C.nb0), dim3(256), 0, C.stream, (const int4*)L0.cl_lin,
+                           L0.n_lin, NP, u, (const float*)L0.u32);
+      else
+        hipLaunchKernelGGL(k_add_nodes<double>, dim3(C.nb0), dim3(256), 0, C.stream, (const int4*)L0.cl_lin,
+                           L0.n_lin, NP, u, (const double*)C.z);
+      apply_level<double, 1>(C, 0, u, f, C.r, C.partial, r32);
+      {
+        Prof pr(C, P_VECTOR);
+        finish0(C, 1, S_RN2, S_NF2, OP_NONE);
+      }
       force_update(nullptr, 0);
       h = read_scal(C);
       ++it;
