@@ -122,15 +122,17 @@ def test_large_worst_load_is_an_eigenpair(hip):
     assert np.linalg.norm(Pu - cu * f) <= 1e-2 * np.linalg.norm(Pu)
 
 
-def test_bench_config_domain_matches_oracle(hip):
-    """The reference's bench input cube24x12x12/config11 at resolution 48 (mesh voxelised,
-    regions and load field from the JSON): two iterations of the worst-case loop and of the
-    fixed-load OC loop, GPU against the oracle."""
+@pytest.mark.parametrize("case,reso", [("bench_cube24x12x12/config11.json", 48), ("bench_fertility/config1.json", 64)])
+def test_bench_config_domain_matches_oracle(hip, case, reso):
+    """The reference's bench inputs (mesh voxelised, regions and load field from the JSON):
+    the box cube24x12x12/config11 at resolution 48 and the irregular fertility mesh
+    (config1) at 64; two iterations of the worst-case loop and of the fixed-load OC loop,
+    GPU against the oracle."""
     import os
     from oracle_api import ROOT
     from topoptspline3d_amd import mesh_domain
     from topoptspline3d_amd.drivers import run_oc, run_worst_oc
-    d = mesh_domain.load_config(os.path.join(ROOT, "tests", "golden", "bench_cube24x12x12", "config11.json"), 48)
+    d = mesh_domain.load_config(os.path.join(ROOT, "tests", "golden", *case.split("/")), reso)
     print(f"\n[bench config] solid {d['solid'].shape} supports {int(d['support'].sum())} "
           f"load nodes {d['load_nodes'].size}", flush=True)
     rg = run_worst_oc(hip, 0, 0, 0, iters=2, domain=d)
