@@ -17,7 +17,9 @@ FP32 and with FP16 coarse stencils, and learned mode, and the worst-case load se
 (modified power method) over the workload's load region on the final operator.
 
 Prints one JSON line.  Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
-[--workload c2|c1|c0] [--no-cpu-baseline] [--no-learned] [--no-fp64-compare]
+[--workload c2|c1|c0|fertility] [--no-cpu-baseline] [--no-learned] [--no-fp64-compare]
+(fertility: the reference's bench mesh fertility/config1, kept under tests/golden,
+voxelised at resolution 511 with its JSON supports and loads, element-density OC)
 """
 import argparse
 import json
@@ -61,6 +63,14 @@ def make_workload(name):
         return dict(name="c1", desc="cantilever 128x64x64, spline spacing 4 (cubic), MMA",
                     solid=solid, support=support, f=f, shell=None, meta={}, spline=(1, 4, 4),
                     optimizer="mma", move=0.06, rtol=1e-6)
+    if name == "fertility":
+        # a reference bench input at the headline resolution: mesh + JSON regions and loads
+        from topoptspline3d_amd.mesh_domain import load_config
+        d = load_config(os.path.join(ROOT, "tests", "golden", "bench_fertility", "config1.json"), 511,
+                        shell_width=3)
+        return dict(name="fertility", desc="bench mesh fertility (config1) voxelised at resolution 511, "
+                    "3-voxel shell, element densities, OC", solid=d["solid"], support=d["support"], f=d["f"],
+                    shell=d["shell"], meta={}, spline=None, optimizer="oc", move=0.03, rtol=1e-6)
     if name == "c0":
         solid, support, f = cantilever(24, 12, 12)
         return dict(name="c0", desc="cantilever 24x12x12, element densities, OC", solid=solid,
@@ -463,6 +473,9 @@ def main():
         line["plain_fp64"] = {"ms_per_iter": round(ms64, 3), "pcg_iterations": its64, "max_rel_res": res64,
                               "speedup_of_value": round(ms64 / ms, 3)}
         # same operator, solve from zero to the same tolerance: FP64, mixed, learned
+        # (an OC step changes the densities themselves: rebuild the operator first)
+        if not wl["spline"]:
+            loop.ctx.update_operator()
         cmp = {}
         modes = [("fp64", 0, False), ("mixed_fp32_stencils", 1, False), ("mixed_fp16_stencils", 2, False)]
         if net is not None:
@@ -478,6 +491,8 @@ def main():
                         "rel_res": st["rel_res"][0]}
         line["solve_from_zero"] = cmp
     if not args.no_worst:
+        if not wl["spline"]:
+            loop.ctx.update_operator()
         line["worst_case_load"] = worst_phase(loop.ctx, wl, args.precision)
     if args.batched_steps > 0 and args.workload == "c2":
         loop.ctx.close()

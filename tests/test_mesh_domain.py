@@ -105,6 +105,16 @@ def test_config11_supports_and_loads():
     assert f[2].sum() == -2 and np.count_nonzero(f) == 2
 
 
+def test_shell_is_the_three_voxel_rind():
+    """shell_width 3 on the bench box at resolution 48 (a 48x24x24 block): the elements
+    within Euclidean distance 3 of the outside, i.e. all but the 42x18x18 core"""
+    d = M.load_config(CFG, 48, shell_width=3)
+    sh = d["shell"].astype(bool)
+    core = np.zeros_like(sh)
+    core[3:-3, 3:-3, 3:-3] = True
+    assert np.array_equal(sh, ~core), (sh.sum(), (~core).sum())
+
+
 def test_worst_case_loop_on_config11():
     from topoptspline3d_amd.drivers import run_worst_oc
     d = M.load_config(CFG, 48)

@@ -82,7 +82,7 @@ def voxelize(verts, tris, R, corner, h):
     crossings of a +z ray below each centre (edges shared by two triangles counted once:
     a column on an edge belongs to the triangle whose edge is a left or top edge)"""
     P = (verts.astype(np.float64) - corner.astype(np.float64)) / h  # voxel units
-    cnt = np.zeros((R, R, R), np.int32)  # [z, y, x]
+    flip = np.zeros((R, R, R), np.uint8)  # [z, y, x]: parity of the crossings starting there
     for t in tris:
         a, b, c = P[t[0]], P[t[1]], P[t[2]]
         area = (b[0] - a[0]) * (c[1] - a[1]) - (b[1] - a[1]) * (c[0] - a[0])
@@ -112,10 +112,9 @@ def voxelize(verts, tris, R, corner, h):
         zs = zhit[iy, ix]
         # crossings strictly below the centre z + 0.5: flip voxels z >= ceil(zhit - 0.5)
         z0 = np.clip(np.floor(zs - 0.5).astype(np.int64) + 1, 0, R)
-        for yy, xx, zz in zip(iy + y0, ix + x0, z0):
-            if zz < R:
-                cnt[zz, yy, xx] += 1
-    return (np.cumsum(cnt, axis=0) & 1).astype(bool)
+        keep = z0 < R
+        np.bitwise_xor.at(flip, (z0[keep], iy[keep] + y0, ix[keep] + x0), 1)
+    return np.bitwise_xor.accumulate(flip, axis=0).astype(bool)
 
 
 # ------------------------------------------------------------------------------------
@@ -257,7 +256,8 @@ def load_config(path, resolution, mesh_path=None, shell_width=0):
         from .domains import _within3_of_exterior
         if shell_width != 3:
             raise NotImplementedError("shell widths other than 3 voxels")
-        shell = (_within3_of_exterior(solid) & (solid > 0)).astype(np.uint8)
+        # one void voxel around the tight crop, so the exterior next to every face counts
+        shell = _within3_of_exterior(np.pad(solid > 0, 1))[1:-1, 1:-1, 1:-1].astype(np.uint8)
     return dict(solid=solid, support=support, f=f.ravel(), shell=shell, load_nodes=ids,
                 normals=outward_normals(solid, (lz, ly, lx)), load_force=fl, origin=origin, h=h,
                 cube=(R, o), config=cfg)
