@@ -597,7 +597,8 @@ __global__ __launch_bounds__(256) void k_worst_apply(int n, const int64_t* __res
                                                      double* __restrict__ f, double* __restrict__ r,
                                                      float* __restrict__ r32, double* __restrict__ fs,
                                                      double* __restrict__ partial) {
-  const double inv = 1.0 / sqrt(scal[g2_slot]);
+  const double g2 = scal[g2_slot];
+  const double inv = g2 > 0 ? 1.0 / sqrt(g2) : 0.0;  // zero candidate: the host reports it
   double sd = 0, so = 0;
   for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
     double g[3];

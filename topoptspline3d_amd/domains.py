@@ -24,8 +24,8 @@ def cantilever(nx, ny, nz):
 
 
 def face_load_region(nx, ny, nz, face="x+", inset=0):
-    """Load region for the worst-case load search: the nodes of one box face (\"x+\",
-    \"y-\", ... ) at least `inset` nodes away from that face's border, with the face's
+    """Load region for the worst-case load search: the nodes of one box face ("x+",
+    "y-", ...) at least `inset` nodes away from that face's border, with the face's
     outward unit normal each.  Returns (lexicographic node ids, (n, 3) normals)."""
     axis, sign = "xyz".index(face[0]), 1 if face[1] == "+" else -1
     dims = (nx + 1, ny + 1, nz + 1)
