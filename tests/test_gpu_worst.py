@@ -144,3 +144,27 @@ def test_bench_config_domain_matches_oracle(hip, case, reso):
     oo = run_oc(oracle_api(), 0, 0, 0, iters=2, rtol=1e-8, precision=0, domain=dom)
     print("oc gpu", og["c"], "oracle", oo["c"], flush=True)
     assert np.allclose(og["c"], oo["c"], rtol=1e-6, atol=0)
+
+
+def test_bad_load_regions_rejected_like_the_oracle(hip):
+    """repeated, supported, out-of-range load nodes and zero normals: TOS_E_ARG from both
+    libraries; an operator-less context: TOS_E_STATE; the context stays usable after"""
+    from topoptspline3d_amd.context import TosError
+    dims = (24, 12, 12)
+    ids, nrm = face_load_region(*dims, "x+")
+    for api in (hip, oracle_api()):
+        c = make(api, *dims)
+        nn = c.nn
+        for nodes, normals in ((np.r_[ids, ids[:1]], None), (np.array([0]), None), (np.array([nn]), None),
+                               (ids, np.zeros((ids.size, 3)))):
+            with pytest.raises(TosError) as e:
+                c.worst_compliance(nodes, normals=normals)
+            assert e.value.code == -1
+        st = c.worst_compliance(ids, normals=nrm, seed=1)
+        assert st["fch"] <= 1e-4 and st["rel_res"] <= 1e-2 and st["compliance"] > 0
+        solid, support, _ = cantilever(*dims)
+        fresh = Context(api, solid, support)
+        fresh.set_material(1e6, 0.4, 3.0, 1e-3)
+        with pytest.raises(TosError) as e:
+            fresh.worst_compliance(ids)
+        assert e.value.code == -6
