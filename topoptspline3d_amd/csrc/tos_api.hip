@@ -72,6 +72,7 @@ void* raw_alloc(std::vector<void*>& list, hipStream_t s, size_t bytes) {
 
 template <class T>
 T* dalloc(Ctx& c, size_t n) {
+  ++c.alloc_epoch;
   return (T*)raw_alloc(c.allocs, c.stream, n * sizeof(T));
 }
 
@@ -263,6 +264,7 @@ void* stage(Ctx& c, size_t bytes) {
   if (c.stage_bytes < bytes) {
     if (c.stage) HIPCHK(hipFree(c.stage));
     c.stage = nullptr;
+    ++c.alloc_epoch;
     HIPCHK(hipMalloc(&c.stage, bytes));
     c.stage_bytes = bytes;
   }
@@ -1040,6 +1042,7 @@ void learn_prepare(Ctx& c, int level) {
   if (S.level == level) return;
   for (void* p : S.mem) (void)hipFree(p);
   S.mem.clear();
+  ++c.alloc_epoch;
   S.acts = nullptr;
   S.level = level;
   const Level& L = c.L[level];
@@ -1056,6 +1059,7 @@ void learn_prepare(Ctx& c, int level) {
 void learn_release(Ctx& c) {
   if (!c.learn) return;
   for (void* p : c.learn->mem) (void)hipFree(p);
+  ++c.alloc_epoch;
   delete c.learn;
   c.learn = nullptr;
 }
@@ -1548,11 +1552,22 @@ int solve_batched(Ctx& c, const tos_solve_opts& o, tos_solve_stats* st) {
       if (!still) break;
     }
   }
+  // true FP64 residual of every case, f - A u recomputed from the final u (the FCG loop
+  // tracks the recursive one); convergence is judged on it, as in the one-case solve
+  hipLaunchKernelGGL((k_fine_apply_b<double, 1, kVecBS>), dim3(nb), dim3(kVecBS), 0, c.stream, L0.v, cl, nch, K,
+                     B.KP, (const double*)c.w64, (const double*)c.Kc64, (const double*)B.U, (const double*)B.F, B.R,
+                     (double*)nullptr, (float*)nullptr);
+  hipLaunchKernelGGL((k_dotb<kVecBS>), dim3(nb), dim3(kVecBS), 0, c.stream, cl, nch, NP, K, B.KP, (const double*)B.R,
+                     (const double*)B.R, (const double*)nullptr, (const double*)nullptr, B.part);
+  foldb(c, B, nb, 1, B_RN2, 0, OPB_NONE);
   hipLaunchKernelGGL(k_interleave, dim3(ni), dim3(256), 0, c.stream, cl, nch, NP, K, (const double*)B.U, c.U, 0);
   LAUNCH_CHECK();
-  HIPCHK(hipStreamSynchronize(c.stream));
+  download(c.stream, B.h_sc, B.sc, (size_t)B_COUNT * K * sizeof(double));
   int rc = TOS_OK;
   for (int k = 0; k < K; ++k) {
+    const double rn2 = B.h_sc[B_RN2 * K + k];
+    if (!std::isfinite(rn2)) { c.err = "NaN/Inf in the batched true residual"; return TOS_E_NAN; }
+    rel[k] = nf[k] > 0 ? std::sqrt(rn2) / nf[k] : 0.0;
     if (st && k < 64) { st->iterations[k] = its[k]; st->rel_res[k] = rel[k]; }
     if (rel[k] > o.rtol) rc = TOS_E_NOCONV;
   }
@@ -1755,6 +1770,17 @@ int solve_case(Ctx& c, int k, const tos_solve_opts& o, int* its, double* relres)
         c.graph_replay && !c.prof_on && !(c.learn && (c.learn->ds || (c.learn->mode == 1 && !nn_off)));
     hipGraphExec_t exec = nullptr;
     if (graphed) {
+      // a graph captured before any later allocation or release of context memory is
+      // dropped (no captured pointer may outlive its buffer), then looked up by configuration
+      bool stale = false;
+      for (const Ctx::SolveGraph& g : c.graphs) stale |= g.epoch != c.alloc_epoch;
+      if (stale) {
+        for (const Ctx::SolveGraph& g : c.graphs) {
+          (void)hipGraphExecDestroy(g.exec);
+          (void)hipGraphDestroy(g.graph);
+        }
+        c.graphs.clear();
+      }
       for (const Ctx::SolveGraph& g : c.graphs)
         if (g.precision == o.precision && g.pre == o.pre && g.post == o.post && g.sweep == c.sweep_mode &&
             g.gal == c.gal_prec && g.u == u)
@@ -1767,12 +1793,13 @@ int solve_case(Ctx& c, int k, const tos_solve_opts& o, int* its, double* relres)
         first = false;
       } else if (graphed) {
         if (!exec) {
-          Ctx::SolveGraph g{o.precision, o.pre, o.post, c.sweep_mode, c.gal_prec, u, nullptr, nullptr};
+          Ctx::SolveGraph g{o.precision, o.pre, o.post, c.sweep_mode, c.gal_prec, u, 0, nullptr, nullptr};
           HIPCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
           head();
           tail();
           HIPCHK(hipStreamEndCapture(c.stream, &g.graph));
           HIPCHK(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+          g.epoch = c.alloc_epoch;
           c.graphs.push_back(g);
           exec = g.exec;
         }

@@ -212,10 +212,13 @@ struct Ctx {
   struct SolveGraph {           // captured FCG iteration body per solve configuration
     int precision, pre, post, sweep, gal;
     const double* u;
+    uint64_t epoch;             // alloc_epoch at capture
     hipGraph_t graph;
     hipGraphExec_t exec;
   };
   std::vector<SolveGraph> graphs;
+  uint64_t alloc_epoch = 0;     // bumped by every context allocation or release after creation:
+                                // a captured graph from an older epoch is destroyed, not replayed
 };
 
 constexpr int kRedBlocks = 1024;  // fixed partial count -> deterministic reductions
