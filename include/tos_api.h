@@ -37,6 +37,7 @@ extern "C" {
 #define TOS_E_NAN -4        /* NaN/Inf detected in an input or a reduction             */
 #define TOS_E_SINGULAR -5   /* coarsest-level Cholesky found a non-positive pivot      */
 #define TOS_E_STATE -6      /* call order violated (e.g. solve before update_operator) */
+#define TOS_E_TOO_LARGE -7  /* input beyond the capacity of a fixed-size entry point    */
 
 typedef struct tos_ctx tos_ctx;
 
@@ -187,6 +188,22 @@ int tos_element_compliance(tos_ctx* ctx, int k, double* out);
  * threshold g and the number of bisection steps. */
 int tos_oc_update(tos_ctx* ctx, double vgoal, double move, double damp,
                   double* g_out, int* steps_out);
+/* OC update without any allocation.  No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same update as tos_oc_update (threshold bisection
+ * over [0, max|s|], at most 31 trials, stop at |V - vgoal| <= 1e-4, V = mean density of
+ * the active elements) over caller-owned DEVICE arrays of n elements in any order:
+ * active, shell (u8, may be NULL: all active / no shell), rho, sens (float) in,
+ * rho_out (float, may equal rho) out; inactive elements are copied through.  Runs on
+ * `stream` (a hipStream_t, NULL = the null stream); g_out / steps_out (host, may be NULL)
+ * make the call wait for the stream.  The bisection's partials and scalars live in one
+ * fixed device array per GPU (2*8192 + 32 doubles), so n > TOS_NOALLOC_MAX_ELEMENTS
+ * returns TOS_E_TOO_LARGE; calls that overlap on one GPU (other threads or streams)
+ * share that array and race. */
+#define TOS_NOALLOC_MAX_ELEMENTS 67108864
+int tos_oc_update_noalloc(int64_t n, const uint8_t* active, const uint8_t* shell, const float* rho,
+                          const float* sens, float* rho_out, double vgoal, double move, double damp,
+                          double rho_min, void* stream, double* g_out, int* steps_out);
 /* Heaviside projection: keeps the current densities as the projection input and
  * replaces them by the projected field (projectDensities, optimization.cu:339-353). */
 int tos_project_density(tos_ctx* ctx, double beta);

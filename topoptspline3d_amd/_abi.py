@@ -131,13 +131,20 @@ LEARN_SIGNATURES = {
     "ctx_learn_stats": (C.c_int, [vp, i32p, i32p, f64p]),
 }
 
+# entry points on caller-owned device arrays (HIP library only)
+DEVICE_SIGNATURES = {
+    "oc_update_noalloc": (C.c_int, [C.c_int64, vp, vp, vp, vp, vp, C.c_double, C.c_double, C.c_double,
+                                    C.c_double, vp, f64p, i32p]),
+}
+
 ERRORS = {0: "ok", -1: "bad argument", -2: "HIP error", -3: "not converged",
-          -4: "NaN/Inf", -5: "singular coarsest system", -6: "call order"}
+          -4: "NaN/Inf", -5: "singular coarsest system", -6: "call order", -7: "beyond capacity"}
 
 
 def bind(lib, prefix, learned=False):
+    """learned=True: also the HIP library's own tables (network, device-array entry points)"""
     ns = types.SimpleNamespace()
-    table = dict(SIGNATURES, **LEARN_SIGNATURES) if learned else SIGNATURES
+    table = dict(SIGNATURES, **LEARN_SIGNATURES, **DEVICE_SIGNATURES) if learned else SIGNATURES
     for name, (res, args) in table.items():
         fn = getattr(lib, prefix + name)
         fn.restype = res
@@ -149,5 +156,5 @@ def bind(lib, prefix, learned=False):
 
 
 def exported_names(prefix, learned=True):
-    names = list(SIGNATURES) + (list(LEARN_SIGNATURES) if learned else [])
+    names = list(SIGNATURES) + (list(LEARN_SIGNATURES) + list(DEVICE_SIGNATURES) if learned else [])
     return [prefix + n for n in names]

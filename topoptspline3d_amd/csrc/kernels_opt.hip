@@ -269,14 +269,15 @@ __global__ __launch_bounds__(256) void k_oc_trial(LevelView L, const uint8_t* __
   if (threadIdx.x == 0) partial[blockIdx.x] = v;
 }
 
+// nact: active element count, read from scal[S_VOL] when nact_dev is set
 __global__ __launch_bounds__(256) void k_oc_decide(const double* __restrict__ partial, int np, double nact,
-                                                   double vgoal, double* __restrict__ scal) {
+                                                   int nact_dev, double vgoal, double* __restrict__ scal) {
   if (scal[S_OC_DONE] != 0.0) return;
   double v = 0;
   for (int i = threadIdx.x; i < np; i += 256) v += partial[i];
   v = block_sum256(v);
   if (threadIdx.x == 0) {
-    const double V = v / nact;
+    const double V = v / (nact_dev ? scal[S_VOL] : nact);
     const double g = scal[S_OC_G];
     scal[S_OC_V] = V;
     scal[S_OC_STEPS] += 1.0;
@@ -302,6 +303,77 @@ __global__ __launch_bounds__(256) void k_sum_active(LevelView L, const uint8_t* 
     if (eact[i]) s += (double)v[i];
   s = block_sum256(s);
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// ---------------------------------------------------------------------------------------
+// OC over a dense element array (the no-allocation entry point): block b covers the
+// elements [b*chunk, (b+1)*chunk) with stride 256 per thread, so the partial count is
+// ceil(n / chunk); masks may be null (every element active / no shell).  Same trial
+// density (oc_rho) and decision kernel as the context's OC.
+// ---------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_absmax_count_flat(int64_t n, int64_t chunk, const uint8_t* __restrict__ act,
+                                                           const float* __restrict__ s, int nb,
+                                                           double* __restrict__ partial) {
+  const int64_t i0 = (int64_t)blockIdx.x * chunk, i1 = min(n, i0 + chunk);
+  double m = 0, cnt = 0;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += 256)
+    if (!act || act[i]) {
+      m = fmax(m, fabs((double)s[i]));
+      cnt += 1.0;
+    }
+  m = block_max256(m);
+  cnt = block_sum256(cnt);
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x] = m;
+    partial[nb + blockIdx.x] = cnt;
+  }
+}
+
+// folds the k_absmax_count_flat partials: bisection bounds as k_oc_init, active count into
+// scal[S_VOL]
+__global__ __launch_bounds__(256) void k_oc_init_flat(const double* __restrict__ partial, int nb,
+                                                      double* __restrict__ scal) {
+  double m = 0, cnt = 0;
+  for (int i = threadIdx.x; i < nb; i += 256) {
+    m = fmax(m, partial[i]);
+    cnt += partial[nb + i];
+  }
+  m = block_max256(m);
+  cnt = block_sum256(cnt);
+  if (threadIdx.x == 0) {
+    scal[S_OC_GMAX] = m;
+    scal[S_OC_LO] = 0.0;
+    scal[S_OC_HI] = m;
+    scal[S_OC_G] = 0.5 * m;
+    scal[S_OC_DONE] = 0.0;
+    scal[S_OC_STEPS] = 0.0;
+    scal[S_VOL] = cnt;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_oc_trial_flat(int64_t n, int64_t chunk, const uint8_t* __restrict__ act,
+                                                       const uint8_t* __restrict__ shell, const float* __restrict__ rho,
+                                                       const float* __restrict__ s, double move, double damp,
+                                                       double rmin, const double* __restrict__ scal,
+                                                       double* __restrict__ partial) {
+  if (scal[S_OC_DONE] != 0.0) return;
+  const double g = scal[S_OC_G];
+  const int64_t i0 = (int64_t)blockIdx.x * chunk, i1 = min(n, i0 + chunk);
+  double v = 0;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += 256)
+    if (!act || act[i]) v += (double)oc_rho(rho[i], s[i], g, move, damp, rmin, shell && shell[i]);
+  v = block_sum256(v);
+  if (threadIdx.x == 0) partial[blockIdx.x] = v;
+}
+
+// active elements get the OC density of the final threshold, the others keep their input
+__global__ void k_oc_apply_flat(int64_t n, const uint8_t* __restrict__ act, const uint8_t* __restrict__ shell,
+                                const float* rho, const float* __restrict__ s, float* rho_out, double move,
+                                double damp, double rmin, const double* __restrict__ scal) {
+  const int64_t i = tid64();
+  if (i >= n) return;
+  const float r0 = rho[i];
+  rho_out[i] = (!act || act[i]) ? oc_rho(r0, s[i], scal[S_OC_G], move, damp, rmin, shell && shell[i]) : r0;
 }
 
 // ---------------------------------------------------------------------------------------

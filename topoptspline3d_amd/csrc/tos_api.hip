@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -2939,7 +2940,7 @@ TOS_EXPORT int tos_oc_update(tos_ctx* c, double vgoal, double move, double damp,
                          (const uint8_t*)c->shell, (const float*)c->rho, (const float*)c->sens, move, damp,
                          c->rho_min, (const double*)c->scal, c->partial);
       hipLaunchKernelGGL(k_oc_decide, dim3(1), dim3(256), 0, c->stream, (const double*)c->partial, kRedBlocks,
-                         (double)c->n_active, vgoal, c->scal);
+                         (double)c->n_active, 0, vgoal, c->scal);
     }
     hipLaunchKernelGGL(k_oc_apply, dim3(nblk(v.NE)), dim3(256), 0, c->stream, v, (const uint8_t*)c->eact,
                        (const uint8_t*)c->shell, c->rho, (const float*)c->sens, move, damp, c->rho_min,
@@ -3099,7 +3100,7 @@ TOS_EXPORT int tos_oc_update_coeffs(tos_ctx* c, double vgoal, double move, doubl
       hipLaunchKernelGGL(k_sum_active, dim3(kRedBlocks), dim3(256), 0, c->stream, v, (const uint8_t*)c->eact,
                          (const float*)c->rho, c->partial);
       hipLaunchKernelGGL(k_oc_decide, dim3(1), dim3(256), 0, c->stream, (const double*)c->partial, kRedBlocks,
-                         (double)c->n_active, vgoal, c->scal);
+                         (double)c->n_active, 0, vgoal, c->scal);
     }
     LAUNCH_CHECK();
     const double* hs = read_scal(*c);
@@ -3681,4 +3682,75 @@ TOS_EXPORT int tos_ctx_learn_stats(tos_ctx* c, int* tried, int* accepted, double
     if (mean_step) *mean_step = on && hs[S_NN_KEPT] > 0 ? hs[S_NN_STEP] / hs[S_NN_KEPT] : 0.0;
     return TOS_OK;
   });
+}
+
+// ---------------------------------------------------------------------------------------
+// OC update without allocation: the bisection's block partials and scalars live in a
+// file-scope device array, one copy per GPU, whose address is taken once per device.
+// ---------------------------------------------------------------------------------------
+namespace {
+constexpr int kNoAllocChunk = 256 * 32;   // elements per block of the flat OC kernels
+constexpr int kNoAllocParts = 8192;       // block partials the scratch holds
+static_assert((int64_t)kNoAllocChunk * kNoAllocParts == TOS_NOALLOC_MAX_ELEMENTS, "capacity");
+// [0, 2*kNoAllocParts): partials (max |s| and active counts, then trial volumes);
+// [2*kNoAllocParts, +S_COUNT): the bisection scalars (S_OC_*, S_VOL = active count)
+static __device__ __align__(256) double scratch[2 * kNoAllocParts + S_COUNT];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// device address of this device's scratch (hipGetSymbolAddress once per device)
+hipError_t scratch_address(int dev, double** out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[dev] = (double*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return hipSuccess;
+}
+}  // namespace
+
+TOS_EXPORT int tos_oc_update_noalloc(int64_t n, const uint8_t* active, const uint8_t* shell, const float* rho,
+                                     const float* sens, float* rho_out, double vgoal, double move, double damp,
+                                     double rho_min, void* stream, double* g_out, int* steps_out) {
+  if (n < 0 || !rho || !sens || !rho_out || !(move > 0) || !(damp > 0)) return TOS_E_ARG;
+  if (n > TOS_NOALLOC_MAX_ELEMENTS) return TOS_E_TOO_LARGE;
+  if (n == 0) {
+    if (g_out) *g_out = 0.0;
+    if (steps_out) *steps_out = 0;
+    return TOS_OK;
+  }
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return TOS_E_HIP;
+  double* S = nullptr;
+  if (scratch_address(dev, &S) != hipSuccess) return TOS_E_HIP;
+  double* part = S;
+  double* scal = S + 2 * kNoAllocParts;
+  const hipStream_t st = (hipStream_t)stream;
+  const int nb = (int)std::max<int64_t>(1, (n + kNoAllocChunk - 1) / kNoAllocChunk);
+  // every scratch value a later kernel reads is written earlier in this call: the partials
+  // by the pass before their fold, the scalars by k_oc_init_flat
+  hipLaunchKernelGGL(k_absmax_count_flat, dim3(nb), dim3(256), 0, st, n, (int64_t)kNoAllocChunk, active, sens, nb,
+                     part);
+  hipLaunchKernelGGL(k_oc_init_flat, dim3(1), dim3(256), 0, st, (const double*)part, nb, scal);
+  for (int it = 0; it < 31; ++it) {
+    hipLaunchKernelGGL(k_oc_trial_flat, dim3(nb), dim3(256), 0, st, n, (int64_t)kNoAllocChunk, active, shell, rho,
+                       sens, move, damp, rho_min, (const double*)scal, part);
+    hipLaunchKernelGGL(k_oc_decide, dim3(1), dim3(256), 0, st, (const double*)part, nb, 0.0, 1, vgoal, scal);
+  }
+  hipLaunchKernelGGL(k_oc_apply_flat, dim3(nblk(n)), dim3(256), 0, st, n, active, shell, rho, sens, rho_out, move,
+                     damp, rho_min, (const double*)scal);
+  if (hipGetLastError() != hipSuccess) return TOS_E_HIP;
+  if (g_out || steps_out) {
+    double h[S_COUNT];
+    if (hipMemcpyAsync(h, scal, sizeof(h), hipMemcpyDeviceToHost, st) != hipSuccess) return TOS_E_HIP;
+    if (hipStreamSynchronize(st) != hipSuccess) return TOS_E_HIP;
+    if (g_out) *g_out = h[S_OC_G];
+    if (steps_out) *steps_out = (int)h[S_OC_STEPS];
+  }
+  return TOS_OK;
 }
