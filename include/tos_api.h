@@ -188,6 +188,16 @@ int tos_element_compliance(tos_ctx* ctx, int k, double* out);
  * threshold g and the number of bisection steps. */
 int tos_oc_update(tos_ctx* ctx, double vgoal, double move, double damp,
                   double* g_out, int* steps_out);
+/* OC update of the design variable behind the Heaviside projection (updateDensities2,
+ * optimization/optimization.cu:462-557): the design variable is the projection input kept
+ * by tos_project_density; each bisection trial updates it as tos_oc_update does and
+ * projects it (beta) before the volume sum; the search stops at |V - vgoal| <= 1e-4, at a
+ * relative bracket (hi - lo)/(lo + hi) <= 1e-3 or after 51 trials; the last trial becomes
+ * the design variable and the densities its projection.  change (may be NULL) receives the
+ * max and the mean |x_new - x_old| over the active elements (the reference divides its
+ * sum by the padded element count). */
+int tos_oc_update_projected(tos_ctx* ctx, double vgoal, double move, double damp, double beta,
+                            double* g_out, int* steps_out, double* change);
 /* OC update without any allocation.  No counterpart in the reference: for callers that
  * cannot allocate device memory (code under HIP graph capture, tight serving loops,
  * harnesses that forbid hipMalloc).  Same update as tos_oc_update (threshold bisection
@@ -226,6 +236,7 @@ int tos_spline_eval(tos_ctx* ctx);
 /* coefficient gradient of sensitivity `which` (Grid::ddensity2dcoeff_update /
  * dvol2dcoeff, grid/Grid.cu:3643-3859), deterministic gather. */
 int tos_spline_adjoint(tos_ctx* ctx, int which);
+/* which: 0 compliance, 1 volume, 2 self-supporting, 3 drip (tos_surface_constraints) */
 int tos_spline_get_coeff_sens(tos_ctx* ctx, int which, float* out);
 /* OC bisection on the spline coefficients with the compliance coefficient sensitivity
  * (updateCoeff / tryCSensMultiplier_kernel, optimization/optimization.cu:559-695): each
@@ -235,6 +246,56 @@ int tos_spline_get_coeff_sens(tos_ctx* ctx, int which, float* out);
  * on top of it). */
 int tos_oc_update_coeffs(tos_ctx* ctx, double vgoal, double move, double damp, double cmin,
                          double cmax, double* g_out, int* steps_out);
+
+/* ---- self-supporting and drip constraints on spline surface points --------------------
+ * Replaces deal_surface_points (optimization/optimization.cpp:1103-1128) with
+ * Grid::generate_spline_surface_nodes (grid/Grid.cpp:3984-4030), the surface-point normals
+ * and their orientation test (grid/Grid.cu:5529-5763), the self-supporting constraint
+ * (:6015-6320) and the drip constraint (:6485-6560, 6744-7156) with their coefficient
+ * gradients.  Surface points: the marching-cubes vertices of the level set
+ * {spline = 1/2} sampled on a background lattice of `cells` cells per axis over the grid
+ * box plus one cell on each side, node values spline - 1/2 inside the box and -1/2
+ * outside (compute_background_mcPoints_value, grid/Grid.cu:3986-4080): one point per
+ * lattice edge whose end values change sign, linearly interpolated, kept when it lies in
+ * an active element, ordered by lattice node then axis.  The normal is direction * grad
+ * spline, direction = +1 when most points see void (Heaviside(spline, beta) < 1/2) a step
+ * 1e-3 * grad spline ahead, else -1.  With inner = n_z / (|n| cos print_angle):
+ *   ss_mode 0/1: (mean inner^p_norm)^(1/p_norm) - 1; 2/3: mean h(inner), h(x) =
+ *   hfunction_c (x-1)^2 for x >= 1; 4/5: mean oh(inner), oh(x) = 1 + sigmoid((x - 1.5) /
+ *   sigmoid_c); even modes over the points with n_z < 0 (value 0 if none), odd over all.
+ *   drip_mode 6/7 (exp_drip / exp2_drip): mean over the n_z < 0 / all points of
+ *   u(hinner) q1(H_xx) q2(H_xx H_yy - H_xy H_yx), hinner = n_z / (|n| cos drip_angle),
+ *   u(x) = sigmoid((x - 1.2) drip_mu) for drip_angle > pi/2 else sigmoid(-(x + 1.2) drip_mu)
+ *   (drip_version 1: exp(-drip_alpha ((n_x/n_z)^2 + (n_y/n_z)^2)) instead of u),
+ *   q1(x) = drip_beta_scale x^2 / (1 + exp(drip_beta x)), q2(x) = drip_beta_scale x^2 /
+ *   (1 + exp(-drip_beta x)), H the spline's Hessian; modes 0-5 give per-point value 0
+ *   (the reference's compute_spline_drip_constraint_test evaluates only 6 and 7), so the
+ *   global value is -1 for modes 0/1 and 0 otherwise.
+ * The coefficient gradients (points held fixed) are exact derivatives of these values and
+ * are read with tos_spline_get_coeff_sens(which = 2: self-support, 3: drip). */
+typedef struct tos_surface_opts {
+  int cells;                 /* background lattice cells per axis (reference: 64)          */
+  double beta;               /* Heaviside sharpness of the orientation test                */
+  double print_angle;        /* radians (reference default 3*pi/4, Grid.cpp:2576)          */
+  int ss_mode;               /* GlobalSSMode 0..5 (reference flag default "oh" = 4)        */
+  int drip_mode;             /* GlobalDripMode 0..7 (reference flag default "oh" = 4)      */
+  double p_norm, hfunction_c, sigmoid_c;          /* 8, 10, 0.09 (grid/Grid.h:351-355)    */
+  double drip_beta, drip_mu, drip_angle, drip_alpha, drip_beta_scale;
+                             /* 128, 32, 160/180*pi, 4, 10 (grid/Grid.h:359-367)           */
+  int drip_version;          /* 0 (grid/Grid.h:366)                                        */
+} tos_surface_opts;
+
+typedef struct tos_surface_stats {
+  int64_t n_points;          /* surface points kept                                        */
+  int64_t n_down;            /* of them with n_z < 0                                       */
+  double direction;          /* normal orientation, +1 or -1                               */
+  double ss_value;           /* global self-supporting value                               */
+  double drip_value;         /* global drip value                                          */
+} tos_surface_stats;
+
+int tos_surface_constraints(tos_ctx* ctx, const tos_surface_opts* opts, tos_surface_stats* stats);
+/* the last call's points and normals (3 per point, x y z), at most `cap` of them */
+int tos_surface_points(tos_ctx* ctx, int64_t cap, double* xyz, double* normals);
 
 /* All K load cases in one solve (no counterpart in the reference, which solves load
  * cases one at a time): case-interleaved vectors so every stencil / density read serves
@@ -268,6 +329,13 @@ int tos_mma_update(tos_mma* mma, const double* df, const double* dg, const doubl
  * g = vol_scale*(volume - vfrac); the new x is written to the coefficients. */
 int tos_ctx_mma_step(tos_ctx* ctx, tos_mma* mma, double sens_scale, double vol_scale,
                      double vfrac, int* converged);
+/* One MMA step of the self-supporting spline driver (utils/test_utils.cpp:1135-1188 with
+ * ENABLE_SELFSUPPORT): m = 3, df = sens_scale * compliance coefficient sensitivity,
+ * g = [vol_scale (volume - vfrac), ss_scale (ss_value - 1), drip_scale drip_value] with
+ * their coefficient gradients (volume, self-support, drip; the last
+ * tos_surface_constraints call's values); the new x is written to the coefficients. */
+int tos_ctx_mma_step_surface(tos_ctx* ctx, tos_mma* mma, double sens_scale, double vol_scale, double vfrac,
+                             double ss_scale, double drip_scale, int* converged);
 
 /* Fine-level Gauss-Seidel schedule (no counterpart in the reference, which launches one
  * pass per colour): 0 = eight colour passes; 1 = colour-slab wavefront, colour c of z-slab

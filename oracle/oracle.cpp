@@ -11,6 +11,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <stdexcept>
@@ -661,6 +663,8 @@ static int solve_case(Ctx& c, int k, const tos_solve_opts& o, int* its, double* 
   *its = 0;
   if (nf == 0) { std::fill(u, u + n3, 0.0); *relres = 0; return TOS_OK; }
   std::vector<double> r(n3), z(n3), p(n3), q(n3);
+  // ORC_PROGRESS set: one line per FCG iteration on stderr (long solves of large domains)
+  const bool progress = std::getenv("ORC_PROGRESS") != nullptr;
   apply_fine(c, u, q.data());
   for (int64_t i = 0; i < n3; ++i) r[i] = f[i] - q[i];
   double rn = std::sqrt(dot3(L, r.data(), r.data()));
@@ -687,6 +691,7 @@ static int solve_case(Ctx& c, int k, const tos_solve_opts& o, int* its, double* 
       ++it;
       rn = std::sqrt(dot3(L, r.data(), r.data()));
       if (!std::isfinite(rn)) { c.err = "NaN in PCG"; return TOS_E_NAN; }
+      if (progress) std::fprintf(stderr, "[oracle] FCG iteration %d rel_res %.3e\n", it, rn / nf);
       if (rn / nf <= o.rtol) break;
       precondition(c, r.data(), z.data(), o.pre, o.post);
       double zr = dot3(L, z.data(), r.data());
@@ -1558,6 +1563,68 @@ ORC_API int orc_oc_update(tos_ctx* c, double vgoal, double move, double damp, do
   return TOS_OK;
 }
 
+// Heaviside projection of one design value (computeProjection_kernel
+// optimization/optimization.cu:313-337): eta = 0.5, clamp [rho_min, 1], shell -> 1.
+static double heaviside(double x, double beta, double rmin, bool shell) {
+  const double eta = 0.5;
+  double r = (std::tanh(beta * eta) + std::tanh(beta * (x - eta))) / (std::tanh(beta * eta) + std::tanh(beta * (1 - eta)));
+  r = std::min(std::max(r, rmin), 1.0);
+  return shell ? 1.0 : r;
+}
+
+// OC update of the projected design (updateDensities2 optimization/optimization.cu:462-557):
+// the trials update the design variable rho_pre (InitRho) and are projected before the
+// volume sum; the bisection stops at |V - vgoal| <= 1e-4, at a relative bracket
+// (hi - lo)/(lo + hi) <= 1e-3, or after 51 trials; the final trial becomes the new design
+// variable and rho its projection (projectDensities).  change = max and mean |x_new - x_old|
+// over the active elements (the reference's mean runs over its padded element count).
+ORC_API int orc_oc_update_projected(tos_ctx* c, double vgoal, double move, double damp, double beta, double* g_out,
+                                    int* steps, double* change) {
+  if (!c || !(move > 0) || !(damp > 0) || !(beta > 0)) return TOS_E_ARG;
+  const Level& L = c->L[0];
+  const bool has_shell = !c->shell.empty();
+  auto trial = [&](int64_t e, double g) {
+    double s = c->sens[e];
+    double gg = s > 0 ? 0.0 : -s;
+    double r0 = c->rho_pre[e];
+    double rn = r0 * std::pow(gg / g, damp);
+    rn = std::min(std::max(rn, r0 - move), r0 + move);
+    rn = std::min(std::max(rn, c->rho_min), 1.0);
+    if (has_shell && c->shell[e]) rn = 1;
+    return (float)rn;
+  };
+  double gmax = 0;
+  for (int64_t e = 0; e < L.ne; ++e)
+    if (L.eact[e]) gmax = std::max(gmax, (double)std::fabs(c->sens[e]));
+  double lo = 0, hi = gmax, g = 0, V = 2;
+  int itn = 0, trials = 0;
+  do {
+    g = (lo + hi) / 2;
+    V = chunked_sum(L.ne, [&](int64_t e) -> double {
+          if (!L.eact[e]) return 0.0;
+          return (double)(float)heaviside(trial(e, g), beta, c->rho_min, has_shell && c->shell[e]);
+        }) / (double)c->n_active;
+    ++trials;
+    if (V > vgoal) lo = g;
+    else if (V < vgoal) hi = g;
+  } while (std::fabs(V - vgoal) > 1e-4 && (hi - lo) / (lo + hi) > 1e-3 && itn++ < 50);
+  double cmax = 0, csum = 0;
+  for (int64_t e = 0; e < L.ne; ++e) {
+    if (!L.eact[e]) { c->rho[e] = 0; continue; }
+    const float xn = trial(e, g);
+    const double d = std::fabs((double)xn - (double)c->rho_pre[e]);
+    cmax = std::max(cmax, d);
+    csum += d;
+    c->rho_pre[e] = xn;
+    c->rho[e] = (float)heaviside(xn, beta, c->rho_min, has_shell && c->shell[e]);
+  }
+  if (g_out) *g_out = g;
+  if (steps) *steps = trials;
+  if (change) { change[0] = cmax; change[1] = csum / (double)c->n_active; }
+  c->op_ready = false;
+  return TOS_OK;
+}
+
 // Heaviside projection (computeProjection_kernel optimization/optimization.cu:313-337).
 ORC_API int orc_project_density(tos_ctx* c, double beta) {
   if (!c) return TOS_E_ARG;
@@ -1675,9 +1742,386 @@ ORC_API int orc_spline_adjoint(tos_ctx* c, int which) {
 }
 
 ORC_API int orc_spline_get_coeff_sens(tos_ctx* c, int which, float* out) {
-  if (!c || !out || c->sp_mode < 0 || (which != 0 && which != 1)) return TOS_E_ARG;
-  auto& v = which == 0 ? c->csens : c->cvsens;
+  if (!c || !out || c->sp_mode < 0 || which < 0 || which > 3) return TOS_E_ARG;
+  auto& v = which == 0 ? c->csens : which == 1 ? c->cvsens : which == 2 ? c->sscsens : c->dripcsens;
+  if (v.size() != c->coef.size()) {
+    std::fill(out, out + c->coef.size(), 0.0f);
+    return TOS_OK;
+  }
   std::copy(v.begin(), v.end(), out);
+  return TOS_OK;
+}
+
+// ------------------------------------------------------------------------------------
+// Self-supporting and drip constraints on spline surface points (deal_surface_points,
+// optimization/optimization.cpp:1103-1128).
+// ------------------------------------------------------------------------------------
+// Basis values and first / second derivatives (NURBS Book algorithm A2.3, as
+// SplineBasisDeriX grid/Grid.cu:469-530) of the p+1 functions nonzero on knot span `span`.
+static void ders_basis_funs(int span, double x, int p, const std::vector<double>& U, double ders[3][16]) {
+  double ndu[16][16], left[16], right[16], a[2][16];
+  ndu[0][0] = 1.0;
+  for (int j = 1; j <= p; ++j) {
+    left[j] = x - U[span + 1 - j];
+    right[j] = U[span + j] - x;
+    double saved = 0.0;
+    for (int r = 0; r < j; ++r) {
+      ndu[j][r] = right[r + 1] + left[j - r];
+      double temp = ndu[r][j - 1] / ndu[j][r];
+      ndu[r][j] = saved + right[r + 1] * temp;
+      saved = left[j - r] * temp;
+    }
+    ndu[j][j] = saved;
+  }
+  for (int j = 0; j <= p; ++j) ders[0][j] = ndu[j][p];
+  for (int k = 1; k <= 2; ++k)
+    for (int j = 0; j <= p; ++j) ders[k][j] = 0.0;
+  for (int r = 0; r <= p; ++r) {
+    int s1 = 0, s2 = 1;
+    a[0][0] = 1.0;
+    for (int k = 1; k <= 2 && k <= p; ++k) {
+      double d = 0.0;
+      int rk = r - k, pk = p - k;
+      if (r >= k) {
+        a[s2][0] = a[s1][0] / ndu[pk + 1][rk];
+        d = a[s2][0] * ndu[rk][pk];
+      }
+      int j1 = rk >= -1 ? 1 : -rk, j2 = (r - 1 <= pk) ? k - 1 : p - r;
+      for (int j = j1; j <= j2; ++j) {
+        a[s2][j] = (a[s1][j] - a[s1][j - 1]) / ndu[pk + 1][rk + j];
+        d += a[s2][j] * ndu[rk + j][pk];
+      }
+      if (r <= pk) {
+        a[s2][k] = -a[s1][k - 1] / ndu[pk + 1][r];
+        d += a[s2][k] * ndu[r][pk];
+      }
+      ders[k][r] = d;
+      std::swap(s1, s2);
+    }
+  }
+  int r = p;
+  for (int k = 1; k <= 2; ++k) {
+    for (int j = 0; j <= p; ++j) ders[k][j] *= r;
+    r *= (p - k);
+  }
+}
+
+// spline basis along one axis at coordinate x: first coefficient index per term, values
+// and d/dx, d2/dx2; false outside the spline's range (mode 0: the knot range)
+struct AxisDers {
+  int idx[16];
+  double N[3][16];
+};
+static bool axis_ders(const Ctx& c, int axis, double x, AxisDers& D) {
+  const int k = c.sp_order;
+  const int ne = axis == 0 ? c.nx : axis == 1 ? c.ny : c.nz;
+  const int nc = axis == 0 ? c.ncx : axis == 1 ? c.ncy : c.ncz;
+  if (c.sp_mode == 1) {  // uniform lattice every param voxels, boundary coefficients replicated
+    const double sh = c.sp_param * c.h, t = x / sh;
+    const int i0 = (int)std::floor(t - k / 2.0) + 1;
+    for (int q = 0; q < k; ++q) {
+      const int i = i0 + q;
+      const double u = t - i + k / 2.0;
+      D.idx[q] = std::min(std::max(i, 0), nc - 1);
+      D.N[0][q] = cardinal(k, u);
+      D.N[1][q] = k >= 2 ? (cardinal(k - 1, u) - cardinal(k - 1, u - 1)) / sh : 0.0;
+      D.N[2][q] = k >= 3 ? (cardinal(k - 2, u) - 2 * cardinal(k - 2, u - 1) + cardinal(k - 2, u - 2)) / (sh * sh) : 0.0;
+    }
+    return true;
+  }
+  // clamped knots on the box enlarged by 2*box/(P+1) per side (Grid.cpp:3253-3377)
+  const int P = c.sp_param;
+  const double delta = ne * c.h, lo = -2.0 * delta / (P + 1), hi = delta + 2.0 * delta / (P + 1);
+  const double step = (hi - lo) / (P + 1);
+  std::vector<double> U;
+  for (int i = 0; i < k; ++i) U.push_back(lo);
+  double tk = lo;
+  for (int i = 0; i < P; ++i) { tk += step; U.push_back(tk); }
+  for (int i = 0; i < k; ++i) U.push_back(hi);
+  const int span1 = (int)((x - lo) / step) + k;  // calc_node grid/Grid.cu:5550-5552
+  if (span1 < k || span1 > nc) return false;
+  double ders[3][16];
+  ders_basis_funs(span1 - 1, x, k - 1, U, ders);
+  for (int q = 0; q < k; ++q) {
+    D.idx[q] = span1 - k + q;
+    for (int d = 0; d < 3; ++d) D.N[d][q] = ders[d][q];
+  }
+  return true;
+}
+
+// spline value, gradient and Hessian (row-major 3x3) at p
+static bool spline_point(const Ctx& c, const double p[3], double& phi, double g[3], double H[9]) {
+  AxisDers A[3];
+  for (int a = 0; a < 3; ++a)
+    if (!axis_ders(c, a, p[a], A[a])) return false;
+  const int k = c.sp_order;
+  phi = 0;
+  for (int i = 0; i < 3; ++i) g[i] = 0;
+  for (int i = 0; i < 9; ++i) H[i] = 0;
+  for (int qz = 0; qz < k; ++qz)
+    for (int qy = 0; qy < k; ++qy)
+      for (int qx = 0; qx < k; ++qx) {
+        const double cv = c.coef[A[0].idx[qx] + (size_t)c.ncx * (A[1].idx[qy] + (size_t)c.ncy * A[2].idx[qz])];
+        const double X[3] = {A[0].N[0][qx], A[0].N[1][qx], A[0].N[2][qx]};
+        const double Y[3] = {A[1].N[0][qy], A[1].N[1][qy], A[1].N[2][qy]};
+        const double Z[3] = {A[2].N[0][qz], A[2].N[1][qz], A[2].N[2][qz]};
+        phi += cv * X[0] * Y[0] * Z[0];
+        g[0] += cv * X[1] * Y[0] * Z[0];
+        g[1] += cv * X[0] * Y[1] * Z[0];
+        g[2] += cv * X[0] * Y[0] * Z[1];
+        H[0] += cv * X[2] * Y[0] * Z[0];
+        H[4] += cv * X[0] * Y[2] * Z[0];
+        H[8] += cv * X[0] * Y[0] * Z[2];
+        H[1] += cv * X[1] * Y[1] * Z[0];
+        H[2] += cv * X[1] * Y[0] * Z[1];
+        H[5] += cv * X[0] * Y[1] * Z[1];
+      }
+  H[3] = H[1]; H[6] = H[2]; H[7] = H[5];
+  return true;
+}
+
+static double sigm(double x) { return 1.0 / (1.0 + std::exp(-x)); }
+
+// per-point self-support value and d/dinner (grid/Grid.cu:117-160, 6015-6100)
+static void ss_point(const tos_surface_opts& o, double inner, bool down, double& v, double& dv) {
+  v = dv = 0;
+  const int m = o.ss_mode;
+  if (m % 2 == 0 && !down) return;
+  if (m <= 1) { v = std::pow(inner, o.p_norm); dv = o.p_norm * std::pow(inner, o.p_norm - 1); }
+  else if (m <= 3) { v = inner >= 1 ? o.hfunction_c * (inner - 1) * (inner - 1) : 0; dv = inner >= 1 ? 2 * o.hfunction_c * (inner - 1) : 0; }
+  else { double s = sigm((inner - 1.5) / o.sigmoid_c); v = s + 1; dv = s * (1 - s) / o.sigmoid_c; }
+}
+
+// Surface constraints and their coefficient gradients (points fixed); see tos_api.h.
+// Constraint values at the stored surface points with the stored orientation, and with
+// `grad` their coefficient gradients (the points held fixed).
+static void surface_eval(Ctx& c, const tos_surface_opts& o, bool grad, tos_surface_stats* st) {
+  const int64_t n = (int64_t)c.surf_p.size() / 3;
+  const double dir = c.surf_dir;
+  std::vector<double> HH(9 * n);
+  c.surf_n.assign(3 * n, 0.0);
+  for (int64_t pi = 0; pi < n; ++pi) {
+    double phi, g[3], H[9];
+    if (!spline_point(c, &c.surf_p[3 * pi], phi, g, H)) { for (double& v : g) v = 0; for (double& v : H) v = 0; }
+    for (int a = 0; a < 3; ++a) c.surf_n[3 * pi + a] = dir * g[a];
+    for (int a = 0; a < 9; ++a) HH[9 * pi + a] = H[a];
+  }
+  // per-point values and gradient weights: d(value)/dc_j = w . grad N_j + wxx N_j,xx +
+  // wyy N_j,yy + wxy N_j,xy
+  const double cs = std::cos(o.print_angle), cd = std::cos(o.drip_angle);
+  std::vector<double> wss(3 * n, 0.0), wdr(6 * n, 0.0);
+  double ss_sum = 0, dr_sum = 0;
+  int64_t n_down = 0;
+  const bool drip_on = o.drip_mode >= 6;
+  for (int64_t pi = 0; pi < n; ++pi) {
+    const double* nv = &c.surf_n[3 * pi];
+    const double* H = &HH[9 * pi];
+    const double nn = std::sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+    const bool down = nv[2] < 0;
+    n_down += down;
+    if (nn == 0) continue;
+    // d inner / d n = (e_z |n| - n_z n / |n|) / (|n|^2 cos); d n / d c_j = dir grad N_j
+    double dinner[3];
+    for (int a = 0; a < 3; ++a) dinner[a] = dir * ((a == 2 ? nn : 0.0) - nv[2] * nv[a] / nn) / (nn * nn);
+    double v, dv;
+    ss_point(o, nv[2] / nn / cs, down, v, dv);
+    ss_sum += v;
+    for (int a = 0; a < 3; ++a) wss[3 * pi + a] = dv * dinner[a] / cs;
+    if (drip_on && (o.drip_mode == 7 || down)) {
+      const double hin = nv[2] / nn / cd;
+      double u, du[3];
+      if (o.drip_version == 1) {
+        const double px = nv[0] / nv[2], py = nv[1] / nv[2];
+        u = std::exp(-o.drip_alpha * (px * px + py * py));
+        // d px / d n = (e_x n_z - n_x e_z) / n_z^2, likewise py
+        for (int a = 0; a < 3; ++a) {
+          const double dpx = ((a == 0 ? nv[2] : 0.0) - (a == 2 ? nv[0] : 0.0)) / (nv[2] * nv[2]);
+          const double dpy = ((a == 1 ? nv[2] : 0.0) - (a == 2 ? nv[1] : 0.0)) / (nv[2] * nv[2]);
+          du[a] = -2 * o.drip_alpha * (px * dpx + py * dpy) * u * dir;
+        }
+      } else {
+        double s, ds;
+        if (o.drip_angle > 0.5 * M_PI) { s = sigm((hin - 1.2) * o.drip_mu); ds = s * (1 - s) * o.drip_mu; }
+        else { s = sigm(-(hin + 1.2) * o.drip_mu); ds = -s * (1 - s) * o.drip_mu; }
+        u = s;
+        for (int a = 0; a < 3; ++a) du[a] = ds * dinner[a] / cd;
+      }
+      const double x1 = H[0], x2 = H[0] * H[4] - H[1] * H[3];
+      const double e1 = std::exp(o.drip_beta * x1), e2 = std::exp(-o.drip_beta * x2);
+      const double q1 = o.drip_beta_scale * x1 * x1 / (1 + e1);
+      const double q2 = o.drip_beta_scale * x2 * x2 / (1 + e2);
+      const double dq1 = o.drip_beta_scale * (2 * x1 / (1 + e1) - x1 * x1 * o.drip_beta * e1 / ((1 + e1) * (1 + e1)));
+      const double dq2 = o.drip_beta_scale * (2 * x2 / (1 + e2) + x2 * x2 * o.drip_beta * e2 / ((1 + e2) * (1 + e2)));
+      dr_sum += u * q1 * q2;
+      for (int a = 0; a < 3; ++a) wdr[6 * pi + a] = du[a] * q1 * q2;
+      // d x1 = N_xx; d x2 = N_xx H_yy + H_xx N_yy - 2 H_xy N_xy (H symmetric)
+      wdr[6 * pi + 3] = u * (dq1 * q2 + q1 * dq2 * H[4]);
+      wdr[6 * pi + 4] = u * q1 * dq2 * H[0];
+      wdr[6 * pi + 5] = u * q1 * dq2 * (-2 * H[1]);
+    }
+  }
+  const double cnt_ss = o.ss_mode % 2 == 0 ? (double)n_down : (double)n;
+  const double cnt_dr = o.drip_mode == 6 ? (double)n_down : o.drip_mode % 2 == 0 ? (double)n_down : (double)n;
+  double ss_val = 0, ss_fac = 0, dr_val = 0, dr_fac = 0;
+  if (cnt_ss > 0) {
+    const double mean = ss_sum / cnt_ss;
+    if (o.ss_mode <= 1) {
+      ss_val = std::pow(mean, 1.0 / o.p_norm) - 1;
+      ss_fac = mean > 0 ? std::pow(mean, 1.0 / o.p_norm - 1) / o.p_norm / cnt_ss : 0.0;
+    } else {
+      ss_val = mean;
+      ss_fac = 1.0 / cnt_ss;
+    }
+  }
+  if (cnt_dr > 0) {
+    dr_val = o.drip_mode <= 1 ? -1.0 : dr_sum / cnt_dr;
+    dr_fac = drip_on ? 1.0 / cnt_dr : 0.0;
+  }
+  c.ss_value = ss_val;
+  c.drip_value = dr_val;
+  if (!grad) {
+    if (st) { st->n_points = n; st->n_down = n_down; st->direction = dir; st->ss_value = ss_val; st->drip_value = dr_val; }
+    return;
+  }
+  // coefficient gradients: every point scatters into its order^3 coefficients in point order
+  const size_t nc = c.coef.size();
+  std::vector<double> gss(nc, 0.0), gdr(nc, 0.0);
+  const int k = c.sp_order;
+  for (int64_t pi = 0; pi < n; ++pi) {
+    AxisDers A[3];
+    bool ok = true;
+    for (int a = 0; a < 3; ++a) ok = ok && axis_ders(c, a, c.surf_p[3 * pi + a], A[a]);
+    if (!ok) continue;
+    const double* ws = &wss[3 * pi];
+    const double* wd = &wdr[6 * pi];
+    for (int qz = 0; qz < k; ++qz)
+      for (int qy = 0; qy < k; ++qy)
+        for (int qx = 0; qx < k; ++qx) {
+          const size_t j = A[0].idx[qx] + (size_t)c.ncx * (A[1].idx[qy] + (size_t)c.ncy * A[2].idx[qz]);
+          const double X0 = A[0].N[0][qx], X1 = A[0].N[1][qx], X2 = A[0].N[2][qx];
+          const double Y0 = A[1].N[0][qy], Y1 = A[1].N[1][qy], Y2 = A[1].N[2][qy];
+          const double Z0 = A[2].N[0][qz], Z1 = A[2].N[1][qz];
+          const double gx = X1 * Y0 * Z0, gy = X0 * Y1 * Z0, gz = X0 * Y0 * Z1;
+          gss[j] += ws[0] * gx + ws[1] * gy + ws[2] * gz;
+          gdr[j] += wd[0] * gx + wd[1] * gy + wd[2] * gz + wd[3] * X2 * Y0 * Z0 + wd[4] * X0 * Y2 * Z0 +
+                    wd[5] * X1 * Y1 * Z0;
+        }
+  }
+  c.sscsens.assign(nc, 0.0f);
+  c.dripcsens.assign(nc, 0.0f);
+  for (size_t j = 0; j < nc; ++j) {
+    c.sscsens[j] = (float)(ss_fac * gss[j]);
+    c.dripcsens[j] = (float)(dr_fac * gdr[j]);
+  }
+  c.ss_value = ss_val;
+  c.drip_value = dr_val;
+  if (st) {
+    st->n_points = n;
+    st->n_down = n_down;
+    st->direction = dir;
+    st->ss_value = ss_val;
+    st->drip_value = dr_val;
+  }
+}
+
+
+ORC_API int orc_surface_constraints(tos_ctx* c, const tos_surface_opts* o, tos_surface_stats* st) {
+  if (!c || !o || c->sp_mode < 0 || o->cells < 1 || o->ss_mode < 0 || o->ss_mode > 5 || o->drip_mode < 0 ||
+      o->drip_mode > 7 || !(o->beta > 0))
+    return TOS_E_ARG;
+  // all coefficients equal: no surface; the driver's constants (utils/test_utils.cpp:1106-1113)
+  bool equal = true;
+  for (float v : c->coef) equal = equal && v == c->coef[0];
+  if (equal) {
+    c->surf_p.clear();
+    c->surf_n.clear();
+    c->sscsens.assign(c->coef.size(), 0.0f);
+    c->dripcsens.assign(c->coef.size(), 0.0f);
+    c->ss_value = 1.0;
+    c->drip_value = 0.0;
+    if (st) { st->n_points = 0; st->n_down = 0; st->direction = 1.0; st->ss_value = 1.0; st->drip_value = 0.0; }
+    return TOS_OK;
+  }
+  const int dims[3] = {c->nx, c->ny, c->nz};
+  const int B = o->cells, NB = B + 2;
+  double eh[3];
+  for (int a = 0; a < 3; ++a) eh[a] = dims[a] * c->h / B;
+  auto pos = [&](int a, int i) { return -eh[a] + (i + 0.5) * eh[a]; };
+  // lattice values: spline - 1/2 inside the box, -1/2 outside
+  std::vector<double> val((size_t)NB * NB * NB);
+  for (int k = 0; k < NB; ++k)
+    for (int j = 0; j < NB; ++j)
+      for (int i = 0; i < NB; ++i) {
+        const double p[3] = {pos(0, i), pos(1, j), pos(2, k)};
+        bool in = true;
+        for (int a = 0; a < 3; ++a) in = in && p[a] >= 0 && p[a] <= dims[a] * c->h;
+        double phi = 0, g[3], H[9];
+        if (in && !spline_point(*c, p, phi, g, H)) phi = -0.2;
+        val[i + (size_t)NB * (j + (size_t)NB * k)] = (in ? phi : 0.0) - 0.5;
+      }
+  // marching-cubes vertices on sign-changing edges, kept inside active elements
+  c->surf_p.clear();
+  const Level& L = c->L[0];
+  for (int k = 0; k < NB; ++k)
+    for (int j = 0; j < NB; ++j)
+      for (int i = 0; i < NB; ++i) {
+        const int ijk[3] = {i, j, k};
+        const double va = val[i + (size_t)NB * (j + (size_t)NB * k)];
+        for (int d = 0; d < 3; ++d) {
+          int q[3] = {i, j, k};
+          if (++q[d] >= NB) continue;
+          const double vb = val[q[0] + (size_t)NB * (q[1] + (size_t)NB * q[2])];
+          if ((va < 0) == (vb < 0)) continue;
+          const double t = (0.0 - va) / (vb - va);
+          double p[3];
+          for (int a = 0; a < 3; ++a) p[a] = pos(a, ijk[a]) + (a == d ? t * eh[a] : 0.0);
+          int e[3];
+          bool ok = true;
+          for (int a = 0; a < 3; ++a) {
+            e[a] = (int)std::floor(p[a] / c->h);
+            ok = ok && e[a] >= 0 && e[a] < dims[a];
+          }
+          if (!ok || !L.eact[L.eid(e[0], e[1], e[2])]) continue;
+          for (int a = 0; a < 3; ++a) c->surf_p.push_back(p[a]);
+        }
+      }
+  const int64_t n = (int64_t)c->surf_p.size() / 3;
+  // gradient, Hessian and the orientation test (correct_spline_surface_point_normal_direction)
+  std::vector<double> G(3 * n), HH(9 * n);
+  double count = 0;
+  const double eta = 0.5, hden = std::tanh(o->beta * eta) + std::tanh(o->beta * (1 - eta));
+  for (int64_t pi = 0; pi < n; ++pi) {
+    double phi, g[3], H[9];
+    if (!spline_point(*c, &c->surf_p[3 * pi], phi, g, H)) { for (double& v : g) v = 0; for (double& v : H) v = 0; }
+    for (int a = 0; a < 3; ++a) G[3 * pi + a] = g[a];
+    for (int a = 0; a < 9; ++a) HH[9 * pi + a] = H[a];
+    double pd[3], v2 = -0.1, g2[3], H2[9];
+    for (int a = 0; a < 3; ++a) pd[a] = c->surf_p[3 * pi + a] + 1e-3 * g[a];
+    if (!spline_point(*c, pd, v2, g2, H2)) v2 = -0.1;
+    const double hs = (std::tanh(o->beta * eta) + std::tanh(o->beta * (v2 - eta))) / hden;
+    count += hs < 0.5 ? 1.0 : -1.0;
+  }
+  c->surf_dir = count > 0 ? 1.0 : -1.0;
+  surface_eval(*c, *o, true, st);
+  return TOS_OK;
+}
+
+// Test access (no counterpart in the product ABI): the constraint values at the points and
+// orientation of the last orc_surface_constraints call, for the current coefficients.
+ORC_API int orc_surface_eval_fixed(tos_ctx* c, const tos_surface_opts* o, double* ss, double* drip) {
+  if (!c || !o || c->sp_mode < 0) return TOS_E_ARG;
+  tos_surface_stats st;
+  surface_eval(*c, *o, false, &st);
+  if (ss) *ss = st.ss_value;
+  if (drip) *drip = st.drip_value;
+  return TOS_OK;
+}
+
+ORC_API int orc_surface_points(tos_ctx* c, int64_t cap, double* xyz, double* normals) {
+  if (!c || cap < 0 || (cap > 0 && (!xyz || !normals))) return TOS_E_ARG;
+  const int64_t n = std::min<int64_t>(cap, (int64_t)c->surf_p.size() / 3);
+  std::copy(c->surf_p.begin(), c->surf_p.begin() + 3 * n, xyz);
+  std::copy(c->surf_n.begin(), c->surf_n.begin() + 3 * n, normals);
   return TOS_OK;
 }
 
@@ -1752,6 +2196,17 @@ ORC_API int orc_mma_get_x(tos_mma* M, double* x) {
   return TOS_OK;
 }
 
+// Test access to the MMA state (no counterpart in the product ABI): asymptotes L, U and
+// move bounds alpha, beta after the last update -- the subproblem data of the next one.
+ORC_API int orc_mma_state(tos_mma* M, double* L, double* U, double* alpha, double* beta) {
+  if (!M || !L || !U || !alpha || !beta) return TOS_E_ARG;
+  std::copy(M->L.begin(), M->L.end(), L);
+  std::copy(M->U.begin(), M->U.end(), U);
+  std::copy(M->alpha.begin(), M->alpha.end(), alpha);
+  std::copy(M->beta.begin(), M->beta.end(), beta);
+  return TOS_OK;
+}
+
 ORC_API int orc_mma_update(tos_mma* M, const double* df, const double* dg, const double* g, int* conv) {
   if (!M || !df || !dg || !g) return TOS_E_ARG;
   bool cv = mma_update(*M, df, dg, g);
@@ -1768,6 +2223,29 @@ ORC_API int orc_ctx_mma_step(tos_ctx* c, tos_mma* M, double sscale, double vscal
   for (int j = 0; j < n; ++j) { df[j] = sscale * (double)c->csens[j]; dg[j] = vscale * (double)c->cvsens[j]; }
   double g = vscale * (vol - vfrac);
   bool cv = mma_update(*M, df.data(), dg.data(), &g);
+  for (int j = 0; j < n; ++j) c->coef[j] = (float)M->x[j];
+  if (conv) *conv = cv ? 1 : 0;
+  return TOS_OK;
+}
+
+// MMA step with the volume, self-supporting and drip constraints (utils/test_utils.cpp:1135-1188).
+ORC_API int orc_ctx_mma_step_surface(tos_ctx* c, tos_mma* M, double sscale, double vscale, double vfrac,
+                                     double ss_scale, double drip_scale, int* conv) {
+  if (!c || !M || c->sp_mode < 0 || M->n != (int)c->coef.size() || M->m != 3 ||
+      c->sscsens.size() != c->coef.size())
+    return TOS_E_ARG;
+  double vol;
+  orc_volume(c, &vol);
+  int n = M->n;
+  std::vector<double> df(n), dg(3 * (size_t)n);
+  for (int j = 0; j < n; ++j) {
+    df[j] = sscale * (double)c->csens[j];
+    dg[j] = vscale * (double)c->cvsens[j];
+    dg[n + j] = ss_scale * (double)c->sscsens[j];
+    dg[2 * (size_t)n + j] = drip_scale * (double)c->dripcsens[j];
+  }
+  double g[3] = {vscale * (vol - vfrac), ss_scale * (c->ss_value - 1.0), drip_scale * c->drip_value};
+  bool cv = mma_update(*M, df.data(), dg.data(), g);
   for (int j = 0; j < n; ++j) c->coef[j] = (float)M->x[j];
   if (conv) *conv = cv ? 1 : 0;
   return TOS_OK;

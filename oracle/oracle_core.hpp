@@ -74,6 +74,10 @@ struct Ctx {
   int ncx = 0, ncy = 0, ncz = 0;
   SplineAxis ax[3];
   std::vector<float> coef, csens, cvsens;
+  std::vector<float> sscsens, dripcsens;        // self-support / drip coefficient gradients
+  std::vector<double> surf_p, surf_n;          // surface points and normals (3 per point)
+  double ss_value = 1.0, drip_value = 0.0;     // last surface-constraint values
+  double surf_dir = 1.0;                        // orientation of the surface normals
   std::string err;
 };
 

@@ -48,6 +48,19 @@ class WorstStats(C.Structure):
                 ("rel_res", C.c_double), ("seconds", C.c_double)]
 
 
+class SurfaceOpts(C.Structure):
+    _fields_ = [("cells", C.c_int), ("beta", C.c_double), ("print_angle", C.c_double), ("ss_mode", C.c_int),
+                ("drip_mode", C.c_int), ("p_norm", C.c_double), ("hfunction_c", C.c_double),
+                ("sigmoid_c", C.c_double), ("drip_beta", C.c_double), ("drip_mu", C.c_double),
+                ("drip_angle", C.c_double), ("drip_alpha", C.c_double), ("drip_beta_scale", C.c_double),
+                ("drip_version", C.c_int)]
+
+
+class SurfaceStats(C.Structure):
+    _fields_ = [("n_points", C.c_int64), ("n_down", C.c_int64), ("direction", C.c_double),
+                ("ss_value", C.c_double), ("drip_value", C.c_double)]
+
+
 class KernelStats(C.Structure):
     _fields_ = [("n_classes", C.c_int), ("names", (C.c_char * 32) * 20),
                 ("total_ms", C.c_double * 20), ("launches", C.c_int64 * 20)]
@@ -86,6 +99,7 @@ SIGNATURES = {
     "filter": (C.c_int, [vp, C.c_int, C.c_double]),
     "element_compliance": (C.c_int, [vp, C.c_int, f64p]),
     "oc_update": (C.c_int, [vp, C.c_double, C.c_double, C.c_double, f64p, i32p]),
+    "oc_update_projected": (C.c_int, [vp, C.c_double, C.c_double, C.c_double, C.c_double, f64p, i32p, f64p]),
     "project_density": (C.c_int, [vp, C.c_double]),
     "projection_sensitivity": (C.c_int, [vp, C.c_int, C.c_double]),
     "spline_init": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
@@ -101,6 +115,9 @@ SIGNATURES = {
     "mma_get_x": (C.c_int, [vp, f64p]),
     "mma_update": (C.c_int, [vp, f64p, f64p, f64p, i32p]),
     "ctx_mma_step": (C.c_int, [vp, vp, C.c_double, C.c_double, C.c_double, i32p]),
+    "ctx_mma_step_surface": (C.c_int, [vp, vp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, i32p]),
+    "surface_constraints": (C.c_int, [vp, C.POINTER(SurfaceOpts), C.POINTER(SurfaceStats)]),
+    "surface_points": (C.c_int, [vp, C.c_int64, f64p, f64p]),
     "profile": (C.c_int, [vp, C.c_int, C.POINTER(KernelStats)]),
     "solve_batched": (C.c_int, [vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     "set_sweep_mode": (C.c_int, [vp, C.c_int]),

@@ -12,7 +12,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtos.so")
 SOURCES = ["tos_api.hip"]  # single translation unit; includes the kernel files
-DEPS = ["tos_internal.hpp", "kernels_mg.hip", "kernels_opt.hip", "mma.hip", "tos_api.hip"]
+DEPS = ["tos_internal.hpp", "kernels_mg.hip", "kernels_opt.hip", "kernels_surface.hip", "mma.hip", "tos_api.hip"]
 
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared", "--offload-arch=gfx950",

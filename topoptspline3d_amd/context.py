@@ -251,6 +251,17 @@ class Context:
         self._chk(self.api.oc_update(self.h, vgoal, move, damp, C.byref(g), C.byref(n)), "oc_update")
         return g.value, n.value
 
+    def oc_update_projected(self, vgoal, move=0.03, damp=0.5, beta=8.0):
+        """OC on the design variable behind the Heaviside projection (updateDensities2,
+        optimization/optimization.cu:462-557); returns (threshold, trials, max change,
+        mean change of the design variable)"""
+        g = C.c_double()
+        n = C.c_int()
+        ch = np.zeros(2)
+        self._chk(self.api.oc_update_projected(self.h, vgoal, move, damp, beta, C.byref(g), C.byref(n),
+                                               _p(ch, C.c_double)), "oc_update_projected")
+        return g.value, n.value, float(ch[0]), float(ch[1])
+
     def project_density(self, beta):
         self._chk(self.api.project_density(self.h, beta), "project_density")
 
@@ -285,6 +296,27 @@ class Context:
         self._chk(self.api.oc_update_coeffs(self.h, vgoal, move, damp, cmin, cmax, C.byref(g), C.byref(n)),
                   "oc_update_coeffs")
         return g.value, n.value
+
+    def surface_constraints(self, cells=64, beta=8.0, print_angle=0.75 * np.pi, ss_mode=4, drip_mode=4,
+                            p_norm=8.0, hfunction_c=10.0, sigmoid_c=0.09, drip_beta=128.0, drip_mu=32.0,
+                            drip_angle=160.0 / 180.0 * np.pi, drip_alpha=4.0, drip_beta_scale=10.0,
+                            drip_version=0):
+        """self-supporting and drip constraints on the spline's surface points (deal_surface_points,
+        optimization/optimization.cpp:1103-1128); gradients via coeff_sens(2) / coeff_sens(3)"""
+        o = _abi.SurfaceOpts(int(cells), float(beta), float(print_angle), int(ss_mode), int(drip_mode),
+                             float(p_norm), float(hfunction_c), float(sigmoid_c), float(drip_beta),
+                             float(drip_mu), float(drip_angle), float(drip_alpha), float(drip_beta_scale),
+                             int(drip_version))
+        st = _abi.SurfaceStats()
+        self._chk(self.api.surface_constraints(self.h, C.byref(o), C.byref(st)), "surface_constraints")
+        return {"n_points": st.n_points, "n_down": st.n_down, "direction": st.direction,
+                "ss_value": st.ss_value, "drip_value": st.drip_value}
+
+    def surface_points(self, n):
+        p = np.zeros((n, 3))
+        nv = np.zeros((n, 3))
+        self._chk(self.api.surface_points(self.h, n, _p(p, C.c_double), _p(nv, C.c_double)), "surface_points")
+        return p, nv
 
     def coeff_sens(self, which=0):
         c = np.zeros(self.n_coeffs, np.float32)
@@ -338,6 +370,15 @@ class MMA:
         rc = self.api.mma_update(self.h, _p(df, C.c_double), _p(dg, C.c_double), _p(g, C.c_double), C.byref(cv))
         if rc:
             raise TosError(rc, "mma_update", "")
+        return bool(cv.value)
+
+    def step_on_surface(self, ctx, sens_scale, vol_scale, vfrac, ss_scale, drip_scale):
+        """m = 3: volume, self-supporting and drip constraints (tos_ctx_mma_step_surface)"""
+        cv = C.c_int()
+        rc = self.api.ctx_mma_step_surface(ctx.h, self.h, sens_scale, vol_scale, vfrac, ss_scale, drip_scale,
+                                           C.byref(cv))
+        if rc:
+            raise TosError(rc, "ctx_mma_step_surface", "")
         return bool(cv.value)
 
     def step_on(self, ctx, sens_scale, vol_scale, vfrac):

@@ -269,9 +269,13 @@ __global__ __launch_bounds__(256) void k_oc_trial(LevelView L, const uint8_t* __
   if (threadIdx.x == 0) partial[blockIdx.x] = v;
 }
 
-// nact: active element count, read from scal[S_VOL] when nact_dev is set
+// nact: active element count, read from scal[S_VOL] when nact_dev is set.  Stop rules:
+// |V - vgoal| <= 1e-4 or max_trials trials (updateDensities: 31, optimization.cu:446), and
+// with rel_bracket > 0 also (hi - lo)/(lo + hi) <= rel_bracket (updateDensities2: 1e-3 and
+// 51 trials, optimization.cu:534)
 __global__ __launch_bounds__(256) void k_oc_decide(const double* __restrict__ partial, int np, double nact,
-                                                   int nact_dev, double vgoal, double* __restrict__ scal) {
+                                                   int nact_dev, double vgoal, int max_trials, double rel_bracket,
+                                                   double* __restrict__ scal) {
   if (scal[S_OC_DONE] != 0.0) return;
   double v = 0;
   for (int i = threadIdx.x; i < np; i += 256) v += partial[i];
@@ -283,8 +287,10 @@ __global__ __launch_bounds__(256) void k_oc_decide(const double* __restrict__ pa
     scal[S_OC_STEPS] += 1.0;
     if (V > vgoal) scal[S_OC_LO] = g;
     else if (V < vgoal) scal[S_OC_HI] = g;
-    if (fabs(V - vgoal) <= 1e-4 || scal[S_OC_STEPS] >= 31.0) scal[S_OC_DONE] = 1.0;
-    else scal[S_OC_G] = 0.5 * (scal[S_OC_LO] + scal[S_OC_HI]);
+    const double lo = scal[S_OC_LO], hi = scal[S_OC_HI];
+    const bool bracket = rel_bracket > 0 && !((hi - lo) / (lo + hi) > rel_bracket);
+    if (fabs(V - vgoal) <= 1e-4 || bracket || scal[S_OC_STEPS] >= (double)max_trials) scal[S_OC_DONE] = 1.0;
+    else scal[S_OC_G] = 0.5 * (lo + hi);
   }
 }
 
@@ -294,6 +300,78 @@ __global__ void k_oc_apply(LevelView L, const uint8_t* __restrict__ eact, const 
   const int64_t i = tid64();
   if (i >= L.NE || !eact[i]) return;
   rho[i] = oc_rho(rho[i], s[i], scal[S_OC_G], move, damp, rmin, shell[i] != 0);
+}
+
+// Heaviside projection of one design value (computeProjection_kernel
+// optimization/optimization.cu:313-337): eta = 0.5, clamp [rmin, 1], shell -> 1
+__device__ __forceinline__ float heaviside_rho(double x, double beta, double rmin, bool shell) {
+  const double eta = 0.5, den = tanh(beta * eta) + tanh(beta * (1 - eta));
+  double r = (tanh(beta * eta) + tanh(beta * (x - eta))) / den;
+  r = fmin(fmax(r, rmin), 1.0);
+  return shell ? 1.0f : (float)r;
+}
+
+// projected OC trial (updateDensities2 optimization.cu:490-520): the design variable x
+// (rho_pre) gets the OC update, its projection enters the volume
+__global__ __launch_bounds__(256) void k_oc_trial_proj(LevelView L, const uint8_t* __restrict__ eact,
+                                                       const uint8_t* __restrict__ shell,
+                                                       const float* __restrict__ x, const float* __restrict__ s,
+                                                       double move, double damp, double rmin, double beta,
+                                                       const double* __restrict__ scal, double* __restrict__ partial) {
+  if (scal[S_OC_DONE] != 0.0) return;
+  const double g = scal[S_OC_G];
+  double v = 0;
+  for (int64_t i = tid64(); i < L.NE; i += (int64_t)gridDim.x * blockDim.x)
+    if (eact[i]) {
+      const bool sh = shell[i] != 0;
+      v += (double)heaviside_rho((double)oc_rho(x[i], s[i], g, move, damp, rmin, sh), beta, rmin, sh);
+    }
+  v = block_sum256(v);
+  if (threadIdx.x == 0) partial[blockIdx.x] = v;
+}
+
+// final trial as the new design variable, its projection as the density, and per-block
+// max / sum of |x_new - x_old| (partial[b], partial[gridDim.x + b])
+__global__ __launch_bounds__(256) void k_oc_apply_proj(LevelView L, const uint8_t* __restrict__ eact,
+                                                       const uint8_t* __restrict__ shell, float* __restrict__ rho,
+                                                       float* __restrict__ x, const float* __restrict__ s, double move,
+                                                       double damp, double rmin, double beta,
+                                                       const double* __restrict__ scal, double* __restrict__ partial) {
+  const double g = scal[S_OC_G];
+  double m = 0, sum = 0;
+  for (int64_t i = tid64(); i < L.NE; i += (int64_t)gridDim.x * blockDim.x)
+    if (eact[i]) {
+      const bool sh = shell[i] != 0;
+      const float x0 = x[i];
+      const float xn = oc_rho(x0, s[i], g, move, damp, rmin, sh);
+      const double d = fabs((double)xn - (double)x0);
+      m = fmax(m, d);
+      sum += d;
+      x[i] = xn;
+      rho[i] = heaviside_rho((double)xn, beta, rmin, sh);
+    }
+  m = block_max256(m);
+  sum = block_sum256(sum);
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x] = m;
+    partial[gridDim.x + blockIdx.x] = sum;
+  }
+}
+
+// folds k_oc_apply_proj's partials: scal[S_TMP0] = max, scal[S_TMP1] = sum
+__global__ __launch_bounds__(256) void k_change_fold(const double* __restrict__ partial, int np,
+                                                     double* __restrict__ scal) {
+  double m = 0, sum = 0;
+  for (int i = threadIdx.x; i < np; i += 256) {
+    m = fmax(m, partial[i]);
+    sum += partial[np + i];
+  }
+  m = block_max256(m);
+  sum = block_sum256(sum);
+  if (threadIdx.x == 0) {
+    scal[S_TMP0] = m;
+    scal[S_TMP1] = sum;
+  }
 }
 
 __global__ __launch_bounds__(256) void k_sum_active(LevelView L, const uint8_t* __restrict__ eact,
@@ -384,13 +462,9 @@ __global__ void k_project(LevelView L, const uint8_t* __restrict__ eact, const u
                           float* __restrict__ rho, float* __restrict__ rho_pre, double beta, double rmin) {
   const int64_t i = tid64();
   if (i >= L.NE || !eact[i]) return;
-  const double eta = 0.5, den = tanh(beta * eta) + tanh(beta * (1 - eta));
   const float r0 = rho[i];
   rho_pre[i] = r0;
-  double r = (tanh(beta * eta) + tanh(beta * ((double)r0 - eta))) / den;
-  r = fmin(fmax(r, rmin), 1.0);
-  if (shell[i]) r = 1.0;
-  rho[i] = (float)r;
+  rho[i] = heaviside_rho((double)r0, beta, rmin, shell[i] != 0);
 }
 
 __global__ void k_project_sens(LevelView L, const uint8_t* __restrict__ eact, const float* __restrict__ rho_pre,

@@ -1,0 +1,457 @@
+// kernels_surface.hip -- self-supporting and drip constraints on the surface points of the
+// spline density (deal_surface_points, optimization/optimization.cpp:1103-1128): the
+// marching-cubes vertices of {spline = 1/2} on a background lattice, their normals and
+// orientation (grid/Grid.cu:5529-5763), the per-point constraint values (:6015-6100,
+// 6744-6925) and the coefficient gradients as a deterministic per-coefficient gather over
+// the points in the coefficient's support (the reference scatters with unsynchronised
+// `+=`, :6290, 7133).
+#include "tos_internal.hpp"
+
+namespace tos {
+
+// one axis of the spline: mode 1 = uniform lattice of spacing sh with replicated boundary
+// coefficients, mode 0 = clamped knots U (Grid::set_spline_knot_series grid/Grid.cpp:3253-3377)
+struct SfAxis {
+  int mode, k, nc;
+  double sh;        // mode 1: param * h
+  double lo, step;  // mode 0: first knot and interior knot step
+  double box;       // grid extent n * h
+  const double* U;  // mode 0: knot vector
+};
+
+struct SfAxes {
+  SfAxis a[3];
+};
+
+// cardinal B-spline of order o on the knots 0..o (Cox-de Boor on uniform knots)
+__device__ __forceinline__ double sf_cardinal(int o, double t) {
+  if (o <= 0 || t < 0.0 || t >= (double)o) return 0.0;
+  double N[9];
+  for (int i = 0; i < o; ++i) N[i] = (t >= i && t < i + 1) ? 1.0 : 0.0;
+  for (int p = 1; p < o; ++p)
+    for (int i = 0; i + p < o; ++i) N[i] = ((t - i) * N[i] + (i + p + 1 - t) * N[i + 1]) / p;
+  return N[0];
+}
+
+// basis values and first / second derivatives of the p+1 functions nonzero on knot span
+// `span` (The NURBS Book algorithm A2.3; SplineBasisDeriX grid/Grid.cu:469-530)
+__device__ void sf_ders_basis(int span, double x, int p, const double* U, double ders[3][8]) {
+  double ndu[8][8], left[8], right[8], a[2][8];
+  ndu[0][0] = 1.0;
+  for (int j = 1; j <= p; ++j) {
+    left[j] = x - U[span + 1 - j];
+    right[j] = U[span + j] - x;
+    double saved = 0.0;
+    for (int r = 0; r < j; ++r) {
+      ndu[j][r] = right[r + 1] + left[j - r];
+      const double temp = ndu[r][j - 1] / ndu[j][r];
+      ndu[r][j] = saved + right[r + 1] * temp;
+      saved = left[j - r] * temp;
+    }
+    ndu[j][j] = saved;
+  }
+  for (int j = 0; j <= p; ++j) {
+    ders[0][j] = ndu[j][p];
+    ders[1][j] = 0.0;
+    ders[2][j] = 0.0;
+  }
+  for (int r = 0; r <= p; ++r) {
+    int s1 = 0, s2 = 1;
+    a[0][0] = 1.0;
+    for (int k = 1; k <= 2 && k <= p; ++k) {
+      double d = 0.0;
+      const int rk = r - k, pk = p - k;
+      if (r >= k) {
+        a[s2][0] = a[s1][0] / ndu[pk + 1][rk];
+        d = a[s2][0] * ndu[rk][pk];
+      }
+      const int j1 = rk >= -1 ? 1 : -rk, j2 = (r - 1 <= pk) ? k - 1 : p - r;
+      for (int j = j1; j <= j2; ++j) {
+        a[s2][j] = (a[s1][j] - a[s1][j - 1]) / ndu[pk + 1][rk + j];
+        d += a[s2][j] * ndu[rk + j][pk];
+      }
+      if (r <= pk) {
+        a[s2][k] = -a[s1][k - 1] / ndu[pk + 1][r];
+        d += a[s2][k] * ndu[r][pk];
+      }
+      ders[k][r] = d;
+      const int t = s1;
+      s1 = s2;
+      s2 = t;
+    }
+  }
+  int r = p;
+  for (int k = 1; k <= 2; ++k) {
+    for (int j = 0; j <= p; ++j) ders[k][j] *= r;
+    r *= (p - k);
+  }
+}
+
+// coefficient index and value / d / d2 of the k terms along one axis at x; false outside
+// the spline's range (mode 0: the knot range)
+__device__ bool sf_axis(const SfAxis& A, double x, int idx[8], double N[3][8]) {
+  const int k = A.k;
+  if (A.mode == 1) {
+    const double t = x / A.sh;
+    const int i0 = (int)floor(t - k / 2.0) + 1;
+    for (int q = 0; q < k; ++q) {
+      const int i = i0 + q;
+      const double u = t - i + k / 2.0;
+      idx[q] = min(max(i, 0), A.nc - 1);
+      N[0][q] = sf_cardinal(k, u);
+      N[1][q] = (sf_cardinal(k - 1, u) - sf_cardinal(k - 1, u - 1)) / A.sh;
+      N[2][q] = (sf_cardinal(k - 2, u) - 2.0 * sf_cardinal(k - 2, u - 1) + sf_cardinal(k - 2, u - 2)) / (A.sh * A.sh);
+    }
+    return true;
+  }
+  const int span1 = (int)((x - A.lo) / A.step) + k;  // calc_node grid/Grid.cu:5550-5552
+  if (span1 < k || span1 > A.nc) return false;
+  double d[3][8];
+  sf_ders_basis(span1 - 1, x, k - 1, A.U, d);
+  for (int q = 0; q < k; ++q) {
+    idx[q] = span1 - k + q;
+    N[0][q] = d[0][q];
+    N[1][q] = d[1][q];
+    N[2][q] = d[2][q];
+  }
+  return true;
+}
+
+// spline value, gradient and Hessian (xx, yy, zz, xy, xz, yz) at p
+__device__ bool sf_point(const SfAxes& S, const float* __restrict__ coef, const double p[3], double& phi,
+                         double g[3], double H[6]) {
+  int ix[8], iy[8], iz[8];
+  double X[3][8], Y[3][8], Z[3][8];
+  if (!sf_axis(S.a[0], p[0], ix, X) || !sf_axis(S.a[1], p[1], iy, Y) || !sf_axis(S.a[2], p[2], iz, Z)) return false;
+  const int k = S.a[0].k, ncx = S.a[0].nc, ncy = S.a[1].nc;
+  phi = 0;
+  for (int i = 0; i < 3; ++i) g[i] = 0;
+  for (int i = 0; i < 6; ++i) H[i] = 0;
+  for (int qz = 0; qz < k; ++qz)
+    for (int qy = 0; qy < k; ++qy)
+      for (int qx = 0; qx < k; ++qx) {
+        const double cv = coef[ix[qx] + (int64_t)ncx * (iy[qy] + (int64_t)ncy * iz[qz])];
+        phi += cv * X[0][qx] * Y[0][qy] * Z[0][qz];
+        g[0] += cv * X[1][qx] * Y[0][qy] * Z[0][qz];
+        g[1] += cv * X[0][qx] * Y[1][qy] * Z[0][qz];
+        g[2] += cv * X[0][qx] * Y[0][qy] * Z[1][qz];
+        H[0] += cv * X[2][qx] * Y[0][qy] * Z[0][qz];
+        H[1] += cv * X[0][qx] * Y[2][qy] * Z[0][qz];
+        H[2] += cv * X[0][qx] * Y[0][qy] * Z[2][qz];
+        H[3] += cv * X[1][qx] * Y[1][qy] * Z[0][qz];
+        H[4] += cv * X[1][qx] * Y[0][qy] * Z[1][qz];
+        H[5] += cv * X[0][qx] * Y[1][qy] * Z[1][qz];
+      }
+  return true;
+}
+
+// background lattice of NB = cells + 2 nodes per axis at (i + 1/2) eh - eh
+struct SfLattice {
+  int NB;
+  double eh[3];
+  __device__ double pos(int a, int i) const { return -eh[a] + (i + 0.5) * eh[a]; }
+};
+
+// node values: spline - 1/2 inside the box (-0.2 - 1/2 outside the spline range), -1/2
+// outside the box (compute_background_mcPoints_value grid/Grid.cu:3986-4080)
+__global__ void k_sf_lattice(SfAxes S, SfLattice G, const float* __restrict__ coef, double* __restrict__ val) {
+  const int64_t n = (int64_t)G.NB * G.NB * G.NB;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i = (int)(t % G.NB), j = (int)((t / G.NB) % G.NB), kk = (int)(t / ((int64_t)G.NB * G.NB));
+  const double p[3] = {G.pos(0, i), G.pos(1, j), G.pos(2, kk)};
+  bool in = true;
+  for (int a = 0; a < 3; ++a) in = in && p[a] >= 0.0 && p[a] <= S.a[a].box;
+  double phi = 0, g[3], H[6];
+  if (in && !sf_point(S, coef, p, phi, g, H)) phi = -0.2;
+  val[t] = (in ? phi : 0.0) - 0.5;
+}
+
+// the point on the lattice edge from node t along axis d, when the edge changes sign and
+// the point lies in an active element
+__device__ __forceinline__ bool sf_edge_point(const SfLattice& G, LevelView L, const uint8_t* __restrict__ eact,
+                                              const double* __restrict__ val, double h, int i, int j, int kk, int d,
+                                              double p[3]) {
+  int q[3] = {i, j, kk};
+  if (++q[d] >= G.NB) return false;
+  const double va = val[i + (int64_t)G.NB * (j + (int64_t)G.NB * kk)];
+  const double vb = val[q[0] + (int64_t)G.NB * (q[1] + (int64_t)G.NB * q[2])];
+  if ((va < 0) == (vb < 0)) return false;
+  const double t = (0.0 - va) / (vb - va);
+  const int ijk[3] = {i, j, kk};
+  const int dims[3] = {L.ex, L.ey, L.ez};
+  int e[3];
+  for (int a = 0; a < 3; ++a) {
+    p[a] = G.pos(a, ijk[a]) + (a == d ? t * G.eh[a] : 0.0);
+    e[a] = (int)floor(p[a] / h);
+    if (e[a] < 0 || e[a] >= dims[a]) return false;
+  }
+  return eact[L.pelem(e[0], e[1], e[2])] != 0;
+}
+
+__global__ void k_sf_count(SfLattice G, LevelView L, const uint8_t* __restrict__ eact, const double* __restrict__ val,
+                           double h, int* __restrict__ cnt) {
+  const int64_t n = (int64_t)G.NB * G.NB * G.NB;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i = (int)(t % G.NB), j = (int)((t / G.NB) % G.NB), kk = (int)(t / ((int64_t)G.NB * G.NB));
+  int c = 0;
+  double p[3];
+  for (int d = 0; d < 3; ++d) c += sf_edge_point(G, L, eact, val, h, i, j, kk, d, p);
+  cnt[t] = c;
+}
+
+// exclusive prefix sum of n counts in one workgroup (1024 threads, contiguous chunks):
+// off[i] = sum of cnt[0..i), off[n] = total
+__global__ __launch_bounds__(1024) void k_sf_scan(const int* __restrict__ cnt, int64_t n, int* __restrict__ off) {
+  __shared__ int part[1024];
+  const int t = threadIdx.x;
+  const int64_t per = (n + 1023) / 1024, b = t * per, e = min(n, b + per);
+  int s = 0;
+  for (int64_t i = b; i < e; ++i) s += cnt[i];
+  part[t] = s;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {  // inclusive scan of the chunk sums
+    const int v = t >= o ? part[t - o] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int run = part[t] - s;
+  for (int64_t i = b; i < e; ++i) {
+    off[i] = run;
+    run += cnt[i];
+  }
+  if (t == 1023) off[n] = part[1023];
+}
+
+__global__ void k_sf_points(SfLattice G, LevelView L, const uint8_t* __restrict__ eact,
+                            const double* __restrict__ val, double h, const int* __restrict__ off,
+                            double* __restrict__ pts) {
+  const int64_t n = (int64_t)G.NB * G.NB * G.NB;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i = (int)(t % G.NB), j = (int)((t / G.NB) % G.NB), kk = (int)(t / ((int64_t)G.NB * G.NB));
+  int w = off[t];
+  double p[3];
+  for (int d = 0; d < 3; ++d)
+    if (sf_edge_point(G, L, eact, val, h, i, j, kk, d, p)) {
+      for (int a = 0; a < 3; ++a) pts[3 * (int64_t)w + a] = p[a];
+      ++w;
+    }
+}
+
+// gradient and Hessian at every point, and the orientation vote (+1 when a step of
+// 1e-3 grad spline ends in void, Heaviside < 1/2; correct_spline_surface_point_normal_direction
+// grid/Grid.cu:5620-5763); per-block vote sums in part[b]
+__global__ __launch_bounds__(256) void k_sf_orient(SfAxes S, const float* __restrict__ coef, int64_t n,
+                                                   const double* __restrict__ pts, double beta,
+                                                   double* __restrict__ grad, double* __restrict__ hess,
+                                                   double* __restrict__ part) {
+  double vote = 0;
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) {
+    const double p[3] = {pts[3 * t], pts[3 * t + 1], pts[3 * t + 2]};
+    double phi, g[3], H[6];
+    if (!sf_point(S, coef, p, phi, g, H)) {
+      for (double& v : g) v = 0;
+      for (double& v : H) v = 0;
+    }
+    for (int a = 0; a < 3; ++a) grad[3 * t + a] = g[a];
+    for (int a = 0; a < 6; ++a) hess[6 * t + a] = H[a];
+    double pd[3], v2 = -0.1, g2[3], H2[6];
+    for (int a = 0; a < 3; ++a) pd[a] = p[a] + 1e-3 * g[a];
+    if (!sf_point(S, coef, pd, v2, g2, H2)) v2 = -0.1;
+    const double eta = 0.5, den = tanh(beta * eta) + tanh(beta * (1 - eta));
+    const double hs = (tanh(beta * eta) + tanh(beta * (v2 - eta))) / den;
+    vote += hs < 0.5 ? 1.0 : -1.0;
+  }
+  vote = block_sum256(vote);
+  if (threadIdx.x == 0) part[blockIdx.x] = vote;
+}
+
+struct SfParams {
+  double cs, cd;  // cos(print_angle), cos(drip_angle)
+  int ss_mode, drip_mode, drip_version;
+  double p_norm, hfunction_c, sigmoid_c, drip_beta, drip_mu, drip_angle, drip_alpha, drip_beta_scale;
+};
+
+__device__ __forceinline__ double sf_sigm(double x) { return 1.0 / (1.0 + exp(-x)); }
+
+// per-point values and gradient weights with the orientation dir = scal[slot]; partials
+// [3][nblk]: self-support sum, drip sum, points with n_z < 0
+__global__ __launch_bounds__(256) void k_sf_values(SfParams P, int64_t n, const double* __restrict__ grad,
+                                                   const double* __restrict__ hess, const double* __restrict__ scal,
+                                                   int dir_slot, double* __restrict__ nrm, double* __restrict__ wss,
+                                                   double* __restrict__ wdr, double* __restrict__ part) {
+  const double dir = scal[dir_slot] > 0 ? 1.0 : -1.0;
+  double s_ss = 0, s_dr = 0, s_dn = 0;
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) {
+    double nv[3];
+    for (int a = 0; a < 3; ++a) nrm[3 * t + a] = nv[a] = dir * grad[3 * t + a];
+    for (int a = 0; a < 3; ++a) wss[3 * t + a] = 0;
+    for (int a = 0; a < 6; ++a) wdr[6 * t + a] = 0;
+    const double nn = sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+    const bool down = nv[2] < 0;
+    s_dn += down ? 1.0 : 0.0;
+    if (nn == 0) continue;
+    // d inner / d c_j = dinner . grad N_j / cos, dinner = dir (e_z |n| - n_z n / |n|) / |n|^2
+    double di[3];
+    for (int a = 0; a < 3; ++a) di[a] = dir * ((a == 2 ? nn : 0.0) - nv[2] * nv[a] / nn) / (nn * nn);
+    // self-support (compute_spline_selfsupp_constraint grid/Grid.cu:6015-6100, h/oh :117-160)
+    const double inner = nv[2] / nn / P.cs;
+    double v = 0, dv = 0;
+    if (!(P.ss_mode % 2 == 0 && !down)) {
+      if (P.ss_mode <= 1) {
+        v = pow(inner, P.p_norm);
+        dv = P.p_norm * pow(inner, P.p_norm - 1);
+      } else if (P.ss_mode <= 3) {
+        v = inner >= 1 ? P.hfunction_c * (inner - 1) * (inner - 1) : 0.0;
+        dv = inner >= 1 ? 2 * P.hfunction_c * (inner - 1) : 0.0;
+      } else {
+        const double s = sf_sigm((inner - 1.5) / P.sigmoid_c);
+        v = s + 1;
+        dv = s * (1 - s) / P.sigmoid_c;
+      }
+    }
+    s_ss += v;
+    for (int a = 0; a < 3; ++a) wss[3 * t + a] = dv * di[a] / P.cs;
+    // drip (compute_spline_drip_constraint_test grid/Grid.cu:6744-6925), modes 6 / 7
+    if (P.drip_mode >= 6 && (P.drip_mode == 7 || down)) {
+      const double* H = &hess[6 * t];  // xx, yy, zz, xy, xz, yz
+      const double hin = nv[2] / nn / P.cd;
+      double u, du[3];
+      if (P.drip_version == 1) {
+        const double px = nv[0] / nv[2], py = nv[1] / nv[2];
+        u = exp(-P.drip_alpha * (px * px + py * py));
+        for (int a = 0; a < 3; ++a) {
+          const double dpx = ((a == 0 ? nv[2] : 0.0) - (a == 2 ? nv[0] : 0.0)) / (nv[2] * nv[2]);
+          const double dpy = ((a == 1 ? nv[2] : 0.0) - (a == 2 ? nv[1] : 0.0)) / (nv[2] * nv[2]);
+          du[a] = -2 * P.drip_alpha * (px * dpx + py * dpy) * u * dir;
+        }
+      } else {
+        double s, ds;
+        if (P.drip_angle > 0.5 * M_PI) {
+          s = sf_sigm((hin - 1.2) * P.drip_mu);
+          ds = s * (1 - s) * P.drip_mu;
+        } else {
+          s = sf_sigm(-(hin + 1.2) * P.drip_mu);
+          ds = -s * (1 - s) * P.drip_mu;
+        }
+        u = s;
+        for (int a = 0; a < 3; ++a) du[a] = ds * di[a] / P.cd;
+      }
+      const double x1 = H[0], x2 = H[0] * H[1] - H[3] * H[3];
+      const double e1 = exp(P.drip_beta * x1), e2 = exp(-P.drip_beta * x2);
+      const double q1 = P.drip_beta_scale * x1 * x1 / (1 + e1);
+      const double q2 = P.drip_beta_scale * x2 * x2 / (1 + e2);
+      const double dq1 = P.drip_beta_scale * (2 * x1 / (1 + e1) - x1 * x1 * P.drip_beta * e1 / ((1 + e1) * (1 + e1)));
+      const double dq2 = P.drip_beta_scale * (2 * x2 / (1 + e2) + x2 * x2 * P.drip_beta * e2 / ((1 + e2) * (1 + e2)));
+      s_dr += u * q1 * q2;
+      for (int a = 0; a < 3; ++a) wdr[6 * t + a] = du[a] * q1 * q2;
+      wdr[6 * t + 3] = u * (dq1 * q2 + q1 * dq2 * H[1]);  // with N_xx
+      wdr[6 * t + 4] = u * q1 * dq2 * H[0];                // with N_yy
+      wdr[6 * t + 5] = u * q1 * dq2 * (-2 * H[3]);         // with N_xy
+    }
+  }
+  s_ss = block_sum256(s_ss);
+  s_dr = block_sum256(s_dr);
+  s_dn = block_sum256(s_dn);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = s_ss;
+    part[gridDim.x + blockIdx.x] = s_dr;
+    part[2 * gridDim.x + blockIdx.x] = s_dn;
+  }
+}
+
+// folds `ncol` columns of nb block partials into scal[slot0 + col]
+__global__ __launch_bounds__(256) void k_sf_fold(const double* __restrict__ part, int nb, int ncol, int slot0,
+                                                 double* __restrict__ scal) {
+  for (int col = 0; col < ncol; ++col) {
+    double s = 0;
+    for (int i = threadIdx.x; i < nb; i += 256) s += part[(int64_t)col * nb + i];
+    s = block_sum256(s);
+    if (threadIdx.x == 0) scal[slot0 + col] = s;
+  }
+}
+
+// support [lo, hi] along one axis of coefficient i (mode 1: clamped indices extend the
+// first and last coefficient to the whole line)
+__device__ __forceinline__ void sf_support(const SfAxis& A, int i, double& lo, double& hi) {
+  if (A.mode == 1) {
+    lo = i == 0 ? -1e300 : (i - A.k / 2.0) * A.sh;
+    hi = i == A.nc - 1 ? 1e300 : (i + A.k / 2.0) * A.sh;
+  } else {
+    lo = A.U[i];
+    hi = A.U[i + A.k];
+  }
+}
+
+// the contributions d/dc_j of this coefficient's basis at one point: sum over the terms q
+// whose (possibly clamped) index equals the coefficient's
+__device__ __forceinline__ void sf_basis_at(const SfAxis& A, double x, int ci, double out[3]) {
+  int idx[8];
+  double N[3][8];
+  out[0] = out[1] = out[2] = 0;
+  if (!sf_axis(A, x, idx, N)) return;
+  for (int q = 0; q < A.k; ++q)
+    if (idx[q] == ci)
+      for (int d = 0; d < 3; ++d) out[d] += N[d][q];
+}
+
+// coefficient gradients: per coefficient, the points of the lattice nodes that can reach
+// its support, in lattice-node order; gss/gdr = factor * sum_points w . (basis derivatives)
+__global__ void k_sf_grad(SfAxes S, SfLattice G, int64_t ncoef, const int* __restrict__ off,
+                          const double* __restrict__ pts, const double* __restrict__ wss,
+                          const double* __restrict__ wdr, double fss, double fdr, float* __restrict__ gss,
+                          float* __restrict__ gdr) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ncoef) return;
+  const int ncx = S.a[0].nc, ncy = S.a[1].nc;
+  const int ci[3] = {(int)(j % ncx), (int)((j / ncx) % ncy), (int)(j / ((int64_t)ncx * ncy))};
+  int nlo[3], nhi[3];
+  for (int a = 0; a < 3; ++a) {
+    double lo, hi;
+    sf_support(S.a[a], ci[a], lo, hi);
+    // node i's points lie in [(i - 1/2) eh, (i + 1/2) eh] along the axis
+    const double l = fmax(lo / G.eh[a] - 0.5, -1.0), r = fmin(hi / G.eh[a] + 0.5, (double)G.NB);
+    nlo[a] = max(0, (int)floor(l) - 1);
+    nhi[a] = min(G.NB - 1, (int)ceil(r) + 1);
+  }
+  double a_ss = 0, a_dr = 0;
+  for (int kk = nlo[2]; kk <= nhi[2]; ++kk)
+    for (int jj = nlo[1]; jj <= nhi[1]; ++jj)
+      for (int ii = nlo[0]; ii <= nhi[0]; ++ii) {
+        const int64_t node = ii + (int64_t)G.NB * (jj + (int64_t)G.NB * kk);
+        for (int t = off[node]; t < off[node + 1]; ++t) {
+          double X[3], Y[3], Z[3];
+          sf_basis_at(S.a[0], pts[3 * (int64_t)t], ci[0], X);
+          if (X[0] == 0 && X[1] == 0 && X[2] == 0) continue;
+          sf_basis_at(S.a[1], pts[3 * (int64_t)t + 1], ci[1], Y);
+          sf_basis_at(S.a[2], pts[3 * (int64_t)t + 2], ci[2], Z);
+          const double gx = X[1] * Y[0] * Z[0], gy = X[0] * Y[1] * Z[0], gz = X[0] * Y[0] * Z[1];
+          const double* ws = &wss[3 * (int64_t)t];
+          const double* wd = &wdr[6 * (int64_t)t];
+          a_ss += ws[0] * gx + ws[1] * gy + ws[2] * gz;
+          a_dr += wd[0] * gx + wd[1] * gy + wd[2] * gz + wd[3] * X[2] * Y[0] * Z[0] + wd[4] * X[0] * Y[2] * Z[0] +
+                  wd[5] * X[1] * Y[1] * Z[0];
+        }
+      }
+  gss[j] = (float)(fss * a_ss);
+  gdr[j] = (float)(fdr * a_dr);
+}
+
+}  // namespace tos
+
+namespace tos {
+// df and the three constraint gradient rows of the self-supporting MMA step
+__global__ void k_sf_mma_inputs(int64_t n, const float* __restrict__ cs, const float* __restrict__ cv,
+                                const float* __restrict__ css, const float* __restrict__ cdr, double sscale,
+                                double vscale, double ss_scale, double drip_scale, double* __restrict__ out) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  out[j] = sscale * (double)cs[j];
+  out[n + j] = vscale * (double)cv[j];
+  out[2 * n + j] = ss_scale * (double)css[j];
+  out[3 * n + j] = drip_scale * (double)cdr[j];
+}
+}  // namespace tos

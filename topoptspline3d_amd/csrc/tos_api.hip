@@ -20,6 +20,7 @@
 #include "kernels_batch.hip"
 #include "kernels_nn.hip"
 #include "kernels_opt.hip"
+#include "kernels_surface.hip"
 #include "mma.hip"
 
 #define TOS_EXPORT extern "C" __attribute__((visibility("default")))
@@ -2940,7 +2941,7 @@ TOS_EXPORT int tos_oc_update(tos_ctx* c, double vgoal, double move, double damp,
                          (const uint8_t*)c->shell, (const float*)c->rho, (const float*)c->sens, move, damp,
                          c->rho_min, (const double*)c->scal, c->partial);
       hipLaunchKernelGGL(k_oc_decide, dim3(1), dim3(256), 0, c->stream, (const double*)c->partial, kRedBlocks,
-                         (double)c->n_active, 0, vgoal, c->scal);
+                         (double)c->n_active, 0, vgoal, 31, -1.0, c->scal);
     }
     hipLaunchKernelGGL(k_oc_apply, dim3(nblk(v.NE)), dim3(256), 0, c->stream, v, (const uint8_t*)c->eact,
                        (const uint8_t*)c->shell, c->rho, (const float*)c->sens, move, damp, c->rho_min,
@@ -2949,6 +2950,40 @@ TOS_EXPORT int tos_oc_update(tos_ctx* c, double vgoal, double move, double damp,
     const double* hs = read_scal(*c);
     if (g_out) *g_out = hs[S_OC_G];
     if (steps) *steps = (int)hs[S_OC_STEPS];
+    c->op_ready = false;
+    return TOS_OK;
+  });
+}
+
+TOS_EXPORT int tos_oc_update_projected(tos_ctx* c, double vgoal, double move, double damp, double beta,
+                                      double* g_out, int* steps, double* change) {
+  return guard(c, [&]() {
+    REQUIRE(move > 0 && damp > 0 && beta > 0, "bad OC parameters");
+    const LevelView& v = c->L[0].v;
+    Prof pr(*c, P_OPT);
+    hipLaunchKernelGGL(k_absmax_partial, dim3(kRedBlocks), dim3(256), 0, c->stream, v, (const uint8_t*)c->eact,
+                       (const float*)c->sens, c->partial);
+    hipLaunchKernelGGL(k_oc_init, dim3(1), dim3(256), 0, c->stream, (const double*)c->partial, kRedBlocks, c->scal);
+    for (int it = 0; it < 51; ++it) {
+      hipLaunchKernelGGL(k_oc_trial_proj, dim3(kRedBlocks), dim3(256), 0, c->stream, v, (const uint8_t*)c->eact,
+                         (const uint8_t*)c->shell, (const float*)c->rho_pre, (const float*)c->sens, move, damp,
+                         c->rho_min, beta, (const double*)c->scal, c->partial);
+      hipLaunchKernelGGL(k_oc_decide, dim3(1), dim3(256), 0, c->stream, (const double*)c->partial, kRedBlocks,
+                         (double)c->n_active, 0, vgoal, 51, 1e-3, c->scal);
+    }
+    hipLaunchKernelGGL(k_oc_apply_proj, dim3(kRedBlocks), dim3(256), 0, c->stream, v, (const uint8_t*)c->eact,
+                       (const uint8_t*)c->shell, c->rho, c->rho_pre, (const float*)c->sens, move, damp, c->rho_min,
+                       beta, (const double*)c->scal, c->partial);
+    hipLaunchKernelGGL(k_change_fold, dim3(1), dim3(256), 0, c->stream, (const double*)c->partial, kRedBlocks,
+                       c->scal);
+    LAUNCH_CHECK();
+    const double* hs = read_scal(*c);
+    if (g_out) *g_out = hs[S_OC_G];
+    if (steps) *steps = (int)hs[S_OC_STEPS];
+    if (change) {
+      change[0] = hs[S_TMP0];
+      change[1] = hs[S_TMP1] / (double)c->n_active;
+    }
     c->op_ready = false;
     return TOS_OK;
   });
@@ -3100,7 +3135,7 @@ TOS_EXPORT int tos_oc_update_coeffs(tos_ctx* c, double vgoal, double move, doubl
       hipLaunchKernelGGL(k_sum_active, dim3(kRedBlocks), dim3(256), 0, c->stream, v, (const uint8_t*)c->eact,
                          (const float*)c->rho, c->partial);
       hipLaunchKernelGGL(k_oc_decide, dim3(1), dim3(256), 0, c->stream, (const double*)c->partial, kRedBlocks,
-                         (double)c->n_active, 0, vgoal, c->scal);
+                         (double)c->n_active, 0, vgoal, 31, -1.0, c->scal);
     }
     LAUNCH_CHECK();
     const double* hs = read_scal(*c);
@@ -3138,9 +3173,177 @@ TOS_EXPORT int tos_spline_adjoint(tos_ctx* c, int which) {
 
 TOS_EXPORT int tos_spline_get_coeff_sens(tos_ctx* c, int which, float* out) {
   return guard(c, [&]() {
-    REQUIRE(out && c->sp_mode >= 0 && (which == 0 || which == 1), "bad request");
-    download(c->stream, out, which == 0 ? c->csens : c->cvsens,
-             (size_t)c->nc[0] * c->nc[1] * c->nc[2] * sizeof(float));
+    REQUIRE(out && c->sp_mode >= 0 && which >= 0 && which <= 3, "bad request");
+    const size_t n = (size_t)c->nc[0] * c->nc[1] * c->nc[2];
+    const float* src = which == 0 ? c->csens : which == 1 ? c->cvsens : which == 2 ? c->sscsens : c->dripcsens;
+    if (!src) {  // no surface constraints evaluated yet
+      std::fill(out, out + n, 0.0f);
+      return TOS_OK;
+    }
+    download(c->stream, out, src, n * sizeof(float));
+    return TOS_OK;
+  });
+}
+
+// ---- self-supporting and drip constraints on spline surface points -------------------
+namespace {
+// device description of the spline axes (knot vectors of mode 0 uploaded on first use)
+SfAxes surface_axes(Ctx& c) {
+  const int dims[3] = {c.nx, c.ny, c.nz};
+  SfAxes S{};
+  for (int a = 0; a < 3; ++a) {
+    SfAxis& A = S.a[a];
+    A.mode = c.sp_mode;
+    A.k = c.sp_order;
+    A.nc = c.nc[a];
+    A.sh = c.sp_param * c.h;
+    A.box = dims[a] * c.h;
+    const int P = c.sp_param;
+    const double box = dims[a] * c.h;
+    A.lo = -2.0 * box / (P + 1);
+    A.step = (box + 4.0 * box / (P + 1)) / (P + 1);
+    A.U = nullptr;
+    if (c.sp_mode == 0) {
+      if (!c.sf_knots[a]) {  // clamped knots (Grid::set_spline_knot_series grid/Grid.cpp:3253-3377)
+        const double hi = box + 2.0 * box / (P + 1);
+        std::vector<double> U(c.sp_order, A.lo);
+        double t = A.lo;
+        for (int i = 0; i < P; ++i) { t += (hi - A.lo) / (P + 1); U.push_back(t); }
+        for (int i = 0; i < c.sp_order; ++i) U.push_back(hi);
+        c.sf_knots[a] = dalloc<double>(c, U.size());
+        upload(c.stream, c.sf_knots[a], U.data(), U.size() * sizeof(double));
+      }
+      A.U = c.sf_knots[a];
+    }
+  }
+  return S;
+}
+}  // namespace
+
+TOS_EXPORT int tos_surface_constraints(tos_ctx* c, const tos_surface_opts* o, tos_surface_stats* st) {
+  return guard(c, [&]() {
+    REQUIRE(o && c->sp_mode >= 0 && o->cells >= 1 && o->cells <= 1000 && o->ss_mode >= 0 && o->ss_mode <= 5 &&
+                o->drip_mode >= 0 && o->drip_mode <= 7 && o->beta > 0 && c->sp_order <= 8,
+            "bad surface-constraint options");
+    Ctx& C = *c;
+    const int NB = o->cells + 2;
+    const int64_t nl = (int64_t)NB * NB * NB, cap = 3 * nl;
+    const size_t ncoef = (size_t)C.nc[0] * C.nc[1] * C.nc[2];
+    if (C.sf_cells != o->cells) {  // lattice buffers for this lattice size
+      C.sf_val = dalloc<double>(C, nl);
+      C.sf_cnt = dalloc<int>(C, nl);
+      C.sf_off = dalloc<int>(C, nl + 1);
+      C.sf_pts = dalloc<double>(C, 3 * cap);
+      C.sf_grad = dalloc<double>(C, 3 * cap);
+      C.sf_hess = dalloc<double>(C, 6 * cap);
+      C.sf_nrm = dalloc<double>(C, 3 * cap);
+      C.sf_wss = dalloc<double>(C, 3 * cap);
+      C.sf_wdr = dalloc<double>(C, 6 * cap);
+      C.sf_cells = o->cells;
+    }
+    if (!C.sscsens) {
+      C.sscsens = dalloc<float>(C, ncoef);
+      C.dripcsens = dalloc<float>(C, ncoef);
+    }
+    {  // all coefficients equal: no surface; the driver's constants (utils/test_utils.cpp:1106-1113)
+      std::vector<float> h(ncoef);
+      download(C.stream, h.data(), C.coef, ncoef * sizeof(float));
+      bool equal = true;
+      for (float x : h) equal = equal && x == h[0];
+      if (equal) {
+        hipLaunchKernelGGL(k_zero<float>, dim3(nblk((int64_t)ncoef)), dim3(256), 0, C.stream, C.sscsens,
+                           (int64_t)ncoef);
+        hipLaunchKernelGGL(k_zero<float>, dim3(nblk((int64_t)ncoef)), dim3(256), 0, C.stream, C.dripcsens,
+                           (int64_t)ncoef);
+        LAUNCH_CHECK();
+        HIPCHK(hipStreamSynchronize(C.stream));
+        C.sf_n = 0;
+        C.ss_value = 1.0;
+        C.drip_value = 0.0;
+        C.sf_valid = true;
+        if (st) *st = tos_surface_stats{0, 0, 1.0, 1.0, 0.0};
+        return TOS_OK;
+      }
+    }
+    const SfAxes S = surface_axes(C);
+    const LevelView& v = C.L[0].v;
+    SfLattice G{NB, {v.ex * C.h / o->cells, v.ey * C.h / o->cells, v.ez * C.h / o->cells}};
+    Prof pr(C, P_SPLINE);
+    hipLaunchKernelGGL(k_sf_lattice, dim3(nblk(nl)), dim3(256), 0, C.stream, S, G, (const float*)C.coef, C.sf_val);
+    hipLaunchKernelGGL(k_sf_count, dim3(nblk(nl)), dim3(256), 0, C.stream, G, v, (const uint8_t*)C.eact,
+                       (const double*)C.sf_val, C.h, C.sf_cnt);
+    hipLaunchKernelGGL(k_sf_scan, dim3(1), dim3(1024), 0, C.stream, (const int*)C.sf_cnt, nl, C.sf_off);
+    LAUNCH_CHECK();
+    int total = 0;
+    download(C.stream, &total, C.sf_off + nl, sizeof(int));
+    const int64_t n = total;
+    C.sf_n = n;
+    hipLaunchKernelGGL(k_sf_points, dim3(nblk(nl)), dim3(256), 0, C.stream, G, v, (const uint8_t*)C.eact,
+                       (const double*)C.sf_val, C.h, (const int*)C.sf_off, C.sf_pts);
+    constexpr int kSfBlocks = 512;  // 3 columns of partials fit the context's partial buffer
+    const unsigned nbp = (unsigned)std::max<int64_t>(1, std::min<int64_t>(kSfBlocks, (n + 255) / 256));
+    hipLaunchKernelGGL(k_sf_orient, dim3(nbp), dim3(256), 0, C.stream, S, (const float*)C.coef, n,
+                       (const double*)C.sf_pts, o->beta, C.sf_grad, C.sf_hess, C.partial);
+    hipLaunchKernelGGL(k_sf_fold, dim3(1), dim3(256), 0, C.stream, (const double*)C.partial, (int)nbp, 1, S_TMP3,
+                       C.scal);
+    SfParams P{std::cos(o->print_angle), std::cos(o->drip_angle), o->ss_mode, o->drip_mode, o->drip_version,
+               o->p_norm, o->hfunction_c, o->sigmoid_c, o->drip_beta, o->drip_mu, o->drip_angle, o->drip_alpha,
+               o->drip_beta_scale};
+    hipLaunchKernelGGL(k_sf_values, dim3(nbp), dim3(256), 0, C.stream, P, n, (const double*)C.sf_grad,
+                       (const double*)C.sf_hess, (const double*)C.scal, (int)S_TMP3, C.sf_nrm, C.sf_wss, C.sf_wdr,
+                       C.partial);
+    hipLaunchKernelGGL(k_sf_fold, dim3(1), dim3(256), 0, C.stream, (const double*)C.partial, (int)nbp, 3, S_TMP0,
+                       C.scal);
+    LAUNCH_CHECK();
+    const double* hs = read_scal(C);
+    const double dir = hs[S_TMP3] > 0 ? 1.0 : -1.0;
+    const double ss_sum = hs[S_TMP0], dr_sum = hs[S_TMP1], n_down = hs[S_TMP2];
+    // global values (global_selfsupp_constraint grid/Grid.cu:6101-6145, global_drip_constraint
+    // :6485-6525) and the factors of their gradients
+    const double cnt_ss = o->ss_mode % 2 == 0 ? n_down : (double)n;
+    const double cnt_dr = o->drip_mode % 2 == 0 ? n_down : (double)n;
+    double ss_val = 0, ss_fac = 0, dr_val = 0, dr_fac = 0;
+    if (cnt_ss > 0) {
+      const double mean = ss_sum / cnt_ss;
+      if (o->ss_mode <= 1) {
+        ss_val = std::pow(mean, 1.0 / o->p_norm) - 1;
+        ss_fac = mean > 0 ? std::pow(mean, 1.0 / o->p_norm - 1) / o->p_norm / cnt_ss : 0.0;
+      } else {
+        ss_val = mean;
+        ss_fac = 1.0 / cnt_ss;
+      }
+    }
+    if (cnt_dr > 0) {
+      dr_val = o->drip_mode <= 1 ? -1.0 : dr_sum / cnt_dr;
+      dr_fac = o->drip_mode >= 6 ? 1.0 / cnt_dr : 0.0;
+    }
+    hipLaunchKernelGGL(k_sf_grad, dim3(nblk((int64_t)ncoef)), dim3(256), 0, C.stream, S, G, (int64_t)ncoef,
+                       (const int*)C.sf_off, (const double*)C.sf_pts, (const double*)C.sf_wss,
+                       (const double*)C.sf_wdr, ss_fac, dr_fac, C.sscsens, C.dripcsens);
+    LAUNCH_CHECK();
+    HIPCHK(hipStreamSynchronize(C.stream));
+    C.ss_value = ss_val;
+    C.drip_value = dr_val;
+    C.sf_valid = true;
+    if (st) {
+      st->n_points = n;
+      st->n_down = (int64_t)n_down;
+      st->direction = dir;
+      st->ss_value = ss_val;
+      st->drip_value = dr_val;
+    }
+    return TOS_OK;
+  });
+}
+
+TOS_EXPORT int tos_surface_points(tos_ctx* c, int64_t cap, double* xyz, double* normals) {
+  return guard(c, [&]() {
+    REQUIRE(cap >= 0 && (cap == 0 || (xyz && normals)), "bad arguments");
+    const int64_t n = std::min<int64_t>(cap, c->sf_valid ? c->sf_n : 0);
+    if (n > 0) {
+      download(c->stream, xyz, c->sf_pts, 3 * n * sizeof(double));
+      download(c->stream, normals, c->sf_nrm, 3 * n * sizeof(double));
+    }
     return TOS_OK;
   });
 }
@@ -3287,6 +3490,34 @@ TOS_EXPORT int tos_ctx_mma_step(tos_ctx* c, tos_mma* M, double sscale, double vs
     {
       Prof pr(*c, P_OPT);
       cv = mma_update_dev(M, c->stream, s, s + n, &g);
+    }
+    if (converged) *converged = cv;
+    hipLaunchKernelGGL((k_copy_cast<double, float>), dim3(nblk(n)), dim3(256), 0, c->stream, (const double*)M->V.x,
+                       c->coef, (int64_t)n);
+    LAUNCH_CHECK();
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return TOS_OK;
+  });
+}
+
+TOS_EXPORT int tos_ctx_mma_step_surface(tos_ctx* c, tos_mma* M, double sscale, double vscale, double vfrac,
+                                       double ss_scale, double drip_scale, int* converged) {
+  return guard(c, [&]() {
+    REQUIRE(M && c->sp_mode >= 0 && M->V.m == 3 && (int64_t)M->V.n == (int64_t)c->nc[0] * c->nc[1] * c->nc[2],
+            "MMA size does not match the spline coefficients and three constraints");
+    if (!c->sf_valid) { c->err = "tos_surface_constraints first"; return TOS_E_STATE; }
+    const double vol = volume_of(*c);
+    const int n = M->V.n;
+    double* s = M->stage;
+    const double g[3] = {vscale * (vol - vfrac), ss_scale * (c->ss_value - 1.0), drip_scale * c->drip_value};
+    hipLaunchKernelGGL(k_sf_mma_inputs, dim3(nblk(n)), dim3(256), 0, c->stream, (int64_t)n, (const float*)c->csens,
+                       (const float*)c->cvsens, (const float*)c->sscsens, (const float*)c->dripcsens, sscale, vscale,
+                       ss_scale, drip_scale, s);
+    LAUNCH_CHECK();
+    bool cv;
+    {
+      Prof pr(*c, P_OPT);
+      cv = mma_update_dev(M, c->stream, s, s + n, g);
     }
     if (converged) *converged = cv;
     hipLaunchKernelGGL((k_copy_cast<double, float>), dim3(nblk(n)), dim3(256), 0, c->stream, (const double*)M->V.x,
@@ -3740,7 +3971,8 @@ TOS_EXPORT int tos_oc_update_noalloc(int64_t n, const uint8_t* active, const uin
   for (int it = 0; it < 31; ++it) {
     hipLaunchKernelGGL(k_oc_trial_flat, dim3(nb), dim3(256), 0, st, n, (int64_t)kNoAllocChunk, active, shell, rho,
                        sens, move, damp, rho_min, (const double*)scal, part);
-    hipLaunchKernelGGL(k_oc_decide, dim3(1), dim3(256), 0, st, (const double*)part, nb, 0.0, 1, vgoal, scal);
+    hipLaunchKernelGGL(k_oc_decide, dim3(1), dim3(256), 0, st, (const double*)part, nb, 0.0, 1, vgoal, 31, -1.0,
+                       scal);
   }
   hipLaunchKernelGGL(k_oc_apply_flat, dim3(nblk(n)), dim3(256), 0, st, n, active, shell, rho, sens, rho_out, move,
                      damp, rho_min, (const double*)scal);

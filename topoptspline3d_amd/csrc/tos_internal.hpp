@@ -188,6 +188,20 @@ struct Ctx {
   double* sp_t1 = nullptr;  // separable adjoint stages
   double* sp_t2 = nullptr;
   double* sp_g = nullptr;   // last adjoint stage (per coefficient, FP64)
+  // surface constraints (tos_surface_constraints): background lattice of sf_cells cells per
+  // axis, its node values and point offsets, the points (capacity 3 per lattice node) with
+  // gradient, Hessian, normal and gradient weights, and the coefficient gradients
+  int sf_cells = 0;
+  int64_t sf_n = 0;
+  double* sf_val = nullptr;
+  int *sf_cnt = nullptr, *sf_off = nullptr;
+  double *sf_pts = nullptr, *sf_grad = nullptr, *sf_hess = nullptr, *sf_nrm = nullptr;
+  double *sf_wss = nullptr, *sf_wdr = nullptr;
+  double* sf_knots[3] = {nullptr, nullptr, nullptr};  // clamped knots (spline mode 0)
+  float* sscsens = nullptr;
+  float* dripcsens = nullptr;
+  double ss_value = 1.0, drip_value = 0.0;
+  bool sf_valid = false;
   std::vector<void*> allocs;
   // kernel timing (HIP events on `stream`), see tos_profile
   bool prof_on = false;
