@@ -1,0 +1,171 @@
+"""Parity with the FP64 oracle at the sizes of the benchmark configurations, through the
+C ABI:
+
+* the 511x320x320 union of 6 seeded ellipsoids (seed 2024; ~36 % solid, 19.06 M active
+  elements, 7 levels, 3-voxel shell, supports at z = 0, unit load on the top surface
+  along the occupancy normal) with a cubic spline every 4 voxels, coefficients
+  0.4 +- 0.05 (seed 11): one operator application and one FP64 V-cycle against the
+  oracle's, and solves from zero to 1e-6 relative residual at precision 0 (FP64) and
+  precision 2 (FP32 V-cycle on FP16 coarse stencils built in FP32, FP64 refinement)
+  against the oracle's FP64 solve -- true FP64 residual <= 1e-6 (also recomputed with the
+  oracle's operator) and compliance within 1e-5;
+* the 128x64x64 cantilever with a 33x17x17 cubic spline lattice and MMA (move 0.06):
+  three optimisation iterations, compliance within 1e-5 of the oracle's per iteration;
+* the 256x128x128 cantilever with K = 16 patch loads of seeded directions: the batched
+  solve against the one-case-at-a-time solve (1e-5) and, for cases 0 and 15, against the
+  oracle's solve; every case's true FP64 residual <= 1e-6.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle_api import oracle_api
+from topoptspline3d_amd.context import Context
+
+pytestmark = pytest.mark.gpu
+
+E, NU, P, RMIN = 1e6, 0.4, 3.0, 1e-3
+
+
+@pytest.fixture(scope="module")
+def hip():
+    from topoptspline3d_amd.backend import gpu_available, hip_api
+    if not gpu_available():
+        pytest.skip("no GPU")
+    return hip_api()
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def spline_context(api, solid, support, f, shell):
+    c = Context(api, solid, support, shell=shell)
+    c.set_material(E, NU, P, RMIN)
+    c.set_loads(f)
+    n = c.spline_init(1, 4, 4)
+    rng = np.random.default_rng(11)
+    c.set_coeffs((0.4 + rng.uniform(-0.05, 0.05, n)).astype(np.float32))
+    c.spline_eval()
+    return c
+
+
+@pytest.fixture(scope="module")
+def ellipsoids511(hip):
+    from topoptspline3d_amd.domains import ellipsoids
+    os.environ["ORC_PROGRESS"] = "1"  # the oracle's solves print one line per iteration
+    print("\n[c2] building the 511x320x320 ellipsoid domain", flush=True)
+    solid, support, f, shell, meta = ellipsoids(511, 320, 320, seed=2024)
+    o = spline_context(oracle_api(), solid, support, f, shell)
+    g = spline_context(hip, solid, support, f, shell)
+    # the FP32 spline evaluations may differ in the last bit: build both operators from
+    # the oracle's densities so the solver comparison sees one operator
+    assert rel(g.density(), o.density()) < 1e-6
+    g.set_density(o.density())
+    o.update_operator()
+    g.update_operator()
+    yield o, g, f
+    g.close()
+    o.close()
+
+
+def test_c2_operator_and_vcycle_full_size(ellipsoids511):
+    o, g, f = ellipsoids511
+    info = g.info()
+    assert info == o.info()
+    assert info["n_levels"] == 7 and info["n_active_elements"][0] == 19057673
+    rng = np.random.default_rng(3)
+    u = rng.normal(size=3 * o.nn)
+    print("[c2] operator application", flush=True)
+    yo = o.apply(u)
+    e_apply = rel(g.apply(u), yo)
+    print(f"[c2] operator application: relative difference {e_apply:.2e}; FP64 V-cycle", flush=True)
+    zo = o.vcycle(yo)
+    e_vc = rel(g.vcycle(yo, precision=0), zo)
+    print(f"[c2] FP64 V-cycle: relative difference {e_vc:.2e}", flush=True)
+    assert e_apply < 1e-12
+    assert e_vc < 1e-9
+
+
+def test_c2_solves_match_oracle(ellipsoids511):
+    o, g, f = ellipsoids511
+    print("[c2] oracle FP64 solve from zero to 1e-6", flush=True)
+    so = o.solve(rtol=1e-6, maxit=200, warm_start=False)
+    co = o.compliance()[0]
+    print(f"[c2] oracle: {so['iterations'][0]} iterations, rel_res {so['rel_res'][0]:.3e}, c {co:.10e}",
+          flush=True)
+    nf = np.linalg.norm(f)
+    for prec in (0, 2):
+        g.set_galerkin_precision(0 if prec == 0 else 1)
+        g.update_operator()
+        sg = g.solve(rtol=1e-6, maxit=200, precision=prec, warm_start=False)
+        cg = g.compliance()[0]
+        ug = g.displacement()
+        # the GPU's own true residual, and the same recomputed with the oracle's operator
+        r_orc = np.linalg.norm(f - o.apply(ug)) / nf
+        print(f"[c2] precision {prec}: {sg['iterations'][0]} iterations, rel_res {sg['rel_res'][0]:.3e} "
+              f"(oracle operator {r_orc:.3e}), c {cg:.10e}, |dc|/c {abs(cg - co) / abs(co):.2e}", flush=True)
+        assert sg["rel_res"][0] <= 1e-6
+        assert r_orc <= 1e-6 * (1 + 1e-6)
+        assert abs(cg - co) <= 1e-5 * abs(co)
+        assert abs(sg["iterations"][0] - so["iterations"][0]) <= 3
+
+
+def test_c1_spline_mma_iterations_match_oracle(hip):
+    from topoptspline3d_amd.drivers import run_spline_mma
+    print("\n[c1] oracle: 3 spline + MMA iterations at 128x64x64", flush=True)
+    ro = run_spline_mma(oracle_api(), 128, 64, 64, iters=3, rtol=1e-6, precision=0)
+    co = np.array(ro["c"])
+    assert ro["ctx"].n_coeffs == 33 * 17 * 17
+    for prec in (0, 2):
+        rg = run_spline_mma(hip, 128, 64, 64, iters=3, rtol=1e-6, precision=prec)
+        cg = np.array(rg["c"])
+        print(f"[c1] precision {prec}: c {cg} vs {co}, its {rg['its']} vs {ro['its']}", flush=True)
+        assert max(rg["rel_res"]) <= 1e-6
+        assert np.all(np.abs(cg - co) <= 1e-5 * np.abs(co)), (prec, cg, co)
+        assert np.abs(np.array(rg["v"]) - np.array(ro["v"])).max() <= 1e-6
+
+
+def test_c3_batched_k16_full_size(hip):
+    from topoptspline3d_amd.domains import batched_patch_loads, cantilever
+    nx, ny, nz, K = 256, 128, 128, 16
+    solid, support, _ = cantilever(nx, ny, nz)
+    F, _ = batched_patch_loads(nx, ny, nz, K=K, patch=8, seed=7)
+    rho = np.random.default_rng(1).uniform(0.2, 1.0, solid.size).astype(np.float32)
+    g = Context(hip, solid, support)
+    g.set_material(E, NU, P, RMIN)
+    g.set_loads(F)
+    g.set_density(rho)
+    g.set_galerkin_precision(1)
+    g.update_operator()
+    print("\n[c3] batched K=16 solve at 256x128x128", flush=True)
+    sb = g.solve_batched(rtol=1e-6, maxit=300)
+    ub = [g.displacement(k) for k in range(K)]
+    cb = g.compliance()
+    ss = g.solve(rtol=1e-6, maxit=300, precision=1, warm_start=False)
+    us = [g.displacement(k) for k in range(K)]
+    cs = g.compliance()
+    du = max(rel(ub[k], us[k]) for k in range(K))
+    dc = max(abs(cb[k] - cs[k]) / abs(cs[k]) for k in range(K))
+    print(f"[c3] iterations batched {sb['iterations']} one at a time {ss['iterations']}; "
+          f"max |du|/|u| {du:.2e}, max |dc|/c {dc:.2e}", flush=True)
+    for k in range(K):
+        assert sb["rel_res"][k] <= 1e-6 and ss["rel_res"][k] <= 1e-6
+    assert dc <= 1e-5 and du <= 1e-4
+    # cases 0 and 15 against the oracle (a two-case context of the same loads)
+    o = Context(oracle_api(), solid, support)
+    o.set_material(E, NU, P, RMIN)
+    o.set_loads(np.stack([F[0], F[K - 1]]))
+    o.set_density(rho)
+    o.update_operator()
+    so = o.solve(rtol=1e-6, maxit=300, warm_start=False)
+    co = o.compliance()
+    for j, k in enumerate((0, K - 1)):
+        assert so["rel_res"][j] <= 1e-6
+        assert abs(cb[k] - co[j]) <= 1e-5 * abs(co[j]), (k, cb[k], co[j])
+        # the batched solve's true residual, recomputed with the oracle's operator
+        assert np.linalg.norm(F[k] - o.apply(ub[k])) <= 1e-6 * (1 + 1e-6) * np.linalg.norm(F[k])
+        assert rel(ub[k], o.displacement(j)) < 1e-3
+    g.close()
+    o.close()

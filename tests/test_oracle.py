@@ -413,3 +413,213 @@ def test_singular_and_nan_inputs_are_reported():
     bad[0] = np.inf
     with pytest.raises(Exception):
         c.set_density(bad)
+
+
+def _mma_state(m):
+    import ctypes as C
+    fn = oracle_api()._lib.orc_mma_state
+    out = [np.zeros(m.n) for _ in range(4)]
+    ptr = [o.ctypes.data_as(C.POINTER(C.c_double)) for o in out]
+    assert fn(m.h, *ptr) == 0
+    return out  # L, U, alpha, beta
+
+
+def test_mma_step_asymptotes_and_subproblem():
+    """One MMA update at a time (mma_t.cpp:18-51), three in a row: (1) the asymptotes and
+    move bounds after each update follow mma_t.cu:164-226 from the previous asymptotes and
+    the last two steps (gamma 1.2 / 0.7 / 1 by the sign of dx_k dx_(k-1), clamps at
+    0.01 and 10 box widths, alpha/beta a tenth of the way to the asymptotes and within the
+    move limit); (2) the returned x solves the convex subproblem of that update --
+    p, q, b from the pre-update asymptotes (mma_t.cu:15-118), a0 = 1, a = 0, c = 10,
+    d = 1 (mma_t.h:206-207) -- solved independently through its dual (Svanberg 1987,
+    closed-form primal per variable, L-BFGS-B on the multipliers).  A wrong gamma,
+    clamp, p/q weight or subproblem solve fails one of them."""
+    from scipy.optimize import minimize
+    n, m_, move = 6, 2, 0.5
+    rng = np.random.default_rng(4)
+    t = rng.uniform(0.1, 0.9, n)
+    sw = rng.uniform(0.5, 1.5, n)
+    mma = MMA(oracle_api(), n, m_, 0.0, 1.0, move=move)
+    mma.set_x(rng.uniform(0.3, 0.7, n))
+
+    def funcs(x):
+        df = 2 * (x - t) + np.r_[x[1:], 0] + np.r_[0, x[:-1]]
+        g = np.array([x.mean() - 0.4, (sw * x * x).sum() - 0.5])
+        dg = np.stack([np.full(n, 1.0 / n), 2 * sw * x])
+        return df, g, dg
+
+    x_prev = None
+    for step in range(3):
+        x = mma.x()
+        L, U, al, be = _mma_state(mma)
+        df, g, dg = funcs(x)
+        # subproblem of this update
+        r = 1e-5 / 1.0
+        D = np.vstack([df, dg])
+        dp, dm = np.maximum(D, 0), np.maximum(-D, 0)
+        p = (U - x) ** 2 * (1.001 * dp + 0.001 * dm + r)
+        q = (x - L) ** 2 * (0.001 * dp + 1.001 * dm + r)
+        b = (p[1:] / (U - x) + q[1:] / (x - L)).sum(1) - g
+        # Svanberg's dual: for multipliers lam >= 0 each x_j minimises
+        # P_j/(U_j - x) + Q_j/(x - L_j) on [alpha_j, beta_j] in closed form, y_i =
+        # max(0, (lam_i - c_i)/d_i), z = 0 (a = 0); maximise the concave dual W(lam)
+        def primal(lam):
+            P, Q = p[0] + lam @ p[1:], q[0] + lam @ q[1:]
+            sp, sq = np.sqrt(P), np.sqrt(Q)
+            xv = np.clip((sp * L + sq * U) / (sp + sq), al, be)
+            return xv, P, Q, np.maximum(0.0, lam - 10.0)
+
+        def negW(lam):
+            xv, P, Q, y = primal(lam)
+            h = (p[1:] / (U - xv) + q[1:] / (xv - L)).sum(1) - b
+            W = (P / (U - xv) + Q / (xv - L)).sum() - lam @ b + (10 * y + 0.5 * y * y - lam * y).sum()
+            return -W, -(h - y)
+
+        sol = minimize(negW, np.ones(m_), jac=True, method="L-BFGS-B", bounds=[(0, None)] * m_,
+                       options={"ftol": 1e-15, "gtol": 1e-12, "maxiter": 1000})
+        x_dual = primal(sol.x)[0]
+        mma.update(df, dg, g)
+        x_new = mma.x()
+        assert np.allclose(x_new, x_dual, atol=2e-4), (x_new, x_dual)
+        # asymptotes and bounds after the update
+        cdx = x_new - x
+        last = x - x_prev if x_prev is not None else np.zeros(n)
+        gam = np.where(cdx * last > 0, 1.2, np.where(cdx * last < 0, 0.7, 1.0))
+        Ln = np.clip(x_new - gam * (x - L), x_new - 10.0, x_new - 0.01)
+        Un = np.clip(x_new + gam * (U - x), x_new + 0.01, x_new + 10.0)
+        an = np.maximum(np.maximum(0.0, Ln + 0.1 * (x_new - Ln)), x_new - move)
+        bn = np.minimum(np.minimum(1.0, Un - 0.1 * (Un - x_new)), x_new + move)
+        L2, U2, al2, be2 = _mma_state(mma)
+        for got, want in ((L2, Ln), (U2, Un), (al2, an), (be2, bn)):
+            assert np.allclose(got, want, rtol=0, atol=1e-12)
+        if step:
+            assert (gam != 1.0).any()  # both gamma branches are exercised after the first step
+        x_prev = x
+
+
+def _coeff_oc_hand_case(api, damp):
+    """4x4x4 solid box, cubic lattice every 4 voxels: 2x2x2 coefficients at the box
+    corners, symmetric, so a uniform element sensitivity -1 gives one common coefficient
+    sensitivity -G.  With uniform coefficients c the density is c everywhere (partition
+    of unity), so a trial threshold t*G gives V(t) = clamp(0.5 t^-damp, 0.5 -+ 0.1)."""
+    solid = np.ones((4, 4, 4), np.uint8)
+    support = np.zeros((5, 5, 5), np.uint8)
+    support[:, :, 0] = 1
+    ctx = Context(api, solid, support)
+    ctx.set_material(E, NU, P, RMIN)
+    n = ctx.spline_init(1, 4, 4)
+    assert n == 8
+    ctx.set_coeffs(np.full(n, 0.5, np.float32))
+    ctx.spline_eval()
+    ctx.set_sensitivity(np.full(64, -1.0, np.float32))
+    ctx.spline_adjoint(0)
+    cs = ctx.coeff_sens(0)
+    assert np.allclose(cs, cs[0], rtol=1e-6, atol=0) and cs[0] < 0
+    g, steps = ctx.oc_update_coeffs(0.55, move=0.1, damp=damp)
+    return ctx, float(np.abs(cs).max()), g, steps
+
+
+@pytest.mark.parametrize("damp", [0.5, 0.3])
+def test_oc_on_coefficients_hand_computed(damp):
+    """updateCoeff (optimization.cu:559-695) on the hand case above.  The bisection over
+    [0, G] keeps t = (lo + hi)/2 and stops at |V - 0.55| <= 1e-4; for damp = 0.5 its trials
+    are t = 1/2 (V clamped to 0.6), 3/4, 7/8, 13/16, 27/32, 53/64, 105/128, 211/256,
+    423/512 with V(423/512) = 0.5 (512/423)^0.5 = 0.550091: 9 trials, the coefficients
+    end at that trial.  The damping exponent sets where the search stops."""
+    ctx, G, g, steps = _coeff_oc_hand_case(oracle_api(), damp)
+    V = lambda t: min(max(0.5 * t ** (-damp), 0.4), 0.6)
+    lo, hi, k = 0.0, 1.0, 0
+    while True:
+        t = 0.5 * (lo + hi)
+        k += 1
+        v = V(t)
+        if v > 0.55:
+            lo = t
+        elif v < 0.55:
+            hi = t
+        if abs(v - 0.55) <= 1e-4 or k >= 31:
+            break
+    if damp == 0.5:
+        assert (k, t) == (9, 423 / 512)
+    assert steps == k
+    assert abs(g - t * G) <= 1e-12 * G
+    c1 = ctx.coeffs()
+    assert np.allclose(c1, np.float32(0.5 * t ** (-damp)), rtol=2e-7, atol=0)
+    assert abs(ctx.volume() - V(t)) <= 1e-6
+
+
+def _heaviside(x, beta, rmin=RMIN):
+    den = np.tanh(beta * 0.5) + np.tanh(beta * 0.5)
+    return np.clip((np.tanh(beta * 0.5) + np.tanh(beta * (x - 0.5))) / den, rmin, 1.0)
+
+
+def _projected_replay(x0, G, move, damp, beta, vgoal):
+    """updateDensities2's bisection (optimization.cu:462-557) on a uniform design x0 with
+    uniform sensitivity -G: V(g) = H(clamp(x0 (G/g)^damp, x0 -+ move))"""
+    V = lambda g: float(_heaviside(np.float32(min(max(x0 * (G / g) ** damp, x0 - move), x0 + move)), beta))
+    lo, hi, itn, k = 0.0, G, 0, 0
+    while True:
+        g = 0.5 * (lo + hi)
+        v = V(g)
+        k += 1
+        if v > vgoal:
+            lo = g
+        elif v < vgoal:
+            hi = g
+        if not (abs(v - vgoal) > 1e-4 and (hi - lo) / (lo + hi) > 1e-3 and itn < 50):
+            return g, k
+        itn += 1
+
+
+def test_projected_oc_hand_computed():
+    """updateDensities2 on a uniform design (0.5, sensitivity -1 everywhere, beta 8, move
+    0.2, damping 0.5, target 0.6): the trials act on the design variable, each projected
+    before the volume; the search ends at |V - 0.6| <= 1e-4 or a relative bracket of 1e-3;
+    the new design is the last trial, the density its projection, and the change outputs
+    are |x_new - x_old|.  A second update starts from that design variable, not from
+    its projection."""
+    solid, support, f = cantilever(8, 4, 4)
+    ctx = Context(oracle_api(), solid, support)
+    ctx.set_material(E, NU, P, RMIN)
+    ctx.set_density(np.full(solid.size, 0.5, np.float32))
+    ctx.project_density(8.0)
+    assert np.allclose(ctx.density(), 0.5, atol=1e-7)
+    ctx.set_sensitivity(np.full(solid.size, -1.0, np.float32))
+    g, steps, chmax, chmean = ctx.oc_update_projected(0.6, move=0.2, damp=0.5, beta=8.0)
+    g_ref, k_ref = _projected_replay(0.5, 1.0, 0.2, 0.5, 8.0, 0.6)
+    assert steps == k_ref and abs(g - g_ref) <= 1e-15
+    x_new = np.float32(min(max(0.5 * g_ref ** -0.5, 0.3), 0.7))
+    assert abs(chmax - (x_new - 0.5)) <= 1e-7 and abs(chmean - (x_new - 0.5)) <= 1e-7
+    assert np.allclose(ctx.density(), _heaviside(x_new, 8.0), rtol=0, atol=2e-7)
+    # (the relative-bracket rule ended this search 7e-4 from the target)
+    assert abs(ctx.volume() - float(_heaviside(x_new, 8.0))) <= 1e-6
+    # second update: from x_new (the design variable), with beta 16
+    g2, steps2, ch2, _ = ctx.oc_update_projected(0.62, move=0.2, damp=0.5, beta=16.0)
+    g2_ref, k2_ref = _projected_replay(float(x_new), 1.0, 0.2, 0.5, 16.0, 0.62)
+    assert steps2 == k2_ref and abs(g2 - g2_ref) <= 1e-15
+    x2 = np.float32(min(max(x_new * g2_ref ** -0.5, x_new - 0.2), x_new + 0.2))
+    assert abs(ch2 - abs(x2 - x_new)) <= 1e-7
+
+
+def test_projected_oc_differs_from_projecting_the_plain_update():
+    """The projected update sums projected trials (the plain OC sums unprojected ones):
+    on a non-uniform design the projected update meets the target on the projected
+    density, where OC followed by a projection misses it."""
+    rng = np.random.default_rng(9)
+    solid, support, f = cantilever(12, 6, 6)
+    x0 = rng.uniform(0.2, 0.8, solid.size).astype(np.float32)
+    s = (-rng.uniform(0.1, 1.0, solid.size)).astype(np.float32)
+    a = Context(oracle_api(), solid, support)
+    a.set_material(E, NU, P, RMIN)
+    a.set_density(x0)
+    a.project_density(8.0)
+    a.set_sensitivity(s)
+    a.oc_update_projected(0.45, move=0.05, damp=0.5, beta=8.0)
+    assert abs(a.volume() - 0.45) <= 1e-4
+    b = Context(oracle_api(), solid, support)
+    b.set_material(E, NU, P, RMIN)
+    b.set_density(x0)
+    b.set_sensitivity(s)
+    b.oc_update(0.45, move=0.05, damp=0.5)
+    b.project_density(8.0)
+    assert abs(b.volume() - 0.45) > 1e-3
