@@ -5,19 +5,26 @@ irregular 511x320x320 ellipsoid-union domain (~36% solid, z = 0 supported, unit 
 the top surface along the occupancy normal, 3-voxel shell), cubic B-spline density on a
 lattice every 4 voxels, MMA on the coefficients, 7-level geometric MG.  One step = one
 optimisation iteration: spline evaluation, operator update (rho^p, Galerkin levels,
-coarsest factor), flexible PCG to 1e-6 relative residual (FP32 V-cycle reading the
-coarse levels' stencils as FP16 -- precision 2 --, FP64 outer residual), compliance, sensitivity, filter, spline adjoint of the compliance and volume
-sensitivities, MMA step.  All inputs are resident on the GPU before the timed region.
+coarsest factor), flexible PCG to 1e-6 relative residual, compliance, sensitivity,
+filter, spline adjoint of the compliance and volume sensitivities, MMA step.  All inputs
+are resident on the GPU before the timed region.
 
-Before the timed region (c2): the learned level-2 correction is trained on pairs logged
-from 200 optimisation iterations on 128^3 ellipsoid domains.  Timed: `steps` iterations of
-plain MG-PCG, then `steps` iterations with the learned correction attached (`value`), then
-plain FP64 iterations; plus solves from zero on the final operator in FP64, mixed with
-FP32 and with FP16 coarse stencils, and learned mode, and the worst-case load search
-(modified power method) over the workload's load region on the final operator.
+Three runs start from the same initial design and cover the same iterations (`warmup`
+untimed, then `steps` timed):
+  value    -- plain mixed-precision MG-PCG (FP32 V-cycle reading the coarse levels'
+              stencils as FP16 -- precision 2 --, FP64 outer residual);
+  fp64     -- the same optimisation with FP64 V-cycles (plain FP64 GPU multigrid);
+  learned  -- as `value` with the learned level-2 correction attached (trained before
+              the timed region on pairs logged from 200 optimisation iterations on 128^3
+              ellipsoid domains).
+Then, on the final operator of the first run: solves from zero in FP64, mixed FP32 and
+FP16 stencils and learned mode; the worst-case load search (modified power method); the
+16-load-case batched workload; and the CPU baseline, one complete optimisation iteration
+of the FP64 oracle on the host cores (including the MMA step).
 
 Prints one JSON line.  Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 [--workload c2|c1|c0|fertility] [--no-cpu-baseline] [--no-learned] [--no-fp64-compare]
+[--learn-only]
 (fertility: the reference's bench mesh fertility/config1, kept under tests/golden,
 voxelised at resolution 511 with its JSON supports and loads, element-density OC)
 """
@@ -102,17 +109,28 @@ class Loop:
         self.its, self.res, self.c = [], [], []
         self.sens_scale = None
 
-    def step(self):
+    def step(self, phases=None):
+        """one optimisation iteration; `phases` (a dict) receives seconds per phase"""
         ctx, wl = self.ctx, self.wl
+        t = [time.perf_counter()]
+
+        def mark(name):
+            t.append(time.perf_counter())
+            if phases is not None:
+                phases[name] = phases.get(name, 0.0) + t[-1] - t[-2]
+
         if wl["spline"]:
             ctx.spline_eval()
+            mark("spline eval")
         # FP32 cycles: coarse operators built into FP32 storage; FP64 cycles: FP64
         ctx.set_galerkin_precision(1 if self.precision else 0)
         ctx.update_operator()
+        mark("operator update")
         st = ctx.solve(rtol=wl["rtol"], maxit=300, precision=self.precision)
         self.its.append(st["iterations"][0])
         self.res.append(st["rel_res"][0])
         self.c.append(ctx.compliance()[0])
+        mark("solve")
         ctx.sensitivity()
         ctx.filter(2.0, 0)
         if wl["spline"]:
@@ -120,11 +138,15 @@ class Loop:
             ctx.filter(2.0, 1)
             ctx.spline_adjoint(0)
             ctx.spline_adjoint(1)
+            mark("sensitivity/filter/adjoint")
             if self.sens_scale is None:  # objective normalised by its first value
                 self.sens_scale = 1.0 / self.c[-1]
             self.mma.step_on(ctx, self.sens_scale, 1e3, 0.4)
+            mark("MMA")
         else:
+            mark("sensitivity/filter")
             ctx.oc_update(0.4, wl["move"], 0.5)
+            mark("OC")
 
 
 def sync():
@@ -202,46 +224,42 @@ def roofline(stats, info, n_active_nodes, steps, workload_tag=None, precision=1,
 
 
 def cpu_baseline(wl, gpu_its):
-    """FP64 oracle on the host cores: a bounded sample of the same workload -- one
-    operator update, one flexible-PCG iteration (operator apply + V-cycle) and one
-    sensitivity + filter + adjoint pass -- scaled to one optimisation iteration with the
-    GPU's mean PCG iteration count."""
+    """FP64 oracle on the host cores: one complete optimisation iteration of the same
+    workload from the same initial design (spline evaluation, operator update, FCG solve
+    from zero to the workload's tolerance with FP64 V-cycles, compliance, sensitivity,
+    filter, spline adjoint and MMA step -- or OC for element-density workloads), timed
+    per phase."""
     import ctypes
     from topoptspline3d_amd import _abi
-    from topoptspline3d_amd.context import Context
     lib = os.path.join(ROOT, "oracle", "liboracle.so")
     if not os.path.exists(lib):
         import subprocess
         subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
     api = _abi.bind(ctypes.CDLL(lib), "orc_")
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    ctx = Context(api, wl["solid"], wl["support"], shell=wl["shell"])
-    ctx.set_material(1e6, 0.4, 3.0, 1e-3)
-    ctx.set_loads(wl["f"])
-    if wl["spline"]:
-        mode, order, spacing = wl["spline"]
-        n = ctx.spline_init(mode, order, spacing)
-        ctx.set_coeffs(np.full(n, 0.4, np.float32))
-        t = time.perf_counter(); ctx.spline_eval(); t_eval = time.perf_counter() - t
-    else:
-        ctx.set_density(np.where(wl["solid"].ravel() > 0, 0.4, 0).astype(np.float32))
-        t_eval = 0.0
-    t = time.perf_counter(); ctx.update_operator(); t_op = time.perf_counter() - t
-    r = ctx.get_loads(0)
-    t = time.perf_counter(); z = ctx.vcycle(r); t_vc = time.perf_counter() - t
-    t = time.perf_counter(); ctx.apply(z); t_ap = time.perf_counter() - t
-    t = time.perf_counter()
-    ctx.sensitivity(); ctx.filter(2.0, 0)
-    if wl["spline"]:
-        ctx.init_vol_sensitivity(); ctx.filter(2.0, 1); ctx.spline_adjoint(0); ctx.spline_adjoint(1)
-    t_post = time.perf_counter() - t
-    it = float(np.mean(gpu_its))
-    est = t_eval + t_op + it * (t_vc + t_ap) + t_post
-    ctx.close()
-    return {"value": round(est * 1e3, 1), "unit": "ms/TO-iter", "cores": cores, "kind": "port",
-            "sample": (f"FP64 oracle on the same domain: spline eval {t_eval:.2f}s + operator update {t_op:.2f}s"
-                       f" + {it:.1f} x (V-cycle {t_vc:.2f}s + operator {t_ap:.2f}s, the GPU's mean PCG"
-                       f" iterations) + sensitivity/filter/adjoint {t_post:.2f}s; MMA step not included")}
+    t0 = time.perf_counter()
+    loop = Loop(api, wl, 0)
+    t_setup = time.perf_counter() - t0
+    ph = {}
+    t_iter = time.perf_counter()
+    loop.step(phases=ph)
+    t_iter = time.perf_counter() - t_iter
+    its = loop.its[-1]
+    loop.ctx.close()
+    cpu = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    parts = ", ".join(f"{k} {v:.2f}s" for k, v in ph.items())
+    return {"value": round(t_iter * 1e3, 1), "unit": "ms/TO-iter", "cores": cores, "kind": "port",
+            "sample": (f"one complete optimisation iteration of the FP64 oracle on the same domain from the same "
+                       f"initial design ({parts}; FCG from zero: {its} iterations, the GPU's timed iterations "
+                       f"took {min(gpu_its)}-{max(gpu_its)} from their warm starts); context set-up "
+                       f"{t_setup:.1f}s not included; CPU: {cpu or 'unknown'}")}
 
 
 def learned_phase(api, n_dom, iters, steps, level=2):
@@ -277,9 +295,14 @@ def learned_phase(api, n_dom, iters, steps, level=2):
     lf, ll = net.train(ds, 0, ntr, steps=steps, batch=8, lr=1e-3, seed=3)
     t_train = time.perf_counter() - t
     after = net.evaluate(ds, ntr, n - ntr) if n > ntr else None
+    # training throughput against the BF16 dense MFMA peak (forward + data and weight
+    # gradients ~ 3 forward passes per sample)
+    train_flops = 3.0 * LN.unet_flops(ds.box) * steps * 8
     info = {"pairs": n, "box": list(ds.box), "domains": n_dom, "to_iterations": n_dom * iters,
             "mean_pcg_its": round(float(np.mean(its)), 2), "generate_s": round(t_gen, 1),
-            "train_steps": steps, "batch": 8, "train_s": round(t_train, 1),
+            "train_steps": steps, "batch": 8, "train_s": round(t_train, 2),
+            "train_tflops": round(train_flops / t_train / 1e12, 2),
+            "train_frac_of_bf16_peak": round(train_flops / t_train / 2.5e15, 4),
             "train_loss_first_last": [round(lf, 4), round(ll, 4)],
             "heldout_loss_untrained_trained": [round(before, 4) if before is not None else None,
                                                round(after, 4) if after is not None else None]}
@@ -358,7 +381,7 @@ def worst_phase(ctx, wl, precision):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--precision", type=int, default=2,
@@ -379,6 +402,8 @@ def main():
                          "3 in-wave pairs (FP32 cycles)")
     ap.add_argument("--no-graph-replay", action="store_true",
                     help="launch the FCG iteration kernel by kernel (for launch-tracing tools)")
+    ap.add_argument("--learn-only", action="store_true",
+                    help="only generate the pairs and train the learned correction (network profiling)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run a device-to-device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
@@ -398,39 +423,47 @@ def main():
     from topoptspline3d_amd.backend import hip_api
     from topoptspline3d_amd import learned as LN
     net, learn_info = None, None
-    if not args.no_learned and args.workload == "c2":
+    if (not args.no_learned or args.learn_only) and args.workload == "c2":
         net, ds, learn_info = learned_phase(hip_api(), args.learn_domains, args.learn_iters, args.train_steps)
         ds.close()
+        if args.learn_only:
+            print(json.dumps({"learned": learn_info}), flush=True)
+            return
     t0 = time.perf_counter()
     wl = make_workload(args.workload)
     log(f"[bench] workload {wl['name']} generated in {time.perf_counter() - t0:.1f}s {wl['meta'].get('fill', '')}")
     api = hip_api()
-    loop = Loop(api, wl, args.precision)
-    loop.ctx.set_sweep_mode(args.sweep_mode)
-    loop.ctx.set_graph_replay(not args.no_graph_replay)
-    info = loop.ctx.info()
-    nv = active_nodes(wl["solid"])
-    log(f"[bench] levels {info['n_levels']} dims {info['level_dims']} coarse dofs {info['coarse_dofs']}")
-    for i in range(args.warmup):
-        t = time.perf_counter(); loop.step()
-        log(f"[bench] warmup {i}: {1e3 * (time.perf_counter() - t):.1f} ms, pcg its {loop.its[-1]}")
 
-    def timed(tag, steps):
+    def run(tag, precision, use_net, steps, keep=False):
+        """warmup + steps iterations from the initial design; returns (loop, ms/iter, its, max res)"""
+        loop = Loop(api, wl, precision)
+        loop.ctx.set_sweep_mode(args.sweep_mode)
+        loop.ctx.set_graph_replay(not args.no_graph_replay)
+        if use_net:
+            LN.attach(loop.ctx, net, level=2)
+        for i in range(args.warmup):
+            t = time.perf_counter(); loop.step()
+            log(f"[bench] {tag} warmup {i}: {1e3 * (time.perf_counter() - t):.1f} ms, pcg its {loop.its[-1]}")
         sync()
-        n0 = len(loop.its)
         t = time.perf_counter()
         for i in range(steps):
             loop.step()
             log(f"[bench] {tag} step {i}: pcg its {loop.its[-1]} rel_res {loop.res[-1]:.2e} c {loop.c[-1]:.6e}")
         sync()
-        return (time.perf_counter() - t) * 1e3 / steps, loop.its[n0:], max(loop.res[n0:])
+        ms = (time.perf_counter() - t) * 1e3 / max(steps, 1)
+        n0 = args.warmup
+        out = (loop, ms, loop.its[n0:], max(loop.res[n0:]) if steps else None)
+        if not keep:
+            if use_net:
+                LN.attach(loop.ctx, None)
+            loop.ctx.close()
+        return out
 
-    # plain MG-PCG at the chosen precision (no learned correction)
-    ms_plain, its_plain, res_plain = timed("plain", args.steps)
-    learned = None
-    if net is not None:
-        LN.attach(loop.ctx, net, level=2)
-    ms, its, res_max = timed("learned" if net is not None else "plain", args.steps)
+    # headline: plain mixed-precision MG-PCG
+    loop, ms, its, res_max = run("plain", args.precision, False, args.steps, keep=True)
+    info = loop.ctx.info()
+    nv = active_nodes(wl["solid"])
+    log(f"[bench] levels {info['n_levels']} dims {info['level_dims']} coarse dofs {info['coarse_dofs']}")
     # per-kernel HIP-event timing in separate, untimed iterations (two events per launch
     # would otherwise add host work to the timed region)
     kernel_stats(api, loop.ctx, True)
@@ -439,12 +472,6 @@ def main():
         log(f"[bench] profiled step {i}: pcg its {loop.its[-1]}")
     sync()
     stats = kernel_stats(api, loop.ctx, False)
-    if net is not None:
-        tried, kept, step = LN.stats(loop.ctx)
-        learned = dict(learn_info, corrections_tried=tried, corrections_kept=kept,
-                       mean_step_kept=round(step, 4), ms_per_iter=round(ms, 3),
-                       pcg_iterations=its)
-        LN.attach(loop.ctx, None)
     line = {
         "metric": METRIC, "value": round(ms, 3), "unit": "ms/TO-iter", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
@@ -454,24 +481,28 @@ def main():
         "config": {"workload": wl["desc"], "grid": list(wl["solid"].shape[::-1]),
                    "active_elements": info["n_active_elements"][0], "active_nodes": nv,
                    "mg_levels": info["n_levels"], "rtol": wl["rtol"], "precision": args.precision,
-                   "solver": ("FCG, " + SOLVERS[args.precision]
-                              + (" + learned level-2 correction" if net is not None else "") + ", FP64 refinement"),
+                   "solver": "FCG, " + SOLVERS[args.precision] + ", FP64 refinement (no learned correction)",
+                   "iterations_timed": f"{args.warmup}..{args.warmup + args.steps - 1} from the initial design",
                    "pcg_iterations": its, "max_rel_res": res_max},
         "roofline": roofline(stats, info, nv, args.profile_steps,
                              "ellipsoids_511x320x320" if wl["name"] == "c2" else None, args.precision,
                              args.sweep_mode),
-        "plain_mixed": {"ms_per_iter": round(ms_plain, 3), "pcg_iterations": its_plain,
-                        "max_rel_res": res_plain},
     }
-    if learned is not None:
-        line["learned"] = learned
     if not args.no_fp64_compare:
-        # plain FP64 MG-PCG, whole iterations, continuing the same trajectory
-        loop.precision = 0
-        ms64, its64, res64 = timed("fp64", max(1, min(2, args.steps)))
-        loop.precision = args.precision
+        # plain FP64 GPU multigrid over the same iterations from the same initial design
+        _, ms64, its64, res64 = run("fp64", 0, False, args.steps)
         line["plain_fp64"] = {"ms_per_iter": round(ms64, 3), "pcg_iterations": its64, "max_rel_res": res64,
                               "speedup_of_value": round(ms64 / ms, 3)}
+    if net is not None:
+        lloop, msl, itsl, resl = run("learned", args.precision, True, args.steps, keep=True)
+        tried, kept, stepk = LN.stats(lloop.ctx)
+        LN.attach(lloop.ctx, None)
+        lloop.ctx.close()
+        line["learned"] = dict(learn_info, corrections_tried=tried, corrections_kept=kept,
+                               mean_step_kept=round(stepk, 4), ms_per_iter=round(msl, 3), pcg_iterations=itsl,
+                               max_rel_res=resl, speedup_over_plain=round(ms / msl, 3),
+                               iterations_saved=int(sum(its) - sum(itsl)))
+    if not args.no_fp64_compare:
         # same operator, solve from zero to the same tolerance: FP64, mixed, learned
         # (an OC step changes the densities themselves: rebuild the operator first)
         if not wl["spline"]:
@@ -486,23 +517,26 @@ def main():
             loop.ctx.set_displacement(np.zeros(3 * loop.ctx.nn))
             st = loop.ctx.solve(rtol=wl["rtol"], maxit=400, precision=prec, warm_start=True)
             if use_net:
+                tried, kept, _ = LN.stats(loop.ctx)
                 LN.attach(loop.ctx, None)
             cmp[tag] = {"solve_ms": round(st["seconds"] * 1e3, 2), "iterations": st["iterations"][0],
                         "rel_res": st["rel_res"][0]}
+            if use_net:
+                cmp[tag].update(corrections_tried=tried, corrections_kept=kept)
         line["solve_from_zero"] = cmp
     if not args.no_worst:
         if not wl["spline"]:
             loop.ctx.update_operator()
         line["worst_case_load"] = worst_phase(loop.ctx, wl, args.precision)
+    loop.ctx.close()
     if args.batched_steps > 0 and args.workload == "c2":
-        loop.ctx.close()
         t = time.perf_counter()
         line["batched_k16"] = batched_phase(api, args.batched_steps)
         log(f"[bench] batched workload took {time.perf_counter() - t:.1f}s")
     if not args.no_cpu_baseline:
         t = time.perf_counter()
         line["cpu_baseline"] = cpu_baseline(wl, its)
-        log(f"[bench] cpu baseline sample took {time.perf_counter() - t:.1f}s")
+        log(f"[bench] cpu baseline took {time.perf_counter() - t:.1f}s")
     print(json.dumps(line), flush=True)
 
 

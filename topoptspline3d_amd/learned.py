@@ -126,6 +126,26 @@ class PairDataset:
         return f, t, m
 
 
+def unet_flops(box, channels=(16, 32, 64)):
+    """Algorithmic FLOPs of one forward pass on a node box (X, Y, Z): 2*27*cin*cout per
+    output voxel of every layer (enc0a/b, enc1a/b, enc2a/b at full, half and quarter
+    resolution, the two decoder blocks on the skip concatenations, and the 3-channel
+    output layer); half a box is ceil(n/2) per axis.  A training step (forward, data and
+    weight gradients) is about three times this."""
+    c0, c1, c2 = channels
+    X, Y, Z = box
+    h = lambda g: tuple((n + 1) // 2 for n in g)
+    V0 = X * Y * Z
+    g1 = h(box)
+    V1 = g1[0] * g1[1] * g1[2]
+    g2 = h(g1)
+    V2 = g2[0] * g2[1] * g2[2]
+    conv = lambda ci, co, V: 2 * 27 * ci * co * V
+    return (conv(4, c0, V0) + conv(c0, c0, V0) + conv(c0, c1, V1) + conv(c1, c1, V1) + conv(c1, c2, V2)
+            + conv(c2, c2, V2) + conv(c2 + c1, c1, V1) + conv(c1, c1, V1) + conv(c1 + c0, c0, V0)
+            + conv(c0, c0, V0) + conv(c0, 3, V0))
+
+
 def level_box(ctx, level):
     """node box (X, Y, Z) of multigrid level `level` of a Context"""
     ex, ey, ez = ctx.info()["level_dims"][level]
