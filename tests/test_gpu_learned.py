@@ -86,9 +86,16 @@ def test_unet_gradient_matches_oracle(hip, shape):
     assert off == g.size
 
 
-def test_learned_correction_trains_and_helps(hip):
+def test_learned_correction_trains_and_is_safe(hip):
     """Log level-2 pairs from the solver on a 64x32x32 cantilever over a few OC
-    iterations, train on 80 %, and compare held-out loss and FCG iterations."""
+    iterations, train on 80 %, and compare held-out loss and FCG iterations.  The trained
+    network learns the level-2 error (held-out loss falls by well over 30 %) and its
+    corrections are kept, but the FCG iteration count does not fall (13 with and without
+    it when measured): after the coarse-grid correction the level-2 error left is small
+    next to the fine-level error that sets the iteration count.  The test therefore checks
+    what the guarded correction promises -- the same solution to the same tolerance at no
+    more than two extra iterations -- and reports the counts; the 511 benchmark measures
+    the learned and plain runs over the same iterations."""
     from topoptspline3d_amd import learned as LN
     nx, ny, nz = 64, 32, 32
     solid, support, f = cantilever(nx, ny, nz)
@@ -132,3 +139,36 @@ def test_learned_correction_trains_and_helps(hip):
     assert abs(c1 - c0) <= 1e-5 * c0
     assert tried > 0 and acc > 0 and step > 0
     assert st1["iterations"][0] <= st0["iterations"][0] + 2
+
+
+def test_zero_and_garbage_networks_fall_back_to_plain_mg(hip):
+    """A network with all-zero parameters proposes e = 0 (e.r = 0: never kept), so the
+    solve is plain MG-PCG: same iteration count and solution.  A network with huge random
+    parameters proposes garbage; the guard keeps only energy-reducing steps in the
+    accepted window, and flexible CG still converges to the same solution."""
+    from topoptspline3d_amd import learned as LN
+    solid, support, f = cantilever(32, 16, 16)
+    ctx = Context(hip, solid, support)
+    ctx.set_material(1e6, 0.4, 3.0, 1e-3)
+    ctx.set_loads(f)
+    rng = np.random.default_rng(4)
+    ctx.set_density(rng.uniform(0.05, 1.0, solid.size).astype(np.float32))
+    ctx.update_operator()
+    st0 = ctx.solve(rtol=1e-6, maxit=200, precision=1, warm_start=False)
+    u0, c0 = ctx.displacement(), ctx.compliance()[0]
+    net = LN.UNet(hip, seed=5)
+    cases = {"zero": np.zeros(net.n_params, np.float32),
+             "garbage": rng.normal(0, 100, net.n_params).astype(np.float32)}
+    for tag, theta in cases.items():
+        net.set_params(theta)
+        LN.attach(ctx, net, level=2)
+        st = ctx.solve(rtol=1e-6, maxit=200, precision=1, warm_start=False)
+        tried, kept, _ = LN.stats(ctx)
+        LN.attach(ctx, None)
+        c = ctx.compliance()[0]
+        print(f"{tag}: iterations {st['iterations'][0]} (plain {st0['iterations'][0]}), tried {tried} kept {kept}")
+        assert st["rel_res"][0] <= 1e-6 and abs(c - c0) <= 1e-5 * c0
+        if tag == "zero":
+            assert kept == 0 and tried > 0
+            assert st["iterations"][0] == st0["iterations"][0]
+            assert np.array_equal(ctx.displacement(), u0)
