@@ -71,7 +71,11 @@ typedef struct tos_solve_opts {
   int precision;            /* 0: FP64 V-cycle; 1: FP32 V-cycle inside FP64 PCG;
                                2: as 1 with the stored stencils of the coarse levels read
                                as FP16 (each node's 243 entries scaled by their largest
-                               magnitude); no counterpart in the reference            */
+                               magnitude); 3 (tos_solve only, use_pcg = 1): as 2 with
+                               the FCG residual and K p stored in FP32 (FP64 operator
+                               arithmetic, FP64 u), restarted from the FP64 residual
+                               if that misses rtol (no learned correction); no
+                               counterpart in the reference                           */
   int pre, post;            /* Gauss-Seidel sweeps per level (reference: 1, 1)          */
   int warm_start;           /* 1: start from the current displacement                   */
   int use_pcg;              /* 1: flexible PCG (default); 0: plain V-cycle iteration as

@@ -6,8 +6,8 @@ C ABI:
   along the occupancy normal) with a cubic spline every 4 voxels, coefficients
   0.4 +- 0.05 (seed 11): one operator application and one FP64 V-cycle against the
   oracle's, and solves from zero to 1e-6 relative residual at precision 0 (FP64) and
-  precision 2 (FP32 V-cycle on FP16 coarse stencils built in FP32, FP64 refinement)
-  against the oracle's FP64 solve -- true FP64 residual <= 1e-6 (also recomputed with the
+  precision 2 (FP32 V-cycle on FP16 coarse stencils built in FP32, FP64 refinement) and
+  precision 3 (as 2 with the FCG residual and K p stored in FP32) against the oracle's FP64 solve -- true FP64 residual <= 1e-6 (also recomputed with the
   oracle's operator) and compliance within 1e-5;
 * the 128x64x64 cantilever with a 33x17x17 cubic spline lattice and MMA (move 0.06):
   three optimisation iterations, compliance within 1e-5 of the oracle's per iteration;
@@ -96,7 +96,7 @@ def test_c2_solves_match_oracle(ellipsoids511):
     print(f"[c2] oracle: {so['iterations'][0]} iterations, rel_res {so['rel_res'][0]:.3e}, c {co:.10e}",
           flush=True)
     nf = np.linalg.norm(f)
-    for prec in (0, 2):
+    for prec in (0, 2, 3):
         g.set_galerkin_precision(0 if prec == 0 else 1)
         g.update_operator()
         sg = g.solve(rtol=1e-6, maxit=200, precision=prec, warm_start=False)
@@ -118,7 +118,7 @@ def test_c1_spline_mma_iterations_match_oracle(hip):
     ro = run_spline_mma(oracle_api(), 128, 64, 64, iters=3, rtol=1e-6, precision=0)
     co = np.array(ro["c"])
     assert ro["ctx"].n_coeffs == 33 * 17 * 17
-    for prec in (0, 2):
+    for prec in (0, 2, 3):
         rg = run_spline_mma(hip, 128, 64, 64, iters=3, rtol=1e-6, precision=prec)
         cg = np.array(rg["c"])
         print(f"[c1] precision {prec}: c {cg} vs {co}, its {rg['its']} vs {ro['its']}", flush=True)

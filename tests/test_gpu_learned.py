@@ -172,3 +172,23 @@ def test_zero_and_garbage_networks_fall_back_to_plain_mg(hip):
             assert kept == 0 and tried > 0
             assert st["iterations"][0] == st0["iterations"][0]
             assert np.array_equal(ctx.displacement(), u0)
+
+
+def test_precision_3_refuses_an_attached_network(hip):
+    """The FP32-residual solve (precision 3) has no learned-correction path: with a
+    network attached it returns TOS_E_ARG; after detaching it solves."""
+    from topoptspline3d_amd import learned as LN
+    from topoptspline3d_amd.context import TosError
+    solid, support, f = cantilever(32, 16, 16)
+    ctx = Context(hip, solid, support)
+    ctx.set_material(1e6, 0.4, 3.0, 1e-3)
+    ctx.set_loads(f)
+    ctx.set_density(np.full(solid.size, 0.4, np.float32))
+    ctx.update_operator()
+    net = LN.UNet(hip, seed=5)
+    LN.attach(ctx, net, level=2)
+    with pytest.raises(TosError, match="precision 3"):
+        ctx.solve(rtol=1e-6, maxit=200, precision=3)
+    LN.attach(ctx, None)
+    st = ctx.solve(rtol=1e-6, maxit=200, precision=3)
+    assert st["rel_res"][0] <= 1e-6

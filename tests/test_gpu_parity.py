@@ -83,7 +83,7 @@ def test_operator_stencils_vcycle(hip, dims, seed, void):
     assert rel(g.vcycle(r, precision=2), zo) < 1e-2
 
 
-@pytest.mark.parametrize("precision", [0, 1, 2])
+@pytest.mark.parametrize("precision", [0, 1, 2, 3])
 def test_solve_c0(hip, precision):
     solid, support, f = cantilever(24, 12, 12)
     rho = np.full(solid.size, 0.4, np.float32)
@@ -192,7 +192,7 @@ def test_c0_trajectory_matches_oracle(hip):
     print("max relative compliance difference (FP64):", np.max(np.abs(cg - co) / np.abs(co)))
     assert np.all(np.abs(cg - co) <= 1e-5 * np.abs(co)), (co, cg)
     assert np.abs(np.array(rg["v"]) - np.array(ro["v"])).max() <= 1e-6
-    for prec in (1, 2):
+    for prec in (1, 2, 3):
         rm = run_oc(hip, 24, 12, 12, iters=50, rtol=1e-8, precision=prec)
         print(f"max relative compliance difference (precision {prec}):",
               np.max(np.abs(np.array(rm["c"]) - co) / np.abs(co)))

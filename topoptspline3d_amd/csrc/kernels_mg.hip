@@ -154,8 +154,9 @@ __device__ __forceinline__ void fine_row(const T (&wa)[8], const T* __restrict__
   }
 }
 
-// MODE 0: y = K u, optional partial sums of u.y;  MODE 1: y = f - K u, optional float
-// copy y32, optional partial sums of y.y (set 0) and f.f (set 1).  Only free nodes are
+// MODE 0: y = K u (into y32 instead when it is set), optional partial sums of u.y;
+// MODE 1: y = f - K u, optional float copy y32, optional partial sums of y.y (set 0) and
+// f.f (set 1).  Only free nodes are
 // touched (the others stay 0 in every vector).
 // TU: type of u (float u with FP64 arithmetic: the FCG search direction of the mixed
 // precision solves).  FP32 arithmetic: all 81 neighbour values live at once, register
@@ -183,10 +184,11 @@ __global__ TOS_FINE_BOUNDS(T) void k_fine_apply(LevelView L, const int4* __restr
     if constexpr (sizeof(T) == 4) TOS_LOADS_DONE();
     T acc[3] = {0, 0, 0}, D[9];
     fine_row<T, true, 0x7ffffffu, TU>(wa, Ke, un, acc, D);
-    if (MODE == 0) {
+    if (MODE == 0) {  // (y32 set: the result stored in FP32 instead of y)
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        y[c * L.NP + n.p] = acc[c];
+        if (y32) y32[c * L.NP + n.p] = (float)acc[c];
+        else y[c * L.NP + n.p] = acc[c];
         d += (double)un[13][c] * (double)acc[c];
       }
     } else {
@@ -1483,6 +1485,57 @@ __global__ __launch_bounds__(256) void k_pcg_r(const int4* __restrict__ cl, int 
     }
   }
   block_partial(s, partial, 0);
+}
+
+// FCG with FP32 residual and K p (solve precision 3): r -= alpha q (FP64 arithmetic), partial ||r||^2
+__global__ __launch_bounds__(256) void k_pcg_r32(const int4* __restrict__ cl, int nch, int64_t NP,
+                                                 float* __restrict__ r, const float* __restrict__ q,
+                                                 const double* __restrict__ scal, double* __restrict__ partial) {
+  NodeRef n;
+  const bool ok = chunk_node(cl, nch, 1, n);
+  double s = 0;
+  if (ok) {
+    const double al = scal[S_ALPHA];
+    double rv[3], qv[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      rv[c] = (double)r[c * NP + n.p];
+      qv[c] = (double)q[c * NP + n.p];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const float rn = (float)(rv[c] - al * qv[c]);
+      r[c * NP + n.p] = rn;
+      s += (double)rn * (double)rn;
+    }
+  }
+  block_partial(s, partial, 0);
+}
+
+// partials of z.r and z.q on FP32 vectors (q may be null)
+__global__ __launch_bounds__(256) void k_pcg_zdots32(const int4* __restrict__ cl, int nch, int64_t NP,
+                                                     const float* __restrict__ z, const float* __restrict__ r,
+                                                     const float* __restrict__ q, double* __restrict__ partial) {
+  NodeRef n;
+  const bool ok = chunk_node(cl, nch, 1, n);
+  double zr = 0, zq = 0;
+  if (ok) {
+    double zv[3], rv[3], qv[3] = {0, 0, 0};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int64_t i = c * NP + n.p;
+      zv[c] = (double)z[i];
+      rv[c] = (double)r[i];
+      if (q) qv[c] = (double)q[i];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      zr += zv[c] * rv[c];
+      zq += zv[c] * qv[c];
+    }
+  }
+  block_partial(zr, partial, 0);
+  block_partial(zq, partial, 1);
 }
 
 // u += alpha p (the previous iteration's step, unless first) ; p = z + beta p (first: p = z)

@@ -1837,6 +1837,136 @@ int solve_case(Ctx& c, int k, const tos_solve_opts& o, int* its, double* relres)
   return *relres <= o.rtol ? TOS_OK : TOS_E_NOCONV;
 }
 
+// Precision 3: the precision-2 solve with the FCG residual r and K p stored in FP32.  The
+// operator keeps FP64 arithmetic on the FP32 direction (q = K p is exact before its
+// rounding to FP32), u stays FP64, and the recursive residual r -= alpha q is kept in FP32
+// (the V-cycle reads it directly); when it reports convergence, f - K u is formed in FP64
+// and, if that misses the tolerance, the FCG restarts from it.  No reference counterpart.
+int solve_case_r32(Ctx& c, int k, const tos_solve_opts& o, int* its, double* relres) {
+  Level& L0 = c.L[0];
+  const int64_t NP = L0.v.NP, n3 = 3 * NP;
+  double* u = c.U + (size_t)k * n3;
+  const double* f = c.F + (size_t)k * n3;
+  const unsigned nb = c.nb0;
+  const int4* cl = L0.cl_lin;
+  const int nch = L0.n_lin;
+  if (!c.q32) c.q32 = dalloc<float>(c, n3);
+  float* r32 = L0.f32;  // the FCG residual is the V-cycle's right-hand side
+  float* q32 = c.q32;
+  float* p32 = c.p32;
+  const float* z32 = L0.u32;
+  if (!o.warm_start && nch)
+    hipLaunchKernelGGL(k_zero_nodes<double>, dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, u);
+  apply_level<double, 1>(c, 0, u, f, c.r, nullptr, r32);  // FP64 residual and its FP32 copy
+  dot_to(c, f, f, c.r, c.r, S_NF2, S_RN2, OP_NONE);
+  const double* hs = read_scal(c);
+  const double nf = std::sqrt(hs[S_NF2]);
+  double rn = std::sqrt(hs[S_RN2]);
+  *its = 0;
+  if (nf == 0) {
+    if (nch) hipLaunchKernelGGL(k_zero_nodes<double>, dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, u);
+    *relres = 0;
+    return TOS_OK;
+  }
+  if (!std::isfinite(rn)) {
+    c.err = "NaN/Inf in the initial residual";
+    return TOS_E_NAN;
+  }
+  // q = K p (FP64 arithmetic, FP32 result), alpha, r -= alpha q, |r|^2
+  auto tail = [&]() {
+    {
+      Prof pr(c, P_FINE_APPLY64);
+      hipLaunchKernelGGL((k_fine_apply<double, 0, float>), dim3(cblk(L0.n_lin)), dim3(256), 0, c.stream, L0.v, cl,
+                         nch, Prec<double>::w(c), Prec<double>::Ke(c), (const float*)p32, (const double*)nullptr,
+                         (double*)nullptr, c.partial, q32);
+    }
+    Prof pr(c, P_VECTOR);
+    finish0(c, 0, S_PQ, 0, OP_ALPHA);
+    hipLaunchKernelGGL(k_pcg_r32, dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, r32, (const float*)q32,
+                       (const double*)c.scal, c.partial);
+    finish0(c, 0, S_RN2, 0, OP_NONE);
+    LAUNCH_CHECK();
+  };
+  // z = B r, beta, u += alpha p (deferred), p = z + beta p
+  auto head = [&]() {
+    precondition(c, 2, o.pre, o.post, true, false);
+    Prof pr(c, P_VECTOR);
+    hipLaunchKernelGGL(k_pcg_zdots32, dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, z32, (const float*)r32,
+                       (const float*)q32, c.partial);
+    finish0(c, 1, S_ZR, S_ZQ, OP_BETA);
+    hipLaunchKernelGGL((k_pcg_up<float, float>), dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, u, p32, z32,
+                       (const double*)c.scal, 0);
+    LAUNCH_CHECK();
+  };
+  const bool graphed = c.graph_replay && !c.prof_on;
+  hipGraphExec_t exec = nullptr;
+  if (graphed) {
+    bool stale = false;
+    for (const Ctx::SolveGraph& g : c.graphs) stale |= g.epoch != c.alloc_epoch;
+    if (stale) {
+      for (const Ctx::SolveGraph& g : c.graphs) {
+        (void)hipGraphExecDestroy(g.exec);
+        (void)hipGraphDestroy(g.graph);
+      }
+      c.graphs.clear();
+    }
+    for (const Ctx::SolveGraph& g : c.graphs)
+      if (g.precision == 3 && g.pre == o.pre && g.post == o.post && g.sweep == c.sweep_mode && g.gal == c.gal_prec &&
+          g.u == u)
+        exec = g.exec;
+  }
+  int it = 0;
+  while (rn / nf > o.rtol && it < o.maxit) {
+    // FCG on r32 (from the true residual: at the start and after a restart)
+    precondition(c, 2, o.pre, o.post, true, false);
+    {
+      Prof pr(c, P_VECTOR);
+      hipLaunchKernelGGL((k_pcg_up<float, float>), dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, u, p32, z32,
+                         (const double*)c.scal, 1);
+      hipLaunchKernelGGL(k_pcg_zdots32, dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, z32, (const float*)r32,
+                         (const float*)nullptr, c.partial);
+      finish0(c, 0, S_RZ, 0, OP_NONE);
+    }
+    bool first = true;
+    while (it < o.maxit) {
+      if (first) {
+        tail();
+        first = false;
+      } else if (graphed) {
+        if (!exec) {
+          Ctx::SolveGraph g{3, o.pre, o.post, c.sweep_mode, c.gal_prec, u, 0, nullptr, nullptr};
+          HIPCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+          head();
+          tail();
+          HIPCHK(hipStreamEndCapture(c.stream, &g.graph));
+          HIPCHK(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+          g.epoch = c.alloc_epoch;
+          c.graphs.push_back(g);
+          exec = g.exec;
+        }
+        HIPCHK(hipGraphLaunch(exec, c.stream));
+      } else {
+        head();
+        tail();
+      }
+      ++it;
+      const double r_in = std::sqrt(read_scal(c)[S_RN2]);
+      if (!std::isfinite(r_in)) { c.err = "NaN/Inf in PCG"; return TOS_E_NAN; }
+      if (r_in / nf <= o.rtol) break;
+    }
+    hipLaunchKernelGGL((k_pcg_ustep<float>), dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, u, (const float*)p32,
+                       (const double*)c.scal);
+    // the true residual in FP64; its FP32 copy restarts the FCG when it misses the tolerance
+    apply_level<double, 1>(c, 0, u, f, c.r, nullptr, r32);
+    dot_to(c, c.r, c.r, nullptr, nullptr, S_RN2, 0, OP_NONE);
+    rn = std::sqrt(read_scal(c)[S_RN2]);
+    if (!std::isfinite(rn)) { c.err = "NaN/Inf in the true residual"; return TOS_E_NAN; }
+  }
+  *relres = rn / nf;
+  *its = it;
+  return *relres <= o.rtol ? TOS_OK : TOS_E_NOCONV;
+}
+
 float* sens_buf(Ctx& c, int which) { return which == 0 ? c.sens : which == 1 ? c.vsens : nullptr; }
 
 double volume_of(Ctx& c) {
@@ -2588,17 +2718,19 @@ TOS_EXPORT int tos_vcycle(tos_ctx* c, const double* r, double* z, int precision,
 TOS_EXPORT int tos_solve(tos_ctx* c, const tos_solve_opts* o, tos_solve_stats* st) {
   return guard(c, [&]() {
     REQUIRE(o && o->rtol > 0 && o->maxit >= 0 && o->pre >= 0 && o->post >= 0 && o->precision >= 0 &&
-                o->precision <= 2,
+                o->precision <= 3 && (o->precision < 3 || o->use_pcg),
             "bad options");
     if (!c->op_ready) { c->err = "update_operator first"; return TOS_E_STATE; }
-    const int rs = select_stencils(*c, o->precision);
+    REQUIRE(o->precision < 3 || !c->learn || (c->learn->mode == 0 && !c->learn->ds),
+            "precision 3 takes no learned correction");
+    const int rs = select_stencils(*c, o->precision == 3 ? 2 : o->precision);
     if (rs) return rs;
     const auto t0 = std::chrono::steady_clock::now();
     int rc_all = TOS_OK;
     for (int k = 0; k < c->K; ++k) {
       int its = 0;
       double rr = 0;
-      const int rc = solve_case(*c, k, *o, &its, &rr);
+      const int rc = o->precision == 3 ? solve_case_r32(*c, k, *o, &its, &rr) : solve_case(*c, k, *o, &its, &rr);
       if (st && k < 64) { st->iterations[k] = its; st->rel_res[k] = rr; }
       if (rc == TOS_E_NAN) return rc;
       if (rc) rc_all = rc;

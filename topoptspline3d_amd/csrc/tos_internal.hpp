@@ -151,6 +151,7 @@ struct Ctx {
   // PCG work vectors (FP64, 3*NP)
   double *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
   float* p32 = nullptr;         // FCG search direction of the FP32-cycle solves (0 off the free nodes)
+  float* q32 = nullptr;         // K p stored in FP32 (solve precision 3; allocated on first use)
   // reductions
   double* partial = nullptr;    // 2 * max(kRedBlocks, blocks of the fine chunk list) + 2 * 256 stage
   int nb0 = 0;                  // blocks of a launch over the fine chunk list
