@@ -3,8 +3,10 @@
 // against this library:
 //   grid::HierarchyGrid / grid::Grid   (grid/Grid.h:214-1037)
 //   setParameters, buildGrids, uploadTemplateMatrix, initDensities, initCoeffs,
-//   update_stencil, computeSensitivity[2], updateDensities, updateCoeff,
-//   projectDensities, getGrids         (optimization/optimization.h:41-126)
+//   update_stencil, computeSensitivity[2], updateDensities[2], updateCoeff,
+//   projectDensities, deal_surface_points, getGrids
+//                                      (optimization/optimization.h:41-126,
+//                                       optimization/optimization.cpp:1103-1128)
 //   MMA::mma_t                         (optimization/mma_t.h:159-266)
 // The entry points are resolved at run time from a shared library (`tos::Api::load`,
 // default prefix "tos_"), so one driver binary can select the library it drives.
@@ -67,6 +69,10 @@ struct Api {
   TOS_FN(int, ctx_mma_step, tos_ctx*, tos_mma*, double, double, double, int*)
   TOS_FN(int, worst_compliance, tos_ctx*, const tos_worst_opts*, tos_worst_stats*)
   TOS_FN(int, get_loads, tos_ctx*, int, double*)
+  TOS_FN(int, oc_update_projected, tos_ctx*, double, double, double, double, double*, int*, double*)
+  TOS_FN(int, surface_constraints, tos_ctx*, const tos_surface_opts*, tos_surface_stats*)
+  TOS_FN(int, spline_get_coeff_sens, tos_ctx*, int, float*)
+  TOS_FN(int, ctx_mma_step_surface, tos_ctx*, tos_mma*, double, double, double, double, double, int*)
 #undef TOS_FN
   void* handle = nullptr;
 
@@ -92,7 +98,8 @@ struct Api {
     TOS_SYM(projection_sensitivity) TOS_SYM(spline_init) TOS_SYM(spline_set_coeffs) TOS_SYM(spline_get_coeffs)
     TOS_SYM(spline_eval) TOS_SYM(spline_adjoint) TOS_SYM(oc_update_coeffs) TOS_SYM(mma_create)
     TOS_SYM(mma_destroy) TOS_SYM(mma_set_x) TOS_SYM(mma_get_x) TOS_SYM(mma_update) TOS_SYM(ctx_mma_step)
-    TOS_SYM(worst_compliance) TOS_SYM(get_loads)
+    TOS_SYM(worst_compliance) TOS_SYM(get_loads) TOS_SYM(oc_update_projected) TOS_SYM(surface_constraints)
+    TOS_SYM(spline_get_coeff_sens) TOS_SYM(ctx_mma_step_surface)
 #undef TOS_SYM
     return a;
   }
@@ -156,6 +163,15 @@ class Grid {
     tos::check(tos::Api::get().get_sensitivity(c_, 0, v.data()), c_, "get_sensitivity");
     return v;
   }
+  // surface-constraint values of the last deal_surface_points (grid/Grid.cu:6101-6145,
+  // 6485-6525; the reference returns them as float, here the double the library computed)
+  double global_selfsupp_constraint() const { return surf_.ss_value; }
+  double global_drip_constraint() const { return surf_.drip_value; }
+  std::vector<float> getSSCSens() const { return coeff_sens(2); }
+  std::vector<float> getDripCSens() const { return coeff_sens(3); }
+  std::vector<float> getCSens() const { return coeff_sens(0); }
+  std::vector<float> getVolCSens() const { return coeff_sens(1); }
+  tos_surface_stats surf_{0, 0, 1.0, 1.0, 0.0};
   int n_rho() const { return (int)info().n_elements; }
   int n_nodes() const { return (int)info().n_nodes; }
   int n_cijk() const { return (int)info().n_coeffs; }
@@ -164,6 +180,11 @@ class Grid {
   tos_ctx* ctx() const { return c_; }
 
  private:
+  std::vector<float> coeff_sens(int which) const {
+    std::vector<float> v(n_cijk());
+    tos::check(tos::Api::get().spline_get_coeff_sens(c_, which, v.data()), c_, "spline_get_coeff_sens");
+    return v;
+  }
   tos_info info() const {
     tos_info i;
     tos::check(tos::Api::get().get_info(c_, &i), c_, "get_info");
@@ -188,6 +209,15 @@ class HierarchyGrid {
     for (int l = 0; l < i.n_levels; ++l) grids_.emplace_back(c_, l);
   }
   void setLoads(int n_cases, const double* f) { tos::check(tos::Api::get().set_loads(c_, n_cases, f), c_, "set_loads"); }
+  // spline knots over the box (set_spline_partition, grid/Grid.cu:3134-3203)
+  void set_spline_partition(int order, int partition) {
+    tos::check(tos::Api::get().spline_init(c_, 0, order, partition), c_, "spline_init");
+  }
+  // uniform lattice: a coefficient every `spacing` voxels (no counterpart in the reference)
+  void set_spline_lattice(int order, int spacing) {
+    tos::check(tos::Api::get().spline_init(c_, 1, order, spacing), c_, "spline_init");
+  }
+  void setCoeffs(const float* c) { tos::check(tos::Api::get().spline_set_coeffs(c_, c), c_, "spline_set_coeffs"); }
   void update_stencil() { tos::check(tos::Api::get().update_operator(c_), c_, "update_operator"); }
   // solve to `rtol`; returns the final relative residual (v_cycle returns the residual of
   // one cycle in the reference, grid/Grid.cpp:2399-2444)
@@ -335,6 +365,29 @@ inline void projectDensities(float beta) {
   auto c = getGrids().ctx();
   tos::check(tos::Api::get().project_density(c, beta), c, "project_density");
 }
+// OC on the design variable behind the projection (updateDensities2, optimization.cu:462-557);
+// change[0], change[1]: max and mean design change
+inline float updateDensities2(float Vgoal, float beta, float change[2]) {
+  auto c = getGrids().ctx();
+  double g, ch[2];
+  int n;
+  tos::check(tos::Api::get().oc_update_projected(c, Vgoal, params().design_step, params().damp_ratio, beta, &g, &n, ch),
+             c, "oc_update_projected");
+  change[0] = (float)ch[0];
+  change[1] = (float)ch[1];
+  return (float)g;
+}
+// Surface points of the spline, their self-supporting and drip constraints and coefficient
+// gradients (deal_surface_points, optimization/optimization.cpp:1103-1128), with the
+// reference's defaults (64 background cells, modes "oh", print angle 3 pi/4); read the
+// results with getGrids()[0].global_selfsupp_constraint() / global_drip_constraint() /
+// getSSCSens() / getDripCSens().
+inline void deal_surface_points(float beta, int ss_mode = 4, int drip_mode = 4) {
+  auto c = getGrids().ctx();
+  tos_surface_opts o{64, beta, 0.75 * M_PI, ss_mode, drip_mode, 8.0, 10.0, 0.09, 128.0, 32.0,
+                     160.0 / 180.0 * M_PI, 4.0, 10.0, 0};
+  tos::check(tos::Api::get().surface_constraints(c, &o, &getGrids()[0].surf_), c, "surface_constraints");
+}
 
 // ------------------------------------------------------------------------------------------
 // MMA::mma_t (optimization/mma_t.h:159-266)
@@ -366,6 +419,15 @@ class mma_t {
   bool update_on(grid::HierarchyGrid& G, double sens_scale, double vol_scale, double vfrac) {
     int cv = 0;
     tos::check(tos::Api::get().ctx_mma_step(G.ctx(), h_, sens_scale, vol_scale, vfrac, &cv), G.ctx(), "ctx_mma_step");
+    return cv != 0;
+  }
+  // the self-supporting spline driver's step (m = 3: volume, self-support, drip; utils/test_utils.cpp:1135-1188)
+  bool update_on_surface(grid::HierarchyGrid& G, double sens_scale, double vol_scale, double vfrac, double ss_scale,
+                         double drip_scale) {
+    int cv = 0;
+    tos::check(tos::Api::get().ctx_mma_step_surface(G.ctx(), h_, sens_scale, vol_scale, vfrac, ss_scale, drip_scale,
+                                                    &cv),
+               G.ctx(), "ctx_mma_step_surface");
     return cv != 0;
   }
   int n_dim() const { return n_; }

@@ -77,3 +77,61 @@ def test_facade_worst_case_loop_on_the_hip_library(worst_driver):
     ref = run(worst_driver, ORACLE_LIB, "orc_", (16, 8, 8), 3)
     assert np.all(np.abs(rows[:, 1] - ref[:, 1]) <= 1e-6 * np.abs(ref[:, 1]))
     assert np.all(np.abs(rows[:, 3] - ref[:, 3]) <= 1)
+
+
+SRC_SS = os.path.join(ROOT, "topoptspline3d_amd", "examples", "self_supporting_topopt.cpp")
+SS_DIMS, SS_ITERS = (16, 8, 8), 4
+
+
+@pytest.fixture(scope="module")
+def ss_driver(tmp_path_factory):
+    return _compile(tmp_path_factory, SRC_SS, "self_supporting_topopt")
+
+
+def _ss_coeffs(tmp_path_factory):
+    """wavy initial coefficients (so the spline has an interior surface from the start)"""
+    from topoptspline3d_amd.context import Context
+    from topoptspline3d_amd.domains import cantilever
+    solid, support, _ = cantilever(*SS_DIMS)
+    ctx = Context(oracle_api(), solid, support)
+    n = ctx.spline_init(1, 4, 4)
+    i = np.arange(n)
+    c0 = (0.4 + 0.3 * np.sin(0.9 * i) * np.cos(0.37 * i)).astype(np.float32)
+    path = str(tmp_path_factory.mktemp("coeffs") / "c0.f32")
+    c0.tofile(path)
+    return c0, path
+
+
+def _run_ss(exe, lib, prefix, path):
+    out = subprocess.check_output([exe, lib, prefix, *map(str, SS_DIMS), str(SS_ITERS), path], text=True,
+                                  timeout=600)
+    return np.array([[float(v) for v in line.split()] for line in out.strip().splitlines()])
+
+
+def test_facade_self_supporting_loop_on_the_oracle(ss_driver, tmp_path_factory):
+    """the self-supporting spline loop (Heaviside projection, surface constraints, MMA
+    m = 3) through the facade equals the Python driver over the same library"""
+    c0, path = _ss_coeffs(tmp_path_factory)
+    rows = _run_ss(ss_driver, ORACLE_LIB, "orc_", path)
+    from topoptspline3d_amd.drivers import run_spline_mma
+    f32 = lambda v: float(np.float32(v))  # the facade's Parameter holds floats (optimization.h:15-35)
+    ref = run_spline_mma(oracle_api(), *SS_DIMS, iters=SS_ITERS, rtol=1e-8, precision=0, heaviside=True,
+                         self_support=True, c0=c0, vfrac=f32(0.4), nu=f32(0.4), rho_min=f32(1e-3))
+    assert rows.shape == (SS_ITERS, 7)
+    assert np.allclose(rows[:, 1], ref["c"], rtol=1e-11, atol=0)
+    assert np.allclose(rows[:, 3], ref["ss"], rtol=1e-11, atol=1e-14)
+    assert np.allclose(rows[:, 4], ref["drip"], rtol=1e-11, atol=1e-14)
+    assert np.all(rows[:, 3] != 1.0)  # the surface exists from the first iteration
+
+
+@pytest.mark.gpu
+def test_facade_self_supporting_loop_on_the_hip_library(ss_driver, tmp_path_factory):
+    from topoptspline3d_amd.backend import LIB, gpu_available
+    if not gpu_available():
+        pytest.skip("no GPU")
+    _, path = _ss_coeffs(tmp_path_factory)
+    rows = _run_ss(ss_driver, LIB, "tos_", path)
+    ref = _run_ss(ss_driver, ORACLE_LIB, "orc_", path)
+    assert np.all(np.abs(rows[:, 1] - ref[:, 1]) <= 1e-5 * np.abs(ref[:, 1]))
+    assert np.all(np.abs(rows[:, 3] - ref[:, 3]) <= 1e-4 * np.abs(ref[:, 3]) + 1e-6)
+    assert np.all(np.abs(rows[:, 4] - ref[:, 4]) <= 1e-4 * np.abs(ref[:, 4]) + 1e-6)

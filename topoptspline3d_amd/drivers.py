@@ -144,7 +144,7 @@ def run_oc(api, nx, ny, nz, iters=50, rtol=1e-8, precision=0, maxit=200, domain=
 
 def run_spline_mma(api, nx, ny, nz, iters=100, rtol=1e-6, precision=1, spacing=4, order=4,
                    move=0.06, seed=11, maxit=200, sens_scale=None, vol_scale=1e3, heaviside=False,
-                   self_support=False, surface=None, **kw):
+                   self_support=False, surface=None, c0=None, **kw):
     """Cubic uniform spline lattice every `spacing` voxels, coefficients 0.4 +- 0.05
     (seeded), MMA on the coefficients with move limit `move`.  The constraint is scaled as
     in the reference driver (volScale 1e3, utils/test_utils.cpp:1015-1020); the objective
@@ -159,7 +159,8 @@ def run_spline_mma(api, nx, ny, nz, iters=100, rtol=1e-6, precision=1, spacing=4
     iteration (`surface`: keyword options of Context.surface_constraints), scaled by
     SSScale / dripScale = 1e-1 / 1e2, from iteration 31 1e0 / 1e3 and from iteration 81
     1e1 / 1e4.  While all coefficients are equal the constraints read ss = 1, drip = 0
-    with zero gradients (grids.areALLCoeffsEqual, utils/test_utils.cpp:1106-1113)."""
+    with zero gradients (grids.areALLCoeffsEqual, utils/test_utils.cpp:1106-1113).
+    `c0`: initial coefficients in place of the seeded ones."""
     p = dict(DEFAULTS, **kw)
     solid, support, f = cantilever(nx, ny, nz)
     ctx = Context(api, solid, support)
@@ -167,7 +168,9 @@ def run_spline_mma(api, nx, ny, nz, iters=100, rtol=1e-6, precision=1, spacing=4
     ctx.set_loads(f)
     n = ctx.spline_init(1, order, spacing)
     rng = np.random.default_rng(seed)
-    c0 = (p["vfrac"] + rng.uniform(-0.05, 0.05, n)).astype(np.float32)
+    if c0 is None:
+        c0 = (p["vfrac"] + rng.uniform(-0.05, 0.05, n)).astype(np.float32)
+    c0 = np.asarray(c0, np.float32)
     ctx.set_coeffs(c0)
     mma = MMA(api, n, 3 if self_support else 1, p["rho_min"], 1.0, move=move)
     mma.set_x(c0.astype(np.float64))
