@@ -41,6 +41,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "ms/TO-iter at 511-reso to 1e-6 residual, learned vs plain MG; HBM GB/s"
 HBM_PEAK = 8000.0  # GB/s, MI355X
+# TFLOP/s, MI355X vector FMA: FP64 78.6 (published); FP32 157.3 published with packed FMAs,
+# 78.6 without them (packed FP32 is compiled out)
+FP32_VECTOR_PEAK, FP64_VECTOR_PEAK = 78.6, 78.6
 
 
 def log(*a):
@@ -170,8 +173,10 @@ def kernel_stats(api, ctx, enable):
     return {ks.names[i].value.decode(): (ks.total_ms[i], ks.launches[i]) for i in range(ks.n_classes)}
 
 
-DTYPES = {0: "f64", 1: "f32 V-cycle / f64 PCG", 2: "f32 V-cycle (f16 coarse stencils) / f64 PCG"}
-SOLVERS = {0: "FP64 V-cycle", 1: "FP32 V-cycle", 2: "FP32 V-cycle on FP16 coarse stencils"}
+DTYPES = {0: "f64", 1: "f32 V-cycle / f64 PCG", 2: "f32 V-cycle (f16 coarse stencils) / f64 PCG",
+          3: "f32 V-cycle (f16 coarse stencils) / f32 PCG residual, f64 u and operator"}
+SOLVERS = {0: "FP64 V-cycle", 1: "FP32 V-cycle", 2: "FP32 V-cycle on FP16 coarse stencils",
+           3: "FP32 V-cycle on FP16 coarse stencils, FP32 residual and K p"}
 
 
 def roofline(stats, info, n_active_nodes, steps, workload_tag=None, precision=1, sweep_mode=0):
@@ -193,7 +198,7 @@ def roofline(stats, info, n_active_nodes, steps, workload_tag=None, precision=1,
         "fine_resid_f32": 4 * Ne + (9 * 4 + 4) * Nv,
         "fine_resid_f64": 4 * Ne + (9 * 8 + 4) * Nv,
         "fine_apply_f64": 4 * Ne + (6 * 8 + 4) * Nv,
-        "st_gs_f32_l1": ((122 * 4 + 4 if precision == 2 else 243 * 4) + 9 * 4) * Nc1 / 8.0,
+        "st_gs_f32_l1": ((122 * 4 + 4 if precision >= 2 else 243 * 4) + 9 * 4) * Nc1 / 8.0,
     }
     flops = {"fine_gs_f32": 1170 * Nv / 8.0 * colours, "fine_gs_f64": 1170 * Nv / 8.0, "fine_resid_f32": 1152 * Nv,
              "fine_resid_f64": 1152 * Nv, "fine_apply_f64": 1152 * Nv, "st_gs_f32_l1": 504 * Nc1 / 8.0}
@@ -211,6 +216,8 @@ def roofline(stats, info, n_active_nodes, steps, workload_tag=None, precision=1,
         if t and (not dom.startswith("fine_gs") or int(js.get("sweep_mode", 0)) == (sweep_mode if dom == "fine_gs_f32" else 0)):
             out["traffic"] = round(t["bytes_per_launch"])
             out["traffic_source"] = os.path.relpath(tf, ROOT)
+            if js.get("valu"):
+                out["valu_busy_counter"] = js["valu"]["valu_busy"]
     if dom in per_launch and stats[dom][1] > 0:
         avg_ms = stats[dom][0] / stats[dom][1]
         ach = per_launch[dom] / (avg_ms * 1e-3) / 1e9
@@ -218,17 +225,23 @@ def roofline(stats, info, n_active_nodes, steps, workload_tag=None, precision=1,
                     "avg_launch_us": round(avg_ms * 1e3, 2), "launches": stats[dom][1],
                     "bytes_per_launch": per_launch[dom],
                     "achieved_tflops": round(flops[dom] / (avg_ms * 1e-3) / 1e12, 2)})
+        # the same launches against the vector FP32/FP64 rate (no packed FP32 FMAs)
+        vpeak = FP32_VECTOR_PEAK if dom.endswith("f32") or dom == "st_gs_f32_l1" else FP64_VECTOR_PEAK
+        out["compute"] = {"achieved_tflops": out["achieved_tflops"], "peak_tflops": vpeak,
+                          "frac": round(out["achieved_tflops"] / vpeak, 4),
+                          "peak_note": "vector FMA rate without packed FP32"}
     else:
         out.update({"achieved": None, "frac": None})
     return out
 
 
-def cpu_baseline(wl, gpu_its):
+def cpu_baseline(wl, gpu_its, coeffs=None):
     """FP64 oracle on the host cores: one complete optimisation iteration of the same
     workload from the same initial design (spline evaluation, operator update, FCG solve
     from zero to the workload's tolerance with FP64 V-cycles, compliance, sensitivity,
     filter, spline adjoint and MMA step -- or OC for element-density workloads), timed
-    per phase."""
+    per phase.  With `coeffs` (the GPU run's final coefficients), the surface constraints
+    on the 64-cell lattice are timed on them as well (reported beside, not in the value)."""
     import ctypes
     from topoptspline3d_amd import _abi
     lib = os.path.join(ROOT, "oracle", "liboracle.so")
@@ -245,6 +258,13 @@ def cpu_baseline(wl, gpu_its):
     loop.step(phases=ph)
     t_iter = time.perf_counter() - t_iter
     its = loop.its[-1]
+    surf = ""
+    if coeffs is not None:
+        loop.ctx.set_coeffs(coeffs)
+        t = time.perf_counter()
+        st = loop.ctx.surface_constraints(cells=64, beta=8.0)
+        surf = (f"; surface constraints on the GPU run's final coefficients, 64 cells: "
+                f"{(time.perf_counter() - t) * 1e3:.1f} ms, {st['n_points']} points")
     loop.ctx.close()
     cpu = ""
     try:
@@ -259,7 +279,7 @@ def cpu_baseline(wl, gpu_its):
             "sample": (f"one complete optimisation iteration of the FP64 oracle on the same domain from the same "
                        f"initial design ({parts}; FCG from zero: {its} iterations, the GPU's timed iterations "
                        f"took {min(gpu_its)}-{max(gpu_its)} from their warm starts); context set-up "
-                       f"{t_setup:.1f}s not included; CPU: {cpu or 'unknown'}")}
+                       f"{t_setup:.1f}s not included{surf}; CPU: {cpu or 'unknown'}")}
 
 
 def learned_phase(api, n_dom, iters, steps, level=2):
@@ -378,6 +398,27 @@ def worst_phase(ctx, wl, precision):
     return out
 
 
+def surface_phase(ctx, reps=5):
+    """Self-supporting and drip constraints on the spline's surface points (deal_surface_points,
+    optimization/optimization.cpp:1103-1128) for the current coefficients: the reference's
+    64-cell background lattice and two finer ones, modes "oh"/"oh", beta 8; median of
+    `reps` calls each (every call runs the whole chain: lattice values, vertex count and
+    scan, points, orientation vote, constraint values and coefficient gradients)."""
+    out = {}
+    for cells in (64, 128, 256):
+        ts = []
+        for _ in range(reps):
+            sync()
+            t = time.perf_counter()
+            st = ctx.surface_constraints(cells=cells, beta=8.0)
+            sync()
+            ts.append(time.perf_counter() - t)
+        out[f"cells_{cells}"] = {"ms": round(float(np.median(ts)) * 1e3, 3), "points": int(st["n_points"]),
+                                 "ss": st["ss_value"], "drip": st["drip_value"]}
+        log(f"[bench] surface constraints, {cells} cells: {out[f'cells_{cells}']}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -385,11 +426,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--precision", type=int, default=2,
-                    help="0 FP64 V-cycle; 1 FP32 V-cycle; 2 FP32 V-cycle on FP16 coarse stencils")
+                    help="0 FP64 V-cycle; 1 FP32 V-cycle; 2 FP32 V-cycle on FP16 coarse stencils; "
+                         "3 as 2 with the FCG residual and K p stored in FP32")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-compare", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-worst", action="store_true", help="skip the worst-case load search")
+    ap.add_argument("--no-surface", action="store_true", help="skip the surface-constraint timing")
     ap.add_argument("--learn-domains", type=int, default=4)
     ap.add_argument("--learn-iters", type=int, default=50)
     ap.add_argument("--train-steps", type=int, default=1500)
@@ -494,7 +537,8 @@ def main():
         line["plain_fp64"] = {"ms_per_iter": round(ms64, 3), "pcg_iterations": its64, "max_rel_res": res64,
                               "speedup_of_value": round(ms64 / ms, 3)}
     if net is not None:
-        lloop, msl, itsl, resl = run("learned", args.precision, True, args.steps, keep=True)
+        # the learned correction runs in the FP64-residual FCG (precision 3 takes none)
+        lloop, msl, itsl, resl = run("learned", min(args.precision, 2), True, args.steps, keep=True)
         tried, kept, stepk = LN.stats(lloop.ctx)
         LN.attach(lloop.ctx, None)
         lloop.ctx.close()
@@ -508,9 +552,10 @@ def main():
         if not wl["spline"]:
             loop.ctx.update_operator()
         cmp = {}
-        modes = [("fp64", 0, False), ("mixed_fp32_stencils", 1, False), ("mixed_fp16_stencils", 2, False)]
+        modes = [("fp64", 0, False), ("mixed_fp32_stencils", 1, False), ("mixed_fp16_stencils", 2, False),
+                 ("mixed_fp32_residual", 3, False)]
         if net is not None:
-            modes.append(("learned", args.precision, True))
+            modes.append(("learned", min(args.precision, 2), True))
         for tag, prec, use_net in modes:
             if use_net:
                 LN.attach(loop.ctx, net, level=2)
@@ -528,6 +573,10 @@ def main():
         if not wl["spline"]:
             loop.ctx.update_operator()
         line["worst_case_load"] = worst_phase(loop.ctx, wl, args.precision)
+    coeffs_final = None
+    if wl["spline"] and not args.no_surface:
+        line["surface_constraints"] = surface_phase(loop.ctx)
+        coeffs_final = loop.ctx.coeffs()
     loop.ctx.close()
     if args.batched_steps > 0 and args.workload == "c2":
         t = time.perf_counter()
@@ -535,7 +584,7 @@ def main():
         log(f"[bench] batched workload took {time.perf_counter() - t:.1f}s")
     if not args.no_cpu_baseline:
         t = time.perf_counter()
-        line["cpu_baseline"] = cpu_baseline(wl, its)
+        line["cpu_baseline"] = cpu_baseline(wl, its, coeffs_final)
         log(f"[bench] cpu baseline took {time.perf_counter() - t:.1f}s")
     print(json.dumps(line), flush=True)
 

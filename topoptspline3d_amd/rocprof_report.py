@@ -14,7 +14,11 @@ many work-items (a kernel launched on several multigrid levels: the largest leve
 launches); with --trace (a `<name>_kernel_trace.csv`) the class's mean durations come from
 the traced dispatches under the same filter, else from the per-name stats.
 
+* issue rates from SQ/GRBM counter runs (``--compute``): VALU busy fraction, FP32 FMA
+  share and rate, MFMA busy fraction per kernel (``compute_rates`` has the formulas).
+
 Usage:
+  python -m topoptspline3d_amd.rocprof_report --compute [--min-grid G] PATTERN... -- P1.csv P2.csv ...
   python -m topoptspline3d_amd.rocprof_report --stats DIR/c2_kernel_stats.csv \\
       --kernel "k_fine_gs<float" [--kernel ...] --fetch F.csv --write W.csv \\
       --cal-fetch CF.csv --cal-write CW.csv --copy-mib 1024 [--json out.json] [--meta k=v]
@@ -99,7 +103,89 @@ def traffic(fetch_csv, write_csv, cal_fetch_csv, cal_write_csv, copy_mib, patter
             "bytes_per_launch": f * 1024.0 * kf + w * 1024.0 * kw}
 
 
+def counters(counter_csvs, pattern, min_grid=0):
+    """{counter: mean value per dispatch} and the mean dispatch duration (us) over the
+    dispatches of kernels matching `pattern` with at least `min_grid` work-items, from one
+    or more counter runs (each run one pass of counters)"""
+    vals, durs = {}, []
+    for f in counter_csvs:
+        for r in csv.DictReader(open(f)):
+            if _short(r["Kernel_Name"]).startswith(pattern) and _grid(r) >= min_grid:
+                vals.setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+                durs.append(float(r["End_Timestamp"]) - float(r["Start_Timestamp"]))
+    mean = {k: sum(v) / len(v) for k, v in vals.items()}
+    return mean, {k: len(v) for k, v in vals.items()}, (sum(durs) / len(durs) / 1e3 if durs else None)
+
+
+def compute_rates(c, us, n_simd, n_xcd):
+    """Derived issue rates from SQ/GRBM counters of one dispatch (means):
+    gpu_cycles = GRBM_GUI_ACTIVE / n_xcd (the counter sums the XCDs);
+    valu_busy = SQ_ACTIVE_INST_VALU * 4 / (n_simd * gpu_cycles) (quad-cycles per SIMD);
+    fma_share = SQ_INSTS_VALU_FMA_F32 / SQ_INSTS_VALU; fp32_fma_tflops = 128 flops per
+    wave-wide FMA instruction over the dispatch duration; mfma_busy =
+    SQ_VALU_MFMA_BUSY_CYCLES / (n_simd * gpu_cycles)."""
+    out = {}
+    cyc = c.get("GRBM_GUI_ACTIVE")
+    if cyc:
+        cyc /= n_xcd
+        out["gpu_cycles"] = cyc
+        out["clock_ghz"] = cyc / (us * 1e3) if us else None
+        if "SQ_ACTIVE_INST_VALU" in c:
+            out["valu_busy"] = c["SQ_ACTIVE_INST_VALU"] * 4 / (n_simd * cyc)
+        if "SQ_VALU_MFMA_BUSY_CYCLES" in c:
+            out["mfma_busy"] = c["SQ_VALU_MFMA_BUSY_CYCLES"] / (n_simd * cyc)
+    if "SQ_INSTS_VALU" in c and "SQ_INSTS_VALU_FMA_F32" in c:
+        out["fma_share"] = c["SQ_INSTS_VALU_FMA_F32"] / c["SQ_INSTS_VALU"]
+    if "SQ_INSTS_VALU" in c and "SQ_WAVES" in c:
+        out["valu_insts_per_wave"] = c["SQ_INSTS_VALU"] / c["SQ_WAVES"]
+    if "SQ_INSTS_VALU_FMA_F32" in c and us:
+        out["fp32_fma_tflops"] = c["SQ_INSTS_VALU_FMA_F32"] * 128 / (us * 1e-6) / 1e12
+    return out
+
+
+def compute_table(counter_csvs, patterns, min_grid=0, n_simd=1024, n_xcd=8):
+    """markdown table of compute_rates per kernel name (exact template instances)"""
+    names = []
+    for f in counter_csvs:
+        for r in csv.DictReader(open(f)):
+            k = _short(r["Kernel_Name"])
+            if any(k.startswith(p) for p in patterns) and _grid(r) >= min_grid and k not in names:
+                names.append(k)
+    out = ["| kernel | dispatches | mean us | VALU busy | FP32 FMA share | VALU insts / wave | FP32 FMA TFLOP/s "
+           "| MFMA busy | clock GHz |", "|---|---:|---:|---:|---:|---:|---:|---:|---:|"]
+    for k in names:
+        c, n, us = _exact(counter_csvs, k, min_grid)
+        d = compute_rates(c, us, n_simd, n_xcd)
+        f = lambda key, fmt: (fmt % d[key]) if d.get(key) is not None else "-"
+        out.append(f"| `{k}` | {max(n.values())} | {us:.2f} | {f('valu_busy', '%.3f')} | {f('fma_share', '%.3f')} | "
+                   f"{f('valu_insts_per_wave', '%.0f')} | {f('fp32_fma_tflops', '%.1f')} | {f('mfma_busy', '%.4f')} | "
+                   f"{f('clock_ghz', '%.2f')} |")
+    return "\n".join(out)
+
+
+def _exact(counter_csvs, name, min_grid):
+    vals, durs = {}, []
+    for f in counter_csvs:
+        for r in csv.DictReader(open(f)):
+            if _short(r["Kernel_Name"]) == name and _grid(r) >= min_grid:
+                vals.setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+                durs.append(float(r["End_Timestamp"]) - float(r["Start_Timestamp"]))
+    return ({k: sum(v) / len(v) for k, v in vals.items()}, {k: len(v) for k, v in vals.items()},
+            sum(durs) / len(durs) / 1e3 if durs else None)
+
+
 def main(argv=None):
+    if argv is None:
+        argv = sys.argv[1:]
+    if argv and argv[0] == "--compute":
+        # python -m topoptspline3d_amd.rocprof_report --compute [--min-grid G] PATTERN... -- CSV...
+        rest = argv[1:]
+        min_grid = 0
+        if rest[:1] == ["--min-grid"]:
+            min_grid, rest = int(rest[1]), rest[2:]
+        i = rest.index("--")
+        print(compute_table(rest[i + 1:], rest[:i], min_grid))
+        return 0
     ap = argparse.ArgumentParser()
     ap.add_argument("--stats", required=True)
     ap.add_argument("--kernel", required=True, action="append")
