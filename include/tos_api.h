@@ -63,6 +63,7 @@ typedef struct tos_info {
   int coarse_dofs;              /* size of the dense coarsest system                   */
   int n_cases;                  /* load cases                                          */
   int64_t n_coeffs;             /* spline coefficients (0 until tos_spline_init)       */
+  int has_support;              /* 0: no fine node is supported (a floating body)      */
 } tos_info;
 
 typedef struct tos_solve_opts {
@@ -138,10 +139,36 @@ int tos_compliance(tos_ctx* ctx, double* c);
 /* sum(rho)/N_active over active elements (Grid::volumeRatio, grid/Grid.cu:5462-5470). */
 int tos_volume(tos_ctx* ctx, double* v);
 
+/* ---- floating bodies ----------------------------------------------------------------
+ * A domain whose fine level has no supported node is a floating body: K(rho) has the six
+ * rigid motions as its null space (Grid::hasSupport false, grid/Grid.h:730).  Its
+ * coarsest level is factored with alpha R R^T added (R the orthonormalised rigid motions
+ * of the coarsest nodes, alpha the mean diagonal), so that for a right-hand side
+ * orthogonal to them the coarsest solve returns the minimum-norm solution, as the
+ * reference's BDCSVD solve does (grid/Grid.cpp:3654-3661, 4072); the loads of a solve must
+ * then be balanced (tos_balance_loads), or the FCG stalls at their unbalanced part.
+ *
+ * forceProject (optimization/projection.cpp:289-350, optimization/projection.cu:193-250)
+ * on load case k: the force is kept on the n given load nodes only (zero elsewhere); with
+ * normals, only its component along each node's normal is kept; on a floating context its
+ * rigid-motion components over those nodes (of the normal components, when normals are
+ * given) are removed as well, so that its resultant force and moment vanish.  Rigid columns
+ * whose norm after orthogonalisation falls below 1e-6 of their own are dropped, as in the
+ * reference's Gram-Schmidt (projection.cpp:234-255).  Nodes: lexicographic ids of free
+ * fine nodes, no repeats. */
+int tos_balance_loads(tos_ctx* ctx, int k, int n, const int64_t* nodes, const double* normals);
+/* displacementProject (optimization/projection.cpp:374-395): removes the rigid-motion
+ * components of load case k's displacement over the active nodes.  Floating contexts only
+ * (TOS_E_STATE otherwise). */
+int tos_remove_rigid(tos_ctx* ctx, int k);
+
 /* ---- worst-case load (modified power method) --------------------------------------
  * Replaces modifiedPM (optimization/optimization.cpp:213-351) with forceProject
- * (optimization/projection.cpp:289-350, optimization/projection.cu:193-250) on a grid
- * with supports.  The force lives on the load nodes only.  Start: a random force on the
+ * (optimization/projection.cpp:289-350, optimization/projection.cu:193-250).  On a
+ * floating context the force update also removes the force's rigid-motion components on
+ * the load nodes (as tos_balance_loads) and every step removes the rigid motion of u over
+ * the active nodes after the V-cycle (displacementProject, optimization.cpp:282-285).
+ * The force lives on the load nodes only.  Start: a random force on the
  * load nodes, projected and scaled to unit 2-norm, u = 0.  Each step runs one
  * V(pre,post)-cycle on the displacement of load case 0 (u += B(f - K u)), takes the new
  * force from u on the load nodes (its component along each node's normal when normals

@@ -68,6 +68,8 @@ struct Api {
   TOS_FN(int, mma_update, tos_mma*, const double*, const double*, const double*, int*)
   TOS_FN(int, ctx_mma_step, tos_ctx*, tos_mma*, double, double, double, int*)
   TOS_FN(int, worst_compliance, tos_ctx*, const tos_worst_opts*, tos_worst_stats*)
+  TOS_FN(int, balance_loads, tos_ctx*, int, int, const int64_t*, const double*)
+  TOS_FN(int, remove_rigid, tos_ctx*, int)
   TOS_FN(int, get_loads, tos_ctx*, int, double*)
   TOS_FN(int, oc_update_projected, tos_ctx*, double, double, double, double, double*, int*, double*)
   TOS_FN(int, surface_constraints, tos_ctx*, const tos_surface_opts*, tos_surface_stats*)
@@ -98,7 +100,7 @@ struct Api {
     TOS_SYM(projection_sensitivity) TOS_SYM(spline_init) TOS_SYM(spline_set_coeffs) TOS_SYM(spline_get_coeffs)
     TOS_SYM(spline_eval) TOS_SYM(spline_adjoint) TOS_SYM(oc_update_coeffs) TOS_SYM(mma_create)
     TOS_SYM(mma_destroy) TOS_SYM(mma_set_x) TOS_SYM(mma_get_x) TOS_SYM(mma_update) TOS_SYM(ctx_mma_step)
-    TOS_SYM(worst_compliance) TOS_SYM(get_loads) TOS_SYM(oc_update_projected) TOS_SYM(surface_constraints)
+    TOS_SYM(worst_compliance) TOS_SYM(balance_loads) TOS_SYM(remove_rigid) TOS_SYM(get_loads) TOS_SYM(oc_update_projected) TOS_SYM(surface_constraints)
     TOS_SYM(spline_get_coeff_sens) TOS_SYM(ctx_mma_step_surface)
 #undef TOS_SYM
     return a;
@@ -279,6 +281,21 @@ class HierarchyGrid {
   int n_grid() const { return (int)grids_.size(); }
   Grid& operator[](int i) { return grids_.at(i); }
   int last_iterations() const { return last_iterations_; }
+  // Grid::hasSupport (grid/Grid.h:730): false for a floating body
+  bool hasSupport() const {
+    tos_info i;
+    tos::check(tos::Api::get().get_info(c_, &i), c_, "get_info");
+    return i.has_support != 0;
+  }
+  // forceProject (optimization/projection.cpp:289-350) of load case 0 over the load region
+  // given to setLoadNodes
+  void forceProject() {
+    tos::check(tos::Api::get().balance_loads(c_, 0, (int)load_nodes_.size(), load_nodes_.data(),
+                                             load_normals_.empty() ? nullptr : load_normals_.data()),
+               c_, "balance_loads");
+  }
+  // displacementProject (optimization/projection.cpp:374-395), floating bodies only
+  void displacementProject() { tos::check(tos::Api::get().remove_rigid(c_, 0), c_, "remove_rigid"); }
   tos_ctx* ctx() const { return c_; }
 
  private:

@@ -208,6 +208,170 @@ static void build_hierarchy(Ctx& c, int n_levels, int threshold) {
     L.f.assign(3 * L.nn, 0);
     L.r.assign(3 * L.nn, 0);
   }
+  // a body with no supported node floats (Grid::hasSupport, grid/Grid.h:730)
+  const Level& F0 = c.L[0];
+  bool any_free = false, any_support = false;
+  for (int64_t n = 0; n < F0.nn; ++n) {
+    any_free |= F0.fre[n] != 0;
+    if (c.support.size() && c.support[n]) {
+      // a supported node counts when it touches an active element
+      const int x = (int)(n % F0.nx), y = (int)((n / F0.nx) % F0.ny), z = (int)(n / ((int64_t)F0.nx * F0.ny));
+      for (int a = 0; a < 8; ++a) {
+        int ex = x - 1 + (a & 1), ey = y - 1 + ((a >> 1) & 1), ez = z - 1 + (a >> 2);
+        if (ex < 0 || ey < 0 || ez < 0 || ex >= F0.ex || ey >= F0.ey || ez >= F0.ez) continue;
+        any_support |= F0.eact[F0.eid(ex, ey, ez)] != 0;
+      }
+    }
+  }
+  c.floating = any_free && !any_support;
+}
+
+// ------------------------------------------------------------------------------------
+// Rigid motions (optimization/projection.cpp).  Column k of the rigid basis at a node at
+// p: k < 3 the unit translation e_k, k >= 3 column k-3 of p^ = [[0,-p2,p1],[p2,0,-p0],
+// [-p1,p0,0]] (projection.cpp:70-77, 212-219).
+// ------------------------------------------------------------------------------------
+static void rigid_column(int k, const double p[3], double out[3]) {
+  out[0] = out[1] = out[2] = 0;
+  if (k < 3) { out[k] = 1; return; }
+  const double P[3][3] = {{0, -p[2], p[1]}, {p[2], 0, -p[0]}, {-p[1], p[0], 0}};
+  for (int r = 0; r < 3; ++r) out[r] = P[r][k - 3];
+}
+
+// Gram-Schmidt over the columns of R (n rows, 6 columns, column-major) in order; with
+// `tol` a column whose residual norm is below 1e-6 of its norm, or whose norm is below
+// 1e-6, is dropped (projection.cpp:234-262), else every column is kept (:90-96).
+// Returns the kept columns, orthonormal, compacted to the front.
+static int gram_schmidt(std::vector<double>& R, int64_t n, bool tol) {
+  bool invalid[6] = {false, false, false, false, false, false};
+  for (int i = 0; i < 6; ++i) {
+    double* ci = &R[(size_t)i * n];
+    double old2 = 0;
+    for (int64_t t = 0; t < n; ++t) old2 += ci[t] * ci[t];
+    for (int j = 0; j < i; ++j) {
+      if (invalid[j]) continue;
+      const double* cj = &R[(size_t)j * n];
+      double w = 0;
+      for (int64_t t = 0; t < n; ++t) w += ci[t] * cj[t];
+      for (int64_t t = 0; t < n; ++t) ci[t] -= w * cj[t];
+    }
+    double r2 = 0;
+    for (int64_t t = 0; t < n; ++t) r2 += ci[t] * ci[t];
+    const double res = std::sqrt(r2), oldres = std::sqrt(old2);
+    if (tol && (res / oldres < 1e-6 || oldres < 1e-6)) { invalid[i] = true; continue; }
+    for (int64_t t = 0; t < n; ++t) ci[t] /= res;
+  }
+  int m = 0;
+  for (int i = 0; i < 6; ++i) {
+    if (invalid[i]) continue;
+    if (m != i) std::copy(R.begin() + (size_t)i * n, R.begin() + (size_t)(i + 1) * n, R.begin() + (size_t)m * n);
+    ++m;
+  }
+  return m;
+}
+
+// displacementProject (projection.cpp:374-395) with the basis of setNodes (:45-103): the
+// rigid motions at the lattice coordinates of every active fine node, orthonormalised,
+// removed from u one column after another
+static void displacement_project(const Ctx& c, double* u) {
+  const Level& L = c.L[0];
+  std::vector<int64_t> ids;
+  for (int64_t i = 0; i < L.nn; ++i)
+    if (L.fre[i]) ids.push_back(i);
+  const int64_t n = 3 * (int64_t)ids.size();
+  std::vector<double> R(6 * (size_t)n);
+  for (size_t q = 0; q < ids.size(); ++q) {
+    const int64_t i = ids[q];
+    const double p[3] = {(double)(i % L.nx), (double)((i / L.nx) % L.ny), (double)(i / ((int64_t)L.nx * L.ny))};
+    for (int k = 0; k < 6; ++k) {
+      double col[3];
+      rigid_column(k, p, col);
+      for (int r = 0; r < 3; ++r) R[(size_t)k * n + 3 * q + r] = col[r];
+    }
+  }
+  const int m = gram_schmidt(R, n, false);
+  for (int k = 0; k < m; ++k) {
+    const double* ck = &R[(size_t)k * n];
+    double s = 0;
+    for (size_t q = 0; q < ids.size(); ++q)
+      for (int r = 0; r < 3; ++r) s += u[r * L.nn + ids[q]] * ck[3 * q + r];
+    for (size_t q = 0; q < ids.size(); ++q)
+      for (int r = 0; r < 3; ++r) u[r * L.nn + ids[q]] -= s * ck[3 * q + r];
+  }
+}
+
+// forceProject on the load-node forces fs (node-major, 3 per node) (projection.cpp:129-268
+// for the bases, :300-345 for the projection): Nd holds two unit tangents per node when
+// the direction is constrained; Rd the rigid motions at the load nodes, minus their
+// tangential part when constrained, orthonormalised with the drop rule; then
+//   free force, no support:         f -= Rd Rd^T f
+//   free force, with support:       f unchanged
+//   constrained, no support:        f -= Nd^T Nd f + Rd Rd^T f
+//   constrained, with support:      f -= Nd^T Nd f
+static void force_project(const Ctx& c, int n, const int64_t* nodes, const double* nrm, double* fs) {
+  const Level& L = c.L[0];
+  const bool constrained = nrm != nullptr, support = !c.floating;
+  std::vector<double> T(6 * (size_t)n, 0.0);  // two tangents per node
+  if (constrained)
+    for (int i = 0; i < n; ++i) {
+      const double* nv = nrm + 3 * (size_t)i;
+      // first tangent: the axis least aligned with the normal, orthogonalised; second: n x t0
+      int a = 0;
+      for (int k = 1; k < 3; ++k)
+        if (std::fabs(nv[k]) < std::fabs(nv[a])) a = k;
+      double t0[3] = {0, 0, 0};
+      t0[a] = 1;
+      const double d = t0[0] * nv[0] + t0[1] * nv[1] + t0[2] * nv[2];
+      for (int k = 0; k < 3; ++k) t0[k] -= d * nv[k];
+      const double l0 = std::sqrt(t0[0] * t0[0] + t0[1] * t0[1] + t0[2] * t0[2]);
+      for (int k = 0; k < 3; ++k) t0[k] /= l0;
+      const double t1[3] = {nv[1] * t0[2] - nv[2] * t0[1], nv[2] * t0[0] - nv[0] * t0[2], nv[0] * t0[1] - nv[1] * t0[0]};
+      for (int k = 0; k < 3; ++k) {
+        T[6 * (size_t)i + k] = t0[k];
+        T[6 * (size_t)i + 3 + k] = t1[k];
+      }
+    }
+  auto tangential = [&](int i, const double* v, double out[3]) {  // Nd^T Nd v at node i
+    const double* t0 = &T[6 * (size_t)i];
+    const double* t1 = t0 + 3;
+    const double w0 = t0[0] * v[0] + t0[1] * v[1] + t0[2] * v[2];
+    const double w1 = t1[0] * v[0] + t1[1] * v[1] + t1[2] * v[2];
+    for (int k = 0; k < 3; ++k) out[k] = w0 * t0[k] + w1 * t1[k];
+  };
+  const int64_t m3 = 3 * (int64_t)n;
+  std::vector<double> Rd;
+  int m = 0;
+  if (!support) {
+    Rd.assign(6 * (size_t)m3, 0.0);
+    for (int i = 0; i < n; ++i) {
+      const int64_t id = nodes[i];
+      const double p[3] = {(double)(id % L.nx), (double)((id / L.nx) % L.ny), (double)(id / ((int64_t)L.nx * L.ny))};
+      for (int k = 0; k < 6; ++k) {
+        double col[3];
+        rigid_column(k, p, col);
+        if (constrained) {
+          double tg[3];
+          tangential(i, col, tg);
+          for (int r = 0; r < 3; ++r) col[r] -= tg[r];
+        }
+        for (int r = 0; r < 3; ++r) Rd[(size_t)k * m3 + 3 * i + r] = col[r];
+      }
+    }
+    m = gram_schmidt(Rd, m3, true);
+  }
+  std::vector<double> w(m, 0.0);
+  for (int k = 0; k < m; ++k)
+    for (int64_t t = 0; t < m3; ++t) w[k] += Rd[(size_t)k * m3 + t] * fs[t];
+  std::vector<double> out(fs, fs + m3);
+  if (constrained)
+    for (int i = 0; i < n; ++i) {
+      double tg[3];
+      tangential(i, fs + 3 * (size_t)i, tg);
+      for (int r = 0; r < 3; ++r) out[3 * (size_t)i + r] -= tg[r];
+    }
+  for (int k = 0; k < m; ++k)
+    for (int64_t t = 0; t < m3; ++t) out[t] -= Rd[(size_t)k * m3 + t] * w[k];
+  std::copy(out.begin(), out.end(), fs);
 }
 
 // ------------------------------------------------------------------------------------
@@ -568,6 +732,32 @@ static int coarse_factor(Ctx& c) {
     }
   std::vector<double> diag0(n);
   for (int j = 0; j < n; ++j) diag0[j] = A[(size_t)j * n + j];
+  if (c.floating) {
+    // null space of a floating body: factor A + alpha R R^T, R the orthonormalised rigid
+    // motions of the coarsest nodes, alpha the mean diagonal; for a right-hand side
+    // orthogonal to R this gives A's minimum-norm solution, which the reference's BDCSVD
+    // solve returns (grid/Grid.cpp:3654-3661, 4072)
+    std::vector<double> R(6 * (size_t)n, 0.0);
+    for (size_t q = 0; q < c.cnode.size(); ++q) {
+      const int64_t i = c.cnode[q];
+      const double p[3] = {(double)(i % L.nx), (double)((i / L.nx) % L.ny), (double)(i / ((int64_t)L.nx * L.ny))};
+      for (int k = 0; k < 6; ++k) {
+        double col[3];
+        rigid_column(k, p, col);
+        for (int r = 0; r < 3; ++r) R[(size_t)k * n + 3 * q + r] = col[r];
+      }
+    }
+    const int m = gram_schmidt(R, n, true);
+    double alpha = 0;
+    for (int j = 0; j < n; ++j) alpha += diag0[j] / n;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) {
+        double t = 0;
+        for (int k = 0; k < m; ++k) t += R[(size_t)k * n + i] * R[(size_t)k * n + j];
+        A[(size_t)i * n + j] += alpha * t;
+      }
+  }
   for (int j = 0; j < n; ++j) {
     double d = A[(size_t)j * n + j];
     for (int k = 0; k < j; ++k) d -= A[(size_t)j * n + k] * A[(size_t)j * n + k];
@@ -1210,6 +1400,7 @@ ORC_API int orc_get_info(tos_ctx* c, tos_info* o) {
   o->coarse_dofs = c->L.size() > 1 ? (int)(3 * nfc) : 0;
   o->n_cases = c->K;
   o->n_coeffs = (int64_t)c->ncx * c->ncy * c->ncz;
+  o->has_support = c->floating ? 0 : 1;
   return TOS_OK;
 }
 
@@ -1412,16 +1603,12 @@ ORC_API int orc_worst_compliance(tos_ctx* c, const tos_worst_opts* o, tos_worst_
   auto t0 = std::chrono::steady_clock::now();
   double* u = c->U.data();  // load case 0
   double* f = c->F.data();
-  // g <- projection of g, then f = g / |g| on the load nodes (0 elsewhere); returns |g|^2
+  // g <- forceProject(g), then f = g / |g| on the load nodes (0 elsewhere); returns |g|^2
   auto set_force = [&]() {
+    force_project(*c, n, o->nodes, o->normals ? nrm.data() : nullptr, g.data());
     double g2 = 0;
     for (int i = 0; i < n; ++i) {
-      double* gi = &g[3 * (size_t)i];
-      if (o->normals) {
-        const double* ni = &nrm[3 * (size_t)i];
-        const double d = ni[0] * gi[0] + ni[1] * gi[1] + ni[2] * gi[2];
-        for (int k = 0; k < 3; ++k) gi[k] = d * ni[k];
-      }
+      const double* gi = &g[3 * (size_t)i];
       g2 += gi[0] * gi[0] + gi[1] * gi[1] + gi[2] * gi[2];
     }
     if (!(g2 > 0) || !std::isfinite(g2)) return g2;
@@ -1446,6 +1633,7 @@ ORC_API int orc_worst_compliance(tos_ctx* c, const tos_worst_opts* o, tos_worst_
     for (int64_t i = 0; i < n3; ++i) r[i] = f[i] - q[i];
     precondition(*c, r.data(), z.data(), o->pre, o->post);
     for (int64_t i = 0; i < n3; ++i) u[i] += z[i];
+    if (c->floating) displacement_project(*c, u);  // optimization/optimization.cpp:282-285
     apply_fine(*c, u, q.data());
     for (int64_t i = 0; i < n3; ++i) r[i] = f[i] - q[i];
     rel = std::sqrt(dot3(L, r.data(), r.data()) / dot3(L, f, f));
@@ -1474,6 +1662,38 @@ ORC_API int orc_worst_compliance(tos_ctx* c, const tos_worst_opts* o, tos_worst_
     st->rel_res = rel;
     st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
+  return TOS_OK;
+}
+
+ORC_API int orc_balance_loads(tos_ctx* c, int k, int n, const int64_t* nodes, const double* normals) {
+  if (!c || k < 0 || k >= c->K || n <= 0 || !nodes) return TOS_E_ARG;
+  const Level& L = c->L[0];
+  std::vector<uint8_t> seen(L.nn, 0);
+  std::vector<double> nrm(3 * (size_t)n), fs(3 * (size_t)n);
+  double* f = c->F.data() + (size_t)k * 3 * L.nn;
+  for (int i = 0; i < n; ++i) {
+    const int64_t id = nodes[i];
+    if (id < 0 || id >= L.nn || !L.fre[id] || seen[id]) { c->err = "bad load node"; return TOS_E_ARG; }
+    seen[id] = 1;
+    if (normals) {
+      const double* a = normals + 3 * (size_t)i;
+      const double len = std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+      if (!(len > 0) || !std::isfinite(len)) { c->err = "bad normal"; return TOS_E_ARG; }
+      for (int r = 0; r < 3; ++r) nrm[3 * (size_t)i + r] = a[r] / len;
+    }
+    for (int r = 0; r < 3; ++r) fs[3 * (size_t)i + r] = f[r * L.nn + id];
+  }
+  force_project(*c, n, nodes, normals ? nrm.data() : nullptr, fs.data());
+  std::fill(f, f + 3 * L.nn, 0.0);
+  for (int i = 0; i < n; ++i)
+    for (int r = 0; r < 3; ++r) f[r * L.nn + nodes[i]] = fs[3 * (size_t)i + r];
+  return TOS_OK;
+}
+
+ORC_API int orc_remove_rigid(tos_ctx* c, int k) {
+  if (!c || k < 0 || k >= c->K) return TOS_E_ARG;
+  if (!c->floating) { c->err = "the context has supports: no rigid motions to remove"; return TOS_E_STATE; }
+  displacement_project(*c, c->U.data() + (size_t)k * 3 * c->L[0].nn);
   return TOS_OK;
 }
 

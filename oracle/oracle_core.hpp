@@ -64,6 +64,7 @@ struct Ctx {
   std::vector<double> F, U; // K * 3 * nn0
   int64_t n_active = 0;
   bool op_ready = false;
+  bool floating = false;    // no supported fine node: the rigid motions are K's null space
   // coarsest dense system
   std::vector<int64_t> cnode;   // free coarsest nodes
   std::vector<int64_t> cmap;    // node -> compact index or -1

@@ -147,7 +147,7 @@ def test_zero_and_garbage_networks_fall_back_to_plain_mg(hip):
     parameters proposes garbage; the guard keeps only energy-reducing steps in the
     accepted window, and flexible CG still converges to the same solution."""
     from topoptspline3d_amd import learned as LN
-    solid, support, f = cantilever(32, 16, 16)
+    solid, support, f = cantilever(64, 32, 32)  # 4 levels: level 2 has one below
     ctx = Context(hip, solid, support)
     ctx.set_material(1e6, 0.4, 3.0, 1e-3)
     ctx.set_loads(f)
@@ -179,7 +179,7 @@ def test_precision_3_refuses_an_attached_network(hip):
     network attached it returns TOS_E_ARG; after detaching it solves."""
     from topoptspline3d_amd import learned as LN
     from topoptspline3d_amd.context import TosError
-    solid, support, f = cantilever(32, 16, 16)
+    solid, support, f = cantilever(64, 32, 32)  # 4 levels: level 2 has one below
     ctx = Context(hip, solid, support)
     ctx.set_material(1e6, 0.4, 3.0, 1e-3)
     ctx.set_loads(f)

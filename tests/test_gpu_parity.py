@@ -201,7 +201,9 @@ def test_c0_trajectory_matches_oracle(hip):
 
 def test_solver_errors_are_reported(hip):
     solid, support, f = cantilever(16, 8, 8)  # two levels
-    c = Context(hip, solid, np.zeros_like(support))  # no supports: singular
+    one = np.zeros_like(support)
+    one[0, 0, 0] = 1  # one supported node: singular (rotations about it), and not a floating body
+    c = Context(hip, solid, one)
     c.set_material(E, NU, P, RMIN)
     c.set_loads(f)
     c.set_density(np.full(solid.size, 0.5, np.float32))

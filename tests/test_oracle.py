@@ -403,7 +403,9 @@ def test_oc_on_spline_coefficients_hits_volume():
 
 def test_singular_and_nan_inputs_are_reported():
     solid, support, f = cantilever(16, 8, 8)
-    c = Context(oracle_api(), solid, np.zeros_like(support))
+    one = np.zeros_like(support)
+    one[0, 0, 0] = 1  # one supported node: the rotations about it stay free (not a floating body)
+    c = Context(oracle_api(), solid, one)
     c.set_material(E, NU, P, RMIN)
     c.set_loads(f)
     c.set_density(np.full(solid.size, 0.5, np.float32))

@@ -23,7 +23,8 @@ class Info(C.Structure):
     _fields_ = [("n_levels", C.c_int), ("level_dims", (C.c_int * 3) * 16),
                 ("n_active_elements", C.c_int64 * 16), ("n_free_nodes", C.c_int64 * 16),
                 ("n_elements", C.c_int64), ("n_nodes", C.c_int64),
-                ("coarse_dofs", C.c_int), ("n_cases", C.c_int), ("n_coeffs", C.c_int64)]
+                ("coarse_dofs", C.c_int), ("n_cases", C.c_int), ("n_coeffs", C.c_int64),
+                ("has_support", C.c_int)]
 
 
 class SolveOpts(C.Structure):
@@ -92,6 +93,8 @@ SIGNATURES = {
     "compliance": (C.c_int, [vp, f64p]),
     "volume": (C.c_int, [vp, f64p]),
     "worst_compliance": (C.c_int, [vp, C.POINTER(WorstOpts), C.POINTER(WorstStats)]),
+    "balance_loads": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
+    "remove_rigid": (C.c_int, [vp, C.c_int]),
     "sensitivity": (C.c_int, [vp, C.c_int]),
     "get_sensitivity": (C.c_int, [vp, C.c_int, f32p]),
     "set_sensitivity": (C.c_int, [vp, C.c_int, f32p]),

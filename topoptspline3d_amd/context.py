@@ -84,6 +84,7 @@ class Context:
             "n_free_nodes": [i.n_free_nodes[l] for l in range(n)],
             "n_elements": i.n_elements, "n_nodes": i.n_nodes,
             "coarse_dofs": i.coarse_dofs, "n_cases": i.n_cases, "n_coeffs": i.n_coeffs,
+            "has_support": bool(i.has_support),
         }
 
     def node_flags(self, level=0):
@@ -221,6 +222,21 @@ class Context:
             self._chk(rc, "worst_compliance")
         return {"rc": rc, "iterations": st.iterations, "compliance": st.compliance, "fch": st.fch,
                 "rel_res": st.rel_res, "seconds": st.seconds}
+
+    def balance_loads(self, nodes, normals=None, k=0):
+        """forceProject (optimization/projection.cpp:289-350): keep load case k's force on the
+        given nodes only, along the normals when given, and on a floating body without its
+        resultant force and moment"""
+        nodes = np.ascontiguousarray(nodes, dtype=np.int64).ravel()
+        nrm = None
+        if normals is not None:
+            nrm = np.ascontiguousarray(normals, dtype=np.float64).reshape(nodes.size, 3)
+        self._chk(self.api.balance_loads(self.h, k, nodes.size, _p(nodes, C.c_int64),
+                                         _p(nrm, C.c_double) if nrm is not None else None), "balance_loads")
+
+    def remove_rigid(self, k=0):
+        """displacementProject (optimization/projection.cpp:374-395), floating bodies only"""
+        self._chk(self.api.remove_rigid(self.h, k), "remove_rigid")
 
     def sensitivity(self, k=0):
         self._chk(self.api.sensitivity(self.h, k), "sensitivity")

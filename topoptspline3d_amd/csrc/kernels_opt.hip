@@ -701,6 +701,39 @@ __global__ void k_unpad_vec(LevelView L, const double* __restrict__ src, double*
 // leaves the same normal component.  Grid-stride loops over a fixed grid: every block
 // writes one partial, so the sums do not depend on timing.
 // ---------------------------------------------------------------------------------------
+// R^T v at one node (b += ...) and the rigid field R w at one node, for the columns
+// t_x, t_y, t_z, (0, z, -y), (-z, 0, x), (y, -x, 0) at coordinates (px, py, pz)
+__device__ __forceinline__ void rigid_dot(double px, double py, double pz, const double v[3], double b[6]) {
+  b[0] += v[0];
+  b[1] += v[1];
+  b[2] += v[2];
+  b[3] += pz * v[1] - py * v[2];
+  b[4] += px * v[2] - pz * v[0];
+  b[5] += py * v[0] - px * v[1];
+}
+__device__ __forceinline__ void rigid_field(double px, double py, double pz, const double w[6], double c[3]) {
+  c[0] = w[0] - pz * w[4] + py * w[5];
+  c[1] = w[1] + pz * w[3] - px * w[5];
+  c[2] = w[2] - py * w[3] + px * w[4];
+}
+// w = M b with b = scal[S_RIG0 .. S_RIG0 + 5]
+__device__ __forceinline__ void rigid_weights(const RigidProj& P, const double* __restrict__ scal, double w[6]) {
+  double b[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) b[k] = scal[S_RIG0 + k];
+#pragma unroll
+  for (int r = 0; r < 6; ++r) {
+    double a = 0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) a = fma(P.M[r * 6 + k], b[k], a);
+    w[r] = a;
+  }
+}
+
+// the load-node candidate g_i: u (or src_nm, node-major) at the node, along its normal when
+// normals are given; with `pos` (floating contexts) minus its rigid-motion part, whose
+// weights come from R^T g in scal (k_worst_rig_moments), the part taken along the normal
+// as well
 __device__ __forceinline__ void worst_candidate(int i, const int64_t* __restrict__ pidx,
                                                 const double* __restrict__ nrm, int64_t NP,
                                                 const double* __restrict__ u, const double* __restrict__ src_nm,
@@ -717,16 +750,134 @@ __device__ __forceinline__ void worst_candidate(int i, const int64_t* __restrict
     g[0] = d * nx; g[1] = d * ny; g[2] = d * nz;
   }
 }
+__device__ __forceinline__ void worst_candidate_bal(int i, const int64_t* __restrict__ pidx,
+                                                    const double* __restrict__ nrm, int64_t NP,
+                                                    const double* __restrict__ u,
+                                                    const double* __restrict__ src_nm,
+                                                    const double* __restrict__ pos, const RigidProj& P,
+                                                    const double* __restrict__ scal, double g[3]) {
+  worst_candidate(i, pidx, nrm, NP, u, src_nm, g);
+  if (!pos) return;
+  double w[6], c[3];
+  rigid_weights(P, scal, w);
+  rigid_field(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], w, c);
+  if (nrm) {
+    const double nx = nrm[3 * i], ny = nrm[3 * i + 1], nz = nrm[3 * i + 2];
+    const double d = fma(nz, c[2], fma(ny, c[1], nx * c[0]));
+    c[0] = d * nx; c[1] = d * ny; c[2] = d * nz;
+  }
+  g[0] -= c[0]; g[1] -= c[1]; g[2] -= c[2];
+}
+
+// partials (6 sets) of R^T g over the load nodes, g the candidate before the rigid part
+// is removed (positions relative to the projector's origin)
+__global__ __launch_bounds__(256) void k_worst_rig_moments(int n, const int64_t* __restrict__ pidx,
+                                                           const double* __restrict__ nrm,
+                                                           const double* __restrict__ pos, int64_t NP,
+                                                           const double* __restrict__ u,
+                                                           const double* __restrict__ src_nm,
+                                                           double* __restrict__ partial) {
+  double b[6] = {0, 0, 0, 0, 0, 0};
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    double g[3];
+    worst_candidate(i, pidx, nrm, NP, u, src_nm, g);
+    rigid_dot(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], g, b);
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) block_partial(b[k], partial, k);
+}
+
+// partials (6 sets) of R^T u over the free nodes of a chunk list (fixed grid, chunks
+// strided over the blocks' waves)
+__global__ __launch_bounds__(256) void k_rigid_moments(const int4* __restrict__ cl, int nch, int64_t NP,
+                                                       const double* __restrict__ u, RigidProj P,
+                                                       double* __restrict__ partial) {
+  double b[6] = {0, 0, 0, 0, 0, 0};
+  const int lane = threadIdx.x & 63;
+  for (int ch = blockIdx.x * 4 + (threadIdx.x >> 6); ch < nch; ch += gridDim.x * 4) {
+    NodeRef nd;
+    if (!chunk_node_at(cl, nch, ch, lane, 1, nd)) continue;
+    const double v[3] = {u[nd.p], u[NP + nd.p], u[2 * NP + nd.p]};
+    rigid_dot(nd.x - P.o[0], nd.y - P.o[1], nd.z - P.o[2], v, b);
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) block_partial(b[k], partial, k);
+}
+
+// scal[S_RIG0 + k] = sum of partial set k (np partials per set)
+__global__ __launch_bounds__(256) void k_fold6(const double* __restrict__ partial, int np,
+                                               double* __restrict__ scal) {
+  for (int k = 0; k < 6; ++k) {
+    double s = 0;
+    for (int i = threadIdx.x; i < np; i += 256) s += partial[(int64_t)k * np + i];
+    s = block_sum256(s);
+    if (threadIdx.x == 0) scal[S_RIG0 + k] = s;
+  }
+}
+
+// u -= R (M b) over the free nodes of a chunk list (displacementProject)
+__global__ __launch_bounds__(256) void k_rigid_sub(const int4* __restrict__ cl, int nch, int64_t NP,
+                                                   double* __restrict__ u, RigidProj P,
+                                                   const double* __restrict__ scal) {
+  NodeRef nd;
+  if (!chunk_node(cl, nch, 1, nd)) return;
+  double w[6], c[3];
+  rigid_weights(P, scal, w);
+  rigid_field(nd.x - P.o[0], nd.y - P.o[1], nd.z - P.o[2], w, c);
+  u[nd.p] -= c[0];
+  u[NP + nd.p] -= c[1];
+  u[2 * NP + nd.p] -= c[2];
+}
+
+// gather the 3-vectors of the load nodes (node-major)
+__global__ __launch_bounds__(256) void k_gather_load(int n, const int64_t* __restrict__ pidx, int64_t NP,
+                                                     const double* __restrict__ v, double* __restrict__ out) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int64_t p = pidx[i];
+  out[3 * i] = v[p];
+  out[3 * i + 1] = v[NP + p];
+  out[3 * i + 2] = v[2 * NP + p];
+}
+
+// forceProject's write-back: f = projected candidate on the load nodes (f is zero elsewhere)
+__global__ __launch_bounds__(256) void k_balance_apply(int n, const int64_t* __restrict__ pidx,
+                                                       const double* __restrict__ nrm,
+                                                       const double* __restrict__ pos, RigidProj P,
+                                                       const double* __restrict__ scal, int64_t NP,
+                                                       const double* __restrict__ src_nm, double* __restrict__ f) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  double g[3];
+  // (u is not read when src_nm is given; a null literal there crashed the device compiler)
+  worst_candidate_bal(i, pidx, nrm, NP, f, src_nm, pos, P, scal, g);
+  const int64_t p = pidx[i];
+  f[p] = g[0];
+  f[NP + p] = g[1];
+  f[2 * NP + p] = g[2];
+}
+
+// coarsest system of a floating context: A += alpha R R^T (R: m orthonormal columns of
+// length n, column k at R[k * n])
+__global__ void k_add_lowrank(double* __restrict__ A, int n, const double* __restrict__ R, int m, double alpha) {
+  const int64_t t = thread_id();
+  if (t >= (int64_t)n * n) return;
+  const int i = (int)(t / n), j = (int)(t % n);
+  double s = 0;
+  for (int k = 0; k < m; ++k) s = fma(R[(int64_t)k * n + i], R[(int64_t)k * n + j], s);
+  A[t] = fma(alpha, s, A[t]);
+}
 
 // partial[b] = sum of |g_i|^2 over the block's load nodes
 __global__ __launch_bounds__(256) void k_worst_norm(int n, const int64_t* __restrict__ pidx,
                                                     const double* __restrict__ nrm, int64_t NP,
                                                     const double* __restrict__ u, const double* __restrict__ src_nm,
-                                                    double* __restrict__ partial) {
+                                                    const double* __restrict__ pos, RigidProj P,
+                                                    const double* __restrict__ scal, double* __restrict__ partial) {
   double s = 0;
   for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
     double g[3];
-    worst_candidate(i, pidx, nrm, NP, u, src_nm, g);
+    worst_candidate_bal(i, pidx, nrm, NP, u, src_nm, pos, P, scal, g);
     s = fma(g[2], g[2], fma(g[1], g[1], fma(g[0], g[0], s)));
   }
   block_partial(s, partial, 0);
@@ -739,6 +890,7 @@ __global__ __launch_bounds__(256) void k_worst_apply(int n, const int64_t* __res
                                                      const double* __restrict__ nrm, int64_t NP,
                                                      const double* __restrict__ u,
                                                      const double* __restrict__ src_nm,
+                                                     const double* __restrict__ pos, RigidProj P,
                                                      const double* __restrict__ scal, int g2_slot, int first,
                                                      double* __restrict__ f, double* __restrict__ r,
                                                      float* __restrict__ r32, double* __restrict__ fs,
@@ -748,7 +900,7 @@ __global__ __launch_bounds__(256) void k_worst_apply(int n, const int64_t* __res
   double sd = 0, so = 0;
   for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
     double g[3];
-    worst_candidate(i, pidx, nrm, NP, u, src_nm, g);
+    worst_candidate_bal(i, pidx, nrm, NP, u, src_nm, pos, P, scal, g);
     const int64_t p = pidx[i];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {

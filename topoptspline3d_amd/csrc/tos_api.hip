@@ -170,6 +170,112 @@ LevelView make_view(int ex, int ey, int ez) {
 // `threshold` active elements (grid/Grid.cpp:747).  Free nodes: fine = touches an active
 // element and is not supported; coarse = some free fine node lies inside the support of
 // its trilinear basis function.
+// ---------------------------------------------------------------------------------------
+// Rigid motions (floating bodies).  R^T v at one node, host side (the device's rigid_dot).
+// ---------------------------------------------------------------------------------------
+void rigid_dot_h(double px, double py, double pz, const double v[3], double b[6]) {
+  b[0] += v[0];
+  b[1] += v[1];
+  b[2] += v[2];
+  b[3] += pz * v[1] - py * v[2];
+  b[4] += px * v[2] - pz * v[0];
+  b[5] += py * v[0] - px * v[1];
+}
+
+// Gram-Schmidt over the six columns given their Gram matrix G, in column order, dropping
+// a column whose residual norm is below 1e-6 of its norm or whose norm is below 1e-6
+// (the reference's orthonormalisation, optimization/projection.cpp:234-255): the kept
+// columns q_j = R t_j are orthonormal and M = sum_j t_j t_j^T projects, R M R^T v.
+RigidProj rigid_projector(const double G[36], const double o[3]) {
+  RigidProj P{};
+  double T[6][6] = {};
+  bool keep[6] = {};
+  for (int i = 0; i < 6; ++i) {
+    double t[6] = {};
+    t[i] = 1.0;
+    for (int j = 0; j < i; ++j) {
+      if (!keep[j]) continue;
+      double d = 0;  // (R e_i) . q_j
+      for (int k = 0; k < 6; ++k) d += T[j][k] * G[i * 6 + k];
+      for (int k = 0; k < 6; ++k) t[k] -= d * T[j][k];
+    }
+    double r2 = 0;
+    for (int a = 0; a < 6; ++a)
+      for (int b = 0; b < 6; ++b) r2 += t[a] * G[a * 6 + b] * t[b];
+    const double old2 = G[i * 6 + i];
+    keep[i] = old2 >= 1e-12 && r2 > 0 && r2 >= 1e-12 * old2;
+    if (!keep[i]) continue;
+    const double inv = 1.0 / std::sqrt(r2);
+    for (int k = 0; k < 6; ++k) T[i][k] = t[k] * inv;
+    ++P.m;
+  }
+  for (int j = 0; j < 6; ++j)
+    if (keep[j])
+      for (int a = 0; a < 6; ++a)
+        for (int b = 0; b < 6; ++b) P.M[a * 6 + b] += T[j][a] * T[j][b];
+  for (int k = 0; k < 3; ++k) P.o[k] = o[k];
+  return P;
+}
+
+// projector over the marked nodes of an nx*ny*nz lattice (no normals): with coordinates
+// relative to their centroid the Gram matrix is [[N I, 0], [0, sum(|p|^2 I - p p^T)]]
+RigidProj rigid_projector_nodes(const std::vector<uint8_t>& mark, int nx, int ny, int nz) {
+  double s0 = 0, s[3] = {0, 0, 0};
+  for (int z = 0; z < nz; ++z)
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x)
+        if (mark[x + (size_t)nx * (y + (size_t)ny * z)]) {
+          s0 += 1;
+          s[0] += x; s[1] += y; s[2] += z;
+        }
+  const double o[3] = {s[0] / s0, s[1] / s0, s[2] / s0};
+  double S[3][3] = {};
+  for (int z = 0; z < nz; ++z)
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x)
+        if (mark[x + (size_t)nx * (y + (size_t)ny * z)]) {
+          const double p[3] = {x - o[0], y - o[1], z - o[2]};
+          for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) S[a][b] += p[a] * p[b];
+        }
+  double G[36] = {};
+  const double tr = S[0][0] + S[1][1] + S[2][2];
+  for (int a = 0; a < 3; ++a) {
+    G[a * 6 + a] = s0;
+    for (int b = 0; b < 3; ++b) G[(3 + a) * 6 + 3 + b] = (a == b ? tr : 0.0) - S[a][b];
+  }
+  return rigid_projector(G, o);
+}
+
+// projector over a load region: node coordinates `pos` (3 per node, any origin), the
+// 3-vectors taken along the unit normals when given; writes the coordinates relative to
+// the projector's origin into `rel`
+RigidProj rigid_projector_region(int n, const double* pos, const double* nrm, std::vector<double>& rel) {
+  double o[3] = {0, 0, 0};
+  for (int i = 0; i < n; ++i)
+    for (int k = 0; k < 3; ++k) o[k] += pos[3 * (size_t)i + k] / n;
+  rel.resize(3 * (size_t)n);
+  double G[36] = {};
+  for (int i = 0; i < n; ++i) {
+    const double p[3] = {pos[3 * (size_t)i] - o[0], pos[3 * (size_t)i + 1] - o[1], pos[3 * (size_t)i + 2] - o[2]};
+    for (int k = 0; k < 3; ++k) rel[3 * (size_t)i + k] = p[k];
+    // G = sum_i R_i^T N_i R_i: with N_i = n n^T one vector a = R_i^T n, else the three R_i^T e_c
+    for (int c = 0; c < (nrm ? 1 : 3); ++c) {
+      double v[3] = {0, 0, 0};
+      if (nrm) {
+        for (int k = 0; k < 3; ++k) v[k] = nrm[3 * (size_t)i + k];
+      } else {
+        v[c] = 1.0;
+      }
+      double a[6] = {0, 0, 0, 0, 0, 0};
+      rigid_dot_h(p[0], p[1], p[2], v, a);
+      for (int r = 0; r < 6; ++r)
+        for (int q = 0; q < 6; ++q) G[r * 6 + q] += a[r] * a[q];
+    }
+  }
+  return rigid_projector(G, o);
+}
+
 void build_levels(Ctx& c, int n_levels, int threshold) {
   struct HL {
     int ex, ey, ez;
@@ -214,9 +320,15 @@ void build_levels(Ctx& c, int n_levels, int threshold) {
             for (int j = 0; j < 8; ++j)
               L.h_free[(x + (j & 1)) + (size_t)v.nx * ((y + ((j >> 1) & 1)) + (size_t)v.ny * (z + (j >> 2)))] = 1;
           }
+      int64_t n_act_nodes = 0;
+      for (auto a : L.h_free) n_act_nodes += a;
       if (!c.h_support.empty())
         for (size_t i = 0; i < L.h_free.size(); ++i)
           if (c.h_support[i]) L.h_free[i] = 0;
+      int64_t n_free0 = 0;
+      for (auto a : L.h_free) n_free0 += a;
+      c.floating = n_free0 > 0 && n_free0 == n_act_nodes;
+      if (c.floating) c.rig_all = rigid_projector_nodes(L.h_free, v.nx, v.ny, v.nz);
     } else {
       const Level& F = c.L[l - 1];
       const LevelView& fv = F.v;
@@ -1584,6 +1696,7 @@ int solve_batched(Ctx& c, const tos_solve_opts& o, tos_solve_stats* st) {
 void coarse_alloc(Ctx& c) {
   Level& L = c.L.back();
   std::vector<int64_t> nodes;
+  std::vector<int> xyz;
   std::vector<int> cmap(L.v.NP, -1);
   for (int z = 0; z < L.v.nz; ++z)
     for (int y = 0; y < L.v.ny; ++y)
@@ -1591,6 +1704,7 @@ void coarse_alloc(Ctx& c) {
         if (L.h_free[x + (size_t)L.v.nx * (y + (size_t)L.v.ny * z)]) {
           cmap[L.v.pnode(x, y, z)] = (int)nodes.size();
           nodes.push_back(L.v.pnode(x, y, z));
+          xyz.push_back(x); xyz.push_back(y); xyz.push_back(z);
         }
   c.cn = 3 * (int)nodes.size();
   c.cidx = dalloc<int64_t>(c, nodes.size());
@@ -1600,6 +1714,45 @@ void coarse_alloc(Ctx& c) {
   c.cinv = dalloc<double>(c, (size_t)c.cn * c.cn);
   c.cwork = dalloc<double>(c, (size_t)c.cn * kGJB * 2 + kGJB * kGJB + c.cn);
   c.cflag = dalloc<int>(c, 1);
+  if (c.floating && c.cn) {
+    // the rigid motions of the coarsest nodes (coordinates relative to their centroid; a
+    // coarse level's nodes sit at 2^l times their indices on the fine lattice, the same
+    // span), orthonormalised by modified Gram-Schmidt; dropped columns as the reference's
+    const int n = c.cn, nq = n / 3;
+    double o[3] = {0, 0, 0};
+    for (int q = 0; q < nq; ++q)
+      for (int k = 0; k < 3; ++k) o[k] += (double)xyz[3 * q + k] / nq;
+    std::vector<double> R(6 * (size_t)n, 0.0);
+    for (int q = 0; q < nq; ++q) {
+      const double px = xyz[3 * q] - o[0], py = xyz[3 * q + 1] - o[1], pz = xyz[3 * q + 2] - o[2];
+      R[0 * (size_t)n + 3 * q] = 1; R[1 * (size_t)n + 3 * q + 1] = 1; R[2 * (size_t)n + 3 * q + 2] = 1;
+      R[3 * (size_t)n + 3 * q + 1] = pz; R[3 * (size_t)n + 3 * q + 2] = -py;
+      R[4 * (size_t)n + 3 * q] = -pz; R[4 * (size_t)n + 3 * q + 2] = px;
+      R[5 * (size_t)n + 3 * q] = py; R[5 * (size_t)n + 3 * q + 1] = -px;
+    }
+    int m = 0;
+    for (int i = 0; i < 6; ++i) {
+      double* ci = &R[(size_t)i * n];
+      double old2 = 0;
+      for (int t = 0; t < n; ++t) old2 += ci[t] * ci[t];
+      for (int j = 0; j < m; ++j) {
+        const double* cj = &R[(size_t)j * n];
+        double d = 0;
+        for (int t = 0; t < n; ++t) d += ci[t] * cj[t];
+        for (int t = 0; t < n; ++t) ci[t] -= d * cj[t];
+      }
+      double r2 = 0;
+      for (int t = 0; t < n; ++t) r2 += ci[t] * ci[t];
+      if (!(old2 >= 1e-12 && r2 >= 1e-12 * old2)) continue;
+      const double inv = 1.0 / std::sqrt(r2);
+      double* dst = &R[(size_t)m * n];
+      for (int t = 0; t < n; ++t) dst[t] = ci[t] * inv;
+      ++m;
+    }
+    c.crig_m = m;
+    c.crig = dalloc<double>(c, (size_t)std::max(m, 1) * n);
+    if (m) upload(c.stream, c.crig, R.data(), (size_t)m * n * sizeof(double));
+  }
 }
 
 // factor of the coarsest level's FP64 stencil into cinv, or (from32) of its FP32 stencil
@@ -1625,6 +1778,17 @@ int coarse_factor(Ctx& c, bool from32) {
   double* R = Cp + (size_t)n * kGJB;
   double* diag0 = R + (size_t)n * kGJB;
   hipLaunchKernelGGL(k_symmetrize, dim3(nblk((int64_t)n * n)), dim3(256), 0, c.stream, A, n, diag0);
+  if (c.floating && c.crig_m) {
+    // null space of a floating body: factor A + alpha R R^T (alpha = mean diagonal); for a
+    // right-hand side orthogonal to R its solution is A's minimum-norm solution, the one
+    // the reference's BDCSVD solve returns (grid/Grid.cpp:3654-3661, 4072)
+    std::vector<double> d(n);
+    download(c.stream, d.data(), diag0, (size_t)n * sizeof(double));
+    double alpha = 0;
+    for (double x : d) alpha += x / n;
+    hipLaunchKernelGGL(k_add_lowrank, dim3(nblk((int64_t)n * n)), dim3(256), 0, c.stream, A, n,
+                       (const double*)c.crig, c.crig_m, alpha);
+  }
   HIPCHK(hipMemsetAsync(c.cflag, 0, sizeof(int), c.stream));
   for (int K0 = 0; K0 < n; K0 += kGJB) {
     const int b = std::min(kGJB, n - K0);
@@ -2260,6 +2424,92 @@ int guard(Ctx* c, F&& f) {
 #define REQUIRE(cond, msg) \
   if (!(cond)) throw ArgError(msg)
 
+double* rig_partials(Ctx& c) {
+  if (!c.rig_part) c.rig_part = dalloc<double>(c, 6 * (size_t)kRedBlocks);
+  return c.rig_part;
+}
+
+// displacementProject (optimization/projection.cpp:374-395) on a fine-level vector of a
+// floating context: u -= R M R^T u over the active nodes
+void remove_rigid_dev(Ctx& c, double* u) {
+  Level& L0 = c.L[0];
+  double* part = rig_partials(c);
+  hipLaunchKernelGGL(k_rigid_moments, dim3(kRedBlocks), dim3(256), 0, c.stream, (const int4*)L0.cl_lin, L0.n_lin,
+                     L0.v.NP, (const double*)u, c.rig_all, part);
+  hipLaunchKernelGGL(k_fold6, dim3(1), dim3(256), 0, c.stream, (const double*)part, (int)kRedBlocks, c.scal);
+  hipLaunchKernelGGL(k_rigid_sub, dim3(c.nb0), dim3(256), 0, c.stream, (const int4*)L0.cl_lin, L0.n_lin, L0.v.NP,
+                     u, c.rig_all, (const double*)c.scal);
+  LAUNCH_CHECK();
+}
+
+// A load region on the device (wc_* buffers): padded indices, unit normals (when given),
+// and on a floating context the coordinates relative to the region's rigid-motion
+// projector, which is returned
+struct Region {
+  int n = 0;
+  const double* nrm = nullptr;
+  const double* pos = nullptr;
+  RigidProj P{};
+};
+Region upload_region(Ctx& C, int n, const int64_t* nodes, const double* normals) {
+  Level& L0 = C.L[0];
+  const LevelView& v = L0.v;
+  const int64_t nn = (int64_t)v.nx * v.ny * v.nz;
+  std::vector<int64_t> pidx(n);
+  std::vector<double> nrm(normals ? 3 * (size_t)n : 0), pos(3 * (size_t)n);
+  std::vector<uint8_t> seen(nn, 0);
+  for (int i = 0; i < n; ++i) {
+    const int64_t id = nodes[i];
+    REQUIRE(id >= 0 && id < nn, "load node out of range");
+    REQUIRE(L0.h_free[id], "load node is not a free node");
+    REQUIRE(!seen[id], "repeated load node");
+    seen[id] = 1;
+    const int x = (int)(id % v.nx), y = (int)((id / v.nx) % v.ny), z = (int)(id / ((int64_t)v.nx * v.ny));
+    pidx[i] = v.pnode(x, y, z);
+    pos[3 * (size_t)i] = x; pos[3 * (size_t)i + 1] = y; pos[3 * (size_t)i + 2] = z;
+    if (normals) {
+      const double* a = normals + 3 * (size_t)i;
+      const double len = std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+      REQUIRE(std::isfinite(len) && len > 0, "zero or non-finite normal");
+      for (int k = 0; k < 3; ++k) nrm[3 * (size_t)i + k] = a[k] / len;
+    }
+  }
+  if (C.wc_cap < n) {
+    C.wc_idx = dalloc<int64_t>(C, n);
+    C.wc_nrm = dalloc<double>(C, 3 * (size_t)n);
+    C.wc_fs = dalloc<double>(C, 3 * (size_t)n);
+    C.wc_src = dalloc<double>(C, 3 * (size_t)n);
+    C.wc_pos = dalloc<double>(C, 3 * (size_t)n);
+    C.wc_cap = n;
+  }
+  upload(C.stream, C.wc_idx, pidx.data(), sizeof(int64_t) * n);
+  Region R;
+  R.n = n;
+  if (normals) {
+    upload(C.stream, C.wc_nrm, nrm.data(), sizeof(double) * 3 * n);
+    R.nrm = C.wc_nrm;
+  }
+  if (C.floating) {
+    std::vector<double> rel;
+    R.P = rigid_projector_region(n, pos.data(), normals ? nrm.data() : nullptr, rel);
+    upload(C.stream, C.wc_pos, rel.data(), sizeof(double) * 3 * n);
+    R.pos = C.wc_pos;
+  }
+  return R;
+}
+
+// scal[S_RIG0..] = R^T g over the region (g: the candidate from u or src_nm); no-op unless
+// the context is floating
+void region_moments(Ctx& C, const Region& R, const double* u, const double* src_nm) {
+  if (!R.pos) return;
+  const unsigned nbw = (unsigned)std::min<int64_t>(kRedBlocks, (R.n + 255) / 256);
+  double* part = rig_partials(C);
+  hipLaunchKernelGGL(k_worst_rig_moments, dim3(nbw), dim3(256), 0, C.stream, R.n, (const int64_t*)C.wc_idx, R.nrm,
+                     R.pos, C.L[0].v.NP, u, src_nm, part);
+  hipLaunchKernelGGL(k_fold6, dim3(1), dim3(256), 0, C.stream, (const double*)part, (int)nbw, C.scal);
+  LAUNCH_CHECK();
+}
+
 }  // namespace
 
 // =========================================================================================
@@ -2402,6 +2652,7 @@ TOS_EXPORT int tos_get_info(tos_ctx* c, tos_info* o) {
   o->coarse_dofs = c->cn;
   o->n_cases = c->K;
   o->n_coeffs = (int64_t)c->nc[0] * c->nc[1] * c->nc[2];
+  o->has_support = c->floating ? 0 : 1;
   return TOS_OK;
 }
 
@@ -2836,38 +3087,14 @@ TOS_EXPORT int tos_worst_compliance(tos_ctx* c, const tos_worst_opts* o, tos_wor
     Ctx& C = *c;
     Level& L0 = C.L[0];
     const LevelView& v = L0.v;
-    const int64_t nn = (int64_t)v.nx * v.ny * v.nz, NP = v.NP, n3p = 3 * NP;
+    const int64_t NP = v.NP, n3p = 3 * NP;
     const int n = o->n_load;
-    std::vector<int64_t> pidx(n);
-    std::vector<double> nrm(o->normals ? 3 * (size_t)n : 0), f0(3 * (size_t)n);
-    std::vector<uint8_t> seen(nn, 0);
-    for (int i = 0; i < n; ++i) {
-      const int64_t id = o->nodes[i];
-      REQUIRE(id >= 0 && id < nn, "load node out of range");
-      REQUIRE(L0.h_free[id], "load node is not a free node");
-      REQUIRE(!seen[id], "repeated load node");
-      seen[id] = 1;
-      pidx[i] = v.pnode((int)(id % v.nx), (int)((id / v.nx) % v.ny), (int)(id / ((int64_t)v.nx * v.ny)));
-      if (o->normals) {
-        const double* a = o->normals + 3 * (size_t)i;
-        const double len = std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
-        REQUIRE(std::isfinite(len) && len > 0, "zero or non-finite normal");
-        for (int k = 0; k < 3; ++k) nrm[3 * (size_t)i + k] = a[k] / len;
-      }
-    }
+    const Region RG = upload_region(C, n, o->nodes, o->normals);
+    std::vector<double> f0(3 * (size_t)n);
     uint64_t seed = o->seed;
     for (double& x : f0) x = splitmix_pm1(seed);
-    if (C.wc_cap < n) {
-      C.wc_idx = dalloc<int64_t>(C, n);
-      C.wc_nrm = dalloc<double>(C, 3 * (size_t)n);
-      C.wc_fs = dalloc<double>(C, 3 * (size_t)n);
-      C.wc_src = dalloc<double>(C, 3 * (size_t)n);
-      C.wc_cap = n;
-    }
-    upload(C.stream, C.wc_idx, pidx.data(), sizeof(int64_t) * n);
     upload(C.stream, C.wc_src, f0.data(), sizeof(double) * 3 * n);
-    if (o->normals) upload(C.stream, C.wc_nrm, nrm.data(), sizeof(double) * 3 * n);
-    const double* nrm_d = o->normals ? C.wc_nrm : nullptr;
+    const double* nrm_d = RG.nrm;
     const int rs = select_stencils(C, o->precision);
     if (rs) return rs;
     const auto t0 = std::chrono::steady_clock::now();
@@ -2880,13 +3107,14 @@ TOS_EXPORT int tos_worst_compliance(tos_ctx* c, const tos_worst_opts* o, tos_wor
     // scal[S_TMP3] = |f_old|^2
     auto force_update = [&](const double* src_nm, int first) {
       Prof pr(C, P_VECTOR);
+      region_moments(C, RG, u, src_nm);  // floating: R^T g of the candidate
       hipLaunchKernelGGL(k_worst_norm, dim3(nbw), dim3(256), 0, C.stream, n, (const int64_t*)C.wc_idx, nrm_d, NP,
-                         (const double*)u, src_nm, C.partial);
+                         (const double*)u, src_nm, RG.pos, RG.P, (const double*)C.scal, C.partial);
       hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, C.stream, (const double*)C.partial, (int)nbw, 0, S_TMP1,
                          0, OP_NONE, C.scal);
       hipLaunchKernelGGL(k_worst_apply, dim3(nbw), dim3(256), 0, C.stream, n, (const int64_t*)C.wc_idx, nrm_d, NP,
-                         (const double*)u, src_nm, (const double*)C.scal, (int)S_TMP1, first, f, C.r, r32, C.wc_fs,
-                         C.partial);
+                         (const double*)u, src_nm, RG.pos, RG.P, (const double*)C.scal, (int)S_TMP1, first, f, C.r,
+                         r32, C.wc_fs, C.partial);
       hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, C.stream, (const double*)C.partial, (int)nbw, 1, S_TMP2,
                          S_TMP3, OP_NONE, C.scal);
       LAUNCH_CHECK();
@@ -2913,6 +3141,7 @@ TOS_EXPORT int tos_worst_compliance(tos_ctx* c, const tos_worst_opts* o, tos_wor
       else
         hipLaunchKernelGGL(k_add_nodes<double>, dim3(C.nb0), dim3(256), 0, C.stream, (const int4*)L0.cl_lin,
                            L0.n_lin, NP, u, (const double*)C.z);
+      if (C.floating) remove_rigid_dev(C, u);  // optimization/optimization.cpp:282-285
       apply_level<double, 1>(C, 0, u, f, C.r, C.partial, r32);
       {
         Prof pr(C, P_VECTOR);
@@ -2939,6 +3168,36 @@ TOS_EXPORT int tos_worst_compliance(tos_ctx* c, const tos_worst_opts* o, tos_wor
       st->rel_res = rel;
       st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
+    return TOS_OK;
+  });
+}
+
+TOS_EXPORT int tos_balance_loads(tos_ctx* c, int k, int n, const int64_t* nodes, const double* normals) {
+  return guard(c, [&]() {
+    REQUIRE(k >= 0 && k < c->K && n > 0 && nodes, "bad arguments");
+    Ctx& C = *c;
+    const int64_t NP = C.L[0].v.NP;
+    const Region RG = upload_region(C, n, nodes, normals);
+    double* f = C.F + (size_t)k * 3 * NP;
+    const unsigned nb = nblk(n);
+    hipLaunchKernelGGL(k_gather_load, dim3(nb), dim3(256), 0, C.stream, n, (const int64_t*)C.wc_idx, NP,
+                       (const double*)f, C.wc_src);
+    region_moments(C, RG, nullptr, C.wc_src);
+    hipLaunchKernelGGL(k_zero<double>, dim3(nblk(3 * NP)), dim3(256), 0, C.stream, f, 3 * NP);
+    hipLaunchKernelGGL(k_balance_apply, dim3(nb), dim3(256), 0, C.stream, n, (const int64_t*)C.wc_idx, RG.nrm,
+                       RG.pos, RG.P, (const double*)C.scal, NP, (const double*)C.wc_src, f);
+    LAUNCH_CHECK();
+    HIPCHK(hipStreamSynchronize(C.stream));
+    return TOS_OK;
+  });
+}
+
+TOS_EXPORT int tos_remove_rigid(tos_ctx* c, int k) {
+  return guard(c, [&]() {
+    REQUIRE(k >= 0 && k < c->K, "bad case");
+    if (!c->floating) { c->err = "the context has supports: no rigid motions to remove"; return TOS_E_STATE; }
+    remove_rigid_dev(*c, c->U + (size_t)k * 3 * c->L[0].v.NP);
+    HIPCHK(hipStreamSynchronize(c->stream));
     return TOS_OK;
   });
 }

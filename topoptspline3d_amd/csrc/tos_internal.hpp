@@ -115,6 +115,19 @@ struct Level {
   uint8_t* gal_m2 = nullptr;
 };
 
+// Projector onto the rigid motions of a node set: columns t_x, t_y, t_z, (0, z, -y),
+// (-z, 0, x), (y, -x, 0) at node coordinates relative to o (the columns of [I | p^],
+// optimization/projection.cpp:70-77, 212-219), optionally each node's 3-vector taken
+// along its normal first.  With R those columns over the set and b = R^T v, the
+// projection of v is R (M b); M is built on the host from the Gram matrix R^T R by
+// Gram-Schmidt over the columns in order, dropping a column whose residual norm is below
+// 1e-6 of its own norm or whose norm is below 1e-6 (projection.cpp:234-255).
+struct RigidProj {
+  double M[36];
+  double o[3];
+  int m;  // columns kept
+};
+
 struct Ctx {
   int dev = 0;
   hipStream_t stream = nullptr;
@@ -173,6 +186,13 @@ struct Ctx {
   int64_t wc_cap = 0;
   int64_t* wc_idx = nullptr;
   double *wc_nrm = nullptr, *wc_fs = nullptr, *wc_src = nullptr;
+  double* wc_pos = nullptr;     // load-node coordinates (3 per node), floating contexts
+  // floating body (no supported fine node): the operator's null space is the rigid motions
+  bool floating = false;
+  double* crig = nullptr;       // coarsest level: orthonormal rigid motions, column k at crig[k * cn]
+  int crig_m = 0;
+  RigidProj rig_all{};          // projector over the fine active nodes (displacementProject)
+  double* rig_part = nullptr;   // 6 x kRedBlocks partial sums
   // spline
   int sp_mode = -1, sp_order = 0, sp_param = 0;
   int nc[3] = {0, 0, 0};
@@ -247,5 +267,7 @@ enum {
   S_TMP0, S_TMP1, S_TMP2, S_TMP3,
   S_NN0, S_NN1, S_NN2,               // learned correction: |r~|^2, e.r, e.(r - A e)
   S_NN_TRIED, S_NN_KEPT, S_NN_STEP,  // its statistics since the last attach
+  S_RIG0,                            // 6 slots: R^T v of a rigid-motion projection
   S_COUNT = 32
 };
+static_assert(S_RIG0 + 6 <= S_COUNT, "scalar slots");
