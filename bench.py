@@ -425,7 +425,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c2")
-    ap.add_argument("--precision", type=int, default=2,
+    ap.add_argument("--precision", type=int, default=3,
                     help="0 FP64 V-cycle; 1 FP32 V-cycle; 2 FP32 V-cycle on FP16 coarse stencils; "
                          "3 as 2 with the FCG residual and K p stored in FP32")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -572,7 +572,7 @@ def main():
     if not args.no_worst:
         if not wl["spline"]:
             loop.ctx.update_operator()
-        line["worst_case_load"] = worst_phase(loop.ctx, wl, args.precision)
+        line["worst_case_load"] = worst_phase(loop.ctx, wl, min(args.precision, 2))  # V-cycle precision
     coeffs_final = None
     if wl["spline"] and not args.no_surface:
         line["surface_constraints"] = surface_phase(loop.ctx)
