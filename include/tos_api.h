@@ -52,6 +52,16 @@ typedef struct tos_grid_desc {
   int n_levels;             /* 0 = coarsen while a level has > coarsen_threshold active
                                elements (grid/Grid.cpp:747); >0 = exactly this many    */
   int coarsen_threshold;    /* 0 = 400                                                 */
+  int skip_layer;           /* 1: the reference's skip-layer hierarchy (set_skip_layer,
+                               optimization/optimization.cpp:20): level 1 of the dyadic
+                               hierarchy is dropped, so the first coarse level is 4x
+                               coarser, with factor-4 trilinear transfers (grid/Grid.cu:
+                               2509-2570, 2635-2671), all nodes of its active elements as
+                               unknowns, and its Galerkin operator formed from the fine
+                               element products with the reference's Dirichlet penalty
+                               (supported fine nodes: rows and columns dropped, 1e6 on the
+                               diagonal per element, grid/Grid.cu:14, 1112-1225); needs 3
+                               or more dyadic levels (ignored otherwise); 0: dyadic      */
 } tos_grid_desc;
 
 typedef struct tos_info {

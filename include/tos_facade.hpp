@@ -204,12 +204,14 @@ class HierarchyGrid {
   // genFromMesh(bits, reso) equivalent (grid/Grid.cpp:977-1236) on a box of voxels
   void genFromBits(int nx, int ny, int nz, const uint8_t* solid, const uint8_t* support, const uint8_t* shell,
                    double h = 1.0) {
-    tos_grid_desc d{nx, ny, nz, h, solid, support, shell, 0, 0};
+    tos_grid_desc d{nx, ny, nz, h, solid, support, shell, 0, 0, skip_ ? 1 : 0};
     tos::check(tos::Api::get().ctx_create(&d, &c_), nullptr, "ctx_create");
     tos_info i;
     tos::check(tos::Api::get().get_info(c_, &i), c_, "get_info");
     for (int l = 0; l < i.n_levels; ++l) grids_.emplace_back(c_, l);
   }
+  // the reference's set_skip_layer (optimization/optimization.cpp:20), before genFromBits
+  void set_skip_layer(bool skip) { skip_ = skip; }
   void setLoads(int n_cases, const double* f) { tos::check(tos::Api::get().set_loads(c_, n_cases, f), c_, "set_loads"); }
   // spline knots over the box (set_spline_partition, grid/Grid.cu:3134-3203)
   void set_spline_partition(int order, int partition) {
@@ -302,6 +304,7 @@ class HierarchyGrid {
   tos_ctx* c_ = nullptr;
   std::vector<Grid> grids_;
   int last_iterations_ = 0;
+  bool skip_ = false;
   std::vector<int64_t> load_nodes_;
   std::vector<double> load_normals_;
   template <class Get>

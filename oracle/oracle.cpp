@@ -146,7 +146,7 @@ static void level_dims(Level& L, int ex, int ey, int ez) {
   L.nn = (int64_t)L.nx * L.ny * L.nz;
 }
 
-static void build_hierarchy(Ctx& c, int n_levels, int threshold) {
+static void build_hierarchy(Ctx& c, int n_levels, int threshold, bool skip) {
   c.L.clear();
   Level L0;
   level_dims(L0, c.nx, c.ny, c.nz);
@@ -171,6 +171,10 @@ static void build_hierarchy(Ctx& c, int n_levels, int threshold) {
           if (F.eact[F.eid(x, y, z)]) C.eact[C.eid(x / 2, y / 2, z / 2)] = 1;
     c.L.push_back(std::move(C));
   }
+  // skip-layer (set_skip_layer, optimization/optimization.cpp:20; grid/Grid.cpp:849-897):
+  // the dyadic level 1 is dropped, level 2 follows level 0 with a factor-4 transfer
+  c.skip = skip && c.L.size() >= 3;
+  if (c.skip) c.L.erase(c.L.begin() + 1);
   // free masks
   for (size_t l = 0; l < c.L.size(); ++l) {
     Level& L = c.L[l];
@@ -188,6 +192,14 @@ static void build_hierarchy(Ctx& c, int n_levels, int threshold) {
             int64_t n = L.nid(x, y, z);
             L.fre[n] = act && !(c.support.size() && c.support[n]);
           }
+    } else if (l == 1 && c.skip) {
+      // every node of an active element is an unknown: the supports enter the operator as
+      // the Dirichlet penalty of the skip-layer Galerkin product
+      for (int z = 0; z < L.ez; ++z)
+        for (int y = 0; y < L.ey; ++y)
+          for (int x = 0; x < L.ex; ++x)
+            if (L.eact[L.eid(x, y, z)])
+              for (int a = 0; a < 8; ++a) L.fre[L.nid(x + (a & 1), y + ((a >> 1) & 1), z + (a >> 2))] = 1;
     } else {
       const Level& F = c.L[l - 1];
       for (int z = 0; z < L.nz; ++z)
@@ -523,7 +535,10 @@ static void gs_sweep(const Ctx& c, int l, double* u, const double* f, bool backw
 // :2594-2632): trilinear P with weights 1, 1/2, 1/4, 1/8; restriction is P^T.  Both are
 // masked to free nodes (P with the rows of eliminated fine nodes removed).
 // ------------------------------------------------------------------------------------
-static void restrict_to(const Level& F, Level& C) {
+// factor q = 2 (dyadic) or 4 (skip-layer, restrict_residual_nondyadic_kernel
+// grid/Grid.cu:2509-2570): f_c[I] = sum over fine nodes at q I + d, |d| < q per axis, of
+// prod (q - |d|) / q times r_f
+static void restrict_to(const Level& F, Level& C, int q = 2) {
   const int64_t nf = F.nn, nc = C.nn;
 #pragma omp parallel for collapse(2) schedule(static)
   for (int z = 0; z < C.nz; ++z)
@@ -532,18 +547,46 @@ static void restrict_to(const Level& F, Level& C) {
         int64_t I = C.nid(x, y, z);
         double acc[3] = {0, 0, 0};
         if (C.fre[I]) {
-          for (int dz = -1; dz <= 1; ++dz)
-            for (int dy = -1; dy <= 1; ++dy)
-              for (int dx = -1; dx <= 1; ++dx) {
-                int fx = 2 * x + dx, fy = 2 * y + dy, fz = 2 * z + dz;
+          for (int dz = 1 - q; dz < q; ++dz)
+            for (int dy = 1 - q; dy < q; ++dy)
+              for (int dx = 1 - q; dx < q; ++dx) {
+                int fx = q * x + dx, fy = q * y + dy, fz = q * z + dz;
                 if (fx < 0 || fy < 0 || fz < 0 || fx >= F.nx || fy >= F.ny || fz >= F.nz) continue;
                 int64_t a = F.nid(fx, fy, fz);
                 if (!F.fre[a]) continue;
-                double w = (dx ? 0.5 : 1.0) * (dy ? 0.5 : 1.0) * (dz ? 0.5 : 1.0);
+                double w = (double)(q - std::abs(dx)) * (q - std::abs(dy)) * (q - std::abs(dz)) / ((double)q * q * q);
                 for (int r = 0; r < 3; ++r) acc[r] += w * F.r[r * nf + a];
               }
         }
         for (int r = 0; r < 3; ++r) C.f[r * nc + I] = acc[r];
+      }
+}
+
+// factor-4 prolongation (prolongate_correction_nondyadic_kernel grid/Grid.cu:2635-2671): a
+// fine node at 4 X + t takes (4 - t) / 4 of coarse node X and t / 4 of X + 1 per axis
+static void prolong_add4(const Level& C, Level& F) {
+  const int64_t nf = F.nn, nc = C.nn;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int z = 0; z < F.nz; ++z)
+    for (int y = 0; y < F.ny; ++y)
+      for (int x = 0; x < F.nx; ++x) {
+        int64_t a = F.nid(x, y, z);
+        if (!F.fre[a]) continue;
+        const int p[3] = {x, y, z};
+        double acc[3] = {0, 0, 0};
+        for (int k = 0; k < 8; ++k) {
+          int cc[3];
+          double w = 1;
+          for (int ax = 0; ax < 3; ++ax) {
+            const int t = p[ax] % 4, up = (k >> ax) & 1;
+            cc[ax] = p[ax] / 4 + up;
+            w *= up ? t / 4.0 : (4 - t) / 4.0;
+          }
+          if (w == 0) continue;
+          const int64_t I = C.nid(cc[0], cc[1], cc[2]);
+          for (int r = 0; r < 3; ++r) acc[r] += w * C.u[r * nc + I];
+        }
+        for (int r = 0; r < 3; ++r) F.u[r * nf + a] += acc[r];
       }
 }
 
@@ -634,6 +677,73 @@ static void galerkin_first(Ctx& c) {
             }
           }
         }
+        for (int k = 0; k < 27; ++k) {
+          int jx = X + k % 3 - 1, jy = Y + (k / 3) % 3 - 1, jz = Z + k / 9 - 1;
+          bool ok = jx >= 0 && jy >= 0 && jz >= 0 && jx < C.nx && jy < C.ny && jz < C.nz &&
+                    C.fre[C.nid(jx, jy, jz)];
+          for (int e = 0; e < 9; ++e) C.st[(k * 9 + e) * nc + I] = ok ? acc[k][e] : 0.0;
+        }
+      }
+}
+
+// Skip-layer level 1 (restrict_stencil_nondyadic_OTFA_WS_kernel grid/Grid.cu:1112-1225):
+// K_c = P4^T K~ P4 per coarse node, K~ the fine element matrices rho^p Ke with the rows and
+// columns of supported nodes dropped and 1e6 (not scaled by rho^p) on their diagonal, one
+// per element (grid/Grid.cu:14, 1199-1207); summed over the 8^3 fine elements around the
+// node, weight prod (4 - |d|) / 4 of a fine node at offset d from a coarse node.
+static void galerkin_skip(Ctx& c) {
+  const Level& F = c.L[0];
+  Level& C = c.L[1];
+  const int64_t nc = C.nn;
+  C.st.assign(243 * nc, 0.0);
+  auto w4 = [](int d) { d = std::abs(d); return d < 4 ? (4 - d) / 4.0 : 0.0; };
+#pragma omp parallel for collapse(2) schedule(dynamic, 1)
+  for (int Z = 0; Z < C.nz; ++Z)
+    for (int Y = 0; Y < C.ny; ++Y)
+      for (int X = 0; X < C.nx; ++X) {
+        const int64_t I = C.nid(X, Y, Z);
+        if (!C.fre[I]) continue;
+        double acc[27][9] = {};
+        for (int fz = 4 * Z - 4; fz < 4 * Z + 4; ++fz)
+          for (int fy = 4 * Y - 4; fy < 4 * Y + 4; ++fy)
+            for (int fx = 4 * X - 4; fx < 4 * X + 4; ++fx) {
+              if (fx < 0 || fy < 0 || fz < 0 || fx >= F.ex || fy >= F.ey || fz >= F.ez) continue;
+              if (!F.eact[F.eid(fx, fy, fz)]) continue;
+              const double w = c.w[F.eid(fx, fy, fz)];
+              bool fix[8];
+              double wi[8];
+              for (int i = 0; i < 8; ++i) {
+                const int x = fx + (i & 1), y = fy + ((i >> 1) & 1), z = fz + (i >> 2);
+                fix[i] = c.support.size() && c.support[F.nid(x, y, z)];
+                wi[i] = w4(x - 4 * X) * w4(y - 4 * Y) * w4(z - 4 * Z);
+              }
+              // t[j][r][s] = sum_i w_I(i) K~[3i + r][3j + s]
+              double t[8][3][3] = {};
+              for (int i = 0; i < 8; ++i) {
+                if (wi[i] == 0) continue;
+                for (int j = 0; j < 8; ++j)
+                  for (int r = 0; r < 3; ++r)
+                    for (int s = 0; s < 3; ++s) {
+                      double v;
+                      if (fix[i] || fix[j])
+                        v = (i == j && r == s) ? 1e6 : 0.0;
+                      else
+                        v = w * c.Ke[3 * i + r][3 * j + s];
+                      t[j][r][s] += wi[i] * v;
+                    }
+              }
+              // column node J = I + (dx, dy, dz): weight of fine node j from J
+              for (int k = 0; k < 27; ++k) {
+                const int JX = X + k % 3 - 1, JY = Y + (k / 3) % 3 - 1, JZ = Z + k / 9 - 1;
+                for (int j = 0; j < 8; ++j) {
+                  const int x = fx + (j & 1), y = fy + ((j >> 1) & 1), z = fz + (j >> 2);
+                  const double wj = w4(x - 4 * JX) * w4(y - 4 * JY) * w4(z - 4 * JZ);
+                  if (wj == 0) continue;
+                  for (int r = 0; r < 3; ++r)
+                    for (int s = 0; s < 3; ++s) acc[k][r * 3 + s] += t[j][r][s] * wj;
+                }
+              }
+            }
         for (int k = 0; k < 27; ++k) {
           int jx = X + k % 3 - 1, jy = Y + (k / 3) % 3 - 1, jz = Z + k / 9 - 1;
           bool ok = jx >= 0 && jy >= 0 && jz >= 0 && jx < C.nx && jy < C.ny && jz < C.nz &&
@@ -818,9 +928,13 @@ static void vcycle_level(Ctx& c, size_t l, int pre, int post) {
   for (int s = 0; s < pre; ++s) gs_sweep(c, l, L.u.data(), L.f.data(), false);
   apply_level(c, (int)l, L.u.data(), L.r.data());
   for (int64_t i = 0; i < 3 * L.nn; ++i) L.r[i] = L.f[i] - L.r[i];
-  restrict_to(L, c.L[l + 1]);
+  const bool four = l == 0 && c.skip;
+  restrict_to(L, c.L[l + 1], four ? 4 : 2);
   vcycle_level(c, l + 1, pre, post);
-  prolong_add(c.L[l + 1], L);
+  if (four)
+    prolong_add4(c.L[l + 1], L);
+  else
+    prolong_add(c.L[l + 1], L);
   for (int s = 0; s < post; ++s) gs_sweep(c, l, L.u.data(), L.f.data(), true);
 }
 
@@ -1363,7 +1477,7 @@ ORC_API int orc_ctx_create(const tos_grid_desc* d, tos_ctx** out) {
   c->solid.assign(d->solid, d->solid + ne);
   if (d->support) c->support.assign(d->support, d->support + nn);
   if (d->shell) c->shell.assign(d->shell, d->shell + ne);
-  build_hierarchy(*c, d->n_levels, d->coarsen_threshold > 0 ? d->coarsen_threshold : 400);
+  build_hierarchy(*c, d->n_levels, d->coarsen_threshold > 0 ? d->coarsen_threshold : 400, d->skip_layer != 0);
   c->n_active = 0;
   for (auto v : c->L[0].eact) c->n_active += v;
   c->rho.assign(ne, 0.0f);
@@ -1479,7 +1593,12 @@ ORC_API int orc_update_operator(tos_ctx* c) {
   const Level& L0 = c->L[0];
   for (int64_t e = 0; e < L0.ne; ++e)
     c->w[e] = L0.eact[e] ? std::pow((double)c->rho[e], c->penal) : 0.0;
-  if (c->L.size() > 1) galerkin_first(*c);
+  if (c->L.size() > 1) {
+    if (c->skip)
+      galerkin_skip(*c);
+    else
+      galerkin_first(*c);
+  }
   for (size_t l = 2; l < c->L.size(); ++l) galerkin_next(c->L[l - 1], c->L[l]);
   int rc = coarse_factor(*c);
   if (rc) return rc;

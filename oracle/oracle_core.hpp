@@ -65,6 +65,7 @@ struct Ctx {
   int64_t n_active = 0;
   bool op_ready = false;
   bool floating = false;    // no supported fine node: the rigid motions are K's null space
+  bool skip = false;        // skip-layer hierarchy: level 1 is 4x coarser than level 0
   // coarsest dense system
   std::vector<int64_t> cnode;   // free coarsest nodes
   std::vector<int64_t> cmap;    // node -> compact index or -1

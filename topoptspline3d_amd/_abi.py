@@ -16,7 +16,7 @@ i32p = C.POINTER(C.c_int)
 class GridDesc(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("h", C.c_double),
                 ("solid", u8p), ("support", u8p), ("shell", u8p),
-                ("n_levels", C.c_int), ("coarsen_threshold", C.c_int)]
+                ("n_levels", C.c_int), ("coarsen_threshold", C.c_int), ("skip_layer", C.c_int)]
 
 
 class Info(C.Structure):

@@ -24,7 +24,7 @@ class Context:
     """One voxel design domain: hierarchy, loads, densities, sensitivities, spline."""
 
     def __init__(self, api, solid, support=None, shell=None, h=1.0, n_levels=0,
-                 coarsen_threshold=0):
+                 coarsen_threshold=0, skip_layer=False):
         self.api = api
         solid = np.ascontiguousarray(solid, dtype=np.uint8)
         nz, ny, nx = solid.shape  # arrays are indexed [z, y, x] (x fastest in memory)
@@ -46,6 +46,7 @@ class Context:
             self._keep.append(shell)
         d.n_levels = int(n_levels)
         d.coarsen_threshold = int(coarsen_threshold)
+        d.skip_layer = int(bool(skip_layer))
         h_ = C.c_void_p()
         rc = api.ctx_create(C.byref(d), C.byref(h_))
         if rc != 0:

@@ -992,6 +992,162 @@ __global__ __launch_bounds__(256) void k_prolong_add(LevelView F, LevelView C, c
   prolong_node(F, C, n, uc, uf);
 }
 
+// ---------------------------------------------------------------------------------------
+// Skip-layer transfers between level 0 and a 4x coarser level 1: the factor-4 trilinear hat,
+// weight prod (4 - |d|) / 4 over the offsets d of a fine node from a coarse node
+// (restrict_residual_nondyadic_kernel grid/Grid.cu:2509-2570, prolongate_correction_
+// nondyadic_kernel :2635-2671), coarse node X at fine node 4X.
+// ---------------------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(256) void k_restrict4(LevelView F, LevelView C, const int4* __restrict__ clc, int nch,
+                                                   const T* __restrict__ rf, T* __restrict__ fc) {
+  NodeRef n;
+  if (!chunk_node(clc, nch, 1, n)) return;
+  T a0 = 0, a1 = 0, a2 = 0;
+  const int x0 = 4 * n.x, y0 = 4 * n.y, z0 = 4 * n.z;
+  for (int dz = -3; dz <= 3; ++dz) {
+    const int z = z0 + dz;
+    if (z < 0 || z >= F.nz) continue;
+    const T wz = T(4 - (dz < 0 ? -dz : dz)) * T(0.25);
+    for (int dy = -3; dy <= 3; ++dy) {
+      const int y = y0 + dy;
+      if (y < 0 || y >= F.ny) continue;
+      const T wzy = wz * (T(4 - (dy < 0 ? -dy : dy)) * T(0.25));
+      const int64_t row = F.pnode(0, y, z);
+#pragma unroll
+      for (int dx = -3; dx <= 3; ++dx) {
+        const int x = x0 + dx;
+        if (x < 0 || x >= F.nx) continue;
+        const T w = wzy * (T(4 - (dx < 0 ? -dx : dx)) * T(0.25));
+        const int64_t q = row + x;
+        a0 = fma(w, rf[q], a0);
+        a1 = fma(w, rf[F.NP + q], a1);
+        a2 = fma(w, rf[2 * F.NP + q], a2);
+      }
+    }
+  }
+  fc[n.p] = a0;
+  fc[C.NP + n.p] = a1;
+  fc[2 * C.NP + n.p] = a2;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_prolong4(LevelView F, LevelView C, const int4* __restrict__ clf, int nch,
+                                                  const T* __restrict__ uc, T* __restrict__ uf) {
+  NodeRef n;
+  if (!chunk_node(clf, nch, 1, n)) return;
+  const int tx = n.x & 3, ty = n.y & 3, tz = n.z & 3;
+  const T wx[2] = {T(4 - tx) * T(0.25), T(tx) * T(0.25)};
+  const T wy[2] = {T(4 - ty) * T(0.25), T(ty) * T(0.25)};
+  const T wz[2] = {T(4 - tz) * T(0.25), T(tz) * T(0.25)};
+  // corners X0..X0+1 (X0 + 1 lies in the padding when the weight is 0)
+  const int64_t I0 = C.pnode(n.x >> 2, n.y >> 2, n.z >> 2);
+  T a0 = 0, a1 = 0, a2 = 0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int i = k & 1, j = (k >> 1) & 1, l = k >> 2;
+    const T wgt = wx[i] * wy[j] * wz[l];
+    const int64_t I = I0 + i + (int64_t)C.NX * (j + (int64_t)C.NY * l);
+    a0 = fma(wgt, uc[I], a0);
+    a1 = fma(wgt, uc[C.NP + I], a1);
+    a2 = fma(wgt, uc[2 * C.NP + I], a2);
+  }
+  const int64_t p = n.p;
+  uf[p] += a0;
+  uf[F.NP + p] += a1;
+  uf[2 * F.NP + p] += a2;
+}
+
+// weight of node i (bits x, y, z) of fine child c of a coarse element from its corner m
+__device__ __forceinline__ double p4_weight(int c, int i, int m) {
+  double w = 1.0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int t = ((c >> (2 * a)) & 3) + ((i >> a) & 1);
+    w *= ((m >> a) & 1) ? t * 0.25 : (4 - t) * 0.25;
+  }
+  return w;
+}
+
+// Galerkin operator of the skip-layer level 1 (restrict_stencil_nondyadic_OTFA_WS_kernel
+// grid/Grid.cu:1112-1225): for coarse node I and block row r (blockIdx.y), the 27 blocks
+// are summed over the coarse elements E around I (I at corner a of E) and their 64 fine
+// children c: rho_c^p Mc4[c] rows 3a + r, columns (corner b, component s) into the block of
+// the neighbour I + b - a.  A child with a supported node takes its element matrix with
+// those rows and columns dropped and 1e6 on their diagonal entries (not scaled by rho^p,
+// grid/Grid.cu:14, 1199-1207), explicitly.  Mc4's rows of corner a for all children are
+// staged in LDS per corner.
+__global__ __launch_bounds__(256) void k_galerkin_skip(LevelView F, LevelView C, const int4* __restrict__ cl,
+                                                       int nch, const double* __restrict__ w64,
+                                                       const uint8_t* __restrict__ efix,
+                                                       const double* __restrict__ Ke,
+                                                       const double* __restrict__ Mc4, double* __restrict__ st64,
+                                                       float* __restrict__ st32, int64_t NS) {
+  __shared__ double sM[64 * 24];
+  const int r = blockIdx.y;
+  NodeRef n;
+  const bool ok = chunk_node(cl, nch, 1, n);
+  double acc[81];
+#pragma unroll
+  for (int t = 0; t < 81; ++t) acc[t] = 0.0;
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    __syncthreads();
+    for (int t = threadIdx.x; t < 64 * 24; t += 256) sM[t] = Mc4[(t / 24) * 576 + (3 * a + r) * 24 + t % 24];
+    __syncthreads();
+    if (!ok) continue;
+    const int EX = n.x - (a & 1), EY = n.y - ((a >> 1) & 1), EZ = n.z - (a >> 2);
+    if (EX < 0 || EY < 0 || EZ < 0 || EX >= C.ex || EY >= C.ey || EZ >= C.ez) continue;
+    for (int c = 0; c < 64; ++c) {
+      const int fx = 4 * EX + (c & 3), fy = 4 * EY + ((c >> 2) & 3), fz = 4 * EZ + (c >> 4);
+      if (fx >= F.ex || fy >= F.ey || fz >= F.ez) continue;
+      const int64_t pe = F.pelem(fx, fy, fz);
+      const double w = w64[pe];  // rho^p, 0 for inactive elements
+      if (w == 0.0) continue;
+      const unsigned fm = efix[pe];
+      if (fm == 0) {
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+          const int k = ((b & 1) - (a & 1) + 1) + 3 * (((b >> 1) & 1) - ((a >> 1) & 1) + 1) +
+                        9 * ((b >> 2) - (a >> 2) + 1);
+#pragma unroll
+          for (int s2 = 0; s2 < 3; ++s2) acc[k * 3 + s2] = fma(w, sM[c * 24 + 3 * b + s2], acc[k * 3 + s2]);
+        }
+      } else {
+        for (int i = 0; i < 8; ++i) {
+          const double wa = p4_weight(c, i, a);
+          if (wa == 0.0) continue;
+          const bool fi = (fm >> i) & 1u;
+          for (int j = 0; j < 8; ++j) {
+            const bool fj = (fm >> j) & 1u;
+            double v[3];
+#pragma unroll
+            for (int s2 = 0; s2 < 3; ++s2)
+              v[s2] = (fi || fj) ? ((i == j && r == s2) ? 1e6 : 0.0) : w * Ke[(3 * i + r) * 24 + 3 * j + s2];
+#pragma unroll
+            for (int b = 0; b < 8; ++b) {
+              const double wab = wa * p4_weight(c, j, b);
+              const int k = ((b & 1) - (a & 1) + 1) + 3 * (((b >> 1) & 1) - ((a >> 1) & 1) + 1) +
+                            9 * ((b >> 2) - (a >> 2) + 1);
+#pragma unroll
+              for (int s2 = 0; s2 < 3; ++s2) acc[k * 3 + s2] = fma(wab, v[s2], acc[k * 3 + s2]);
+            }
+          }
+        }
+      }
+    }
+  }
+  if (!ok) return;
+#pragma unroll
+  for (int k = 0; k < 27; ++k)
+#pragma unroll
+    for (int s2 = 0; s2 < 3; ++s2) {
+      const int64_t e = (int64_t)(k * 9 + r * 3 + s2) * NS + n.slot;
+      if (st64) st64[e] = acc[k * 3 + s2];
+      if (st32) st32[e] = (float)acc[k * 3 + s2];
+    }
+}
+
 // u_f += P u_c over row segments (even-x first, all rows): lane l takes the fine nodes
 // x1 = xs + 2l (coarse column X = x1/2, weight 1 along x) and x1 + 1 (columns X and X+1,
 // 1/2 each), so one wave reads each coarse column once for two fine nodes.  Lane 0 of a

@@ -154,6 +154,40 @@ void child_products(const double* Ke, double* Mc) {
   }
 }
 
+// The same for a factor-4 coarsening (skip-layer): the 64 fine elements of a coarse
+// element at positions c = cx + 4 cy + 16 cz; fine node i of child c sits at t = c + i
+// (0..4 per axis) inside the coarse element, with weight prod (m ? t : 4 - t) / 4 from
+// coarse corner m (the factor-4 hat of grid/Grid.cu:1130-1136, 2635-2671):
+// Mc4[c] = P_c^T Ke P_c.
+void child_products4(const double* Ke, double* Mc4) {
+  for (int c = 0; c < 64; ++c) {
+    const int cp[3] = {c & 3, (c >> 2) & 3, c >> 4};
+    double P[8][8];
+    for (int i = 0; i < 8; ++i)
+      for (int m = 0; m < 8; ++m) {
+        double w = 1;
+        for (int a = 0; a < 3; ++a) {
+          const int t = cp[a] + ((i >> a) & 1);
+          w *= ((m >> a) & 1) ? t / 4.0 : (4 - t) / 4.0;
+        }
+        P[i][m] = w;
+      }
+    for (int m = 0; m < 8; ++m)
+      for (int r = 0; r < 3; ++r)
+        for (int n = 0; n < 8; ++n)
+          for (int s2 = 0; s2 < 3; ++s2) {
+            double acc = 0;
+            for (int i = 0; i < 8; ++i) {
+              if (P[i][m] == 0) continue;
+              double row = 0;
+              for (int j = 0; j < 8; ++j) row += Ke[(3 * i + r) * 24 + 3 * j + s2] * P[j][n];
+              acc += P[i][m] * row;
+            }
+            Mc4[c * 576 + (3 * m + r) * 24 + 3 * n + s2] = acc;
+          }
+  }
+}
+
 LevelView make_view(int ex, int ey, int ez) {
   LevelView v{};
   v.ex = ex; v.ey = ey; v.ez = ez;
@@ -276,7 +310,7 @@ RigidProj rigid_projector_region(int n, const double* pos, const double* nrm, st
   return rigid_projector(G, o);
 }
 
-void build_levels(Ctx& c, int n_levels, int threshold) {
+void build_levels(Ctx& c, int n_levels, int threshold, bool skip) {
   struct HL {
     int ex, ey, ez;
     std::vector<uint8_t> act;
@@ -302,6 +336,9 @@ void build_levels(Ctx& c, int n_levels, int threshold) {
             C.act[(x >> 1) + (size_t)C.ex * ((y >> 1) + (size_t)C.ey * (z >> 1))] = 1;
     hl.push_back(std::move(C));
   }
+  // skip-layer: drop dyadic level 1 (level 2's elements are the 4x4x4 blocks of level 0)
+  c.skip = skip && hl.size() >= 3;
+  if (c.skip) hl.erase(hl.begin() + 1);
   c.L.resize(hl.size());
   for (size_t l = 0; l < hl.size(); ++l) {
     Level& L = c.L[l];
@@ -329,6 +366,16 @@ void build_levels(Ctx& c, int n_levels, int threshold) {
       for (auto a : L.h_free) n_free0 += a;
       c.floating = n_free0 > 0 && n_free0 == n_act_nodes;
       if (c.floating) c.rig_all = rigid_projector_nodes(L.h_free, v.nx, v.ny, v.nz);
+    } else if (l == 1 && c.skip) {
+      // every node of an active element carries an unknown (supports enter the Galerkin
+      // operator as a penalty, not as eliminated nodes)
+      for (int z = 0; z < v.ez; ++z)
+        for (int y = 0; y < v.ey; ++y)
+          for (int x = 0; x < v.ex; ++x) {
+            if (!L.h_eact[x + (size_t)v.ex * (y + (size_t)v.ey * z)]) continue;
+            for (int j = 0; j < 8; ++j)
+              L.h_free[(x + (j & 1)) + (size_t)v.nx * ((y + ((j >> 1) & 1)) + (size_t)v.ny * (z + (j >> 2)))] = 1;
+          }
     } else {
       const Level& F = c.L[l - 1];
       const LevelView& fv = F.v;
@@ -1059,15 +1106,23 @@ void vcycle_level(Ctx& c, int l, const T* f, T* u, int pre, int post) {
   else
     apply_level<T, 1>(c, l, u, f, r);
   Level& C = c.L[l + 1];
+  const bool four = l == 0 && c.skip;  // skip-layer: factor-4 transfers
   if (C.n_lin) {
     Prof pr(c, P_TRANSFER);
-    hipLaunchKernelGGL((k_restrict<T>), dim3(cblk(C.n_lin)), dim3(256), 0, c.stream, L.v, C.v,
-                       (const int4*)C.cl_lin, C.n_lin, (const T*)r, Prec<T>::f(C));
+    if (four)
+      hipLaunchKernelGGL((k_restrict4<T>), dim3(cblk(C.n_lin)), dim3(256), 0, c.stream, L.v, C.v,
+                         (const int4*)C.cl_lin, C.n_lin, (const T*)r, Prec<T>::f(C));
+    else
+      hipLaunchKernelGGL((k_restrict<T>), dim3(cblk(C.n_lin)), dim3(256), 0, c.stream, L.v, C.v,
+                         (const int4*)C.cl_lin, C.n_lin, (const T*)r, Prec<T>::f(C));
   }
   vcycle_level<T>(c, l + 1, Prec<T>::f(C), Prec<T>::u(C), pre, post);
   {
     Prof pr(c, P_TRANSFER);
-    if (L.n_segp)
+    if (four)
+      hipLaunchKernelGGL((k_prolong4<T>), dim3(cblk(L.n_lin)), dim3(256), 0, c.stream, L.v, C.v,
+                         (const int4*)L.cl_lin, L.n_lin, (const T*)Prec<T>::u(C), u);
+    else if (L.n_segp)
       hipLaunchKernelGGL((k_prolong_add_pairw<T>), dim3((L.n_segp + 3) / 4), dim3(256), 0, c.stream, L.v, C.v,
                          (const int4*)L.segp, L.n_segp, (const uint8_t*)L.fre, (const T*)Prec<T>::u(C), u);
     else
@@ -2529,7 +2584,7 @@ TOS_EXPORT int tos_ctx_create(const tos_grid_desc* d, tos_ctx** out) {
     c->h_solid.assign(d->solid, d->solid + ne);
     if (d->support) c->h_support.assign(d->support, d->support + nn);
     if (d->shell) c->h_shell.assign(d->shell, d->shell + ne);
-    build_levels(*c, d->n_levels, d->coarsen_threshold > 0 ? d->coarsen_threshold : 400);
+    build_levels(*c, d->n_levels, d->coarsen_threshold > 0 ? d->coarsen_threshold : 400, d->skip_layer != 0);
     c->n_active = c->L[0].n_active;
     for (size_t l = 0; l < c->L.size(); ++l) {
       Level& L = c->L[l];
@@ -2574,6 +2629,25 @@ TOS_EXPORT int tos_ctx_create(const tos_grid_desc* d, tos_ctx** out) {
           }
       c->allfree = dalloc<uint8_t>(*c, v0.NE);
       upload(c->stream, c->allfree, af.data(), v0.NE);
+      if (c->skip) {
+        // which nodes of each active fine element are supported (the penalty of the
+        // skip-layer Galerkin operator)
+        std::vector<uint8_t> fx(v0.NE, 0);
+        if (!c->h_support.empty())
+          for (int z = 0; z < v0.ez; ++z)
+            for (int y = 0; y < v0.ey; ++y)
+              for (int x = 0; x < v0.ex; ++x) {
+                if (!pe[v0.pelem(x, y, z)]) continue;
+                uint8_t m = 0;
+                for (int j = 0; j < 8; ++j)
+                  if (c->h_support[(x + (j & 1)) + (size_t)v0.nx * ((y + ((j >> 1) & 1)) + (size_t)v0.ny * (z + (j >> 2)))])
+                    m |= (uint8_t)(1u << j);
+                fx[v0.pelem(x, y, z)] = m;
+              }
+        c->efix = dalloc<uint8_t>(*c, v0.NE);
+        upload(c->stream, c->efix, fx.data(), v0.NE);
+        c->Mc4 = dalloc<double>(*c, 64 * 576);
+      }
     }
     c->shell = dalloc<uint8_t>(*c, v0.NE);
     auto ps = pad_elems(v0, c->h_shell.empty() ? nullptr : c->h_shell.data());
@@ -2710,6 +2784,11 @@ TOS_EXPORT int tos_set_material(tos_ctx* c, double E, double nu, double p, doubl
     upload(c->stream, c->Kc64, kc, sizeof(kc));
     upload(c->stream, c->Kc32, kc32, sizeof(kc32));
     upload(c->stream, c->Mc64, c->h_Mc, sizeof(c->h_Mc));
+    if (c->skip) {
+      std::vector<double> m4(64 * 576);
+      child_products4(c->h_Ke, m4.data());
+      upload(c->stream, c->Mc4, m4.data(), m4.size() * sizeof(double));
+    }
     c->op_ready = false;
     return TOS_OK;
   });
@@ -2859,7 +2938,12 @@ void galerkin_levels(Ctx& c, bool copy32) {
     Level& F = c.L[l - 1];
     Level& C = c.L[l];
     if (C.n_col == 0) continue;
-    if (l == 1)
+    if (l == 1 && c.skip)
+      hipLaunchKernelGGL(k_galerkin_skip, dim3(cblk(C.n_col), 3), dim3(256), 0, c.stream, F.v, C.v,
+                         (const int4*)C.cl_col, C.n_col, (const double*)c.w64, (const uint8_t*)c.efix,
+                         (const double*)c.Ke64, (const double*)c.Mc4, d64 ? C.st64 : (double*)nullptr,
+                         (!d64 || copy32) ? C.st32 : (float*)nullptr, C.NS);
+    else if (l == 1)
       hipLaunchKernelGGL(k_galerkin_first_row, dim3(cblk(C.n_col), 9), dim3(256), 0, c.stream, F.v, C.v,
                          (const int4*)C.cl_col, C.n_col, (const uint8_t*)F.fre, (const uint8_t*)c.allfree,
                          (const uint8_t*)C.fre, (const double*)c.w64, (const double*)c.Ke64,
@@ -2996,6 +3080,7 @@ TOS_EXPORT int tos_solve_batched(tos_ctx* c, const tos_solve_opts* o, tos_solve_
   return guard(c, [&]() {
     REQUIRE(o && o->rtol > 0 && o->maxit >= 0 && o->pre >= 0 && o->post >= 0, "bad options");
     REQUIRE(o->precision == 1 && o->use_pcg, "the batched solve is FCG with an FP32 V-cycle");
+    REQUIRE(!c->skip, "the batched solve has no skip-layer transfers");
     if (!c->op_ready) { c->err = "update_operator first"; return TOS_E_STATE; }
     const auto t0 = std::chrono::steady_clock::now();
     const int rc = solve_batched(*c, *o, st);
@@ -4260,6 +4345,7 @@ TOS_EXPORT int tos_ctx_learn_attach(tos_ctx* c, tos_unet* net, int level, int mo
       return TOS_OK;
     }
     REQUIRE(level >= 1 && level + 1 < (int)c->L.size(), "learned level must have a coarser level below it");
+    REQUIRE(!c->skip, "the learned correction takes dyadic hierarchies only");
     learn_prepare(*c, level);
     LearnState& S = *c->learn;
     const Level& L = c->L[level];
@@ -4282,6 +4368,7 @@ TOS_EXPORT int tos_ctx_learn_log(tos_ctx* c, tos_dataset* ds, int level, int n_i
       return TOS_OK;
     }
     REQUIRE(level >= 1 && level + 1 < (int)c->L.size(), "logged level must have a coarser level below it");
+    REQUIRE(!c->skip, "the learned correction takes dyadic hierarchies only");
     REQUIRE(n_inner >= 1 && stride >= 1, "bad arguments");
     const Level& L = c->L[level];
     REQUIRE(ds->g.X == L.v.nx && ds->g.Y == L.v.ny && ds->g.Z == L.v.nz, "dataset grid differs from the level");

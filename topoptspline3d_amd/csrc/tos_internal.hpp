@@ -187,6 +187,11 @@ struct Ctx {
   int64_t* wc_idx = nullptr;
   double *wc_nrm = nullptr, *wc_fs = nullptr, *wc_src = nullptr;
   double* wc_pos = nullptr;     // load-node coordinates (3 per node), floating contexts
+  // skip-layer hierarchy: level 1 is 4x coarser than level 0 (factor-4 transfers, Galerkin
+  // with the Dirichlet penalty)
+  bool skip = false;
+  uint8_t* efix = nullptr;      // per padded fine element: bit k = its node k is supported
+  double* Mc4 = nullptr;        // 64 x 576: T_c^T Ke T_c per child position c of a coarse element
   // floating body (no supported fine node): the operator's null space is the rigid motions
   bool floating = false;
   double* crig = nullptr;       // coarsest level: orthonormal rigid motions, column k at crig[k * cn]

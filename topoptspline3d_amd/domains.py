@@ -23,6 +23,41 @@ def cantilever(nx, ny, nz):
     return solid, support, f.reshape(-1)
 
 
+def embed_in_cube(solid, support=None, f=None, shell=None):
+    """The reference's grid shape (padd2SatifyResolution, voxelizer/voxelize_main.cpp:
+    312-340): the domain centred in the smallest power-of-two cube strictly larger than its
+    longest side, at offset (side - n) // 2 per axis (24x12x12 -> 32^3 at (4, 10, 10)).
+    Arrays are [z, y, x]; f is the flat (3, nz+1, ny+1, nx+1) load vector.  Returns
+    (solid, support, f, shell, offset (x, y, z)) of the cube; absent inputs stay None."""
+    nz, ny, nx = solid.shape
+    side = 1
+    while side <= max(nx, ny, nz):
+        side *= 2
+    ox, oy, oz = ((side - nx) // 2, (side - ny) // 2, (side - nz) // 2)
+    S = np.zeros((side, side, side), solid.dtype)
+    S[oz:oz + nz, oy:oy + ny, ox:ox + nx] = solid
+    out = [S]
+    if support is not None:
+        U = np.zeros((side + 1,) * 3, support.dtype)
+        U[oz:oz + nz + 1, oy:oy + ny + 1, ox:ox + nx + 1] = support
+        out.append(U)
+    else:
+        out.append(None)
+    if f is not None:
+        F = np.zeros((3,) + (side + 1,) * 3)
+        F[:, oz:oz + nz + 1, oy:oy + ny + 1, ox:ox + nx + 1] = np.asarray(f).reshape(3, nz + 1, ny + 1, nx + 1)
+        out.append(F.reshape(-1))
+    else:
+        out.append(None)
+    if shell is not None:
+        H = np.zeros((side, side, side), shell.dtype)
+        H[oz:oz + nz, oy:oy + ny, ox:ox + nx] = shell
+        out.append(H)
+    else:
+        out.append(None)
+    return (*out, (ox, oy, oz))
+
+
 def face_load_region(nx, ny, nz, face="x+", inset=0):
     """Load region for the worst-case load search: the nodes of one box face ("x+",
     "y-", ...) at least `inset` nodes away from that face's border, with the face's
