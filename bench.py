@@ -433,6 +433,8 @@ def main():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-worst", action="store_true", help="skip the worst-case load search")
     ap.add_argument("--no-surface", action="store_true", help="skip the surface-constraint timing")
+    ap.add_argument("--surface-only", action="store_true",
+                    help="only the warmup iterations and the surface-constraint timing (profiling)")
     ap.add_argument("--learn-domains", type=int, default=4)
     ap.add_argument("--learn-iters", type=int, default=50)
     ap.add_argument("--train-steps", type=int, default=1500)
@@ -502,6 +504,11 @@ def main():
             loop.ctx.close()
         return out
 
+    if args.surface_only:
+        loop, *_ = run("plain", args.precision, False, 0, keep=True)
+        print(json.dumps({"surface_constraints": surface_phase(loop.ctx)}), flush=True)
+        loop.ctx.close()
+        return
     # headline: plain mixed-precision MG-PCG
     loop, ms, its, res_max = run("plain", args.precision, False, args.steps, keep=True)
     info = loop.ctx.info()
