@@ -433,6 +433,8 @@ def main():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-worst", action="store_true", help="skip the worst-case load search")
     ap.add_argument("--no-surface", action="store_true", help="skip the surface-constraint timing")
+    ap.add_argument("--probe-element-mfma", action="store_true",
+                    help="only the warmup iterations and the MFMA element-product measurement")
     ap.add_argument("--surface-only", action="store_true",
                     help="only the warmup iterations and the surface-constraint timing (profiling)")
     ap.add_argument("--learn-domains", type=int, default=4)
@@ -504,6 +506,18 @@ def main():
             loop.ctx.close()
         return out
 
+    if args.probe_element_mfma:
+        loop, *_ = run("plain", args.precision, False, 0, keep=True)
+        ne = loop.ctx.info()["n_active_elements"][0]
+        pr = loop.ctx.probe_element_mfma(reps=10)
+        # useful work: 576 FMAs per element (the MFMA tiles execute 32/24 of it and more
+        # where a row run has fewer than 16 elements)
+        pr["useful_tflops"] = 2 * 576 * ne / (pr["mfma_products_ms"] * 1e-3) / 1e12
+        pr["valu_useful_tflops"] = 2 * 576 * ne / (pr["valu_ms"] * 1e-3) / 1e12
+        pr["active_elements"] = ne
+        print(json.dumps({"element_mfma": pr}), flush=True)
+        loop.ctx.close()
+        return
     if args.surface_only:
         loop, *_ = run("plain", args.precision, False, 0, keep=True)
         print(json.dumps({"surface_constraints": surface_phase(loop.ctx)}), flush=True)

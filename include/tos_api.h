@@ -378,6 +378,16 @@ int tos_ctx_mma_step(tos_ctx* ctx, tos_mma* mma, double sens_scale, double vol_s
 int tos_ctx_mma_step_surface(tos_ctx* ctx, tos_mma* mma, double sens_scale, double vol_scale, double vfrac,
                              double ss_scale, double drip_scale, int* converged);
 
+/* Matrix-core element products against the matrix-free operator: a measurement, no
+ * counterpart in the reference.  On the fine level and a seeded u (free nodes) it times
+ * `reps` launches each of y = K u in FP32 by the matrix-free VALU kernel, by element
+ * products on v_mfma_f32_16x16x4f32 (16 elements per wave as the columns of a 24x24 by
+ * 24x16 product; gathers and products only), and by the same with a float-atomic scatter
+ * into the nodes.  out[4] = {VALU ms, MFMA products ms, MFMA + scatter ms, relative
+ * difference of the scattered result from the VALU one over the free nodes}.  Allocates
+ * and frees its own buffers. */
+int tos_probe_element_mfma(tos_ctx* ctx, int reps, double* out);
+
 /* Fine-level Gauss-Seidel schedule (no counterpart in the reference, which launches one
  * pass per colour): 0 = eight colour passes; 1 = colour-slab wavefront, colour c of z-slab
  * s relaxed in step s + 2c (about 32 slabs); 2 = four launches of two colours each, one

@@ -2628,3 +2628,12 @@ ORC_API int orc_set_graph_replay(tos_ctx* c, int enable) {
   (void)enable;
   return c ? TOS_OK : TOS_E_ARG;
 }
+
+// a measurement of matrix-core kernels: nothing to measure on the host
+ORC_API int orc_probe_element_mfma(tos_ctx* c, int reps, double* out) {
+  (void)reps;
+  (void)out;
+  if (!c) return TOS_E_ARG;
+  c->err = "the matrix-core measurement runs on the GPU library only";
+  return TOS_E_STATE;
+}

@@ -95,6 +95,7 @@ SIGNATURES = {
     "worst_compliance": (C.c_int, [vp, C.POINTER(WorstOpts), C.POINTER(WorstStats)]),
     "balance_loads": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
     "remove_rigid": (C.c_int, [vp, C.c_int]),
+    "probe_element_mfma": (C.c_int, [vp, C.c_int, C.POINTER(C.c_double)]),
     "sensitivity": (C.c_int, [vp, C.c_int]),
     "get_sensitivity": (C.c_int, [vp, C.c_int, f32p]),
     "set_sensitivity": (C.c_int, [vp, C.c_int, f32p]),

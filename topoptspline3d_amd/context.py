@@ -235,6 +235,14 @@ class Context:
         self._chk(self.api.balance_loads(self.h, k, nodes.size, _p(nodes, C.c_int64),
                                          _p(nrm, C.c_double) if nrm is not None else None), "balance_loads")
 
+    def probe_element_mfma(self, reps=10):
+        """times y = K u (FP32) by the matrix-free VALU kernel and by MFMA element products
+        (products only / with a float-atomic scatter); returns ms each and the relative
+        difference of the scattered result"""
+        out = np.zeros(4)
+        self._chk(self.api.probe_element_mfma(self.h, int(reps), _p(out, C.c_double)), "probe_element_mfma")
+        return {"valu_ms": out[0], "mfma_products_ms": out[1], "mfma_scatter_ms": out[2], "rel_diff": out[3]}
+
     def remove_rigid(self, k=0):
         """displacementProject (optimization/projection.cpp:374-395), floating bodies only"""
         self._chk(self.api.remove_rigid(self.h, k), "remove_rigid")

@@ -1086,7 +1086,7 @@ __global__ __launch_bounds__(256) void k_galerkin_skip(LevelView F, LevelView C,
   __shared__ double sM[64 * 24];
   const int r = blockIdx.y;
   NodeRef n;
-  const bool ok = chunk_node(cl, nch, 1, n);
+  const bool ok = chunk_node(cl, nch, 2, n);  // colour list: every other x
   double acc[81];
 #pragma unroll
   for (int t = 0; t < 81; ++t) acc[t] = 0.0;

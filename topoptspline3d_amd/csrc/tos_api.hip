@@ -21,6 +21,7 @@
 #include "kernels_nn.hip"
 #include "kernels_opt.hip"
 #include "kernels_surface.hip"
+#include "kernels_probe.hip"
 #include "mma.hip"
 
 #define TOS_EXPORT extern "C" __attribute__((visibility("default")))
@@ -4463,4 +4464,91 @@ TOS_EXPORT int tos_oc_update_noalloc(int64_t n, const uint8_t* active, const uin
     if (steps_out) *steps_out = (int)h[S_OC_STEPS];
   }
   return TOS_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// Matrix-core element products against the matrix-free operator (a measurement; no
+// counterpart in the reference).  On the fine level and a seeded u (free nodes), times
+// `reps` launches each of y = K u in FP32 by the VALU kernel and by element products on
+// v_mfma_f32_16x16x4f32 with the gathers and products only, then with the float-atomic
+// scatter into the nodes; out = {VALU ms, MFMA products ms, MFMA + scatter ms, relative
+// difference of the scattered result from the VALU one over the free nodes}.
+TOS_EXPORT int tos_probe_element_mfma(tos_ctx* c, int reps, double* out) {
+  return guard(c, [&]() {
+    REQUIRE(out && reps >= 1, "bad arguments");
+    if (!c->op_ready) { c->err = "update_operator first"; return TOS_E_STATE; }
+    Ctx& C = *c;
+    Level& L0 = C.L[0];
+    const LevelView& v = L0.v;
+    std::vector<int4> tiles;  // runs of up to 16 active elements along x
+    for (int z = 0; z < v.ez; ++z)
+      for (int y = 0; y < v.ey; ++y) {
+        int x = 0;
+        while (x < v.ex) {
+          if (!L0.h_eact[x + (size_t)v.ex * (y + (size_t)v.ey * z)]) { ++x; continue; }
+          int x1 = x;
+          while (x1 < v.ex && x1 - x < 16 && L0.h_eact[x1 + (size_t)v.ex * (y + (size_t)v.ey * z)]) ++x1;
+          tiles.push_back(make_int4(x, x1 - x, y, z));
+          x = x1;
+        }
+      }
+    const int nt = (int)tiles.size();
+    const unsigned nbt = (unsigned)std::max(1, std::min(4 * 2048, (nt + 3) / 4));
+    std::vector<void*> mem;
+    int4* d_t = (int4*)raw_alloc(mem, C.stream, std::max<size_t>(1, tiles.size()) * sizeof(int4));
+    float* uu = (float*)raw_alloc(mem, C.stream, 3 * v.NP * sizeof(float));
+    float* y1 = (float*)raw_alloc(mem, C.stream, 3 * v.NP * sizeof(float));
+    float* y2 = (float*)raw_alloc(mem, C.stream, 3 * v.NP * sizeof(float));
+    float* chk = (float*)raw_alloc(mem, C.stream, (size_t)nbt * 256 * sizeof(float));
+    auto release = [&]() {
+      (void)hipStreamSynchronize(C.stream);
+      for (void* p : mem) (void)hipFree(p);
+    };
+    try {
+      if (nt) upload(C.stream, d_t, tiles.data(), tiles.size() * sizeof(int4));
+      hipLaunchKernelGGL(k_probe_fill, dim3(cblk(L0.n_lin)), dim3(256), 0, C.stream, (const int4*)L0.cl_lin,
+                         L0.n_lin, v.NP, uu, 12345u);
+      hipEvent_t e0, e1;
+      HIPCHK(hipEventCreate(&e0));
+      HIPCHK(hipEventCreate(&e1));
+      auto timed = [&](auto launch) {
+        launch();  // warm-up
+        HIPCHK(hipEventRecord(e0, C.stream));
+        for (int i = 0; i < reps; ++i) launch();
+        HIPCHK(hipEventRecord(e1, C.stream));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        return (double)ms / reps;
+      };
+      out[0] = timed([&]() {
+        hipLaunchKernelGGL((k_fine_apply<float, 0, float>), dim3(cblk(L0.n_lin)), dim3(256), 0, C.stream, v,
+                           (const int4*)L0.cl_lin, L0.n_lin, (const float*)C.w32, (const float*)C.Ke32,
+                           (const float*)uu, (const float*)nullptr, y1, (double*)nullptr, (float*)nullptr);
+      });
+      out[1] = timed([&]() {
+        hipLaunchKernelGGL(k_probe_elem_mfma<0>, dim3(nbt), dim3(256), 0, C.stream, v, (const int4*)d_t, nt,
+                           (const float*)C.w32, (const float*)C.Ke32, (const float*)uu, y2, chk);
+      });
+      out[2] = timed([&]() {
+        hipLaunchKernelGGL(k_zero<float>, dim3(nblk(3 * v.NP)), dim3(256), 0, C.stream, y2, 3 * v.NP);
+        hipLaunchKernelGGL(k_probe_elem_mfma<1>, dim3(nbt), dim3(256), 0, C.stream, v, (const int4*)d_t, nt,
+                           (const float*)C.w32, (const float*)C.Ke32, (const float*)uu, y2, chk);
+      });
+      hipLaunchKernelGGL(k_probe_diff, dim3(C.nb0), dim3(256), 0, C.stream, (const int4*)L0.cl_lin, L0.n_lin, v.NP,
+                         (const float*)y2, (const float*)y1, C.partial);
+      hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, C.stream, (const double*)C.partial, (int)C.nb0, 1, S_TMP0,
+                         S_TMP1, OP_NONE, C.scal);
+      LAUNCH_CHECK();
+      const double* h = read_scal(C);
+      out[3] = h[S_TMP1] > 0 ? std::sqrt(h[S_TMP0] / h[S_TMP1]) : 0.0;
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+    } catch (...) {
+      release();
+      throw;
+    }
+    release();
+    return TOS_OK;
+  });
 }
