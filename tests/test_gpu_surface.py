@@ -87,6 +87,27 @@ def test_surface_constraints_match_oracle(hip, mode, order, param, opts):
         assert np.abs(gg - go).max() <= 1e-5 * scale, which
 
 
+@pytest.mark.parametrize("cells", [64, 150])
+def test_surface_constraints_at_finer_lattices(hip, cells):
+    """the reference's 64-cell background lattice and a finer one (the lattice counts are
+    scanned over 140 and 1700 blocks of 2048)"""
+    nx, ny, nz = 28, 20, 24
+    (o, g), n = pair(hip, nx, ny, nz, 1, 4, 4)
+    cf = coeffs_for(o, nx, ny, nz, 1, 4, 4)
+    out = []
+    for c in (o, g):
+        c.set_coeffs(cf)
+        out.append(c.surface_constraints(cells=cells, beta=8.0))
+    so, sg = out
+    assert so["n_points"] == sg["n_points"] > 1000
+    po, _ = o.surface_points(so["n_points"])
+    pg, _ = g.surface_points(sg["n_points"])
+    assert np.abs(pg - po).max() <= 1e-9
+    assert abs(sg["ss_value"] - so["ss_value"]) <= 1e-9 * max(1.0, abs(so["ss_value"]))
+    go, gg = o.coeff_sens(2), g.coeff_sens(2)
+    assert np.abs(gg - go).max() <= 1e-5 * max(np.abs(go).max(), 1e-30)
+
+
 def test_equal_coefficients_give_the_drivers_constants(hip):
     (o, g), n = pair(hip, 16, 12, 12, 1, 4, 4)
     for c in (o, g):

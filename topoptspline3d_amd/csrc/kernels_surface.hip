@@ -225,6 +225,67 @@ __global__ __launch_bounds__(1024) void k_sf_scan(const int* __restrict__ cnt, i
   if (t == 1023) off[n] = part[1023];
 }
 
+// Exclusive prefix sum over many workgroups: k_sf_scan_blocks sums each 2048-count block,
+// k_sf_scan (one workgroup) scans those sums, k_sf_scan_apply scans each block locally
+// and adds its block offset; off[n] = total.
+constexpr int kSfScanBlock = 2048;
+__global__ __launch_bounds__(1024) void k_sf_scan_blocks(const int* __restrict__ cnt, int64_t n,
+                                                         int* __restrict__ bsum) {
+  __shared__ int sh[1024];
+  const int64_t b0 = (int64_t)blockIdx.x * kSfScanBlock;
+  const int t = threadIdx.x;
+  int s = 0;
+  for (int k = 0; k < 2; ++k) {
+    const int64_t i = b0 + 2 * t + k;
+    if (i < n) s += cnt[i];
+  }
+  sh[t] = s;
+  __syncthreads();
+  for (int o = 512; o > 0; o >>= 1) {
+    if (t < o) sh[t] += sh[t + o];
+    __syncthreads();
+  }
+  if (t == 0) bsum[blockIdx.x] = sh[0];
+}
+
+__global__ __launch_bounds__(1024) void k_sf_scan_apply(const int* __restrict__ cnt, int64_t n,
+                                                        const int* __restrict__ boff, int* __restrict__ off) {
+  __shared__ int sh[1024];
+  const int64_t b0 = (int64_t)blockIdx.x * kSfScanBlock;
+  const int t = threadIdx.x;
+  const int64_t i0 = b0 + 2 * t;
+  const int c0 = i0 < n ? cnt[i0] : 0, c1 = i0 + 1 < n ? cnt[i0 + 1] : 0;
+  sh[t] = c0 + c1;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {  // inclusive scan of the thread pairs
+    const int v = t >= o ? sh[t - o] : 0;
+    __syncthreads();
+    sh[t] += v;
+    __syncthreads();
+  }
+  const int base = boff[blockIdx.x] + sh[t] - (c0 + c1);
+  if (i0 < n) off[i0] = base;
+  if (i0 + 1 < n) off[i0 + 1] = base + c0;
+  if (i0 + 1 == n - 1 || i0 == n - 1) off[n] = base + c0 + c1;  // the last count's block writes the total
+}
+
+// points per lattice row (all i at fixed j, k): lets k_sf_grad skip empty rows
+__global__ void k_sf_rows(SfLattice G, const int* __restrict__ off, int* __restrict__ rows) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= (int64_t)G.NB * G.NB) return;
+  rows[r] = off[(r + 1) * G.NB] - off[r * G.NB];
+}
+
+// 1 when some coefficient differs from the first (partials summed)
+__global__ __launch_bounds__(256) void k_sf_coef_differ(const float* __restrict__ coef, int64_t n,
+                                                        double* __restrict__ partial) {
+  double s = 0;
+  const float c0 = coef[0];
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+    s += coef[i] != c0 ? 1.0 : 0.0;
+  block_partial(s, partial, 0);
+}
+
 __global__ void k_sf_points(SfLattice G, LevelView L, const uint8_t* __restrict__ eact,
                             const double* __restrict__ val, double h, const int* __restrict__ off,
                             double* __restrict__ pts) {
@@ -400,7 +461,7 @@ __device__ __forceinline__ void sf_basis_at(const SfAxis& A, double x, int ci, d
 
 // coefficient gradients: per coefficient, the points of the lattice nodes that can reach
 // its support, in lattice-node order; gss/gdr = factor * sum_points w . (basis derivatives)
-__global__ void k_sf_grad(SfAxes S, SfLattice G, int64_t ncoef, const int* __restrict__ off,
+__global__ void k_sf_grad(SfAxes S, SfLattice G, int64_t ncoef, const int* __restrict__ off, const int* __restrict__ rows,
                           const double* __restrict__ pts, const double* __restrict__ wss,
                           const double* __restrict__ wdr, double fss, double fdr, float* __restrict__ gss,
                           float* __restrict__ gdr) {
@@ -419,7 +480,8 @@ __global__ void k_sf_grad(SfAxes S, SfLattice G, int64_t ncoef, const int* __res
   }
   double a_ss = 0, a_dr = 0;
   for (int kk = nlo[2]; kk <= nhi[2]; ++kk)
-    for (int jj = nlo[1]; jj <= nhi[1]; ++jj)
+    for (int jj = nlo[1]; jj <= nhi[1]; ++jj) {
+      if (rows[jj + (int64_t)G.NB * kk] == 0) continue;  // no point in this lattice row
       for (int ii = nlo[0]; ii <= nhi[0]; ++ii) {
         const int64_t node = ii + (int64_t)G.NB * (jj + (int64_t)G.NB * kk);
         for (int t = off[node]; t < off[node + 1]; ++t) {
@@ -436,6 +498,7 @@ __global__ void k_sf_grad(SfAxes S, SfLattice G, int64_t ncoef, const int* __res
                   wd[5] * X[1] * Y[1] * Z[0];
         }
       }
+    }
   gss[j] = (float)(fss * a_ss);
   gdr[j] = (float)(fdr * a_dr);
 }

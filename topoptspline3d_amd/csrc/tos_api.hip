@@ -3710,6 +3710,10 @@ TOS_EXPORT int tos_surface_constraints(tos_ctx* c, const tos_surface_opts* o, to
       C.sf_val = dalloc<double>(C, nl);
       C.sf_cnt = dalloc<int>(C, nl);
       C.sf_off = dalloc<int>(C, nl + 1);
+      const int64_t nsb = (nl + kSfScanBlock - 1) / kSfScanBlock;
+      C.sf_bsum = dalloc<int>(C, nsb);
+      C.sf_boff = dalloc<int>(C, nsb + 1);
+      C.sf_rows = dalloc<int>(C, (int64_t)NB * NB);
       C.sf_pts = dalloc<double>(C, 3 * cap);
       C.sf_grad = dalloc<double>(C, 3 * cap);
       C.sf_hess = dalloc<double>(C, 6 * cap);
@@ -3723,10 +3727,13 @@ TOS_EXPORT int tos_surface_constraints(tos_ctx* c, const tos_surface_opts* o, to
       C.dripcsens = dalloc<float>(C, ncoef);
     }
     {  // all coefficients equal: no surface; the driver's constants (utils/test_utils.cpp:1106-1113)
-      std::vector<float> h(ncoef);
-      download(C.stream, h.data(), C.coef, ncoef * sizeof(float));
-      bool equal = true;
-      for (float x : h) equal = equal && x == h[0];
+      const unsigned nbq = (unsigned)std::min<int64_t>(kRedBlocks, ((int64_t)ncoef + 255) / 256);
+      hipLaunchKernelGGL(k_sf_coef_differ, dim3(nbq), dim3(256), 0, C.stream, (const float*)C.coef,
+                         (int64_t)ncoef, C.partial);
+      hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, C.stream, (const double*)C.partial, (int)nbq, 0, S_TMP0, 0,
+                         OP_NONE, C.scal);
+      LAUNCH_CHECK();
+      const bool equal = read_scal(C)[S_TMP0] == 0.0;
       if (equal) {
         hipLaunchKernelGGL(k_zero<float>, dim3(nblk((int64_t)ncoef)), dim3(256), 0, C.stream, C.sscsens,
                            (int64_t)ncoef);
@@ -3749,7 +3756,16 @@ TOS_EXPORT int tos_surface_constraints(tos_ctx* c, const tos_surface_opts* o, to
     hipLaunchKernelGGL(k_sf_lattice, dim3(nblk(nl)), dim3(256), 0, C.stream, S, G, (const float*)C.coef, C.sf_val);
     hipLaunchKernelGGL(k_sf_count, dim3(nblk(nl)), dim3(256), 0, C.stream, G, v, (const uint8_t*)C.eact,
                        (const double*)C.sf_val, C.h, C.sf_cnt);
-    hipLaunchKernelGGL(k_sf_scan, dim3(1), dim3(1024), 0, C.stream, (const int*)C.sf_cnt, nl, C.sf_off);
+    {
+      const unsigned nsb = (unsigned)((nl + kSfScanBlock - 1) / kSfScanBlock);
+      hipLaunchKernelGGL(k_sf_scan_blocks, dim3(nsb), dim3(1024), 0, C.stream, (const int*)C.sf_cnt, nl, C.sf_bsum);
+      hipLaunchKernelGGL(k_sf_scan, dim3(1), dim3(1024), 0, C.stream, (const int*)C.sf_bsum, (int64_t)nsb,
+                         C.sf_boff);
+      hipLaunchKernelGGL(k_sf_scan_apply, dim3(nsb), dim3(1024), 0, C.stream, (const int*)C.sf_cnt, nl,
+                         (const int*)C.sf_boff, C.sf_off);
+      hipLaunchKernelGGL(k_sf_rows, dim3(nblk((int64_t)NB * NB)), dim3(256), 0, C.stream, G, (const int*)C.sf_off,
+                         C.sf_rows);
+    }
     LAUNCH_CHECK();
     int total = 0;
     download(C.stream, &total, C.sf_off + nl, sizeof(int));
@@ -3795,7 +3811,7 @@ TOS_EXPORT int tos_surface_constraints(tos_ctx* c, const tos_surface_opts* o, to
       dr_fac = o->drip_mode >= 6 ? 1.0 / cnt_dr : 0.0;
     }
     hipLaunchKernelGGL(k_sf_grad, dim3(nblk((int64_t)ncoef)), dim3(256), 0, C.stream, S, G, (int64_t)ncoef,
-                       (const int*)C.sf_off, (const double*)C.sf_pts, (const double*)C.sf_wss,
+                       (const int*)C.sf_off, (const int*)C.sf_rows, (const double*)C.sf_pts, (const double*)C.sf_wss,
                        (const double*)C.sf_wdr, ss_fac, dr_fac, C.sscsens, C.dripcsens);
     LAUNCH_CHECK();
     HIPCHK(hipStreamSynchronize(C.stream));
@@ -4523,7 +4539,7 @@ TOS_EXPORT int tos_probe_element_mfma(tos_ctx* c, int reps, double* out) {
       };
       out[0] = timed([&]() {
         hipLaunchKernelGGL((k_fine_apply<float, 0, float>), dim3(cblk(L0.n_lin)), dim3(256), 0, C.stream, v,
-                           (const int4*)L0.cl_lin, L0.n_lin, (const float*)C.w32, (const float*)C.Ke32,
+                           (const int4*)L0.cl_lin, L0.n_lin, (const float*)C.w32, (const float*)C.Kc32,
                            (const float*)uu, (const float*)nullptr, y1, (double*)nullptr, (float*)nullptr);
       });
       out[1] = timed([&]() {
