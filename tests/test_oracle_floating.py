@@ -212,3 +212,29 @@ def test_floating_worst_case_fixed_point_is_top_eigenpair(with_normals):
     assert abs(abs(fv @ v) - 1.0) <= 1e-7
     F, M = resultant(coords(c, ids), fs)
     assert np.abs(F).max() <= 1e-12 and np.abs(M).max() <= 1e-10
+
+
+def test_floating_solve_with_the_skip_layer():
+    """a floating body on the skip-layer hierarchy: the coarsest factor's rigid motions sit
+    on a level 8x coarser than the fine one, and the solve still returns pinv(K) f"""
+    nx, ny, nz = 16, 8, 8
+    solid = np.ones((nz, ny, nx), np.uint8)
+    c = Context(oracle_api(), solid, None, n_levels=4, skip_layer=True)
+    c.set_material(E, NU, P, RMIN)
+    rho = np.random.default_rng(8).uniform(0.2, 1.0, solid.size).astype(np.float32)
+    c.set_density(rho)
+    assert c.info()["n_levels"] == 3 and c.info()["level_dims"][2] == (2, 1, 1)
+    ids, _ = face_load_region(nx, ny, nz, "x+")
+    ids0, _ = face_load_region(nx, ny, nz, "x-")
+    both = np.r_[ids, ids0]
+    f = np.zeros((3, c.nn))
+    f[:, both] = np.random.default_rng(9).normal(size=(3, both.size))
+    c.set_loads(f.ravel())
+    c.balance_loads(both)
+    c.update_operator()
+    st = c.solve(rtol=1e-10, maxit=600)
+    assert st["rel_res"][0] <= 1e-10
+    c.remove_rigid()
+    K = global_K(c, solid, c.density()).toarray()
+    u_ref = np.linalg.pinv(K, rcond=1e-10, hermitian=True) @ c.get_loads(0)
+    assert np.linalg.norm(c.displacement() - u_ref) <= 1e-8 * np.linalg.norm(u_ref)
