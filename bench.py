@@ -398,14 +398,14 @@ def worst_phase(ctx, wl, precision):
     return out
 
 
-def surface_phase(ctx, reps=5):
+def surface_phase(ctx, reps=5, cells_list=(64, 128, 256)):
     """Self-supporting and drip constraints on the spline's surface points (deal_surface_points,
     optimization/optimization.cpp:1103-1128) for the current coefficients: the reference's
     64-cell background lattice and two finer ones, modes "oh"/"oh", beta 8; median of
     `reps` calls each (every call runs the whole chain: lattice values, vertex count and
     scan, points, orientation vote, constraint values and coefficient gradients)."""
     out = {}
-    for cells in (64, 128, 256):
+    for cells in cells_list:
         ts = []
         for _ in range(reps):
             sync()
@@ -437,6 +437,8 @@ def main():
                     help="only the warmup iterations and the MFMA element-product measurement")
     ap.add_argument("--surface-only", action="store_true",
                     help="only the warmup iterations and the surface-constraint timing (profiling)")
+    ap.add_argument("--surface-cells", type=int, nargs="*", default=[64, 128, 256],
+                    help="background lattice sizes of the surface-constraint timing")
     ap.add_argument("--learn-domains", type=int, default=4)
     ap.add_argument("--learn-iters", type=int, default=50)
     ap.add_argument("--train-steps", type=int, default=1500)
@@ -520,7 +522,7 @@ def main():
         return
     if args.surface_only:
         loop, *_ = run("plain", args.precision, False, 0, keep=True)
-        print(json.dumps({"surface_constraints": surface_phase(loop.ctx)}), flush=True)
+        print(json.dumps({"surface_constraints": surface_phase(loop.ctx, cells_list=args.surface_cells)}), flush=True)
         loop.ctx.close()
         return
     # headline: plain mixed-precision MG-PCG
@@ -596,7 +598,7 @@ def main():
         line["worst_case_load"] = worst_phase(loop.ctx, wl, min(args.precision, 2))  # V-cycle precision
     coeffs_final = None
     if wl["spline"] and not args.no_surface:
-        line["surface_constraints"] = surface_phase(loop.ctx)
+        line["surface_constraints"] = surface_phase(loop.ctx, cells_list=args.surface_cells)
         coeffs_final = loop.ctx.coeffs()
     loop.ctx.close()
     if args.batched_steps > 0 and args.workload == "c2":
