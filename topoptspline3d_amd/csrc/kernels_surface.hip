@@ -447,22 +447,41 @@ __device__ __forceinline__ void sf_support(const SfAxis& A, int i, double& lo, d
   }
 }
 
-// the contributions d/dc_j of this coefficient's basis at one point: sum over the terms q
-// whose (possibly clamped) index equals the coefficient's
-__device__ __forceinline__ void sf_basis_at(const SfAxis& A, double x, int ci, double out[3]) {
-  int idx[8];
-  double N[3][8];
+// per point and axis, the k coefficient indices and basis values / d / d2 (sf_axis), once:
+// bidx[(3 t + a) * 8 + q] (-1 past k or outside the spline's range),
+// bN[((3 t + a) * 3 + d) * 8 + q]
+__global__ void k_sf_basis(SfAxes S, int64_t n, const double* __restrict__ pts, int* __restrict__ bidx,
+                           double* __restrict__ bN) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  for (int a = 0; a < 3; ++a) {
+    int idx[8];
+    double N[3][8];
+    const bool ok = sf_axis(S.a[a], pts[3 * t + a], idx, N);
+    for (int q = 0; q < 8; ++q) {
+      const bool use = ok && q < S.a[a].k;
+      bidx[(3 * t + a) * 8 + q] = use ? idx[q] : -1;
+      for (int d = 0; d < 3; ++d) bN[((3 * t + a) * 3 + d) * 8 + q] = use ? N[d][q] : 0.0;
+    }
+  }
+}
+
+// the contributions d/dc of coefficient index ci along axis a at point t (sf_basis_at from
+// the per-point table): sum over the terms q whose (possibly clamped) index equals ci
+__device__ __forceinline__ void sf_basis_tab(const int* __restrict__ bidx, const double* __restrict__ bN, int64_t t,
+                                             int a, int ci, double out[3]) {
   out[0] = out[1] = out[2] = 0;
-  if (!sf_axis(A, x, idx, N)) return;
-  for (int q = 0; q < A.k; ++q)
-    if (idx[q] == ci)
-      for (int d = 0; d < 3; ++d) out[d] += N[d][q];
+  const int* ix = bidx + (3 * t + a) * 8;
+  const double* nn = bN + (3 * t + a) * 24;
+  for (int q = 0; q < 8; ++q)
+    if (ix[q] == ci)
+      for (int d = 0; d < 3; ++d) out[d] += nn[d * 8 + q];
 }
 
 // coefficient gradients: per coefficient, the points of the lattice nodes that can reach
 // its support, in lattice-node order; gss/gdr = factor * sum_points w . (basis derivatives)
 __global__ void k_sf_grad(SfAxes S, SfLattice G, int64_t ncoef, const int* __restrict__ off, const int* __restrict__ rows,
-                          const double* __restrict__ pts, const double* __restrict__ wss,
+                          const int* __restrict__ bidx, const double* __restrict__ bN, const double* __restrict__ wss,
                           const double* __restrict__ wdr, double fss, double fdr, float* __restrict__ gss,
                           float* __restrict__ gdr) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -486,10 +505,10 @@ __global__ void k_sf_grad(SfAxes S, SfLattice G, int64_t ncoef, const int* __res
         const int64_t node = ii + (int64_t)G.NB * (jj + (int64_t)G.NB * kk);
         for (int t = off[node]; t < off[node + 1]; ++t) {
           double X[3], Y[3], Z[3];
-          sf_basis_at(S.a[0], pts[3 * (int64_t)t], ci[0], X);
+          sf_basis_tab(bidx, bN, t, 0, ci[0], X);
           if (X[0] == 0 && X[1] == 0 && X[2] == 0) continue;
-          sf_basis_at(S.a[1], pts[3 * (int64_t)t + 1], ci[1], Y);
-          sf_basis_at(S.a[2], pts[3 * (int64_t)t + 2], ci[2], Z);
+          sf_basis_tab(bidx, bN, t, 1, ci[1], Y);
+          sf_basis_tab(bidx, bN, t, 2, ci[2], Z);
           const double gx = X[1] * Y[0] * Z[0], gy = X[0] * Y[1] * Z[0], gz = X[0] * Y[0] * Z[1];
           const double* ws = &wss[3 * (int64_t)t];
           const double* wd = &wdr[6 * (int64_t)t];

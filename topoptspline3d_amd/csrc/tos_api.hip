@@ -3810,8 +3810,16 @@ TOS_EXPORT int tos_surface_constraints(tos_ctx* c, const tos_surface_opts* o, to
       dr_val = o->drip_mode <= 1 ? -1.0 : dr_sum / cnt_dr;
       dr_fac = o->drip_mode >= 6 ? 1.0 / cnt_dr : 0.0;
     }
+    if (C.sf_bcap < std::max<int64_t>(n, 1)) {
+      C.sf_bcap = std::max<int64_t>(n, 1024) * 2;
+      C.sf_bidx = dalloc<int>(C, 24 * C.sf_bcap);
+      C.sf_bN = dalloc<double>(C, 72 * C.sf_bcap);
+    }
+    if (n)
+      hipLaunchKernelGGL(k_sf_basis, dim3(nblk(n)), dim3(256), 0, C.stream, S, n, (const double*)C.sf_pts,
+                         C.sf_bidx, C.sf_bN);
     hipLaunchKernelGGL(k_sf_grad, dim3(nblk((int64_t)ncoef)), dim3(256), 0, C.stream, S, G, (int64_t)ncoef,
-                       (const int*)C.sf_off, (const int*)C.sf_rows, (const double*)C.sf_pts, (const double*)C.sf_wss,
+                       (const int*)C.sf_off, (const int*)C.sf_rows, (const int*)C.sf_bidx, (const double*)C.sf_bN, (const double*)C.sf_wss,
                        (const double*)C.sf_wdr, ss_fac, dr_fac, C.sscsens, C.dripcsens);
     LAUNCH_CHECK();
     HIPCHK(hipStreamSynchronize(C.stream));

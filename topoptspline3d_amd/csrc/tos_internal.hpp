@@ -222,6 +222,9 @@ struct Ctx {
   double* sf_val = nullptr;
   int *sf_cnt = nullptr, *sf_off = nullptr;
   int *sf_bsum = nullptr, *sf_boff = nullptr, *sf_rows = nullptr;  // scan block sums / offsets, points per row
+  int* sf_bidx = nullptr;       // per point and axis: coefficient indices and basis values (grown on demand)
+  double* sf_bN = nullptr;
+  int64_t sf_bcap = 0;
   double *sf_pts = nullptr, *sf_grad = nullptr, *sf_hess = nullptr, *sf_nrm = nullptr;
   double *sf_wss = nullptr, *sf_wdr = nullptr;
   double* sf_knots[3] = {nullptr, nullptr, nullptr};  // clamped knots (spline mode 0)
