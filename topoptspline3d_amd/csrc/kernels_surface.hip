@@ -152,19 +152,75 @@ struct SfLattice {
   __device__ double pos(int a, int i) const { return -eh[a] + (i + 0.5) * eh[a]; }
 };
 
+// The lattice is separable: per axis and lattice index r = a * NB + i, the k coefficient
+// indices and basis values of the spline at the lattice position (sf_axis),
+// tab_i / tab_v[r * 8 + q], and tab_ok[r]: bit 0 inside the box, bit 1 inside the spline's
+// range.
+__global__ void k_sf_axis_tab(SfAxes S, SfLattice G, int* __restrict__ tab_i, double* __restrict__ tab_v,
+                              uint8_t* __restrict__ tab_ok) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= 3 * G.NB) return;
+  const int a = r / G.NB, i = r % G.NB;
+  const double p = G.pos(a, i);
+  int idx[8];
+  double N[3][8];
+  const bool ok = sf_axis(S.a[a], p, idx, N);
+  for (int q = 0; q < 8; ++q) {
+    const bool use = ok && q < S.a[a].k;
+    tab_i[r * 8 + q] = use ? idx[q] : 0;
+    tab_v[r * 8 + q] = use ? N[0][q] : 0.0;
+  }
+  tab_ok[r] = (uint8_t)((p >= 0.0 && p <= S.a[a].box ? 1 : 0) | (ok ? 2 : 0));
+}
+
 // node values: spline - 1/2 inside the box (-0.2 - 1/2 outside the spline range), -1/2
-// outside the box (compute_background_mcPoints_value grid/Grid.cu:3986-4080)
-__global__ void k_sf_lattice(SfAxes S, SfLattice G, const float* __restrict__ coef, double* __restrict__ val) {
+// outside the box (compute_background_mcPoints_value grid/Grid.cu:3986-4080), from the
+// per-axis tables; K = the spline order when fixed at compile time (registers), else 0
+template <int K>
+__global__ __launch_bounds__(256) void k_sf_lattice(SfLattice G, int k, int ncx, int ncy, const int* __restrict__ tab_i,
+                                                    const double* __restrict__ tab_v,
+                                                    const uint8_t* __restrict__ tab_ok, const float* __restrict__ coef,
+                                                    double* __restrict__ val) {
   const int64_t n = (int64_t)G.NB * G.NB * G.NB;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const int i = (int)(t % G.NB), j = (int)((t / G.NB) % G.NB), kk = (int)(t / ((int64_t)G.NB * G.NB));
-  const double p[3] = {G.pos(0, i), G.pos(1, j), G.pos(2, kk)};
-  bool in = true;
-  for (int a = 0; a < 3; ++a) in = in && p[a] >= 0.0 && p[a] <= S.a[a].box;
-  double phi = 0, g[3], H[6];
-  if (in && !sf_point(S, coef, p, phi, g, H)) phi = -0.2;
-  val[t] = (in ? phi : 0.0) - 0.5;
+  const int rx = i, ry = G.NB + j, rz = 2 * G.NB + kk;
+  const int ok = tab_ok[rx] & tab_ok[ry] & tab_ok[rz];
+  double phi = 0;
+  if ((ok & 1) && !(ok & 2)) phi = -0.2;
+  if (ok == 3 && K > 0) {
+    int xi[K > 0 ? K : 1];
+    double xv[K > 0 ? K : 1];
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+      xi[q] = tab_i[rx * 8 + q];
+      xv[q] = tab_v[rx * 8 + q];
+    }
+    for (int qz = 0; qz < K; ++qz) {
+      const int iz = tab_i[rz * 8 + qz];
+      const double z = tab_v[rz * 8 + qz];
+      for (int qy = 0; qy < K; ++qy) {
+        const int iy = tab_i[ry * 8 + qy];
+        const double y = tab_v[ry * 8 + qy];
+        const float* row = coef + (int64_t)ncx * (iy + (int64_t)ncy * iz);
+#pragma unroll
+        for (int qx = 0; qx < K; ++qx) phi += (double)row[xi[qx]] * xv[qx] * y * z;
+      }
+    }
+  } else if (ok == 3) {
+    for (int qz = 0; qz < k; ++qz) {
+      const int iz = tab_i[rz * 8 + qz];
+      const double z = tab_v[rz * 8 + qz];
+      for (int qy = 0; qy < k; ++qy) {
+        const int iy = tab_i[ry * 8 + qy];
+        const double y = tab_v[ry * 8 + qy];
+        const float* row = coef + (int64_t)ncx * (iy + (int64_t)ncy * iz);
+        for (int qx = 0; qx < k; ++qx) phi += (double)row[tab_i[rx * 8 + qx]] * tab_v[rx * 8 + qx] * y * z;
+      }
+    }
+  }
+  val[t] = ((ok & 1) ? phi : 0.0) - 0.5;
 }
 
 // the point on the lattice edge from node t along axis d, when the edge changes sign and
@@ -267,13 +323,6 @@ __global__ __launch_bounds__(1024) void k_sf_scan_apply(const int* __restrict__ 
   if (i0 < n) off[i0] = base;
   if (i0 + 1 < n) off[i0 + 1] = base + c0;
   if (i0 + 1 == n - 1 || i0 == n - 1) off[n] = base + c0 + c1;  // the last count's block writes the total
-}
-
-// points per lattice row (all i at fixed j, k): lets k_sf_grad skip empty rows
-__global__ void k_sf_rows(SfLattice G, const int* __restrict__ off, int* __restrict__ rows) {
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= (int64_t)G.NB * G.NB) return;
-  rows[r] = off[(r + 1) * G.NB] - off[r * G.NB];
 }
 
 // 1 when some coefficient differs from the first (partials summed)
@@ -435,18 +484,6 @@ __global__ __launch_bounds__(256) void k_sf_fold(const double* __restrict__ part
   }
 }
 
-// support [lo, hi] along one axis of coefficient i (mode 1: clamped indices extend the
-// first and last coefficient to the whole line)
-__device__ __forceinline__ void sf_support(const SfAxis& A, int i, double& lo, double& hi) {
-  if (A.mode == 1) {
-    lo = i == 0 ? -1e300 : (i - A.k / 2.0) * A.sh;
-    hi = i == A.nc - 1 ? 1e300 : (i + A.k / 2.0) * A.sh;
-  } else {
-    lo = A.U[i];
-    hi = A.U[i + A.k];
-  }
-}
-
 // per point and axis, the k coefficient indices and basis values / d / d2 (sf_axis), once:
 // bidx[(3 t + a) * 8 + q] (-1 past k or outside the spline's range),
 // bN[((3 t + a) * 3 + d) * 8 + q]
@@ -478,48 +515,95 @@ __device__ __forceinline__ void sf_basis_tab(const int* __restrict__ bidx, const
       for (int d = 0; d < 3; ++d) out[d] += nn[d * 8 + q];
 }
 
-// coefficient gradients: per coefficient, the points of the lattice nodes that can reach
-// its support, in lattice-node order; gss/gdr = factor * sum_points w . (basis derivatives)
-__global__ void k_sf_grad(SfAxes S, SfLattice G, int64_t ncoef, const int* __restrict__ off, const int* __restrict__ rows,
-                          const int* __restrict__ bidx, const double* __restrict__ bN, const double* __restrict__ wss,
-                          const double* __restrict__ wdr, double fss, double fdr, float* __restrict__ gss,
-                          float* __restrict__ gdr) {
-  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= ncoef) return;
-  const int ncx = S.a[0].nc, ncy = S.a[1].nc;
-  const int ci[3] = {(int)(j % ncx), (int)((j / ncx) % ncy), (int)(j / ((int64_t)ncx * ncy))};
-  int nlo[3], nhi[3];
-  for (int a = 0; a < 3; ++a) {
-    double lo, hi;
-    sf_support(S.a[a], ci[a], lo, hi);
-    // node i's points lie in [(i - 1/2) eh, (i + 1/2) eh] along the axis
-    const double l = fmax(lo / G.eh[a] - 0.5, -1.0), r = fmin(hi / G.eh[a] + 0.5, (double)G.NB);
-    nlo[a] = max(0, (int)floor(l) - 1);
-    nhi[a] = min(G.NB - 1, (int)ceil(r) + 1);
+// The coefficient gradients gather over points grouped by knot cell: a point's cell is the
+// first coefficient index of its span along each axis (clamped into the coefficient range,
+// sf_axis), and every coefficient the point touches lies in the k^3 block that starts at
+// that cell, so coefficient c gathers from the cells c - [0, k)^3.  key[t] = the cell
+// (-1 outside the spline's range), cnt[cell] += 1.
+__global__ void k_sf_cell_count(int64_t n, int ncx, int ncy, const int* __restrict__ bidx, int* __restrict__ key,
+                                int* __restrict__ cnt) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int bx = bidx[(3 * t + 0) * 8], by = bidx[(3 * t + 1) * 8], bz = bidx[(3 * t + 2) * 8];
+  int c = -1;
+  if (bx >= 0 && by >= 0 && bz >= 0) {
+    c = bx + ncx * (by + ncy * bz);
+    atomicAdd(&cnt[c], 1);
   }
-  double a_ss = 0, a_dr = 0;
-  for (int kk = nlo[2]; kk <= nhi[2]; ++kk)
-    for (int jj = nlo[1]; jj <= nhi[1]; ++jj) {
-      if (rows[jj + (int64_t)G.NB * kk] == 0) continue;  // no point in this lattice row
-      for (int ii = nlo[0]; ii <= nhi[0]; ++ii) {
-        const int64_t node = ii + (int64_t)G.NB * (jj + (int64_t)G.NB * kk);
-        for (int t = off[node]; t < off[node + 1]; ++t) {
-          double X[3], Y[3], Z[3];
-          sf_basis_tab(bidx, bN, t, 0, ci[0], X);
-          if (X[0] == 0 && X[1] == 0 && X[2] == 0) continue;
-          sf_basis_tab(bidx, bN, t, 1, ci[1], Y);
-          sf_basis_tab(bidx, bN, t, 2, ci[2], Z);
-          const double gx = X[1] * Y[0] * Z[0], gy = X[0] * Y[1] * Z[0], gz = X[0] * Y[0] * Z[1];
-          const double* ws = &wss[3 * (int64_t)t];
-          const double* wd = &wdr[6 * (int64_t)t];
-          a_ss += ws[0] * gx + ws[1] * gy + ws[2] * gz;
-          a_dr += wd[0] * gx + wd[1] * gy + wd[2] * gz + wd[3] * X[2] * Y[0] * Z[0] + wd[4] * X[0] * Y[2] * Z[0] +
-                  wd[5] * X[1] * Y[1] * Z[0];
-        }
-      }
+  key[t] = c;
+}
+
+// list[start[cell] ..) = the cell's points; cur (zeroed) hands out the slots in arrival
+// order, k_sf_cell_sort puts each cell's points back in ascending order
+__global__ void k_sf_cell_fill(int64_t n, const int* __restrict__ key, const int* __restrict__ start,
+                               int* __restrict__ cur, int* __restrict__ list) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n || key[t] < 0) return;
+  list[start[key[t]] + atomicAdd(&cur[key[t]], 1)] = (int)t;
+}
+
+__global__ void k_sf_cell_sort(int64_t ncell, const int* __restrict__ start, int* __restrict__ list) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  const int b = start[c], e = start[c + 1];
+  for (int i = b + 1; i < e; ++i) {  // insertion sort: a cell holds a few points
+    const int v = list[i];
+    int p = i - 1;
+    while (p >= b && list[p] > v) {
+      list[p + 1] = list[p];
+      --p;
     }
-  gss[j] = (float)(fss * a_ss);
-  gdr[j] = (float)(fdr * a_dr);
+    list[p + 1] = v;
+  }
+}
+
+// gss/gdr[c] = factor * sum over the points of the cells c - [0, k)^3 of w . (basis
+// derivatives of c).  One wavefront per coefficient: lane l takes the window cells l,
+// l + 64, ... (cell order, then point order), and lane 0 sums the 64 lanes in a fixed
+// butterfly order, so the result does not depend on scheduling.
+__global__ __launch_bounds__(256) void k_sf_grad(SfAxes S, int64_t ncoef, const int* __restrict__ start,
+                                                 const int* __restrict__ list, const int* __restrict__ bidx,
+                                                 const double* __restrict__ bN, const double* __restrict__ wss,
+                                                 const double* __restrict__ wdr, double fss, double fdr,
+                                                 float* __restrict__ gss, float* __restrict__ gdr) {
+  const int64_t j = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (j >= ncoef) return;  // whole wavefronts
+  const int k = S.a[0].k, ncx = S.a[0].nc, ncy = S.a[1].nc;
+  const int ci[3] = {(int)(j % ncx), (int)((j / ncx) % ncy), (int)(j / ((int64_t)ncx * ncy))};
+  double a_ss = 0, a_dr = 0;
+  int seen = 0;
+  for (int w = lane; w < k * k * k; w += 64) {
+    const int bx = ci[0] - w % k, by = ci[1] - (w / k) % k, bz = ci[2] - w / (k * k);
+    if (bx < 0 || by < 0 || bz < 0) continue;
+    const int64_t cell = bx + (int64_t)ncx * (by + (int64_t)ncy * bz);
+    const int e = start[cell + 1];
+    for (int p = start[cell]; p < e; ++p) {
+      const int64_t t = list[p];
+      ++seen;
+      double X[3], Y[3], Z[3];
+      sf_basis_tab(bidx, bN, t, 0, ci[0], X);
+      if (X[0] == 0 && X[1] == 0 && X[2] == 0) continue;
+      sf_basis_tab(bidx, bN, t, 1, ci[1], Y);
+      sf_basis_tab(bidx, bN, t, 2, ci[2], Z);
+      const double gx = X[1] * Y[0] * Z[0], gy = X[0] * Y[1] * Z[0], gz = X[0] * Y[0] * Z[1];
+      const double* ws = &wss[3 * t];
+      const double* wd = &wdr[6 * t];
+      a_ss += ws[0] * gx + ws[1] * gy + ws[2] * gz;
+      a_dr += wd[0] * gx + wd[1] * gy + wd[2] * gz + wd[3] * X[2] * Y[0] * Z[0] + wd[4] * X[0] * Y[2] * Z[0] +
+              wd[5] * X[1] * Y[1] * Z[0];
+    }
+  }
+  if (__ballot(seen > 0) != 0) {
+    for (int o = 1; o < 64; o <<= 1) {
+      a_ss += __shfl_xor(a_ss, o, 64);
+      a_dr += __shfl_xor(a_dr, o, 64);
+    }
+  }
+  if (lane == 0) {
+    gss[j] = (float)(fss * a_ss);
+    gdr[j] = (float)(fdr * a_dr);
+  }
 }
 
 }  // namespace tos

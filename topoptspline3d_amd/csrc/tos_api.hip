@@ -3498,6 +3498,11 @@ TOS_EXPORT int tos_spline_init(tos_ctx* c, int mode, int order, int param) {
     AxisTable T[3];
     for (int a = 0; a < 3; ++a) T[a] = spline_table(mode, order, param, dims[a], c->h);
     c->sp_mode = mode; c->sp_order = order; c->sp_param = param;
+    // the surface-constraint state belongs to the previous spline (knots, coefficient count)
+    for (double*& U : c->sf_knots) U = nullptr;
+    c->sscsens = c->dripcsens = nullptr;
+    c->sf_ncoef = 0;
+    c->sf_valid = false;
     for (int a = 0; a < 3; ++a) {
       c->nc[a] = T[a].nc;
       c->sp_idx[a] = dalloc<int>(*c, T[a].idx.size());
@@ -3713,7 +3718,9 @@ TOS_EXPORT int tos_surface_constraints(tos_ctx* c, const tos_surface_opts* o, to
       const int64_t nsb = (nl + kSfScanBlock - 1) / kSfScanBlock;
       C.sf_bsum = dalloc<int>(C, nsb);
       C.sf_boff = dalloc<int>(C, nsb + 1);
-      C.sf_rows = dalloc<int>(C, (int64_t)NB * NB);
+      C.sf_tab_i = dalloc<int>(C, 3 * 8 * (int64_t)NB);
+      C.sf_tab_v = dalloc<double>(C, 3 * 8 * (int64_t)NB);
+      C.sf_tab_ok = dalloc<uint8_t>(C, 3 * (int64_t)NB);
       C.sf_pts = dalloc<double>(C, 3 * cap);
       C.sf_grad = dalloc<double>(C, 3 * cap);
       C.sf_hess = dalloc<double>(C, 6 * cap);
@@ -3722,9 +3729,15 @@ TOS_EXPORT int tos_surface_constraints(tos_ctx* c, const tos_surface_opts* o, to
       C.sf_wdr = dalloc<double>(C, 6 * cap);
       C.sf_cells = o->cells;
     }
-    if (!C.sscsens) {
+    if (C.sf_ncoef != (int64_t)ncoef) {  // per-coefficient (and per-knot-cell) buffers
       C.sscsens = dalloc<float>(C, ncoef);
       C.dripcsens = dalloc<float>(C, ncoef);
+      C.sf_ccnt = dalloc<int>(C, ncoef);
+      C.sf_cstart = dalloc<int>(C, ncoef + 1);
+      const int64_t nsb = ((int64_t)ncoef + kSfScanBlock - 1) / kSfScanBlock;
+      C.sf_cbsum = dalloc<int>(C, nsb);
+      C.sf_cboff = dalloc<int>(C, nsb + 1);
+      C.sf_ncoef = (int64_t)ncoef;
     }
     {  // all coefficients equal: no surface; the driver's constants (utils/test_utils.cpp:1106-1113)
       const unsigned nbq = (unsigned)std::min<int64_t>(kRedBlocks, ((int64_t)ncoef + 255) / 256);
@@ -3753,7 +3766,16 @@ TOS_EXPORT int tos_surface_constraints(tos_ctx* c, const tos_surface_opts* o, to
     const LevelView& v = C.L[0].v;
     SfLattice G{NB, {v.ex * C.h / o->cells, v.ey * C.h / o->cells, v.ez * C.h / o->cells}};
     Prof pr(C, P_SPLINE);
-    hipLaunchKernelGGL(k_sf_lattice, dim3(nblk(nl)), dim3(256), 0, C.stream, S, G, (const float*)C.coef, C.sf_val);
+    hipLaunchKernelGGL(k_sf_axis_tab, dim3(nblk(3 * (int64_t)NB)), dim3(256), 0, C.stream, S, G, C.sf_tab_i,
+                       C.sf_tab_v, C.sf_tab_ok);
+    if (C.sp_order == 4)
+      hipLaunchKernelGGL(k_sf_lattice<4>, dim3(nblk(nl)), dim3(256), 0, C.stream, G, C.sp_order, C.nc[0], C.nc[1],
+                         (const int*)C.sf_tab_i, (const double*)C.sf_tab_v, (const uint8_t*)C.sf_tab_ok,
+                         (const float*)C.coef, C.sf_val);
+    else
+      hipLaunchKernelGGL(k_sf_lattice<0>, dim3(nblk(nl)), dim3(256), 0, C.stream, G, C.sp_order, C.nc[0], C.nc[1],
+                         (const int*)C.sf_tab_i, (const double*)C.sf_tab_v, (const uint8_t*)C.sf_tab_ok,
+                         (const float*)C.coef, C.sf_val);
     hipLaunchKernelGGL(k_sf_count, dim3(nblk(nl)), dim3(256), 0, C.stream, G, v, (const uint8_t*)C.eact,
                        (const double*)C.sf_val, C.h, C.sf_cnt);
     {
@@ -3763,8 +3785,6 @@ TOS_EXPORT int tos_surface_constraints(tos_ctx* c, const tos_surface_opts* o, to
                          C.sf_boff);
       hipLaunchKernelGGL(k_sf_scan_apply, dim3(nsb), dim3(1024), 0, C.stream, (const int*)C.sf_cnt, nl,
                          (const int*)C.sf_boff, C.sf_off);
-      hipLaunchKernelGGL(k_sf_rows, dim3(nblk((int64_t)NB * NB)), dim3(256), 0, C.stream, G, (const int*)C.sf_off,
-                         C.sf_rows);
     }
     LAUNCH_CHECK();
     int total = 0;
@@ -3814,13 +3834,36 @@ TOS_EXPORT int tos_surface_constraints(tos_ctx* c, const tos_surface_opts* o, to
       C.sf_bcap = std::max<int64_t>(n, 1024) * 2;
       C.sf_bidx = dalloc<int>(C, 24 * C.sf_bcap);
       C.sf_bN = dalloc<double>(C, 72 * C.sf_bcap);
+      C.sf_key = dalloc<int>(C, C.sf_bcap);
+      C.sf_list = dalloc<int>(C, C.sf_bcap);
     }
-    if (n)
+    // points grouped by knot cell (count, scan, fill, per-cell sort), then the gather
+    const int64_t nc64 = (int64_t)ncoef;
+    hipLaunchKernelGGL(k_zero<int>, dim3(nblk(nc64)), dim3(256), 0, C.stream, C.sf_ccnt, nc64);
+    if (n) {
       hipLaunchKernelGGL(k_sf_basis, dim3(nblk(n)), dim3(256), 0, C.stream, S, n, (const double*)C.sf_pts,
                          C.sf_bidx, C.sf_bN);
-    hipLaunchKernelGGL(k_sf_grad, dim3(nblk((int64_t)ncoef)), dim3(256), 0, C.stream, S, G, (int64_t)ncoef,
-                       (const int*)C.sf_off, (const int*)C.sf_rows, (const int*)C.sf_bidx, (const double*)C.sf_bN, (const double*)C.sf_wss,
-                       (const double*)C.sf_wdr, ss_fac, dr_fac, C.sscsens, C.dripcsens);
+      hipLaunchKernelGGL(k_sf_cell_count, dim3(nblk(n)), dim3(256), 0, C.stream, n, C.nc[0], C.nc[1],
+                         (const int*)C.sf_bidx, C.sf_key, C.sf_ccnt);
+    }
+    {
+      const unsigned nsb = (unsigned)((nc64 + kSfScanBlock - 1) / kSfScanBlock);
+      hipLaunchKernelGGL(k_sf_scan_blocks, dim3(nsb), dim3(1024), 0, C.stream, (const int*)C.sf_ccnt, nc64,
+                         C.sf_cbsum);
+      hipLaunchKernelGGL(k_sf_scan, dim3(1), dim3(1024), 0, C.stream, (const int*)C.sf_cbsum, (int64_t)nsb,
+                         C.sf_cboff);
+      hipLaunchKernelGGL(k_sf_scan_apply, dim3(nsb), dim3(1024), 0, C.stream, (const int*)C.sf_ccnt, nc64,
+                         (const int*)C.sf_cboff, C.sf_cstart);
+    }
+    hipLaunchKernelGGL(k_zero<int>, dim3(nblk(nc64)), dim3(256), 0, C.stream, C.sf_ccnt, nc64);
+    if (n)
+      hipLaunchKernelGGL(k_sf_cell_fill, dim3(nblk(n)), dim3(256), 0, C.stream, n, (const int*)C.sf_key,
+                         (const int*)C.sf_cstart, C.sf_ccnt, C.sf_list);
+    hipLaunchKernelGGL(k_sf_cell_sort, dim3(nblk(nc64)), dim3(256), 0, C.stream, nc64, (const int*)C.sf_cstart,
+                       C.sf_list);
+    hipLaunchKernelGGL(k_sf_grad, dim3(nblk(64 * nc64)), dim3(256), 0, C.stream, S, nc64, (const int*)C.sf_cstart,
+                       (const int*)C.sf_list, (const int*)C.sf_bidx, (const double*)C.sf_bN,
+                       (const double*)C.sf_wss, (const double*)C.sf_wdr, ss_fac, dr_fac, C.sscsens, C.dripcsens);
     LAUNCH_CHECK();
     HIPCHK(hipStreamSynchronize(C.stream));
     C.ss_value = ss_val;

@@ -221,10 +221,18 @@ struct Ctx {
   int64_t sf_n = 0;
   double* sf_val = nullptr;
   int *sf_cnt = nullptr, *sf_off = nullptr;
-  int *sf_bsum = nullptr, *sf_boff = nullptr, *sf_rows = nullptr;  // scan block sums / offsets, points per row
-  int* sf_bidx = nullptr;       // per point and axis: coefficient indices and basis values (grown on demand)
+  int *sf_bsum = nullptr, *sf_boff = nullptr;  // scan block sums / offsets
+  int* sf_tab_i = nullptr;   // per axis and lattice index: coefficient indices, basis values, flags
+  double* sf_tab_v = nullptr;
+  uint8_t* sf_tab_ok = nullptr;
+  int* sf_bidx = nullptr;    // per point and axis: coefficient indices and basis values (grown on demand)
   double* sf_bN = nullptr;
+  int* sf_key = nullptr;     // per point: knot cell (grown with sf_bidx)
+  int* sf_list = nullptr;    // points grouped by knot cell
   int64_t sf_bcap = 0;
+  int64_t sf_ncoef = 0;      // coefficient count the per-coefficient buffers below are sized for
+  int *sf_ccnt = nullptr, *sf_cstart = nullptr;  // points per knot cell (then fill cursor), cell offsets
+  int *sf_cbsum = nullptr, *sf_cboff = nullptr;  // scan block sums / offsets over the cells
   double *sf_pts = nullptr, *sf_grad = nullptr, *sf_hess = nullptr, *sf_nrm = nullptr;
   double *sf_wss = nullptr, *sf_wdr = nullptr;
   double* sf_knots[3] = {nullptr, nullptr, nullptr};  // clamped knots (spline mode 0)
