@@ -92,12 +92,13 @@ def make_workload(name):
 class Loop:
     """One optimisation iteration over a context (api = HIP or oracle binding)."""
 
-    def __init__(self, api, wl, precision):
+    def __init__(self, api, wl, precision, recycle=0):
         from topoptspline3d_amd.context import MMA, Context
         self.api, self.wl, self.precision = api, wl, precision
         ctx = Context(api, wl["solid"], wl["support"], shell=wl["shell"])
         ctx.set_material(1e6, 0.4, 3.0, 1e-3)
         ctx.set_loads(wl["f"])
+        ctx.set_start_recycling(recycle)  # 0: plain warm start
         self.ctx = ctx
         if wl["spline"]:
             mode, order, spacing = wl["spline"]
@@ -428,6 +429,11 @@ def main():
     ap.add_argument("--precision", type=int, default=3,
                     help="0 FP64 V-cycle; 1 FP32 V-cycle; 2 FP32 V-cycle on FP16 coarse stencils; "
                          "3 as 2 with the FCG residual and K p stored in FP32")
+    ap.add_argument("--recycle", type=int, default=4,
+                    help="start of each solve projected onto the warm start and the earlier solutions "
+                         "(basis size; 0: plain warm start)")
+    ap.add_argument("--no-recycle-compare", action="store_true",
+                    help="skip the run from the plain warm start")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-compare", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
@@ -483,9 +489,9 @@ def main():
     log(f"[bench] workload {wl['name']} generated in {time.perf_counter() - t0:.1f}s {wl['meta'].get('fill', '')}")
     api = hip_api()
 
-    def run(tag, precision, use_net, steps, keep=False):
+    def run(tag, precision, use_net, steps, keep=False, recycle=0):
         """warmup + steps iterations from the initial design; returns (loop, ms/iter, its, max res)"""
-        loop = Loop(api, wl, precision)
+        loop = Loop(api, wl, precision, recycle)
         loop.ctx.set_sweep_mode(args.sweep_mode)
         loop.ctx.set_graph_replay(not args.no_graph_replay)
         if use_net:
@@ -509,7 +515,7 @@ def main():
         return out
 
     if args.probe_element_mfma:
-        loop, *_ = run("plain", args.precision, False, 0, keep=True)
+        loop, *_ = run("plain", args.precision, False, 0, keep=True, recycle=args.recycle)
         ne = loop.ctx.info()["n_active_elements"][0]
         pr = loop.ctx.probe_element_mfma(reps=10)
         # useful work: 576 FMAs per element (the MFMA tiles execute 32/24 of it and more
@@ -521,12 +527,12 @@ def main():
         loop.ctx.close()
         return
     if args.surface_only:
-        loop, *_ = run("plain", args.precision, False, 0, keep=True)
+        loop, *_ = run("plain", args.precision, False, 0, keep=True, recycle=args.recycle)
         print(json.dumps({"surface_constraints": surface_phase(loop.ctx, cells_list=args.surface_cells)}), flush=True)
         loop.ctx.close()
         return
-    # headline: plain mixed-precision MG-PCG
-    loop, ms, its, res_max = run("plain", args.precision, False, args.steps, keep=True)
+    # headline: mixed-precision MG-PCG with the recycled start
+    loop, ms, its, res_max = run("plain", args.precision, False, args.steps, keep=True, recycle=args.recycle)
     info = loop.ctx.info()
     nv = active_nodes(wl["solid"])
     log(f"[bench] levels {info['n_levels']} dims {info['level_dims']} coarse dofs {info['coarse_dofs']}")
@@ -547,18 +553,29 @@ def main():
         "config": {"workload": wl["desc"], "grid": list(wl["solid"].shape[::-1]),
                    "active_elements": info["n_active_elements"][0], "active_nodes": nv,
                    "mg_levels": info["n_levels"], "rtol": wl["rtol"], "precision": args.precision,
-                   "solver": "FCG, " + SOLVERS[args.precision] + ", FP64 refinement (no learned correction)",
+                   "solver": "FCG, " + SOLVERS[args.precision] + ", FP64 refinement (no learned correction)"
+                             + (f", start projected onto the warm start and {args.recycle - 1} earlier solutions"
+                                if args.recycle > 1 else ""),
+                   "recycle": args.recycle,
                    "iterations_timed": f"{args.warmup}..{args.warmup + args.steps - 1} from the initial design",
                    "pcg_iterations": its, "max_rel_res": res_max},
         "roofline": roofline(stats, info, nv, args.profile_steps,
                              "ellipsoids_511x320x320" if wl["name"] == "c2" else None, args.precision,
                              args.sweep_mode),
     }
+    ms_plain, its_plain = ms, its
+    if args.recycle > 0 and not args.no_recycle_compare:
+        # the same mixed-precision solver from the plain warm start (attribution)
+        _, ms_plain, its_plain, res_plain = run("no-recycle", args.precision, False, args.steps)
+        line["without_recycling"] = {"ms_per_iter": round(ms_plain, 3), "pcg_iterations": its_plain,
+                                     "max_rel_res": res_plain, "speedup_of_recycling": round(ms_plain / ms, 3)}
     if not args.no_fp64_compare:
-        # plain FP64 GPU multigrid over the same iterations from the same initial design
+        # plain FP64 GPU multigrid (plain warm start) over the same iterations from the same
+        # initial design
         _, ms64, its64, res64 = run("fp64", 0, False, args.steps)
         line["plain_fp64"] = {"ms_per_iter": round(ms64, 3), "pcg_iterations": its64, "max_rel_res": res64,
-                              "speedup_of_value": round(ms64 / ms, 3)}
+                              "speedup_of_value": round(ms64 / ms, 3),
+                              "speedup_of_precision_alone": round(ms64 / ms_plain, 3)}
     if net is not None:
         # the learned correction runs in the FP64-residual FCG (precision 3 takes none)
         lloop, msl, itsl, resl = run("learned", min(args.precision, 2), True, args.steps, keep=True)
@@ -567,13 +584,14 @@ def main():
         lloop.ctx.close()
         line["learned"] = dict(learn_info, corrections_tried=tried, corrections_kept=kept,
                                mean_step_kept=round(stepk, 4), ms_per_iter=round(msl, 3), pcg_iterations=itsl,
-                               max_rel_res=resl, speedup_over_plain=round(ms / msl, 3),
-                               iterations_saved=int(sum(its) - sum(itsl)))
+                               max_rel_res=resl, speedup_over_plain=round(ms_plain / msl, 3),
+                               iterations_saved=int(sum(its_plain) - sum(itsl)))
     if not args.no_fp64_compare:
         # same operator, solve from zero to the same tolerance: FP64, mixed, learned
         # (an OC step changes the densities themselves: rebuild the operator first)
         if not wl["spline"]:
             loop.ctx.update_operator()
+        loop.ctx.set_start_recycling(0)  # from zero means from zero: no projected start
         cmp = {}
         modes = [("fp64", 0, False), ("mixed_fp32_stencils", 1, False), ("mixed_fp16_stencils", 2, False),
                  ("mixed_fp32_residual", 3, False)]

@@ -416,6 +416,20 @@ int tos_set_galerkin_precision(tos_ctx* ctx, int prec);
  * launches.  Kernel timing (tos_profile) always launches kernel by kernel. */
 int tos_set_graph_replay(tos_ctx* ctx, int enable);
 
+/* Recycled start for sequences of solves (no counterpart in the reference, whose solves
+ * start from the previous displacement).  With depth m (1..8; 0 = off, the default) every
+ * tos_solve with warm_start = 1 first replaces the current displacement u of each case by
+ * the Galerkin projection of the solution onto span{u, s_2, ..., s_m}, where s_i is the
+ * i-th newest solution a converged tos_solve of this case left (the newest, s_1, is u
+ * itself unless the displacement was set since): u0 = V c with (V^T K V) c = V^T f, the
+ * point of the span with the smallest energy-norm error, so never worse than u in that
+ * norm.  Costs m FP64 operator applications, one pass over 2m + 1 vectors and one over
+ * m + 1, and m stored solutions per case (FP64, the fine level's padded size).  The Gram
+ * system is scaled to unit diagonal and solved through its eigenvalues; those below 1e-10
+ * of the largest are dropped (nearly dependent solutions).  A changed depth starts an
+ * empty history.  tos_solve_batched ignores it. */
+int tos_set_start_recycling(tos_ctx* ctx, int depth);
+
 /* ---- measurement ---------------------------------------------------------------------
  * Kernel timing with HIP events recorded on the context's stream around every launch of
  * the solver kernels.  enable = 1 clears and starts recording; enable = 0 stops, waits for

@@ -1631,6 +1631,127 @@ ORC_API int orc_vcycle(tos_ctx* c, const double* r, double* z, int precision, in
   return TOS_OK;
 }
 
+// ------------------------------------------------------------------------------------
+// Recycled start (no reference counterpart): with depth m the warm start u of case k is
+// replaced by the Galerkin projection of the solution onto span{u, s_2, ..., s_m}, s_i
+// the i-th newest stored solution of the case: u0 = V c, (V^T K V) c = V^T f.  The Gram
+// system is scaled to unit diagonal and solved through its eigenvalues (cyclic Jacobi,
+// eigenvalues below 1e-10 of the largest dropped).
+// ------------------------------------------------------------------------------------
+static bool rc_coefficients(int m, const double* A, const double* b, double* cf) {
+  double d[8], S[8][8], Q[8][8], bs[8];
+  for (int i = 0; i < m; ++i) d[i] = A[i * m + i] > 0 ? 1.0 / std::sqrt(A[i * m + i]) : 0.0;
+  for (int i = 0; i < m; ++i) {
+    bs[i] = b[i] * d[i];
+    for (int j = 0; j < m; ++j) {
+      S[i][j] = A[i * m + j] * d[i] * d[j];
+      Q[i][j] = i == j;
+    }
+  }
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    double off = 0, dia = 0;
+    for (int i = 0; i < m; ++i)
+      for (int j = 0; j < m; ++j) (i == j ? dia : off) += S[i][j] * S[i][j];
+    if (off <= 1e-30 * dia) break;
+    for (int p = 0; p < m; ++p)
+      for (int q = p + 1; q < m; ++q) {
+        if (S[p][q] == 0) continue;
+        // rotation in the (p, q) plane that zeroes S[p][q]: t = tan of its angle, the
+        // smaller root of t^2 + 2 th t - 1 = 0
+        const double th = (S[q][q] - S[p][p]) / (2 * S[p][q]);
+        const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1));
+        const double cs = 1 / std::sqrt(t * t + 1), sn = t * cs;
+        for (int k = 0; k < m; ++k) {
+          const double a = S[k][p], e = S[k][q];
+          S[k][p] = cs * a - sn * e;
+          S[k][q] = sn * a + cs * e;
+        }
+        for (int k = 0; k < m; ++k) {
+          const double a = S[p][k], e = S[q][k];
+          S[p][k] = cs * a - sn * e;
+          S[q][k] = sn * a + cs * e;
+        }
+        for (int k = 0; k < m; ++k) {
+          const double a = Q[k][p], e = Q[k][q];
+          Q[k][p] = cs * a - sn * e;
+          Q[k][q] = sn * a + cs * e;
+        }
+      }
+  }
+  double lmax = 0;
+  for (int k = 0; k < m; ++k) lmax = std::max(lmax, S[k][k]);
+  if (!(lmax > 0)) return false;
+  double ch[8] = {0};
+  for (int k = 0; k < m; ++k) {
+    if (!(S[k][k] > 1e-10 * lmax)) continue;
+    double w = 0;
+    for (int i = 0; i < m; ++i) w += Q[i][k] * bs[i];
+    w /= S[k][k];
+    for (int i = 0; i < m; ++i) ch[i] += w * Q[i][k];
+  }
+  for (int i = 0; i < m; ++i) {
+    cf[i] = ch[i] * d[i];
+    if (!std::isfinite(cf[i])) return false;
+  }
+  return true;
+}
+
+static void rc_reserve(Ctx& c) {
+  if ((int)c.rc_ring.size() == c.K * c.rc_depth) return;
+  c.rc_ring.assign((size_t)c.K * c.rc_depth, std::vector<double>(3 * c.L[0].nn, 0.0));
+  c.rc_head.assign(c.K, 0);
+  c.rc_cnt.assign(c.K, 0);
+}
+
+static const std::vector<double>& rc_slot(const Ctx& c, int k, int i) {  // i = 1: newest
+  return c.rc_ring[(size_t)k * c.rc_depth + (c.rc_head[k] - (i - 1) + c.rc_depth) % c.rc_depth];
+}
+
+static void rc_start(Ctx& c, int k) {
+  rc_reserve(c);
+  const Level& L = c.L[0];
+  const int64_t n3 = 3 * L.nn;
+  double* u = c.U.data() + k * n3;
+  const double* f = c.F.data() + k * n3;
+  mask_free(L, u);
+  std::vector<const double*> V{u};
+  for (int i = 2; i <= c.rc_cnt[k] && (int)V.size() < c.rc_depth; ++i) V.push_back(rc_slot(c, k, i).data());
+  const int m = (int)V.size();
+  std::vector<std::vector<double>> KV(m, std::vector<double>(n3));
+  for (int a = 0; a < m; ++a) apply_fine(c, V[a], KV[a].data());
+  double A[64], b[8], cf[8];
+  for (int a = 0; a < m; ++a) {
+    for (int e = a; e < m; ++e) A[a * m + e] = A[e * m + a] = dot3(L, V[a], KV[e].data());
+    b[a] = dot3(L, V[a], f);
+  }
+  if (!rc_coefficients(m, A, b, cf)) return;
+  std::vector<double> u0(n3, 0.0);
+  for (int64_t i = 0; i < n3; ++i) {
+    double s = 0;
+    for (int a = 0; a < m; ++a) s = std::fma(cf[a], V[a][i], s);
+    u0[i] = s;
+  }
+  std::copy(u0.begin(), u0.end(), u);
+}
+
+static void rc_push(Ctx& c, int k) {
+  rc_reserve(c);
+  const int64_t n3 = 3 * c.L[0].nn;
+  c.rc_head[k] = (c.rc_head[k] + 1) % c.rc_depth;
+  c.rc_cnt[k] = std::min(c.rc_cnt[k] + 1, c.rc_depth);
+  std::vector<double>& s = c.rc_ring[(size_t)k * c.rc_depth + c.rc_head[k]];
+  std::copy(c.U.begin() + k * n3, c.U.begin() + (k + 1) * n3, s.begin());
+}
+
+ORC_API int orc_set_start_recycling(tos_ctx* c, int depth) {
+  if (!c || depth < 0 || depth > 8) return TOS_E_ARG;
+  if (depth != c->rc_depth) {
+    c->rc_depth = depth;
+    c->rc_ring.clear();
+  }
+  return TOS_OK;
+}
+
 ORC_API int orc_solve(tos_ctx* c, const tos_solve_opts* o, tos_solve_stats* st) {
   if (!c || !o) return TOS_E_ARG;
   if (!c->op_ready) { c->err = "update_operator first"; return TOS_E_STATE; }
@@ -1638,10 +1759,12 @@ ORC_API int orc_solve(tos_ctx* c, const tos_solve_opts* o, tos_solve_stats* st) 
   int rc_all = TOS_OK;
   for (int k = 0; k < c->K; ++k) {
     int its; double rr;
+    if (c->rc_depth > 0 && o->warm_start) rc_start(*c, k);
     int rc = solve_case(*c, k, *o, &its, &rr);
     if (st && k < 64) { st->iterations[k] = its; st->rel_res[k] = rr; }
     if (rc == TOS_E_NAN) return rc;
     if (rc) rc_all = rc;
+    if (c->rc_depth > 0 && rc == TOS_OK) rc_push(*c, k);
   }
   if (st) st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return rc_all;
@@ -2600,7 +2723,12 @@ ORC_API int orc_profile(tos_ctx* c, int enable, tos_kernel_stats* out) {
 
 // The batched solve is the same mathematics per case: the oracle solves the cases in turn.
 ORC_API int orc_solve_batched(tos_ctx* c, const tos_solve_opts* o, tos_solve_stats* st) {
-  return orc_solve(c, o, st);
+  if (!c) return TOS_E_ARG;
+  const int depth = c->rc_depth;  // the batched solve starts without the recycled projection
+  c->rc_depth = 0;
+  const int rc = orc_solve(c, o, st);
+  c->rc_depth = depth;
+  return rc;
 }
 
 ORC_API int orc_batch_tuning(tos_ctx* c, char* buf, int len) {

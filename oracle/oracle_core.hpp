@@ -62,6 +62,10 @@ struct Ctx {
   std::vector<double> w;    // rho^p on active elements, 0 elsewhere
   int K = 0;
   std::vector<double> F, U; // K * 3 * nn0
+  // recycled start: per case the last rc_depth solutions, newest at rc_head
+  int rc_depth = 0;
+  std::vector<std::vector<double>> rc_ring;  // [case * rc_depth + slot]
+  std::vector<int> rc_head, rc_cnt;
   int64_t n_active = 0;
   bool op_ready = false;
   bool floating = false;    // no supported fine node: the rigid motions are K's null space

@@ -127,6 +127,7 @@ SIGNATURES = {
     "set_sweep_mode": (C.c_int, [vp, C.c_int]),
     "set_galerkin_precision": (C.c_int, [vp, C.c_int]),
     "set_graph_replay": (C.c_int, [vp, C.c_int]),
+    "set_start_recycling": (C.c_int, [vp, C.c_int]),
     "batch_tuning": (C.c_int, [vp, C.c_char_p, C.c_int]),
 }
 

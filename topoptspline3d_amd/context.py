@@ -180,6 +180,10 @@ class Context:
         """replay the captured FCG iteration body (default) or launch kernel by kernel"""
         self._chk(self.api.set_graph_replay(self.h, int(bool(enable))), "set_graph_replay")
 
+    def set_start_recycling(self, depth):
+        """warm starts projected onto span{u, earlier solutions} (tos_set_start_recycling)"""
+        self._chk(self.api.set_start_recycling(self.h, int(depth)), "set_start_recycling")
+
     def batch_tuning(self):
         buf = C.create_string_buffer(512)
         self._chk(self.api.batch_tuning(self.h, buf, 512), "batch_tuning")

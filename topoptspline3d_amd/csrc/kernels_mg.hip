@@ -1785,6 +1785,77 @@ __global__ __launch_bounds__(256) void k_pcg_p(const int4* __restrict__ cl, int 
   for (int c = 0; c < 3; ++c) p[c * NP + n.p] = (TP)(zv[c] + be * pv[c]);
 }
 
+// ---- recycled start: Galerkin projection of the warm start onto earlier solutions ----
+constexpr int kRcMax = 8;  // basis vectors: the current displacement and up to 7 earlier solutions
+struct RcVecs {
+  const double* v[kRcMax];   // basis
+  const double* kv[kRcMax];  // K times the basis
+};
+struct RcCoef {
+  double c[kRcMax];
+};
+
+// partial sums over the free nodes (grid-stride over the chunk list, gridDim.x partials per
+// column): columns 0 .. M(M+1)/2-1 = v_a . K v_b for a <= b (row by row), then v_a . f
+template <int M>
+__global__ __launch_bounds__(256) void k_rc_gram(const int4* __restrict__ cl, int nch, int64_t NP, RcVecs V,
+                                                 const double* __restrict__ f, double* __restrict__ partial) {
+  constexpr int NG = M * (M + 1) / 2 + M;
+  double g[NG];
+#pragma unroll
+  for (int i = 0; i < NG; ++i) g[i] = 0;
+  const int lane = threadIdx.x & 63;
+  for (int ch = blockIdx.x * 4 + (threadIdx.x >> 6); ch < nch; ch += gridDim.x * 4) {
+    NodeRef n;
+    if (!chunk_node_at(cl, nch, ch, lane, 1, n)) continue;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int64_t i = c * NP + n.p;
+      double v[M], kv[M];
+#pragma unroll
+      for (int a = 0; a < M; ++a) {
+        v[a] = V.v[a][i];
+        kv[a] = V.kv[a][i];
+      }
+      const double fv = f[i];
+      int q = 0;
+#pragma unroll
+      for (int a = 0; a < M; ++a)
+#pragma unroll
+        for (int b = a; b < M; ++b) g[q++] += v[a] * kv[b];
+#pragma unroll
+      for (int a = 0; a < M; ++a) g[q++] += v[a] * fv;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NG; ++q) block_partial(g[q], partial, q);
+}
+
+// u = sum_a c_a v_a on the free nodes (v_0 may be u itself)
+template <int M>
+__global__ __launch_bounds__(256) void k_rc_combine(const int4* __restrict__ cl, int nch, int64_t NP, RcVecs V,
+                                                    RcCoef C, double* __restrict__ u) {
+  NodeRef n;
+  if (!chunk_node(cl, nch, 1, n)) return;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const int64_t i = c * NP + n.p;
+    double s = 0;
+#pragma unroll
+    for (int a = 0; a < M; ++a) s = fma(C.c[a], V.v[a][i], s);
+    u[i] = s;
+  }
+}
+
+// dst = src on the free nodes
+__global__ __launch_bounds__(256) void k_copy_nodes(const int4* __restrict__ cl, int nch, int64_t NP,
+                                                    const double* __restrict__ src, double* __restrict__ dst) {
+  NodeRef n;
+  if (!chunk_node(cl, nch, 1, n)) return;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) dst[c * NP + n.p] = src[c * NP + n.p];
+}
+
 // generic partials of a.b (and a2.b2 when a2 != null)
 __global__ __launch_bounds__(256) void k_dot_nodes(const int4* __restrict__ cl, int nch, int64_t NP,
                                                    const double* __restrict__ a, const double* __restrict__ b,

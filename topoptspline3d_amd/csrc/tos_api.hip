@@ -3051,6 +3051,190 @@ TOS_EXPORT int tos_vcycle(tos_ctx* c, const double* r, double* z, int precision,
   });
 }
 
+// ---- recycled start -----------------------------------------------------------------
+// A sequence of solves whose operator changes a little per step (an optimisation loop)
+// leaves earlier solutions that span most of the next one.  With depth m >= 1 the warm
+// start u is replaced by the K-orthogonal (Galerkin) projection of the solution onto
+// span{u, s_2, ..., s_m}, s_i the i-th newest stored solution of the case (the newest,
+// s_1, is the warm start itself unless the displacement was set since): u0 = V c with
+// (V^T K V) c = V^T f, i.e. the point of that span with the smallest energy error, never
+// worse than u in the energy norm.  The Gram system is scaled to unit diagonal and solved
+// through its eigenvalues, dropping those below 1e-10 of the largest (nearly dependent
+// solutions).  No reference counterpart.
+namespace {
+bool rc_coefficients(int m, const double* A, const double* b, double* cf) {
+  double d[kRcMax], S[kRcMax][kRcMax], Q[kRcMax][kRcMax], bs[kRcMax];
+  for (int i = 0; i < m; ++i) d[i] = A[i * m + i] > 0 ? 1.0 / std::sqrt(A[i * m + i]) : 0.0;
+  for (int i = 0; i < m; ++i) {
+    bs[i] = b[i] * d[i];
+    for (int j = 0; j < m; ++j) {
+      S[i][j] = A[i * m + j] * d[i] * d[j];
+      Q[i][j] = i == j ? 1.0 : 0.0;
+    }
+  }
+  for (int sweep = 0; sweep < 60; ++sweep) {  // cyclic Jacobi
+    double off = 0, dia = 0;
+    for (int i = 0; i < m; ++i)
+      for (int j = 0; j < m; ++j) (i == j ? dia : off) += S[i][j] * S[i][j];
+    if (off <= 1e-30 * dia) break;
+    for (int p = 0; p < m; ++p)
+      for (int q = p + 1; q < m; ++q) {
+        if (S[p][q] == 0) continue;
+        const double th = (S[q][q] - S[p][p]) / (2 * S[p][q]);
+        const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1));
+        const double cs = 1 / std::sqrt(t * t + 1), sn = t * cs;
+        for (int k = 0; k < m; ++k) {
+          const double a = S[k][p], e = S[k][q];
+          S[k][p] = cs * a - sn * e;
+          S[k][q] = sn * a + cs * e;
+        }
+        for (int k = 0; k < m; ++k) {
+          const double a = S[p][k], e = S[q][k];
+          S[p][k] = cs * a - sn * e;
+          S[q][k] = sn * a + cs * e;
+        }
+        for (int k = 0; k < m; ++k) {
+          const double a = Q[k][p], e = Q[k][q];
+          Q[k][p] = cs * a - sn * e;
+          Q[k][q] = sn * a + cs * e;
+        }
+      }
+  }
+  double lmax = 0;
+  for (int k = 0; k < m; ++k) lmax = std::max(lmax, S[k][k]);
+  if (!(lmax > 0)) return false;
+  double ch[kRcMax] = {0};
+  for (int k = 0; k < m; ++k) {
+    if (!(S[k][k] > 1e-10 * lmax)) continue;
+    double w = 0;
+    for (int i = 0; i < m; ++i) w += Q[i][k] * bs[i];
+    w /= S[k][k];
+    for (int i = 0; i < m; ++i) ch[i] += w * Q[i][k];
+  }
+  for (int i = 0; i < m; ++i) {
+    cf[i] = ch[i] * d[i];
+    if (!std::isfinite(cf[i])) return false;
+  }
+  return true;
+}
+
+// ring storage for c.K cases (allocated on the first solve after the depth is set or the
+// case count changed)
+void rc_reserve(Ctx& c) {
+  if (c.rc_ring && c.rc_K == c.K) return;
+  const int64_t n3 = 3 * c.L[0].v.NP;
+  c.rc_ring = dalloc<double>(c, (size_t)c.K * c.rc_depth * n3);
+  c.rc_kv = dalloc<double>(c, (size_t)c.rc_depth * n3);
+  constexpr int ncol = kRcMax * (kRcMax + 1) / 2 + kRcMax;
+  c.rc_part = dalloc<double>(c, (size_t)ncol * kRedBlocks);
+  c.rc_out = dalloc<double>(c, ncol);
+  c.rc_K = c.K;
+  c.rc_head.assign(c.K, 0);
+  c.rc_cnt.assign(c.K, 0);
+}
+
+double* rc_slot(Ctx& c, int k, int i) {  // i-th newest stored solution of case k (i = 1 newest)
+  const int64_t n3 = 3 * c.L[0].v.NP;
+  const int s = (c.rc_head[k] - (i - 1) + c.rc_depth) % c.rc_depth;
+  return c.rc_ring + ((size_t)k * c.rc_depth + s) * n3;
+}
+
+template <int M>
+void rc_launch(Ctx& c, const RcVecs& V, const double* f, unsigned nbg) {
+  Level& L0 = c.L[0];
+  hipLaunchKernelGGL((k_rc_gram<M>), dim3(nbg), dim3(256), 0, c.stream, (const int4*)L0.cl_lin, L0.n_lin, L0.v.NP,
+                     V, f, c.rc_part);
+}
+
+template <int M>
+void rc_combine(Ctx& c, const RcVecs& V, const RcCoef& C, double* u) {
+  Level& L0 = c.L[0];
+  hipLaunchKernelGGL((k_rc_combine<M>), dim3(cblk(L0.n_lin)), dim3(256), 0, c.stream, (const int4*)L0.cl_lin,
+                     L0.n_lin, L0.v.NP, V, C, u);
+}
+
+// u_k <- the projected start (before the solve of case k)
+void rc_start(Ctx& c, int k) {
+  rc_reserve(c);
+  Level& L0 = c.L[0];
+  const int64_t n3 = 3 * L0.v.NP;
+  double* u = c.U + (size_t)k * n3;
+  const double* f = c.F + (size_t)k * n3;
+  RcVecs V{};
+  int m = 0;
+  V.v[m++] = u;
+  for (int i = 2; i <= c.rc_cnt[k] && m < c.rc_depth; ++i) V.v[m++] = rc_slot(c, k, i);
+  for (int a = 0; a < m; ++a) {
+    V.kv[a] = c.rc_kv + (size_t)a * n3;
+    apply_level<double, 0>(c, 0, V.v[a], (const double*)nullptr, const_cast<double*>(V.kv[a]));
+  }
+  const int ncol = m * (m + 1) / 2 + m;
+  {
+    Prof pr(c, P_VECTOR);
+    const unsigned nbg = (unsigned)std::min<int64_t>(kRedBlocks, std::max<int64_t>(1, (L0.n_lin + 3) / 4));
+    switch (m) {
+      case 1: rc_launch<1>(c, V, f, nbg); break;
+      case 2: rc_launch<2>(c, V, f, nbg); break;
+      case 3: rc_launch<3>(c, V, f, nbg); break;
+      case 4: rc_launch<4>(c, V, f, nbg); break;
+      case 5: rc_launch<5>(c, V, f, nbg); break;
+      case 6: rc_launch<6>(c, V, f, nbg); break;
+      case 7: rc_launch<7>(c, V, f, nbg); break;
+      default: rc_launch<8>(c, V, f, nbg); break;
+    }
+    hipLaunchKernelGGL(k_sf_fold, dim3(1), dim3(256), 0, c.stream, (const double*)c.rc_part, (int)nbg, ncol, 0,
+                       c.rc_out);
+    LAUNCH_CHECK();
+  }
+  double g[kRcMax * (kRcMax + 1) / 2 + kRcMax];
+  download(c.stream, g, c.rc_out, ncol * sizeof(double));
+  double A[kRcMax * kRcMax], b[kRcMax];
+  int q = 0;
+  for (int a = 0; a < m; ++a)
+    for (int e = a; e < m; ++e) A[a * m + e] = A[e * m + a] = g[q++];
+  for (int a = 0; a < m; ++a) b[a] = g[q++];
+  RcCoef C{};
+  if (!rc_coefficients(m, A, b, C.c)) return;  // keep the warm start
+  Prof pr(c, P_VECTOR);
+  switch (m) {
+    case 1: rc_combine<1>(c, V, C, u); break;
+    case 2: rc_combine<2>(c, V, C, u); break;
+    case 3: rc_combine<3>(c, V, C, u); break;
+    case 4: rc_combine<4>(c, V, C, u); break;
+    case 5: rc_combine<5>(c, V, C, u); break;
+    case 6: rc_combine<6>(c, V, C, u); break;
+    case 7: rc_combine<7>(c, V, C, u); break;
+    default: rc_combine<8>(c, V, C, u); break;
+  }
+  LAUNCH_CHECK();
+}
+
+// store the solution of case k as its newest
+void rc_push(Ctx& c, int k) {
+  rc_reserve(c);
+  Level& L0 = c.L[0];
+  const int64_t n3 = 3 * L0.v.NP;
+  c.rc_head[k] = (c.rc_head[k] + 1) % c.rc_depth;
+  c.rc_cnt[k] = std::min(c.rc_cnt[k] + 1, c.rc_depth);
+  if (L0.n_lin)
+    hipLaunchKernelGGL(k_copy_nodes, dim3(cblk(L0.n_lin)), dim3(256), 0, c.stream, (const int4*)L0.cl_lin,
+                       L0.n_lin, L0.v.NP, (const double*)(c.U + (size_t)k * n3), rc_slot(c, k, 1));
+  LAUNCH_CHECK();
+}
+}  // namespace
+
+TOS_EXPORT int tos_set_start_recycling(tos_ctx* c, int depth) {
+  return guard(c, [&]() {
+    REQUIRE(depth >= 0 && depth <= kRcMax, "bad recycling depth");
+    if (depth != c->rc_depth) {  // a new ring (the old buffers stay with the context)
+      c->rc_depth = depth;
+      c->rc_ring = c->rc_kv = c->rc_part = c->rc_out = nullptr;
+      c->rc_K = 0;
+    }
+    return TOS_OK;
+  });
+}
+
 TOS_EXPORT int tos_solve(tos_ctx* c, const tos_solve_opts* o, tos_solve_stats* st) {
   return guard(c, [&]() {
     REQUIRE(o && o->rtol > 0 && o->maxit >= 0 && o->pre >= 0 && o->post >= 0 && o->precision >= 0 &&
@@ -3066,10 +3250,12 @@ TOS_EXPORT int tos_solve(tos_ctx* c, const tos_solve_opts* o, tos_solve_stats* s
     for (int k = 0; k < c->K; ++k) {
       int its = 0;
       double rr = 0;
+      if (c->rc_depth > 0 && o->warm_start) rc_start(*c, k);
       const int rc = o->precision == 3 ? solve_case_r32(*c, k, *o, &its, &rr) : solve_case(*c, k, *o, &its, &rr);
       if (st && k < 64) { st->iterations[k] = its; st->rel_res[k] = rr; }
       if (rc == TOS_E_NAN) return rc;
       if (rc) rc_all = rc;
+      if (c->rc_depth > 0 && rc == TOS_OK) rc_push(*c, k);
     }
     if (rc_all == TOS_E_NOCONV) c->err = "solver did not reach rtol";
     if (st) st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

@@ -165,6 +165,15 @@ struct Ctx {
   double *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
   float* p32 = nullptr;         // FCG search direction of the FP32-cycle solves (0 off the free nodes)
   float* q32 = nullptr;         // K p stored in FP32 (solve precision 3; allocated on first use)
+  // recycled start (tos_set_start_recycling): per case a ring of the last rc_depth
+  // solutions (FP64, 3*NP each), K times the basis, the Gram partials and their sums
+  int rc_depth = 0;
+  int rc_K = 0;                 // cases the ring is sized for
+  double* rc_ring = nullptr;    // [case][slot] x 3*NP
+  double* rc_kv = nullptr;      // rc_depth x 3*NP
+  double* rc_part = nullptr;    // Gram columns x kRedBlocks
+  double* rc_out = nullptr;     // Gram column sums
+  std::vector<int> rc_head, rc_cnt;  // per case: newest slot, stored solutions
   // reductions
   double* partial = nullptr;    // 2 * max(kRedBlocks, blocks of the fine chunk list) + 2 * 256 stage
   int nb0 = 0;                  // blocks of a launch over the fine chunk list
