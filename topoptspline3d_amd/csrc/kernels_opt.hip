@@ -144,55 +144,72 @@ __global__ __launch_bounds__(256) void k_sensitivity(LevelView L, const uint8_t*
 // ---------------------------------------------------------------------------------------
 // Filter (filterSensitivity_kernel grid/Grid.cu:4131-4216): weighted mean over the active
 // elements of a ball; offsets and weights come from a host-built table.
-// k_filter_tile<H> (table offsets within H per axis): a workgroup owns a 64 x 4 x 4
-// element tile, stages it with an H-element halo in LDS as (value, 1) for active
-// elements and (0, 0) otherwise, and every thread forms four outputs from LDS.  Both
-// kernels walk the table in its order with FP64 sums, so they give the same result (an
-// inactive or outside tap adds exactly zero to both sums).  k_filter serves larger radii.
+// k_filter_zm<H> (table offsets within H per axis): a workgroup owns a 64 x 8 column of
+// elements and marches it along z over kFzZ planes, keeping the 2H + 1 element planes of the
+// current output plane, each with an H-element halo in x and y, in an LDS ring as
+// (value, 1) for active elements and (0, 0) otherwise; every thread forms two outputs per
+// plane from LDS.  Each element is loaded (64 + 2H)(8 + 2H)/512 * (kFzZ + 2H)/kFzZ times
+// (1.8 for H = 2) instead of 4.25 times with cube tiles of 64 x 4 x 4.  Both kernels walk
+// the table in its order with FP64 sums, so they give the same result (an inactive or
+// outside tap adds exactly zero to both sums).  k_filter serves larger radii.
 // ---------------------------------------------------------------------------------------
-constexpr int kFiltTX = 64, kFiltTY = 4, kFiltTZ = 4;
+constexpr int kFzX = 64, kFzY = 8, kFzZ = 32;
 
 template <int H>
-__global__ __launch_bounds__(256) void k_filter_tile(LevelView L, const uint8_t* __restrict__ eact,
-                                                     const float* __restrict__ src, float* __restrict__ dst,
-                                                     const int* __restrict__ off, const double* __restrict__ wt,
-                                                     int nt) {
-  constexpr int LX = kFiltTX + 2 * H, LY = kFiltTY + 2 * H, LZ = kFiltTZ + 2 * H;
+__global__ __launch_bounds__(256) void k_filter_zm(LevelView L, const uint8_t* __restrict__ eact,
+                                                   const float* __restrict__ src, float* __restrict__ dst,
+                                                   const int* __restrict__ off, const double* __restrict__ wt,
+                                                   int nt) {
+  constexpr int LX = kFzX + 2 * H, LY = kFzY + 2 * H, NZ = 2 * H + 1, PL = LX * LY;
   constexpr int NT = (2 * H + 1) * (2 * H + 1) * (2 * H + 1);  // table size bound
-  __shared__ float2 tile[LZ * LY * LX];
-  __shared__ int toff[NT];  // tap offsets within the tile
+  __shared__ float2 ring[NZ * PL];
+  __shared__ int tdz[NT], tin[NT];  // tap plane (dz + H) and offset within a plane
+  __shared__ int toffz[NT];         // LDS offset of each tap for the current output plane
   __shared__ double twt[NT];
-  const int x0 = blockIdx.x * kFiltTX, y0 = blockIdx.y * kFiltTY, z0 = blockIdx.z * kFiltTZ;
+  const int x0 = blockIdx.x * kFzX, y0 = blockIdx.y * kFzY, z0 = blockIdx.z * kFzZ;
   for (int i = threadIdx.x; i < nt; i += 256) {
-    toff[i] = off[3 * i] + LX * (off[3 * i + 1] + LY * off[3 * i + 2]);
+    tdz[i] = off[3 * i + 2] + H;
+    tin[i] = off[3 * i] + LX * off[3 * i + 1];
     twt[i] = wt[i];
   }
-  for (int i = threadIdx.x; i < LZ * LY * LX; i += 256) {
-    const int gx = x0 + i % LX - H, gy = y0 + (i / LX) % LY - H, gz = z0 + i / (LX * LY) - H;
-    const bool in = gx >= 0 && gy >= 0 && gz >= 0 && gx < L.ex && gy < L.ey && gz < L.ez;
-    const int64_t q = L.pelem(in ? gx : 0, in ? gy : 0, in ? gz : 0);
-    const uint8_t a = eact[q];  // both loads issue before either is used
-    const float v = src[q];
-    tile[i] = (in && a) ? make_float2(v, 1.f) : make_float2(0.f, 0.f);
-  }
-  __syncthreads();
-  const int tx = threadIdx.x % kFiltTX, ty = threadIdx.x / kFiltTX;
-  const int x = x0 + tx, y = y0 + ty;
-  if (x >= L.ex || y >= L.ey) return;
-  for (int tz = 0; tz < kFiltTZ && z0 + tz < L.ez; ++tz) {
-    const int64_t e = L.pelem(x, y, z0 + tz);
-    const int c = ((tz + H) * LY + ty + H) * LX + tx + H;
-    if (tile[c].y == 0.f) {
-      dst[e] = 0.f;
-      continue;
+  const int tx = threadIdx.x % kFzX, ty = threadIdx.x / kFzX;  // rows ty and ty + 4
+  const int nplanes = min(kFzZ, L.ez - z0) + 2 * H;
+  for (int zi = 0; zi < nplanes; ++zi) {
+    const int gz = z0 - H + zi;  // plane loaded in this step, into slot zi % NZ
+    float2* pl = ring + (zi % NZ) * PL;
+    for (int i = threadIdx.x; i < PL; i += 256) {
+      const int gx = x0 + i % LX - H, gy = y0 + i / LX - H;
+      const bool in = gx >= 0 && gy >= 0 && gz >= 0 && gx < L.ex && gy < L.ey && gz < L.ez;
+      const int64_t q = L.pelem(in ? gx : 0, in ? gy : 0, in ? gz : 0);
+      const uint8_t a = eact[q];  // both loads issue before either is used
+      const float v = src[q];
+      pl[i] = (in && a) ? make_float2(v, 1.f) : make_float2(0.f, 0.f);
     }
-    double gs = 0, ws = 0;
-    for (int i = 0; i < nt; ++i) {
-      const float2 s = tile[c + toff[i]];
-      gs += twt[i] * (double)s.x;
-      ws += twt[i] * (double)s.y;
+    // output plane gz - H reads the planes loaded in steps zi - 2H .. zi
+    if (zi >= 2 * H)
+      for (int i = threadIdx.x; i < nt; i += 256) toffz[i] = ((zi - 2 * H + tdz[i]) % NZ) * PL + tin[i];
+    __syncthreads();
+    if (zi >= 2 * H) {
+      const float2* cen = ring + ((zi - H) % NZ) * PL;
+      for (int r = 0; r < 2; ++r) {
+        const int yy = ty + 4 * r, x = x0 + tx, y = y0 + yy;
+        if (x >= L.ex || y >= L.ey) continue;
+        const int64_t e = L.pelem(x, y, gz - H);
+        const int c = (yy + H) * LX + tx + H;
+        if (cen[c].y == 0.f) {
+          dst[e] = 0.f;
+          continue;
+        }
+        double gs = 0, ws = 0;
+        for (int i = 0; i < nt; ++i) {
+          const float2 sv = ring[toffz[i] + c];
+          gs += twt[i] * (double)sv.x;
+          ws += twt[i] * (double)sv.y;
+        }
+        dst[e] = (float)(gs / ws);
+      }
     }
-    dst[e] = (float)(gs / ws);
+    __syncthreads();
   }
 }
 
@@ -585,28 +602,53 @@ __global__ __launch_bounds__(256) void k_spline_eval_o(LevelView L, const uint8_
   rho[e] = (float)r;
 }
 
-// First adjoint stage (x) read straight from the padded element field g: the masked
-// value (active, outside the shell, y and z rows inside the spline support; x is masked
-// by the CSR list, which holds only supported elements) of element (x, y, z) is
-// g[e] or 0, the same FP64 sum as a masked copy followed by k_spline_stage axis 0.
-__global__ void k_spline_stage0(LevelView L, const uint8_t* __restrict__ eact, const uint8_t* __restrict__ shell,
-                                const float* __restrict__ g, const uint8_t* __restrict__ oky,
-                                const uint8_t* __restrict__ okz, double* __restrict__ out, int n0,
-                                const int* __restrict__ cptr, const int* __restrict__ celm,
-                                const float* __restrict__ cw) {
-  const int64_t t = tid64();
-  if (t >= (int64_t)n0 * L.ey * L.ez) return;
-  const int c = t % n0, y = (t / n0) % L.ey, z = t / ((int64_t)n0 * L.ey);
-  double acc = 0;
-  if (oky[y] && okz[z]) {
-    const int64_t row = L.pelem(0, y, z);
-    for (int q = cptr[c]; q < cptr[c + 1]; ++q) {
-      const int64_t e = row + celm[q];
-      const double v = (eact[e] && !shell[e]) ? (double)g[e] : 0.0;
-      acc += (double)cw[q] * v;
+// First adjoint stage (x) read straight from the padded element field g.  A workgroup
+// (256 threads) takes `rows` element rows (y0 .. y0 + rows - 1, z): it stages the
+// coefficient lists of axis x (cptr, celm, cw: the same for every row) and the rows' masked
+// values (active, outside the shell, zero when the y or z row lies outside the spline
+// support; x is masked by the lists, which hold only supported elements) in LDS with
+// coalesced loads, then forms out[c, y, z] = the FP64 sum over the list of coefficient c
+// of w * row[x] in list order -- the same sum as a masked copy followed by k_spline_stage
+// axis 0.  Dynamic LDS: rows * ex floats, n0 + 1 + 2 * nq words (nq = cptr[n0]).
+__global__ __launch_bounds__(256) void k_spline_stage0(LevelView L, const uint8_t* __restrict__ eact,
+                                                       const uint8_t* __restrict__ shell, const float* __restrict__ g,
+                                                       const uint8_t* __restrict__ oky, const uint8_t* __restrict__ okz,
+                                                       double* __restrict__ out, int n0, const int* __restrict__ cptr,
+                                                       const int* __restrict__ celm, const float* __restrict__ cw,
+                                                       int rows) {
+  extern __shared__ float sp_lds[];
+  const int nq = cptr[n0];
+  float* srow = sp_lds;                           // rows x ex
+  int* tcp = (int*)(sp_lds + rows * L.ex);        // n0 + 1
+  int* tce = tcp + n0 + 1;                        // nq
+  float* tcw = (float*)(tce + nq);                // nq
+  const unsigned nyb = (unsigned)((L.ey + rows - 1) / rows);
+  const int y0 = (int)(blockIdx.x % nyb) * rows, z = (int)(blockIdx.x / nyb);
+  for (int i = threadIdx.x; i <= n0; i += 256) tcp[i] = cptr[i];
+  for (int i = threadIdx.x; i < nq; i += 256) {
+    tce[i] = celm[i];
+    tcw[i] = cw[i];
+  }
+  const int nr = min(rows, L.ey - y0);
+  const bool okzz = okz[z] != 0;
+  for (int r = 0; r < nr; ++r) {
+    const bool ok = okzz && oky[y0 + r];
+    const int64_t row = L.pelem(0, y0 + r, z);
+    for (int x = threadIdx.x; x < L.ex; x += 256) {
+      const int64_t e = row + x;
+      srow[r * L.ex + x] = (ok && eact[e] && !shell[e]) ? g[e] : 0.f;
     }
   }
-  out[t] = acc;
+  __syncthreads();
+  // rows fastest across the lanes: the lanes' row reads sit ex words apart (spread over
+  // the LDS banks) instead of the coefficient spacing apart (a few banks)
+  for (int i = threadIdx.x; i < nr * n0; i += 256) {
+    const int r = i % nr, c = i / nr;
+    const float* sr = srow + r * L.ex;
+    double acc = 0;
+    for (int q = tcp[c]; q < tcp[c + 1]; ++q) acc += (double)tcw[q] * (double)sr[tce[q]];
+    out[c + (int64_t)n0 * (y0 + r + (int64_t)L.ey * z)] = acc;
+  }
 }
 
 // One separable adjoint stage along axis a of a 3-D array with dims (d0, d1, d2) (x

@@ -3570,17 +3570,17 @@ TOS_EXPORT int tos_filter(tos_ctx* c, int which, double R) {
     HIPCHK(hipMemcpyAsync(c->tmpf, b, v.NE * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
     const int64_t n = (int64_t)v.ex * v.ey * v.ez;
     Prof pr(*c, P_SENS_FILTER);
-    const dim3 tg((v.ex + kFiltTX - 1) / kFiltTX, (v.ey + kFiltTY - 1) / kFiltTY, (v.ez + kFiltTZ - 1) / kFiltTZ);
+    const dim3 tg((v.ex + kFzX - 1) / kFzX, (v.ey + kFzY - 1) / kFzY, (v.ez + kFzZ - 1) / kFzZ);
     const uint8_t* ea = (const uint8_t*)c->eact;
     const float* sb = (const float*)c->tmpf;
     if (Ri <= 1)
-      hipLaunchKernelGGL(k_filter_tile<1>, tg, dim3(256), 0, c->stream, v, ea, sb, b, (const int*)doff,
+      hipLaunchKernelGGL(k_filter_zm<1>, tg, dim3(256), 0, c->stream, v, ea, sb, b, (const int*)doff,
                          (const double*)dwt, nt);
     else if (Ri == 2)
-      hipLaunchKernelGGL(k_filter_tile<2>, tg, dim3(256), 0, c->stream, v, ea, sb, b, (const int*)doff,
+      hipLaunchKernelGGL(k_filter_zm<2>, tg, dim3(256), 0, c->stream, v, ea, sb, b, (const int*)doff,
                          (const double*)dwt, nt);
     else if (Ri == 3)
-      hipLaunchKernelGGL(k_filter_tile<3>, tg, dim3(256), 0, c->stream, v, ea, sb, b, (const int*)doff,
+      hipLaunchKernelGGL(k_filter_zm<3>, tg, dim3(256), 0, c->stream, v, ea, sb, b, (const int*)doff,
                          (const double*)dwt, nt);
     else
       hipLaunchKernelGGL(k_filter, dim3(nblk(n)), dim3(256), 0, c->stream, v, ea, sb, b, (const int*)doff,
@@ -3716,6 +3716,7 @@ TOS_EXPORT int tos_spline_init(tos_ctx* c, int mode, int order, int param) {
             cw[fill[i]] = T[a].wt[(size_t)e * order + q];
             fill[i]++;
           }
+      c->sp_nq[a] = ptr.back();
       c->sp_cptr[a] = dalloc<int>(*c, ptr.size());
       c->sp_celm[a] = dalloc<int>(*c, elm.size());
       c->sp_cw[a] = dalloc<float>(*c, cw.size());
@@ -3822,10 +3823,16 @@ TOS_EXPORT int tos_spline_adjoint(tos_ctx* c, int which) {
     Prof pr(*c, P_SPLINE);
     const int n0 = c->nc[0], n1 = c->nc[1], n2 = c->nc[2];
     int64_t m1 = (int64_t)n0 * v.ey * v.ez, m2 = (int64_t)n0 * n1 * v.ez, m3 = (int64_t)n0 * n1 * n2;
-    hipLaunchKernelGGL(k_spline_stage0, dim3(nblk(m1)), dim3(256), 0, c->stream, v, (const uint8_t*)c->eact,
+    (void)m1;  // a workgroup per `rows` element rows, rows and lists staged in LDS (<= 60 KiB)
+    const size_t tab0 = ((size_t)n0 + 1 + 2 * (size_t)c->sp_nq[0]) * 4;
+    REQUIRE(tab0 + (size_t)v.ex * 4 <= 60 * 1024, "spline lists too long for the first adjoint stage");
+    const int rows = (int)std::max<size_t>(1, std::min<size_t>(8, (60 * 1024 - tab0) / ((size_t)v.ex * 4)));
+    const size_t lds0 = tab0 + (size_t)rows * v.ex * 4;
+    hipLaunchKernelGGL(k_spline_stage0, dim3((unsigned)((int64_t)((v.ey + rows - 1) / rows) * v.ez)),
+                       dim3(256), lds0, c->stream, v, (const uint8_t*)c->eact,
                        (const uint8_t*)c->shell, (const float*)b, (const uint8_t*)c->sp_ok[1],
                        (const uint8_t*)c->sp_ok[2], c->sp_t1, n0, (const int*)c->sp_cptr[0],
-                       (const int*)c->sp_celm[0], (const float*)c->sp_cw[0]);
+                       (const int*)c->sp_celm[0], (const float*)c->sp_cw[0], rows);
     hipLaunchKernelGGL(k_spline_stage, dim3(nblk(m2)), dim3(256), 0, c->stream, (const double*)c->sp_t1, c->sp_t2, 1,
                        n0, n1, v.ez, v.ey, (const int*)c->sp_cptr[1], (const int*)c->sp_celm[1],
                        (const float*)c->sp_cw[1]);

@@ -213,6 +213,7 @@ struct Ctx {
   int* sp_idx[3] = {nullptr, nullptr, nullptr};    // n_elem_axis * order
   float* sp_wt[3] = {nullptr, nullptr, nullptr};
   uint8_t* sp_ok[3] = {nullptr, nullptr, nullptr};
+  int sp_nq[3] = {0, 0, 0};                          // list lengths (sp_cptr[a][nc[a]])
   int* sp_cptr[3] = {nullptr, nullptr, nullptr};   // CSR transpose: per coefficient
   int* sp_celm[3] = {nullptr, nullptr, nullptr};
   float* sp_cw[3] = {nullptr, nullptr, nullptr};
