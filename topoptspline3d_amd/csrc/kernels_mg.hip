@@ -208,6 +208,128 @@ __global__ TOS_FINE_BOUNDS(T) void k_fine_apply(LevelView L, const int4* __restr
   }
 }
 
+// The elements of half AZ of a node's cube (az = AZ: a = 4 AZ .. 4 AZ + 3), the node's
+// row of each, accumulated as in fine_row<T, true> (same FMAs in the same order) from two
+// node planes: lower = the plane at dz = AZ - 1, upper = dz = AZ, index (dx+1) + 3 (dy+1).
+template <int AZ, class T, class TU>
+__device__ __forceinline__ void fine_row_half(const T (&wa)[8], const T* __restrict__ Kc, const TU (&lower)[9][3],
+                                              const TU (&upper)[9][3], T (&acc)[3]) {
+#pragma unroll
+  for (int a = 4 * AZ; a < 4 * AZ + 4; ++a) {
+    const int ax = a & 1, ay = (a >> 1) & 1;
+    const T we = wa[a];
+    const int li = 7 - a;
+    const T sx = (li & 1) ? T(-1) : T(1), sy = (li & 2) ? T(-1) : T(1), sz = (li & 4) ? T(-1) : T(1);
+    T t0 = 0, t1 = 0, t2 = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int nb = (ax + (j & 1)) + 3 * (ay + ((j >> 1) & 1));
+      const TU(&pl)[9][3] = (j >> 2) ? upper : lower;
+      const int d = li ^ j;
+      const T v0 = sx * T(pl[nb][0]), v1 = sy * T(pl[nb][1]), v2 = sz * T(pl[nb][2]);
+      t0 = fma(Kc[ke_key(d, 0, 0)], v0, t0); t0 = fma(Kc[ke_key(d, 0, 1)], v1, t0);
+      t0 = fma(Kc[ke_key(d, 0, 2)], v2, t0);
+      t1 = fma(Kc[ke_key(d, 1, 0)], v0, t1); t1 = fma(Kc[ke_key(d, 1, 1)], v1, t1);
+      t1 = fma(Kc[ke_key(d, 1, 2)], v2, t1);
+      t2 = fma(Kc[ke_key(d, 2, 0)], v0, t2); t2 = fma(Kc[ke_key(d, 2, 1)], v1, t2);
+      t2 = fma(Kc[ke_key(d, 2, 2)], v2, t2);
+    }
+    acc[0] = fma(we * sx, t0, acc[0]);
+    acc[1] = fma(we * sy, t1, acc[1]);
+    acc[2] = fma(we * sz, t2, acc[2]);
+  }
+}
+
+// q = K p for an FP32 search direction p with FP64 arithmetic (the FCG operator of the
+// mixed-precision solves), q stored as TQ (FP64, or FP32 at solve precision 3), with the
+// partial sums of p.q.  The elements below the node (az = 0) read node planes z-1 and z,
+// those above planes z and z+1; plane z+1 is loaded into plane z-1's registers after the
+// first half, so 54 instead of 81 neighbour values are live: 116 VGPRs, 4 waves per SIMD
+// (k_fine_apply<double, 0, float> holds all 81: 250 VGPRs, 2 waves).  Same sums in the
+// same order as fine_row<double, true> with float neighbours (it visits the elements
+// a = 0 .. 7 in order), so the same results bit for bit.
+template <class TQ>
+__global__ __launch_bounds__(256) void k_fine_apply_dir(LevelView L, const int4* __restrict__ cl, int nch,
+                                                        const double* __restrict__ w, const double* __restrict__ Kc,
+                                                        const float* __restrict__ p, double* __restrict__ partial,
+                                                        TQ* __restrict__ q) {
+  NodeRef n;
+  const bool ok = chunk_node(cl, nch, 1, n);
+  double d = 0;
+  if (ok) {
+    float lo[9][3], mid[9][3];
+    const int64_t NXY = (int64_t)L.NX * L.NY;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      const int64_t a = n.p + (k % 3 - 1) + (int64_t)L.NX * (k / 3 - 1);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        lo[k][c] = p[c * L.NP + a - NXY];
+        mid[k][c] = p[c * L.NP + a];
+      }
+    }
+    double wa[8];
+    load_weights(L, w, n.x, n.y, n.z, wa);
+    double acc[3] = {0, 0, 0};
+    fine_row_half<0>(wa, Kc, lo, mid, acc);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      const int64_t a = n.p + (k % 3 - 1) + (int64_t)L.NX * (k / 3 - 1) + NXY;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) lo[k][c] = p[c * L.NP + a];
+    }
+    fine_row_half<1>(wa, Kc, mid, lo, acc);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      q[c * L.NP + n.p] = (TQ)acc[c];
+      d += (double)mid[4][c] * acc[c];
+    }
+  }
+  if (partial) block_partial(d, partial, 0);
+}
+
+// y = K u for an FP64 u in the same two halves (54 FP64 neighbour values live; 102 VGPRs,
+// 4 waves per SIMD), with the partial sums of u.y.  Same results bit for bit as
+// k_fine_apply<double, 0> (fine_row's element order and FMAs).
+__global__ __launch_bounds__(256) void k_fine_apply_u64(LevelView L, const int4* __restrict__ cl, int nch,
+                                                        const double* __restrict__ w, const double* __restrict__ Kc,
+                                                        const double* __restrict__ u, double* __restrict__ y,
+                                                        double* __restrict__ partial) {
+  NodeRef n;
+  const bool ok = chunk_node(cl, nch, 1, n);
+  double d = 0;
+  if (ok) {
+    double lo[9][3], mid[9][3];
+    const int64_t NXY = (int64_t)L.NX * L.NY;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      const int64_t a = n.p + (k % 3 - 1) + (int64_t)L.NX * (k / 3 - 1);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        lo[k][c] = u[c * L.NP + a - NXY];
+        mid[k][c] = u[c * L.NP + a];
+      }
+    }
+    double wa[8];
+    load_weights(L, w, n.x, n.y, n.z, wa);
+    double acc[3] = {0, 0, 0};
+    fine_row_half<0>(wa, Kc, lo, mid, acc);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      const int64_t a = n.p + (k % 3 - 1) + (int64_t)L.NX * (k / 3 - 1) + NXY;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) lo[k][c] = u[c * L.NP + a];
+    }
+    fine_row_half<1>(wa, Kc, mid, lo, acc);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      y[c * L.NP + n.p] = acc[c];
+      d += mid[4][c] * acc[c];
+    }
+  }
+  if (partial) block_partial(d, partial, 0);
+}
+
 // Gauss-Seidel colour pass on the fine level.  BACKWARD relaxes components z, y, x.
 // ZC: see load_nbhd (-1 = every neighbour, c = first forward sweep from u = 0 at colour c)
 // point Gauss-Seidel on the node's 3x3 diagonal block: components x, y, z (BACKWARD:

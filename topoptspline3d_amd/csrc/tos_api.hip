@@ -977,9 +977,17 @@ void apply_level(Ctx& c, int l, const T* u, const T* f, T* y, double* partial = 
   Prof pr(c, l > 0 ? P_ST_RESID
                    : (MODE == 0 ? P_FINE_APPLY64 : (sizeof(T) == 8 ? P_FINE_RESID64 : P_FINE_RESID32)));
   if (l == 0) {
-    if (L.n_lin)
-      hipLaunchKernelGGL((k_fine_apply<T, MODE>), dim3(cblk(L.n_lin)), dim3(256), 0, c.stream, L.v,
-                         (const int4*)L.cl_lin, L.n_lin, Prec<T>::w(c), Prec<T>::Ke(c), u, f, y, partial, y32);
+    if (L.n_lin) {
+      // y = K u in FP64: two halves of the element loop (102 VGPRs, 4 waves per SIMD; 852
+      // against 954 us at the 511 workload); the residual form measured slower that way
+      // (1249 against 1140 us at 256 VGPRs) and keeps k_fine_apply
+      if constexpr (sizeof(T) == 8 && MODE == 0)
+        hipLaunchKernelGGL(k_fine_apply_u64, dim3(cblk(L.n_lin)), dim3(256), 0, c.stream, L.v,
+                           (const int4*)L.cl_lin, L.n_lin, Prec<T>::w(c), Prec<T>::Ke(c), u, y, partial);
+      else
+        hipLaunchKernelGGL((k_fine_apply<T, MODE>), dim3(cblk(L.n_lin)), dim3(256), 0, c.stream, L.v,
+                           (const int4*)L.cl_lin, L.n_lin, Prec<T>::w(c), Prec<T>::Ke(c), u, f, y, partial, y32);
+    }
   } else if (L.n_col) {
     st_dispatch<T>(c, L, [&](auto st) {
       hipLaunchKernelGGL((k_st_apply<T, MODE, decltype(st)>), dim3(cblk(L.n_col)), dim3(256), 0, c.stream, L.v,
@@ -995,9 +1003,8 @@ void apply_fine_dir(Ctx& c, const float* p, double* q, double* partial) {
   Level& L = c.L[0];
   Prof pr(c, P_FINE_APPLY64);
   if (L.n_lin)
-    hipLaunchKernelGGL((k_fine_apply<double, 0, float>), dim3(cblk(L.n_lin)), dim3(256), 0, c.stream, L.v,
-                       (const int4*)L.cl_lin, L.n_lin, Prec<double>::w(c), Prec<double>::Ke(c), p,
-                       (const double*)nullptr, q, partial, (float*)nullptr);
+    hipLaunchKernelGGL((k_fine_apply_dir<double>), dim3(cblk(L.n_lin)), dim3(256), 0, c.stream, L.v,
+                       (const int4*)L.cl_lin, L.n_lin, Prec<double>::w(c), Prec<double>::Ke(c), p, partial, q);
   LAUNCH_CHECK();
 }
 
@@ -2096,9 +2103,8 @@ int solve_case_r32(Ctx& c, int k, const tos_solve_opts& o, int* its, double* rel
   auto tail = [&]() {
     {
       Prof pr(c, P_FINE_APPLY64);
-      hipLaunchKernelGGL((k_fine_apply<double, 0, float>), dim3(cblk(L0.n_lin)), dim3(256), 0, c.stream, L0.v, cl,
-                         nch, Prec<double>::w(c), Prec<double>::Ke(c), (const float*)p32, (const double*)nullptr,
-                         (double*)nullptr, c.partial, q32);
+      hipLaunchKernelGGL((k_fine_apply_dir<float>), dim3(cblk(L0.n_lin)), dim3(256), 0, c.stream, L0.v, cl, nch,
+                         Prec<double>::w(c), Prec<double>::Ke(c), (const float*)p32, c.partial, q32);
     }
     Prof pr(c, P_VECTOR);
     finish0(c, 0, S_PQ, 0, OP_ALPHA);
