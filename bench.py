@@ -68,6 +68,33 @@ def make_workload(name):
         return dict(name="c2", desc="ellipsoid union 511x320x320, spline spacing 4 (cubic), MMA",
                     solid=solid, support=support, f=f, shell=shell, meta=meta, spline=(1, 4, 4),
                     optimizer="mma", move=0.06, rtol=1e-6)
+    if name == "c2slab":
+        # one GPU's share of an 8-way x-slab split of the ellipsoid workload: 64 of the 511
+        # element columns, taken where the body stands on its supports (z = 0 touches the
+        # solid for x in 331..464) so that the slab alone is a supported problem, keeping
+        # the part connected to those supports inside the slab; same spline / MMA loop
+        from scipy import ndimage
+        solid, support, f, shell, meta = ellipsoids(511, 320, 320, seed=2024)
+        x0, w = 368, 64
+        sol = solid[:, :, x0:x0 + w].astype(bool)
+        lab, _ = ndimage.label(sol)
+        keep = np.zeros(lab.max() + 1, bool)
+        keep[np.unique(lab[0])] = True
+        keep[0] = False
+        sol = keep[lab]
+        nz, ny = sol.shape[:2]
+        act = np.zeros((nz + 1, ny + 1, w + 1), bool)
+        for dz in (0, 1):
+            for dy in (0, 1):
+                for dx in (0, 1):
+                    act[dz:dz + nz, dy:dy + ny, dx:dx + w] |= sol
+        sup = support.reshape(nz + 1, ny + 1, -1)[:, :, x0:x0 + w + 1] & act
+        ff = f.reshape(3, nz + 1, ny + 1, -1)[:, :, :, x0:x0 + w + 1] * act
+        sh = shell[:, :, x0:x0 + w] & sol
+        return dict(name="c2slab", desc="x-slab 64x320x320 (columns 368..431) of the ellipsoid union 511x320x320, "
+                    "spline spacing 4 (cubic), MMA", solid=sol.astype(np.uint8), support=sup.astype(np.uint8),
+                    f=ff.reshape(-1), shell=sh.astype(np.uint8), meta={}, spline=(1, 4, 4), optimizer="mma",
+                    move=0.06, rtol=1e-6)
     if name == "c1":
         solid, support, f = cantilever(128, 64, 64)
         return dict(name="c1", desc="cantilever 128x64x64, spline spacing 4 (cubic), MMA",
