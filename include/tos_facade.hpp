@@ -75,6 +75,7 @@ struct Api {
   TOS_FN(int, surface_constraints, tos_ctx*, const tos_surface_opts*, tos_surface_stats*)
   TOS_FN(int, spline_get_coeff_sens, tos_ctx*, int, float*)
   TOS_FN(int, ctx_mma_step_surface, tos_ctx*, tos_mma*, double, double, double, double, double, int*)
+  TOS_FN(int, set_start_recycling, tos_ctx*, int)
 #undef TOS_FN
   void* handle = nullptr;
 
@@ -101,7 +102,7 @@ struct Api {
     TOS_SYM(spline_eval) TOS_SYM(spline_adjoint) TOS_SYM(oc_update_coeffs) TOS_SYM(mma_create)
     TOS_SYM(mma_destroy) TOS_SYM(mma_set_x) TOS_SYM(mma_get_x) TOS_SYM(mma_update) TOS_SYM(ctx_mma_step)
     TOS_SYM(worst_compliance) TOS_SYM(balance_loads) TOS_SYM(remove_rigid) TOS_SYM(get_loads) TOS_SYM(oc_update_projected) TOS_SYM(surface_constraints)
-    TOS_SYM(spline_get_coeff_sens) TOS_SYM(ctx_mma_step_surface)
+    TOS_SYM(spline_get_coeff_sens) TOS_SYM(ctx_mma_step_surface) TOS_SYM(set_start_recycling)
 #undef TOS_SYM
     return a;
   }
@@ -231,6 +232,13 @@ class HierarchyGrid {
     tos::check(tos::Api::get().solve(c_, &o, &st), c_, "solve");
     last_iterations_ = st.iterations[0];
     return st.rel_res[0];
+  }
+  // no counterpart in the reference (its solves start from the previous displacement):
+  // every later solve starts at the Galerkin projection of the solution onto the current
+  // displacement and the depth - 1 previous solutions (tos_set_start_recycling); call
+  // after the grids are built, 0 switches it off
+  void set_start_recycling(int depth) {
+    tos::check(tos::Api::get().set_start_recycling(c_, depth), c_, "set_start_recycling");
   }
   double v_cycle(int pre = 1, int post = 1) {
     tos_solve_opts o{1e-300, 1, 0, pre, post, 1, 0};

@@ -30,8 +30,9 @@ def worst_driver(tmp_path_factory):
     return _compile(tmp_path_factory, SRC_WORST, "worst_case_topopt")
 
 
-def run(exe, lib, prefix, dims, iters):
-    out = subprocess.check_output([exe, lib, prefix, *map(str, dims), str(iters)], text=True, timeout=600)
+def run(exe, lib, prefix, dims, iters, *extra):
+    out = subprocess.check_output([exe, lib, prefix, *map(str, dims), str(iters), *map(str, extra)], text=True,
+                                  timeout=600)
     return np.array([[float(v) for v in line.split()] for line in out.strip().splitlines()])
 
 
@@ -42,6 +43,16 @@ def test_facade_drives_the_oracle(driver):
     ref = run_oc(oracle_api(), 24, 12, 12, iters=4, rtol=1e-8)
     assert np.allclose(rows[:, 1], ref["c"], rtol=1e-10)
     assert np.all(rows[:, 4] <= 1e-8)
+
+
+def test_facade_recycled_start_on_the_oracle(driver):
+    """getGrids().set_start_recycling(4) through the facade: the same compliance trajectory
+    with no more iterations than the plain warm start"""
+    oracle_api()
+    plain = run(driver, ORACLE_LIB, "orc_", (24, 12, 12), 8)
+    rec = run(driver, ORACLE_LIB, "orc_", (24, 12, 12), 8, 4)
+    assert np.all(np.abs(rec[:, 1] - plain[:, 1]) <= 1e-7 * plain[:, 1])
+    assert rec[:, 3].sum() < plain[:, 3].sum() and np.all(rec[:, 4] <= 1e-8)
 
 
 @pytest.mark.gpu

@@ -1,10 +1,12 @@
 // ordinary_topopt.cpp -- the reference's ordinary SIMP loop (TestSuit::testOrdinaryTopopt,
 // utils/test_utils.cpp:604-706) written against the reference-named facade.
 //
-//   ordinary_topopt <library.so> <symbol prefix> <nx> <ny> <nz> <iterations>
+//   ordinary_topopt <library.so> <symbol prefix> <nx> <ny> <nz> <iterations> [recycle depth]
 //
 // Cantilever: x = 0 fixed, unit total -z load on the bottom edge at x = nx.  Prints one
-// line per iteration: "itn c v pcg_iterations rel_res".
+// line per iteration: "itn c v pcg_iterations rel_res".  A recycle depth m > 0 starts
+// every solve at the Galerkin projection onto the last m solutions (an addition of this
+// library; the reference starts from the previous displacement).
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -19,6 +21,7 @@ int main(int argc, char** argv) {
   try {
     tos::Api::load(argv[1], argv[2]);
     const int nx = std::atoi(argv[3]), ny = std::atoi(argv[4]), nz = std::atoi(argv[5]), iters = std::atoi(argv[6]);
+    const int recycle = argc > 7 ? std::atoi(argv[7]) : 0;
     const size_t ne = (size_t)nx * ny * nz, nn = (size_t)(nx + 1) * (ny + 1) * (nz + 1);
     std::vector<uint8_t> solid(ne, 1), support(nn, 0);
     for (int z = 0; z <= nz; ++z)
@@ -33,6 +36,7 @@ int main(int argc, char** argv) {
     buildGrids(nx, ny, nz, solid.data(), support.data(), nullptr);
     uploadTemplateMatrix();
     getGrids().setLoads(1, f.data());
+    if (recycle > 0) getGrids().set_start_recycling(recycle);
     initDensities(params().volume_ratio);
     const float Vgoal = params().volume_ratio;
     for (int itn = 1; itn <= iters; ++itn) {
