@@ -1843,6 +1843,55 @@ __global__ __launch_bounds__(256) void k_pcg_up(const int4* __restrict__ cl, int
   }
 }
 
+// Precision-3 FCG with four direction buffers: p_j goes to buffer j % 4, and u takes the
+// steps alpha_k p_k of four directions in one pass (k_pcg_flush) instead of one read and
+// write of the FP64 u per iteration.
+// p_new = z + beta p_old
+__global__ __launch_bounds__(256) void k_pcg_pnew(const int4* __restrict__ cl, int nch, int64_t NP,
+                                                  float* __restrict__ pn, const float* __restrict__ z,
+                                                  const float* __restrict__ po, const double* __restrict__ scal) {
+  NodeRef n;
+  if (!chunk_node(cl, nch, 1, n)) return;
+  const double be = scal[S_BETA];
+  double zv[3], pv[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const int64_t i = c * NP + n.p;
+    zv[c] = (double)z[i];
+    pv[c] = (double)po[i];
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) pn[c * NP + n.p] = (float)(zv[c] + be * pv[c]);
+}
+
+// u += alpha_k p_k for the m directions of D in order (the same rounding per step as
+// one step per iteration)
+struct PcgDirs {
+  const float* p[4];
+  int aslot[4];  // scal slot of each direction's alpha
+};
+__global__ __launch_bounds__(256) void k_pcg_flush(const int4* __restrict__ cl, int nch, int64_t NP,
+                                                   double* __restrict__ u, PcgDirs D, int m,
+                                                   const double* __restrict__ scal) {
+  NodeRef n;
+  if (!chunk_node(cl, nch, 1, n)) return;
+  double al[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) al[k] = k < m ? scal[D.aslot[k]] : 0.0;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const int64_t i = c * NP + n.p;
+    double uv = u[i];
+    float pv[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) pv[k] = k < m ? D.p[k][i] : 0.f;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (k < m) uv = uv + al[k] * (double)pv[k];
+    u[i] = uv;
+  }
+}
+
 // u += alpha p (the last deferred step of a solve)
 template <class TP>
 __global__ __launch_bounds__(256) void k_pcg_ustep(const int4* __restrict__ cl, int nch, int64_t NP,

@@ -79,7 +79,10 @@ __global__ __launch_bounds__(256) void k_finish(const double* __restrict__ parti
   if (threadIdx.x == 0) {
     scal[slot0] = s0;
     if (two) scal[slot1] = s1;
-    if (op == OP_ALPHA) scal[S_ALPHA] = scal[S_RZ] / scal[S_PQ];
+    if (op == OP_ALPHA) {
+      scal[S_ALPHA] = scal[S_RZ] / scal[S_PQ];
+      if (!two && slot1 > 0) scal[slot1] = scal[S_ALPHA];  // (the direction's alpha kept)
+    }
     if (op == OP_BETA) {  // flexible (Polak-Ribiere) beta = -alpha <z,q> / rz_old
       scal[S_BETA] = -scal[S_ALPHA] * scal[S_ZQ] / scal[S_RZ];
       scal[S_RZ] = scal[S_ZR];

@@ -2022,7 +2022,7 @@ int solve_case(Ctx& c, int k, const tos_solve_opts& o, int* its, double* relres)
         first = false;
       } else if (graphed) {
         if (!exec) {
-          Ctx::SolveGraph g{o.precision, o.pre, o.post, c.sweep_mode, c.gal_prec, u, 0, nullptr, nullptr};
+          Ctx::SolveGraph g{o.precision, o.pre, o.post, c.sweep_mode, c.gal_prec, 0, u, 0, nullptr, nullptr};
           HIPCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
           head();
           tail();
@@ -2099,33 +2099,48 @@ int solve_case_r32(Ctx& c, int k, const tos_solve_opts& o, int* its, double* rel
     c.err = "NaN/Inf in the initial residual";
     return TOS_E_NAN;
   }
-  // q = K p (FP64 arithmetic, FP32 result), alpha, r -= alpha q, |r|^2
-  auto tail = [&]() {
+  // four direction buffers: direction j lives in P[j % 4] with its alpha in S_AH0 + j % 4,
+  // and u takes four steps at a time (k_pcg_flush) instead of one read and write of the
+  // FP64 u per iteration; the same steps in the same order as one at a time
+  for (float*& b : c.p32s)
+    if (!b) b = dalloc<float>(c, n3);
+  float* const P[4] = {p32, c.p32s[0], c.p32s[1], c.p32s[2]};
+  auto flush = [&](int m) {  // u += alpha_k p_k for the directions in P[0 .. m-1]
+    PcgDirs D{};
+    for (int s = 0; s < m; ++s) {
+      D.p[s] = P[s];
+      D.aslot[s] = S_AH0 + s;
+    }
+    hipLaunchKernelGGL(k_pcg_flush, dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, u, D, m, (const double*)c.scal);
+  };
+  // q = K p_j (FP64 arithmetic, FP32 result), alpha_j, r -= alpha_j q, |r|^2
+  auto tail = [&](int s) {
     {
       Prof pr(c, P_FINE_APPLY64);
       hipLaunchKernelGGL((k_fine_apply_dir<float>), dim3(cblk(L0.n_lin)), dim3(256), 0, c.stream, L0.v, cl, nch,
-                         Prec<double>::w(c), Prec<double>::Ke(c), (const float*)p32, c.partial, q32);
+                         Prec<double>::w(c), Prec<double>::Ke(c), (const float*)P[s], c.partial, q32);
     }
     Prof pr(c, P_VECTOR);
-    finish0(c, 0, S_PQ, 0, OP_ALPHA);
+    finish0(c, 0, S_PQ, S_AH0 + s, OP_ALPHA);
     hipLaunchKernelGGL(k_pcg_r32, dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, r32, (const float*)q32,
                        (const double*)c.scal, c.partial);
     finish0(c, 0, S_RN2, 0, OP_NONE);
     LAUNCH_CHECK();
   };
-  // z = B r, beta, u += alpha p (deferred), p = z + beta p
-  auto head = [&]() {
+  // z = B r, beta, (buffer 0: the four pending steps into u), p_j = z + beta p_{j-1}
+  auto head = [&](int s) {
     precondition(c, 2, o.pre, o.post, true, false);
     Prof pr(c, P_VECTOR);
     hipLaunchKernelGGL(k_pcg_zdots32, dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, z32, (const float*)r32,
                        (const float*)q32, c.partial);
     finish0(c, 1, S_ZR, S_ZQ, OP_BETA);
-    hipLaunchKernelGGL((k_pcg_up<float, float>), dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, u, p32, z32,
-                       (const double*)c.scal, 0);
+    if (s == 0) flush(4);
+    hipLaunchKernelGGL(k_pcg_pnew, dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, P[s], z32,
+                       (const float*)P[(s + 3) % 4], (const double*)c.scal);
     LAUNCH_CHECK();
   };
   const bool graphed = c.graph_replay && !c.prof_on;
-  hipGraphExec_t exec = nullptr;
+  hipGraphExec_t exec[4] = {nullptr, nullptr, nullptr, nullptr};
   if (graphed) {
     bool stale = false;
     for (const Ctx::SolveGraph& g : c.graphs) stale |= g.epoch != c.alloc_epoch;
@@ -2139,49 +2154,48 @@ int solve_case_r32(Ctx& c, int k, const tos_solve_opts& o, int* its, double* rel
     for (const Ctx::SolveGraph& g : c.graphs)
       if (g.precision == 3 && g.pre == o.pre && g.post == o.post && g.sweep == c.sweep_mode && g.gal == c.gal_prec &&
           g.u == u)
-        exec = g.exec;
+        exec[g.slot] = g.exec;
   }
   int it = 0;
   while (rn / nf > o.rtol && it < o.maxit) {
-    // FCG on r32 (from the true residual: at the start and after a restart)
+    // FCG on r32 (from the true residual: at the start and after a restart); p_0 = z
     precondition(c, 2, o.pre, o.post, true, false);
     {
       Prof pr(c, P_VECTOR);
-      hipLaunchKernelGGL((k_pcg_up<float, float>), dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, u, p32, z32,
+      hipLaunchKernelGGL((k_pcg_up<float, float>), dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, u, P[0], z32,
                          (const double*)c.scal, 1);
       hipLaunchKernelGGL(k_pcg_zdots32, dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, z32, (const float*)r32,
                          (const float*)nullptr, c.partial);
       finish0(c, 0, S_RZ, 0, OP_NONE);
     }
-    bool first = true;
+    int j = -1;  // index of the last direction of this run
     while (it < o.maxit) {
-      if (first) {
-        tail();
-        first = false;
+      const int s = ++j % 4;
+      if (j == 0) {
+        tail(0);
       } else if (graphed) {
-        if (!exec) {
-          Ctx::SolveGraph g{3, o.pre, o.post, c.sweep_mode, c.gal_prec, u, 0, nullptr, nullptr};
+        if (!exec[s]) {
+          Ctx::SolveGraph g{3, o.pre, o.post, c.sweep_mode, c.gal_prec, s, u, 0, nullptr, nullptr};
           HIPCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
-          head();
-          tail();
+          head(s);
+          tail(s);
           HIPCHK(hipStreamEndCapture(c.stream, &g.graph));
           HIPCHK(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
           g.epoch = c.alloc_epoch;
           c.graphs.push_back(g);
-          exec = g.exec;
+          exec[s] = g.exec;
         }
-        HIPCHK(hipGraphLaunch(exec, c.stream));
+        HIPCHK(hipGraphLaunch(exec[s], c.stream));
       } else {
-        head();
-        tail();
+        head(s);
+        tail(s);
       }
       ++it;
       const double r_in = std::sqrt(read_scal(c)[S_RN2]);
       if (!std::isfinite(r_in)) { c.err = "NaN/Inf in PCG"; return TOS_E_NAN; }
       if (r_in / nf <= o.rtol) break;
     }
-    hipLaunchKernelGGL((k_pcg_ustep<float>), dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, u, (const float*)p32,
-                       (const double*)c.scal);
+    flush(j % 4 + 1);  // the steps of the directions since the last flush
     // the true residual in FP64; its FP32 copy restarts the FCG when it misses the tolerance
     apply_level<double, 1>(c, 0, u, f, c.r, nullptr, r32);
     dot_to(c, c.r, c.r, nullptr, nullptr, S_RN2, 0, OP_NONE);

@@ -165,6 +165,7 @@ struct Ctx {
   double *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
   float* p32 = nullptr;         // FCG search direction of the FP32-cycle solves (0 off the free nodes)
   float* q32 = nullptr;         // K p stored in FP32 (solve precision 3; allocated on first use)
+  float* p32s[3] = {nullptr, nullptr, nullptr};  // direction buffers 1..3 of the precision-3 FCG
   // recycled start (tos_set_start_recycling): per case a ring of the last rc_depth
   // solutions (FP64, 3*NP each), K times the basis, the Gram partials and their sums
   int rc_depth = 0;
@@ -273,6 +274,7 @@ struct Ctx {
   bool graph_replay = true;     // replay the captured FCG iteration body
   struct SolveGraph {           // captured FCG iteration body per solve configuration
     int precision, pre, post, sweep, gal;
+    int slot;                   // precision 3: the direction buffer the body writes
     const double* u;
     uint64_t epoch;             // alloc_epoch at capture
     hipGraph_t graph;
@@ -295,6 +297,8 @@ enum {
   S_NN0, S_NN1, S_NN2,               // learned correction: |r~|^2, e.r, e.(r - A e)
   S_NN_TRIED, S_NN_KEPT, S_NN_STEP,  // its statistics since the last attach
   S_RIG0,                            // 6 slots: R^T v of a rigid-motion projection
-  S_COUNT = 32
+  S_AH0 = S_RIG0 + 6,                // 4 slots: alpha of the directions in the four direction
+                                     // buffers of the precision-3 FCG
+  S_COUNT = 40
 };
-static_assert(S_RIG0 + 6 <= S_COUNT, "scalar slots");
+static_assert(S_AH0 + 4 <= S_COUNT, "scalar slots");
