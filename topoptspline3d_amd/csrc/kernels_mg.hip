@@ -2002,19 +2002,28 @@ __global__ __launch_bounds__(256) void k_rc_gram(const int4* __restrict__ cl, in
   for (int q = 0; q < NG; ++q) block_partial(g[q], partial, q);
 }
 
-// u = sum_a c_a v_a on the free nodes (v_0 may be u itself)
+// u = sum_a c_a v_a on the free nodes (v_0 may be u itself), and the solve's first
+// residual r = f - sum_a c_a (K v_a) from the products the Gram pass read (K u by
+// linearity, without another operator application), with its FP32 copy when r32 is set
 template <int M>
 __global__ __launch_bounds__(256) void k_rc_combine(const int4* __restrict__ cl, int nch, int64_t NP, RcVecs V,
-                                                    RcCoef C, double* __restrict__ u) {
+                                                    RcCoef C, const double* __restrict__ f, double* __restrict__ u,
+                                                    double* __restrict__ r, float* __restrict__ r32) {
   NodeRef n;
   if (!chunk_node(cl, nch, 1, n)) return;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     const int64_t i = c * NP + n.p;
-    double s = 0;
+    double s = 0, t = 0;
 #pragma unroll
-    for (int a = 0; a < M; ++a) s = fma(C.c[a], V.v[a][i], s);
+    for (int a = 0; a < M; ++a) {
+      s = fma(C.c[a], V.v[a][i], s);
+      t = fma(C.c[a], V.kv[a][i], t);
+    }
     u[i] = s;
+    const double rv = f[i] - t;
+    r[i] = rv;
+    if (r32) r32[i] = (float)rv;
   }
 }
 

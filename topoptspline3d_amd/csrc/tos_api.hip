@@ -1886,7 +1886,8 @@ int solve_case(Ctx& c, int k, const tos_solve_opts& o, int* its, double* relres)
   const int nch = L0.n_lin;
   if (!o.warm_start && nch)
     hipLaunchKernelGGL(k_zero_nodes<double>, dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, u);
-  apply_level<double, 1>(c, 0, u, f, c.r, nullptr, f32 ? L0.f32 : nullptr);  // r = f - K u
+  if (c.r_ready) c.r_ready = false;  // r = f - K u formed by the recycled start
+  else apply_level<double, 1>(c, 0, u, f, c.r, nullptr, f32 ? L0.f32 : nullptr);  // r = f - K u
   dot_to(c, f, f, c.r, c.r, S_NF2, S_RN2, OP_NONE);
   const double* hs = read_scal(c);
   const double nf = std::sqrt(hs[S_NF2]);
@@ -2084,7 +2085,8 @@ int solve_case_r32(Ctx& c, int k, const tos_solve_opts& o, int* its, double* rel
   const float* z32 = L0.u32;
   if (!o.warm_start && nch)
     hipLaunchKernelGGL(k_zero_nodes<double>, dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, u);
-  apply_level<double, 1>(c, 0, u, f, c.r, nullptr, r32);  // FP64 residual and its FP32 copy
+  if (c.r_ready) c.r_ready = false;  // formed by the recycled start
+  else apply_level<double, 1>(c, 0, u, f, c.r, nullptr, r32);  // FP64 residual and its FP32 copy
   dot_to(c, f, f, c.r, c.r, S_NF2, S_RN2, OP_NONE);
   const double* hs = read_scal(c);
   const double nf = std::sqrt(hs[S_NF2]);
@@ -3167,15 +3169,17 @@ void rc_launch(Ctx& c, const RcVecs& V, const double* f, unsigned nbg) {
 }
 
 template <int M>
-void rc_combine(Ctx& c, const RcVecs& V, const RcCoef& C, double* u) {
+void rc_combine(Ctx& c, const RcVecs& V, const RcCoef& C, const double* f, double* u, float* r32) {
   Level& L0 = c.L[0];
   hipLaunchKernelGGL((k_rc_combine<M>), dim3(cblk(L0.n_lin)), dim3(256), 0, c.stream, (const int4*)L0.cl_lin,
-                     L0.n_lin, L0.v.NP, V, C, u);
+                     L0.n_lin, L0.v.NP, V, C, f, u, c.r, r32);
+  c.r_ready = true;
 }
 
 // u_k <- the projected start (before the solve of case k)
-void rc_start(Ctx& c, int k) {
+void rc_start(Ctx& c, int k, bool f32) {
   rc_reserve(c);
+  float* r32 = f32 ? c.L[0].f32 : nullptr;
   Level& L0 = c.L[0];
   const int64_t n3 = 3 * L0.v.NP;
   double* u = c.U + (size_t)k * n3;
@@ -3214,17 +3218,19 @@ void rc_start(Ctx& c, int k) {
     for (int e = a; e < m; ++e) A[a * m + e] = A[e * m + a] = g[q++];
   for (int a = 0; a < m; ++a) b[a] = g[q++];
   RcCoef C{};
-  if (!rc_coefficients(m, A, b, C.c)) return;  // keep the warm start
+  if (!rc_coefficients(m, A, b, C.c)) {  // keep the warm start (u = 1 u exactly, r = f - K u)
+    for (int a = 0; a < m; ++a) C.c[a] = a == 0 ? 1.0 : 0.0;
+  }
   Prof pr(c, P_VECTOR);
   switch (m) {
-    case 1: rc_combine<1>(c, V, C, u); break;
-    case 2: rc_combine<2>(c, V, C, u); break;
-    case 3: rc_combine<3>(c, V, C, u); break;
-    case 4: rc_combine<4>(c, V, C, u); break;
-    case 5: rc_combine<5>(c, V, C, u); break;
-    case 6: rc_combine<6>(c, V, C, u); break;
-    case 7: rc_combine<7>(c, V, C, u); break;
-    default: rc_combine<8>(c, V, C, u); break;
+    case 1: rc_combine<1>(c, V, C, f, u, r32); break;
+    case 2: rc_combine<2>(c, V, C, f, u, r32); break;
+    case 3: rc_combine<3>(c, V, C, f, u, r32); break;
+    case 4: rc_combine<4>(c, V, C, f, u, r32); break;
+    case 5: rc_combine<5>(c, V, C, f, u, r32); break;
+    case 6: rc_combine<6>(c, V, C, f, u, r32); break;
+    case 7: rc_combine<7>(c, V, C, f, u, r32); break;
+    default: rc_combine<8>(c, V, C, f, u, r32); break;
   }
   LAUNCH_CHECK();
 }
@@ -3270,7 +3276,8 @@ TOS_EXPORT int tos_solve(tos_ctx* c, const tos_solve_opts* o, tos_solve_stats* s
     for (int k = 0; k < c->K; ++k) {
       int its = 0;
       double rr = 0;
-      if (c->rc_depth > 0 && o->warm_start) rc_start(*c, k);
+      c->r_ready = false;
+      if (c->rc_depth > 0 && o->warm_start) rc_start(*c, k, o->precision != 0);
       const int rc = o->precision == 3 ? solve_case_r32(*c, k, *o, &its, &rr) : solve_case(*c, k, *o, &its, &rr);
       if (st && k < 64) { st->iterations[k] = its; st->rel_res[k] = rr; }
       if (rc == TOS_E_NAN) return rc;

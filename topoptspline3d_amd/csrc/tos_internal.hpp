@@ -172,6 +172,7 @@ struct Ctx {
   int rc_K = 0;                 // cases the ring is sized for
   double* rc_ring = nullptr;    // [case][slot] x 3*NP
   double* rc_kv = nullptr;      // rc_depth x 3*NP
+  bool r_ready = false;         // c.r (and its FP32 copy) = f - K u of the next solve, from the recycled start
   double* rc_part = nullptr;    // Gram columns x kRedBlocks
   double* rc_out = nullptr;     // Gram column sums
   std::vector<int> rc_head, rc_cnt;  // per case: newest slot, stored solutions
