@@ -421,11 +421,16 @@ __global__ TOS_FINE_BOUNDS(T) void k_fine_gs_pair(LevelView L, const int4* __res
 // seg = {padded index of (xs, y, z), (xs + 1) | (xe + 1) << 15 | cont << 30, y, z}: xe the
 // row's last free node, cont = 1 when lane 0 repeats the previous segment's lane 63.
 // ZA / ZB: -1, or the colours of a first forward sweep from zero (see load_nbhd).
-template <class T, bool BACKWARD, int ZA, int ZB>
+// DOTS (the last sweep of a V-cycle's post-smoothing, whose new values are final): each
+// wave also writes its sums of u.f and u.q over the nodes it relaxed to dpart[si] and
+// dpart[dT + si], so the FCG's z.r and z.q need no pass of their own.
+template <class T, bool BACKWARD, int ZA, int ZB, bool DOTS = false>
 __global__ TOS_FINE_BOUNDS(T) void k_fine_gs_pairw(LevelView L, const int4* __restrict__ seg, int nseg,
                                                    const uint8_t* __restrict__ fre, const T* __restrict__ w,
                                                    const T* __restrict__ Kc, T* __restrict__ u,
-                                                   const T* __restrict__ f, T* __restrict__ ub) {
+                                                   const T* __restrict__ f, T* __restrict__ ub,
+                                                   const T* __restrict__ q, double* __restrict__ dpart,
+                                                   int64_t dT) {
   constexpr unsigned MA = nb_mask_of(ZA), MB = nb_mask_of(ZB);
   const int si = (int)xcd_block() * 4 + (threadIdx.x >> 6);
   if (si >= nseg) return;  // whole wave
@@ -465,11 +470,15 @@ __global__ TOS_FINE_BOUNDS(T) void k_fine_gs_pairw(LevelView L, const int4* __re
     for (int a = 0; a < 8; ++a)
       wa[a] = w[((a & 1) ? e1 : e0) + (int64_t)L.EX * (((a >> 1) & 1) + (int64_t)L.EY * (a >> 2))];
   }
-  T fa[3], fb[3];
+  T fa[3], fb[3], qa[3], qb[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     fa[c] = f[c * L.NP + pa];
     fb[c] = f[c * L.NP + pa + 1];
+    if constexpr (DOTS) {
+      qa[c] = q[c * L.NP + pa];
+      qb[c] = q[c * L.NP + pa + 1];
+    }
   }
   if constexpr (sizeof(T) == 4) TOS_LOADS_DONE();
   // c1 (from zero: its own start value is 0; un[13] may hold the old value, loaded for c2)
@@ -524,6 +533,26 @@ __global__ TOS_FINE_BOUNDS(T) void k_fine_gs_pairw(LevelView L, const int4* __re
   if (f2) {
     T* dst = (lane == 0 || lane == 62) ? ub : u;
     dst[p1 + 1] = t0; dst[L.NP + p1 + 1] = t1; dst[2 * L.NP + p1 + 1] = t2;
+  }
+  if constexpr (DOTS) {  // each node once: where it is written (to u or to ub)
+    double zr = 0, zq = 0;
+    if (f1 && !(cont && lane == 0)) {
+      zr = (double)v0 * (double)fa[0] + (double)v1 * (double)fa[1] + (double)v2 * (double)fa[2];
+      zq = (double)v0 * (double)qa[0] + (double)v1 * (double)qa[1] + (double)v2 * (double)qa[2];
+    }
+    if (f2) {
+      zr += (double)t0 * (double)fb[0] + (double)t1 * (double)fb[1] + (double)t2 * (double)fb[2];
+      zq += (double)t0 * (double)qb[0] + (double)t1 * (double)qb[1] + (double)t2 * (double)qb[2];
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      zr += __shfl_xor(zr, o, 64);
+      zq += __shfl_xor(zq, o, 64);
+    }
+    if (lane == 0) {
+      dpart[si] = zr;
+      dpart[dT + si] = zq;
+    }
   }
 }
 

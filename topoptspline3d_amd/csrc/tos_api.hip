@@ -651,6 +651,13 @@ void finish0(Ctx& c, int two, int s0, int s1, int op) {
                      c.scal);
 }
 
+// scal[s0], scal[s1] = sums of the two sets of np partials part[0 .. np), part[np .. 2 np)
+void finish_parts(Ctx& c, const double* part, int64_t np, int s0, int s1, int op) {
+  double* st = c.partial + 2 * (size_t)std::max(kRedBlocks, c.nb0);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(kStage), dim3(256), 0, c.stream, part, (int)np, 1, st);
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, c.stream, (const double*)st, kStage, 1, s0, s1, op, c.scal);
+}
+
 // scal[s0] = a.b (and scal[s1] = a2.b2) over the free fine nodes
 void dot_to(Ctx& c, const double* a, const double* b, const double* a2, const double* b2, int s0, int s1, int op) {
   Prof pr(c, P_VECTOR);
@@ -829,6 +836,12 @@ void gs_sweep_pairw(Ctx& c, float* u, const float* f, bool backward, bool from_z
   const LevelView& v = L.v;
   if (!L.ub32) L.ub32 = dalloc<float>(c, 3 * (size_t)v.NP);
   const int P = backward ? 1 : 0;
+  // the FCG's z.r and z.q from this sweep's final values (one sum per wave and pass)
+  // (c.gsdots is sized by the solve before any capture: one zr and one zq per segment)
+  int64_t nd = 0;
+  for (int i = 0; i < 4; ++i) nd += L.n_segw[i][1];
+  const bool dots = backward && c.dots_last && c.dots_q && c.gsdots && c.gsdots_n == nd;
+  int64_t doff = 0;
   for (int i = 0; i < 4; ++i) {
     const int cls = backward ? 3 - i : i;
     const int n = L.n_segw[cls][P];
@@ -837,15 +850,19 @@ void gs_sweep_pairw(Ctx& c, float* u, const float* f, bool backward, bool from_z
     const dim3 g((n + 3) / 4), b(256);
     const int4* sg = L.segw[cls][P];
     const uint8_t* fr = L.fre;
-    if (backward) {
+    if (dots) {
+      hipLaunchKernelGGL((k_fine_gs_pairw<float, true, -1, -1, true>), g, b, 0, c.stream, v, sg, n, fr, c.w32,
+                         c.Kc32, u, f, L.ub32, c.dots_q, c.gsdots + doff, nd);
+      doff += n;
+    } else if (backward) {
       hipLaunchKernelGGL((k_fine_gs_pairw<float, true, -1, -1>), g, b, 0, c.stream, v, sg, n, fr, c.w32, c.Kc32, u, f,
-                         L.ub32);
+                         L.ub32, (const float*)nullptr, (double*)nullptr, (int64_t)0);
     } else if (!from_zero) {
       hipLaunchKernelGGL((k_fine_gs_pairw<float, false, -1, -1>), g, b, 0, c.stream, v, sg, n, fr, c.w32, c.Kc32, u,
-                         f, L.ub32);
+                         f, L.ub32, (const float*)nullptr, (double*)nullptr, (int64_t)0);
     } else {
 #define GO(A) hipLaunchKernelGGL((k_fine_gs_pairw<float, false, A, A + 1>), g, b, 0, c.stream, v, sg, n, fr, c.w32, \
-                                 c.Kc32, u, f, L.ub32)
+                                 c.Kc32, u, f, L.ub32, (const float*)nullptr, (double*)nullptr, (int64_t)0)
       switch (cls) {
         case 0: GO(0); break;
         case 1: GO(2); break;
@@ -857,6 +874,7 @@ void gs_sweep_pairw(Ctx& c, float* u, const float* f, bool backward, bool from_z
     hipLaunchKernelGGL((k_fine_pairw_copy<float>), dim3(nblk(3 * (int64_t)n)), dim3(256), 0, c.stream, v, sg, n, fr,
                        (const float*)L.ub32, u);
   }
+  if (dots) c.dots_done = true;
   LAUNCH_CHECK();
 }
 
@@ -1140,7 +1158,11 @@ void vcycle_level(Ctx& c, int l, const T* f, T* u, int pre, int post) {
   LAUNCH_CHECK();
   if constexpr (std::is_same<T, float>::value)
     if (learned_here(c, l)) learned_step(c, l, f, u, pre, post);
-  for (int s = 0; s < post; ++s) gs_sweep<T>(c, l, u, f, true);
+  for (int s = 0; s < post; ++s) {
+    c.dots_last = l == 0 && s == post - 1;
+    gs_sweep<T>(c, l, u, f, true);
+  }
+  c.dots_last = false;
 }
 
 // z = B r.  precision 0: FP64 cycle from c.r into c.z.  precision 1: FP32 cycle from
@@ -2106,6 +2128,14 @@ int solve_case_r32(Ctx& c, int k, const tos_solve_opts& o, int* its, double* rel
   // FP64 u per iteration; the same steps in the same order as one at a time
   for (float*& b : c.p32s)
     if (!b) b = dalloc<float>(c, n3);
+  {
+    int64_t nd = 0;  // per-wave z.r / z.q sums of the last fine sweep (gs_sweep_pairw)
+    for (int i = 0; i < 4; ++i) nd += L0.n_segw[i][1];
+    if (nd && c.gsdots_n != nd) {
+      c.gsdots = dalloc<double>(c, 2 * (size_t)nd);
+      c.gsdots_n = nd;
+    }
+  }
   float* const P[4] = {p32, c.p32s[0], c.p32s[1], c.p32s[2]};
   auto flush = [&](int m) {  // u += alpha_k p_k for the directions in P[0 .. m-1]
     PcgDirs D{};
@@ -2131,11 +2161,19 @@ int solve_case_r32(Ctx& c, int k, const tos_solve_opts& o, int* its, double* rel
   };
   // z = B r, beta, (buffer 0: the four pending steps into u), p_j = z + beta p_{j-1}
   auto head = [&](int s) {
+    c.dots_q = q32;  // z.r and z.q from the last fine sweep where it can
+    c.dots_done = false;
     precondition(c, 2, o.pre, o.post, true, false);
+    c.dots_q = nullptr;
     Prof pr(c, P_VECTOR);
-    hipLaunchKernelGGL(k_pcg_zdots32, dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, z32, (const float*)r32,
-                       (const float*)q32, c.partial);
-    finish0(c, 1, S_ZR, S_ZQ, OP_BETA);
+    if (c.dots_done) {
+      finish_parts(c, c.gsdots, c.gsdots_n, S_ZR, S_ZQ, OP_BETA);
+    } else {
+      hipLaunchKernelGGL(k_pcg_zdots32, dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, z32, (const float*)r32,
+                         (const float*)q32, c.partial);
+      finish0(c, 1, S_ZR, S_ZQ, OP_BETA);
+    }
+    c.dots_done = false;
     if (s == 0) flush(4);
     hipLaunchKernelGGL(k_pcg_pnew, dim3(nb), dim3(256), 0, c.stream, cl, nch, NP, P[s], z32,
                        (const float*)P[(s + 3) % 4], (const double*)c.scal);

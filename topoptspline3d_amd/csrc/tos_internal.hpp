@@ -166,6 +166,13 @@ struct Ctx {
   float* p32 = nullptr;         // FCG search direction of the FP32-cycle solves (0 off the free nodes)
   float* q32 = nullptr;         // K p stored in FP32 (solve precision 3; allocated on first use)
   float* p32s[3] = {nullptr, nullptr, nullptr};  // direction buffers 1..3 of the precision-3 FCG
+  // z.r and z.q from the last fine post-smoothing sweep (precision-3 FCG): dots_q = q while
+  // the FCG asks for them, dots_last while that sweep runs, dots_done once it has written
+  // the per-wave sums (gsdots: zr of every backward segment, then zq)
+  const float* dots_q = nullptr;
+  bool dots_last = false, dots_done = false;
+  double* gsdots = nullptr;
+  int64_t gsdots_n = 0;
   // recycled start (tos_set_start_recycling): per case a ring of the last rc_depth
   // solutions (FP64, 3*NP each), K times the basis, the Gram partials and their sums
   int rc_depth = 0;
