@@ -705,8 +705,9 @@ void st_dispatch(Ctx& c, Level& L, Fn&& fn) {
   fn(StFull<T>{Prec<T>::st(L), L.NS});
 }
 
-// FP16 copies of the FP32 stencils of every stored-stencil level (allocated on first use)
-void pack_half_stencils(Ctx& c) {
+// FP16 copies of the FP32 stencils of every stored-stencil level (allocated on first use),
+// on stream s (c.stream, or c.stream2 beside the coarsest factor once allocated)
+void pack_half_stencils(Ctx& c, hipStream_t s) {
   for (size_t l = 1; l < c.L.size(); ++l) {
     Level& L = c.L[l];
     if (!L.n_col) continue;
@@ -715,11 +716,19 @@ void pack_half_stencils(Ctx& c) {
       L.st_m = dalloc<float>(c, L.NS);
     }
     Prof pr(c, P_GALERKIN);
-    hipLaunchKernelGGL(k_st_pack16, dim3(cblk(L.n_col)), dim3(256), 0, c.stream, (const int4*)L.cl_col, L.n_col,
+    hipLaunchKernelGGL(k_st_pack16, dim3(cblk(L.n_col)), dim3(256), 0, s, (const int4*)L.cl_col, L.n_col,
                        (const float*)L.st32, L.NS, L.st16, L.st_m);
   }
   LAUNCH_CHECK();
   c.st16_valid = true;
+}
+
+// the FP16 copies can be packed on the second stream: allocated, and no event timing
+bool pack_beside_ok(const Ctx& c) {
+  if (c.prof_on || !c.want16 || c.gal_prec == 0) return false;
+  for (size_t l = 1; l < c.L.size(); ++l)
+    if (c.L[l].n_col && !c.L[l].st16) return false;
+  return true;
 }
 
 int ensure_fp64_operator(Ctx& c);
@@ -728,7 +737,8 @@ int ensure_fp64_operator(Ctx& c);
 // precision 0 = FP64 cycle: make sure the FP64 stencils are (Galerkin precision 1)
 int select_stencils(Ctx& c, int precision) {
   c.half_st = precision == 2;
-  if (c.half_st && !c.st16_valid) pack_half_stencils(c);
+  c.want16 = c.half_st;
+  if (c.half_st && !c.st16_valid) pack_half_stencils(c, c.stream);
   return precision == 0 ? ensure_fp64_operator(c) : TOS_OK;
 }
 
@@ -2639,6 +2649,9 @@ TOS_EXPORT int tos_ctx_create(const tos_grid_desc* d, tos_ctx** out) {
   try {
     HIPCHK(hipGetDevice(&c->dev));
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&c->ev_gal, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&c->ev_pack, hipEventDisableTiming));
     c->nx = d->nx; c->ny = d->ny; c->nz = d->nz; c->h = d->h;
     const size_t ne = (size_t)d->nx * d->ny * d->nz;
     const size_t nn = (size_t)(d->nx + 1) * (d->ny + 1) * (d->nz + 1);
@@ -2764,6 +2777,9 @@ TOS_EXPORT int tos_ctx_destroy(tos_ctx* c) {
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
   if (c->stage) (void)hipFree(c->stage);
   if (c->h_scal) (void)hipHostFree(c->h_scal);
+  if (c->ev_gal) (void)hipEventDestroy(c->ev_gal);
+  if (c->ev_pack) (void)hipEventDestroy(c->ev_pack);
+  if (c->stream2) (void)hipStreamDestroy(c->stream2);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return TOS_OK;
@@ -3047,7 +3063,22 @@ TOS_EXPORT int tos_update_operator(tos_ctx* c) {
       else
         galerkin_levels<float>(*c, false);
     }
+    // FP16 copies (read by the next solve when the last one read them) packed on the
+    // second stream while the coarsest level is factored: they read the stencils the
+    // factor reads and write nothing it touches
+    const bool beside = pack_beside_ok(*c);
+    if (beside) {
+      HIPCHK(hipEventRecord(c->ev_gal, c->stream));
+      HIPCHK(hipStreamWaitEvent(c->stream2, c->ev_gal, 0));
+      pack_half_stencils(*c, c->stream2);
+      c->st16_valid = false;  // (until the library stream has waited for them)
+      HIPCHK(hipEventRecord(c->ev_pack, c->stream2));
+    }
     const int rc = coarse_factor(*c, c->gal_prec != 0);
+    if (beside) {
+      HIPCHK(hipStreamWaitEvent(c->stream, c->ev_pack, 0));
+      c->st16_valid = true;
+    }
     if (rc) return rc;
     c->st64_valid = c->gal_prec == 0;
     if (c->learn && c->learn->lrho) learn_logrho(*c);

@@ -131,6 +131,11 @@ struct RigidProj {
 struct Ctx {
   int dev = 0;
   hipStream_t stream = nullptr;
+  // second stream (default priority): the FP16 stencil copies of a new operator, packed
+  // while the coarsest level is factored on `stream` (ev_gal: stencils built, ev_pack: packed)
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_gal = nullptr, ev_pack = nullptr;
+  bool want16 = false;  // the last solve read FP16 stencils (pack them with the next operator)
   std::string err;
   int nx, ny, nz;
   double h;
