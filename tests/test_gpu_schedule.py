@@ -56,10 +56,12 @@ def test_wavefront_schedule_is_bit_identical(hip, dims, seed, zsup):
         assert out[0][3] == out[m][3], m
 
 
-def test_graph_replay_matches_eager(hip):
+@pytest.mark.parametrize("precision", [1, 3])
+def test_graph_replay_matches_eager(hip, precision):
     """The FCG iteration body is replayed from a captured hipGraph unless kernels are being
-    timed; eager (timed) and replayed solves must agree bit for bit, call after call, with
-    operator updates (eager work) between the replays."""
+    timed (precision 3: four bodies, one per direction buffer); eager (timed) and replayed
+    solves must agree bit for bit, call after call, with operator updates (eager work)
+    between the replays."""
     import ctypes
     from topoptspline3d_amd import _abi
     nx, ny, nz = 36, 18, 20
@@ -77,7 +79,7 @@ def test_graph_replay_matches_eager(hip):
         for timed in (True, False, False):
             if timed:
                 hip.profile(c.h, 1, ctypes.byref(ks))
-            st = c.solve(rtol=1e-8, maxit=200, precision=1, warm_start=False)
+            st = c.solve(rtol=1e-8, maxit=200, precision=precision, warm_start=False)
             if timed:
                 hip.profile(c.h, 0, ctypes.byref(ks))
             out.append((c.displacement().copy(), st["iterations"][0]))
@@ -136,3 +138,35 @@ def test_large_stencil_level_residual_identical(hip):
     for rep in range(3):
         assert np.array_equal(c.vcycle(r, precision=1), out[3][0]), rep
         assert np.array_equal(c.vcycle(r, precision=2), out[3][1]), rep
+
+
+@pytest.mark.parametrize("dims,seed", [((300, 12, 10), 2), ((40, 20, 24), 0)])
+def test_fcg_dots_from_the_sweep(hip, dims, seed):
+    """At precision 3 with the in-wave paired schedule, z.r and z.q come from the waves of
+    the last fine post-smoothing sweep; under the eight-colour schedule (the same V-cycle
+    updates bit for bit) from a pass of their own. Only the summation order of the two
+    dots differs: after 2, 3 and 4 FCG iterations (the first beta uses them at the second)
+    the iterates agree to rounding, where a node counted twice or missed (rows of several
+    segments on the first domain, of one on the second) would move beta by about 1e-4 and
+    the iterate with it; the full solves take the same iterations to the same solution
+    within what the 1e-8 tolerance leaves open (later betas can come from nearly
+    cancelling z.q sums, whose last bits depend on the summation order: 1.7e-8 after 5
+    iterations on the first domain, 3e-10 at convergence)."""
+    rng = np.random.default_rng(seed)
+    solid, support, f = cantilever(*dims)
+    c = Context(hip, solid, support)
+    c.set_material(1e6, 0.4, 3.0, 1e-3)
+    c.set_loads(f)
+    c.set_density(rng.uniform(0.1, 1.0, solid.size).astype(np.float32))
+    c.update_operator()
+    for maxit, tol in ((2, 1e-12), (3, 1e-12), (4, 1e-12), (300, 1e-7)):
+        out = {}
+        for mode in (0, 3):
+            c.set_sweep_mode(mode)
+            st = c.solve(rtol=1e-8, maxit=maxit, precision=3, warm_start=False, check=False)
+            out[mode] = (c.displacement().copy(), st["iterations"][0])
+        d = np.linalg.norm(out[3][0] - out[0][0]) / np.linalg.norm(out[0][0])
+        print(f"\n[fcg dots] {dims} maxit {maxit}: iterations {out[0][1]} / {out[3][1]}, "
+              f"relative difference {d:.2e}", flush=True)
+        assert out[0][1] == out[3][1]
+        assert d <= tol
