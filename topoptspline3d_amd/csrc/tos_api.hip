@@ -2171,10 +2171,15 @@ int solve_case_r32(Ctx& c, int k, const tos_solve_opts& o, int* its, double* rel
   };
   // z = B r, beta, (buffer 0: the four pending steps into u), p_j = z + beta p_{j-1}
   auto head = [&](int s) {
-    c.dots_q = q32;  // z.r and z.q from the last fine sweep where it can
     c.dots_done = false;
-    precondition(c, 2, o.pre, o.post, true, false);
-    c.dots_q = nullptr;
+    {
+      struct DotsReq {  // z.r and z.q from the last fine sweep where it can (cleared on any exit)
+        Ctx& x;
+        DotsReq(Ctx& x_, const float* q) : x(x_) { x.dots_q = q; }
+        ~DotsReq() { x.dots_q = nullptr; x.dots_last = false; }
+      } req(c, q32);
+      precondition(c, 2, o.pre, o.post, true, false);
+    }
     Prof pr(c, P_VECTOR);
     if (c.dots_done) {
       finish_parts(c, c.gsdots, c.gsdots_n, S_ZR, S_ZQ, OP_BETA);
