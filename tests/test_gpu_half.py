@@ -67,3 +67,32 @@ def test_half_vcycle_close_to_fp32(hip):
         err = np.abs(z2 - z1).max() / np.abs(z1).max()
         print(f"\n[half] vcycle rel diff {err:.2e}", flush=True)
         assert 0 < err < 1e-2
+
+
+def test_copies_packed_beside_the_factor_match_packed_on_demand(hip):
+    """After a solve that read FP16 stencils, the next operator update packs the copies on
+    the context's second stream while the coarsest level is factored; a context that never
+    read them packs them when the first FP16 cycle asks. Both must hold the same copies:
+    precision-2 V-cycles equal bit for bit, update after update."""
+    nx, ny, nz = 64, 32, 32
+    rng = np.random.default_rng(7)
+    solid, support, f = cantilever(nx, ny, nz)
+    a, b = Context(hip, solid, support), Context(hip, solid, support)
+    for c in (a, b):
+        c.set_material(1e6, 0.4, 3.0, 1e-3)
+        c.set_loads(f)
+        c.set_galerkin_precision(1)
+    rho0 = rng.uniform(0.1, 1.0, solid.size).astype(np.float32)
+    a.set_density(rho0)
+    a.update_operator()
+    a.solve(rtol=1e-6, maxit=200, precision=3)  # a now packs beside every update
+    r = rng.normal(size=3 * a.nn)
+    for rep in range(3):
+        rho = rng.uniform(0.1, 1.0, solid.size).astype(np.float32)
+        print(f"\n[fp16 pack beside] update {rep}", flush=True)
+        for c in (a, b):
+            c.set_density(rho)
+            c.update_operator()
+        za, zb = a.vcycle(r, precision=2), b.vcycle(r, precision=2)
+        assert np.array_equal(za, zb), rep
+        b.solve(rtol=1e-6, maxit=200, precision=1)  # b keeps packing on demand
