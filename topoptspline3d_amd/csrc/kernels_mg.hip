@@ -910,7 +910,7 @@ __global__ __launch_bounds__(256) void k_st_gs_pairw(LevelView L, const int4* __
                                                      const int* __restrict__ slotmap, ST st,
                                                      T* __restrict__ u, const T* __restrict__ f, T* __restrict__ ub) {
   constexpr unsigned MA = nb_mask_of(ZA), MB = nb_mask_of(ZB);
-  const int si = (int)xcd_block() * 4 + (threadIdx.x >> 6);
+  const int si = (int)xcd_block() * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6);  // (1 or 4 waves per block)
   if (si >= nseg) return;  // whole wave
   const int lane = threadIdx.x & 63;
   const int4 S = seg[si];

@@ -81,6 +81,15 @@ T* dalloc(Ctx& c, size_t n) {
 
 inline unsigned nblk(int64_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
 
+// launch shape of the stored-stencil paired GS (one wave per segment, no block-wide
+// reduction): four waves per workgroup, or on small levels (fewer than 1024 segments) one,
+// so that the few waves of a latency-bound pass spread over as many CUs
+constexpr int kSpreadBelow = 1024;
+inline dim3 spread_grid(int64_t waves) {
+  return dim3(waves < kSpreadBelow ? (unsigned)waves : (unsigned)((waves + 3) / 4));
+}
+inline dim3 spread_block(int64_t waves) { return dim3(waves < kSpreadBelow ? 64 : 256); }
+
 }  // namespace
 
 #include "unet_host.hpp"
@@ -903,7 +912,9 @@ void gs_sweep_st_pairw(Ctx& c, int l, T* u, const T* f, bool backward, bool from
     const int n = L.n_segw[cls][P];
     if (!n) continue;
     Prof pr(c, sizeof(T) == 8 ? P_ST_GS64 : (l == 1 ? P_ST_GS32_L1 : P_ST_GS32));
-    const dim3 g((n + 3) / 4), b(256);
+    // small levels: one wave per workgroup, so the few waves of a pass spread over as many
+    // CUs (a latency-bound pass then shares no CU's load pipeline among its waves)
+    const dim3 g = spread_grid(n), b = spread_block(n);
     const int4* sg = L.segw[cls][P];
     st_dispatch<T>(c, L, [&](auto st) {
       using ST = decltype(st);
